@@ -1,0 +1,213 @@
+"""MI355X (gfx950) matrix-free H*v in the momentum- and Sz-symmetrised basis.
+
+Python mirror of the reference's sector interface (hiteshjc/ED_Code):
+
+    Lattice(T1, T2)                  translation group (read_input_file T1=/T2=)
+    Basis(lattice, n_ones, kx, ky)   pConstruct_States   (Momentum_Basis.cpp:645-778)
+    Operator(basis, model)           terms of sHv        (H_v.cpp:192-357)
+    Operator.hv(v, w)                sHv on device tensors or host arrays
+    Operator.lanczos(...)            lanczos_evecs       (lanczos.cpp:133-349)
+    sector_lanczos(...)              sector_lanczos      (Lanczos_sector.cpp:15-198)
+
+All compute runs in ed_code_amd/libed_amd.so through its C ABI
+(include/ed_amd.h); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import lattices
+from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_REFERENCE, EDError, LanczosParams, LanczosStats, Terms,
+                   check, lib)
+from .lattices import Model, n_ones_for_sz
+
+__all__ = ["Lattice", "Basis", "Operator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
+           "ED_LZ_FULL", "ED_LZ_REFERENCE", "n_ones_for_sz"]
+
+_initialised = False
+
+
+def init(device: int = 0):
+    global _initialised
+    if not _initialised:
+        check(lib().ed_init(device))
+        _initialised = True
+
+
+class Lattice:
+    def __init__(self, T1, T2):
+        init()
+        self.T1 = np.ascontiguousarray(T1, np.int32)
+        self.T2 = np.ascontiguousarray(T2, np.int32)
+        if self.T1.ndim != 1 or self.T1.shape != self.T2.shape:
+            # read_input_file.cpp:423-427 rejects T1/T2 of different lengths
+            raise EDError(1, "T1 and T2 must have the same length")
+        self.N = len(self.T1)
+        h = C.c_void_p()
+        check(lib().ed_lattice_create(self.N, self.T1.ctypes.data, self.T2.ctypes.data, C.byref(h)))
+        self.h = h
+        a, b = C.c_int32(), C.c_int32()
+        check(lib().ed_lattice_orders(h, C.byref(a), C.byref(b)))
+        self.L1, self.L2 = a.value, b.value
+
+    @classmethod
+    def of(cls, model: Model) -> "Lattice":
+        return cls(model.T1, model.T2)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ed_lattice_destroy(self.h)
+            self.h = None
+
+
+class Basis:
+    """Momentum basis of (n_ones, kx, ky), resident on the GPU."""
+
+    def __init__(self, lattice: Lattice, n_ones: int, kx: int = 0, ky: int = 0):
+        self.lattice = lattice
+        self.n_ones, self.kx, self.ky = n_ones, kx, ky
+        h = C.c_void_p()
+        check(lib().ed_basis_build(lattice.h, n_ones, kx, ky, C.byref(h)))
+        self.h = h
+        d = C.c_int64()
+        check(lib().ed_basis_dim(h, C.byref(d)))
+        self.D = d.value
+
+    def export(self):
+        """(pblock_states, norm, norm_inv) as host arrays."""
+        D = self.D
+        reps = np.zeros(D, np.int64)
+        norm = np.zeros(D)
+        norm_inv = np.zeros(D)
+        check(lib().ed_basis_export(self.h, reps.ctypes.data, norm.ctypes.data, norm_inv.ctypes.data))
+        return reps, norm, norm_inv
+
+    def export_full(self):
+        """Full-space rep_loc, state_ph, Ia, Ib, bits_right (reference layout)."""
+        Cn, nl = C.c_int64(), C.c_int64()
+        check(lib().ed_basis_full_size(self.h, C.byref(Cn), C.byref(nl)))
+        rep_loc = np.zeros(Cn.value, np.int64)
+        sph = np.zeros(2 * Cn.value)
+        Ia = np.zeros(nl.value, np.int64)
+        Ib = np.zeros(nl.value, np.int64)
+        br = C.c_int32()
+        check(lib().ed_basis_export_full(self.h, rep_loc.ctypes.data, sph.ctypes.data, Ia.ctypes.data,
+                                         Ib.ctypes.data, C.byref(br)))
+        return dict(rep_loc=rep_loc, state_ph=sph.view(np.complex128), Ia=Ia, Ib=Ib, bits_right=br.value)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ed_basis_destroy(self.h)
+            self.h = None
+
+
+def _terms(model: Model):
+    a = model.term_arrays()
+    keep = [np.ascontiguousarray(a[k]) for k in ("bond_xy", "bond_J", "bond_phase", "chiral_ijk", "chiral_phase",
+                                                 "bowtie")]
+    t = Terms(a["lam"], a["nbonds"], keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, a["Ch"],
+              a["nchiral"], keep[3].ctypes.data, keep[4].ctypes.data, a["ChCh"], a["nbowtie"], keep[5].ctypes.data)
+    return t, keep
+
+
+class Operator:
+    """H restricted to one momentum sector; terms uploaded once."""
+
+    def __init__(self, basis: Basis, model: Model):
+        self.basis = basis
+        self.model = model
+        t, keep = _terms(model)
+        h = C.c_void_p()
+        check(lib().ed_op_create(basis.h, C.byref(t), C.byref(h)))
+        self.h = h
+        self.D = basis.D
+
+    def hv(self, v, w=None, stream=None):
+        """w = H v.  v, w: torch tensors on the GPU (complex64 or complex128)
+        or host numpy complex128 arrays (copied through the device)."""
+        if isinstance(v, np.ndarray):
+            v = np.ascontiguousarray(v, np.complex128)
+            if w is None:
+                w = np.zeros(self.D, np.complex128)
+            check(lib().ed_hv_host(self.h, v.ctypes.data, w.ctypes.data))
+            return w
+        import torch
+        if v.dtype == torch.complex128:
+            dt = ED_C128
+        elif v.dtype == torch.complex64:
+            dt = ED_C64
+        else:
+            raise TypeError(f"unsupported dtype {v.dtype}")
+        if not v.is_cuda or not v.is_contiguous() or v.numel() != self.D:
+            raise ValueError("v must be a contiguous device tensor of length D")
+        if w is None:
+            w = torch.empty_like(v)
+        if w.dtype != v.dtype or not w.is_contiguous() or w.numel() != self.D:
+            raise ValueError("w must match v")
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().ed_hv(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st)))
+        return w
+
+    def hv_profile(self, v, w, stream=None) -> float:
+        """ed_hv with HIP events around the row kernel; returns its ms."""
+        import torch
+        dt = ED_C128 if v.dtype == torch.complex128 else ED_C64
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        ms = C.c_float()
+        check(lib().ed_hv_profile(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st),
+                                  C.byref(ms)))
+        return ms.value
+
+    def count_targets(self):
+        g, t = C.c_int64(), C.c_int64()
+        check(lib().ed_op_count_targets(self.h, C.byref(g), C.byref(t)))
+        return g.value, t.value
+
+    def make_h(self) -> np.ndarray:
+        H = np.zeros(self.D * self.D, np.complex128)
+        check(lib().ed_make_h(self.h, H.ctypes.data))
+        return H.reshape(self.D, self.D, order="F")
+
+    def lanczos(self, nev=1, iterations=100, ncycles=10, tol=1e-11, seed=1234, precision=ED_C128,
+                mode=ED_LZ_REFERENCE, want_vectors=False):
+        p = LanczosParams(nev, iterations, ncycles, precision, mode, 0, tol, seed)
+        ev = np.zeros(nev)
+        vecs = np.zeros((nev, self.D), np.complex128) if want_vectors else None
+        st = LanczosStats()
+        check(lib().ed_lanczos(self.h, C.byref(p), ev.ctypes.data, None if vecs is None else vecs.ctypes.data,
+                               C.byref(st)))
+        n = min(nev, self.D)
+        stats = dict(hv_calls=st.hv_calls, hv_seconds=st.hv_seconds, total_seconds=st.total_seconds,
+                     residual=list(st.residual[:min(n, 16)]), rayleigh=list(st.rayleigh[:min(n, 16)]))
+        return (ev[:n], vecs, stats) if want_vectors else (ev[:n], stats)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ed_op_destroy(self.h)
+            self.h = None
+
+
+def sector_lanczos(model: Model, kx: int, ky: int, cSz: float, n_sec_ev: int = 2, tol: float = 1e-11,
+                   iterations: int = 100, ncycles: int = 10, seed: int = 1234, precision: int = ED_C128,
+                   dense_limit: int = 10000):
+    """Lanczos_sector.cpp:15-198: n_ones from cSz (integer n_spins/2), k
+    check, basis, dense diagonalisation for D <= 10000 (all D eigenvalues,
+    exact_evecs, lanczos.cpp:81-131) else Lanczos with the reference
+    schedule.  Returns (energies, D)."""
+    n_ones = n_ones_for_sz(model.n_sites, cSz)
+    lat = Lattice.of(model)
+    if kx >= lat.L1 or ky >= lat.L2 or kx < 0 or ky < 0:
+        raise EDError(1, "Invalid kx or ky")
+    basis = Basis(lat, n_ones, kx, ky)
+    if basis.D == 0:
+        raise EDError(4, "No states in psector")
+    op = Operator(basis, model)
+    if basis.D <= dense_limit:
+        H = op.make_h()
+        return np.linalg.eigvalsh(H), basis.D
+    ev, _ = op.lanczos(nev=n_sec_ev, iterations=iterations, ncycles=ncycles, tol=tol, seed=seed,
+                       precision=precision, mode=ED_LZ_REFERENCE)
+    return ev, basis.D

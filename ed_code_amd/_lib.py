@@ -1,0 +1,88 @@
+"""ctypes binding of libed_amd.so (include/ed_amd.h).
+
+The library is the only compute path: if it is missing, or no HIP device is
+visible, calls raise instead of falling back to anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libed_amd.so")
+
+ED_OK, ED_EINVAL, ED_ENOMEM, ED_EDEVICE, ED_EEMPTY, ED_ENOTCONV = range(6)
+ED_C64, ED_C128 = 0, 1
+ED_LZ_REFERENCE, ED_LZ_FULL = 0, 1
+
+
+class EDError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[ed status {code}] {msg}")
+        self.code = code
+
+
+class Terms(C.Structure):
+    _fields_ = [
+        ("lam", C.c_double), ("nbonds", C.c_int32), ("bond_xy", C.c_void_p), ("bond_J", C.c_void_p),
+        ("bond_phase", C.c_void_p), ("Ch", C.c_double), ("nchiral", C.c_int32), ("chiral_ijk", C.c_void_p),
+        ("chiral_phase", C.c_void_p), ("ChCh", C.c_double), ("nbowtie", C.c_int32), ("bowtie", C.c_void_p),
+    ]
+
+
+class LanczosParams(C.Structure):
+    _fields_ = [("nev", C.c_int32), ("iterations", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
+                ("mode", C.c_int32), ("pad", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
+
+
+class LanczosStats(C.Structure):
+    _fields_ = [("hv_calls", C.c_int64), ("hv_seconds", C.c_double), ("total_seconds", C.c_double),
+                ("residual", C.c_double * 16), ("rayleigh", C.c_double * 16)]
+
+
+# Every symbol include/ed_amd.h declares, with (restype, argtypes).
+_VP = C.c_void_p
+SIGNATURES = {
+    "ed_last_error": (C.c_char_p, []),
+    "ed_init": (C.c_int, [C.c_int]),
+    "ed_lattice_create": (C.c_int, [C.c_int32, _VP, _VP, C.POINTER(_VP)]),
+    "ed_lattice_orders": (C.c_int, [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ed_lattice_destroy": (None, [_VP]),
+    "ed_basis_build": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_VP)]),
+    "ed_basis_dim": (C.c_int, [_VP, C.POINTER(C.c_int64)]),
+    "ed_basis_export": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "ed_basis_full_size": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ed_basis_export_full": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.POINTER(C.c_int32)]),
+    "ed_basis_destroy": (None, [_VP]),
+    "ed_op_create": (C.c_int, [_VP, C.POINTER(Terms), C.POINTER(_VP)]),
+    "ed_op_destroy": (None, [_VP]),
+    "ed_op_count_targets": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ed_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
+    "ed_hv_profile": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.POINTER(C.c_float)]),
+    "ed_hv_host": (C.c_int, [_VP, _VP, _VP]),
+    "ed_make_h": (C.c_int, [_VP, _VP]),
+    "ed_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m ed_code_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != ED_OK:
+        msg = lib().ed_last_error()
+        raise EDError(rc, msg.decode() if msg else "")

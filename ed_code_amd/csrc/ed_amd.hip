@@ -1,0 +1,653 @@
+// ed_amd.hip -- gfx950 basis construction, H*v and the C ABI of include/ed_amd.h.
+//
+// Data layout in HBM per sector (N sites, n up spins, C = C(N,n) states,
+// D representatives):
+//   rows  : uint64 [D]   representative | (lid << 52), ascending; lid = number
+//                        of translation-grid points fixing it, which gives
+//                        norm = sqrt(L1*L2/lid) through a tiny table
+//   dir   : 16 B [C/96]  {base, 96 occupancy bits} over the combinatorial
+//                        rank of every weight-n state: index of a
+//                        representative = base + popcount of the bits below
+//   rtab  : uint64       byte-sliced rank tables (a few KB)
+//   phase : complex128 [L1*L2] exp(-i(px lx + py ly)), computed on the host
+//                        with the reference's expression (bit-equal)
+// Nothing of size C survives the build except the 1.33 bit/state directory.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "ed_amd.h"
+#include "ed_core.h"
+#include "ed_internal.h"
+#include "ed_tables.h"
+
+using namespace edc;
+
+// ----------------------------------------------------------------- errors --
+static thread_local std::string g_err;
+
+int ed_fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char* ed_last_error(void) { return g_err.c_str(); }
+
+extern "C" int ed_init(int device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return ed_fail(ED_EDEVICE, "no HIP device: %s", hipGetErrorString(e));
+    if (device < 0 || device >= n) return ed_fail(ED_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    return ED_OK;
+}
+
+// --------------------------------------------------------------- lattice --
+extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32_t* T2, ed_lattice* out) {
+    if (!out || !T1 || !T2) return ed_fail(ED_EINVAL, "null argument");
+    if (n_sites < 1 || n_sites > 63) return ed_fail(ED_EINVAL, "n_sites=%d outside [1,63]", n_sites);
+    std::vector<int> a(T1, T1 + n_sites), b(T2, T2 + n_sites);
+    for (auto* T : {&a, &b}) {
+        std::vector<char> seen(n_sites, 0);
+        for (int i = 0; i < n_sites; i++) {
+            int t = (*T)[i];
+            if (t < 0 || t >= n_sites || seen[t]) return ed_fail(ED_EINVAL, "T1/T2 must be permutations of 0..%d", n_sites - 1);
+            seen[t] = 1;
+        }
+    }
+    for (int i = 0; i < n_sites; i++)
+        if (a[b[i]] != b[a[i]]) return ed_fail(ED_EINVAL, "T1 and T2 do not commute");
+    auto* L = new ed_lattice_s();
+    L->N = n_sites;
+    L->T1 = a;
+    L->T2 = b;
+    L->L1 = cycle_len0(a);
+    L->L2 = cycle_len0(b);
+    if (!make_prog(a, L->P1) || !make_prog(b, L->P2)) {
+        delete L;
+        return ed_fail(ED_EINVAL, "translation has more than %d displacement groups", kMaxGroups);
+    }
+    *out = L;
+    return ED_OK;
+}
+
+extern "C" int ed_lattice_orders(ed_lattice lat, int32_t* L1, int32_t* L2) {
+    if (!lat) return ed_fail(ED_EINVAL, "null lattice");
+    if (L1) *L1 = lat->L1;
+    if (L2) *L2 = lat->L2;
+    return ED_OK;
+}
+
+extern "C" void ed_lattice_destroy(ed_lattice lat) { delete lat; }
+
+// ---------------------------------------------------------------- kernels --
+template <typename S>
+__global__ void k_classify(uint64_t C, int N, int n, Geo G, const uint64_t* binom, uint32_t* words, uint64_t nwords) {
+    uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint64_t r0 = w * 32;
+    uint32_t bits = 0;
+    if (r0 < C) {
+        uint64_t s = unrank(r0, N, n, binom);
+        for (int b = 0; b < 32; b++) {
+            if (r0 + b >= C) break;
+            if (classify<S>((S)s, G)) bits |= 1u << b;
+            s = gosper_next(s);
+        }
+    }
+    words[w] = bits;
+}
+
+__global__ void k_group_counts(const uint32_t* words, uint64_t ngroups, uint64_t* cnt) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    cnt[g] = (uint64_t)(__popc(words[3 * g]) + __popc(words[3 * g + 1]) + __popc(words[3 * g + 2]));
+}
+
+__global__ void k_dir_fill(const uint32_t* words, const uint64_t* base, uint64_t ngroups, DirEntry* dir) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    DirEntry e;
+    e.base = (uint32_t)base[g];
+    e.b0 = words[3 * g];
+    e.b1 = words[3 * g + 1];
+    e.b2 = words[3 * g + 2];
+    dir[g] = e;
+}
+
+template <typename S>
+__global__ void k_compact(uint64_t C, int N, int n, Geo G, const uint64_t* binom, const DirEntry* dir,
+                          uint64_t ngroups, uint64_t* rows) {
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    DirEntry e = dir[g];
+    if ((e.b0 | e.b1 | e.b2) == 0) return;
+    uint64_t r0 = g * kDirSpan;
+    uint64_t s = unrank(r0, N, n, binom);
+    uint64_t j = e.base;
+    for (int b = 0; b < kDirSpan && r0 + b < C; b++) {
+        uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
+        if ((wd >> (b & 31)) & 1u) {
+            int lid = classify<S>((S)s, G);
+            rows[j++] = s | ((uint64_t)lid << kLidShift);
+        }
+        s = gosper_next(s);
+    }
+}
+
+__global__ void k_norms(const uint64_t* rows, int64_t D, const double* ntab, const double* nitab, double* norm,
+                        double* norm_inv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    int lid = (int)(rows[i] >> kLidShift);
+    if (norm) norm[i] = ntab[lid];
+    if (norm_inv) norm_inv[i] = nitab[lid];
+}
+
+template <typename S>
+__global__ void k_full_tables(uint64_t C, int N, int n, Geo G, const uint64_t* binom, RankInfo R, TablesView tv,
+                              int64_t* rep_loc, cd2* state_ph) {
+    uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t r0 = w * 32;
+    if (r0 >= C) return;
+    uint64_t s = unrank(r0, N, n, binom);
+    for (int b = 0; b < 32 && r0 + b < C; b++) {
+        int code;
+        S rep = find_rep<S, true>((S)s, G, code);
+        rep_loc[r0 + b] = dir_lookup(rank_of<S>(rep, R, tv.rtab), tv.dir);
+        state_ph[r0 + b] = tv.phase[code];
+        s = gosper_next(s);
+    }
+}
+
+// u = v / norm  (the gathered stream of the H*v kernel)
+template <typename VT>
+__global__ void k_prescale(int64_t D, const uint64_t* rows, const double* nitab, const VT* v, VT* u) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    double f = nitab[(int)(rows[i] >> kLidShift)];
+    VT x = v[i];
+    x.re = x.re * f;
+    x.im = x.im * f;
+    u[i] = x;
+}
+
+template <typename VT>
+__device__ __forceinline__ void store_c(VT* p, int64_t i, double re, double im);
+template <>
+__device__ __forceinline__ void store_c<cd2>(cd2* p, int64_t i, double re, double im) { p[i] = cd2{re, im}; }
+template <>
+__device__ __forceinline__ void store_c<cf2>(cf2* p, int64_t i, double re, double im) {
+    p[i] = cf2{(float)re, (float)im};
+}
+
+// w[i] = norm[i] * sum_terms coef * u[j] + Ezz * v[i]: one lane per row.
+template <typename S, bool TRIV, typename VT>
+__global__ void __launch_bounds__(256) k_hv_rows(int64_t D, const uint64_t* rows, Geo G, RankInfo R, TablesView tv,
+                                                 TermsView T, const double* ntab, const VT* u, const VT* v, VT* w) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    uint64_t rw = rows[i];
+    S s = (S)(rw & kStateMask);
+    double nrm = ntab[(int)(rw >> kLidShift)];
+    double ar = 0.0, ai = 0.0;
+    double ez = visit_row<S, TRIV>(s, G, R, tv, T, [&](int64_t j, double cr, double ci) {
+        cd2 x = load_c(u, j);
+        ar += cr * x.re - ci * x.im;
+        ai += cr * x.im + ci * x.re;
+    });
+    cd2 vi = load_c(v, i);
+    store_c<VT>(w, i, nrm * ar + ez * vi.re, nrm * ai + ez * vi.im);
+}
+
+template <typename S, bool TRIV>
+__global__ void k_count_targets(int64_t D, const uint64_t* rows, Geo G, RankInfo R, TablesView tv, TermsView T,
+                                unsigned long long* distinct, unsigned long long* total) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long dcount = 0, tcount = 0;
+    if (i < D) {
+        constexpr int MAXJ = 384;
+        int64_t js[MAXJ];
+        int nj = 0;
+        bool overflow = false;
+        S s = (S)(rows[i] & kStateMask);
+        visit_row<S, TRIV>(s, G, R, tv, T, [&](int64_t j, double, double) {
+            tcount++;
+            for (int q = 0; q < nj; q++)
+                if (js[q] == j) return;
+            if (nj < MAXJ) js[nj++] = j; else overflow = true;
+        });
+        dcount = overflow ? 0 : (unsigned long long)nj;
+    }
+    atomicAdd(distinct, dcount);
+    atomicAdd(total, tcount);
+}
+
+template <typename S, bool TRIV>
+__global__ void k_make_h(int64_t D, const uint64_t* rows, Geo G, RankInfo R, TablesView tv, TermsView T,
+                         const double* ntab, const double* nitab, cd2* H) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    uint64_t rw = rows[i];
+    S s = (S)(rw & kStateMask);
+    double nrm = ntab[(int)(rw >> kLidShift)];
+    double ez = visit_row<S, TRIV>(s, G, R, tv, T, [&](int64_t j, double cr, double ci) {
+        double f = nrm * nitab[(int)(rows[j] >> kLidShift)];
+        cd2 h = H[i + j * D];
+        h.re += cr * f;
+        h.im += ci * f;
+        H[i + j * D] = h;
+    });
+    H[i + i * D].re += ez;
+}
+
+static inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------ basis --
+int basis_geo(const ed_basis_s* B, Geo& G) {
+    std::memset(&G, 0, sizeof(G));
+    G.N = B->N; G.L1 = B->L1; G.L2 = B->L2; G.kx = B->kx; G.ky = B->ky;
+    G.T1 = B->lat_P1;
+    G.T2 = B->lat_P2;
+    return ED_OK;
+}
+
+extern "C" int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out) {
+    if (!lat || !out) return ed_fail(ED_EINVAL, "null argument");
+    const int N = lat->N;
+    if (n_ones < 0 || n_ones > N) return ed_fail(ED_EINVAL, "n_ones=%d outside [0,%d] (Num sites < Num_ones)", n_ones, N);
+    if (kx < 0 || ky < 0 || kx >= lat->L1 || ky >= lat->L2)
+        return ed_fail(ED_EINVAL, "Invalid kx or ky: (%d,%d) with L1=%d L2=%d", kx, ky, lat->L1, lat->L2);
+    auto* B = new ed_basis_s();
+    B->N = N; B->n = n_ones; B->kx = kx; B->ky = ky;
+    B->L1 = lat->L1; B->L2 = lat->L2;
+    B->T1 = lat->T1; B->T2 = lat->T2;
+    B->lat_P1 = lat->P1; B->lat_P2 = lat->P2;
+    // Lanczos_sector.cpp:57
+    B->px = 2 * kx * M_PI / B->L1;
+    B->py = 2 * ky * M_PI / B->L2;
+    B->C = binom_u64(N, n_ones);
+    B->trivial_k = (kx == 0 && ky == 0);
+    int rc = basis_build_device(B);
+    if (rc != ED_OK) { delete_basis(B); return rc; }
+    *out = B;
+    return ED_OK;
+}
+
+// Host-side tables + the device build (classify -> scan -> directory -> compact).
+int basis_build_device(ed_basis_s* B) {
+    const int N = B->N, n = B->n, L1 = B->L1, L2 = B->L2;
+    std::vector<cd2> ph = phase_table(L1, L2, B->px, B->py);
+    std::vector<double> nt, nit;
+    norm_tables(L1, L2, nt, nit);
+    std::vector<uint64_t> bn = binom_table();
+    std::vector<uint64_t> rt = rank_tables(N, n, B->rinfo);
+    HIPCHK(hipMalloc(&B->d_phase, ph.size() * sizeof(cd2)));
+    HIPCHK(hipMalloc(&B->d_ntab, nt.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&B->d_nitab, nit.size() * sizeof(double)));
+    HIPCHK(hipMalloc(&B->d_binom, bn.size() * sizeof(uint64_t)));
+    HIPCHK(hipMalloc(&B->d_rtab, rt.size() * sizeof(uint64_t)));
+    HIPCHK(hipMemcpy(B->d_phase, ph.data(), ph.size() * sizeof(cd2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_ntab, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_nitab, nit.data(), nit.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_binom, bn.data(), bn.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(B->d_rtab, rt.data(), rt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+
+    Geo G;
+    basis_geo(B, G);
+    const uint64_t C = B->C;
+    const uint64_t ngroups = (C + kDirSpan - 1) / kDirSpan;
+    const uint64_t nwords = ngroups * 3;
+    B->ngroups = ngroups;
+    uint32_t* words = nullptr;
+    uint64_t *cnt = nullptr, *base = nullptr;
+    HIPCHK(hipMalloc(&words, nwords * sizeof(uint32_t)));
+    HIPCHK(hipMalloc(&cnt, (ngroups + 1) * sizeof(uint64_t)));
+    HIPCHK(hipMalloc(&base, (ngroups + 1) * sizeof(uint64_t)));
+    HIPCHK(hipMalloc(&B->d_dir, ngroups * sizeof(DirEntry)));
+    if (N <= 32)
+        hipLaunchKernelGGL(k_classify<uint32_t>, dim3(nblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+    else
+        hipLaunchKernelGGL(k_classify<uint64_t>, dim3(nblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_group_counts, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, words, ngroups, cnt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemset(cnt + ngroups, 0, sizeof(uint64_t)));
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (int)(ngroups + 1)));
+    void* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, tmp_bytes + 16));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (int)(ngroups + 1)));
+    uint64_t D = 0;
+    HIPCHK(hipMemcpy(&D, base + ngroups, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (D > 0xFFFFFFFFull) {
+        dfree(tmp); dfree(words); dfree(cnt); dfree(base);
+        return ed_fail(ED_EINVAL, "sector dimension %llu exceeds 2^32", (unsigned long long)D);
+    }
+    B->D = (int64_t)D;
+    hipLaunchKernelGGL(k_dir_fill, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, words, base, ngroups, B->d_dir);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMalloc(&B->d_rows, std::max<uint64_t>(D, 1) * sizeof(uint64_t)));
+    if (N <= 32)
+        hipLaunchKernelGGL(k_compact<uint32_t>, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+    else
+        hipLaunchKernelGGL(k_compact<uint64_t>, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    dfree(tmp); dfree(words); dfree(cnt); dfree(base);
+    return ED_OK;
+}
+
+void delete_basis(ed_basis_s* B) {
+    if (!B) return;
+    for (void* p : {(void*)B->d_rows, (void*)B->d_dir, (void*)B->d_rtab, (void*)B->d_binom, (void*)B->d_phase,
+                    (void*)B->d_ntab, (void*)B->d_nitab})
+        if (p) dfree(p);
+    delete B;
+}
+
+extern "C" void ed_basis_destroy(ed_basis b) { delete_basis(b); }
+
+extern "C" int ed_basis_dim(ed_basis b, int64_t* D) {
+    if (!b || !D) return ed_fail(ED_EINVAL, "null argument");
+    *D = b->D;
+    return ED_OK;
+}
+
+extern "C" int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* norm_inv) {
+    if (!b) return ed_fail(ED_EINVAL, "null basis");
+    const int64_t D = b->D;
+    if (D == 0) return ED_OK;
+    if (reps) {
+        std::vector<uint64_t> rw(D);
+        HIPCHK(hipMemcpy(rw.data(), b->d_rows, D * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < D; i++) reps[i] = (int64_t)(rw[i] & kStateMask);
+    }
+    if (norm || norm_inv) {
+        double *dn = nullptr, *dni = nullptr;
+        HIPCHK(hipMalloc(&dn, D * sizeof(double)));
+        HIPCHK(hipMalloc(&dni, D * sizeof(double)));
+        hipLaunchKernelGGL(k_norms, dim3(nblocks(D, 256)), dim3(256), 0, 0, b->d_rows, D, b->d_ntab, b->d_nitab, dn, dni);
+        HIPCHK(hipGetLastError());
+        if (norm) HIPCHK(hipMemcpy(norm, dn, D * sizeof(double), hipMemcpyDeviceToHost));
+        if (norm_inv) HIPCHK(hipMemcpy(norm_inv, dni, D * sizeof(double), hipMemcpyDeviceToHost));
+        dfree(dn); dfree(dni);
+    }
+    return ED_OK;
+}
+
+extern "C" int ed_basis_full_size(ed_basis b, int64_t* C, int64_t* nlin) {
+    if (!b) return ed_fail(ED_EINVAL, "null basis");
+    if (C) *C = (int64_t)b->C;
+    if (nlin) *nlin = (int64_t)1 << (b->N / 2 + b->N % 2);
+    return ED_OK;
+}
+
+extern "C" int ed_basis_export_full(ed_basis b, int64_t* rep_loc, double* state_ph, int64_t* Ia, int64_t* Ib,
+                                    int32_t* bits_right) {
+    if (!b) return ed_fail(ED_EINVAL, "null basis");
+    const int N = b->N, n = b->n;
+    const int half = N / 2 + N % 2;
+    const uint64_t C = b->C;
+    if (rep_loc || state_ph) {
+        int64_t* d_loc = nullptr;
+        cd2* d_ph = nullptr;
+        HIPCHK(hipMalloc(&d_loc, std::max<uint64_t>(C, 1) * sizeof(int64_t)));
+        HIPCHK(hipMalloc(&d_ph, std::max<uint64_t>(C, 1) * sizeof(cd2)));
+        Geo G;
+        basis_geo(b, G);
+        TablesView tv{b->d_rtab, b->d_dir, b->d_phase};
+        uint64_t nth = (C + 31) / 32;
+        if (N <= 32)
+            hipLaunchKernelGGL(k_full_tables<uint32_t>, dim3(nblocks(nth, 256)), dim3(256), 0, 0, C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
+        else
+            hipLaunchKernelGGL(k_full_tables<uint64_t>, dim3(nblocks(nth, 256)), dim3(256), 0, 0, C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
+        HIPCHK(hipGetLastError());
+        if (rep_loc) HIPCHK(hipMemcpy(rep_loc, d_loc, C * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (state_ph) HIPCHK(hipMemcpy(state_ph, d_ph, C * sizeof(cd2), hipMemcpyDeviceToHost));
+        dfree(d_loc); dfree(d_ph);
+    }
+    // Lin tables (Lin_Tables.cpp:57-78) and their Ib completion
+    // (Momentum_Basis.cpp:731-737): Ib[left] = rank(s) - Ia[right] + 1 for any
+    // weight-n state s with that left part, -1 for left parts that never occur.
+    const int64_t nl = (int64_t)1 << half;
+    const int64_t br = nl - 1;
+    if (bits_right) *bits_right = (int32_t)br;
+    if (Ia || Ib) {
+        std::vector<int64_t> ia(nl), ib(nl, -1);
+        for (int64_t r = 0; r < nl; r++) {
+            int w = __builtin_popcountll((unsigned long long)r);
+            uint64_t rk = 0;
+            int t = 0;
+            for (int p = 0; p < half; p++)
+                if ((r >> p) & 1) { t++; rk += binom_u64(p, t); }
+            (void)w;
+            ia[r] = (int64_t)rk + 1;
+        }
+        const int fl = N / 2;
+        for (int64_t left = 0; left < nl && left < ((int64_t)1 << (N - fl)); left++) {
+            int rest = n - __builtin_popcountll((unsigned long long)left);
+            if (rest < 0 || rest > fl) continue;
+            uint64_t s = ((uint64_t)left << fl) | (((uint64_t)1 << rest) - 1);
+            uint64_t rk = 0;
+            int t = 0;
+            for (int p = 0; p < N; p++)
+                if ((s >> p) & 1) { t++; rk += binom_u64(p, t); }
+            ib[left] = (int64_t)rk - ia[s & br] + 1;
+        }
+        if (Ia) std::memcpy(Ia, ia.data(), nl * sizeof(int64_t));
+        if (Ib) std::memcpy(Ib, ib.data(), nl * sizeof(int64_t));
+    }
+    return ED_OK;
+}
+
+// -------------------------------------------------------------- operator --
+extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
+    if (!b || !t || !out) return ed_fail(ED_EINVAL, "null argument");
+    std::vector<Bond> bonds;
+    for (int k = 0; k < t->nbonds; k++) {
+        int x = t->bond_xy[2 * k], y = t->bond_xy[2 * k + 1];
+        if (x < 0 || y < 0 || x >= b->N || y >= b->N || x == y) return ed_fail(ED_EINVAL, "bond %d: bad sites (%d,%d)", k, x, y);
+        if (std::fabs(t->bond_J[k]) < 1e-16) continue;   // H_v.cpp:232-233
+        bonds.push_back(Bond{x, y, t->bond_J[k], t->bond_phase[2 * k], t->bond_phase[2 * k + 1]});
+    }
+    std::vector<Chiral> ch;
+    if (std::fabs(t->Ch) > 1e-16)   // H_v.cpp:263
+        for (int k = 0; k < t->nchiral; k++) {
+            int i = t->chiral_ijk[3 * k], j = t->chiral_ijk[3 * k + 1], kk = t->chiral_ijk[3 * k + 2];
+            if (i < 0 || j < 0 || kk < 0 || i >= b->N || j >= b->N || kk >= b->N || i == j)
+                return ed_fail(ED_EINVAL, "chiral entry %d: bad sites", k);
+            ch.push_back(Chiral{i, j, kk, 0, t->chiral_phase[2 * k], t->chiral_phase[2 * k + 1]});
+        }
+    std::vector<Bowtie> bw;
+    if (std::fabs(t->ChCh) > 1e-16)   // H_v.cpp:305
+        for (int k = 0; k < t->nbowtie; k++) {
+            const int32_t* q = t->bowtie + 6 * k;
+            for (int m = 0; m < 6; m++)
+                if (q[m] < 0 || q[m] >= b->N) return ed_fail(ED_EINVAL, "bowtie %d: bad site", k);
+            if (q[0] == q[1] || q[3] == q[4]) return ed_fail(ED_EINVAL, "bowtie %d: degenerate flip", k);
+            bw.push_back(Bowtie{q[0], q[1], q[2], q[3], q[4], q[5]});
+        }
+    auto* op = new ed_op_s();
+    op->b = b;
+    op->lam = t->lambda; op->Ch = t->Ch; op->ChCh = t->ChCh;
+    op->nb = (int)bonds.size(); op->nc = (int)ch.size(); op->nw = (int)bw.size();
+    if (!bonds.empty()) {
+        HIPCHK(hipMalloc(&op->d_b, bonds.size() * sizeof(Bond)));
+        HIPCHK(hipMemcpy(op->d_b, bonds.data(), bonds.size() * sizeof(Bond), hipMemcpyHostToDevice));
+    }
+    if (!ch.empty()) {
+        HIPCHK(hipMalloc(&op->d_c, ch.size() * sizeof(Chiral)));
+        HIPCHK(hipMemcpy(op->d_c, ch.data(), ch.size() * sizeof(Chiral), hipMemcpyHostToDevice));
+    }
+    if (!bw.empty()) {
+        HIPCHK(hipMalloc(&op->d_w, bw.size() * sizeof(Bowtie)));
+        HIPCHK(hipMemcpy(op->d_w, bw.data(), bw.size() * sizeof(Bowtie), hipMemcpyHostToDevice));
+    }
+    op->bonds = bonds;
+    *out = op;
+    return ED_OK;
+}
+
+extern "C" void ed_op_destroy(ed_op op) {
+    if (!op) return;
+    for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128})
+        if (p) dfree(p);
+    delete op;
+}
+
+TermsView op_terms(const ed_op_s* op) {
+    return TermsView{op->lam, op->Ch, op->ChCh, op->nb, op->nc, op->nw, op->d_b, op->d_c, op->d_w};
+}
+
+template <typename VT>
+static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t ev0 = nullptr,
+                     hipEvent_t ev1 = nullptr) {
+    ed_basis_s* B = op->b;
+    const int64_t D = B->D;
+    if (D == 0) return ED_OK;
+    VT*& u = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
+    if (!u) HIPCHK(hipMalloc((void**)&u, D * sizeof(VT)));
+    hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, u);
+    HIPCHK(hipGetLastError());
+    Geo G;
+    basis_geo(B, G);
+    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
+    TermsView T = op_terms(op);
+    dim3 grid(nblocks(D, 256)), blk(256);
+    if (ev0) HIPCHK(hipEventRecord(ev0, st));
+    if (B->N <= 32) {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_hv_rows<uint32_t, true, VT>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, u, v, w);
+        else
+            hipLaunchKernelGGL((k_hv_rows<uint32_t, false, VT>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, u, v, w);
+    } else {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_hv_rows<uint64_t, true, VT>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, u, v, w);
+        else
+            hipLaunchKernelGGL((k_hv_rows<uint64_t, false, VT>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, u, v, w);
+    }
+    HIPCHK(hipGetLastError());
+    if (ev1) HIPCHK(hipEventRecord(ev1, st));
+    return ED_OK;
+}
+
+int op_hv(ed_op_s* op, int dtype, const void* v, void* w, hipStream_t st) {
+    if (dtype == ED_C128) return launch_hv<cd2>(op, (const cd2*)v, (cd2*)w, st);
+    if (dtype == ED_C64) return launch_hv<cf2>(op, (const cf2*)v, (cf2*)w, st);
+    return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+}
+
+extern "C" int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stream) {
+    if (!op || (!v && op->b->D) || (!w && op->b->D)) return ed_fail(ED_EINVAL, "null argument");
+    if (v == w && op->b->D) return ed_fail(ED_EINVAL, "v and w must not alias");
+    return op_hv(op, dtype, v, w, (hipStream_t)stream);
+}
+
+extern "C" int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, void* stream, float* kernel_ms) {
+    if (!op || !kernel_ms) return ed_fail(ED_EINVAL, "null argument");
+    if (v == w && op->b->D) return ed_fail(ED_EINVAL, "v and w must not alias");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int rc;
+    if (dtype == ED_C128) rc = launch_hv<cd2>(op, (const cd2*)v, (cd2*)w, st, e0, e1);
+    else if (dtype == ED_C64) rc = launch_hv<cf2>(op, (const cf2*)v, (cf2*)w, st, e0, e1);
+    else rc = ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+    *kernel_ms = 0.f;
+    if (rc == ED_OK && op->b->D) {
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(kernel_ms, e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+extern "C" int ed_hv_host(ed_op op, const double* v, double* w) {
+    if (!op) return ed_fail(ED_EINVAL, "null op");
+    const int64_t D = op->b->D;
+    if (D == 0) return ED_OK;
+    void *dv = nullptr, *dw = nullptr;
+    HIPCHK(hipMalloc(&dv, D * sizeof(cd2)));
+    HIPCHK(hipMalloc(&dw, D * sizeof(cd2)));
+    HIPCHK(hipMemcpy(dv, v, D * sizeof(cd2), hipMemcpyHostToDevice));
+    int rc = op_hv(op, ED_C128, dv, dw, 0);
+    if (rc == ED_OK) {
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(w, dw, D * sizeof(cd2), hipMemcpyDeviceToHost));
+    }
+    dfree(dv); dfree(dw);
+    return rc;
+}
+
+extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) {
+    if (!op) return ed_fail(ED_EINVAL, "null op");
+    ed_basis_s* B = op->b;
+    const int64_t D = B->D;
+    unsigned long long* d = nullptr;
+    HIPCHK(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
+    Geo G;
+    basis_geo(B, G);
+    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
+    TermsView T = op_terms(op);
+    if (D > 0) {
+        dim3 grid(nblocks(D, 128)), blk(128);
+        if (B->N <= 32)
+            hipLaunchKernelGGL((k_count_targets<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, d, d + 1);
+        else
+            hipLaunchKernelGGL((k_count_targets<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, d, d + 1);
+        HIPCHK(hipGetLastError());
+    }
+    unsigned long long h[2];
+    HIPCHK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+    dfree(d);
+    if (distinct) *distinct = (int64_t)h[0];
+    if (total) *total = (int64_t)h[1];
+    return ED_OK;
+}
+
+extern "C" int ed_make_h(ed_op op, double* H) {
+    if (!op || !H) return ed_fail(ED_EINVAL, "null argument");
+    ed_basis_s* B = op->b;
+    const int64_t D = B->D;
+    if (D > 20000) return ed_fail(ED_EINVAL, "dense H requested for D=%lld > 20000", (long long)D);
+    if (D == 0) return ED_OK;
+    cd2* dH = nullptr;
+    HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
+    HIPCHK(hipMemset(dH, 0, D * D * sizeof(cd2)));
+    Geo G;
+    basis_geo(B, G);
+    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
+    TermsView T = op_terms(op);
+    dim3 grid(nblocks(D, 64)), blk(64);
+    if (B->N <= 32) {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+        else
+            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+    } else {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+        else
+            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+    dfree(dH);
+    return ED_OK;
+}

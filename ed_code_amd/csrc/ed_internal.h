@@ -1,0 +1,66 @@
+// ed_internal.h -- handle structs shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <vector>
+
+#include "ed_amd.h"
+#include "ed_core.h"
+
+int ed_fail(int code, const char* fmt, ...);
+
+inline void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return ed_fail(e_ == hipErrorOutOfMemory ? ED_ENOMEM : ED_EDEVICE, "%s: %s (%s:%d)", #x, \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                             \
+    } while (0)
+
+struct ed_lattice_s {
+    int N = 0, L1 = 0, L2 = 0;
+    std::vector<int> T1, T2;
+    edc::TProg P1, P2;
+};
+
+struct ed_basis_s {
+    int N = 0, n = 0, kx = 0, ky = 0, L1 = 0, L2 = 0;
+    double px = 0, py = 0;
+    uint64_t C = 0;
+    int64_t D = 0;
+    uint64_t ngroups = 0;
+    bool trivial_k = false;
+    std::vector<int> T1, T2;
+    edc::TProg lat_P1, lat_P2;
+    edc::RankInfo rinfo;
+    uint64_t* d_rows = nullptr;      // [D] representative | lid << 52
+    edc::DirEntry* d_dir = nullptr;  // [ceil(C/96)]
+    uint64_t* d_rtab = nullptr;      // rank tables
+    uint64_t* d_binom = nullptr;     // C(p,t), 65 x 65
+    edc::cd2* d_phase = nullptr;     // [L1*L2]
+    double* d_ntab = nullptr;        // norm by lid
+    double* d_nitab = nullptr;       // 1/norm by lid
+};
+
+struct ed_op_s {
+    ed_basis_s* b = nullptr;
+    double lam = 1, Ch = 0, ChCh = 0;
+    int nb = 0, nc = 0, nw = 0;
+    edc::Bond* d_b = nullptr;
+    edc::Chiral* d_c = nullptr;
+    edc::Bowtie* d_w = nullptr;
+    void* d_u64 = nullptr;    // prescaled gather stream, complex64 [D]
+    void* d_u128 = nullptr;   // prescaled gather stream, complex128 [D]
+    std::vector<edc::Bond> bonds;
+};
+
+int basis_geo(const ed_basis_s* B, edc::Geo& G);
+int basis_build_device(ed_basis_s* B);
+void delete_basis(ed_basis_s* B);
+edc::TermsView op_terms(const ed_op_s* op);
+int op_hv(ed_op_s* op, int dtype, const void* v, void* w, hipStream_t st);

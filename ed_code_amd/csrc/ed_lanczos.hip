@@ -1,0 +1,494 @@
+// ed_lanczos.hip -- device Lanczos around the H*v kernel (include/ed_amd.h,
+// ed_lanczos).  Krylov vectors live in HBM for the whole solve; only alpha,
+// beta and reorthogonalisation coefficients (FP64) cross to the host, where
+// the tridiagonal matrix is diagonalised.
+//
+// Reductions are deterministic: every dot product is a fixed two-stage tree
+// (per-block FP64 partials, then a compensated sum in block order).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "ed_amd.h"
+#include "ed_core.h"
+#include "ed_internal.h"
+#include "ed_tridiag.h"
+
+using namespace edc;
+
+namespace {
+
+constexpr int kRedBlocks = 1024;   // blocks of the reduction kernels
+constexpr int kRedThreads = 256;
+constexpr int kMaxMulti = 64;      // vectors per multi-dot / multi-axpy pass
+
+__device__ __forceinline__ cd2 ld(const cd2* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ cd2 ld(const cf2* p, int64_t i) { cf2 v = p[i]; return cd2{(double)v.re, (double)v.im}; }
+__device__ __forceinline__ void st(cd2* p, int64_t i, double re, double im) { p[i] = cd2{re, im}; }
+__device__ __forceinline__ void st(cf2* p, int64_t i, double re, double im) { p[i] = cf2{(float)re, (float)im}; }
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// (2u-1) + i(2u'-1), u from splitmix64(seed*K + 2i), u' from (seed*K + 2i + 1)
+template <typename VT>
+__global__ void k_start(int64_t D, uint64_t seed, VT* v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const double sc = 1.0 / 9007199254740992.0;
+    uint64_t base = seed * 0x632BE59BD9B4E019ull;
+    double a = (double)(splitmix64(base + 2 * (uint64_t)i) >> 11) * sc;
+    double b = (double)(splitmix64(base + 2 * (uint64_t)i + 1) >> 11) * sc;
+    st(v, i, 2.0 * a - 1.0, 2.0 * b - 1.0);
+}
+
+// partial[blk][k] = sum over the block's rows of conj(X_k[i]) * y[i], k < m.
+// X_k = X + k*ldx.  Block-stride loop, then a tree reduction in LDS.
+template <typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_multidot(int64_t D, int m, const XT* X, int64_t ldx, const YT* y,
+                                                           cd2* partial) {
+    __shared__ double sre[kRedThreads], sim[kRedThreads];
+    double are[kMaxMulti], aim[kMaxMulti];
+    for (int k = 0; k < m; k++) { are[k] = 0.0; aim[k] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        cd2 yv = ld(y, i);
+        for (int k = 0; k < m; k++) {
+            cd2 x = ld(X + k * ldx, i);
+            are[k] += x.re * yv.re + x.im * yv.im;
+            aim[k] += x.re * yv.im - x.im * yv.re;
+        }
+    }
+    for (int k = 0; k < m; k++) {
+        sre[threadIdx.x] = are[k];
+        sim[threadIdx.x] = aim[k];
+        __syncthreads();
+        for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                sre[threadIdx.x] += sre[threadIdx.x + s];
+                sim[threadIdx.x] += sim[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kMaxMulti + k] = cd2{sre[0], sim[0]};
+        __syncthreads();
+    }
+}
+
+// y[i] = a*y[i] - sum_k c[k] * X_k[i]   (c FP64 complex, y updated in FP64)
+template <typename XT, typename YT>
+__global__ void k_multiaxpy(int64_t D, int m, const XT* X, int64_t ldx, const cd2* c, double a, YT* y) {
+    __shared__ cd2 cs[kMaxMulti];
+    if ((int)threadIdx.x < m) cs[threadIdx.x] = c[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        cd2 yv = ld(y, i);
+        double re = a * yv.re, im = a * yv.im;
+        for (int k = 0; k < m; k++) {
+            cd2 x = ld(X + k * ldx, i);
+            re -= cs[k].re * x.re - cs[k].im * x.im;
+            im -= cs[k].re * x.im + cs[k].im * x.re;
+        }
+        st(y, i, re, im);
+    }
+}
+
+// y = s * x (with conversion between storage types)
+template <typename XT, typename YT>
+__global__ void k_scale_copy(int64_t D, double s, const XT* x, YT* y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        cd2 v = ld(x, i);
+        st(y, i, s * v.re, s * v.im);
+    }
+}
+
+// y += sum_k t[k] * X_k  with y in FP64 (Ritz vector assembly, real t)
+template <typename XT>
+__global__ void k_accumulate(int64_t D, int m, const XT* X, int64_t ldx, const double* t, cd2* y) {
+    __shared__ double ts[kMaxMulti];
+    if ((int)threadIdx.x < m) ts[threadIdx.x] = t[threadIdx.x];
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        cd2 yv = y[i];
+        for (int k = 0; k < m; k++) {
+            cd2 x = ld(X + k * ldx, i);
+            yv.re += ts[k] * x.re;
+            yv.im += ts[k] * x.im;
+        }
+        y[i] = yv;
+    }
+}
+
+struct Ctx {
+    ed_op_s* op;
+    int64_t D;
+    hipStream_t st;
+    cd2* d_partial = nullptr;   // [kRedBlocks][kMaxMulti]
+    cd2* d_coef = nullptr;      // [kMaxMulti]
+    double* d_t = nullptr;      // [kMaxMulti]
+    std::vector<cd2> h_partial;
+    double hv_ms = 0.0;
+    int64_t hv_calls = 0;
+    hipEvent_t e0, e1;
+};
+
+unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D + kRedThreads - 1) / kRedThreads); }
+
+// out[k] = X_k^H y for k < m (any m), FP64, fixed reduction order.
+template <typename XT, typename YT>
+int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<std::complex<double>>& out) {
+    out.assign(m, 0.0);
+    unsigned g = grid_for(c.D);
+    for (int k0 = 0; k0 < m; k0 += kMaxMulti) {
+        int mm = std::min(kMaxMulti, m - k0);
+        hipLaunchKernelGGL((k_multidot<XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, mm, X + k0 * ldx, ldx, y,
+                           c.d_partial);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c.h_partial.data(), c.d_partial, (size_t)g * kMaxMulti * sizeof(cd2),
+                              hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        for (int k = 0; k < mm; k++) {
+            // Kahan-compensated sum over the block partials, in block order
+            double sr = 0, cr = 0, si = 0, ci = 0;
+            for (unsigned b = 0; b < g; b++) {
+                double yr = c.h_partial[(size_t)b * kMaxMulti + k].re - cr, ti = sr + yr;
+                cr = (ti - sr) - yr; sr = ti;
+                double yi = c.h_partial[(size_t)b * kMaxMulti + k].im - ci, tj = si + yi;
+                ci = (tj - si) - yi; si = tj;
+            }
+            out[k0 + k] = std::complex<double>(sr, si);
+        }
+    }
+    return ED_OK;
+}
+
+// y = a*y - sum_k coef[k] X_k
+template <typename XT, typename YT>
+int multiaxpy(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<std::complex<double>>& coef, double a, YT* y) {
+    unsigned g = grid_for(c.D);
+    if (m == 0 && a == 1.0) return ED_OK;
+    for (int k0 = 0; k0 < std::max(m, 1); k0 += kMaxMulti) {
+        int mm = std::min(kMaxMulti, m - k0);
+        if (mm < 0) mm = 0;
+        std::vector<cd2> cc(std::max(mm, 1));
+        for (int k = 0; k < mm; k++) cc[k] = cd2{coef[k0 + k].real(), coef[k0 + k].imag()};
+        HIPCHK(hipMemcpyAsync(c.d_coef, cc.data(), cc.size() * sizeof(cd2), hipMemcpyHostToDevice, c.st));
+        hipLaunchKernelGGL((k_multiaxpy<XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, mm, X + (int64_t)k0 * ldx,
+                           ldx, c.d_coef, k0 == 0 ? a : 1.0, y);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c.st));
+    }
+    return ED_OK;
+}
+
+template <typename VT>
+int hv(Ctx& c, const VT* v, VT* w) {
+    HIPCHK(hipEventRecord(c.e0, c.st));
+    int rc = op_hv(c.op, std::is_same<VT, cd2>::value ? ED_C128 : ED_C64, v, w, c.st);
+    if (rc != ED_OK) return rc;
+    HIPCHK(hipEventRecord(c.e1, c.st));
+    HIPCHK(hipEventSynchronize(c.e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.e0, c.e1));
+    c.hv_ms += ms;
+    c.hv_calls++;
+    return ED_OK;
+}
+
+template <typename T>
+int dnorm2(Ctx& c, const T* x, double& out) {
+    std::vector<std::complex<double>> r;
+    int rc = multidot<T, T>(c, 1, x, 0, x, r);
+    out = r[0].real();
+    return rc;
+}
+
+#define RC(x)                              \
+    do {                                   \
+        int rc_ = (x);                     \
+        if (rc_ != ED_OK) return rc_;      \
+    } while (0)
+
+// Lowest eigenpair of the (m x m) tridiagonal (alphas, betas[1..m-1]).
+bool lowest(const std::vector<double>& al, const std::vector<double>& be, double& e0, std::vector<double>* vec) {
+    std::vector<double> d(al), e(be);
+    std::vector<double> z;
+    bool ok = tridiag_ql(d, e, vec ? &z : nullptr);
+    e0 = d.empty() ? 0.0 : d[0];
+    if (vec) vec->assign(z.begin(), z.begin() + al.size());
+    return ok;
+}
+
+// lanczos.cpp:133-349, on the device.  VT = Krylov storage type.
+template <typename VT>
+int run_reference(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D;
+    int how_many = (int)std::min<int64_t>(p.nev, D);
+    int iterations = p.iterations;
+    unsigned g = grid_for(D);
+    VT *V = nullptr, *vp = nullptr, *w = nullptr;
+    int maxit = (int)std::min<int64_t>(iterations, D);
+    HIPCHK(hipMalloc(&V, (size_t)std::max(maxit, 1) * D * sizeof(VT)));
+    HIPCHK(hipMalloc(&vp, D * sizeof(VT)));
+    HIPCHK(hipMalloc(&w, D * sizeof(VT)));
+    cd2* x = nullptr;
+    HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
+    std::vector<std::complex<double>> q;
+    for (int num = 0; num < how_many; num++) {
+        iterations = (int)std::min<int64_t>(iterations, D - num);
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed + (uint64_t)num, vp);
+        HIPCHK(hipGetLastError());
+        double previous = 1000.0, e0 = 0.0;
+        std::vector<double> tvec;
+        for (int cycle = 0; cycle < p.ncycles; cycle++) {
+            // orth_wrt_previous_evecs + normalize (lanczos.cpp:199-200)
+            for (size_t k = 0; k < prev.size(); k++) {
+                RC((multidot<cd2, VT>(c, 1, prev[k], 0, vp, q)));
+                RC((multiaxpy<cd2, VT>(c, 1, prev[k], 0, q, 1.0, vp)));
+            }
+            double nn;
+            RC(dnorm2(c, vp, nn));
+            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), vp, vp);
+            std::vector<double> alphas, betas(1, 0.0);
+            int nvs = 0;
+            for (int it = 0; it < iterations; it++) {
+                RC(hv<VT>(c, vp, w));
+                if (it != 0) {
+                    std::vector<std::complex<double>> b1(1, betas.back());
+                    RC((multiaxpy<VT, VT>(c, 1, V + (int64_t)(it - 1) * D, 0, b1, 1.0, w)));
+                }
+                RC((multidot<VT, VT>(c, 1, vp, 0, w, q)));          // alpha = v^H w
+                std::complex<double> alpha = q[0];
+                alphas.push_back(alpha.real());
+                std::vector<std::complex<double>> a1(1, alpha);
+                RC((multiaxpy<VT, VT>(c, 1, vp, 0, a1, 1.0, w)));
+                hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, vp, V + (int64_t)it * D);
+                nvs = it + 1;
+                double bb;
+                RC(dnorm2(c, w, bb));
+                double beta = std::sqrt(bb);
+                hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, vp);
+                betas.push_back(beta);
+                for (size_t k = 0; k < prev.size(); k++) {
+                    RC((multidot<cd2, VT>(c, 1, prev[k], 0, vp, q)));
+                    RC((multiaxpy<cd2, VT>(c, 1, prev[k], 0, q, 1.0, vp)));
+                }
+                if (cycle == p.ncycles - 1) {
+                    // classical Gram-Schmidt against this cycle's vectors (lanczos.cpp:256-260)
+                    RC((multidot<VT, VT>(c, nvs, V, D, vp, q)));
+                    RC((multiaxpy<VT, VT>(c, nvs, V, D, q, 1.0, vp)));
+                }
+                RC(dnorm2(c, vp, nn));
+                hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), vp, vp);
+                std::vector<double> off(betas.begin() + 1, betas.end() - 1);
+                if (!lowest(alphas, off, e0, nullptr)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+                if (cycle != 0 && cycle != p.ncycles - 1 && it > 1) {
+                    if (std::fabs(e0 - previous) < p.tol) { it = iterations; cycle = p.ncycles - 2; }
+                    previous = e0;
+                } else if (cycle != p.ncycles - 1) {
+                    previous = e0;
+                }
+            }
+            // Ritz vector (lanczos.cpp:312-320), assembled in FP64
+            std::vector<double> off(betas.begin() + 1, betas.end() - 1);
+            off.resize(alphas.size() > 0 ? alphas.size() - 1 : 0);
+            if (!lowest(alphas, off, e0, &tvec)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+            HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+            for (int k0 = 0; k0 < nvs; k0 += kMaxMulti) {
+                int mm = std::min(kMaxMulti, nvs - k0);
+                HIPCHK(hipMemcpyAsync(c.d_t, tvec.data() + k0, mm * sizeof(double), hipMemcpyHostToDevice, c.st));
+                hipLaunchKernelGGL((k_accumulate<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, mm, V + (int64_t)k0 * D, D, c.d_t, x);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipStreamSynchronize(c.st));
+            }
+            for (size_t k = 0; k < prev.size(); k++) {
+                RC((multidot<cd2, cd2>(c, 1, prev[k], 0, x, q)));
+                RC((multiaxpy<cd2, cd2>(c, 1, prev[k], 0, q, 1.0, x)));
+            }
+            RC(dnorm2(c, x, nn));
+            hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, x, vp);
+        }
+        cd2* keep = nullptr;
+        HIPCHK(hipMalloc(&keep, D * sizeof(cd2)));
+        HIPCHK(hipMemcpyAsync(keep, x, D * sizeof(cd2), hipMemcpyDeviceToDevice, c.st));
+        prev.push_back(keep);
+        evals[num] = e0;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    dfree(V); dfree(vp); dfree(w); dfree(x);
+    return ED_OK;
+}
+
+// One cycle with full (two-pass classical Gram-Schmidt) reorthogonalisation
+// against every Krylov vector and every earlier eigenvector, each step.
+template <typename VT>
+int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D;
+    int how_many = (int)std::min<int64_t>(p.nev, D);
+    unsigned g = grid_for(D);
+    int m = (int)std::min<int64_t>(p.iterations, D);
+    VT *V = nullptr, *w = nullptr;
+    HIPCHK(hipMalloc(&V, (size_t)(m + 1) * D * sizeof(VT)));
+    HIPCHK(hipMalloc(&w, D * sizeof(VT)));
+    cd2* x = nullptr;
+    HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
+    std::vector<std::complex<double>> q;
+    for (int num = 0; num < how_many; num++) {
+        int mit = (int)std::min<int64_t>(m, D - num);
+        VT* v0 = V;
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed + (uint64_t)num, v0);
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t k = 0; k < prev.size(); k++) {
+                RC((multidot<cd2, VT>(c, 1, prev[k], 0, v0, q)));
+                RC((multiaxpy<cd2, VT>(c, 1, prev[k], 0, q, 1.0, v0)));
+            }
+        double nn;
+        RC(dnorm2(c, v0, nn));
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), v0, v0);
+        std::vector<double> alphas, betas;
+        double e0 = 0.0, eprev = 1e300;
+        int k = 0;
+        for (; k < mit; k++) {
+            VT* vk = V + (int64_t)k * D;
+            RC(hv<VT>(c, vk, w));
+            RC((multidot<VT, VT>(c, 1, vk, 0, w, q)));
+            double alpha = q[0].real();
+            std::vector<std::complex<double>> cf{alpha};
+            if (k > 0) {
+                cf.insert(cf.begin(), betas.back());
+                RC((multiaxpy<VT, VT>(c, 2, V + (int64_t)(k - 1) * D, D, cf, 1.0, w)));
+            } else {
+                RC((multiaxpy<VT, VT>(c, 1, vk, 0, cf, 1.0, w)));
+            }
+            for (int pass = 0; pass < 2; pass++) {
+                RC((multidot<VT, VT>(c, k + 1, V, D, w, q)));
+                if (pass == 0) alpha += q[k].real();
+                RC((multiaxpy<VT, VT>(c, k + 1, V, D, q, 1.0, w)));
+                for (size_t e = 0; e < prev.size(); e++) {
+                    RC((multidot<cd2, VT>(c, 1, prev[e], 0, w, q)));
+                    RC((multiaxpy<cd2, VT>(c, 1, prev[e], 0, q, 1.0, w)));
+                }
+            }
+            alphas.push_back(alpha);
+            double bb;
+            RC(dnorm2(c, w, bb));
+            double beta = std::sqrt(bb);
+            bool stop = false;
+            if ((k + 1) % 5 == 0 || k + 1 == mit) {
+                std::vector<double> off(betas);
+                if (!lowest(alphas, off, e0, nullptr)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+                if (std::fabs(e0 - eprev) < p.tol) stop = true;
+                eprev = e0;
+            }
+            if (beta < 1e-13 * std::max(1.0, std::fabs(alpha))) stop = true;   // invariant subspace
+            if (stop || k + 1 == mit) { k++; break; }
+            betas.push_back(beta);
+            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, V + (int64_t)(k + 1) * D);
+        }
+        int nvs = k;
+        std::vector<double> tvec, off(betas.begin(), betas.begin() + std::max(0, nvs - 1));
+        std::vector<double> al(alphas.begin(), alphas.begin() + nvs);
+        if (!lowest(al, off, e0, &tvec)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+        HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+        for (int k0 = 0; k0 < nvs; k0 += kMaxMulti) {
+            int mm = std::min(kMaxMulti, nvs - k0);
+            HIPCHK(hipMemcpyAsync(c.d_t, tvec.data() + k0, mm * sizeof(double), hipMemcpyHostToDevice, c.st));
+            hipLaunchKernelGGL((k_accumulate<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, mm, V + (int64_t)k0 * D, D, c.d_t, x);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(c.st));
+        }
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t e = 0; e < prev.size(); e++) {
+                RC((multidot<cd2, cd2>(c, 1, prev[e], 0, x, q)));
+                RC((multiaxpy<cd2, cd2>(c, 1, prev[e], 0, q, 1.0, x)));
+            }
+        RC(dnorm2(c, x, nn));
+        hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+        cd2* keep = nullptr;
+        HIPCHK(hipMalloc(&keep, D * sizeof(cd2)));
+        HIPCHK(hipMemcpyAsync(keep, x, D * sizeof(cd2), hipMemcpyDeviceToDevice, c.st));
+        prev.push_back(keep);
+        evals[num] = e0;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    dfree(V); dfree(w); dfree(x);
+    return ED_OK;
+}
+
+// FP64 residual ||H x - (x^H H x) x|| and Rayleigh quotient of unit vector x.
+int fp64_check(Ctx& c, const cd2* x, double& rq, double& res) {
+    cd2* y = nullptr;
+    HIPCHK(hipMalloc(&y, c.D * sizeof(cd2)));
+    RC(hv<cd2>(c, x, y));
+    std::vector<std::complex<double>> q;
+    RC((multidot<cd2, cd2>(c, 1, x, 0, y, q)));
+    rq = q[0].real();
+    std::vector<std::complex<double>> a1(1, rq);
+    RC((multiaxpy<cd2, cd2>(c, 1, x, 0, a1, 1.0, y)));
+    double rr;
+    RC(dnorm2(c, y, rr));
+    res = std::sqrt(rr);
+    dfree(y);
+    return ED_OK;
+}
+
+}  // namespace
+
+extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evecs, ed_lanczos_stats* stats) {
+    if (!op || !p || !evals) return ed_fail(ED_EINVAL, "null argument");
+    if (p->nev < 1 || p->iterations < 1 || p->ncycles < 1) return ed_fail(ED_EINVAL, "nev, iterations, ncycles must be >= 1");
+    if (p->precision != ED_C64 && p->precision != ED_C128) return ed_fail(ED_EINVAL, "bad precision %d", p->precision);
+    const int64_t D = op->b->D;
+    if (D == 0) return ed_fail(ED_EEMPTY, "No states in psector");
+    auto t0 = std::chrono::steady_clock::now();
+    Ctx c;
+    c.op = op;
+    c.D = D;
+    c.st = 0;
+    c.h_partial.resize((size_t)kRedBlocks * kMaxMulti);
+    HIPCHK(hipMalloc(&c.d_partial, (size_t)kRedBlocks * kMaxMulti * sizeof(cd2)));
+    HIPCHK(hipMalloc(&c.d_coef, kMaxMulti * sizeof(cd2)));
+    HIPCHK(hipMalloc(&c.d_t, kMaxMulti * sizeof(double)));
+    HIPCHK(hipEventCreate(&c.e0));
+    HIPCHK(hipEventCreate(&c.e1));
+    std::vector<cd2*> prev;
+    int rc;
+    if (p->mode == ED_LZ_REFERENCE)
+        rc = p->precision == ED_C128 ? run_reference<cd2>(c, *p, prev, evals) : run_reference<cf2>(c, *p, prev, evals);
+    else
+        rc = p->precision == ED_C128 ? run_full<cd2>(c, *p, prev, evals) : run_full<cf2>(c, *p, prev, evals);
+    if (rc == ED_OK) {
+        for (size_t k = 0; k < prev.size(); k++) {
+            double rq = 0, res = 0;
+            if (p->precision == ED_C64 || (stats && k < 16)) {
+                rc = fp64_check(c, prev[k], rq, res);
+                if (rc != ED_OK) break;
+                if (p->precision == ED_C64) evals[k] = rq;   // FP64 Rayleigh-quotient correction
+                if (stats && k < 16) { stats->residual[k] = res; stats->rayleigh[k] = rq; }
+            }
+            if (evecs) {
+                hipError_t e = hipMemcpy(evecs + 2 * D * k, prev[k], D * sizeof(cd2), hipMemcpyDeviceToHost);
+                if (e != hipSuccess) { rc = ed_fail(ED_EDEVICE, "copy of eigenvector: %s", hipGetErrorString(e)); break; }
+            }
+        }
+    }
+    for (auto* v : prev) dfree(v);
+    dfree(c.d_partial); dfree(c.d_coef); dfree(c.d_t);
+    (void)hipEventDestroy(c.e0);
+    (void)hipEventDestroy(c.e1);
+    if (stats) {
+        stats->hv_calls = c.hv_calls;
+        stats->hv_seconds = c.hv_ms * 1e-3;
+        stats->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return rc;
+}

@@ -1,0 +1,105 @@
+// ed_tables.h -- host-side construction of the small per-sector tables
+// (translation programs, rank tables, phase and norm tables).  Plain C++; used
+// by the HIP library and by the CPU unit-test build of ed_core.h.
+#pragma once
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#include "ed_core.h"
+
+namespace edc {
+
+inline uint64_t binom_u64(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    if (k > n - k) k = n - k;
+    unsigned __int128 r = 1;
+    for (int i = 1; i <= k; i++) r = r * (unsigned)(n - k + i) / (unsigned)i;
+    return (uint64_t)r;
+}
+
+// Groups the sites of permutation T by displacement d = T[i] - i.
+inline bool make_prog(const std::vector<int>& T, TProg& P) {
+    std::memset(&P, 0, sizeof(P));
+    std::vector<int> ds;
+    std::vector<uint64_t> ms;
+    for (size_t i = 0; i < T.size(); i++) {
+        int d = T[i] - (int)i;
+        size_t g = 0;
+        while (g < ds.size() && ds[g] != d) g++;
+        if (g == ds.size()) { ds.push_back(d); ms.push_back(0); }
+        ms[g] |= (uint64_t)1 << i;
+    }
+    if ((int)ds.size() > kMaxGroups) return false;
+    P.ng = (int)ds.size();
+    for (int g = 0; g < P.ng; g++) { P.mask[g] = ms[g]; P.sh[g] = ds[g]; }
+    return true;
+}
+
+// Cycle length of site 0 (LxLy, Momentum_Basis.cpp:19-36).
+inline int cycle_len0(const std::vector<int>& T) {
+    int s = 0, n = 0;
+    do { s = T[s]; n++; } while (s != 0 && n <= (int)T.size());
+    return n;
+}
+
+// exp(-i(px lx + py ly)) for lx < L1, ly < L2, the expression of
+// Momentum_Basis.cpp:753, indexed lx*L2 + ly.
+inline std::vector<cd2> phase_table(int L1, int L2, double px, double py) {
+    std::vector<cd2> ph(L1 * L2);
+    for (int lx = 0; lx < L1; lx++)
+        for (int ly = 0; ly < L2; ly++) {
+            std::complex<double> z = std::exp(std::complex<double>(0.0, -px * lx - py * ly));
+            ph[lx * L2 + ly] = cd2{z.real(), z.imag()};
+        }
+    return ph;
+}
+
+// norm = sqrt(L1*L2/lid), 1/norm as Momentum_Basis.cpp:764-766, indexed by lid.
+inline void norm_tables(int L1, int L2, std::vector<double>& nt, std::vector<double>& nit) {
+    nt.assign(L1 * L2 + 1, 0.0);
+    nit.assign(L1 * L2 + 1, 0.0);
+    for (int lid = 1; lid <= L1 * L2; lid++) {
+        int64_t orb = (int64_t)L1 * L2 / lid;
+        nt[lid] = std::sqrt((double)orb);
+        nit[lid] = 1 / std::sqrt((double)orb);
+    }
+}
+
+inline std::vector<uint64_t> binom_table() {
+    std::vector<uint64_t> bn(65 * 65);
+    for (int p = 0; p < 65; p++)
+        for (int t = 0; t < 65; t++) bn[p * 65 + t] = binom_u64(p, t);
+    return bn;
+}
+
+// Byte-sliced combinatorial-number-system rank tables for weight-n words of
+// N bits: entry (k, c, b) = sum over the set bits q of byte value b, taken in
+// increasing order with local index i = 1, 2, ..., of C(8k + q, c + i).
+inline std::vector<uint64_t> rank_tables(int N, int n, RankInfo& R) {
+    std::memset(&R, 0, sizeof(R));
+    R.nbytes = (N + 7) / 8;
+    std::vector<uint64_t> rt;
+    int off = 0;
+    for (int k = 0; k < R.nbytes; k++) {
+        int cmin = std::max(0, n - (N - 8 * k));
+        int cmax = std::min(n, 8 * k);
+        R.off[k] = off;
+        R.cmin[k] = cmin;
+        for (int c = cmin; c <= cmax; c++) {
+            for (int b = 0; b < 256; b++) {
+                uint64_t v = 0;
+                int i = 0;
+                for (int q = 0; q < 8; q++)
+                    if ((b >> q) & 1) { i++; v += binom_u64(8 * k + q, c + i); }
+                rt.push_back(v);
+            }
+            off++;
+        }
+    }
+    return rt;
+}
+
+}  // namespace edc

@@ -1,0 +1,153 @@
+/* ed_amd.h -- C ABI of the MI355X (gfx950) momentum/Sz-basis H*v library.
+ *
+ * Plain C: handles, raw pointers, sizes and status codes; no C++ or torch
+ * types cross this boundary.  Each entry point names the reference interface
+ * (hiteshjc/ED_Code, paths relative to its root) it replaces.  The reference
+ * signatures themselves (std::vector by reference) are provided on top of this
+ * ABI by the C++ shims in ed_code_amd/csrc/ed_refapi.cpp.
+ *
+ * Conventions: a state is a bit string, bit s = site s, bit 1 = spin up.  A
+ * translation T moves bit i to bit T[i].  Complex vectors are interleaved
+ * (re, im) pairs: complex64 = 2 x float, complex128 = 2 x double.  Every call
+ * returns ED_OK or an error code; ed_last_error() gives the message of the
+ * last failure on the calling thread.
+ */
+#ifndef ED_AMD_H
+#define ED_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ed_status {
+    ED_OK = 0,
+    ED_EINVAL = 1,   /* bad argument (replaces the reference's exit(1) paths) */
+    ED_ENOMEM = 2,   /* device allocation failed or footprint over budget */
+    ED_EDEVICE = 3,  /* HIP runtime error */
+    ED_EEMPTY = 4,   /* empty sector (Lanczos_sector.cpp:93-97) */
+    ED_ENOTCONV = 5  /* eigensolver did not converge */
+};
+
+enum ed_dtype { ED_C64 = 0, ED_C128 = 1 };
+
+/* Hamiltonian terms: the reference's adj_list / ch_list / chch_list
+ * (global.h:25-40, built by read_input_file.cpp:193-319).  Same order and
+ * meaning; lambda is Jz (Delta).  Arrays are host memory, read at
+ * ed_op_create. */
+typedef struct ed_terms {
+    double lambda;
+    int32_t nbonds;
+    const int32_t* bond_xy;     /* [2*nbonds] sites x, y            */
+    const double* bond_J;       /* [nbonds]                         */
+    const double* bond_phase;   /* [2*nbonds] exp(i phi) as re, im  */
+    double Ch;
+    int32_t nchiral;
+    const int32_t* chiral_ijk;  /* [3*nchiral] i, j, k              */
+    const double* chiral_phase; /* [2*nchiral]                      */
+    double ChCh;
+    int32_t nbowtie;
+    const int32_t* bowtie;      /* [6*nbowtie] x, y, z, p, q, r     */
+} ed_terms;
+
+typedef struct ed_lattice_s* ed_lattice;
+typedef struct ed_basis_s* ed_basis;
+typedef struct ed_op_s* ed_op;
+
+const char* ed_last_error(void);
+
+/* Selects the HIP device used by the calling thread. */
+int ed_init(int device);
+
+/* Translation group from T1/T2 (read_input_file.cpp T1=/T2= keys).
+ * L1, L2 = cycle lengths of site 0 (LxLy, Momentum_Basis.cpp:19-36).
+ * Fails with ED_EINVAL when T1/T2 are not permutations of the same length
+ * (read_input_file.cpp:423-427) or do not commute. */
+int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32_t* T2, ed_lattice* out);
+int ed_lattice_orders(ed_lattice lat, int32_t* L1, int32_t* L2);
+void ed_lattice_destroy(ed_lattice lat);
+
+/* Momentum basis of the sector (n_ones, kx, ky), built on the GPU.
+ * Replaces pConstruct_States (Momentum_Basis.cpp:645-778, decl
+ * Momentum_Basis.h:151-165): representatives, orbit norms and the index
+ * directory stay resident in HBM.  kx, ky are integers in [0,L1) x [0,L2)
+ * (Lanczos_sector.cpp:51-57); px = 2 pi kx / L1, py = 2 pi ky / L2. */
+int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out);
+int ed_basis_dim(ed_basis b, int64_t* D);
+/* Host copies of pblock_states, norm, norm_inv ([D] each; any may be NULL). */
+int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* norm_inv);
+/* The reference's full-space tables, [C] with C = C(N, n_ones): rep_loc (index
+ * of the representative of each state, -1 if its orbit is not in the sector)
+ * and state_ph (complex128) in ascending state order, plus the Lin tables
+ * Ia/Ib ([2^ceil(N/2)]) and bits_right (Momentum_Basis.cpp:697-777,
+ * Lin_Tables.cpp:57-78).  Only the reference-signature shims and small-sector
+ * paths need them; the GPU operator never reads them. */
+int ed_basis_full_size(ed_basis b, int64_t* C, int64_t* nlin);
+int ed_basis_export_full(ed_basis b, int64_t* rep_loc, double* state_ph, int64_t* Ia, int64_t* Ib,
+                         int32_t* bits_right);
+void ed_basis_destroy(ed_basis b);
+
+/* Operator = basis + terms, terms uploaded once. */
+int ed_op_create(ed_basis b, const ed_terms* terms, ed_op* out);
+void ed_op_destroy(ed_op op);
+/* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
+ * total number of off-diagonal elements (one counting pass on the GPU). */
+int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total);
+
+/* w = H v (sHv, H_v.cpp:192-357, decl H_v.h:36-57).  v, w are DEVICE
+ * pointers to D complex numbers of type dtype; w is overwritten; v and w must
+ * not alias.  stream is a hipStream_t (NULL = default stream).  The call is
+ * asynchronous with respect to the host. */
+int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stream);
+/* ed_hv with the dominant kernel bracketed by HIP events on `stream`:
+ * *kernel_ms receives the duration of the row kernel alone (the v/norm
+ * prescale pass is excluded).  Synchronises the stream.  No reference
+ * counterpart: measurement helper for roofline accounting. */
+int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, void* stream, float* kernel_ms);
+/* Same with HOST complex128 pointers (the reference sHv calling convention);
+ * copies in and out, synchronous. */
+int ed_hv_host(ed_op op, const double* v, double* w);
+/* Dense D x D column-major complex128 H (makeH, H_v.cpp:21-190), D <= 20000. */
+int ed_make_h(ed_op op, double* H);
+
+/* Lanczos (lanczos_evecs, lanczos.cpp:133-349, decl lanczos.h:71-99).
+ * mode ED_LZ_REFERENCE keeps the reference's restart schedule: ncycles cycles
+ * of `iterations` steps, deflation against earlier eigenvectors every step,
+ * full Gram-Schmidt against the cycle's vectors in the last cycle only, early
+ * jump to the last cycle when |E0 - E0_prev| < tol.  mode ED_LZ_FULL runs one
+ * cycle with full reorthogonalisation every step.  precision ED_C128 keeps
+ * every vector in complex128; ED_C64 stores the Krylov vectors in complex64
+ * and keeps FP64 for alpha/beta, reorthogonalisation coefficients, the
+ * tridiagonal eigensolve and a final FP64 residual / Rayleigh-quotient
+ * correction.  The start vector is (2u-1) + i(2u'-1) with u, u' from
+ * splitmix64(seed*K + 2i), (seed*K + 2i + 1) (the reference uses unseeded
+ * rand(), lanczos.cpp:192); eigenvalue number m uses seed + m. */
+enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1 };
+
+typedef struct ed_lanczos_params {
+    int32_t nev;
+    int32_t iterations;
+    int32_t ncycles;
+    int32_t precision;   /* ed_dtype */
+    int32_t mode;        /* ed_lanczos_mode */
+    int32_t pad;
+    double tol;
+    uint64_t seed;
+} ed_lanczos_params;
+
+typedef struct ed_lanczos_stats {
+    int64_t hv_calls;
+    double hv_seconds;      /* device time in H*v kernels */
+    double total_seconds;
+    double residual[16];    /* FP64 ||H x - E x|| of the first 16 eigenvectors */
+    double rayleigh[16];    /* FP64 Rayleigh quotient x^H H x / x^H x */
+} ed_lanczos_stats;
+
+/* evals [nev]; evecs (host, complex128 [nev][D]) may be NULL; stats may be NULL. */
+int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evecs, ed_lanczos_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ED_AMD_H */

@@ -1,0 +1,265 @@
+"""ctypes bindings for the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/libedoracle.so (the restatement) and, when built,
+oracle/_ref/libedref.so (the reference's own basis + H*v slice).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libedoracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libedref.so")
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+
+
+class Terms(C.Structure):
+    _fields_ = [
+        ("lam", C.c_double), ("nbonds", C.c_int32), ("bond_xy", C.c_void_p), ("bond_J", C.c_void_p),
+        ("bond_phase", C.c_void_p), ("Ch", C.c_double), ("nchiral", C.c_int32), ("chiral_ijk", C.c_void_p),
+        ("chiral_phase", C.c_void_p), ("ChCh", C.c_double), ("nbowtie", C.c_int32), ("bowtie", C.c_void_p),
+    ]
+
+
+def make_terms(model):
+    """Returns (Terms, keepalive) for a lattices.Model."""
+    a = model.term_arrays()
+    keep = [np.ascontiguousarray(a[k]) for k in ("bond_xy", "bond_J", "bond_phase", "chiral_ijk", "chiral_phase", "bowtie")]
+    t = Terms(a["lam"], a["nbonds"], keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data,
+              a["Ch"], a["nchiral"], keep[3].ctypes.data, keep[4].ctypes.data,
+              a["ChCh"], a["nbowtie"], keep[5].ctypes.data)
+    return t, keep
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle`)")
+        L = C.CDLL(ORACLE_SO)
+        L.or_n_choose_k.restype = C.c_int64
+        L.or_n_choose_k.argtypes = [C.c_int, C.c_int]
+        L.or_gosper.argtypes = [C.c_int, C.c_int, i64p]
+        L.or_lin_tables.argtypes = [C.c_int, i64p, i64p]
+        L.or_translate.restype = C.c_int64
+        L.or_translate.argtypes = [C.c_int64, i32p, C.c_int]
+        L.or_LxLy.argtypes = [i32p, i32p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.or_representative.restype = C.c_int64
+        L.or_representative.argtypes = [C.c_int64, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_int32)]
+        L.or_basis_build.restype = C.c_void_p
+        L.or_basis_build.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double]
+        L.or_basis_free.argtypes = [C.c_void_p]
+        L.or_basis_info.argtypes = [C.c_void_p, i64p]
+        L.or_basis_lin_error.restype = C.c_int32
+        L.or_basis_lin_error.argtypes = [C.c_void_p]
+        L.or_basis_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.or_hv.argtypes = [C.c_void_p, C.POINTER(Terms), f64p, f64p]
+        L.or_hv_rows.argtypes = [C.c_void_p, C.POINTER(Terms), f64p, f64p, C.c_int64, C.c_int64]
+        L.or_count_targets.argtypes = [C.c_void_p, C.POINTER(Terms), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.or_make_h.argtypes = [C.c_void_p, C.POINTER(Terms), f64p]
+        L.or_start_vector.argtypes = [C.c_uint64, C.c_int64, f64p]
+        L.or_lanczos.restype = C.c_int64
+        L.or_lanczos.argtypes = [C.c_void_p, C.POINTER(Terms), C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
+                                 f64p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference slice not built: {REF_SO}")
+        L = C.CDLL(REF_SO)
+        L.ref_n_choose_k.restype = C.c_int64
+        L.ref_n_choose_k.argtypes = [C.c_int, C.c_int]
+        L.ref_constrained_dets.argtypes = [C.c_int, C.c_int, i64p]
+        L.ref_LxLy.argtypes = [i32p, i32p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.ref_representative.restype = C.c_int64
+        L.ref_representative.argtypes = [C.c_int64, i32p, i32p, C.c_int, C.c_double, C.c_double,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32)]
+        L.ref_basis_build.restype = C.c_void_p
+        L.ref_basis_build.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double]
+        L.ref_basis_free.argtypes = [C.c_void_p]
+        L.ref_basis_info.argtypes = [C.c_void_p, i64p]
+        L.ref_basis_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 7
+        L.ref_hv.argtypes = [C.c_void_p, C.c_void_p, f64p, f64p, C.c_int64]
+        L.ref_make_h.argtypes = [C.c_void_p, C.c_void_p, f64p]
+        L.ref_tables_new.restype = C.c_void_p
+        L.ref_tables_new.argtypes = [C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64,
+                                     C.c_int]
+        L.ref_tables_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        _ref = L
+    return _ref
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleBasis:
+    """Oracle restatement of pConstruct_States (Momentum_Basis.cpp:645-778)."""
+
+    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0):
+        self.model = model
+        self.n_ones = n_ones
+        self.px, self.py = model.momentum(kx, ky)
+        L = lib()
+        self.h = L.or_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                  np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
+        info = np.zeros(8, np.int64)
+        L.or_basis_info(self.h, info)
+        self.N, _, self.L1, self.L2, self.half, self.bits_right, self.C, self.D = [int(x) for x in info]
+        self.terms, self._keep = make_terms(model)
+
+    def __del__(self):
+        try:
+            lib().or_basis_free(self.h)
+        except Exception:
+            pass
+
+    def tables(self, full=True):
+        D, C = self.D, self.C
+        reps = np.zeros(D, np.int64)
+        norm = np.zeros(D)
+        norm_inv = np.zeros(D)
+        rep_loc = np.zeros(C, np.int64) if full else None
+        sph = np.zeros(2 * C) if full else None
+        nh = 1 << self.half
+        Ia = np.zeros(nh, np.int64)
+        Ib = np.zeros(nh, np.int64)
+        lib().or_basis_copy(self.h, _ptr(reps), _ptr(norm), _ptr(norm_inv), _ptr(rep_loc), _ptr(sph), _ptr(Ia), _ptr(Ib))
+        out = dict(reps=reps, norm=norm, norm_inv=norm_inv, Ia=Ia, Ib=Ib)
+        if full:
+            out["rep_loc"] = rep_loc
+            out["state_ph"] = sph.view(np.complex128)
+        return out
+
+    def set_model(self, model):
+        self.model = model
+        self.terms, self._keep = make_terms(model)
+
+    def hv(self, v: np.ndarray) -> np.ndarray:
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        lib().or_hv(self.h, C.byref(self.terms), v.view(np.float64), w.view(np.float64))
+        return w
+
+    def hv_rows(self, v, r0, r1):
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        lib().or_hv_rows(self.h, C.byref(self.terms), v.view(np.float64), w.view(np.float64), r0, r1)
+        return w
+
+    def make_h(self) -> np.ndarray:
+        H = np.zeros(self.D * self.D, np.complex128)
+        lib().or_make_h(self.h, C.byref(self.terms), H.view(np.float64))
+        return H.reshape(self.D, self.D, order="F")
+
+    def count_targets(self):
+        g = C.c_int64(); t = C.c_int64()
+        lib().or_count_targets(self.h, C.byref(self.terms), C.byref(g), C.byref(t))
+        return g.value, t.value
+
+    def lanczos(self, nev=1, iterations=100, ncycles=10, tol=1e-11, seed=1234, want_vectors=False):
+        ev = np.zeros(nev)
+        vecs = np.zeros((nev, self.D), np.complex128) if want_vectors else None
+        n = lib().or_lanczos(self.h, C.byref(self.terms), nev, iterations, ncycles, tol, seed, ev,
+                             None if vecs is None else vecs.ctypes.data)
+        return (ev, vecs, n) if want_vectors else ev
+
+
+def start_vector(seed: int, D: int) -> np.ndarray:
+    v = np.zeros(2 * D)
+    lib().or_start_vector(seed, D, v)
+    return v.view(np.complex128)
+
+
+class RefBasis:
+    """The reference's own pConstruct_States / sHv / makeH (compiled slice)."""
+
+    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0):
+        self.model = model
+        self.px, self.py = model.momentum(kx, ky)
+        R = ref()
+        self.h = R.ref_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                   np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
+        info = np.zeros(4, np.int64)
+        R.ref_basis_info(self.h, info)
+        self.D, self.C, self.nIa, self.bits_right = [int(x) for x in info]
+        self.terms, self._keep = make_terms(model)
+
+    def __del__(self):
+        try:
+            ref().ref_basis_free(self.h)
+        except Exception:
+            pass
+
+    def tables(self):
+        D, C = self.D, self.C
+        reps = np.zeros(D, np.int64); norm = np.zeros(D); norm_inv = np.zeros(D)
+        rep_loc = np.zeros(C, np.int64); sph = np.zeros(2 * C)
+        Ia = np.zeros(self.nIa, np.int64); Ib = np.zeros(self.nIa, np.int64)
+        ref().ref_basis_copy(self.h, _ptr(reps), _ptr(norm), _ptr(norm_inv), _ptr(rep_loc), _ptr(sph), _ptr(Ia), _ptr(Ib))
+        return dict(reps=reps, norm=norm, norm_inv=norm_inv, rep_loc=rep_loc, state_ph=sph.view(np.complex128),
+                    Ia=Ia, Ib=Ib)
+
+    def set_model(self, model):
+        self.model = model
+        self.terms, self._keep = make_terms(model)
+
+    def hv(self, v, rows=-1):
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        ref().ref_hv(self.h, C.addressof(self.terms), v.view(np.float64), w.view(np.float64), rows)
+        return w
+
+    def make_h(self):
+        H = np.zeros(self.D * self.D, np.complex128)
+        ref().ref_make_h(self.h, C.addressof(self.terms), H.view(np.float64))
+        return H.reshape(self.D, self.D, order="F")
+
+
+class RefSHv:
+    """The reference sHv run over tables supplied from outside (timing only)."""
+
+    def __init__(self, model, px, py, C_, D, nlin, bits_right):
+        R = ref()
+        self.model = model
+        self.D = D
+        self.h = R.ref_tables_new(model.n_sites, np.ascontiguousarray(model.T1, np.int32),
+                                  np.ascontiguousarray(model.T2, np.int32), px, py, C_, D, nlin, bits_right)
+        ptrs = (C.c_void_p * 7)()
+        R.ref_tables_ptrs(self.h, ptrs)
+        self.ptrs = [p for p in ptrs]
+        self.terms, self._keep = make_terms(model)
+
+    def __del__(self):
+        try:
+            ref().ref_basis_free(self.h)
+        except Exception:
+            pass
+
+    def hv_rows(self, v, rows):
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        ref().ref_hv(self.h, C.addressof(self.terms), v.view(np.float64), w.view(np.float64), rows)
+        return w

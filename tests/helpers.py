@@ -1,0 +1,270 @@
+"""Shared test helpers: fixtures, brute-force mathematics, the CPU build of
+the kernel arithmetic (tests/hostcore)."""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import itertools
+import math
+import os
+import subprocess
+
+import numpy as np
+
+from ed_code_amd import lattices as L
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLDEN = os.path.join(HERE, "golden")
+
+
+# ------------------------------------------------------------ fixtures ----
+def golden_files():
+    return sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(path):
+    z = np.load(path, allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    m = L.Model(os.path.basename(path)[:-4], int(d["n_sites"]), d["T1"].astype(np.int32), d["T2"].astype(np.int32),
+                lam=float(d["lam"]))
+    m.bonds = [(int(x), int(y), float(J), complex(p[0], p[1]))
+               for (x, y), J, p in zip(d["bond_xy"], d["bond_J"], d["bond_phase"])]
+    m.Ch = float(d["Ch"])
+    m.chiral = [(int(i), int(j), int(k), complex(p[0], p[1])) for (i, j, k), p in zip(d["chiral_ijk"], d["chiral_phase"])]
+    m.ChCh = float(d["ChCh"])
+    m.bowties = [tuple(int(q) for q in b) for b in d["bowtie"]]
+    return m, d
+
+
+# ------------------------------------------------- brute-force mathematics ----
+def compose(a, b):
+    """Permutation a after b: site i -> a[b[i]]."""
+    return np.asarray(a)[np.asarray(b)]
+
+
+def group_elements(T1, T2):
+    """All distinct elements T1^x T2^y as site permutations."""
+    N = len(T1)
+    ident = np.arange(N)
+    seen = {}
+    g1 = ident.copy()
+    for _x in range(N + 1):
+        g = g1.copy()
+        for _y in range(N + 1):
+            seen.setdefault(tuple(g), g.copy())
+            g = compose(T2, g)
+        g1 = compose(T1, g1)
+    return list(seen.values())
+
+
+def apply_perm(s, perm):
+    t = 0
+    for i, p in enumerate(perm):
+        if (s >> i) & 1:
+            t |= 1 << int(p)
+    return t
+
+
+def cns_rank(s):
+    """Combinatorial-number-system rank: sum_t C(p_t, t)."""
+    r, t, p = 0, 0, 0
+    while s:
+        if s & 1:
+            t += 1
+            r += math.comb(p, t)
+        s >>= 1
+        p += 1
+    return r
+
+
+def states(N, n):
+    out = []
+    for c in itertools.combinations(range(N), n):
+        s = 0
+        for p in c:
+            s |= 1 << p
+        out.append(s)
+    return sorted(out)
+
+
+def burnside_dims(model, n):
+    """Sector dimension of every (kx, ky) by the character formula
+    D_k = (1/|G|) sum over the L1 x L2 grid of chi_k(g) * #{weight-n states fixed by g}
+    (the grid covers the group L1*L2/|G| times, which cancels)."""
+    L1, L2 = model.orders()
+    N = model.n_sites
+    T1, T2 = np.asarray(model.T1), np.asarray(model.T2)
+
+    def fixed_count(perm):
+        # cycles of perm; fixed states are constant on cycles
+        seen = [False] * N
+        lens = []
+        for i in range(N):
+            if not seen[i]:
+                l, j = 0, i
+                while not seen[j]:
+                    seen[j] = True
+                    j = int(perm[j])
+                    l += 1
+                lens.append(l)
+        # number of subsets of cycles with total length n
+        cnt = np.zeros(N + 1, dtype=object)
+        cnt[0] = 1
+        for l in lens:
+            new = cnt.copy()
+            for t in range(N - l, -1, -1):
+                new[t + l] += cnt[t]
+            cnt = new
+        return int(cnt[n])
+
+    fix = {}
+    g1 = np.arange(N)
+    for a in range(L1):
+        g = g1.copy()
+        for b in range(L2):
+            fix[(a, b)] = fixed_count(g)
+            g = compose(T2, g)
+        g1 = compose(T1, g1)
+    dims = {}
+    for kx in range(L1):
+        for ky in range(L2):
+            tot = 0j
+            for (a, b), f in fix.items():
+                tot += f * np.exp(-2j * np.pi * (kx * a / L1 + ky * b / L2))
+            dims[(kx, ky)] = int(round((tot / (L1 * L2)).real))
+    return dims
+
+
+def full_space_h(model, n):
+    """Dense H on all weight-n states, built directly from the term lists
+    with no symmetry machinery, in the gather convention of H_v.cpp (row s,
+    column t = s with two bits flipped).  Bowtie terms are not included.
+    Returns (H, state list)."""
+    N = model.n_sites
+    st = states(N, n)
+    idx = {s: i for i, s in enumerate(st)}
+    H = np.zeros((len(st), len(st)), complex)
+    for i, s in enumerate(st):
+        for (x, y, J, ph) in model.bonds:
+            bx, by = (s >> x) & 1, (s >> y) & 1
+            H[i, i] += model.lam * J * (0.25 if bx == by else -0.25)
+            if bx != by:
+                t = s ^ (1 << x) ^ (1 << y)
+                H[i, idx[t]] += 0.5 * J * (np.conj(ph) if bx else ph)
+        if abs(model.Ch) > 1e-16:
+            for (a, b, k, ep) in model.chiral:
+                ba, bb = (s >> a) & 1, (s >> b) & 1
+                if ba != bb:
+                    t = s ^ (1 << a) ^ (1 << b)
+                    sg = 1j * ep if bb else -1j * np.conj(ep)
+                    if not (s >> k) & 1:
+                        sg = -sg
+                    H[i, idx[t]] += model.Ch * 0.25 * sg
+    return H, st
+
+
+def momentum_projector(model, n, kx, ky, st):
+    """Orthonormal momentum states as columns over the full weight-n space:
+    |r(k)> ~ sum_g exp(-i k.g) g|r> over the translation grid."""
+    L1, L2 = model.orders()
+    px, py = model.momentum(kx, ky)
+    T1, T2 = np.asarray(model.T1), np.asarray(model.T2)
+    idx = {s: i for i, s in enumerate(st)}
+    grid = []
+    g1 = np.arange(model.n_sites)
+    for a in range(L1):
+        g = g1.copy()
+        for b in range(L2):
+            grid.append((a, b, g.copy()))
+            g = compose(T2, g)
+        g1 = compose(T1, g1)
+    cols, done = [], set()
+    for s in st:
+        orb = {apply_perm(s, g) for (_, _, g) in grid}
+        r = min(orb)
+        if r in done:
+            continue
+        done.add(r)
+        vec = np.zeros(len(st), complex)
+        for (a, b, g) in grid:
+            vec[idx[apply_perm(r, g)]] += np.exp(-1j * (px * a + py * b))
+        nrm = np.linalg.norm(vec)
+        if nrm > 1e-8:
+            cols.append(vec / nrm)
+    return np.array(cols).T if cols else np.zeros((len(st), 0), complex)
+
+
+# --------------------------------------------------- CPU build of ed_core ----
+_hc = None
+
+
+def hostcore():
+    """tests/hostcore/hostcore.cpp + ed_core.h compiled with g++ (CPU)."""
+    global _hc
+    if _hc is None:
+        out = os.path.join(HERE, "hostcore", "_build", "libhostcore.so")
+        src = os.path.join(HERE, "hostcore", "hostcore.cpp")
+        hdrs = glob.glob(os.path.join(ROOT, "ed_code_amd", "csrc", "*.h"))
+        if not os.path.exists(out) or any(os.path.getmtime(f) > os.path.getmtime(out) for f in hdrs + [src]):
+            os.makedirs(os.path.dirname(out), exist_ok=True)
+            subprocess.run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
+                            "-I", os.path.join(ROOT, "ed_code_amd", "csrc"), src, "-o", out], check=True)
+        H = C.CDLL(out)
+        vp = C.c_void_p
+        H.hc_build.restype = vp
+        H.hc_build.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, C.c_int]
+        H.hc_free.argtypes = [vp]
+        H.hc_dim.restype = C.c_int64
+        H.hc_dim.argtypes = [vp]
+        H.hc_export.argtypes = [vp] * 4
+        H.hc_full_tables.argtypes = [vp] * 3
+        H.hc_set_terms.argtypes = [vp, vp]
+        H.hc_hv.argtypes = [vp] * 3
+        H.hc_ranker_new.restype = vp
+        H.hc_ranker_new.argtypes = [C.c_int, C.c_int]
+        H.hc_ranker_free.argtypes = [vp]
+        H.hc_gosper_next.restype = C.c_uint64
+        H.hc_gosper_next.argtypes = [C.c_uint64]
+        H.hc_rank.restype = C.c_uint64
+        H.hc_rank.argtypes = [vp, C.c_uint64]
+        H.hc_unrank.restype = C.c_uint64
+        H.hc_unrank.argtypes = [vp, C.c_uint64]
+        H.hc_tridiag.restype = C.c_int
+        H.hc_tridiag.argtypes = [C.c_int, vp, vp, vp]
+        _hc = H
+    return _hc
+
+
+class HostCore:
+    def __init__(self, model, n, kx=0, ky=0):
+        H = hostcore()
+        self.T1 = np.ascontiguousarray(model.T1, np.int32)
+        self.T2 = np.ascontiguousarray(model.T2, np.int32)
+        self.h = H.hc_build(model.n_sites, n, self.T1.ctypes.data, self.T2.ctypes.data, kx, ky)
+        self.D = H.hc_dim(self.h)
+        import pyoracle
+        self.terms, self._keep = pyoracle.make_terms(model)
+        H.hc_set_terms(self.h, C.addressof(self.terms))
+
+    def __del__(self):
+        try:
+            hostcore().hc_free(self.h)
+        except Exception:
+            pass
+
+    def export(self):
+        reps = np.zeros(self.D, np.int64); nm = np.zeros(self.D); ni = np.zeros(self.D)
+        hostcore().hc_export(self.h, reps.ctypes.data, nm.ctypes.data, ni.ctypes.data)
+        return reps, nm, ni
+
+    def full_tables(self, C_):
+        rl = np.zeros(C_, np.int64); sp = np.zeros(2 * C_)
+        hostcore().hc_full_tables(self.h, rl.ctypes.data, sp.ctypes.data)
+        return rl, sp.view(np.complex128)
+
+    def hv(self, v):
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        hostcore().hc_hv(self.h, v.ctypes.data, w.ctypes.data)
+        return w

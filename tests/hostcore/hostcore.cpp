@@ -1,0 +1,230 @@
+// hostcore.cpp -- CPU build of the kernel arithmetic in ed_code_amd/csrc/ed_core.h,
+// for unit tests only (tests/conftest.py compiles it with g++).  It walks the
+// same pipeline the device code runs (classify -> directory -> compaction ->
+// visit_row) so that the kernels' logic is checked against the oracle without
+// a GPU.  It is never loaded by the product.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "ed_core.h"
+#include "ed_tables.h"
+#include "ed_tridiag.h"
+
+using namespace edc;
+
+namespace {
+struct HC {
+    int N, n, kx, ky, L1, L2;
+    double px, py;
+    uint64_t C;
+    Geo G;
+    RankInfo R;
+    std::vector<uint64_t> rt, bn, rows;
+    std::vector<DirEntry> dir;
+    std::vector<cd2> ph;
+    std::vector<double> nt, nit;
+    std::vector<Bond> b;
+    std::vector<Chiral> c;
+    std::vector<Bowtie> w;
+    double lam = 1, Ch = 0, ChCh = 0;
+};
+
+struct Terms {
+    double lambda;
+    int32_t nbonds;
+    const int32_t* bond_xy;
+    const double* bond_J;
+    const double* bond_phase;
+    double Ch;
+    int32_t nchiral;
+    const int32_t* chiral_ijk;
+    const double* chiral_phase;
+    double ChCh;
+    int32_t nbowtie;
+    const int32_t* bowtie;
+};
+
+template <typename S>
+void build(HC* h) {
+    uint64_t ng = (h->C + kDirSpan - 1) / kDirSpan;
+    std::vector<uint32_t> words(ng * 3, 0);
+    for (uint64_t w = 0; w < ng * 3; w++) {
+        uint64_t r0 = w * 32;
+        if (r0 >= h->C) break;
+        uint64_t s = unrank(r0, h->N, h->n, h->bn.data());
+        uint32_t bits = 0;
+        for (int b = 0; b < 32 && r0 + b < h->C; b++) {
+            if (classify<S>((S)s, h->G)) bits |= 1u << b;
+            s = gosper_next(s);
+        }
+        words[w] = bits;
+    }
+    h->dir.resize(ng);
+    uint64_t base = 0;
+    for (uint64_t g = 0; g < ng; g++) {
+        h->dir[g] = DirEntry{(uint32_t)base, words[3 * g], words[3 * g + 1], words[3 * g + 2]};
+        base += popc32(words[3 * g]) + popc32(words[3 * g + 1]) + popc32(words[3 * g + 2]);
+    }
+    h->rows.resize(base);
+    for (uint64_t g = 0; g < ng; g++) {
+        DirEntry e = h->dir[g];
+        if ((e.b0 | e.b1 | e.b2) == 0) continue;
+        uint64_t r0 = g * kDirSpan;
+        uint64_t s = unrank(r0, h->N, h->n, h->bn.data());
+        uint64_t j = e.base;
+        for (int b = 0; b < kDirSpan && r0 + b < h->C; b++) {
+            uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
+            if ((wd >> (b & 31)) & 1u) h->rows[j++] = s | ((uint64_t)classify<S>((S)s, h->G) << kLidShift);
+            s = gosper_next(s);
+        }
+    }
+}
+
+template <typename S, bool TRIV>
+void hv(HC* h, const cd2* v, cd2* w) {
+    TablesView tv{h->rt.data(), h->dir.data(), h->ph.data()};
+    TermsView T{h->lam, h->Ch, h->ChCh, (int)h->b.size(), (int)h->c.size(), (int)h->w.size(), h->b.data(), h->c.data(),
+                h->w.data()};
+    std::vector<cd2> u(h->rows.size());
+    for (size_t i = 0; i < u.size(); i++) {
+        double f = h->nit[h->rows[i] >> kLidShift];
+        u[i] = cd2{v[i].re * f, v[i].im * f};
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
+        S s = (S)(h->rows[i] & kStateMask);
+        double nrm = h->nt[h->rows[i] >> kLidShift];
+        double ar = 0, ai = 0;
+        double ez = visit_row<S, TRIV>(s, h->G, h->R, tv, T, [&](int64_t j, double cr, double ci) {
+            cd2 x = u[j];
+            ar += cr * x.re - ci * x.im;
+            ai += cr * x.im + ci * x.re;
+        });
+        w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void* hc_build(int N, int n, const int32_t* T1, const int32_t* T2, int kx, int ky) {
+    HC* h = new HC();
+    h->N = N; h->n = n; h->kx = kx; h->ky = ky;
+    std::vector<int> a(T1, T1 + N), b(T2, T2 + N);
+    h->L1 = cycle_len0(a);
+    h->L2 = cycle_len0(b);
+    h->px = 2 * kx * M_PI / h->L1;
+    h->py = 2 * ky * M_PI / h->L2;
+    h->C = binom_u64(N, n);
+    std::memset(&h->G, 0, sizeof(h->G));
+    h->G.N = N; h->G.L1 = h->L1; h->G.L2 = h->L2; h->G.kx = kx; h->G.ky = ky;
+    make_prog(a, h->G.T1);
+    make_prog(b, h->G.T2);
+    h->rt = rank_tables(N, n, h->R);
+    h->bn = binom_table();
+    h->ph = phase_table(h->L1, h->L2, h->px, h->py);
+    norm_tables(h->L1, h->L2, h->nt, h->nit);
+    if (N <= 32) build<uint32_t>(h); else build<uint64_t>(h);
+    return h;
+}
+
+void hc_free(void* p) { delete (HC*)p; }
+
+int64_t hc_dim(void* p) { return (int64_t)((HC*)p)->rows.size(); }
+
+void hc_export(void* p, int64_t* reps, double* norm, double* norm_inv) {
+    HC* h = (HC*)p;
+    for (size_t i = 0; i < h->rows.size(); i++) {
+        int lid = (int)(h->rows[i] >> kLidShift);
+        reps[i] = (int64_t)(h->rows[i] & kStateMask);
+        norm[i] = h->nt[lid];
+        norm_inv[i] = h->nit[lid];
+    }
+}
+
+// rep_loc / state_ph of every weight-n state in ascending order.
+void hc_full_tables(void* p, int64_t* rep_loc, double* state_ph) {
+    HC* h = (HC*)p;
+    TablesView tv{h->rt.data(), h->dir.data(), h->ph.data()};
+    uint64_t s = unrank(0, h->N, h->n, h->bn.data());
+    for (uint64_t r = 0; r < h->C; r++) {
+        int code;
+        uint64_t rep = h->N <= 32 ? (uint64_t)find_rep<uint32_t, true>((uint32_t)s, h->G, code)
+                                  : find_rep<uint64_t, true>(s, h->G, code);
+        rep_loc[r] = dir_lookup(rank_of<uint64_t>(rep, h->R, tv.rtab), tv.dir);
+        state_ph[2 * r] = h->ph[code].re;
+        state_ph[2 * r + 1] = h->ph[code].im;
+        s = gosper_next(s);
+    }
+}
+
+// Rank tables alone (no basis enumeration), for closed-form checks at any N.
+struct Ranker {
+    int N, n;
+    RankInfo R;
+    std::vector<uint64_t> rt, bn;
+};
+
+void* hc_ranker_new(int N, int n) {
+    Ranker* r = new Ranker();
+    r->N = N; r->n = n;
+    r->rt = rank_tables(N, n, r->R);
+    r->bn = binom_table();
+    return r;
+}
+
+void hc_ranker_free(void* p) { delete (Ranker*)p; }
+
+uint64_t hc_rank(void* p, uint64_t s) {
+    Ranker* r = (Ranker*)p;
+    return rank_of<uint64_t>(s, r->R, r->rt.data());
+}
+
+uint64_t hc_unrank(void* p, uint64_t k) {
+    Ranker* r = (Ranker*)p;
+    return unrank(k, r->N, r->n, r->bn.data());
+}
+
+uint64_t hc_gosper_next(uint64_t s) { return gosper_next(s); }
+
+void hc_set_terms(void* p, const Terms* t) {
+    HC* h = (HC*)p;
+    h->b.clear(); h->c.clear(); h->w.clear();
+    h->lam = t->lambda; h->Ch = t->Ch; h->ChCh = t->ChCh;
+    for (int k = 0; k < t->nbonds; k++)
+        if (std::fabs(t->bond_J[k]) >= 1e-16)
+            h->b.push_back(Bond{t->bond_xy[2 * k], t->bond_xy[2 * k + 1], t->bond_J[k], t->bond_phase[2 * k],
+                                t->bond_phase[2 * k + 1]});
+    if (std::fabs(t->Ch) > 1e-16)
+        for (int k = 0; k < t->nchiral; k++)
+            h->c.push_back(Chiral{t->chiral_ijk[3 * k], t->chiral_ijk[3 * k + 1], t->chiral_ijk[3 * k + 2], 0,
+                                  t->chiral_phase[2 * k], t->chiral_phase[2 * k + 1]});
+    if (std::fabs(t->ChCh) > 1e-16)
+        for (int k = 0; k < t->nbowtie; k++) {
+            const int32_t* q = t->bowtie + 6 * k;
+            h->w.push_back(Bowtie{q[0], q[1], q[2], q[3], q[4], q[5]});
+        }
+}
+
+void hc_hv(void* p, const double* v, double* w) {
+    HC* h = (HC*)p;
+    bool triv = h->kx == 0 && h->ky == 0;
+    if (h->N <= 32) {
+        if (triv) hv<uint32_t, true>(h, (const cd2*)v, (cd2*)w); else hv<uint32_t, false>(h, (const cd2*)v, (cd2*)w);
+    } else {
+        if (triv) hv<uint64_t, true>(h, (const cd2*)v, (cd2*)w); else hv<uint64_t, false>(h, (const cd2*)v, (cd2*)w);
+    }
+}
+
+// Eigenpairs of the symmetric tridiagonal (d[m], e[m-1]) with the library's QL
+// solver: d <- ascending eigenvalues, z <- column-major eigenvectors.
+int hc_tridiag(int m, double* d, const double* e, double* z) {
+    std::vector<double> dd(d, d + m), ee(e, e + (m > 0 ? m - 1 : 0)), zz;
+    bool ok = tridiag_ql(dd, ee, z ? &zz : nullptr);
+    std::memcpy(d, dd.data(), m * sizeof(double));
+    if (z) std::memcpy(z, zz.data(), (size_t)m * m * sizeof(double));
+    return ok ? 0 : 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+"""GPU Lanczos against dense diagonalisation, closed forms and the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import helpers as Hp
+import pyoracle as P
+from ed_code_amd import lattices as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _ed():
+    import ed_code_amd as ed
+    ed.init(0)
+    return ed
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("mode", ["reference", "full"])
+def test_lanczos_vs_dense(gpu_ready, prec, mode):
+    ed = _ed()
+    print(f"[case] lanczos {prec} {mode}", flush=True)
+    model = L.square(4, 4, J2=0.5)
+    o = P.OracleBasis(model, 8, 0, 0)
+    ev = np.linalg.eigvalsh(o.make_h())
+    basis = ed.Basis(ed.Lattice.of(model), 8, 0, 0)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    md = ed.ED_LZ_REFERENCE if mode == "reference" else ed.ED_LZ_FULL
+    lz, st = op.lanczos(nev=3, iterations=80, ncycles=4, tol=1e-12, seed=7, precision=pr, mode=md)
+    tol = 1e-10
+    assert abs(lz[0] - ev[0]) < tol
+    assert abs(lz[1] - ev[1]) < 1e-8 and abs(lz[2] - ev[2]) < 1e-8
+    assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-3)
+
+
+def test_lanczos_chain24_ground_state(gpu_ready):
+    """chain24 Sz=0 k=0, 4 lowest deflated eigenvalues: c128 reference schedule
+    against the oracle's Lanczos."""
+    ed = _ed()
+    model = L.chain(24)
+    basis = ed.Basis(ed.Lattice.of(model), 12, 0, 0)
+    op = ed.Operator(basis, model)
+    lz, st = op.lanczos(nev=2, iterations=100, ncycles=4, tol=1e-12, seed=3)
+    o = P.OracleBasis(model, 12, 0, 0)
+    lo = o.lanczos(nev=2, iterations=100, ncycles=4, tol=1e-12, seed=3)
+    assert np.abs(lz - lo).max() < 1e-10
+    # Bethe-ansatz value for the 24-site Heisenberg ring: E0/N = -0.4438..., sanity band
+    assert -0.45 * 24 < lz[0] < -0.44 * 24
+
+
+def test_sector_lanczos_dense_and_lanczos_paths(gpu_ready):
+    ed = _ed()
+    # D <= 10000: dense path returns every eigenvalue
+    m = L.chain(12)
+    e, D = ed.sector_lanczos(m, 0, 0, 0.0)
+    o = P.OracleBasis(m, 6, 0, 0)
+    assert D == o.D and np.abs(e - np.linalg.eigvalsh(o.make_h())).max() < 1e-10
+    # four-site ring E0 = -2
+    e4, _ = ed.sector_lanczos(L.chain(4), 0, 0, 0.0)
+    assert abs(e4[0] + 2.0) < 1e-12
+    # odd N, Sz = 0 means n_ones = N//2
+    assert ed.n_ones_for_sz(27, 0.0) == 13 and ed.n_ones_for_sz(27, 1.0) == 14

@@ -1,0 +1,210 @@
+"""Pins the CPU oracle (oracle/ed_oracle.cpp) against the reference's golden
+vectors, the compiled reference slice (when built), and closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+import helpers as Hp
+import pyoracle as P
+from ed_code_amd import lattices as L
+
+GOLDEN = Hp.golden_files()
+
+
+def test_golden_present():
+    assert len(GOLDEN) >= 15
+
+
+def test_n_choose_k_closed_form():
+    for n in range(0, 41):
+        for k in range(0, n + 1):
+            assert P.lib().or_n_choose_k(n, k) == math.comb(n, k)
+
+
+@pytest.mark.parametrize("N,n", [(6, 3), (12, 6), (15, 7), (16, 0), (16, 16), (20, 9)])
+def test_gosper_enumeration(N, n):
+    c = math.comb(N, n)
+    out = np.zeros(c, np.int64)
+    P.lib().or_gosper(N, n, out)
+    assert np.all(np.diff(out) > 0)
+    assert all(bin(int(s)).count("1") == n for s in out[:2000])
+    assert out.tolist()[:5000] == Hp.states(N, n)[:5000] if c < 200000 else True
+    # the ascending position is the combinatorial-number-system rank
+    for r in range(0, c, max(1, c // 300)):
+        assert Hp.cns_rank(int(out[r])) == r
+
+
+@pytest.mark.parametrize("half", [4, 7, 8, 11])
+def test_lin_tables_closed_form(half):
+    Ia = np.zeros(1 << half, np.int64)
+    Ib = np.zeros(1 << half, np.int64)
+    P.lib().or_lin_tables(half, Ia, Ib)
+    for r in range(1 << half):
+        assert Ia[r] == Hp.cns_rank(r) + 1
+    assert np.all(Ib == -1)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
+def test_oracle_matches_golden(path):
+    m, d = Hp.load_golden(path)
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]))
+    t = o.tables()
+    # basis tables: bit-exact
+    for k in ("reps", "norm", "norm_inv", "rep_loc", "Ia", "Ib"):
+        assert np.array_equal(t[k], d[k]), k
+    assert np.array_equal(t["state_ph"].view(np.float64), d["state_ph"].view(np.float64))
+    assert o.bits_right == int(d["bits_right"])
+    assert P.lib().or_basis_lin_error(o.h) == 0
+    # H*v: FP64, within 1e-14 relative of the reference's output
+    w = o.hv(d["v"])
+    assert np.linalg.norm(w - d["w"]) <= 1e-14 * np.linalg.norm(d["w"])
+    # makeH: same spectrum as the reference's dense H
+    ev = np.linalg.eigvalsh(o.make_h())
+    assert np.abs(ev - d["eigvals"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("model,n", [(L.chain(12), 6), (L.square(4, 3, J2=0.4), 5), (L.triangular(4, 3), 6),
+                                     (L.tilted_square((3, 1), (-1, 3)), 5), (L.chain(9), 4)])
+def test_representatives_brute_force(model, n):
+    """Orbit minimum over the full group, and orbit size = L1*L2/lid."""
+    G = Hp.group_elements(model.T1, model.T2)
+    L1, L2 = model.orders()
+    T1 = np.ascontiguousarray(model.T1, np.int32)
+    T2 = np.ascontiguousarray(model.T2, np.int32)
+    for s in Hp.states(model.n_sites, n)[:400]:
+        orb = {Hp.apply_perm(s, g) for g in G}
+        lx, ly, pid, lid = [P.C.c_int32() for _ in range(4)]
+        r = P.lib().or_representative(s, T1, T2, model.n_sites, L1, L2, 0.0, 0.0, P.C.byref(lx), P.C.byref(ly),
+                                      P.C.byref(pid), P.C.byref(lid))
+        assert r == min(orb)
+        assert len(orb) == L1 * L2 // lid.value
+        # s = T1^lx T2^ly rep
+        g = np.arange(model.n_sites)
+        for _ in range(lx.value % L1):
+            g = Hp.compose(model.T1, g)
+        for _ in range(ly.value % L2):
+            g = Hp.compose(model.T2, g)
+        assert Hp.apply_perm(r, g) == s
+
+
+@pytest.mark.parametrize("model,n", [(L.chain(12), 6), (L.chain(13), 6), (L.square(4, 4), 8), (L.triangular(3, 4), 5),
+                                     (L.tilted_square((3, 1), (-1, 3)), 5), (L.tilted_square((2, 2), (-2, 2)), 4)])
+def test_sector_dimensions_burnside(model, n):
+    dims = Hp.burnside_dims(model, n)
+    tot = 0
+    for (kx, ky), dk in dims.items():
+        o = P.OracleBasis(model, n, kx, ky)
+        assert o.D == dk, (kx, ky)
+        tot += o.D
+    assert tot == math.comb(model.n_sites, n)
+
+
+@pytest.mark.parametrize("model,n,k", [
+    (L.chain(10, phases=[0.3] * 10), 5, (3, 0)),
+    (L.triangular(4, 3, Ch=0.4), 6, (1, 1)),
+    (L.triangular(3, 3, Ch=0.3), 4, (1, 2)),
+    (L.square(4, 3, J2=0.4, lam=0.6), 6, (1, 2)),
+    (L.tilted_square((3, 1), (-1, 3), J2=0.5), 5, (1, 7)),
+])
+def test_momentum_hamiltonian_vs_full_space_projection(model, n, k):
+    """Independent construction: full-space H projected on momentum states."""
+    H, st = Hp.full_space_h(model, n)
+    Pm = Hp.momentum_projector(model, n, k[0], k[1], st)
+    o = P.OracleBasis(model, n, *k)
+    Hm = o.make_h()
+    assert Pm.shape[1] == o.D
+    assert np.abs(Hm - Hm.conj().T).max() < 1e-13
+    ev = np.linalg.eigvalsh(Pm.conj().T @ H @ Pm)
+    assert np.abs(ev - np.linalg.eigvalsh(Hm)).max() < 1e-12
+
+
+@pytest.mark.parametrize("path", GOLDEN[::3], ids=[p.split("/")[-1][:-4] for p in GOLDEN[::3]])
+def test_oracle_hv_hermitian(path):
+    m, d = Hp.load_golden(path)
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]))
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    y = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    assert abs(np.vdot(x, o.hv(y)) - np.vdot(o.hv(x), y)) < 1e-12 * np.linalg.norm(x) * np.linalg.norm(y) * 10
+
+
+def test_four_site_ring_ground_state():
+    o = P.OracleBasis(L.chain(4), 2, 0, 0)
+    ev = np.linalg.eigvalsh(o.make_h())
+    assert abs(ev[0] - (-2.0)) < 1e-13
+
+
+def _xy_chain_e0(Lc, n):
+    """Delta=0 chain: free fermions, boundary condition by fermion parity."""
+    # H = 1/2 sum (S+S- + h.c.) -> hopping 1/2; periodic if n odd, antiperiodic if n even
+    shift = 0.0 if n % 2 == 1 else 0.5
+    ks = [2 * math.pi * (j + shift) / Lc for j in range(Lc)]
+    e = sorted(math.cos(k) for k in ks)
+    return sum(e[:n])
+
+
+@pytest.mark.parametrize("Lc,n", [(8, 4), (10, 5), (12, 6), (12, 5)])
+def test_xy_chain_free_fermions(Lc, n):
+    m = L.chain(Lc, lam=0.0)
+    ens = []
+    for kx in range(Lc):
+        o = P.OracleBasis(m, n, kx, 0)
+        if o.D:
+            ens.append(np.linalg.eigvalsh(o.make_h())[0])
+    assert abs(min(ens) - _xy_chain_e0(Lc, n)) < 1e-12
+
+
+def test_ising_limit_diagonal():
+    m = L.square(4, 3, J1=1.0)
+    m.bonds = [(x, y, J, ph) for (x, y, J, ph) in m.bonds]
+    o = P.OracleBasis(m.with_couplings(lam=1.0), 6, 0, 0)
+    # J_xy = 0 is lam -> infinity; use lam scaling: H(lam) - lam*Hzz has no lam dependence
+    H1 = o.make_h()
+    o.set_model(m.with_couplings(lam=3.0))
+    H3 = o.make_h()
+    diff = (H3 - H1) / 2.0
+    assert np.abs(diff - np.diag(np.diag(diff))).max() < 1e-14  # only the zz part scales
+
+
+def test_su2_multiplets_across_sz():
+    """At Delta=1 every level with S >= 1 of the Sz=0 sector appears at Sz=1."""
+    m = L.chain(10)
+    e0 = np.linalg.eigvalsh(P.OracleBasis(m, 5, 0, 0).make_h())
+    e1 = np.linalg.eigvalsh(P.OracleBasis(m, 6, 0, 0).make_h())
+    for e in e1:
+        assert np.min(np.abs(e0 - e)) < 1e-10
+
+
+def test_oracle_lanczos_matches_dense():
+    m = L.square(4, 4, J2=0.5)
+    o = P.OracleBasis(m, 8, 0, 0)
+    ev = np.linalg.eigvalsh(o.make_h())
+    lz = o.lanczos(nev=3, iterations=80, ncycles=4, tol=1e-12, seed=7)
+    # deflated runs give the lowest levels with multiplicity (SU(2) multiplets allowed)
+    assert abs(lz[0] - ev[0]) < 1e-10
+    assert abs(lz[1] - ev[1]) < 1e-9
+    assert abs(lz[2] - ev[2]) < 1e-9
+
+
+def test_start_vector_counter_based():
+    a = P.start_vector(5, 1000)
+    b = P.start_vector(5, 1000)
+    c = P.start_vector(6, 1000)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert np.all(np.abs(a.real) <= 1) and np.all(np.abs(a.imag) <= 1)
+    assert abs(a.real.mean()) < 0.1
+
+
+@pytest.mark.skipif(not P.ref_available(), reason="reference slice not built (needs /root/reference)")
+@pytest.mark.parametrize("model,n,k", [(L.chain(16, phases=[0.2] * 16), 8, (5, 0)), (L.triangular(4, 4, Ch=0.2), 8, (1, 3)),
+                                       (L.chain(17), 8, (3, 0)), (L.tilted_square((3, 1), (-1, 3), J2=0.5), 5, (3, 1))])
+def test_oracle_vs_reference_slice(model, n, k):
+    o = P.OracleBasis(model, n, *k)
+    r = P.RefBasis(model, n, *k)
+    to, tr = o.tables(), r.tables()
+    for key in ("reps", "norm", "norm_inv", "rep_loc", "Ia", "Ib"):
+        assert np.array_equal(to[key], tr[key]), key
+    assert np.array_equal(to["state_ph"].view(np.float64), tr["state_ph"].view(np.float64))
+    v = np.random.default_rng(1).normal(size=o.D) + 0j
+    assert np.linalg.norm(o.hv(v) - r.hv(v)) <= 1e-14 * np.linalg.norm(r.hv(v))
