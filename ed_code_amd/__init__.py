@@ -151,6 +151,10 @@ class Operator:
         check(lib().ed_hv(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st)))
         return w
 
+    def tune(self, path: int = 0, block: int = 0):
+        """Kernel choice: path 0 auto, 1 generic walk, 2 rotation fast path."""
+        check(lib().ed_op_tune(self.h, path, block))
+
     def hv_profile(self, v, w, stream=None) -> float:
         """ed_hv with HIP events around the row kernel; returns its ms."""
         import torch

@@ -27,6 +27,7 @@
 #include "ed_core.h"
 #include "ed_internal.h"
 #include "ed_tables.h"
+#include "ed_torus_host.h"
 
 using namespace edc;
 
@@ -75,6 +76,7 @@ extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32
     L->T2 = b;
     L->L1 = cycle_len0(a);
     L->L2 = cycle_len0(b);
+    L->has_torus = torus_detect(a, b, L->torus);
     if (!make_prog(a, L->P1) || !make_prog(b, L->P2)) {
         delete L;
         return ed_fail(ED_EINVAL, "translation has more than %d displacement groups", kMaxGroups);
@@ -275,6 +277,8 @@ extern "C" int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_
     B->L1 = lat->L1; B->L2 = lat->L2;
     B->T1 = lat->T1; B->T2 = lat->T2;
     B->lat_P1 = lat->P1; B->lat_P2 = lat->P2;
+    B->has_torus = lat->has_torus;
+    B->torus = lat->torus;
     // Lanczos_sector.cpp:57
     B->px = 2 * kx * M_PI / B->L1;
     B->py = 2 * ky * M_PI / B->L2;
@@ -304,6 +308,13 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipMemcpy(B->d_nitab, nit.data(), nit.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(B->d_binom, bn.data(), bn.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(B->d_rtab, rt.data(), rt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (N <= 32) {
+        std::vector<uint32_t> rt32(rt.size());
+        for (size_t q = 0; q < rt.size(); q++) rt32[q] = (uint32_t)rt[q];   // C(N,n) < 2^32
+        B->rtab32_n = (int)rt32.size();
+        HIPCHK(hipMalloc(&B->d_rtab32, rt32.size() * sizeof(uint32_t)));
+        HIPCHK(hipMemcpy(B->d_rtab32, rt32.data(), rt32.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
 
     Geo G;
     basis_geo(B, G);
@@ -353,7 +364,7 @@ int basis_build_device(ed_basis_s* B) {
 void delete_basis(ed_basis_s* B) {
     if (!B) return;
     for (void* p : {(void*)B->d_rows, (void*)B->d_dir, (void*)B->d_rtab, (void*)B->d_binom, (void*)B->d_phase,
-                    (void*)B->d_ntab, (void*)B->d_nitab})
+                    (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32})
         if (p) dfree(p);
     delete B;
 }
@@ -497,13 +508,41 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
         HIPCHK(hipMemcpy(op->d_w, bw.data(), bw.size() * sizeof(Bowtie), hipMemcpyHostToDevice));
     }
     op->bonds = bonds;
+    // rotation fast path: rotational lattice, no bowtie terms, families fit
+    if (b->has_torus && bw.empty() && b->N <= 32 && b->d_rtab32) {
+        std::vector<TFam> fam;
+        std::vector<TEntry> ent;
+        double ezz0 = 0;
+        if (torus_families(op->lam, op->Ch, b->torus, bonds, ch, fam, ent, ezz0) && !fam.empty()) {
+            op->use_torus = true;
+            op->torus = b->torus;
+            op->nfam = (int)fam.size();
+            op->ezz0 = ezz0;
+            op->tune_block = 512;
+            HIPCHK(hipMalloc(&op->d_tfam, fam.size() * sizeof(TFam)));
+            HIPCHK(hipMemcpy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&op->d_tent, ent.size() * sizeof(TEntry)));
+            HIPCHK(hipMemcpy(op->d_tent, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+        }
+    }
     *out = op;
+    return ED_OK;
+}
+
+extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
+    if (!op) return ed_fail(ED_EINVAL, "null op");
+    if (path < 0 || path > 2) return ed_fail(ED_EINVAL, "path %d not in {0,1,2}", path);
+    if (path == 2 && !op->use_torus) return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
+    if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
+    op->tune_path = path;
+    if (block) op->tune_block = block;
     return ED_OK;
 }
 
 extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
-    for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128})
+    for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
+                    (void*)op->d_tent})
         if (p) dfree(p);
     delete op;
 }
@@ -528,7 +567,10 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 256)), blk(256);
     if (ev0) HIPCHK(hipEventRecord(ev0, st));
-    if (B->N <= 32) {
+    if (op->use_torus && op->tune_path != 1) {
+        int rc = launch_hv_torus<VT>(op, u, v, w, st);
+        if (rc != ED_OK) return rc;
+    } else if (B->N <= 32) {
         if (B->trivial_k)
             hipLaunchKernelGGL((k_hv_rows<uint32_t, true, VT>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, u, v, w);
         else

@@ -7,6 +7,7 @@
 
 #include "ed_amd.h"
 #include "ed_core.h"
+#include "ed_torus.h"
 
 int ed_fail(int code, const char* fmt, ...);
 
@@ -26,6 +27,8 @@ struct ed_lattice_s {
     int N = 0, L1 = 0, L2 = 0;
     std::vector<int> T1, T2;
     edc::TProg P1, P2;
+    bool has_torus = false;   // translations are bit rotations (rotation fast path)
+    edc::Torus torus;
 };
 
 struct ed_basis_s {
@@ -41,6 +44,10 @@ struct ed_basis_s {
     uint64_t* d_rows = nullptr;      // [D] representative | lid << 52
     edc::DirEntry* d_dir = nullptr;  // [ceil(C/96)]
     uint64_t* d_rtab = nullptr;      // rank tables
+    uint32_t* d_rtab32 = nullptr;    // same, 32-bit (N <= 32)
+    int rtab32_n = 0;
+    bool has_torus = false;
+    edc::Torus torus;
     uint64_t* d_binom = nullptr;     // C(p,t), 65 x 65
     edc::cd2* d_phase = nullptr;     // [L1*L2]
     double* d_ntab = nullptr;        // norm by lid
@@ -54,6 +61,15 @@ struct ed_op_s {
     edc::Bond* d_b = nullptr;
     edc::Chiral* d_c = nullptr;
     edc::Bowtie* d_w = nullptr;
+    // rotation fast path
+    bool use_torus = false;
+    edc::Torus torus;
+    int nfam = 0;
+    edc::TFam* d_tfam = nullptr;
+    edc::TEntry* d_tent = nullptr;
+    double ezz0 = 0;
+    int tune_path = 0;        // 0 auto, 1 generic walk, 2 rotation fast path
+    int tune_block = 256;
     void* d_u64 = nullptr;    // prescaled gather stream, complex64 [D]
     void* d_u128 = nullptr;   // prescaled gather stream, complex128 [D]
     std::vector<edc::Bond> bonds;
@@ -64,3 +80,6 @@ int basis_build_device(ed_basis_s* B);
 void delete_basis(ed_basis_s* B);
 edc::TermsView op_terms(const ed_op_s* op);
 int op_hv(ed_op_s* op, int dtype, const void* v, void* w, hipStream_t st);
+size_t torus_smem_bytes(const ed_op_s* op);
+template <typename VT>
+int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);

@@ -1,0 +1,99 @@
+// ed_torus_host.h -- host-side detection of rotational lattices and grouping of
+// the Hamiltonian terms into families for the rotation fast path (ed_torus.h).
+// Plain C++: used by the HIP library and by the CPU unit-test build.
+#pragma once
+#include <cstring>
+#include <vector>
+
+#include "ed_core.h"
+#include "ed_torus.h"
+
+namespace edc {
+
+inline bool torus_detect(const std::vector<int>& T1, const std::vector<int>& T2, Torus& T) {
+    std::memset(&T, 0, sizeof(T));
+    const int N = (int)T1.size();
+    if (N > 32 || N < 2) return false;
+    T.N = N;
+    T.nmask = N == 32 ? 0xffffffffu : ((1u << N) - 1u);
+    bool t2id = true, t1ring = true;
+    for (int i = 0; i < N; i++) {
+        if (T2[i] != i) t2id = false;
+        if (T1[i] != (i + 1) % N) t1ring = false;
+    }
+    if (t2id && t1ring) {
+        T.kind = 2; T.w = N; T.L1 = N; T.L2 = 1;
+        return true;
+    }
+    for (int w = 2; w <= N; w++) {
+        if (N % w) continue;
+        bool ok = true;
+        for (int i = 0; i < N && ok; i++) {
+            if (T1[i] != (i / w) * w + (i % w + 1) % w) ok = false;
+            if (T2[i] != (i + w) % N) ok = false;
+        }
+        if (!ok) continue;
+        T.kind = 1; T.w = w; T.L1 = w; T.L2 = N / w;
+        for (int r = 0; r < w; r++) {
+            uint32_t h = 0;
+            for (int i = 0; i < N; i++)
+                if (i % w >= r) h |= 1u << i;
+            T.hi[r] = h;
+        }
+        return T.L2 >= 1;
+    }
+    return false;
+}
+
+// group element (a, b) = T1^a T2^b with h(x) = y, as (field, word) rotations of h^-1
+inline void rel_element(const Torus& T, int x, int y, int& a_inv, int& q_inv) {
+    if (T.kind == 2) {
+        int a = ((y - x) % T.N + T.N) % T.N;
+        a_inv = 0;
+        q_inv = (T.N - a) % T.N;
+        return;
+    }
+    int w = T.w, L2 = T.L2;
+    int a = ((y % w - x % w) % w + w) % w;
+    int b = ((y / w - x / w) % L2 + L2) % L2;
+    a_inv = (w - a) % w;
+    q_inv = ((L2 - b) % L2) * w;
+}
+
+// Groups the terms into families; false if they do not fit the fast path.
+inline bool torus_families(double lam, double Ch, const Torus& T, const std::vector<Bond>& bonds, const std::vector<Chiral>& ch,
+                    std::vector<TFam>& fam, std::vector<TEntry>& ent, double& ezz0) {
+    fam.clear();
+    ent.assign(kMaxFam * 32, TEntry{0, 0, 0, 0, 0});
+    ezz0 = 0;
+    std::vector<int> keyk;   // chiral: relation of k to x (encoded), -1 for bonds
+    auto place = [&](int type, int x, int y, int krel, const TEntry& e) -> bool {
+        int a, q;
+        rel_element(T, x, y, a, q);
+        for (size_t f = 0; f < fam.size(); f++)
+            if (fam[f].type == type && fam[f].a == a && fam[f].q == q && keyk[f] == krel && !((fam[f].S >> x) & 1u)) {
+                fam[f].S |= 1u << x;
+                ent[f * 32 + x] = e;
+                return true;
+            }
+        if ((int)fam.size() >= kMaxFam) return false;
+        fam.push_back(TFam{type, a, q, 1u << x});
+        keyk.push_back(krel);
+        ent[(fam.size() - 1) * 32 + x] = e;
+        return true;
+    };
+    for (const Bond& b : bonds) {
+        TEntry e{0.5 * b.J * b.re, 0.5 * b.J * b.im, 0.5 * lam * b.J, b.y, 0};
+        ezz0 += 0.25 * lam * b.J;
+        if (!place(0, b.x, b.y, -1, e)) return false;
+    }
+    for (const Chiral& c : ch) {
+        int a2, q2;
+        rel_element(T, c.i, c.k, a2, q2);
+        TEntry e{0.25 * Ch * c.re, 0.25 * Ch * c.im, 0.0, c.j, c.k};
+        if (!place(1, c.i, c.j, a2 * 64 + q2, e)) return false;
+    }
+    return true;
+}
+
+}  // namespace edc

@@ -91,6 +91,12 @@ void ed_basis_destroy(ed_basis b);
 /* Operator = basis + terms, terms uploaded once. */
 int ed_op_create(ed_basis b, const ed_terms* terms, ed_op* out);
 void ed_op_destroy(ed_op op);
+/* Kernel selection for ed_hv (no reference counterpart; used by the
+ * autotuner and by tests that compare both paths): path 0 = automatic,
+ * 1 = generic translation walk, 2 = rotation fast path (lattices whose
+ * translations are bit rotations, N <= 32, no bowtie terms); block = threads
+ * per block (64..512, multiple of 64; 0 keeps the current value). */
+int ed_op_tune(ed_op op, int32_t path, int32_t block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
  * total number of off-diagonal elements (one counting pass on the GPU). */
 int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total);

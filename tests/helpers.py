@@ -230,6 +230,11 @@ def hostcore():
         H.hc_rank.argtypes = [vp, C.c_uint64]
         H.hc_unrank.restype = C.c_uint64
         H.hc_unrank.argtypes = [vp, C.c_uint64]
+        H.hc_hv_torus.restype = C.c_int
+        H.hc_hv_torus.argtypes = [vp] * 3
+        H.hc_torus_rep.restype = C.c_int
+        H.hc_torus_rep.argtypes = [vp, C.c_uint32, vp, vp]
+        H.hc_generic_rep.argtypes = [vp, C.c_uint64, vp, vp]
         H.hc_tridiag.restype = C.c_int
         H.hc_tridiag.argtypes = [C.c_int, vp, vp, vp]
         _hc = H
@@ -268,3 +273,22 @@ class HostCore:
         w = np.zeros(self.D, np.complex128)
         hostcore().hc_hv(self.h, v.ctypes.data, w.ctypes.data)
         return w
+
+    def hv_torus(self, v):
+        """Rotation fast path; None if the lattice does not fit it."""
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        if hostcore().hc_hv_torus(self.h, v.ctypes.data, w.ctypes.data):
+            return None
+        return w
+
+    def torus_rep(self, t):
+        rep, code = C.c_uint32(), C.c_int()
+        if hostcore().hc_torus_rep(self.h, t, C.addressof(rep), C.addressof(code)):
+            return None
+        return rep.value, code.value
+
+    def generic_rep(self, t):
+        rep, code = C.c_uint64(), C.c_int()
+        hostcore().hc_generic_rep(self.h, t, C.addressof(rep), C.addressof(code))
+        return rep.value, code.value

@@ -10,6 +10,8 @@
 #include "ed_core.h"
 #include "ed_tables.h"
 #include "ed_tridiag.h"
+#include "ed_torus.h"
+#include "ed_torus_host.h"
 
 using namespace edc;
 
@@ -215,6 +217,71 @@ void hc_hv(void* p, const double* v, double* w) {
     } else {
         if (triv) hv<uint64_t, true>(h, (const cd2*)v, (cd2*)w); else hv<uint64_t, false>(h, (const cd2*)v, (cd2*)w);
     }
+}
+
+// H*v through the rotation fast path (ed_torus.h).  Returns 1 if the
+// lattice/terms do not fit the fast path (w untouched).
+int hc_hv_torus(void* p, const double* v_, double* w_) {
+    HC* h = (HC*)p;
+    const cd2* v = (const cd2*)v_;
+    cd2* w = (cd2*)w_;
+    if (h->N > 32 || !h->w.empty()) return 1;
+    Torus T;
+    std::vector<int> a(h->N), b(h->N);
+    // recover the permutations from the translation programs
+    for (int i = 0; i < h->N; i++) {
+        uint64_t s1 = apply_T<uint64_t>(1ull << i, h->G.T1), s2 = apply_T<uint64_t>(1ull << i, h->G.T2);
+        a[i] = ctz64(s1); b[i] = ctz64(s2);
+    }
+    if (!torus_detect(a, b, T)) return 1;
+    std::vector<TFam> fam;
+    std::vector<TEntry> ent;
+    double ezz0;
+    if (!torus_families(h->lam, h->Ch, T, h->b, h->c, fam, ent, ezz0)) return 1;
+    std::vector<uint32_t> rt32(h->rt.begin(), h->rt.end());
+    std::vector<cd2> u(h->rows.size());
+    for (size_t i = 0; i < u.size(); i++) {
+        double f = h->nit[h->rows[i] >> kLidShift];
+        u[i] = cd2{v[i].re * f, v[i].im * f};
+    }
+    bool triv = h->kx == 0 && h->ky == 0;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
+        uint32_t s = (uint32_t)(h->rows[i] & kStateMask);
+        double nrm = h->nt[h->rows[i] >> kLidShift];
+        double ar = 0, ai = 0;
+        auto g = [&](int64_t j, double cr, double ci) {
+            cd2 x = u[j];
+            ar += cr * x.re - ci * x.im;
+            ai += cr * x.im + ci * x.re;
+        };
+        double ez = triv ? torus_row<true>(s, T, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(),
+                                           h->dir.data(), h->ph.data(), g)
+                         : torus_row<false>(s, T, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(),
+                                            h->dir.data(), h->ph.data(), g);
+        w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
+    }
+    return 0;
+}
+
+// Representative and phase code of t through the rotation search (TRACK).
+int hc_torus_rep(void* p, uint32_t t, uint32_t* rep, int* code) {
+    HC* h = (HC*)p;
+    Torus T;
+    std::vector<int> a(h->N), b(h->N);
+    for (int i = 0; i < h->N; i++) {
+        a[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T1));
+        b[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T2));
+    }
+    if (!torus_detect(a, b, T)) return 1;
+    *rep = torus_rep<true>(t, T, *code);
+    return 0;
+}
+
+// Same through the generic walk of ed_core.h.
+void hc_generic_rep(void* p, uint64_t t, uint64_t* rep, int* code) {
+    HC* h = (HC*)p;
+    *rep = find_rep<uint64_t, true>(t, h->G, *code);
 }
 
 // Eigenpairs of the symmetric tridiagonal (d[m], e[m-1]) with the library's QL

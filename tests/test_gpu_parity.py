@@ -165,3 +165,30 @@ def test_gpu_sq4x8_full_size(gpu_ready):
     # reference rows: the first rows of the sector against the oracle's row formula
     reps, _, _ = basis.export()
     assert np.all(np.diff(reps[:100000]) > 0)
+
+
+FAST = [
+    ("chain16_k5", L.chain(16, phases=[0.3] * 16), 8, 5, 0),
+    ("sq4x4_k13", L.square(4, 4, J2=0.5, lam=0.7), 8, 1, 3),
+    ("sq4x5_k00", L.square(4, 5, J2=0.5), 10, 0, 0),
+    ("tri4x4ch_k12", L.triangular(4, 4, Ch=0.3), 8, 1, 2),
+    ("tri6x3ch_k31", L.triangular(6, 3, Ch=0.2, J2=0.1), 9, 3, 1),
+]
+
+
+@pytest.mark.parametrize("name,model,n,kx,ky", FAST, ids=[c[0] for c in FAST])
+def test_gpu_rotation_fast_path_vs_generic(gpu_ready, name, model, n, kx, ky):
+    """Both kernel paths against the oracle, complex128 and complex64."""
+    import torch
+    ed = _ed()
+    print(f"[case] {name}", flush=True)
+    o = P.OracleBasis(model, n, kx, ky)
+    basis = ed.Basis(ed.Lattice.of(model), n, kx, ky)
+    op = ed.Operator(basis, model)
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
+    wo = o.hv(v)
+    for path in (2, 1):
+        op.tune(path)
+        assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo), path
+        assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo), path

@@ -81,3 +81,37 @@ def test_tridiagonal_ql_vs_dense():
         assert np.abs(dd - ref).max() < 1e-12 * max(1, np.abs(ref).max())
         Z = z.reshape(m, m, order="F")
         assert np.abs(T @ Z - Z * dd).max() < 1e-11 * max(1, np.abs(ref).max())
+
+
+TORUS = [
+    (L.chain(16), 8, (0, 0)), (L.chain(16, phases=[0.3] * 16), 8, (5, 0)), (L.chain(13), 6, (2, 0)),
+    (L.square(4, 4, J2=0.5), 8, (0, 0)), (L.square(4, 4, J2=0.5, lam=0.7), 8, (1, 3)),
+    (L.square(4, 5, J2=0.5), 9, (2, 1)),
+    (L.triangular(4, 4, Ch=0.3), 8, (0, 0)), (L.triangular(4, 4, Ch=0.3), 8, (1, 2)),
+    (L.triangular(3, 6, Ch=0.2, J2=0.1), 9, (2, 5)), (L.triangular(6, 3, Ch=0.2), 9, (3, 1)),
+]
+
+
+@pytest.mark.parametrize("model,n,k", TORUS, ids=[f"{m.name}_n{n}_k{k[0]}{k[1]}" for m, n, k in TORUS])
+def test_rotation_fast_path_vs_oracle(model, n, k):
+    """The rotation fast path (ed_torus.h) against the oracle; its
+    representative search against the generic walk, phase code included."""
+    o = P.OracleBasis(model, n, *k)
+    h = Hp.HostCore(model, n, *k)
+    v = np.random.default_rng(4).normal(size=o.D) + 1j * np.random.default_rng(5).normal(size=o.D)
+    w = h.hv_torus(v)
+    assert w is not None, "lattice should take the rotation fast path"
+    wo = o.hv(v)
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+    rng = np.random.default_rng(6)
+    for _ in range(300):
+        bits = rng.choice(model.n_sites, size=n, replace=False)
+        t = int(sum(1 << int(b) for b in bits))
+        assert h.torus_rep(t) == h.generic_rep(t)
+
+
+def test_rotation_fast_path_declines_other_lattices():
+    h = Hp.HostCore(L.tilted_square((3, 1), (-1, 3)), 5, 0, 0)
+    assert h.hv_torus(np.zeros(h.D, complex)) is None
+    h = Hp.HostCore(L.triangular(4, 3, ChCh=0.1), 6, 0, 0)
+    assert h.hv_torus(np.zeros(h.D, complex)) is None
