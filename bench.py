@@ -86,6 +86,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c128", action="store_true")
+    ap.add_argument("--compare-paths", action="store_true", help="also time the generic translation-walk kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -140,7 +141,13 @@ def main():
         kms = [op.hv_profile(v, w, st.cuda_stream) for _ in range(args.steps)]
         kavg = sum(kms) / len(kms)
         B = algorithmic_bytes(D, G, s)
-        results[dt_name] = dict(hv_per_s=args.steps / el, ms_per_step=1e3 * el / args.steps, kernel_ms=kavg,
+        gen_ms = None
+        if dt_name == "c64" and args.compare_paths:
+            op.tune(1)
+            gen = [op.hv_profile(v, w, st.cuda_stream) for _ in range(max(2, args.steps // 4))]
+            gen_ms = sum(gen) / len(gen)
+            op.tune(0)
+        results[dt_name] = dict(generic_walk_kernel_ms=gen_ms, hv_per_s=args.steps / el, ms_per_step=1e3 * el / args.steps, kernel_ms=kavg,
                                 alg_bytes=B, achieved_gbs=B / (kavg * 1e-3) / 1e9)
         log(f"[bench] {dt_name}: {results[dt_name]}")
         del v, w

@@ -45,14 +45,30 @@ inline int cycle_len0(const std::vector<int>& T) {
     return n;
 }
 
+}  // namespace edc
+
+// C99 complex exponential from the C library.  The reference's
+// std::exp(complex<double>) (Momentum_Basis.cpp:753) compiles to this call
+// under g++; calling it directly keeps the phase table bit-equal under any
+// host compiler (clang's own lowering of std::exp differs in the last bit).
+extern "C" double _Complex cexp(double _Complex);
+
+namespace edc {
+
 // exp(-i(px lx + py ly)) for lx < L1, ly < L2, the expression of
 // Momentum_Basis.cpp:753, indexed lx*L2 + ly.
 inline std::vector<cd2> phase_table(int L1, int L2, double px, double py) {
+#if defined(__clang__)
+#pragma clang fp contract(off)   // no FMA: round -px*lx and py*ly separately, as g++ does
+#endif
     std::vector<cd2> ph(L1 * L2);
     for (int lx = 0; lx < L1; lx++)
         for (int ly = 0; ly < L2; ly++) {
-            std::complex<double> z = std::exp(std::complex<double>(0.0, -px * lx - py * ly));
-            ph[lx * L2 + ly] = cd2{z.real(), z.imag()};
+            double _Complex z;
+            __real__ z = 0.0;
+            __imag__ z = -px * lx - py * ly;
+            double _Complex e = cexp(z);
+            ph[lx * L2 + ly] = cd2{__real__ e, __imag__ e};
         }
     return ph;
 }

@@ -40,7 +40,13 @@ def test_gpu_vs_golden(gpu_ready, path):
     assert np.array_equal(norm, d["norm"]) and np.array_equal(norm_inv, d["norm_inv"])
     full = basis.export_full()
     assert np.array_equal(full["rep_loc"], d["rep_loc"])
-    assert np.array_equal(full["state_ph"].view(np.float64), d["state_ph"].view(np.float64))
+    bad = np.nonzero(full["state_ph"] != d["state_ph"])[0]
+    if len(bad):
+        st = Hp.states(m.n_sites, int(d["n_ones"]))
+        for q in bad[:8]:
+            print(f"[mismatch] rank {q} state {st[q]:#x} gpu {full['state_ph'][q].real.hex()} {full['state_ph'][q].imag.hex()}"
+                  f" ref {d['state_ph'][q].real.hex()} {d['state_ph'][q].imag.hex()}", flush=True)
+    assert len(bad) == 0, f"{len(bad)} state_ph entries differ"
     assert np.array_equal(full["Ia"], d["Ia"]) and np.array_equal(full["Ib"], d["Ib"])
     assert full["bits_right"] == int(d["bits_right"])
     op = ed.Operator(basis, m)
