@@ -76,6 +76,50 @@ def cpu_baseline(model, basis, kx, ky, budget_s):
                       f"bit-equal to pConstruct_States"}
 
 
+def _pmc_run(counter, extra, tag, tmp):
+    """One rocprofv3 --pmc pass over a child bench.py; returns [(kernel, value KiB)]."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    rp = shutil.which("rocprofv3")
+    d = os.path.join(tmp, tag)
+    cmd = ["timeout", "-s", "KILL", "120", rp, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--",
+           sys.executable, os.path.abspath(__file__)] + extra
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if r.returncode != 0 or not files:
+        raise RuntimeError(f"rocprofv3 {counter} failed rc={r.returncode}: {r.stderr[-400:]}")
+    return [(row["Kernel_Name"], float(row["Counter_Value"])) for row in csv.DictReader(open(files[0]))]
+
+
+def measure_traffic(workload, dtype, kernel_key):
+    """HBM-side bytes per launch of the dominant kernel: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) from two rocprofv3 passes, each scaled by
+    what the same counter reads for a device copy of 2 GiB."""
+    import shutil
+    import statistics
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None
+    tmp = tempfile.mkdtemp(prefix="edprof_", dir="/tmp")
+    inner = ["--inner", "--workload", workload, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
+    copy = ["--copy-calibration", "--steps", "2"]
+    f_k = [v for n, v in _pmc_run("FETCH_SIZE", inner, "f", tmp) if kernel_key in n]
+    w_k = [v for n, v in _pmc_run("WRITE_SIZE", inner, "w", tmp) if kernel_key in n]
+    f_c = [v for n, v in _pmc_run("FETCH_SIZE", copy, "cf", tmp) if "copyBuffer" in n]
+    w_c = [v for n, v in _pmc_run("WRITE_SIZE", copy, "cw", tmp) if "copyBuffer" in n]
+    copy_kib = (1 << 31) / 1024.0
+    fr = copy_kib / statistics.mean(f_c)
+    fw = copy_kib / statistics.mean(w_c)
+    fetch = statistics.mean(f_k) * 1024 * fr
+    write = statistics.mean(w_k) * 1024 * fw
+    shutil.rmtree(tmp, ignore_errors=True)
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "fetch_scale": fr,
+            "write_scale": fw, "dispatches": len(f_k)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,7 +131,28 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c128", action="store_true")
     ap.add_argument("--compare-paths", action="store_true", help="also time the generic translation-walk kernel")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
+    ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time a plain device copy of 2 GiB (reference for rocprof byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 31
+        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "ms": el * 1e3,
+                          "GBps_read_plus_write": 2 * n / el / 1e9}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -118,6 +183,15 @@ def main():
     op = ed.Operator(basis, model)
     D = basis.D
     log(f"[bench] {model.name} N={model.n_sites} n={n_ones} D={D} basis build {t_basis:.2f}s")
+    if args.inner:
+        # child of a rocprofv3 counter pass: just the H*v launches
+        cdt = torch.complex64 if args.dtype == "c64" else torch.complex128
+        v = torch.randn(D, dtype=cdt, device="cuda")
+        w = torch.empty_like(v)
+        for _ in range(args.steps):
+            op.hv(v, w)
+        torch.cuda.synchronize()
+        return
     G, T = op.count_targets()
     log(f"[bench] distinct targets G={G} ({G / D:.2f}/row), elements {T}")
 
@@ -154,6 +228,13 @@ def main():
 
     main_dt = args.dtype if args.dtype in results else "c64"
     r = results[main_dt]
+    traffic = None
+    if not args.no_traffic:
+        try:
+            traffic = measure_traffic(args.workload, main_dt, "k_hv_torus")
+            log(f"[bench] traffic {traffic}")
+        except Exception as e:  # counters are reported, never required
+            log(f"[bench] traffic measurement failed: {e!r}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -176,8 +257,10 @@ def main():
         "config": {"workload": f"{model.name} J1=1 J2=0.5 Delta=1 Sz=0 k=(0,0) single H·v", "n_sites": model.n_sites,
                    "n_up": n_ones, "D": D, "distinct_targets": G, "vector": main_dt},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_hv_rows", "kernel_ms": r["kernel_ms"], "alg_bytes_per_launch": r["alg_bytes"]},
+                     "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic["bytes"],
+                     "traffic_detail": traffic,
+                     "kernel": "k_hv_torus_c", "kernel_ms": r["kernel_ms"], "alg_bytes_per_launch": r["alg_bytes"]},
         "cpu_baseline": cpu,
         "other_precision": {k: v for k, v in results.items() if k != main_dt},
         "basis_build_s": t_basis,

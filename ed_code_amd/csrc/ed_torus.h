@@ -193,3 +193,268 @@ ED_HD double torus_row(uint32_t s, const Torus& T, int nfam, const TFam* fam, co
 }
 
 }  // namespace edc
+
+namespace edc {
+
+// Compile-time torus shape: N sites, rows of W sites, L2 rows (L2 == 1: ring,
+// T1 = rotation of the whole word, T2 = identity).
+template <int N_, int W_, int L2_>
+struct TorusC {
+    static constexpr int N = N_, W = W_, L2 = L2_;
+    static constexpr bool ring = (L2_ == 1);
+    static constexpr int L1 = ring ? N_ : W_;
+    static constexpr uint32_t nmask = N_ == 32 ? 0xffffffffu : ((1u << N_) - 1u);
+    static constexpr uint32_t rowones() {   // bit 0 of every row
+        uint32_t m = 0;
+        for (int y = 0; y < L2_; y++) m |= 1u << (y * W_);
+        return m;
+    }
+    static constexpr uint32_t wmask = W_ >= 32 ? 0xffffffffu : ((1u << W_) - 1u);
+    static constexpr uint32_t hi(int r) {    // sites with x >= r
+        return rowones() * ((wmask >> r) << r);
+    }
+};
+
+template <class TC>
+ED_HD uint32_t rot_c(uint32_t hi, uint32_t lo, int q) {
+    if (q == 0) return lo & TC::nmask;
+    if (TC::N == 32) return alignbit32(lo, lo, (uint32_t)(32 - q));
+    return alignbit32(hi, lo, (uint32_t)(TC::N - q)) & TC::nmask;
+}
+
+template <class TC>
+ED_HD void cat_c(uint32_t s, uint32_t& hi, uint32_t& lo) {
+    if (TC::N == 32) { hi = s; lo = s; }
+    else { hi = s >> (32 - TC::N); lo = (s << TC::N) | s; }
+}
+
+template <class TC, int R>
+ED_HD uint32_t field_rot_c(uint32_t s) {
+    if (R == 0) return s;
+    constexpr uint32_t h = TC::hi(R);
+    return ((s << R) & h) | ((s >> (TC::W - R)) & ~h);
+}
+
+// runtime rotation amount (family masks): hi(r) computed arithmetically
+template <class TC>
+ED_HD uint32_t field_rot_rt(uint32_t s, int r) {
+    if (r == 0) return s;
+    uint32_t h = TC::rowones() * ((TC::wmask >> r) << r);
+    return ((s << r) & h) | ((s >> (TC::W - r)) & ~h);
+}
+
+template <class TC>
+ED_HD uint32_t fam_mask_c(uint32_t s, const TFam& F) {
+    uint32_t t = TC::ring ? s : field_rot_rt<TC>(s, F.a);
+    uint32_t hi, lo;
+    cat_c<TC>(t, hi, lo);
+    return (s ^ rot_c<TC>(hi, lo, F.q)) & F.S;
+}
+
+// torus_rep for a compile-time shape; same result and phase code.
+template <class TC, bool TRACK>
+ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
+    uint32_t best = 0xffffffffu;
+    if (TC::ring) {
+        uint32_t hi, lo;
+        cat_c<TC>(t, hi, lo);
+#pragma unroll
+        for (int a = 0; a < TC::L1; a += 2) {
+            uint32_t c0 = rot_c<TC>(hi, lo, a);
+            uint32_t c1 = (a + 1 < TC::L1) ? rot_c<TC>(hi, lo, a + 1) : 0xffffffffu;
+            best = min3u(best, c0, c1);
+        }
+        if (TRACK) {
+            int bx = 0;
+            for (int a = TC::L1 - 1; a >= 0; a--)
+                if (rot_c<TC>(hi, lo, a) == best) bx = a;
+            code = (TC::L1 - bx) % TC::L1;
+        } else {
+            code = 0;
+        }
+        return best;
+    }
+    uint32_t mins[TC::L1];
+#pragma unroll
+    for (int a = 0; a < TC::L1; a++) {
+        uint32_t sa = field_rot_rt<TC>(t, a);   // a is a constant after unrolling
+        uint32_t hi, lo;
+        cat_c<TC>(sa, hi, lo);
+        uint32_t m = 0xffffffffu;
+#pragma unroll
+        for (int b = 0; b < TC::L2; b += 2) {
+            uint32_t c0 = rot_c<TC>(hi, lo, b * TC::W);
+            uint32_t c1 = (b + 1 < TC::L2) ? rot_c<TC>(hi, lo, (b + 1) * TC::W) : 0xffffffffu;
+            m = min3u(m, c0, c1);
+        }
+        mins[a] = m;
+        best = m < best ? m : best;
+    }
+    if (TRACK) {
+        int bx = 0;
+        for (int a = TC::L1 - 1; a >= 0; a--)
+            if (mins[a] == best) bx = a;
+        uint32_t sa = field_rot_rt<TC>(t, bx);
+        uint32_t hi, lo;
+        cat_c<TC>(sa, hi, lo);
+        int by = 0;
+        for (int y = TC::L2 - 1; y >= 0; y--) {
+            int b = ((y - bx) % TC::L2 + TC::L2) % TC::L2;
+            if (rot_c<TC>(hi, lo, b * TC::W) == best) by = y;
+        }
+        int ty = bx * (TC::L2 - 1) + by;
+        int lx = (TC::L1 - bx) % TC::L1;
+        int ly = ((TC::L2 - ty) % TC::L2 + TC::L2) % TC::L2;
+        code = lx * TC::L2 + ly;
+    } else {
+        code = 0;
+    }
+    return best;
+}
+
+// torus_row for a compile-time shape.
+template <class TC, bool TRIV, class Gf>
+ED_HD double torus_row_c(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0, const RankInfo& R,
+                         const uint32_t* rtab, const DirEntry* dir, const cd2* phase, Gf&& g) {
+    double ezz = ezz0;
+    int f = 0;
+    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    while (true) {
+        while (cur == 0u) {
+            if (++f >= nfam) break;
+            cur = fam_mask_c<TC>(s, fam[f]);
+        }
+        if (f >= nfam) break;
+        int x = ctz32(cur);
+        cur &= cur - 1u;
+        const TEntry E = ent[f * 32 + x];
+        uint32_t t = s ^ (1u << x) ^ (1u << E.y);
+        double cr, ci;
+        if (fam[f].type == 0) {
+            ezz -= E.dJ2;
+            cr = E.cre;
+            ci = ((s >> x) & 1u) ? -E.cim : E.cim;
+        } else {
+            double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
+            cr = -E.cim * sg;
+            ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
+        }
+        int code;
+        uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
+        uint32_t r = 0;
+        int c = 0;
+#pragma unroll
+        for (int k = 0; k < (TC::N + 7) / 8; k++) {
+            unsigned b = (rep >> (8 * k)) & 255u;
+            r += rtab[(R.off[k] + c - R.cmin[k]) * 256 + b];
+            c += popc32(b);
+        }
+        int64_t j = dir_lookup(r, dir);
+        if (j < 0) continue;
+        if (!TRIV) {
+            cd2 ph = phase[code];
+            double nr = cr * ph.re - ci * ph.im, ni = cr * ph.im + ci * ph.re;
+            cr = nr; ci = ni;
+        }
+        g(j, cr, ci);
+    }
+    return ezz;
+}
+
+}  // namespace edc
+
+namespace edc {
+
+// Decodes a directory entry fetched for rank r (see dir_lookup).
+// Branch-free (word select by masks, so the entry stays in registers).
+ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) {
+    const unsigned o = r % kDirSpan;
+    const unsigned wsel = o >> 5, bit = o & 31u;
+    const uint32_t low = (1u << bit) - 1u;
+    const uint32_t s0 = wsel == 0 ? 0xffffffffu : 0u, s1 = wsel == 1 ? 0xffffffffu : 0u,
+                   s2 = wsel == 2 ? 0xffffffffu : 0u;
+    const uint32_t bits = (e.b0 & s0) | (e.b1 & s1) | (e.b2 & s2);
+    if (!((bits >> bit) & 1u)) return -1;
+    // ones below position o: whole words before wsel, the low bits of word wsel
+    const uint32_t m0 = s0 ? low : 0xffffffffu;
+    const uint32_t m1 = s0 ? 0u : (s1 ? low : 0xffffffffu);
+    const uint32_t m2 = s2 ? low : 0u;
+    return (int64_t)(e.base + popc32(e.b0 & m0) + popc32(e.b1 & m1) + popc32(e.b2 & m2));
+}
+
+// torus_row_c with K elements in flight per lane: the orbit searches and
+// ranks of K elements are computed first, then their K directory loads and
+// K gathers are issued back to back, so that a lane waits for memory once
+// per K elements instead of twice per element.  g(j, cr, ci) as before.
+template <class TC, bool TRIV, int K, class LoadU, class Gf>
+ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
+                               const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
+                               LoadU&& loadu, Gf&& g) {
+    double ezz = ezz0;
+    int f = 0;
+    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    bool more = true;
+    while (more) {
+        uint32_t rk[K];
+        double cre[K], cim[K];
+        bool ok[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            while (cur == 0u && f < nfam) {
+                if (++f < nfam) cur = fam_mask_c<TC>(s, fam[f]);
+            }
+            ok[q] = f < nfam;
+            rk[q] = 0; cre[q] = 0; cim[q] = 0;
+            if (ok[q]) {
+                int x = ctz32(cur);
+                cur &= cur - 1u;
+                const TEntry E = ent[f * 32 + x];
+                uint32_t t = s ^ (1u << x) ^ (1u << E.y);
+                double cr, ci;
+                if (fam[f].type == 0) {
+                    ezz -= E.dJ2;
+                    cr = E.cre;
+                    ci = ((s >> x) & 1u) ? -E.cim : E.cim;
+                } else {
+                    double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
+                    cr = -E.cim * sg;
+                    ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
+                }
+                int code;
+                uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
+                uint32_t r = 0;
+                int c = 0;
+#pragma unroll
+                for (int k = 0; k < (TC::N + 7) / 8; k++) {
+                    unsigned b = (rep >> (8 * k)) & 255u;
+                    r += rtab[(R.off[k] + c - R.cmin[k]) * 256 + b];
+                    c += popc32(b);
+                }
+                if (!TRIV) {
+                    cd2 ph = phase[code];
+                    double nr = cr * ph.re - ci * ph.im, ni = cr * ph.im + ci * ph.re;
+                    cr = nr; ci = ni;
+                }
+                rk[q] = r; cre[q] = cr; cim[q] = ci;
+            }
+        }
+        more = ok[K - 1];
+        DirEntry e[K];
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (ok[q]) e[q] = dir[rk[q] / kDirSpan];
+        int64_t j[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) j[q] = ok[q] ? dir_decode(e[q], rk[q]) : -1;
+        cd2 x[K];
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (j[q] >= 0) x[q] = loadu(j[q]);
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (j[q] >= 0) g(x[q], cre[q], cim[q]);
+    }
+    return ezz;
+}
+
+}  // namespace edc

@@ -92,9 +92,11 @@ void ed_basis_destroy(ed_basis b);
 int ed_op_create(ed_basis b, const ed_terms* terms, ed_op* out);
 void ed_op_destroy(ed_op op);
 /* Kernel selection for ed_hv (no reference counterpart; used by the
- * autotuner and by tests that compare both paths): path 0 = automatic,
+ * autotuner and by tests that compare the paths): path 0 = automatic,
  * 1 = generic translation walk, 2 = rotation fast path (lattices whose
- * translations are bit rotations, N <= 32, no bowtie terms); block = threads
+ * translations are bit rotations, N <= 32, no bowtie terms), with the lattice
+ * shape compiled in when it is one of the built-in shapes, 3 = rotation fast
+ * path with the shape as run-time parameters; block = threads
  * per block (64..512, multiple of 64; 0 keeps the current value). */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the

@@ -235,6 +235,10 @@ def hostcore():
         H.hc_torus_rep.restype = C.c_int
         H.hc_torus_rep.argtypes = [vp, C.c_uint32, vp, vp]
         H.hc_generic_rep.argtypes = [vp, C.c_uint64, vp, vp]
+        H.hc_hv_torus_c.restype = C.c_int
+        H.hc_hv_torus_c.argtypes = [vp] * 3
+        H.hc_torus_rep_c.restype = C.c_int
+        H.hc_torus_rep_c.argtypes = [vp, C.c_uint32, vp, vp]
         H.hc_tridiag.restype = C.c_int
         H.hc_tridiag.argtypes = [C.c_int, vp, vp, vp]
         _hc = H
@@ -282,9 +286,23 @@ class HostCore:
             return None
         return w
 
+    def hv_torus_c(self, v):
+        """Compiled-shape batched row function; None if no compiled shape fits."""
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        if hostcore().hc_hv_torus_c(self.h, v.ctypes.data, w.ctypes.data):
+            return None
+        return w
+
     def torus_rep(self, t):
         rep, code = C.c_uint32(), C.c_int()
         if hostcore().hc_torus_rep(self.h, t, C.addressof(rep), C.addressof(code)):
+            return None
+        return rep.value, code.value
+
+    def torus_rep_c(self, t):
+        rep, code = C.c_uint32(), C.c_int()
+        if hostcore().hc_torus_rep_c(self.h, t, C.addressof(rep), C.addressof(code)):
             return None
         return rep.value, code.value
 

@@ -106,6 +106,25 @@ void hv(HC* h, const cd2* v, cd2* w) {
         w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
     }
 }
+template <class TC, bool TRIV>
+void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
+                       double ezz0, const std::vector<uint32_t>& rt32, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
+        uint32_t s = (uint32_t)(h->rows[i] & kStateMask);
+        double nrm = h->nt[h->rows[i] >> kLidShift];
+        double ar = 0, ai = 0;
+        double ez = torus_row_batched<TC, TRIV, 4>(
+            s, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(), h->dir.data(), h->ph.data(),
+            [&](int64_t j) { return u[j]; },
+            [&](const cd2& x, double cr, double ci) {
+                ar += cr * x.re - ci * x.im;
+                ai += cr * x.im + ci * x.re;
+            });
+        w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -276,6 +295,63 @@ int hc_torus_rep(void* p, uint32_t t, uint32_t* rep, int* code) {
     if (!torus_detect(a, b, T)) return 1;
     *rep = torus_rep<true>(t, T, *code);
     return 0;
+}
+
+// H*v through the compiled-shape, batched row function of the GPU kernel.
+// Returns 1 if the lattice has none of the compiled shapes.
+int hc_hv_torus_c(void* p, const double* v_, double* w_) {
+    HC* h = (HC*)p;
+    const cd2* v = (const cd2*)v_;
+    cd2* w = (cd2*)w_;
+    if (h->N > 32 || !h->w.empty()) return 1;
+    Torus T;
+    std::vector<int> a(h->N), b(h->N);
+    for (int i = 0; i < h->N; i++) {
+        a[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T1));
+        b[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T2));
+    }
+    if (!torus_detect(a, b, T)) return 1;
+    std::vector<TFam> fam;
+    std::vector<TEntry> ent;
+    double ezz0;
+    if (!torus_families(h->lam, h->Ch, T, h->b, h->c, fam, ent, ezz0)) return 1;
+    std::vector<uint32_t> rt32(h->rt.begin(), h->rt.end());
+    std::vector<cd2> u(h->rows.size());
+    for (size_t i = 0; i < u.size(); i++) {
+        double f = h->nit[h->rows[i] >> kLidShift];
+        u[i] = cd2{v[i].re * f, v[i].im * f};
+    }
+    bool triv = h->kx == 0 && h->ky == 0;
+    int L2 = T.kind == 2 ? 1 : T.L2;
+#define HC_SHAPE(n, wd, l2)                                                                       \
+    if (T.N == n && T.w == wd && L2 == l2) {                                                      \
+        if (triv) hv_batched<TorusC<n, wd, l2>, true>(h, T, fam, ent, ezz0, rt32, u, v, w);       \
+        else hv_batched<TorusC<n, wd, l2>, false>(h, T, fam, ent, ezz0, rt32, u, v, w);           \
+        return 0;                                                                                 \
+    }
+    HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1) HC_SHAPE(24, 24, 1)
+#undef HC_SHAPE
+    return 1;
+}
+
+// Compile-time-shape search (TorusC) for the shapes the tests use; returns 1
+// if the lattice has none of them.
+int hc_torus_rep_c(void* p, uint32_t t, uint32_t* rep, int* code) {
+    HC* h = (HC*)p;
+    Torus T;
+    std::vector<int> a(h->N), b(h->N);
+    for (int i = 0; i < h->N; i++) {
+        a[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T1));
+        b[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T2));
+    }
+    if (!torus_detect(a, b, T)) return 1;
+    int L2 = T.kind == 2 ? 1 : T.L2;
+#define HC_SHAPE(n, wd, l2) \
+    if (T.N == n && T.w == wd && L2 == l2) { *rep = torus_rep_c<TorusC<n, wd, l2>, true>(t, *code); return 0; }
+    HC_SHAPE(32, 4, 8) HC_SHAPE(30, 6, 5) HC_SHAPE(27, 3, 9) HC_SHAPE(24, 24, 1)
+    HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1)
+#undef HC_SHAPE
+    return 1;
 }
 
 // Same through the generic walk of ed_core.h.

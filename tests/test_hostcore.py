@@ -115,3 +115,45 @@ def test_rotation_fast_path_declines_other_lattices():
     assert h.hv_torus(np.zeros(h.D, complex)) is None
     h = Hp.HostCore(L.triangular(4, 3, ChCh=0.1), 6, 0, 0)
     assert h.hv_torus(np.zeros(h.D, complex)) is None
+
+
+SHAPES = [(L.square(4, 8), 16, (1, 3)), (L.triangular(6, 5), 15, (2, 4)), (L.triangular(3, 9), 13, (1, 7)),
+          (L.chain(24), 12, (5, 0)), (L.square(4, 4), 8, (1, 2)), (L.square(4, 5), 10, (3, 4)),
+          (L.triangular(6, 3), 9, (5, 2)), (L.chain(16), 8, (3, 0))]
+
+
+@pytest.mark.parametrize("model,n,k", SHAPES, ids=[m.name for m, n, k in SHAPES])
+def test_compiled_shape_search_matches_generic_walk(model, n, k):
+    """Representative and first-minimum phase code of the compile-time shapes
+    against the generic translation walk, on random states of the full
+    benchmark sizes (no basis enumeration needed)."""
+    # a tiny-weight build gives the handle (n_ones=1 enumerates N states only)
+    h = Hp.HostCore(model, 1, *k)
+    rng = np.random.default_rng(model.n_sites)
+    for _ in range(2000):
+        bits = rng.choice(model.n_sites, size=n, replace=False)
+        t = int(sum(1 << int(b) for b in bits))
+        assert h.torus_rep_c(t) == h.generic_rep(t)
+    # states with nontrivial stabilisers (periodic patterns) exercise the tie-break
+    L1, L2 = model.orders()
+    if L2 > 1:
+        w = L1
+        row = sum(1 << q for q in range(w // 2))
+        t = sum(row << (w * y) for y in range(L2))
+        assert h.torus_rep_c(t) == h.generic_rep(t)
+
+
+BATCHED = [(L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 3)), (L.square(4, 4, J2=0.5), 8, (0, 0)),
+           (L.square(4, 5, J2=0.3), 10, (2, 1)), (L.triangular(6, 3, Ch=0.2, J2=0.1), 9, (3, 1)),
+           (L.chain(16, phases=[0.4] * 16), 8, (5, 0)), (L.triangular(4, 4, Ch=0.3), 7, (2, 3))]
+
+
+@pytest.mark.parametrize("model,n,k", BATCHED, ids=[f"{m.name}_n{n}_k{k[0]}{k[1]}" for m, n, k in BATCHED])
+def test_batched_compiled_row_vs_oracle(model, n, k):
+    o = P.OracleBasis(model, n, *k)
+    h = Hp.HostCore(model, n, *k)
+    v = np.random.default_rng(8).normal(size=o.D) + 1j * np.random.default_rng(9).normal(size=o.D)
+    w = h.hv_torus_c(v)
+    assert w is not None
+    wo = o.hv(v)
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
