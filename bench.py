@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--no-c128", action="store_true")
     ap.add_argument("--compare-paths", action="store_true", help="also time the generic translation-walk kernel")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
+    ap.add_argument("--lanczos-iters", type=int, default=300,
+                    help="c128 vs c64 full-reorthogonalisation Lanczos length (0 = skip)")
     ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of 2 GiB (reference for rocprof byte counters)")
@@ -226,6 +228,24 @@ def main():
         log(f"[bench] {dt_name}: {results[dt_name]}")
         del v, w
 
+    lanczos = None
+    if args.lanczos_iters > 0:
+        # 300-iteration Lanczos, one cycle, full reorthogonalisation: complex128
+        # Krylov storage against complex64 storage with FP64 alpha/beta,
+        # reorthogonalisation coefficients and a final FP64 residual sweep.
+        lanczos = {"iterations": args.lanczos_iters}
+        for name, prec in (("c128", ed.ED_C128), ("c64", ed.ED_C64)):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ev, st = op.lanczos(nev=1, iterations=args.lanczos_iters, ncycles=1, tol=0.0, seed=7, precision=prec,
+                                mode=ed.ED_LZ_FULL)
+            el = time.perf_counter() - t1
+            lanczos[name] = {"seconds": el, "E0": float(ev[0]), "hv_seconds": st["hv_seconds"],
+                             "hv_calls": st["hv_calls"], "residual": st["residual"][0], "rayleigh": st["rayleigh"][0]}
+            log(f"[bench] lanczos {name}: {lanczos[name]}")
+        lanczos["speedup_c64_vs_c128"] = lanczos["c128"]["seconds"] / lanczos["c64"]["seconds"]
+        lanczos["E0_rel_diff_c64_vs_c128"] = abs(lanczos["c64"]["E0"] - lanczos["c128"]["E0"]) / abs(lanczos["c128"]["E0"])
+
     main_dt = args.dtype if args.dtype in results else "c64"
     r = results[main_dt]
     traffic = None
@@ -262,6 +282,7 @@ def main():
                      "traffic_detail": traffic,
                      "kernel": "k_hv_torus_c", "kernel_ms": r["kernel_ms"], "alg_bytes_per_launch": r["alg_bytes"]},
         "cpu_baseline": cpu,
+        "lanczos": lanczos,
         "other_precision": {k: v for k, v in results.items() if k != main_dt},
         "basis_build_s": t_basis,
     }

@@ -51,48 +51,58 @@ __global__ void k_start(int64_t D, uint64_t seed, VT* v) {
     st(v, i, 2.0 * a - 1.0, 2.0 * b - 1.0);
 }
 
-// partial[blk][k] = sum over the block's rows of conj(X_k[i]) * y[i], k < m.
-// X_k = X + k*ldx.  Block-stride loop, then a tree reduction in LDS.
-template <typename XT, typename YT>
-__global__ void __launch_bounds__(kRedThreads) k_multidot(int64_t D, int m, const XT* X, int64_t ldx, const YT* y,
-                                                           cd2* partial) {
-    __shared__ double sre[kRedThreads], sim[kRedThreads];
-    double are[kMaxMulti], aim[kMaxMulti];
-    for (int k = 0; k < m; k++) { are[k] = 0.0; aim[k] = 0.0; }
+// partial[blk][k] = sum over the block's rows of conj(X_k[i]) * y[i] for the
+// MB vectors X_k = X + k*ldx (MB a compile-time chunk, so the accumulators
+// stay in registers).  Wave reduction by shuffles, then across the block's
+// waves through LDS: one barrier per chunk.
+template <int MB, typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_multidot(int64_t D, const XT* __restrict__ X, int64_t ldx,
+                                                           const YT* __restrict__ y, cd2* __restrict__ partial,
+                                                           int kofs) {
+    __shared__ double red[kRedThreads / 64][MB][2];
+    double are[MB], aim[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) { are[k] = 0.0; aim[k] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
         cd2 yv = ld(y, i);
-        for (int k = 0; k < m; k++) {
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
             cd2 x = ld(X + k * ldx, i);
             are[k] += x.re * yv.re + x.im * yv.im;
             aim[k] += x.re * yv.im - x.im * yv.re;
         }
     }
-    for (int k = 0; k < m; k++) {
-        sre[threadIdx.x] = are[k];
-        sim[threadIdx.x] = aim[k];
-        __syncthreads();
-        for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) {
-                sre[threadIdx.x] += sre[threadIdx.x + s];
-                sim[threadIdx.x] += sim[threadIdx.x + s];
-            }
-            __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < MB; k++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            are[k] += __shfl_xor(are[k], off, 64);
+            aim[k] += __shfl_xor(aim[k], off, 64);
         }
-        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * kMaxMulti + k] = cd2{sre[0], sim[0]};
-        __syncthreads();
+        if (lane == 0) { red[wv][k][0] = are[k]; red[wv][k][1] = aim[k]; }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < MB) {
+        double r = 0.0, im = 0.0;
+        for (int q = 0; q < kRedThreads / 64; q++) { r += red[q][threadIdx.x][0]; im += red[q][threadIdx.x][1]; }
+        partial[(int64_t)blockIdx.x * kMaxMulti + kofs + threadIdx.x] = cd2{r, im};
     }
 }
 
-// y[i] = a*y[i] - sum_k c[k] * X_k[i]   (c FP64 complex, y updated in FP64)
-template <typename XT, typename YT>
-__global__ void k_multiaxpy(int64_t D, int m, const XT* X, int64_t ldx, const cd2* c, double a, YT* y) {
-    __shared__ cd2 cs[kMaxMulti];
-    if ((int)threadIdx.x < m) cs[threadIdx.x] = c[threadIdx.x];
-    __syncthreads();
+// y[i] = a*y[i] - sum_k c[k] * X_k[i]   (c FP64 complex, y updated in FP64),
+// MB vectors per pass.
+template <int MB, typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_multiaxpy(int64_t D, const XT* __restrict__ X, int64_t ldx,
+                                                            const cd2* __restrict__ c, double a, YT* __restrict__ y) {
+    cd2 cs[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) cs[k] = c[k];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
         cd2 yv = ld(y, i);
         double re = a * yv.re, im = a * yv.im;
-        for (int k = 0; k < m; k++) {
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
             cd2 x = ld(X + k * ldx, i);
             re -= cs[k].re * x.re - cs[k].im * x.im;
             im -= cs[k].re * x.im + cs[k].im * x.re;
@@ -111,14 +121,16 @@ __global__ void k_scale_copy(int64_t D, double s, const XT* x, YT* y) {
 }
 
 // y += sum_k t[k] * X_k  with y in FP64 (Ritz vector assembly, real t)
-template <typename XT>
-__global__ void k_accumulate(int64_t D, int m, const XT* X, int64_t ldx, const double* t, cd2* y) {
-    __shared__ double ts[kMaxMulti];
-    if ((int)threadIdx.x < m) ts[threadIdx.x] = t[threadIdx.x];
-    __syncthreads();
+template <int MB, typename XT>
+__global__ void __launch_bounds__(kRedThreads) k_accumulate(int64_t D, const XT* __restrict__ X, int64_t ldx,
+                                                             const double* __restrict__ t, cd2* __restrict__ y) {
+    double ts[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) ts[k] = t[k];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
         cd2 yv = y[i];
-        for (int k = 0; k < m; k++) {
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
             cd2 x = ld(X + k * ldx, i);
             yv.re += ts[k] * x.re;
             yv.im += ts[k] * x.im;
@@ -134,6 +146,9 @@ struct Ctx {
     cd2* d_partial = nullptr;   // [kRedBlocks][kMaxMulti]
     cd2* d_coef = nullptr;      // [kMaxMulti]
     double* d_t = nullptr;      // [kMaxMulti]
+    cd2* d_coef_big = nullptr;  // [coef_cap]
+    double* d_t_big = nullptr;  // [coef_cap]
+    int coef_cap = 0;
     std::vector<cd2> h_partial;
     double hv_ms = 0.0;
     int64_t hv_calls = 0;
@@ -142,6 +157,18 @@ struct Ctx {
 
 unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D + kRedThreads - 1) / kRedThreads); }
 
+constexpr int kChunk = 8;   // vectors per pass of the multi-vector kernels
+
+template <typename XT, typename YT>
+void launch_multidot(Ctx& c, int mm, const XT* X, int64_t ldx, const YT* y, unsigned g, int kofs) {
+    switch (mm) {
+        case 8: hipLaunchKernelGGL((k_multidot<8, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        case 4: hipLaunchKernelGGL((k_multidot<4, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        case 2: hipLaunchKernelGGL((k_multidot<2, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        default: hipLaunchKernelGGL((k_multidot<1, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+    }
+}
+
 // out[k] = X_k^H y for k < m (any m), FP64, fixed reduction order.
 template <typename XT, typename YT>
 int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<std::complex<double>>& out) {
@@ -149,9 +176,13 @@ int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<s
     unsigned g = grid_for(c.D);
     for (int k0 = 0; k0 < m; k0 += kMaxMulti) {
         int mm = std::min(kMaxMulti, m - k0);
-        hipLaunchKernelGGL((k_multidot<XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, mm, X + k0 * ldx, ldx, y,
-                           c.d_partial);
-        HIPCHK(hipGetLastError());
+        // passes of 8, 4, 2, 1 vectors fill columns [0, mm) of the partials
+        for (int q = 0; q < mm;) {
+            int step = (mm - q) >= 8 ? 8 : (mm - q) >= 4 ? 4 : (mm - q) >= 2 ? 2 : 1;
+            launch_multidot<XT, YT>(c, step, X + (int64_t)(k0 + q) * ldx, ldx, y, g, q);
+            HIPCHK(hipGetLastError());
+            q += step;
+        }
         HIPCHK(hipMemcpyAsync(c.h_partial.data(), c.d_partial, (size_t)g * kMaxMulti * sizeof(cd2),
                               hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
@@ -174,18 +205,54 @@ int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<s
 template <typename XT, typename YT>
 int multiaxpy(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<std::complex<double>>& coef, double a, YT* y) {
     unsigned g = grid_for(c.D);
-    if (m == 0 && a == 1.0) return ED_OK;
-    for (int k0 = 0; k0 < std::max(m, 1); k0 += kMaxMulti) {
-        int mm = std::min(kMaxMulti, m - k0);
-        if (mm < 0) mm = 0;
-        std::vector<cd2> cc(std::max(mm, 1));
-        for (int k = 0; k < mm; k++) cc[k] = cd2{coef[k0 + k].real(), coef[k0 + k].imag()};
-        HIPCHK(hipMemcpyAsync(c.d_coef, cc.data(), cc.size() * sizeof(cd2), hipMemcpyHostToDevice, c.st));
-        hipLaunchKernelGGL((k_multiaxpy<XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, mm, X + (int64_t)k0 * ldx,
-                           ldx, c.d_coef, k0 == 0 ? a : 1.0, y);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c.st));
+    if (m == 0) {
+        if (a != 1.0)
+            hipLaunchKernelGGL((k_multiaxpy<1, YT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, y, 0, c.d_coef, a, y);
+        return ED_OK;
     }
+    std::vector<cd2> cc(m);
+    for (int k = 0; k < m; k++) cc[k] = cd2{coef[k].real(), coef[k].imag()};
+    cd2* dc = c.d_coef_big;
+    if (m > c.coef_cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", m);
+    HIPCHK(hipMemcpyAsync(dc, cc.data(), m * sizeof(cd2), hipMemcpyHostToDevice, c.st));
+    bool first = true;
+    for (int q = 0; q < m;) {
+        int step = (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
+        double aa = first ? a : 1.0;
+        const XT* Xq = X + (int64_t)q * ldx;
+        switch (step) {
+            case 8: hipLaunchKernelGGL((k_multiaxpy<8, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
+            case 4: hipLaunchKernelGGL((k_multiaxpy<4, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
+            case 2: hipLaunchKernelGGL((k_multiaxpy<2, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
+            default: hipLaunchKernelGGL((k_multiaxpy<1, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
+        }
+        HIPCHK(hipGetLastError());
+        first = false;
+        q += step;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    return ED_OK;
+}
+
+// y += sum_k t[k] X_k (Ritz vector), t real
+template <typename XT>
+int accumulate(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<double>& t, cd2* y) {
+    unsigned g = grid_for(c.D);
+    if (m > c.coef_cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", m);
+    HIPCHK(hipMemcpyAsync(c.d_t_big, t.data(), m * sizeof(double), hipMemcpyHostToDevice, c.st));
+    for (int q = 0; q < m;) {
+        int step = (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
+        const XT* Xq = X + (int64_t)q * ldx;
+        switch (step) {
+            case 8: hipLaunchKernelGGL((k_accumulate<8, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
+            case 4: hipLaunchKernelGGL((k_accumulate<4, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
+            case 2: hipLaunchKernelGGL((k_accumulate<2, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
+            default: hipLaunchKernelGGL((k_accumulate<1, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
+        }
+        HIPCHK(hipGetLastError());
+        q += step;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
     return ED_OK;
 }
 
@@ -302,13 +369,7 @@ int run_reference(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, d
             off.resize(alphas.size() > 0 ? alphas.size() - 1 : 0);
             if (!lowest(alphas, off, e0, &tvec)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
             HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
-            for (int k0 = 0; k0 < nvs; k0 += kMaxMulti) {
-                int mm = std::min(kMaxMulti, nvs - k0);
-                HIPCHK(hipMemcpyAsync(c.d_t, tvec.data() + k0, mm * sizeof(double), hipMemcpyHostToDevice, c.st));
-                hipLaunchKernelGGL((k_accumulate<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, mm, V + (int64_t)k0 * D, D, c.d_t, x);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipStreamSynchronize(c.st));
-            }
+            RC((accumulate<VT>(c, nvs, V, D, tvec, x)));
             for (size_t k = 0; k < prev.size(); k++) {
                 RC((multidot<cd2, cd2>(c, 1, prev[k], 0, x, q)));
                 RC((multiaxpy<cd2, cd2>(c, 1, prev[k], 0, q, 1.0, x)));
@@ -399,13 +460,7 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
         std::vector<double> al(alphas.begin(), alphas.begin() + nvs);
         if (!lowest(al, off, e0, &tvec)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
         HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
-        for (int k0 = 0; k0 < nvs; k0 += kMaxMulti) {
-            int mm = std::min(kMaxMulti, nvs - k0);
-            HIPCHK(hipMemcpyAsync(c.d_t, tvec.data() + k0, mm * sizeof(double), hipMemcpyHostToDevice, c.st));
-            hipLaunchKernelGGL((k_accumulate<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, mm, V + (int64_t)k0 * D, D, c.d_t, x);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(c.st));
-        }
+        RC((accumulate<VT>(c, nvs, V, D, tvec, x)));
         for (int pass = 0; pass < 2; pass++)
             for (size_t e = 0; e < prev.size(); e++) {
                 RC((multidot<cd2, cd2>(c, 1, prev[e], 0, x, q)));
@@ -458,6 +513,9 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
     HIPCHK(hipMalloc(&c.d_partial, (size_t)kRedBlocks * kMaxMulti * sizeof(cd2)));
     HIPCHK(hipMalloc(&c.d_coef, kMaxMulti * sizeof(cd2)));
     HIPCHK(hipMalloc(&c.d_t, kMaxMulti * sizeof(double)));
+    c.coef_cap = std::max(p->iterations, 1) + 64;
+    HIPCHK(hipMalloc(&c.d_coef_big, c.coef_cap * sizeof(cd2)));
+    HIPCHK(hipMalloc(&c.d_t_big, c.coef_cap * sizeof(double)));
     HIPCHK(hipEventCreate(&c.e0));
     HIPCHK(hipEventCreate(&c.e1));
     std::vector<cd2*> prev;
@@ -482,7 +540,7 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
         }
     }
     for (auto* v : prev) dfree(v);
-    dfree(c.d_partial); dfree(c.d_coef); dfree(c.d_t);
+    dfree(c.d_partial); dfree(c.d_coef); dfree(c.d_t); dfree(c.d_coef_big); dfree(c.d_t_big);
     (void)hipEventDestroy(c.e0);
     (void)hipEventDestroy(c.e1);
     if (stats) {
