@@ -175,6 +175,13 @@ class Operator:
         check(lib().ed_make_h(self.h, H.ctypes.data))
         return H.reshape(self.D, self.D, order="F")
 
+    def dense_eigh(self, want_vectors=False):
+        """All eigenvalues (and eigenvectors) of the sector matrix, on the GPU."""
+        ev = np.zeros(self.D)
+        vecs = np.zeros(self.D * self.D, np.complex128) if want_vectors else None
+        check(lib().ed_dense_eigh(self.h, ev.ctypes.data, None if vecs is None else vecs.ctypes.data))
+        return (ev, vecs.reshape(self.D, self.D, order="F")) if want_vectors else ev
+
     def lanczos(self, nev=1, iterations=100, ncycles=10, tol=1e-11, seed=1234, precision=ED_C128,
                 mode=ED_LZ_REFERENCE, want_vectors=False):
         p = LanczosParams(nev, iterations, ncycles, precision, mode, 0, tol, seed)
@@ -210,8 +217,7 @@ def sector_lanczos(model: Model, kx: int, ky: int, cSz: float, n_sec_ev: int = 2
         raise EDError(4, "No states in psector")
     op = Operator(basis, model)
     if basis.D <= dense_limit:
-        H = op.make_h()
-        return np.linalg.eigvalsh(H), basis.D
+        return op.dense_eigh(), basis.D
     ev, _ = op.lanczos(nev=n_sec_ev, iterations=iterations, ncycles=ncycles, tol=tol, seed=seed,
                        precision=precision, mode=ED_LZ_REFERENCE)
     return ev, basis.D

@@ -62,6 +62,7 @@ SIGNATURES = {
     "ed_hv_profile": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.POINTER(C.c_float)]),
     "ed_hv_host": (C.c_int, [_VP, _VP, _VP]),
     "ed_make_h": (C.c_int, [_VP, _VP]),
+    "ed_dense_eigh": (C.c_int, [_VP, _VP, _VP]),
     "ed_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
 }
 

@@ -14,6 +14,8 @@
 // Nothing of size C survives the build except the 1.33 bit/state directory.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 #include <complex>
@@ -691,5 +693,55 @@ extern "C" int ed_make_h(ed_op op, double* H) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
     dfree(dH);
+    return ED_OK;
+}
+
+// Dense Hermitian eigensolve of the sector matrix on the GPU (rocSOLVER
+// zheevd): the small-sector path of exact_evecs (lanczos.cpp:81-131), which
+// the reference runs through LAPACK zheev (Matrix_Functions.cpp:28-48).
+extern "C" int ed_dense_eigh(ed_op op, double* evals, double* evecs) {
+    if (!op || !evals) return ed_fail(ED_EINVAL, "null argument");
+    ed_basis_s* B = op->b;
+    const int64_t D = B->D;
+    if (D == 0) return ed_fail(ED_EEMPTY, "No states in psector");
+    if (D > 20000) return ed_fail(ED_EINVAL, "dense eigensolve requested for D=%lld > 20000", (long long)D);
+    cd2* dH = nullptr;
+    double *dW = nullptr, *dE = nullptr;
+    int* dinfo = nullptr;
+    HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
+    HIPCHK(hipMalloc(&dW, D * sizeof(double)));
+    HIPCHK(hipMalloc(&dE, D * sizeof(double)));
+    HIPCHK(hipMalloc(&dinfo, sizeof(int)));
+    HIPCHK(hipMemset(dH, 0, D * D * sizeof(cd2)));
+    Geo G;
+    basis_geo(B, G);
+    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
+    TermsView T = op_terms(op);
+    dim3 grid(nblocks(D, 64)), blk(64);
+    if (B->N <= 32) {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+        else
+            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+    } else {
+        if (B->trivial_k)
+            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+        else
+            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+    }
+    HIPCHK(hipGetLastError());
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
+    // column-major H, upper triangle referenced (the matrix is Hermitian)
+    rocblas_status rs = rocsolver_zheevd(h, evecs ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
+                                         (rocblas_int)D, (rocblas_double_complex*)dH, (rocblas_int)D, dW, dE, dinfo);
+    int info = 0;
+    hipError_t e1 = hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
+    rocblas_destroy_handle(h);
+    if (rs != rocblas_status_success || e1 != hipSuccess) return ed_fail(ED_EDEVICE, "rocsolver_zheevd failed (%d)", (int)rs);
+    if (info != 0) return ed_fail(ED_ENOTCONV, "zheevd info=%d", info);
+    HIPCHK(hipMemcpy(evals, dW, D * sizeof(double), hipMemcpyDeviceToHost));
+    if (evecs) HIPCHK(hipMemcpy(evecs, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+    dfree(dH); dfree(dW); dfree(dE); dfree(dinfo);
     return ED_OK;
 }

@@ -1,0 +1,477 @@
+// ed_refapi.cpp -- the reference's C++ sector interface (include/ed_refapi.h)
+// on top of the C ABI (include/ed_amd.h).
+//
+// Sectors are built on the GPU and cached by (T1, T2, n_ones, kx, ky); the
+// operator is cached by its term lists, so a reference Lanczos loop that calls
+// sHv once per iteration pays one host<->device copy of v and w per call and
+// nothing else.  Fatal conditions keep the reference's behaviour: a message to
+// the log, then exit(1) (Lanczos_sector.cpp:51-55,93-97).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <memory>
+#include <vector>
+
+#include "ed_amd.h"
+#include "ed_refapi.h"
+
+extern "C" double _Complex cexp(double _Complex);
+
+namespace {
+
+using cplx = std::complex<double>;
+
+[[noreturn]] void fatal(std::ofstream& out, const std::string& what) {
+    out << "\n" << what << "\n";
+    out.flush();
+    std::exit(1);
+}
+
+void check(int rc, std::ofstream& out, const char* where) {
+    if (rc != ED_OK) fatal(out, std::string("ED GPU error in ") + where + ": " + ed_last_error());
+}
+
+int k_from_p(double p, int L) {
+    // px = 2*pi*kx/L1 (Lanczos_sector.cpp:57); recover the integer
+    long k = std::lround(p * L / (2.0 * M_PI));
+    return (int)(((k % L) + L) % L);
+}
+
+struct TermsCopy {
+    double lambda = 0, Ch = 0, ChCh = 0;
+    std::vector<int32_t> bxy, cijk, bw;
+    std::vector<double> bJ, bph, cph;
+    bool operator==(const TermsCopy& o) const {
+        return lambda == o.lambda && Ch == o.Ch && ChCh == o.ChCh && bxy == o.bxy && cijk == o.cijk && bw == o.bw &&
+               bJ == o.bJ && bph == o.bph && cph == o.cph;
+    }
+    ed_terms view() const {
+        ed_terms t;
+        t.lambda = lambda;
+        t.nbonds = (int32_t)bJ.size();
+        t.bond_xy = bxy.data(); t.bond_J = bJ.data(); t.bond_phase = bph.data();
+        t.Ch = Ch;
+        t.nchiral = (int32_t)(cijk.size() / 3);
+        t.chiral_ijk = cijk.data(); t.chiral_phase = cph.data();
+        t.ChCh = ChCh;
+        t.nbowtie = (int32_t)(bw.size() / 6);
+        t.bowtie = bw.data();
+        return t;
+    }
+};
+
+TermsCopy terms_of(double lambda, const std::vector<coordinates>& adj, double Ch, const std::vector<triangles>& ch,
+                   double ChCh, const std::vector<bowties>& chch) {
+    TermsCopy t;
+    t.lambda = lambda; t.Ch = Ch; t.ChCh = ChCh;
+    for (const auto& c : adj) {
+        t.bxy.push_back(c.x); t.bxy.push_back(c.y);
+        t.bJ.push_back(c.J);
+        t.bph.push_back(c.expphase.real()); t.bph.push_back(c.expphase.imag());
+    }
+    for (const auto& c : ch) {
+        t.cijk.push_back(c.i); t.cijk.push_back(c.j); t.cijk.push_back(c.k);
+        t.cph.push_back(c.expphase.real()); t.cph.push_back(c.expphase.imag());
+    }
+    for (const auto& b : chch) {
+        int32_t q[6] = {b.x, b.y, b.z, b.p, b.q, b.r};
+        t.bw.insert(t.bw.end(), q, q + 6);
+    }
+    return t;
+}
+
+struct Sector {
+    std::vector<int> T1, T2;
+    int n = -1, kx = -1, ky = -1;
+    ed_lattice lat = nullptr;
+    ed_basis basis = nullptr;
+    int64_t D = 0;
+    int L1 = 0, L2 = 0;
+    TermsCopy terms;
+    ed_op op = nullptr;
+    void* dv = nullptr;
+    void* dw = nullptr;
+    ~Sector() {
+        if (dv) (void)hipFree(dv);
+        if (dw) (void)hipFree(dw);
+        if (op) ed_op_destroy(op);
+        if (basis) ed_basis_destroy(basis);
+        if (lat) ed_lattice_destroy(lat);
+    }
+};
+
+std::vector<std::unique_ptr<Sector>>& cache() {
+    static std::vector<std::unique_ptr<Sector>> c;
+    return c;
+}
+
+Sector& sector(const std::vector<int>& T1, const std::vector<int>& T2, int n, double px, double py,
+               std::ofstream& out) {
+    static bool init = false;
+    if (!init) { check(ed_init(0), out, "ed_init"); init = true; }
+    if (T1.size() != T2.size()) fatal(out, "T1 and T2 lengths differ");
+    ed_lattice lat = nullptr;
+    check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), out, "ed_lattice_create");
+    int32_t L1, L2;
+    ed_lattice_orders(lat, &L1, &L2);
+    int kx = k_from_p(px, L1), ky = k_from_p(py, L2);
+    auto& C = cache();
+    for (auto& s : C)
+        if (s->T1 == T1 && s->T2 == T2 && s->n == n && s->kx == kx && s->ky == ky) {
+            ed_lattice_destroy(lat);
+            return *s;
+        }
+    std::unique_ptr<Sector> s(new Sector());
+    s->T1 = T1; s->T2 = T2; s->n = n; s->kx = kx; s->ky = ky;
+    s->lat = lat; s->L1 = L1; s->L2 = L2;
+    check(ed_basis_build(lat, n, kx, ky, &s->basis), out, "ed_basis_build");
+    ed_basis_dim(s->basis, &s->D);
+    if (C.size() >= 4) C.erase(C.begin());
+    C.push_back(std::move(s));
+    return *C.back();
+}
+
+ed_op op_for(Sector& S, const TermsCopy& t, std::ofstream& out) {
+    if (S.op && S.terms == t) return S.op;
+    if (S.op) { ed_op_destroy(S.op); S.op = nullptr; }
+    ed_terms tv = t.view();
+    check(ed_op_create(S.basis, &tv, &S.op), out, "ed_op_create");
+    S.terms = t;
+    return S.op;
+}
+
+int popcount64(int64_t s) { return __builtin_popcountll((unsigned long long)s); }
+
+// Momentum_Basis.cpp:978-1037 (Unwrap_pState): every state of each orbit with
+// coefficient v_i * exp(-i k.g) / Norm_i, the first visit of a state kept
+// (idsClass.cpp:33-53), entries with |re| or |im| > 1e-8, sorted by state.
+struct Ids {
+    int64_t stateID;
+    cplx coeff;
+};
+
+cplx cexp_c(double im) {
+    double _Complex z;
+    __real__ z = 0.0;
+    __imag__ z = im;
+    double _Complex e = cexp(z);
+    return cplx(__real__ e, __imag__ e);
+}
+
+int64_t translate(int64_t s, const std::vector<int>& T) {
+    int64_t t = 0;
+    for (size_t i = 0; i < T.size(); i++)
+        if ((s >> i) & 1) t |= int64_t(1) << T[i];
+    return t;
+}
+
+std::vector<Ids> unwrap(const std::vector<cplx>& v, const std::vector<int64_t>& reps, const std::vector<double>& norm,
+                        double px, double py, const std::vector<int>& T1, const std::vector<int>& T2, int L1, int L2) {
+    std::vector<Ids> out;
+    const cplx ex = cexp_c(-px), ey = cexp_c(-py);
+    for (size_t i = 0; i < reps.size(); i++) {
+        int64_t id = reps[i];
+        std::vector<Ids> tmp;
+        auto add = [&](int64_t sid, cplx c) {
+            for (auto& e : tmp)
+                if (e.stateID == sid) return;
+            tmp.push_back(Ids{sid, c});
+        };
+        add(id, v[i] / norm[i]);
+        cplx ph(1.0, 0.0);
+        for (int x = 0; x < L1; x++) {
+            if (x > 0) {
+                id = translate(id, T1);
+                ph = ph * ex;
+                add(id, v[i] * ph / norm[i]);
+            }
+            for (int y = 1; y < L2; y++) {
+                id = translate(id, T2);
+                ph = ph * ey;
+                add(id, v[i] * ph / norm[i]);
+            }
+        }
+        for (auto& e : tmp)
+            if (std::fabs(e.coeff.real()) > 1e-8 || std::fabs(e.coeff.imag()) > 1e-8) out.push_back(e);
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Ids& a, const Ids& b) { return a.stateID < b.stateID; });
+    return out;
+}
+
+void export_basis(Sector& S, std::vector<int64_t>& reps, std::vector<double>& norm, std::vector<double>& norm_inv,
+                  std::ofstream& out) {
+    reps.assign(S.D, 0);
+    norm.assign(S.D, 0.0);
+    norm_inv.assign(S.D, 0.0);
+    check(ed_basis_export(S.basis, reps.data(), norm.data(), norm_inv.data()), out, "ed_basis_export");
+}
+
+// Writes the _EV<i>.txt / _pEV<i>.txt files of Lanczos_sector.cpp:136-197.
+void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, int kx, int ky, double Sz,
+                   double lambda, const std::vector<cplx>& eigs, const std::vector<std::vector<cplx>>& vecs,
+                   Sector& S, double px, double py, std::ofstream& out) {
+    std::vector<int64_t> reps;
+    std::vector<double> norm, norm_inv;
+    export_basis(S, reps, norm, norm_inv, out);
+    for (int iib = 0; iib < std::min(n_sec_ev, (int)eigs.size()); iib++) {
+        std::string fp = path + (unwrap_peigvec ? "_EV" : "_pEV") + std::to_string(iib) + ".txt";
+        std::ofstream f(fp.c_str());
+        f << "\n Eigenvector for state \n";
+        f << "Sz\t(kx,ky)\tJz/lambda\n";
+        f << Sz << "\t(" << kx << "," << ky << ")\t" << lambda << std::endl;
+        f << "\n\n-------------------------------------------";
+        f << "\nState Energy " << eigs[iib] << std::endl;
+        if (unwrap_peigvec) {
+            std::vector<Ids> psi = unwrap(vecs[iib], reps, norm, px, py, S.T1, S.T2, S.L1, S.L2);
+            f << "\nEIGENVECTOR\n";
+            for (auto& e : psi) f << e.stateID << "\t" << std::fixed << std::setprecision(15) << e.coeff << std::endl;
+        } else {
+            f << "\nMomentum EIGENVECTOR\n";
+            f << "\nnp \t\t px \t\tpy\n";
+            f << S.D << "\t\t" << px << "\t\t" << py << std::endl;
+            f << "\n\npblock_states \t\t Norm \t\t\t v_coeff\n\n";
+            for (int64_t jb = 0; jb < S.D; jb++)
+                f << reps[jb] << "\t\t\t" << norm[jb] << "\t\t\t" << std::fixed << std::setprecision(15)
+                  << vecs[iib][jb] << std::endl;
+        }
+    }
+}
+
+void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, double tol, int mode,
+                 std::vector<cplx>& eigs, std::vector<std::vector<cplx>>& evecs, std::ofstream& out) {
+    int nev = (int)std::min<int64_t>(n_ev, D);
+    ed_lanczos_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.nev = nev; p.iterations = iterations; p.ncycles = ncycles; p.precision = ED_C128; p.mode = mode;
+    p.tol = tol; p.seed = 1;
+    std::vector<double> ev(nev);
+    std::vector<cplx> vecs((size_t)nev * D);
+    ed_lanczos_stats st;
+    check(ed_lanczos(op, &p, ev.data(), (double*)vecs.data(), &st), out, "ed_lanczos");
+    eigs.clear();
+    for (int k = 0; k < nev; k++) {
+        eigs.push_back(ev[k]);
+        evecs.emplace_back(vecs.begin() + (size_t)k * D, vecs.begin() + (size_t)(k + 1) * D);
+        out << "Eigenvalue number " << k << " = " << std::showpos << std::fixed << std::setprecision(15) << ev[k]
+            << std::noshowpos << std::endl;
+    }
+    out << "\n\n--------------------------------------------------\n";
+    out << "\n Lanczos SUMMARY \n" << std::endl;
+    out << " Hv calls\t\t\t" << st.hv_calls << std::endl;
+    out << " Hv total time \t\t\t" << st.hv_seconds / 60.0 << " minutes" << std::endl;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ sHv ----
+void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int64_t>& pblock_states,
+         const std::vector<double>&, const std::vector<double>&, const std::vector<int64_t>&,
+         const std::vector<std::complex<double> >&, const double& lambda, const std::vector<coordinates>& adj_list,
+         const double& Ch, const std::vector<triangles>& ch_list, const double& ChCh,
+         const std::vector<bowties>& chch_list, const int&, const std::vector<int64_t>&,
+         const std::vector<int64_t>&, const int&, const std::vector<int>& T1, const std::vector<int>& T2,
+         const double& kx, const double& ky, std::ofstream& outfile) {
+    if (pblock_states.empty()) return;
+    Sector& S = sector(T1, T2, popcount64(pblock_states[0]), kx, ky, outfile);
+    if (S.D != (int64_t)pblock_states.size())
+        fatal(outfile, "sHv: pblock_states does not match the GPU basis of this sector");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    const size_t bytes = (size_t)S.D * sizeof(cplx);
+    if (!S.dv) {
+        if (hipMalloc(&S.dv, bytes) != hipSuccess || hipMalloc(&S.dw, bytes) != hipSuccess)
+            fatal(outfile, "sHv: device allocation failed");
+    }
+    if (hipMemcpy(S.dv, v, bytes, hipMemcpyHostToDevice) != hipSuccess) fatal(outfile, "sHv: copy in failed");
+    check(ed_hv(op, ED_C128, S.dv, S.dw, nullptr), outfile, "ed_hv");
+    if (hipMemcpy(w, S.dw, bytes, hipMemcpyDeviceToHost) != hipSuccess) fatal(outfile, "sHv: copy out failed");
+}
+
+// ---------------------------------------------------- pConstruct_States ----
+void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
+                       std::vector<int>& T1, std::vector<int>& T2, int& num_ones, double& px, double& py,
+                       int64_t& n_states, std::vector<int64_t>& rep_loc,
+                       std::vector<std::complex<double> >& state_ph, std::vector<int64_t>& Ia,
+                       std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile) {
+    if ((int)T1.size() < num_ones) fatal(outfile, "ERROR: Num sites < Num_ones");
+    Sector& S = sector(T1, T2, num_ones, px, py, outfile);
+    export_basis(S, v_sz, norm, norm_inv, outfile);
+    n_states = S.D;
+    int64_t C = 0, nl = 0;
+    ed_basis_full_size(S.basis, &C, &nl);
+    rep_loc.assign(C, -1);
+    state_ph.assign(C, cplx(0, 0));
+    Ia.assign(nl, 0);
+    Ib.assign(nl, 0);
+    int32_t br = 0;
+    check(ed_basis_export_full(S.basis, rep_loc.data(), (double*)state_ph.data(), Ia.data(), Ib.data(), &br), outfile,
+          "ed_basis_export_full");
+    bits_right = br;
+}
+
+// ---------------------------------------------------------- exact_evecs ----
+void exact_evecs(std::vector<int64_t>& pblock_states, std::vector<double>&, std::vector<double>&,
+                 std::vector<int64_t>&, std::vector<std::complex<double> >&, std::vector<std::complex<double> >& eigs,
+                 std::vector<std::vector<std::complex<double> > >& previous_evecs, bool, double& lambda, int&,
+                 double& px, double& py, int& n_ones, int64_t&, std::vector<coordinates>& adj_list, double& Ch,
+                 std::vector<triangles>& ch_list, double& ChCh, std::vector<bowties>& chch_list,
+                 std::vector<int>& T1, std::vector<int>& T2, int, double, int, int, std::vector<int64_t>&,
+                 std::vector<int64_t>&, int, std::ofstream& outfile) {
+    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
+    if (S.D != (int64_t)pblock_states.size()) fatal(outfile, "exact_evecs: basis mismatch");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    std::vector<double> ev(S.D);
+    std::vector<cplx> vecs((size_t)S.D * S.D);
+    check(ed_dense_eigh(op, ev.data(), (double*)vecs.data()), outfile, "ed_dense_eigh");
+    eigs.clear();
+    for (double e : ev) eigs.push_back(e);
+    for (int64_t i = 0; i < S.D; i++)
+        previous_evecs.emplace_back(vecs.begin() + (size_t)i * S.D, vecs.begin() + (size_t)(i + 1) * S.D);
+    outfile << "\n\n--------------------------------------------------\n";
+    outfile << "\n EXACT SUMMARY \n" << std::endl;
+}
+
+// -------------------------------------------------------- lanczos_evecs ----
+void lanczos_evecs(std::vector<int64_t>& pblock_states, std::vector<double>&, std::vector<double>&,
+                   std::vector<int64_t>&, std::vector<std::complex<double> >&, std::vector<std::complex<double> >& eigs,
+                   std::vector<std::vector<std::complex<double> > >& previous_evecs, bool, double& lambda, int&,
+                   double& px, double& py, int& n_ones, int64_t&, std::vector<coordinates>& adj_list, double& Ch,
+                   std::vector<triangles>& ch_list, double& ChCh, std::vector<bowties>& chch_list,
+                   std::vector<int>& T1, std::vector<int>& T2, int n_ev, double tol, int iterations, int ncycles,
+                   std::vector<int64_t>&, std::vector<int64_t>&, int, std::ofstream& outfile) {
+    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
+    if (S.D != (int64_t)pblock_states.size()) fatal(outfile, "lanczos_evecs: basis mismatch");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    run_lanczos(op, S.D, n_ev, iterations, ncycles, tol, ED_LZ_REFERENCE, eigs, previous_evecs, outfile);
+}
+
+// ------------------------------------------------------- sector_lanczos ----
+void sector_lanczos(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
+                    std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
+                    std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2, int& n_sec_ev,
+                    bool& find_ev, bool& unwrap_peigvec, std::string& filepath_EV, double& tol, int& iterations,
+                    int& ncycles, std::ofstream& outfile) {
+    int n_ones = (int)(cSz + n_spins / 2 + 1.0e-6);   // Lanczos_sector.cpp:45-46 (integer n_spins/2)
+    ed_lattice lat = nullptr;
+    if (T1.size() != T2.size()) fatal(outfile, "T1 and T2 lengths differ");
+    check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), outfile, "ed_lattice_create");
+    int32_t L1, L2;
+    ed_lattice_orders(lat, &L1, &L2);
+    ed_lattice_destroy(lat);
+    if (kx >= L1 || ky >= L2 || kx < 0 || ky < 0) fatal(outfile, "Invalid kx or ky");
+    double px = 2 * kx * M_PI / L1, py = 2 * ky * M_PI / L2;
+    double Sz = 0.5 * n_ones - 0.5 * (n_spins - n_ones);
+    outfile << "\n--------------------------------------------------------------------\n";
+    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
+    outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << ")\t" << S.D << std::endl;
+    if (S.D == 0) fatal(outfile, "No states in psector");
+    outfile << "------------------------------------------------------------------" << std::endl;
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    std::vector<cplx> eigs;
+    std::vector<std::vector<cplx>> vecs;
+    if (S.D > 10000) {   // Lanczos_sector.cpp:105
+        run_lanczos(op, S.D, n_sec_ev, iterations, ncycles, tol, ED_LZ_REFERENCE, eigs, vecs, outfile);
+    } else {
+        std::vector<double> ev(S.D);
+        std::vector<cplx> dv((size_t)S.D * S.D);
+        check(ed_dense_eigh(op, ev.data(), find_ev ? (double*)dv.data() : nullptr), outfile, "ed_dense_eigh");
+        for (double e : ev) eigs.push_back(e);
+        if (find_ev)
+            for (int64_t i = 0; i < std::min<int64_t>(n_sec_ev, S.D); i++)
+                vecs.emplace_back(dv.begin() + (size_t)i * S.D, dv.begin() + (size_t)(i + 1) * S.D);
+    }
+    outfile << "pblock_eigs.size() = " << eigs.size() << std::endl;
+    outfile << "\n\nEnergies\t\tEnergies/site\t\tSz\tkx\tky\tJz\tJ2\n\n";
+    for (size_t i = 0; i < eigs.size(); i++)
+        outfile << std::fixed << std::setprecision(15) << eigs[i].real() << "\t" << eigs[i].real() / double(n_spins)
+                << "\t" << std::setprecision(3) << Sz << "\t" << px << "\t" << py << "\t" << lambda << "\t" << J2
+                << "\n";
+    outfile.flush();
+    if (find_ev) write_vectors(filepath_EV, unwrap_peigvec, n_sec_ev, kx, ky, Sz, lambda, eigs, vecs, S, px, py, outfile);
+}
+
+void sector_lanczos(int&, int&, int&, double&, int&, double&, double&, std::vector<coordinates>&, double&,
+                    std::vector<triangles>&, double&, std::vector<bowties>&, std::vector<int>&, std::vector<int>&,
+                    std::vector<int>&, int&, bool&, bool&, std::string&, double&, int&, int&, std::ofstream& outfile) {
+    fatal(outfile, "sector_lanczos: three-generator (3-D) translation groups are not supported by the GPU library");
+}
+
+// ----------------------------------------------------------- ARPACK path ----
+void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>&, std::vector<double>&,
+                   std::vector<int64_t>&, std::vector<std::complex<double> >&,
+                   std::vector<std::complex<double> >& pblock_eigs, zMatrix& pblock_eigvecs, bool find_ev,
+                   double& lambda, int&, double& px, double& py, int& n_ones, int64_t& n_p,
+                   std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
+                   std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2, int n_ev,
+                   double tol, int maxiter, std::vector<int64_t>&, std::vector<int64_t>&, int&,
+                   std::ofstream& outfile) {
+    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
+    if (S.D != (int64_t)pblock_states.size() || S.D != n_p) fatal(outfile, "ARPACK_sector: basis mismatch");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    // ARPACK_sector.cpp:213-217: n_ev eigenvalues, all of them when n_p <= n_ev + 1
+    int it = n_ev > n_p - 1 ? (int)n_p : n_ev;
+    std::vector<cplx> eigs;
+    std::vector<std::vector<cplx>> vecs;
+    if (n_p > n_ev + 1) {
+        int iters = (int)std::min<int64_t>(n_p, std::max(4 * it + 60, std::min(maxiter, 400)));
+        run_lanczos(op, S.D, it, iters, 1, tol, ED_LZ_FULL, eigs, vecs, outfile);
+    } else {
+        std::vector<double> ev(S.D);
+        std::vector<cplx> dv((size_t)S.D * S.D);
+        check(ed_dense_eigh(op, ev.data(), (double*)dv.data()), outfile, "ed_dense_eigh");
+        for (int k = 0; k < it; k++) {
+            eigs.push_back(ev[k]);
+            vecs.emplace_back(dv.begin() + (size_t)k * S.D, dv.begin() + (size_t)(k + 1) * S.D);
+        }
+    }
+    // ARPACK returns the wanted Ritz values largest first; the caller prints
+    // them in reverse (ARPACK_sector.cpp:339-340), i.e. ascending.
+    pblock_eigs.assign(eigs.rbegin(), eigs.rend());
+    if (find_ev) {
+        pblock_eigvecs.resize(it, S.D);   // Evecs(i, j) = component j of vector i (ARPACK_sector.cpp:176-181)
+        for (int i = 0; i < it; i++)
+            for (int64_t j = 0; j < S.D; j++) pblock_eigvecs(i, j) = vecs[it - 1 - i][j];
+    }
+    outfile << "\nEigenvalues\n";
+    for (int kb = 0; kb < it; kb++) outfile << std::setprecision(10) << pblock_eigs[kb] << "\t";
+}
+
+void sector_eigs_eigvecs(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
+                         std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list,
+                         double& ChCh, std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
+                         int& n_sec_ev, bool& find_ev, bool&, std::string&, double& tol, int& maxiter,
+                         std::ofstream& outfile) {
+    int n_ones = (int)(cSz + n_spins / 2 + 1.0e-6);
+    ed_lattice lat = nullptr;
+    if (T1.size() != T2.size()) fatal(outfile, "T1 and T2 lengths differ");
+    check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), outfile, "ed_lattice_create");
+    int32_t L1, L2;
+    ed_lattice_orders(lat, &L1, &L2);
+    ed_lattice_destroy(lat);
+    if (kx >= L1 || ky >= L2 || kx < 0 || ky < 0) fatal(outfile, "Invalid kx or ky");
+    double px = 2 * kx * M_PI / L1, py = 2 * ky * M_PI / L2;
+    double Sz = 0.5 * n_ones - 0.5 * (n_spins - n_ones);
+    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
+    outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << ")\t" << S.D << std::endl;
+    if (S.D == 0) fatal(outfile, "No states in psector");
+    std::vector<int64_t> reps;
+    std::vector<double> norm, norm_inv;
+    export_basis(S, reps, norm, norm_inv, outfile);
+    std::vector<int64_t> rep_loc, Ia, Ib;
+    std::vector<cplx> state_ph, pblock_eigs;
+    int bits_right = 0;
+    int64_t n_p = S.D;
+    zMatrix pblock_eigvecs;
+    ARPACK_sector(reps, norm, norm_inv, rep_loc, state_ph, pblock_eigs, pblock_eigvecs, find_ev, lambda, n_spins, px,
+                  py, n_ones, n_p, adj_list, Ch, ch_list, ChCh, chch_list, T1, T2, n_sec_ev, tol, maxiter, Ia, Ib,
+                  bits_right, outfile);
+    outfile << "\n\nEnergies\t\tEnergies/site\t\tSz\tkx\tky\tJz\tJ2\n\n";
+    for (int i = (int)pblock_eigs.size() - 1; i >= 0; i--)
+        outfile << std::fixed << std::setprecision(15) << pblock_eigs[i].real() << "\t"
+                << pblock_eigs[i].real() / double(n_spins) << "\t" << std::setprecision(3) << Sz << "\t" << px << "\t"
+                << py << "\t" << lambda << "\t" << J2 << "\n";
+}

@@ -119,6 +119,13 @@ int ed_hv_host(ed_op op, const double* v, double* w);
 /* Dense D x D column-major complex128 H (makeH, H_v.cpp:21-190), D <= 20000. */
 int ed_make_h(ed_op op, double* H);
 
+/* All D eigenvalues (ascending) and, if evecs is non-NULL, the eigenvectors
+ * (column-major D x D complex128, column k = eigenvector k) of the sector
+ * matrix, D <= 20000, on the GPU (exact_evecs, lanczos.cpp:81-131, decl
+ * lanczos.h:8-38; the reference diagonalises with LAPACK zheev,
+ * Matrix_Functions.cpp:28-48). */
+int ed_dense_eigh(ed_op op, double* evals, double* evecs);
+
 /* Lanczos (lanczos_evecs, lanczos.cpp:133-349, decl lanczos.h:71-99).
  * mode ED_LZ_REFERENCE keeps the reference's restart schedule: ncycles cycles
  * of `iterations` steps, deflation against earlier eigenvectors every step,
