@@ -152,8 +152,16 @@ class Operator:
         return w
 
     def tune(self, path: int = 0, block: int = 0):
-        """Kernel choice: path 0 auto, 1 generic walk, 2 rotation fast path."""
+        """Kernel choice: path 0 auto (tuned on the first large H*v), 1 generic
+        walk, 2 compiled-shape rotation kernel (K=4), 3 run-time-shape
+        rotation kernel, 4 compiled-shape K=2 at 8 waves; block 0 = default."""
         check(lib().ed_op_tune(self.h, path, block))
+
+    def tuning(self, dtype: str = "c128"):
+        """(path, variant, block) the next H*v of this precision runs."""
+        p, v, b = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().ed_op_tuning(self.h, ED_C64 if dtype == "c64" else ED_C128, C.byref(p), C.byref(v), C.byref(b)))
+        return p.value, v.value, b.value
 
     def hv_profile(self, v, w, stream=None) -> float:
         """ed_hv with HIP events around the row kernel; returns its ms."""

@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libed_amd.so")
+# ED_AMD_LIB overrides the library path (for comparing builds); default in-tree.
+LIB_PATH = os.environ.get("ED_AMD_LIB", os.path.join(HERE, "libed_amd.so"))
 
 ED_OK, ED_EINVAL, ED_ENOMEM, ED_EDEVICE, ED_EEMPTY, ED_ENOTCONV = range(6)
 ED_C64, ED_C128 = 0, 1
@@ -57,6 +58,7 @@ SIGNATURES = {
     "ed_op_create": (C.c_int, [_VP, C.POINTER(Terms), C.POINTER(_VP)]),
     "ed_op_destroy": (None, [_VP]),
     "ed_op_tune": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "ed_op_tuning": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ed_op_count_targets": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ed_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
     "ed_hv_profile": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.POINTER(C.c_float)]),

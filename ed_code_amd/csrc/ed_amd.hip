@@ -533,11 +533,33 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 3) return ed_fail(ED_EINVAL, "path %d not in {0,1,2,3}", path);
+    if (path < 0 || path > 4) return ed_fail(ED_EINVAL, "path %d not in {0,1,2,3,4}", path);
     if (path >= 2 && !op->use_torus) return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
+    if ((path == 2 || path == 4) && !torus_shape_compiled(op->torus))
+        return ed_fail(ED_EINVAL, "lattice shape is not one of the compiled shapes");
     if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
     op->tune_path = path;
+    op->block_fixed = block != 0;
     if (block) op->tune_block = block;
+    else if (op->use_torus) op->tune_block = 512;
+    op->sel_var[0] = op->sel_var[1] = -1;
+    return ED_OK;
+}
+
+extern "C" int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block) {
+    if (!op || !path || !variant || !block) return ed_fail(ED_EINVAL, "null argument");
+    if (dtype != ED_C128 && dtype != ED_C64) return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+    if (!op->use_torus || op->tune_path == 1) {
+        *path = 1;
+        *variant = -1;
+        *block = 256;
+        return ED_OK;
+    }
+    int p, v, bs;
+    torus_config(op, dtype == ED_C128 ? 0 : 1, p, v, bs);
+    *path = p;
+    *variant = v;
+    *block = bs;
     return ED_OK;
 }
 
@@ -568,6 +590,10 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 256)), blk(256);
+    if (op->use_torus && op->tune_path != 1) {
+        int rc = torus_prepare<VT>(op, u, v, w, st);
+        if (rc != ED_OK) return rc;
+    }
     if (ev0) HIPCHK(hipEventRecord(ev0, st));
     if (op->use_torus && op->tune_path != 1) {
         int rc = launch_hv_torus<VT>(op, u, v, w, st);

@@ -133,13 +133,16 @@ ED_HD int classify(S s, const Geo& G) {
     return ok ? lid : 0;
 }
 
-template <typename S>
-ED_HD uint64_t rank_of(S s, const RankInfo& R, const uint64_t* tab) {
-    uint64_t r = 0;
+// Rank of a weight-n word through the byte-sliced tables.  The last byte's
+// table has a single block: its count of lower set bits is n - popcount(byte).
+template <typename T, typename S>
+ED_HD T rank_tab(S s, const RankInfo& R, const T* tab, int nbytes) {
+    T r = 0;
     int c = 0;
-    for (int k = 0; k < R.nbytes; k++) {
+    for (int k = 0; k < nbytes; k++) {
         unsigned b = (unsigned)(s >> (8 * k)) & 255u;
-        r += tab[(R.off[k] + c - R.cmin[k]) * 256 + b];
+        int blk = (k == nbytes - 1) ? R.off[k] : (R.off[k] + c - R.cmin[k]);
+        r += tab[blk * 256 + b];
 #if defined(__HIP_DEVICE_COMPILE__)
         c += __popc(b);
 #else
@@ -147,6 +150,11 @@ ED_HD uint64_t rank_of(S s, const RankInfo& R, const uint64_t* tab) {
 #endif
     }
     return r;
+}
+
+template <typename S>
+ED_HD uint64_t rank_of(S s, const RankInfo& R, const uint64_t* tab) {
+    return rank_tab<uint64_t, S>(s, R, tab, R.nbytes);
 }
 
 ED_HD int popc32(uint32_t x) {

@@ -21,10 +21,6 @@
 
 using namespace edc;
 
-#ifndef ED_HV_BATCH
-#define ED_HV_BATCH 4   // elements in flight per lane in the rotation kernel
-#endif
-
 template <bool TRIV, typename VT>
 __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __restrict__ rows, Torus T, int nfam,
                                                   const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
@@ -61,21 +57,23 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
     w[i] = o;
 }
 
-// Same kernel with the lattice shape as compile-time constants (fully
-// unrolled rotation search, masks as immediates).
-template <class TC, bool TRIV, typename VT>
-__global__ void __launch_bounds__(512) k_hv_torus_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
-                                                    const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
-                                                    double ezz0, RankInfo R, const uint32_t* __restrict__ grtab,
-                                                    int rtab_n, const DirEntry* __restrict__ dir,
-                                                    const cd2* __restrict__ gphase, int nphase,
-                                                    const double* __restrict__ ntab, const VT* __restrict__ u,
-                                                    const VT* __restrict__ v, VT* __restrict__ w) {
+// Row body of the compiled-shape kernels: the lattice shape as compile-time
+// constants (fully unrolled rotation search, masks as immediates), K gathered
+// elements in flight per lane.  The order in which elements are accumulated
+// does not depend on K, so every variant returns bit-identical results.
+template <class TC, bool TRIV, int K, typename VT>
+__device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+                                                const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
+                                                double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
+                                                int rtab_n, const DirEntry* __restrict__ dir,
+                                                const cd2* __restrict__ gphase, int nphase,
+                                                const double* __restrict__ ntab, const VT* __restrict__ u,
+                                                const VT* __restrict__ v, VT* __restrict__ w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    TEntry* ent = (TEntry*)smem;
-    TFam* fam = (TFam*)(ent + kMaxFam * 32);
-    cd2* phase = (cd2*)(fam + kMaxFam);
-    uint32_t* rtab = (uint32_t*)(phase + nphase);
+    TEntry* ent = (TEntry*)smem;                     // nfam*32
+    TFam* fam = (TFam*)(ent + nfam * 32);            // nfam
+    cd2* phase = (cd2*)(fam + nfam + (nfam & 1));    // nphase (16-byte aligned)
+    uint32_t* rtab = (uint32_t*)(phase + nphase);    // rtab_n
     for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
@@ -87,25 +85,42 @@ __global__ void __launch_bounds__(512) k_hv_torus_c(int64_t D, const uint64_t* _
     const unsigned nb = gridDim.x, q = nb / 8, rem = nb % 8;
     const unsigned xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
     const unsigned lb = xcd * q + (xcd < rem ? xcd : rem) + slot;
-    {
-        int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
-        if (i >= D) return;
-        uint64_t rw = rows[i];
-        uint32_t s = (uint32_t)(rw & kStateMask);
-        double nrm = ntab[(int)(rw >> kLidShift)];
-        double ar = 0.0, ai = 0.0;
-        double ez = torus_row_batched<TC, TRIV, ED_HV_BATCH>(
-            s, nfam, fam, ent, ezz0, R, rtab, dir, phase, [&](int64_t j) { return load_c(u, j); },
-            [&](const cd2& x, double cr, double ci) {
-                ar += cr * x.re - ci * x.im;
-                ai += cr * x.im + ci * x.re;
-            });
-        cd2 vi = load_c(v, i);
-        VT o;
-        o.re = nrm * ar + ez * vi.re;
-        o.im = nrm * ai + ez * vi.im;
-        w[i] = o;
-    }
+    int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    uint64_t rw = rows[i];
+    uint32_t s = (uint32_t)(rw & kStateMask);
+    double nrm = ntab[(int)(rw >> kLidShift)];
+    double ar = 0.0, ai = 0.0;
+    double ez = torus_row_batched<TC, TRIV, K>(
+        s, nfam, fam, ent, ezz0, R, rtab, dir, phase, [&](int64_t j) { return load_c(u, j); },
+        [&](const cd2& x, double cr, double ci) {
+            ar += cr * x.re - ci * x.im;
+            ai += cr * x.im + ci * x.re;
+        });
+    cd2 vi = load_c(v, i);
+    VT o;
+    o.re = nrm * ar + ez * vi.re;
+    o.im = nrm * ai + ez * vi.im;
+    w[i] = o;
+}
+
+#define ED_TORUS_C_ARGS                                                                                   \
+    int64_t D, const uint64_t *__restrict__ rows, int nfam, const TFam *__restrict__ gfam,               \
+        const TEntry *__restrict__ gent, double ezz0, RankInfo R, const uint32_t *__restrict__ grtab,     \
+        int rtab_n, const DirEntry *__restrict__ dir, const cd2 *__restrict__ gphase, int nphase,         \
+        const double *__restrict__ ntab, const VT *__restrict__ u, const VT *__restrict__ v, VT *__restrict__ w
+#define ED_TORUS_C_PASS D, rows, nfam, gfam, gent, ezz0, R, grtab, rtab_n, dir, gphase, nphase, ntab, u, v, w
+
+// Variant A: four elements in flight, occupancy left to the register count.
+template <class TC, bool TRIV, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_c4(ED_TORUS_C_ARGS) {
+    hv_torus_c_body<TC, TRIV, 4, VT>(ED_TORUS_C_PASS);
+}
+
+// Variant B: two elements in flight, registers capped for 8 waves per SIMD.
+template <class TC, bool TRIV, typename VT>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) k_hv_torus_c2w8(ED_TORUS_C_ARGS) {
+    hv_torus_c_body<TC, TRIV, 2, VT>(ED_TORUS_C_PASS);
 }
 
 // Shapes compiled with constant geometry: the benchmark lattices and the
@@ -121,21 +136,22 @@ __global__ void __launch_bounds__(512) k_hv_torus_c(int64_t D, const uint64_t* _
     X(16, 16, 1)
 
 template <class TC, typename VT>
-static bool launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, size_t sm) {
+static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, size_t sm, int var, int bs) {
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
-    const int bs = op->tune_block;
     dim3 grid((unsigned)((D + bs - 1) / bs)), blk(bs);
     int nph = B->L1 * B->L2;
-    if (B->trivial_k)
-        hipLaunchKernelGGL((k_hv_torus_c<TC, true, VT>), grid, blk, sm, st, D, B->d_rows, op->nfam, op->d_tfam,
-                           op->d_tent, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph,
-                           B->d_ntab, u, v, w);
-    else
-        hipLaunchKernelGGL((k_hv_torus_c<TC, false, VT>), grid, blk, sm, st, D, B->d_rows, op->nfam, op->d_tfam,
-                           op->d_tent, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph,
-                           B->d_ntab, u, v, w);
-    return true;
+#define ED_GO(KER) \
+    hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, op->nfam, op->d_tfam, op->d_tent, op->ezz0, B->rinfo, \
+                       B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
+    if (var == 0) {
+        if (B->trivial_k) ED_GO((k_hv_torus_c4<TC, true, VT>));
+        else ED_GO((k_hv_torus_c4<TC, false, VT>));
+    } else {
+        if (B->trivial_k) ED_GO((k_hv_torus_c2w8<TC, true, VT>));
+        else ED_GO((k_hv_torus_c2w8<TC, false, VT>));
+    }
+#undef ED_GO
 }
 
 bool torus_shape_compiled(const Torus& T) {
@@ -152,26 +168,121 @@ size_t torus_smem_bytes(const ed_op_s* op) {
            (size_t)B->rtab32_n * sizeof(uint32_t);
 }
 
+// LDS of the compiled-shape kernel: entries and families sized by the
+// operator's family count.
+static size_t torus_c_smem_bytes(const ed_op_s* op) {
+    const ed_basis_s* B = op->b;
+    const int nf = op->nfam;
+    return (size_t)nf * 32 * sizeof(TEntry) + (size_t)(nf + (nf & 1)) * sizeof(TFam) +
+           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)B->rtab32_n * sizeof(uint32_t);
+}
+
+// Launches the compiled-shape kernel for the operator's lattice; false when
+// the shape is not one of the built-in ones.
+template <typename VT>
+static bool launch_compiled(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, int var, int bs) {
+    const Torus& T = op->torus;
+    int L2 = T.kind == 2 ? 1 : T.L2;
+#define ED_LAUNCH(n, wd, l2)                                                            \
+    if (T.N == n && T.w == wd && L2 == l2) {                                            \
+        launch_c<TorusC<n, wd, l2>, VT>(op, u, v, w, st, torus_c_smem_bytes(op), var, bs); \
+        return true;                                                                    \
+    }
+    ED_TORUS_SHAPES(ED_LAUNCH)
+#undef ED_LAUNCH
+    return false;
+}
+
+template <typename VT>
+static constexpr int dtype_slot() {
+    return std::is_same<VT, cd2>::value ? 0 : 1;
+}
+
+// Candidate (variant, block) pairs of the compiled-shape kernel.  Which one is
+// fastest depends on the register pressure of the shape and on the vector
+// precision (measured: K=4 at 512 threads for complex64 on the 32-site torus,
+// K=4 at 256 for complex128, K=2 at 8 waves close behind).
+static const int kTuneCfg[4][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}};
+static const int64_t kTuneMinRows = int64_t(1) << 20;
+
+// Chooses the configuration of the compiled-shape kernel for this operator
+// and precision on the first call: below kTuneMinRows rows, or while the
+// stream is being captured, the default (K=4, 512 threads, or the block fixed
+// by ed_op_tune) is taken; above, each candidate runs on the caller's vectors
+// (one untimed launch, then the best of two timed ones) and the fastest is
+// kept.  Every candidate writes the same w.
+template <typename VT>
+int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
+    if (!op->use_torus || op->tune_path != 0 || !torus_shape_compiled(op->torus)) return ED_OK;
+    const int k = dtype_slot<VT>();
+    if (op->sel_var[k] >= 0) return ED_OK;
+    const int64_t D = op->b->D;
+    int best_var = 0, best_bs = op->block_fixed ? op->tune_block : 512;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    if (D >= kTuneMinRows && cs == hipStreamCaptureStatusNone) {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (const auto& c : kTuneCfg) {
+            if (op->block_fixed && c[1] != op->tune_block) continue;
+            float t = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(e0, st));
+                launch_compiled<VT>(op, u, v, w, st, c[0], c[1]);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) t = std::min(t, ms);
+            }
+            if (t < best) {
+                best = t;
+                best_var = c[0];
+                best_bs = c[1];
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    op->sel_var[k] = best_var;
+    op->sel_block[k] = best_bs;
+    return ED_OK;
+}
+
+// Kernel configuration the next launch uses: path 2 (compiled shape),
+// 3 (run-time shape); variant 0 (K=4) or 1 (K=2 at 8 waves), -1 before
+// tuning or for the run-time kernel.
+void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
+    const bool compiled = op->tune_path != 3 && torus_shape_compiled(op->torus);
+    path = compiled ? 2 : 3;
+    bs = op->tune_block;
+    var = -1;
+    if (!compiled) return;
+    if (op->tune_path == 2) var = 0;
+    else if (op->tune_path == 4) var = 1;
+    else if (op->sel_var[dslot] >= 0) {
+        var = op->sel_var[dslot];
+        bs = op->sel_block[dslot];
+    }
+}
+
 template <typename VT>
 int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
-    const int bs = op->tune_block;
+    int path, var, bs;
+    torus_config(op, dtype_slot<VT>(), path, var, bs);
+    if (path == 2) {
+        launch_compiled<VT>(op, u, v, w, st, var < 0 ? 0 : var, bs);
+        HIPCHK(hipGetLastError());
+        return ED_OK;
+    }
     dim3 grid((unsigned)((D + bs - 1) / bs)), blk(bs);
     size_t sm = torus_smem_bytes(op);
     int nph = B->L1 * B->L2;
-    if (op->tune_path != 3) {
-        const Torus& T = op->torus;
-        int L2 = T.kind == 2 ? 1 : T.L2;
-#define ED_LAUNCH(n, wd, l2)                                               \
-        if (T.N == n && T.w == wd && L2 == l2) {                           \
-            launch_c<TorusC<n, wd, l2>, VT>(op, u, v, w, st, sm);          \
-            HIPCHK(hipGetLastError());                                     \
-            return ED_OK;                                                  \
-        }
-        ED_TORUS_SHAPES(ED_LAUNCH)
-#undef ED_LAUNCH
-    }
     if (B->trivial_k)
         hipLaunchKernelGGL((k_hv_torus<true, VT>), grid, blk, sm, st, D, B->d_rows, op->torus, op->nfam, op->d_tfam,
                            op->d_tent, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph,
@@ -186,3 +297,5 @@ int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st
 
 template int launch_hv_torus<cd2>(ed_op_s*, const cd2*, const cd2*, cd2*, hipStream_t);
 template int launch_hv_torus<cf2>(ed_op_s*, const cf2*, const cf2*, cf2*, hipStream_t);
+template int torus_prepare<cd2>(ed_op_s*, const cd2*, const cd2*, cd2*, hipStream_t);
+template int torus_prepare<cf2>(ed_op_s*, const cf2*, const cf2*, cf2*, hipStream_t);

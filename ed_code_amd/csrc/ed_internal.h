@@ -68,8 +68,11 @@ struct ed_op_s {
     edc::TFam* d_tfam = nullptr;
     edc::TEntry* d_tent = nullptr;
     double ezz0 = 0;
-    int tune_path = 0;        // 0 auto, 1 generic walk, 2 rotation fast path
+    int tune_path = 0;        // 0 auto, 1 generic walk, 2/4 compiled shape K=4 / K=2, 3 run-time shape
     int tune_block = 256;
+    bool block_fixed = false; // block set by ed_op_tune
+    int sel_var[2] = {-1, -1};   // tuned variant per precision (0 complex128, 1 complex64)
+    int sel_block[2] = {0, 0};
     void* d_u64 = nullptr;    // prescaled gather stream, complex64 [D]
     void* d_u128 = nullptr;   // prescaled gather stream, complex128 [D]
     std::vector<edc::Bond> bonds;
@@ -83,3 +86,7 @@ int op_hv(ed_op_s* op, int dtype, const void* v, void* w, hipStream_t st);
 size_t torus_smem_bytes(const ed_op_s* op);
 template <typename VT>
 int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
+template <typename VT>
+int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
+void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs);
+bool torus_shape_compiled(const edc::Torus& T);

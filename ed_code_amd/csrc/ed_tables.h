@@ -93,7 +93,9 @@ inline std::vector<uint64_t> binom_table() {
 
 // Byte-sliced combinatorial-number-system rank tables for weight-n words of
 // N bits: entry (k, c, b) = sum over the set bits q of byte value b, taken in
-// increasing order with local index i = 1, 2, ..., of C(8k + q, c + i).
+// increasing order with local index i = 1, 2, ..., of C(8k + q, c + i), for
+// every possible count c of set bits below byte k; the top byte has one block,
+// c = n - popcount(b).
 inline std::vector<uint64_t> rank_tables(int N, int n, RankInfo& R) {
     std::memset(&R, 0, sizeof(R));
     R.nbytes = (N + 7) / 8;
@@ -104,6 +106,21 @@ inline std::vector<uint64_t> rank_tables(int N, int n, RankInfo& R) {
         int cmax = std::min(n, 8 * k);
         R.off[k] = off;
         R.cmin[k] = cmin;
+        if (k == R.nbytes - 1) {
+            // the set bits below the top byte number n - popcount(byte)
+            for (int b = 0; b < 256; b++) {
+                int c = n - __builtin_popcount(b);
+                uint64_t v = 0;
+                if (c >= 0 && c <= 8 * k) {
+                    int i = 0;
+                    for (int q = 0; q < 8; q++)
+                        if ((b >> q) & 1) { i++; v += binom_u64(8 * k + q, c + i); }
+                }
+                rt.push_back(v);
+            }
+            off++;
+            continue;
+        }
         for (int c = cmin; c <= cmax; c++) {
             for (int b = 0; b < 256; b++) {
                 uint64_t v = 0;

@@ -173,13 +173,7 @@ ED_HD double torus_row(uint32_t s, const Torus& T, int nfam, const TFam* fam, co
         int code;
         uint32_t rep = torus_rep<!TRIV>(t, T, code);
         // rank of the representative (byte-sliced tables)
-        uint32_t r = 0;
-        int c = 0;
-        for (int k = 0; k < R.nbytes; k++) {
-            unsigned b = (rep >> (8 * k)) & 255u;
-            r += rtab[(R.off[k] + c - R.cmin[k]) * 256 + b];
-            c += popc32(b);
-        }
+        uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, R.nbytes);
         int64_t j = dir_lookup(r, dir);
         if (j < 0) continue;
         if (!TRIV) {
@@ -341,14 +335,7 @@ ED_HD double torus_row_c(uint32_t s, int nfam, const TFam* fam, const TEntry* en
         }
         int code;
         uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
-        uint32_t r = 0;
-        int c = 0;
-#pragma unroll
-        for (int k = 0; k < (TC::N + 7) / 8; k++) {
-            unsigned b = (rep >> (8 * k)) & 255u;
-            r += rtab[(R.off[k] + c - R.cmin[k]) * 256 + b];
-            c += popc32(b);
-        }
+        uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
         int64_t j = dir_lookup(r, dir);
         if (j < 0) continue;
         if (!TRIV) {
@@ -422,14 +409,7 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
                 }
                 int code;
                 uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
-                uint32_t r = 0;
-                int c = 0;
-#pragma unroll
-                for (int k = 0; k < (TC::N + 7) / 8; k++) {
-                    unsigned b = (rep >> (8 * k)) & 255u;
-                    r += rtab[(R.off[k] + c - R.cmin[k]) * 256 + b];
-                    c += popc32(b);
-                }
+                uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
                 if (!TRIV) {
                     cd2 ph = phase[code];
                     double nr = cr * ph.re - ci * ph.im, ni = cr * ph.im + ci * ph.re;

@@ -91,14 +91,23 @@ void ed_basis_destroy(ed_basis b);
 /* Operator = basis + terms, terms uploaded once. */
 int ed_op_create(ed_basis b, const ed_terms* terms, ed_op* out);
 void ed_op_destroy(ed_op op);
-/* Kernel selection for ed_hv (no reference counterpart; used by the
- * autotuner and by tests that compare the paths): path 0 = automatic,
- * 1 = generic translation walk, 2 = rotation fast path (lattices whose
- * translations are bit rotations, N <= 32, no bowtie terms), with the lattice
- * shape compiled in when it is one of the built-in shapes, 3 = rotation fast
- * path with the shape as run-time parameters; block = threads
- * per block (64..512, multiple of 64; 0 keeps the current value). */
+/* Kernel selection for ed_hv (no reference counterpart; used by tests that
+ * compare the paths and by benchmarks): path 0 = automatic (rotation fast path
+ * when available; for the built-in lattice shapes the first H*v of each
+ * precision on a sector of at least 2^20 rows times the kernel variants and
+ * block sizes and keeps the fastest -- all give identical results),
+ * 1 = generic translation walk, 2 = rotation fast path with the lattice shape
+ * compiled in (variant K=4 elements in flight), 3 = rotation fast path with
+ * the shape as run-time parameters, 4 = as 2 with K=2 at 8 waves per SIMD.
+ * Paths 2-4 need a lattice whose translations are bit rotations, N <= 32 and
+ * no bowtie terms; 2 and 4 also a built-in shape.  block = threads per block
+ * (64..512, multiple of 64; 0 = default, tuned under path 0).  Resets any
+ * earlier tuning. */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
+/* The configuration the next ed_hv of precision dtype runs: path (1, 2 or 3
+ * as above), variant (0 = K=4, 1 = K=2 at 8 waves, -1 = not yet tuned or
+ * not applicable) and threads per block. */
+int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
  * total number of off-diagonal elements (one counting pass on the GPU). */
 int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total);

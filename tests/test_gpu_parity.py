@@ -194,7 +194,38 @@ def test_gpu_rotation_fast_path_vs_generic(gpu_ready, name, model, n, kx, ky):
     rng = np.random.default_rng(12)
     v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
     wo = o.hv(v)
-    for path in (2, 1):
+    out = {}
+    for path in (2, 4, 3, 1):
         op.tune(path)
-        assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo), path
+        out[path] = _hv(op, v, torch.complex128)
+        assert np.linalg.norm(out[path] - wo) <= 1e-12 * np.linalg.norm(wo), path
         assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo), path
+    # the two compiled variants accumulate in the same order
+    assert np.array_equal(out[2], out[4])
+
+
+def test_gpu_autotune_picks_identical_result(gpu_ready):
+    """On a sector above the tuning threshold (30-site triangular torus,
+    D ~ 5.2e6) the automatic path times the variants on the first H*v; its
+    result is bit-identical to each fixed variant, and the choice is reported."""
+    import torch
+    ed = _ed()
+    m = L.triangular(6, 5, Ch=0.1)
+    print("[case] tri6x5 autotune", flush=True)
+    basis = ed.Basis(ed.Lattice.of(m), 15, 0, 0)
+    assert basis.D > 1 << 20
+    op = ed.Operator(basis, m)
+    assert op.tuning("c64")[1] == -1
+    g = torch.Generator(device="cuda").manual_seed(3)
+    v = torch.randn(basis.D, dtype=torch.complex64, device="cuda", generator=g)
+    w_auto = op.hv(v)
+    path, var, block = op.tuning("c64")
+    assert path == 2 and var in (0, 1) and block in (256, 512)
+    assert op.tuning("c128")[1] == -1          # tuned per precision
+    for p in (2, 4):
+        for b in (256, 512):
+            op.tune(p, b)
+            assert torch.equal(op.hv(v), w_auto), (p, b)
+    op.tune(0, 256)                            # fixed block: only the variant is tuned
+    assert torch.equal(op.hv(v), w_auto)
+    assert op.tuning("c64")[2] == 256
