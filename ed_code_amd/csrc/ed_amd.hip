@@ -533,9 +533,9 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 4) return ed_fail(ED_EINVAL, "path %d not in {0,1,2,3,4}", path);
+    if (path < 0 || path > 5) return ed_fail(ED_EINVAL, "path %d not in 0..5", path);
     if (path >= 2 && !op->use_torus) return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
-    if ((path == 2 || path == 4) && !torus_shape_compiled(op->torus))
+    if ((path == 2 || path == 4 || path == 5) && !torus_shape_compiled(op->torus))
         return ed_fail(ED_EINVAL, "lattice shape is not one of the compiled shapes");
     if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
     op->tune_path = path;

@@ -92,7 +92,7 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     double nrm = ntab[(int)(rw >> kLidShift)];
     double ar = 0.0, ai = 0.0;
     double ez = torus_row_batched<TC, TRIV, K>(
-        s, nfam, fam, ent, ezz0, R, rtab, dir, phase, [&](int64_t j) { return load_c(u, j); },
+        s, nfam, fam, ent, ezz0, R, rtab, dir, phase, i, [&](int64_t j) { return load_c(u, j); },
         [&](const cd2& x, double cr, double ci) {
             ar += cr * x.re - ci * x.im;
             ai += cr * x.im + ci * x.re;
@@ -115,6 +115,12 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
 template <class TC, bool TRIV, typename VT>
 __global__ void __launch_bounds__(512) k_hv_torus_c4(ED_TORUS_C_ARGS) {
     hv_torus_c_body<TC, TRIV, 4, VT>(ED_TORUS_C_PASS);
+}
+
+// Variant C: eight elements in flight.
+template <class TC, bool TRIV, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_c8(ED_TORUS_C_ARGS) {
+    hv_torus_c_body<TC, TRIV, 8, VT>(ED_TORUS_C_PASS);
 }
 
 // Variant B: two elements in flight, registers capped for 8 waves per SIMD.
@@ -147,9 +153,12 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
     if (var == 0) {
         if (B->trivial_k) ED_GO((k_hv_torus_c4<TC, true, VT>));
         else ED_GO((k_hv_torus_c4<TC, false, VT>));
-    } else {
+    } else if (var == 1) {
         if (B->trivial_k) ED_GO((k_hv_torus_c2w8<TC, true, VT>));
         else ED_GO((k_hv_torus_c2w8<TC, false, VT>));
+    } else {
+        if (B->trivial_k) ED_GO((k_hv_torus_c8<TC, true, VT>));
+        else ED_GO((k_hv_torus_c8<TC, false, VT>));
     }
 #undef ED_GO
 }
@@ -199,10 +208,10 @@ static constexpr int dtype_slot() {
 }
 
 // Candidate (variant, block) pairs of the compiled-shape kernel.  Which one is
-// fastest depends on the register pressure of the shape and on the vector
-// precision (measured: K=4 at 512 threads for complex64 on the 32-site torus,
-// K=4 at 256 for complex128, K=2 at 8 waves close behind).
-static const int kTuneCfg[4][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}};
+// fastest depends on the register pressure of the shape, the vector precision
+// and the momentum (measured on the 32-site torus: K=4 at 256 threads for
+// complex64 at k=0, K=8 at 256 for complex128, K=2 at 8 waves for k != 0).
+static const int kTuneCfg[6][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256}};
 static const int64_t kTuneMinRows = int64_t(1) << 20;
 
 // Chooses the configuration of the compiled-shape kernel for this operator
@@ -263,6 +272,7 @@ void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     if (!compiled) return;
     if (op->tune_path == 2) var = 0;
     else if (op->tune_path == 4) var = 1;
+    else if (op->tune_path == 5) var = 2;
     else if (op->sel_var[dslot] >= 0) {
         var = op->sel_var[dslot];
         bs = op->sel_block[dslot];

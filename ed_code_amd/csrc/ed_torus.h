@@ -361,22 +361,27 @@ ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) {
     const uint32_t s0 = wsel == 0 ? 0xffffffffu : 0u, s1 = wsel == 1 ? 0xffffffffu : 0u,
                    s2 = wsel == 2 ? 0xffffffffu : 0u;
     const uint32_t bits = (e.b0 & s0) | (e.b1 & s1) | (e.b2 & s2);
-    if (!((bits >> bit) & 1u)) return -1;
     // ones below position o: whole words before wsel, the low bits of word wsel
     const uint32_t m0 = s0 ? low : 0xffffffffu;
     const uint32_t m1 = s0 ? 0u : (s1 ? low : 0xffffffffu);
     const uint32_t m2 = s2 ? low : 0u;
-    return (int64_t)(e.base + popc32(e.b0 & m0) + popc32(e.b1 & m1) + popc32(e.b2 & m2));
+    const int64_t idx = (int64_t)(e.base + popc32(e.b0 & m0) + popc32(e.b1 & m1) + popc32(e.b2 & m2));
+    return ((bits >> bit) & 1u) ? idx : int64_t(-1);
 }
 
 // torus_row_c with K elements in flight per lane: the orbit searches and
 // ranks of K elements are computed first, then their K directory loads and
 // K gathers are issued back to back, so that a lane waits for memory once
-// per K elements instead of twice per element.  g(j, cr, ci) as before.
+// per K elements instead of twice per element.  The loads are unconditional
+// (an empty slot reads directory entry 0 and the row's own element `self`):
+// a load under a lane mask would make the compiler wait for it inside the
+// masked block and serialise the K loads.  loadu(j) returns element j of u;
+// g(x, cr, ci) accumulates one element and is also called with cr = ci = 0
+// for empty slots (u[self] is non-finite only when w[self] is anyway).
 template <class TC, bool TRIV, int K, class LoadU, class Gf>
 ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
                                const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
-                               LoadU&& loadu, Gf&& g) {
+                               int64_t self, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
@@ -421,18 +426,22 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
         more = ok[K - 1];
         DirEntry e[K];
 #pragma unroll
-        for (int q = 0; q < K; q++)
-            if (ok[q]) e[q] = dir[rk[q] / kDirSpan];
+        for (int q = 0; q < K; q++) e[q] = dir[rk[q] / kDirSpan];
+        bool use[K];
         int64_t j[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) j[q] = ok[q] ? dir_decode(e[q], rk[q]) : -1;
+        for (int q = 0; q < K; q++) {
+            j[q] = dir_decode(e[q], rk[q]);
+            use[q] = ok[q] && j[q] >= 0;
+        }
         cd2 x[K];
 #pragma unroll
-        for (int q = 0; q < K; q++)
-            if (j[q] >= 0) x[q] = loadu(j[q]);
+        for (int q = 0; q < K; q++) x[q] = loadu(use[q] ? j[q] : self);
+        // Empty slots multiply the row's own element by a zero coefficient,
+        // so every loaded value is used on every lane and the compiler keeps
+        // the loads out of masked blocks.
 #pragma unroll
-        for (int q = 0; q < K; q++)
-            if (j[q] >= 0) g(x[q], cre[q], cim[q]);
+        for (int q = 0; q < K; q++) g(x[q], use[q] ? cre[q] : 0.0, use[q] ? cim[q] : 0.0);
     }
     return ezz;
 }

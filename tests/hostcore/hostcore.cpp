@@ -115,7 +115,7 @@ void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::
         double nrm = h->nt[h->rows[i] >> kLidShift];
         double ar = 0, ai = 0;
         double ez = torus_row_batched<TC, TRIV, 4>(
-            s, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(), h->dir.data(), h->ph.data(),
+            s, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(), h->dir.data(), h->ph.data(), i,
             [&](int64_t j) { return u[j]; },
             [&](const cd2& x, double cr, double ci) {
                 ar += cr * x.re - ci * x.im;
