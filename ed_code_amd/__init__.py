@@ -24,7 +24,7 @@ from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_REFERENCE, EDError, Lanczo
                    check, lib)
 from .lattices import Model, n_ones_for_sz
 
-__all__ = ["Lattice", "Basis", "Operator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
+__all__ = ["Lattice", "Basis", "Operator", "BlockOperator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
            "ED_LZ_FULL", "ED_LZ_REFERENCE", "n_ones_for_sz"]
 
 _initialised = False
@@ -207,6 +207,66 @@ class Operator:
     def __del__(self):
         if getattr(self, "h", None):
             lib().ed_op_destroy(self.h)
+            self.h = None
+
+
+class BlockOperator:
+    """M coupling instances (models with the same bond / triangle lists,
+    different couplings) on one sector: W[:, m] = H_m V[:, m] in one pass.
+    Vectors are (D, M) row-major torch tensors on the GPU (element (i, m) at
+    i*M + m)."""
+
+    def __init__(self, basis: Basis, models):
+        self.basis = basis
+        self.models = list(models)
+        self.M = len(self.models)
+        ts, self._keep = [], []
+        for m in self.models:
+            t, keep = _terms(m)
+            ts.append(t)
+            self._keep.append(keep)
+        arr = (Terms * self.M)(*ts)
+        h = C.c_void_p()
+        check(lib().ed_block_create(basis.h, self.M, arr, C.byref(h)))
+        self.h = h
+        self.D = basis.D
+
+    @property
+    def fast(self) -> bool:
+        return bool(lib().ed_block_fast(self.h))
+
+    def hv(self, V, W=None, stream=None):
+        import torch
+        if V.dtype == torch.complex128:
+            dt = ED_C128
+        elif V.dtype == torch.complex64:
+            dt = ED_C64
+        else:
+            raise TypeError(f"unsupported dtype {V.dtype}")
+        if not V.is_cuda or not V.is_contiguous() or tuple(V.shape) != (self.D, self.M):
+            raise ValueError("V must be a contiguous (D, M) device tensor")
+        if W is None:
+            W = torch.empty_like(V)
+        if W.dtype != V.dtype or not W.is_contiguous() or W.shape != V.shape:
+            raise ValueError("W must match V")
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().ed_block_hv(self.h, dt, C.c_void_p(V.data_ptr()), C.c_void_p(W.data_ptr()), C.c_void_p(st)))
+        return W
+
+    def lanczos(self, nev=1, iterations=200, tol=1e-10, seed=1234, precision=ED_C64):
+        """Lowest nev eigenvalues of every instance: array (M, nev), stats."""
+        p = LanczosParams(nev, iterations, 1, precision, ED_LZ_FULL, 0, tol, seed)
+        n = min(nev, self.D)
+        ev = np.zeros((self.M, n))
+        st = LanczosStats()
+        check(lib().ed_block_lanczos(self.h, C.byref(p), ev.ctypes.data, C.byref(st)))
+        stats = dict(hv_calls=st.hv_calls, hv_seconds=st.hv_seconds, total_seconds=st.total_seconds,
+                     residual=list(st.residual[:min(self.M, 16)]), rayleigh=list(st.rayleigh[:min(self.M, 16)]))
+        return ev, stats
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ed_block_destroy(self.h)
             self.h = None
 
 

@@ -66,6 +66,11 @@ SIGNATURES = {
     "ed_make_h": (C.c_int, [_VP, _VP]),
     "ed_dense_eigh": (C.c_int, [_VP, _VP, _VP]),
     "ed_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
+    "ed_block_create": (C.c_int, [_VP, C.c_int32, _VP, C.POINTER(_VP)]),
+    "ed_block_destroy": (None, [_VP]),
+    "ed_block_fast": (C.c_int, [_VP]),
+    "ed_block_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
+    "ed_block_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, C.POINTER(LanczosStats)]),
 }
 
 _lib = None

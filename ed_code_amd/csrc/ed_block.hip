@@ -1,0 +1,430 @@
+// ed_block.hip -- block H*V: M coupling instances on one momentum sector,
+// W[:, m] = H_m V[:, m], in one pass over the rows.
+//
+// The instances share the term structure (the same bonds and triangles, in
+// the same order; their couplings, anisotropy, chirality and phases may
+// differ), so every gathered element -- flip, orbit minimum, rank, directory
+// entry -- is found once and serves all M instances: the integer work and
+// the directory traffic of a single H*v are paid once per block, the gather
+// reads M adjacent values.  Vectors are interleaved: element (i, m) at
+// i*M + m, i.e. a row-major D x M matrix, so that one target's M values are
+// one contiguous load.
+//
+// Lattices without the compiled rotation fast path (or with bowtie terms)
+// run each instance through its own operator on de-interleaved copies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ed_amd.h"
+#include "ed_core.h"
+#include "ed_internal.h"
+#include "ed_torus.h"
+#include "ed_torus_host.h"
+
+using namespace edc;
+
+namespace {
+
+struct BEnt {            // per term: flip partner y, chiral z-site k (-1 for a bond)
+    int32_t y, k;
+};
+struct BCoef {           // per (term, instance)
+    double cre, cim, dJ2;
+};
+
+inline unsigned nblk(int64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename VT>
+__global__ void k_prescale_block(int64_t D, int M, const uint64_t* rows, const double* nitab, const VT* v, VT* u) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= D * M) return;
+    double f = nitab[(int)(rows[e / M] >> kLidShift)];
+    VT x = v[e];
+    x.re = x.re * f;
+    x.im = x.im * f;
+    u[e] = x;
+}
+
+template <typename VT>
+__global__ void k_deinterleave(int64_t D, int M, int m, const VT* V, VT* x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < D) x[i] = V[i * M + m];
+}
+
+template <typename VT>
+__global__ void k_interleave(int64_t D, int M, int m, const VT* x, VT* V) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < D) V[i * M + m] = x[i];
+}
+
+__device__ __forceinline__ void st_c(cd2* p, double re, double im) { *p = cd2{re, im}; }
+__device__ __forceinline__ void st_c(cf2* p, double re, double im) { *p = cf2{(float)re, (float)im}; }
+
+// One lane per row; K elements in flight; M instances accumulated per element.
+// Same element order, coefficient expressions and accumulation order as the
+// single-instance rotation kernel (torus_row_batched).  Terms are numbered
+// densely: term = foff[f] + (number of sites of family f below x).
+template <class TC, bool TRIV, int M, int K, typename VT>
+__global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+                                                    const TFam* __restrict__ gfam, const int* __restrict__ gfoff,
+                                                    int nterm, const BEnt* __restrict__ gent,
+                                                    const BCoef* __restrict__ gcoef, const double* __restrict__ gezz0,
+                                                    RankInfo R, const uint32_t* __restrict__ grtab, int rtab_n,
+                                                    const DirEntry* __restrict__ dir, const cd2* __restrict__ gphase,
+                                                    int nphase, const double* __restrict__ ntab,
+                                                    const VT* __restrict__ u, const VT* __restrict__ v,
+                                                    VT* __restrict__ w) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cd2* phase = (cd2*)smem;                                   // nphase
+    BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M
+    TFam* fam = (TFam*)(coef + nterm * M);                     // nfam
+    int* foff = (int*)(fam + nfam);                            // nfam (+1 pad)
+    BEnt* ent = (BEnt*)(foff + nfam + (nfam & 1));             // nterm
+    double* ezz0 = (double*)(ent + nterm);                     // M
+    uint32_t* rtab = (uint32_t*)(ezz0 + M);                    // rtab_n
+    for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
+    for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
+    for (int q = threadIdx.x; q < nfam; q += blockDim.x) { fam[q] = gfam[q]; foff[q] = gfoff[q]; }
+    for (int q = threadIdx.x; q < nterm; q += blockDim.x) ent[q] = gent[q];
+    for (int q = threadIdx.x; q < M; q += blockDim.x) ezz0[q] = gezz0[q];
+    for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
+    __syncthreads();
+    // XCD-contiguous row ranges (see the single-instance kernel)
+    const unsigned nb = gridDim.x, qq = nb / 8, rem = nb % 8;
+    const unsigned xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const unsigned lb = xcd * qq + (xcd < rem ? xcd : rem) + slot;
+    const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t rw = rows[i];
+    const uint32_t s = (uint32_t)(rw & kStateMask);
+    const double nrm = ntab[(int)(rw >> kLidShift)];
+    double ar[M], ai[M], ez[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) { ar[m] = 0.0; ai[m] = 0.0; ez[m] = ezz0[m]; }
+    int f = 0;
+    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    bool more = true;
+    while (more) {
+        int ti[K], xs[K], code[K];
+        uint32_t rk[K];
+        bool ok[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            while (cur == 0u && f < nfam) {
+                if (++f < nfam) cur = fam_mask_c<TC>(s, fam[f]);
+            }
+            ok[q] = f < nfam;
+            ti[q] = 0; xs[q] = 0; rk[q] = 0; code[q] = 0;
+            if (ok[q]) {
+                const int x = ctz32(cur);
+                cur &= cur - 1u;
+                xs[q] = x;
+                ti[q] = foff[f] + popc32(fam[f].S & ((1u << x) - 1u));
+                const uint32_t t = s ^ (1u << x) ^ (1u << ent[ti[q]].y);
+                int cd;
+                const uint32_t rep = torus_rep_c<TC, !TRIV>(t, cd);
+                code[q] = cd;
+                rk[q] = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
+            }
+        }
+        more = ok[K - 1];
+        DirEntry e[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) e[q] = dir[rk[q] / kDirSpan];
+        bool use[K];
+        int64_t j[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            j[q] = dir_decode(e[q], rk[q]);
+            use[q] = ok[q] && j[q] >= 0;
+        }
+        // unconditional loads (an empty slot reads the row's own values)
+        VT xv[K][M];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const VT* p = u + (use[q] ? j[q] : i) * M;
+#pragma unroll
+            for (int m = 0; m < M; m++) xv[q][m] = p[m];
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const BEnt E = ent[ti[q]];
+            const bool bond = E.k < 0;
+            const bool sx = (s >> xs[q]) & 1u, sy = (s >> E.y) & 1u, sk = (s >> (E.k & 31)) & 1u;
+            cd2 ph{1.0, 0.0};
+            if (!TRIV) ph = phase[code[q]];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const BCoef C = coef[ti[q] * M + m];
+                double cr, ci;
+                if (bond) {
+                    if (ok[q]) ez[m] -= C.dJ2;
+                    cr = C.cre;
+                    ci = sx ? -C.cim : C.cim;
+                } else {
+                    const double sg = sk ? 1.0 : -1.0;
+                    cr = -C.cim * sg;
+                    ci = (sy ? C.cre : -C.cre) * sg;
+                }
+                if (!TRIV) {
+                    const double nr = cr * ph.re - ci * ph.im, ni = cr * ph.im + ci * ph.re;
+                    cr = nr;
+                    ci = ni;
+                }
+                if (!use[q]) { cr = 0.0; ci = 0.0; }
+                const cd2 xx = load_c(&xv[q][m], 0);
+                ar[m] += cr * xx.re - ci * xx.im;
+                ai[m] += cr * xx.im + ci * xx.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const cd2 vi = load_c(v, i * M + m);
+        st_c(w + i * M + m, nrm * ar[m] + ez[m] * vi.re, nrm * ai[m] + ez[m] * vi.im);
+    }
+}
+
+}  // namespace
+
+struct ed_block_s {
+    ed_basis_s* b = nullptr;
+    int M = 0;
+    std::vector<ed_op_s*> ops;      // one operator per instance (fallback path)
+    bool fast = false;              // compiled rotation block kernel applies
+    int nfam = 0, nterm = 0;
+    TFam* d_fam = nullptr;
+    int* d_foff = nullptr;
+    BEnt* d_ent = nullptr;
+    BCoef* d_coef = nullptr;
+    double* d_ezz0 = nullptr;
+    void* d_u64 = nullptr;          // prescaled interleaved gather stream, complex64 [D*M]
+    void* d_u128 = nullptr;         // complex128 [D*M]
+    void* d_tmp = nullptr;          // fallback: two de-interleaved vectors, complex128-sized
+};
+
+namespace {
+
+// Bond and chiral lists of instance m over the union of the instances' terms:
+// a term that is nonzero in any instance is kept in every instance (with its
+// own, possibly zero, coupling), so that all instances produce the same
+// families; terms zero in all instances are dropped as ed_op_create drops them.
+void union_lists(const ed_terms* T, int M, int m, std::vector<Bond>& bonds, std::vector<Chiral>& ch) {
+    const ed_terms* t = &T[m];
+    bonds.clear();
+    ch.clear();
+    for (int k = 0; k < t->nbonds; k++) {
+        bool any = false;
+        for (int q = 0; q < M; q++) any |= std::fabs(T[q].bond_J[k]) >= 1e-16;
+        if (any)
+            bonds.push_back(Bond{t->bond_xy[2 * k], t->bond_xy[2 * k + 1], t->bond_J[k], t->bond_phase[2 * k],
+                                 t->bond_phase[2 * k + 1]});
+    }
+    bool anych = false;
+    for (int q = 0; q < M; q++) anych |= std::fabs(T[q].Ch) > 1e-16;
+    if (anych)
+        for (int k = 0; k < t->nchiral; k++)
+            ch.push_back(Chiral{t->chiral_ijk[3 * k], t->chiral_ijk[3 * k + 1], t->chiral_ijk[3 * k + 2], 0,
+                                t->chiral_phase[2 * k], t->chiral_phase[2 * k + 1]});
+}
+
+size_t block_smem(const ed_block_s* K) {
+    const ed_basis_s* B = K->b;
+    const int nf = K->nfam;
+    return (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
+           (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)K->nterm * sizeof(BEnt) + (size_t)K->M * sizeof(double) +
+           (size_t)B->rtab32_n * sizeof(uint32_t);
+}
+
+template <class TC, int M, typename VT>
+void launch_block_c(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t st) {
+    ed_basis_s* B = K->b;
+    constexpr int KB = M >= 4 ? 2 : 4;
+    const unsigned bs = 256;
+    dim3 grid(nblk(B->D, bs)), blk(bs);
+    size_t sm = block_smem(K);
+    int nph = B->L1 * B->L2;
+    if (B->trivial_k)
+        hipLaunchKernelGGL((k_hv_block_c<TC, true, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam, K->d_fam,
+                           K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,
+                           B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
+    else
+        hipLaunchKernelGGL((k_hv_block_c<TC, false, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam,
+                           K->d_fam, K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_ezz0, B->rinfo, B->d_rtab32,
+                           B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
+}
+
+template <int M, typename VT>
+bool dispatch_shape(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t st) {
+    const Torus& T = K->b->torus;
+    int L2 = T.kind == 2 ? 1 : T.L2;
+#define ED_BLAUNCH(n, wd, l2)                                       \
+    if (T.N == n && T.w == wd && L2 == l2) {                        \
+        launch_block_c<TorusC<n, wd, l2>, M, VT>(K, u, v, w, st);   \
+        return true;                                                \
+    }
+    ED_TORUS_SHAPES(ED_BLAUNCH)
+#undef ED_BLAUNCH
+    return false;
+}
+
+template <typename VT>
+int block_hv(ed_block_s* K, const VT* v, VT* w, hipStream_t st) {
+    ed_basis_s* B = K->b;
+    const int64_t D = B->D, M = K->M;
+    if (D == 0) return ED_OK;
+    const int dt = std::is_same<VT, cd2>::value ? ED_C128 : ED_C64;
+    if (M == 1) return op_hv(K->ops[0], dt, v, w, st);
+    if (!K->fast) {
+        if (!K->d_tmp) HIPCHK(hipMalloc(&K->d_tmp, 2 * D * sizeof(cd2)));
+        VT* x = (VT*)K->d_tmp;
+        VT* y = x + D;
+        for (int m = 0; m < M; m++) {
+            hipLaunchKernelGGL(k_deinterleave<VT>, dim3(nblk(D, 256)), dim3(256), 0, st, D, (int)M, m, v, x);
+            HIPCHK(hipGetLastError());
+            int rc = op_hv(K->ops[m], dt, x, y, st);
+            if (rc != ED_OK) return rc;
+            hipLaunchKernelGGL(k_interleave<VT>, dim3(nblk(D, 256)), dim3(256), 0, st, D, (int)M, m, y, w);
+            HIPCHK(hipGetLastError());
+        }
+        return ED_OK;
+    }
+    VT*& u = std::is_same<VT, cd2>::value ? (VT*&)K->d_u128 : (VT*&)K->d_u64;
+    if (!u) HIPCHK(hipMalloc((void**)&u, D * M * sizeof(VT)));
+    hipLaunchKernelGGL(k_prescale_block<VT>, dim3(nblk(D * M, 256)), dim3(256), 0, st, D, (int)M, B->d_rows,
+                       B->d_nitab, v, u);
+    HIPCHK(hipGetLastError());
+    bool done = false;
+    switch (M) {
+        case 2: done = dispatch_shape<2, VT>(K, u, v, w, st); break;
+        case 4: done = dispatch_shape<4, VT>(K, u, v, w, st); break;
+        case 8: done = dispatch_shape<8, VT>(K, u, v, w, st); break;
+        default: break;
+    }
+    if (!done) return ed_fail(ED_EINVAL, "no block kernel for this shape and M=%d", (int)M);
+    HIPCHK(hipGetLastError());
+    return ED_OK;
+}
+
+}  // namespace
+
+int block_op_hv(ed_block_s* K, int dtype, const void* v, void* w, hipStream_t st) {
+    if (dtype == ED_C128) return block_hv<cd2>(K, (const cd2*)v, (cd2*)w, st);
+    if (dtype == ED_C64) return block_hv<cf2>(K, (const cf2*)v, (cf2*)w, st);
+    return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+}
+
+ed_basis_s* block_basis(const ed_block_s* K) { return K->b; }
+int block_size(const ed_block_s* K) { return K->M; }
+
+extern "C" void ed_block_destroy(ed_block blk) {
+    if (!blk) return;
+    for (auto* op : blk->ops) ed_op_destroy(op);
+    for (void* p : {(void*)blk->d_fam, (void*)blk->d_foff, (void*)blk->d_ent, (void*)blk->d_coef, (void*)blk->d_ezz0, blk->d_u64,
+                    blk->d_u128, blk->d_tmp})
+        if (p) dfree(p);
+    delete blk;
+}
+
+extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_block* out) {
+    if (!b || !terms || !out) return ed_fail(ED_EINVAL, "null argument");
+    if (M < 1 || M > 8) return ed_fail(ED_EINVAL, "M=%d not in 1..8", M);
+    const ed_terms& t0 = terms[0];
+    for (int m = 1; m < M; m++) {
+        const ed_terms& t = terms[m];
+        if (t.nbonds != t0.nbonds || t.nchiral != t0.nchiral || t.nbowtie != t0.nbowtie)
+            return ed_fail(ED_EINVAL, "instance %d: term counts differ from instance 0", m);
+        if (t.nbonds && std::memcmp(t.bond_xy, t0.bond_xy, 2 * sizeof(int32_t) * t.nbonds))
+            return ed_fail(ED_EINVAL, "instance %d: bond sites differ from instance 0", m);
+        if (t.nchiral && std::memcmp(t.chiral_ijk, t0.chiral_ijk, 3 * sizeof(int32_t) * t.nchiral))
+            return ed_fail(ED_EINVAL, "instance %d: chiral sites differ from instance 0", m);
+        if (t.nbowtie && std::memcmp(t.bowtie, t0.bowtie, 6 * sizeof(int32_t) * t.nbowtie))
+            return ed_fail(ED_EINVAL, "instance %d: bowtie sites differ from instance 0", m);
+    }
+    auto* K = new ed_block_s();
+    K->b = b;
+    K->M = M;
+    for (int m = 0; m < M; m++) {
+        ed_op op = nullptr;
+        int rc = ed_op_create(b, &terms[m], &op);
+        if (rc != ED_OK) {
+            ed_block_destroy(K);
+            return rc;
+        }
+        K->ops.push_back(op);
+    }
+    // compiled rotation block kernel: rotational lattice with a compiled
+    // shape, no bowtie terms in any instance, M in {2, 4, 8}
+    bool nobow = true;
+    for (int m = 0; m < M; m++)
+        if (terms[m].nbowtie && std::fabs(terms[m].ChCh) > 1e-16) nobow = false;
+    if (b->has_torus && b->N <= 32 && b->d_rtab32 && nobow && (M == 2 || M == 4 || M == 8) &&
+        torus_shape_compiled(b->torus)) {
+        std::vector<std::vector<TFam>> fams(M);
+        std::vector<std::vector<TEntry>> ents(M);
+        std::vector<double> ezz0(M);
+        bool ok = true;
+        for (int m = 0; m < M && ok; m++) {
+            std::vector<Bond> bonds;
+            std::vector<Chiral> ch;
+            union_lists(terms, M, m, bonds, ch);
+            ok = torus_families(terms[m].lambda, terms[m].Ch, b->torus, bonds, ch, fams[m], ents[m], ezz0[m]) &&
+                 !fams[m].empty();
+        }
+        for (int m = 1; m < M && ok; m++) {
+            ok = fams[m].size() == fams[0].size();
+            for (size_t f = 0; ok && f < fams[0].size(); f++)
+                ok = fams[m][f].type == fams[0][f].type && fams[m][f].a == fams[0][f].a &&
+                     fams[m][f].q == fams[0][f].q && fams[m][f].S == fams[0][f].S;
+        }
+        if (ok) {
+            const int nf = (int)fams[0].size();
+            std::vector<int> foff(nf);
+            std::vector<BEnt> be;
+            std::vector<BCoef> bc;
+            for (int f = 0; f < nf; f++) {
+                foff[f] = (int)be.size();
+                for (int x = 0; x < 32; x++) {
+                    if (!((fams[0][f].S >> x) & 1u)) continue;
+                    const TEntry& E0 = ents[0][f * 32 + x];
+                    be.push_back(BEnt{E0.y, fams[0][f].type == 0 ? -1 : E0.k});
+                    for (int m = 0; m < M; m++) {
+                        const TEntry& E = ents[m][f * 32 + x];
+                        bc.push_back(BCoef{E.cre, E.cim, E.dJ2});
+                    }
+                }
+            }
+            K->nfam = nf;
+            K->nterm = (int)be.size();
+            auto up = [&](void** d, const void* h, size_t n) -> int {
+                HIPCHK(hipMalloc(d, n));
+                HIPCHK(hipMemcpy(*d, h, n, hipMemcpyHostToDevice));
+                return ED_OK;
+            };
+            int rc = up((void**)&K->d_fam, fams[0].data(), nf * sizeof(TFam));
+            if (rc == ED_OK) rc = up((void**)&K->d_foff, foff.data(), nf * sizeof(int));
+            if (rc == ED_OK) rc = up((void**)&K->d_ent, be.data(), be.size() * sizeof(BEnt));
+            if (rc == ED_OK) rc = up((void**)&K->d_coef, bc.data(), bc.size() * sizeof(BCoef));
+            if (rc == ED_OK) rc = up((void**)&K->d_ezz0, ezz0.data(), M * sizeof(double));
+            if (rc != ED_OK) {
+                ed_block_destroy(K);
+                return rc;
+            }
+            K->fast = block_smem(K) <= 64 * 1024;
+        }
+    }
+    *out = K;
+    return ED_OK;
+}
+
+extern "C" int ed_block_hv(ed_block blk, int32_t dtype, const void* V, void* W, void* stream) {
+    if (!blk || ((!V || !W) && blk->b->D)) return ed_fail(ED_EINVAL, "null argument");
+    if (V == W && blk->b->D) return ed_fail(ED_EINVAL, "V and W must not alias");
+    return block_op_hv(blk, dtype, V, W, (hipStream_t)stream);
+}
+
+extern "C" int ed_block_fast(ed_block blk) { return blk && blk->fast ? 1 : 0; }

@@ -129,18 +129,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) k
     hv_torus_c_body<TC, TRIV, 2, VT>(ED_TORUS_C_PASS);
 }
 
-// Shapes compiled with constant geometry: the benchmark lattices and the
-// small lattices the parity tests use.  Others run k_hv_torus.
-#define ED_TORUS_SHAPES(X) \
-    X(32, 4, 8)            \
-    X(30, 6, 5)            \
-    X(27, 3, 9)            \
-    X(24, 24, 1)           \
-    X(16, 4, 4)            \
-    X(20, 4, 5)            \
-    X(18, 6, 3)            \
-    X(16, 16, 1)
-
 template <class TC, typename VT>
 static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, size_t sm, int var, int bs) {
     ed_basis_s* B = op->b;

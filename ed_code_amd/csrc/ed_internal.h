@@ -90,3 +90,6 @@ template <typename VT>
 int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs);
 bool torus_shape_compiled(const edc::Torus& T);
+int block_op_hv(ed_block_s* K, int dtype, const void* v, void* w, hipStream_t st);
+ed_basis_s* block_basis(const ed_block_s* K);
+int block_size(const ed_block_s* K);

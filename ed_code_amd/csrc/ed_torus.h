@@ -447,3 +447,17 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
 }
 
 }  // namespace edc
+
+// Shapes compiled with constant geometry (N, row width, rows): the benchmark
+// lattices and the small lattices the parity tests use.  Others run the
+// kernels that take the shape at run time.
+#define ED_TORUS_SHAPES(X) \
+    X(32, 4, 8)            \
+    X(30, 6, 5)            \
+    X(27, 3, 9)            \
+    X(24, 24, 1)           \
+    X(16, 4, 4)            \
+    X(20, 4, 5)            \
+    X(18, 6, 3)            \
+    X(16, 16, 1)
+

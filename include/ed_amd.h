@@ -100,9 +100,9 @@ void ed_op_destroy(ed_op op);
  * compiled in (variant K=4 elements in flight), 3 = rotation fast path with
  * the shape as run-time parameters, 4 = as 2 with K=2 at 8 waves per SIMD,
  * 5 = as 2 with K=8.  Paths 2-5 need a lattice whose translations are bit
- * rotations, N <= 32 and no bowtie terms; 2, 4 and 5 also a built-in shape.  block = threads per block
- * (64..512, multiple of 64; 0 = default, tuned under path 0).  Resets any
- * earlier tuning. */
+ * rotations, N <= 32 and no bowtie terms; 2, 4 and 5 also a built-in shape.
+ * block = threads per block (64..512, multiple of 64; 0 = default, tuned
+ * under path 0).  Resets any earlier tuning. */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
 /* The configuration the next ed_hv of precision dtype runs: path (1, 2 or 3
  * as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, -1 = not yet
@@ -170,6 +170,40 @@ typedef struct ed_lanczos_stats {
 
 /* evals [nev]; evecs (host, complex128 [nev][D]) may be NULL; stats may be NULL. */
 int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evecs, ed_lanczos_stats* stats);
+
+/* ---- Block operators: M coupling instances on one sector ----------------
+ * The reference solves one coupling set per run (ED.cpp:118-128 ->
+ * sector_lanczos) and sweeps sectors one at a time (ARPACK_full_eigs.cpp:
+ * 169-277); a block holds M term sets on one basis and applies all M
+ * Hamiltonians in one pass over the rows, sharing the orbit search, rank and
+ * directory work of every gathered element.  The term sets must have the same
+ * bond, chiral and bowtie site lists (same order); J, Jz, Ch, ChCh and phases
+ * may differ per instance.  M in 1..8; M in {2, 4, 8} on a built-in torus
+ * shape without bowtie terms runs the shared kernel, other cases apply the
+ * instances one by one. */
+typedef struct ed_block_s* ed_block;
+int ed_block_create(ed_basis basis, int32_t M, const ed_terms* terms, ed_block* out);
+void ed_block_destroy(ed_block blk);
+/* 1 when the shared block kernel applies, 0 when instances run one by one. */
+int ed_block_fast(ed_block blk);
+/* W = [H_0 V_0, ..., H_{M-1} V_{M-1}] for interleaved DEVICE vectors:
+ * element (i, m) at index i*M + m (row-major D x M), complex of type dtype.
+ * V and W must not alias.  Asynchronous on `stream`. */
+int ed_block_hv(ed_block blk, int32_t dtype, const void* V, void* W, void* stream);
+/* Lowest p->nev eigenvalues of each instance (evals[m*nev + k], ascending),
+ * all instances advanced together: full-reorthogonalisation Lanczos, one
+ * eigenvalue at a time with deflation against the earlier Ritz vectors, at
+ * most p->iterations steps each; an instance stops when the residual
+ * estimate of its lowest Ritz pair is below p->tol * max(1, |E|).  Krylov
+ * vectors are stored in p->precision, alpha/beta and reorthogonalisation
+ * coefficients are FP64.  With complex64 storage the run stops at residual
+ * 1e-6 and, if p->tol is smaller, its Ritz vectors start a complex128 run
+ * that reaches p->tol; each eigenvalue is then the FP64 Rayleigh quotient of
+ * its Ritz vector.  M must be 1, 2, 4 or 8.  stats (optional): H*V calls and
+ * times, and the FP64 residual ||H x - E x|| and Rayleigh quotient of the
+ * lowest Ritz vector of each instance (residual[m], rayleigh[m]).
+ * p->mode and p->ncycles are ignored. */
+int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double* evals, ed_lanczos_stats* stats);
 
 #ifdef __cplusplus
 }
