@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("ED_AMD_LIB", os.path.join(HERE, "libed_amd.so"))
 ED_OK, ED_EINVAL, ED_ENOMEM, ED_EDEVICE, ED_EEMPTY, ED_ENOTCONV = range(6)
 ED_C64, ED_C128 = 0, 1
 ED_LZ_REFERENCE, ED_LZ_FULL = 0, 1
+ED_LZ_SELECTIVE = 1
 
 
 class EDError(RuntimeError):
@@ -33,7 +34,7 @@ class Terms(C.Structure):
 
 class LanczosParams(C.Structure):
     _fields_ = [("nev", C.c_int32), ("iterations", C.c_int32), ("ncycles", C.c_int32), ("precision", C.c_int32),
-                ("mode", C.c_int32), ("pad", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
+                ("mode", C.c_int32), ("flags", C.c_int32), ("tol", C.c_double), ("seed", C.c_uint64)]
 
 
 class LanczosStats(C.Structure):

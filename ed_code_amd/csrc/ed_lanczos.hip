@@ -430,7 +430,15 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
             } else {
                 RC((multiaxpy<VT, VT>(c, 1, vk, 0, cf, 1.0, w)));
             }
+            const bool sel = (p.flags & ED_LZ_SELECTIVE) != 0;
+            double nb2 = 0.0, bb = 0.0;
+            bool have_bb = false;
+            if (sel) RC(dnorm2(c, w, nb2));
             for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1 && sel) {
+                    RC(dnorm2(c, w, bb));
+                    if (bb > 0.5 * nb2) { have_bb = true; break; }
+                }
                 RC((multidot<VT, VT>(c, k + 1, V, D, w, q)));
                 if (pass == 0) alpha += q[k].real();
                 RC((multiaxpy<VT, VT>(c, k + 1, V, D, q, 1.0, w)));
@@ -440,8 +448,7 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
                 }
             }
             alphas.push_back(alpha);
-            double bb;
-            RC(dnorm2(c, w, bb));
+            if (!have_bb) RC(dnorm2(c, w, bb));
             double beta = std::sqrt(bb);
             bool stop = false;
             if ((k + 1) % 5 == 0 || k + 1 == mit) {
@@ -845,7 +852,18 @@ int run_block_full(BCtx& c, const ed_lanczos_params& p, double tol, const std::v
                 for (int m = 0; m < M; m++) cf[m] = alpha[m];
                 RC((baxpy<VT, VT>(c, 1, vk, 0, cf, nullptr, w)));
             }
+            // two Gram-Schmidt passes; the second only when the first removed
+            // more than 1 - 1/sqrt(2) of some instance's norm
+            std::vector<double> nb2, bb;
+            bool have_bb = false;
+            RC(bnorm2(c, w, nb2));
             for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1) {
+                    RC(bnorm2(c, w, bb));
+                    bool need = false;
+                    for (int m = 0; m < M; m++) need |= active[m] && bb[m] <= 0.5 * nb2[m];
+                    if (!need) { have_bb = true; break; }
+                }
                 RC((bdot<VT, VT>(c, k + 1, V, n, w, q)));
                 if (pass == 0)
                     for (int m = 0; m < M; m++) alpha[m] += q[(size_t)k * M + m].real();
@@ -856,8 +874,7 @@ int run_block_full(BCtx& c, const ed_lanczos_params& p, double tol, const std::v
                     RC((baxpy<cd2, VT>(c, 1, prev[e], 0, qe, nullptr, w)));
                 }
             }
-            std::vector<double> bb;
-            RC(bnorm2(c, w, bb));
+            if (!have_bb) RC(bnorm2(c, w, bb));
             bool any = false;
             std::vector<double> scale(M, 0.0);
             for (int m = 0; m < M; m++) {

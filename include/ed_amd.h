@@ -149,13 +149,18 @@ int ed_dense_eigh(ed_op op, double* evals, double* evecs);
  * rand(), lanczos.cpp:192); eigenvalue number m uses seed + m. */
 enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1 };
 
+/* flags: ED_LZ_SELECTIVE makes the second Gram-Schmidt pass of ED_LZ_FULL
+ * conditional: it runs only when the first pass removed more than
+ * 1 - 1/sqrt(2) of the vector's norm (Daniel-Gragg-Kaufman-Stewart). */
+enum ed_lanczos_flags { ED_LZ_SELECTIVE = 1 };
+
 typedef struct ed_lanczos_params {
     int32_t nev;
     int32_t iterations;
     int32_t ncycles;
     int32_t precision;   /* ed_dtype */
     int32_t mode;        /* ed_lanczos_mode */
-    int32_t pad;
+    int32_t flags;       /* ed_lanczos_flags */
     double tol;
     uint64_t seed;
 } ed_lanczos_params;

@@ -36,6 +36,23 @@ def test_lanczos_vs_dense(gpu_ready, prec, mode):
     assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-3)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_lanczos_selective_reorthogonalisation(gpu_ready, prec):
+    """Full mode with the conditional second Gram-Schmidt pass: same
+    eigenvalues as dense diagonalisation, fewer passes over the basis."""
+    ed = _ed()
+    model = L.triangular(4, 4, Ch=0.2, J2=0.3)
+    o = P.OracleBasis(model, 8, 1, 2)
+    ev = np.linalg.eigvalsh(o.make_h())
+    basis = ed.Basis(ed.Lattice.of(model), 8, 1, 2)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    lz, st = op.lanczos(nev=3, iterations=200, ncycles=1, tol=1e-13, seed=5, precision=pr, mode=ed.ED_LZ_FULL,
+                        selective=True)
+    assert np.abs(lz - ev[:3]).max() < 1e-9
+    assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-3)
+
+
 def test_lanczos_chain24_ground_state(gpu_ready):
     """chain24 Sz=0 k=0, 4 lowest deflated eigenvalues: c128 reference schedule
     against the oracle's Lanczos."""
