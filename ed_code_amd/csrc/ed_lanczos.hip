@@ -962,7 +962,9 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
     Ctx c;
     c.op = op;
     c.D = D;
-    c.st = 0;
+    // a private non-blocking stream: concurrent solves from several host
+    // threads overlap instead of serialising on the legacy default stream
+    HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     c.h_partial.resize((size_t)kRedBlocks * kMaxMulti);
     HIPCHK(hipMalloc(&c.d_partial, (size_t)kRedBlocks * kMaxMulti * sizeof(cd2)));
     HIPCHK(hipMalloc(&c.d_coef, kMaxMulti * sizeof(cd2)));
@@ -997,6 +999,7 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
     dfree(c.d_partial); dfree(c.d_coef); dfree(c.d_t); dfree(c.d_coef_big); dfree(c.d_t_big);
     (void)hipEventDestroy(c.e0);
     (void)hipEventDestroy(c.e1);
+    (void)hipStreamDestroy(c.st);
     if (stats) {
         stats->hv_calls = c.hv_calls;
         stats->hv_seconds = c.hv_ms * 1e-3;
@@ -1019,7 +1022,9 @@ extern "C" int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double
     c.D = D;
     c.M = M;
     c.n = D * M;
-    c.st = 0;
+    // a private non-blocking stream: concurrent solves from several host
+    // threads overlap instead of serialising on the legacy default stream
+    HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
     c.g = (unsigned)std::min<int64_t>(kRedBlocks, (c.n + kRedThreads - 1) / kRedThreads);
     c.cap = std::max(p->iterations, 1) + 64;
     HIPCHK(hipMalloc(&c.d_partial, (size_t)kBCols * c.g * sizeof(cd2)));
@@ -1065,6 +1070,7 @@ extern "C" int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double
     dfree(c.d_partial); dfree(c.d_sum); dfree(c.d_coef); dfree(c.d_t); dfree(c.d_s);
     (void)hipEventDestroy(c.e0);
     (void)hipEventDestroy(c.e1);
+    (void)hipStreamDestroy(c.st);
     if (stats) {
         stats->hv_calls = c.hv_calls;
         stats->hv_seconds = c.hv_ms * 1e-3;

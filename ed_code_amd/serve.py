@@ -1,0 +1,306 @@
+"""Serving loop: a stream of eigenvalue requests (lattice, Sz, momentum,
+couplings, nev, tol) answered on one GPU.
+
+The reference answers one request per process run (ED.cpp reads one input
+file and calls sector_lanczos once).  The server keeps the GPU busy across
+requests:
+
+* basis cache -- sectors are built once on the device and kept while they
+  fit the cache budget (least recently used out first);
+* batching -- pending requests for the same sector whose couplings share a
+  term structure are solved together by one block operator (M = 2, 4 or 8
+  instances: one H*V pass serves all of them);
+* concurrency -- several worker threads, each solve on its own HIP stream,
+  so small sectors do not leave the GPU idle between Lanczos steps;
+* HBM admission -- a job starts only when its footprint (basis, Krylov
+  vectors, work vectors) fits next to the running jobs under the budget.
+
+Latency is completion time minus arrival time.  The baseline is the plain
+loop: first come first served, one request at a time, basis built per
+request, same solver.
+
+    python -m ed_code_amd.serve --requests 48 --rate 4 --workers 4
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import json
+import math
+import threading
+import time
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import ED_C64, Basis, BlockOperator, Lattice, init
+from . import lattices as L
+from .lattices import Model
+
+GiB = 1 << 30
+
+
+@dataclass
+class Request:
+    rid: int
+    lattice: str
+    sz: float
+    kx: int
+    ky: int
+    J2: float
+    lam: float
+    Ch: float
+    nev: int
+    tol: float
+    arrival: float
+    done: Optional[float] = None
+    evals: Optional[List[float]] = None
+    batch: int = 0
+
+
+def model_for(lattice: str, J2: float, lam: float, Ch: float) -> Model:
+    """The request's Hamiltonian.  J2 > 0 keeps the second-neighbour bonds in
+    the term list, so requests on one lattice share a term structure."""
+    if lattice.startswith("chain"):
+        return L.chain(int(lattice[5:]), lam=lam)
+    if lattice == "tri4x4":
+        return L.triangular(4, 4, lam=lam, Ch=Ch)
+    if lattice == "tri3x9":
+        return L.triangular(3, 9, lam=lam, Ch=Ch)
+    if lattice == "tri6x5":
+        return L.triangular(6, 5, lam=lam, Ch=Ch)
+    if lattice == "sq4x8":
+        return L.square(4, 8, J2=J2, lam=lam)
+    raise ValueError(lattice)
+
+
+_ALLOWED: Dict[Tuple[str, int], List[Tuple[int, int]]] = {}
+
+
+def allowed_sectors(lattice: str, n_up: int) -> List[Tuple[int, int]]:
+    """Momenta whose sector is non-empty (each sector's basis is built once
+    on the device to find out), cached per (lattice, n_up)."""
+    key = (lattice, n_up)
+    if key not in _ALLOWED:
+        m = model_for(lattice, 0.3, 1.0, 0.1)
+        lat = Lattice.of(m)
+        out = []
+        for kx in range(lat.L1):
+            for ky in range(lat.L2):
+                if Basis(lat, n_up, kx, ky).D > 0:
+                    out.append((kx, ky))
+        _ALLOWED[key] = out
+    return _ALLOWED[key]
+
+
+def make_requests(n: int, rate: float, seed: int, lattices: List[str], k_zero: bool = False) -> List[Request]:
+    """Seeded Poisson arrivals; lattice, Sz in {0, 1}, random allowed k,
+    J2 ~ U[0, 0.6], Delta ~ U[0.5, 1.5], Ch ~ U[0, 0.3] (triangular lattices),
+    nev in {1, 4}, tol 1e-10."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate)
+        lat = lattices[rng.integers(len(lattices))]
+        sz = float(rng.integers(2))
+        N = model_for(lat, 0.3, 1.0, 0.1).n_sites
+        n_up = L.n_ones_for_sz(N, sz)
+        ks = [(0, 0)] if k_zero else allowed_sectors(lat, n_up)
+        kx, ky = ks[rng.integers(len(ks))]
+        out.append(Request(rid, lat, sz, kx, ky, J2=float(rng.uniform(1e-3, 0.6)),
+                           lam=float(rng.uniform(0.5, 1.5)), Ch=float(rng.uniform(0.0, 0.3)),
+                           nev=int(rng.choice([1, 4])), tol=1e-10, arrival=t))
+    return out
+
+
+def sector_key(r: Request) -> tuple:
+    return (r.lattice, L.n_ones_for_sz(model_for(r.lattice, 0.3, 1.0, 0.1).n_sites, r.sz), r.kx, r.ky)
+
+
+def batch_key(r: Request) -> tuple:
+    return sector_key(r) + (r.nev, r.tol)
+
+
+def estimate_rows(lattice: str, n_up: int) -> int:
+    m = model_for(lattice, 0.3, 1.0, 0.1)
+    L1, L2 = m.orders()
+    return math.comb(m.n_sites, n_up) // (L1 * L2) + 1
+
+
+def footprint(rows: int, M: int, iterations: int, nev: int) -> int:
+    """Device bytes of one block solve in complex64 with a complex128 polish:
+    Krylov vectors, work vectors, FP64 Ritz vectors, basis and directory."""
+    krylov = (iterations + 1) * rows * M * 8 + (min(iterations, 60) + 1) * rows * M * 16
+    work = rows * M * (8 + 16 * 3) + nev * rows * M * 16
+    basis = rows * 8 * 2
+    return krylov + work + basis
+
+
+class BasisCache:
+    def __init__(self, budget: int):
+        self.budget = budget
+        self.entries: "collections.OrderedDict[tuple, Basis]" = collections.OrderedDict()
+        self.lock = threading.Lock()
+        self.hits = self.misses = 0
+
+    @staticmethod
+    def _bytes(b: Basis) -> int:
+        return b.D * 16 + 200 * (1 << 20)
+
+    def get(self, key: tuple) -> Basis:
+        with self.lock:
+            if key in self.entries:
+                self.entries.move_to_end(key)
+                self.hits += 1
+                return self.entries[key]
+        lattice, n_up, kx, ky = key
+        b = Basis(Lattice.of(model_for(lattice, 0.3, 1.0, 0.1)), n_up, kx, ky)
+        with self.lock:
+            self.misses += 1
+            self.entries[key] = b
+            while sum(self._bytes(x) for x in self.entries.values()) > self.budget and len(self.entries) > 1:
+                self.entries.popitem(last=False)
+        return b
+
+
+def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
+    models = [model_for(r.lattice, r.J2, r.lam, r.Ch) for r in reqs]
+    blk = BlockOperator(basis, models)
+    ev, _ = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
+    now = time.perf_counter()
+    for m, r in enumerate(reqs):
+        r.evals = [float(x) for x in ev[m]]
+        r.done = now
+        r.batch = len(reqs)
+
+
+class Server:
+    def __init__(self, workers: int = 4, hbm_budget: int = 200 * GiB, cache_budget: int = 24 * GiB,
+                 max_batch: int = 8, iterations: int = 300):
+        self.workers = workers
+        self.budget = hbm_budget
+        self.max_batch = max_batch
+        self.iterations = iterations
+        self.cache = BasisCache(cache_budget)
+        self.in_use = 0
+        self.cv = threading.Condition()
+        self.running = 0
+        self.errors: List[str] = []
+
+    def _job(self, reqs: List[Request], fp: int):
+        try:
+            basis = self.cache.get(sector_key(reqs[0]))
+            solve(basis, reqs, self.iterations)
+        except Exception as e:  # recorded and reported; the request stays unanswered
+            self.errors.append(f"{[r.rid for r in reqs]}: {e!r}")
+        finally:
+            with self.cv:
+                self.in_use -= fp
+                self.running -= 1
+                self.cv.notify_all()
+
+    def run(self, requests: List[Request]) -> Tuple[float, float]:
+        """Serves the requests at their arrival offsets; returns (start, wall)."""
+        t0 = time.perf_counter()
+        for r in requests:
+            r.arrival_abs = t0 + r.arrival
+        pending: List[Request] = []
+        future = sorted(requests, key=lambda r: r.arrival)
+        threads = []
+        with self.cv:
+            while future or pending or self.running:
+                now = time.perf_counter()
+                while future and future[0].arrival_abs <= now:
+                    pending.append(future.pop(0))
+                started = False
+                while pending and self.running < self.workers:
+                    head = pending[0]
+                    group = [p for p in pending if batch_key(p) == batch_key(head)]
+                    M = 8 if len(group) >= 8 else 4 if len(group) >= 4 else 2 if len(group) >= 2 else 1
+                    M = min(M, self.max_batch)
+                    group = group[:M]
+                    rows = estimate_rows(head.lattice, sector_key(head)[1])
+                    fp = footprint(rows, M, self.iterations, head.nev)
+                    if self.in_use + fp > self.budget and self.running:
+                        break
+                    for g in group:
+                        pending.remove(g)
+                    self.in_use += fp
+                    self.running += 1
+                    th = threading.Thread(target=self._job, args=(group, fp), daemon=True)
+                    th.start()
+                    threads.append(th)
+                    started = True
+                if not started:
+                    wait = 0.05
+                    if future and not pending:
+                        wait = max(0.0, min(wait, future[0].arrival_abs - time.perf_counter()))
+                    self.cv.wait(timeout=wait if wait > 0 else 0.001)
+        for th in threads:
+            th.join()
+        return t0, time.perf_counter() - t0
+
+
+def run_baseline(requests: List[Request], iterations: int = 300) -> Tuple[float, float]:
+    """First come first served, one request at a time, basis per request;
+    returns (start, wall)."""
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda r: r.arrival):
+        now = time.perf_counter() - t0
+        if now < r.arrival:
+            time.sleep(r.arrival - now)
+        lattice, n_up, kx, ky = sector_key(r)
+        basis = Basis(Lattice.of(model_for(lattice, 0.3, 1.0, 0.1)), n_up, kx, ky)
+        solve(basis, [r], iterations)
+        del basis
+    return t0, time.perf_counter() - t0
+
+
+def summarize(requests: List[Request], wall: float, t_base: float) -> dict:
+    lat = np.array([(r.done - t_base) - r.arrival for r in requests if r.done is not None])
+    batches = collections.Counter(r.batch for r in requests if r.done is not None)
+    return dict(answered=int(len(lat)), wall_seconds=wall, solves_per_s=len(lat) / wall if wall else 0.0,
+                p50_s=float(np.percentile(lat, 50)) if len(lat) else None,
+                p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
+                mean_s=float(lat.mean()) if len(lat) else None,
+                batch_sizes={int(k): int(v) for k, v in sorted(batches.items())})
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
+    ap.add_argument("--seed", type=int, default=11)
+    ap.add_argument("--lattices", default="chain24,tri3x9,tri6x5")
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--iterations", type=int, default=300)
+    ap.add_argument("--k-zero", action="store_true", help="every request at k=0 (coupling scans)")
+    ap.add_argument("--no-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    init(0)
+    lats = a.lattices.split(",")
+    reqs = make_requests(a.requests, a.rate, a.seed, lats, a.k_zero)
+    srv = Server(workers=a.workers, iterations=a.iterations)
+    # warm-up outside the timed runs (GPU context, library load, first kernels)
+    warm = Request(-1, "chain24", 0.0, 0, 0, 0.1, 1.0, 0.0, 1, 1e-10, 0.0)
+    solve(srv.cache.get(sector_key(warm)), [warm], a.iterations)
+    t0, wall = srv.run(reqs)
+    out = dict(config=dict(requests=a.requests, rate=a.rate, lattices=lats, workers=a.workers, k_zero=a.k_zero),
+               server=summarize(reqs, wall, t0), cache=dict(hits=srv.cache.hits, misses=srv.cache.misses),
+               errors=srv.errors)
+    served = {r.rid: r.evals for r in reqs}
+    if not a.no_baseline:
+        base = make_requests(a.requests, a.rate, a.seed, lats, a.k_zero)
+        t0b, wall_b = run_baseline(base, a.iterations)
+        out["baseline"] = summarize(base, wall_b, t0b)
+        diff = 0.0
+        for r in base:
+            if r.evals is not None and served.get(r.rid) is not None:
+                diff = max(diff, max(abs(x - y) for x, y in zip(r.evals, served[r.rid])))
+        out["max_abs_eval_diff_vs_baseline"] = diff
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
