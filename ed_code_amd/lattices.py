@@ -134,6 +134,137 @@ def bowtie_entries(triangles, bonds):
     return out
 
 
+# -------------------------------------------------------------- relabelling
+@dataclass
+class Relabel:
+    """Site relabelling of a cluster whose translation group acts regularly:
+    new label of old site s = i + n2*j where s = a^i b^j (site 0), with a of
+    order n2 (the group's exponent) and b of order n1 = N/n2 generating a
+    complement.  a = T1^xa T2^ya, b = T1^xb T2^yb in the original generators."""
+    sigma: np.ndarray            # old site -> new site
+    n2: int
+    n1: int
+    xa: int
+    ya: int
+    xb: int
+    yb: int
+    L1: int                      # orders of the original T1, T2
+    L2: int
+    t1: Tuple[int, int]          # T1 = a^i b^j, T2 = a^i' b^j'
+    t2: Tuple[int, int]
+
+    def momentum(self, kx: int, ky: int):
+        """The relabelled model's (kx', ky') carrying the same character,
+        exp(-i p'.(i, j)) = exp(-i p.(x, y)) for a^i b^j = T1^x T2^y; None
+        when (kx, ky) is not a character of the group (the reference's sector
+        is then empty, Lanczos_sector.cpp:93-97)."""
+        fa = self.n2 * (kx * self.xa * self.L2 + ky * self.ya * self.L1)
+        fb = self.n1 * (kx * self.xb * self.L2 + ky * self.yb * self.L1)
+        den = self.L1 * self.L2
+        if fa % den or fb % den:
+            return None
+        k2 = ((fa // den) % self.n2, (fb // den) % self.n1)
+        # the character must also reproduce exp(-i px), exp(-i py) on T1, T2:
+        # k/L == k2[0]*i/n2 + k2[1]*j/n1 (mod 1)
+        for k, Lk, (i, j) in ((kx, self.L1, self.t1), (ky, self.L2, self.t2)):
+            lhs = k * self.n2 * self.n1
+            rhs = (k2[0] * i * self.n1 + k2[1] * j * self.n2) * Lk
+            if (lhs - rhs) % (Lk * self.n2 * self.n1):
+                return None
+        return k2
+
+
+def _perm_pow(T: np.ndarray, e: int) -> np.ndarray:
+    g = np.arange(len(T))
+    for _ in range(e):
+        g = T[g]
+    return g
+
+
+def regular_relabel(model: Model) -> Tuple[Model, Relabel]:
+    """Relabels the sites so that T1' is a rotation inside rows of n2 sites
+    and T2' moves whole rows (the layout of a rectangular torus).  On a
+    cluster whose translation group acts regularly (|G| = N, e.g. the tilted
+    40-site square cluster, group Z2 x Z20) every translation then is a bit
+    rotation, which the rotation fast path needs.  Energies are unchanged;
+    representatives and vectors are those of the relabelled model."""
+    N = model.n_sites
+    T1 = np.asarray(model.T1, np.int64)
+    T2 = np.asarray(model.T2, np.int64)
+    L1, L2 = model.orders()
+    elems = {}
+    for x in range(L1):
+        for y in range(L2):
+            g = _perm_pow(T1, x)[_perm_pow(T2, y)]
+            elems.setdefault(tuple(g.tolist()), (x, y))
+    if len(elems) != N:
+        raise ValueError(f"translation group of order {len(elems)} does not act regularly on {N} sites")
+    perms = {k: np.array(k) for k in elems}
+
+    def order(g):
+        e, h = 1, g.copy()
+        ident = np.arange(N)
+        while not np.array_equal(h, ident):
+            h = g[h]
+            e += 1
+        return e
+
+    ords = {k: order(v) for k, v in perms.items()}
+    n2 = max(ords.values())
+    ka = min((k for k in perms if ords[k] == n2), key=lambda k: elems[k])
+    a = perms[ka]
+    cyc_a = set()
+    h = np.arange(N)
+    for _ in range(n2):
+        cyc_a.add(tuple(h.tolist()))
+        h = a[h]
+    n1 = N // n2
+    kb = None
+    for k in sorted(perms, key=lambda k: elems[k]):
+        if ords[k] != n1:
+            continue
+        g, ok = np.arange(N), True
+        for e in range(1, n1):
+            g = perms[k][g]
+            if tuple(g.tolist()) in cyc_a:
+                ok = False
+                break
+        if ok:
+            kb = k
+            break
+    if kb is None:
+        raise ValueError("no complement to a maximal cyclic subgroup")
+    b = perms[kb]
+    sigma = np.full(N, -1, np.int64)
+    gj = np.arange(N)
+    for j in range(n1):
+        gi = gj.copy()
+        for i in range(n2):
+            sigma[gi[0]] = i + n2 * j
+            gi = a[gi]
+        gj = b[gj]
+    if (sigma < 0).any():
+        raise ValueError("relabelling does not cover every site")
+    T1n = np.zeros(N, np.int32)
+    T2n = np.zeros(N, np.int32)
+    for j in range(n1):
+        for i in range(n2):
+            T1n[i + n2 * j] = (i + 1) % n2 + n2 * j
+            T2n[i + n2 * j] = i + n2 * ((j + 1) % n1)
+    S = lambda q: int(sigma[q])
+    m = Model(model.name + "_rl", N, T1n, T2n, lam=model.lam,
+              bonds=[(S(x), S(y), J, ph) for (x, y, J, ph) in model.bonds], Ch=model.Ch,
+              chiral=[(S(i), S(j), S(k), ph) for (i, j, k, ph) in model.chiral], ChCh=model.ChCh,
+              bowties=[tuple(S(q) for q in bw) for bw in model.bowties],
+              triangles=[(S(i), S(j), S(k)) for (i, j, k) in model.triangles])
+    (xa, ya), (xb, yb) = elems[ka], elems[kb]
+    # coordinates of T1 and T2 in the new generators: where they send site 0
+    def coords(T):
+        q = int(sigma[int(T[0])])
+        return q % n2, q // n2
+    return m, Relabel(sigma, n2, n1, xa, ya, xb, yb, L1, L2, coords(T1), coords(T2))
+
+
 # ----------------------------------------------------------------- lattices
 def chain(L: int, J: float = 1.0, lam: float = 1.0, phases: Sequence[float] | None = None) -> Model:
     """Periodic chain; T1: i -> i+1, T2 = identity (so L2 = 1)."""
