@@ -20,12 +20,12 @@ from typing import Optional
 import numpy as np
 
 from . import lattices
-from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, EDError, LanczosParams, LanczosStats, Terms,
+from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_LEAN, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, EDError, LanczosParams, LanczosStats, Terms,
                    check, lib)
 from .lattices import Model, n_ones_for_sz
 
 __all__ = ["Lattice", "Basis", "Operator", "BlockOperator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
-           "ED_LZ_FULL", "ED_LZ_REFERENCE", "n_ones_for_sz"]
+           "ED_LZ_FULL", "ED_LZ_REFERENCE", "ED_LZ_LEAN", "ED_LZ_SELECTIVE", "n_ones_for_sz"]
 
 _initialised = False
 
@@ -58,7 +58,7 @@ class Lattice:
         return cls(model.T1, model.T2)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_lattice_destroy(self.h)
             self.h = None
 
@@ -99,7 +99,7 @@ class Basis:
         return dict(rep_loc=rep_loc, state_ph=sph.view(np.complex128), Ia=Ia, Ib=Ib, bits_right=br.value)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_basis_destroy(self.h)
             self.h = None
 
@@ -205,7 +205,7 @@ class Operator:
         return (ev[:n], vecs, stats) if want_vectors else (ev[:n], stats)
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_op_destroy(self.h)
             self.h = None
 
@@ -265,7 +265,7 @@ class BlockOperator:
         return ev, stats
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_block_destroy(self.h)
             self.h = None
 

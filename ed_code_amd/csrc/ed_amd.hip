@@ -79,6 +79,7 @@ extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32
     L->L1 = cycle_len0(a);
     L->L2 = cycle_len0(b);
     L->has_torus = torus_detect(a, b, L->torus);
+    L->rows2_w = rows2_detect(a, b);
     if (!make_prog(a, L->P1) || !make_prog(b, L->P2)) {
         delete L;
         return ed_fail(ED_EINVAL, "translation has more than %d displacement groups", kMaxGroups);
@@ -97,57 +98,63 @@ extern "C" int ed_lattice_orders(ed_lattice lat, int32_t* L1, int32_t* L2) {
 extern "C" void ed_lattice_destroy(ed_lattice lat) { delete lat; }
 
 // ---------------------------------------------------------------- kernels --
+// The basis kernels loop over their index range with a grid stride: one
+// dispatch covers at most 2^32 work-items per dimension, and at N=40 the
+// classification runs over 4.3e9 words.
+#define ED_GRID_STRIDE(i, n) \
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
 template <typename S>
 __global__ void k_classify(uint64_t C, int N, int n, Geo G, const uint64_t* binom, uint32_t* words, uint64_t nwords) {
-    uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nwords) return;
-    uint64_t r0 = w * 32;
-    uint32_t bits = 0;
-    if (r0 < C) {
-        uint64_t s = unrank(r0, N, n, binom);
-        for (int b = 0; b < 32; b++) {
-            if (r0 + b >= C) break;
-            if (classify<S>((S)s, G)) bits |= 1u << b;
-            s = gosper_next(s);
+    ED_GRID_STRIDE(w, nwords) {
+        uint64_t r0 = w * 32;
+        uint32_t bits = 0;
+        if (r0 < C) {
+            uint64_t s = unrank(r0, N, n, binom);
+            for (int b = 0; b < 32; b++) {
+                if (r0 + b >= C) break;
+                if (classify<S>((S)s, G)) bits |= 1u << b;
+                s = gosper_next(s);
+            }
         }
+        words[w] = bits;
     }
-    words[w] = bits;
 }
 
 __global__ void k_group_counts(const uint32_t* words, uint64_t ngroups, uint64_t* cnt) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    cnt[g] = (uint64_t)(__popc(words[3 * g]) + __popc(words[3 * g + 1]) + __popc(words[3 * g + 2]));
+    ED_GRID_STRIDE(g, ngroups) {
+        cnt[g] = (uint64_t)(__popc(words[3 * g]) + __popc(words[3 * g + 1]) + __popc(words[3 * g + 2]));
+    }
 }
 
 __global__ void k_dir_fill(const uint32_t* words, const uint64_t* base, uint64_t ngroups, DirEntry* dir) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    DirEntry e;
-    e.base = (uint32_t)base[g];
-    e.b0 = words[3 * g];
-    e.b1 = words[3 * g + 1];
-    e.b2 = words[3 * g + 2];
-    dir[g] = e;
+    ED_GRID_STRIDE(g, ngroups) {
+        DirEntry e;
+        e.base = (uint32_t)base[g];
+        e.b0 = words[3 * g];
+        e.b1 = words[3 * g + 1];
+        e.b2 = words[3 * g + 2];
+        dir[g] = e;
+    }
 }
 
 template <typename S>
 __global__ void k_compact(uint64_t C, int N, int n, Geo G, const uint64_t* binom, const DirEntry* dir,
                           uint64_t ngroups, uint64_t* rows) {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    DirEntry e = dir[g];
-    if ((e.b0 | e.b1 | e.b2) == 0) return;
-    uint64_t r0 = g * kDirSpan;
-    uint64_t s = unrank(r0, N, n, binom);
-    uint64_t j = e.base;
-    for (int b = 0; b < kDirSpan && r0 + b < C; b++) {
-        uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
-        if ((wd >> (b & 31)) & 1u) {
-            int lid = classify<S>((S)s, G);
-            rows[j++] = s | ((uint64_t)lid << kLidShift);
+    ED_GRID_STRIDE(g, ngroups) {
+        DirEntry e = dir[g];
+        if ((e.b0 | e.b1 | e.b2) == 0) continue;
+        uint64_t r0 = g * kDirSpan;
+        uint64_t s = unrank(r0, N, n, binom);
+        uint64_t j = e.base;
+        for (int b = 0; b < kDirSpan && r0 + b < C; b++) {
+            uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
+            if ((wd >> (b & 31)) & 1u) {
+                int lid = classify<S>((S)s, G);
+                rows[j++] = s | ((uint64_t)lid << kLidShift);
+            }
+            s = gosper_next(s);
         }
-        s = gosper_next(s);
     }
 }
 
@@ -258,6 +265,8 @@ __global__ void k_make_h(int64_t D, const uint64_t* rows, Geo G, RankInfo R, Tab
 }
 
 static inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+// grid for the grid-stride kernels: at most 2^22 blocks (2^30 work-items)
+static inline unsigned sblocks(uint64_t n, unsigned bs) { return (unsigned)std::min<uint64_t>((n + bs - 1) / bs, 1u << 22); }
 
 // ------------------------------------------------------------------ basis --
 int basis_geo(const ed_basis_s* B, Geo& G) {
@@ -281,6 +290,7 @@ extern "C" int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_
     B->lat_P1 = lat->P1; B->lat_P2 = lat->P2;
     B->has_torus = lat->has_torus;
     B->torus = lat->torus;
+    B->rows2_w = lat->rows2_w;
     // Lanczos_sector.cpp:57
     B->px = 2 * kx * M_PI / B->L1;
     B->py = 2 * ky * M_PI / B->L2;
@@ -310,6 +320,10 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipMemcpy(B->d_nitab, nit.data(), nit.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(B->d_binom, bn.data(), bn.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(B->d_rtab, rt.data(), rt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    if (B->rows2_w) {
+        int rc = rows2_tables(B, rt);
+        if (rc != ED_OK) return rc;
+    }
     if (N <= 32) {
         std::vector<uint32_t> rt32(rt.size());
         for (size_t q = 0; q < rt.size(); q++) rt32[q] = (uint32_t)rt[q];   // C(N,n) < 2^32
@@ -331,11 +345,11 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipMalloc(&base, (ngroups + 1) * sizeof(uint64_t)));
     HIPCHK(hipMalloc(&B->d_dir, ngroups * sizeof(DirEntry)));
     if (N <= 32)
-        hipLaunchKernelGGL(k_classify<uint32_t>, dim3(nblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+        hipLaunchKernelGGL(k_classify<uint32_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
     else
-        hipLaunchKernelGGL(k_classify<uint64_t>, dim3(nblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+        hipLaunchKernelGGL(k_classify<uint64_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_group_counts, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, words, ngroups, cnt);
+    hipLaunchKernelGGL(k_group_counts, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, words, ngroups, cnt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemset(cnt + ngroups, 0, sizeof(uint64_t)));
     size_t tmp_bytes = 0;
@@ -350,13 +364,13 @@ int basis_build_device(ed_basis_s* B) {
         return ed_fail(ED_EINVAL, "sector dimension %llu exceeds 2^32", (unsigned long long)D);
     }
     B->D = (int64_t)D;
-    hipLaunchKernelGGL(k_dir_fill, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, words, base, ngroups, B->d_dir);
+    hipLaunchKernelGGL(k_dir_fill, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, words, base, ngroups, B->d_dir);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMalloc(&B->d_rows, std::max<uint64_t>(D, 1) * sizeof(uint64_t)));
     if (N <= 32)
-        hipLaunchKernelGGL(k_compact<uint32_t>, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+        hipLaunchKernelGGL(k_compact<uint32_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
     else
-        hipLaunchKernelGGL(k_compact<uint64_t>, dim3(nblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+        hipLaunchKernelGGL(k_compact<uint64_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     dfree(tmp); dfree(words); dfree(cnt); dfree(base);
@@ -366,7 +380,7 @@ int basis_build_device(ed_basis_s* B) {
 void delete_basis(ed_basis_s* B) {
     if (!B) return;
     for (void* p : {(void*)B->d_rows, (void*)B->d_dir, (void*)B->d_rtab, (void*)B->d_binom, (void*)B->d_phase,
-                    (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32})
+                    (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32, (void*)B->d_rlo, (void*)B->d_rhi})
         if (p) dfree(p);
     delete B;
 }
@@ -527,14 +541,24 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
             HIPCHK(hipMemcpy(op->d_tent, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
         }
     }
+    // two-row 64-bit rotation path: no bowtie terms, compiled shape
+    if (b->rows2_w && bw.empty() && b->d_rlo && rows2_shape_compiled(b->N, b->rows2_w)) {
+        int rc = rows2_op_setup(op, bonds, ch);
+        if (rc != ED_OK) {
+            ed_op_destroy(op);
+            return rc;
+        }
+    }
     *out = op;
     return ED_OK;
 }
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 5) return ed_fail(ED_EINVAL, "path %d not in 0..5", path);
-    if (path >= 2 && !op->use_torus) return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
+    if (path < 0 || path > 6) return ed_fail(ED_EINVAL, "path %d not in 0..6", path);
+    if (path == 6 && !op->use_rows2) return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
+    if (path >= 2 && path <= 5 && !op->use_torus)
+        return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
     if ((path == 2 || path == 4 || path == 5) && !torus_shape_compiled(op->torus))
         return ed_fail(ED_EINVAL, "lattice shape is not one of the compiled shapes");
     if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
@@ -549,6 +573,12 @@ extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
 extern "C" int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block) {
     if (!op || !path || !variant || !block) return ed_fail(ED_EINVAL, "null argument");
     if (dtype != ED_C128 && dtype != ED_C64) return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+    if (op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus))) {
+        *path = 6;
+        *variant = -1;
+        *block = 256;
+        return ED_OK;
+    }
     if (!op->use_torus || op->tune_path == 1) {
         *path = 1;
         *variant = -1;
@@ -566,7 +596,7 @@ extern "C" int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* var
 extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
-                    (void*)op->d_tent})
+                    (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64})
         if (p) dfree(p);
     delete op;
 }
@@ -590,12 +620,15 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 256)), blk(256);
-    if (op->use_torus && op->tune_path != 1) {
+    if (op->use_torus && op->tune_path != 1 && op->tune_path != 6) {
         int rc = torus_prepare<VT>(op, u, v, w, st);
         if (rc != ED_OK) return rc;
     }
     if (ev0) HIPCHK(hipEventRecord(ev0, st));
-    if (op->use_torus && op->tune_path != 1) {
+    if (op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus))) {
+        int rc = launch_hv_rows2<VT>(op, u, v, w, st);
+        if (rc != ED_OK) return rc;
+    } else if (op->use_torus && op->tune_path != 1) {
         int rc = launch_hv_torus<VT>(op, u, v, w, st);
         if (rc != ED_OK) return rc;
     } else if (B->N <= 32) {

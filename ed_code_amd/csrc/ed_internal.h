@@ -29,6 +29,7 @@ struct ed_lattice_s {
     edc::TProg P1, P2;
     bool has_torus = false;   // translations are bit rotations (rotation fast path)
     edc::Torus torus;
+    int rows2_w = 0;          // > 0: two rows of rows2_w sites (64-bit rotation path)
 };
 
 struct ed_basis_s {
@@ -48,6 +49,10 @@ struct ed_basis_s {
     int rtab32_n = 0;
     bool has_torus = false;
     edc::Torus torus;
+    int rows2_w = 0;                 // two-row rotation structure (see ed_torus64.h)
+    uint32_t* d_rlo = nullptr;       // rank tables without the top-byte block, 32-bit (N <= 40)
+    uint64_t* d_rhi = nullptr;       // top-byte block, 64-bit [256]
+    int rlo_n = 0;
     uint64_t* d_binom = nullptr;     // C(p,t), 65 x 65
     edc::cd2* d_phase = nullptr;     // [L1*L2]
     double* d_ntab = nullptr;        // norm by lid
@@ -68,7 +73,15 @@ struct ed_op_s {
     edc::TFam* d_tfam = nullptr;
     edc::TEntry* d_tent = nullptr;
     double ezz0 = 0;
-    int tune_path = 0;        // 0 auto, 1 generic walk, 2/4 compiled shape K=4 / K=2, 3 run-time shape
+    // two-row 64-bit rotation path (ed_hv64.hip)
+    bool use_rows2 = false;
+    int nfam64 = 0, nterm64 = 0;
+    void* d_fam64 = nullptr;   // TFam64[nfam64]
+    int* d_foff64 = nullptr;   // [nfam64]
+    edc::TEntry* d_ent64 = nullptr;   // [nterm64], dense term order
+    double ezz0_64 = 0;
+    int tune_path = 0;        // 0 auto, 1 generic walk, 2/4/5 compiled shape K=4 / K=2 / K=8, 3 run-time shape,
+                              // 6 two-row 64-bit kernel
     int tune_block = 256;
     bool block_fixed = false; // block set by ed_op_tune
     int sel_var[2] = {-1, -1};   // tuned variant per precision (0 complex128, 1 complex64)
@@ -91,5 +104,11 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs);
 bool torus_shape_compiled(const edc::Torus& T);
 int block_op_hv(ed_block_s* K, int dtype, const void* v, void* w, hipStream_t st);
+int rows2_detect(const std::vector<int>& T1, const std::vector<int>& T2);
+int rows2_tables(ed_basis_s* B, const std::vector<uint64_t>& rt);
+int rows2_op_setup(ed_op_s* op, const std::vector<edc::Bond>& bonds, const std::vector<edc::Chiral>& ch);
+bool rows2_shape_compiled(int N, int W);
+template <typename VT>
+int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
 ed_basis_s* block_basis(const ed_block_s* K);
 int block_size(const ed_block_s* K);

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -483,6 +485,101 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
     }
     HIPCHK(hipStreamSynchronize(c.st));
     dfree(V); dfree(w); dfree(x);
+    return ED_OK;
+}
+
+// Memory-lean Lanczos: three Krylov vectors, no reorthogonalisation, lowest
+// eigenvalue; a second pass regenerates the vectors from the stored alpha and
+// beta to assemble the FP64 Ritz vector x (returned, unit norm).
+template <typename VT>
+int run_lean(Ctx& c, const ed_lanczos_params& p, cd2*& x_out, double& e0_out, int& steps) {
+    const int64_t D = c.D;
+    const unsigned g = grid_for(D);
+    const bool progress = std::getenv("ED_PROGRESS") != nullptr;
+    const int mit = (int)std::min<int64_t>(p.iterations, D);
+    VT* buf[3] = {nullptr, nullptr, nullptr};
+    for (auto*& b : buf) HIPCHK(hipMalloc(&b, D * sizeof(VT)));
+    std::vector<double> alphas, betas;
+    std::vector<std::complex<double>> q;
+    auto start = [&](VT* v0) -> int {
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed, v0);
+        HIPCHK(hipGetLastError());
+        double nn;
+        RC(dnorm2(c, v0, nn));
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), v0, v0);
+        HIPCHK(hipGetLastError());
+        return ED_OK;
+    };
+    // pass 1: alpha, beta
+    VT *vp = buf[0], *vk = buf[1], *w = buf[2];
+    RC(start(vk));
+    double e0 = 0.0, eprev = 1e300;
+    int k = 0;
+    for (; k < mit; k++) {
+        RC(hv<VT>(c, vk, w));
+        RC((multidot<VT, VT>(c, 1, vk, 0, w, q)));
+        const double alpha = q[0].real();
+        std::vector<std::complex<double>> a1(1, alpha);
+        RC((multiaxpy<VT, VT>(c, 1, vk, 0, a1, 1.0, w)));
+        if (k > 0) {
+            std::vector<std::complex<double>> b1(1, betas.back());
+            RC((multiaxpy<VT, VT>(c, 1, vp, 0, b1, 1.0, w)));
+        }
+        alphas.push_back(alpha);
+        double bb;
+        RC(dnorm2(c, w, bb));
+        const double beta = std::sqrt(bb);
+        bool stop = false;
+        if ((k + 1) % 5 == 0 || k + 1 == mit) {
+            if (!lowest(alphas, betas, e0, nullptr)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+            if (std::fabs(e0 - eprev) < p.tol * std::max(1.0, std::fabs(e0))) stop = true;
+            eprev = e0;
+        }
+        if (progress) std::fprintf(stderr, "[lanczos-lean] pass 1 step %d alpha %.15g beta %.6g E0 %.15g\n", k, alpha, beta, e0);
+        if (beta < 1e-13 * std::max(1.0, std::fabs(alpha))) stop = true;
+        if (stop || k + 1 == mit) { k++; break; }
+        betas.push_back(beta);
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, w);
+        HIPCHK(hipGetLastError());
+        VT* t = vp; vp = vk; vk = w; w = t;
+    }
+    steps = k;
+    std::vector<double> s;
+    {
+        std::vector<double> off(betas.begin(), betas.begin() + std::max(0, k - 1));
+        std::vector<double> al(alphas.begin(), alphas.begin() + k);
+        if (!lowest(al, off, e0, &s)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+    }
+    e0_out = e0;
+    // pass 2: the same vectors from the stored coefficients, x += s_j v_j
+    cd2* x = nullptr;
+    HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
+    HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+    vp = buf[0]; vk = buf[1]; w = buf[2];
+    RC(start(vk));
+    for (int j = 0; j < k; j++) {
+        std::vector<double> sj(1, s[j]);
+        RC(accumulate<VT>(c, 1, vk, 0, sj, x));
+        if (progress) std::fprintf(stderr, "[lanczos-lean] pass 2 step %d\n", j);
+        if (j + 1 == k) break;
+        RC(hv<VT>(c, vk, w));
+        std::vector<std::complex<double>> a1(1, alphas[j]);
+        RC((multiaxpy<VT, VT>(c, 1, vk, 0, a1, 1.0, w)));
+        if (j > 0) {
+            std::vector<std::complex<double>> b1(1, betas[j - 1]);
+            RC((multiaxpy<VT, VT>(c, 1, vp, 0, b1, 1.0, w)));
+        }
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / betas[j], w, w);
+        HIPCHK(hipGetLastError());
+        VT* t = vp; vp = vk; vk = w; w = t;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    for (auto* b : buf) dfree(b);
+    double nn;
+    RC(dnorm2(c, x, nn));
+    hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+    HIPCHK(hipGetLastError());
+    x_out = x;
     return ED_OK;
 }
 
@@ -976,17 +1073,32 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
     HIPCHK(hipEventCreate(&c.e1));
     std::vector<cd2*> prev;
     int rc;
-    if (p->mode == ED_LZ_REFERENCE)
+    if (p->mode == ED_LZ_LEAN) {
+        if (p->nev != 1) {
+            rc = ed_fail(ED_EINVAL, "the lean mode computes the lowest eigenvalue only (nev=1)");
+        } else {
+            cd2* x = nullptr;
+            int steps = 0;
+            rc = p->precision == ED_C128 ? run_lean<cd2>(c, *p, x, evals[0], steps)
+                                         : run_lean<cf2>(c, *p, x, evals[0], steps);
+            if (x) {
+                // the complex64 gather stream is no longer needed: room for the
+                // complex128 one of the FP64 check
+                if (op->d_u64) { dfree(op->d_u64); op->d_u64 = nullptr; }
+                prev.push_back(x);
+            }
+        }
+    } else if (p->mode == ED_LZ_REFERENCE)
         rc = p->precision == ED_C128 ? run_reference<cd2>(c, *p, prev, evals) : run_reference<cf2>(c, *p, prev, evals);
     else
         rc = p->precision == ED_C128 ? run_full<cd2>(c, *p, prev, evals) : run_full<cf2>(c, *p, prev, evals);
     if (rc == ED_OK) {
         for (size_t k = 0; k < prev.size(); k++) {
             double rq = 0, res = 0;
-            if (p->precision == ED_C64 || (stats && k < 16)) {
+            if (p->precision == ED_C64 || p->mode == ED_LZ_LEAN || (stats && k < 16)) {
                 rc = fp64_check(c, prev[k], rq, res);
                 if (rc != ED_OK) break;
-                if (p->precision == ED_C64) evals[k] = rq;   // FP64 Rayleigh-quotient correction
+                if (p->precision == ED_C64 || p->mode == ED_LZ_LEAN) evals[k] = rq;   // FP64 Rayleigh-quotient correction
                 if (stats && k < 16) { stats->residual[k] = res; stats->rayleigh[k] = rq; }
             }
             if (evecs) {

@@ -354,8 +354,7 @@ namespace edc {
 
 // Decodes a directory entry fetched for rank r (see dir_lookup).
 // Branch-free (word select by masks, so the entry stays in registers).
-ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) {
-    const unsigned o = r % kDirSpan;
+ED_HD int64_t dir_decode_off(const DirEntry& e, unsigned o) {
     const unsigned wsel = o >> 5, bit = o & 31u;
     const uint32_t low = (1u << bit) - 1u;
     const uint32_t s0 = wsel == 0 ? 0xffffffffu : 0u, s1 = wsel == 1 ? 0xffffffffu : 0u,
@@ -368,6 +367,8 @@ ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) {
     const int64_t idx = (int64_t)(e.base + popc32(e.b0 & m0) + popc32(e.b1 & m1) + popc32(e.b2 & m2));
     return ((bits >> bit) & 1u) ? idx : int64_t(-1);
 }
+
+ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) { return dir_decode_off(e, r % kDirSpan); }
 
 // torus_row_c with K elements in flight per lane: the orbit searches and
 // ranks of K elements are computed first, then their K directory loads and

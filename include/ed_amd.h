@@ -146,8 +146,16 @@ int ed_dense_eigh(ed_op op, double* evals, double* evecs);
  * tridiagonal eigensolve and a final FP64 residual / Rayleigh-quotient
  * correction.  The start vector is (2u-1) + i(2u'-1) with u, u' from
  * splitmix64(seed*K + 2i), (seed*K + 2i + 1) (the reference uses unseeded
- * rand(), lanczos.cpp:192); eigenvalue number m uses seed + m. */
-enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1 };
+ * rand(), lanczos.cpp:192); eigenvalue number m uses seed + m.  mode
+ * ED_LZ_LEAN is the memory-lean solver for sectors whose Krylov basis does
+ * not fit (N=40: 27.6 GB per complex64 vector): three vectors, no
+ * reorthogonalisation, lowest eigenvalue only (nev must be 1), stopping when
+ * it changes by less than tol*max(1,|E0|) over 5 steps; a second pass
+ * regenerates the same vectors to assemble the FP64 Ritz vector, whose FP64
+ * residual and Rayleigh quotient go to stats (the eigenvalue reported is the
+ * Rayleigh quotient).  Setting the environment variable ED_PROGRESS prints
+ * one line per step to stderr. */
+enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1, ED_LZ_LEAN = 2 };
 
 /* flags: ED_LZ_SELECTIVE makes the second Gram-Schmidt pass of ED_LZ_FULL
  * conditional: it runs only when the first pass removed more than

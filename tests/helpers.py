@@ -239,6 +239,11 @@ def hostcore():
         H.hc_hv_torus_c.argtypes = [vp] * 3
         H.hc_torus_rep_c.restype = C.c_int
         H.hc_torus_rep_c.argtypes = [vp, C.c_uint32, vp, vp]
+        H.hc_rank_split.restype = C.c_uint64
+        H.hc_rank_split.argtypes = [vp, C.c_uint64]
+        H.hc_div96.argtypes = [C.c_uint64, vp, vp]
+        H.hc_hv_rows2.restype = C.c_int
+        H.hc_hv_rows2.argtypes = [vp] * 3
         H.hc_tridiag.restype = C.c_int
         H.hc_tridiag.argtypes = [C.c_int, vp, vp, vp]
         _hc = H
@@ -291,6 +296,15 @@ class HostCore:
         v = np.ascontiguousarray(v, np.complex128)
         w = np.zeros(self.D, np.complex128)
         if hostcore().hc_hv_torus_c(self.h, v.ctypes.data, w.ctypes.data):
+            return None
+        return w
+
+    def hv_rows2(self, v):
+        """Two-row 64-bit row function (ed_torus64.h); None if the lattice is
+        not a compiled two-row shape."""
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        if hostcore().hc_hv_rows2(self.h, v.ctypes.data, w.ctypes.data):
             return None
         return w
 

@@ -12,6 +12,7 @@
 #include "ed_tridiag.h"
 #include "ed_torus.h"
 #include "ed_torus_host.h"
+#include "ed_torus64.h"
 
 using namespace edc;
 
@@ -125,6 +126,26 @@ void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::
     }
 }
 
+template <class TC, bool TRIV>
+void hv_rows2(HC* h, const std::vector<TFam64>& fam, const std::vector<int>& foff, const std::vector<TEntry>& ent,
+              double ezz0, const std::vector<uint32_t>& rlo, const uint64_t* rhi, const std::vector<cd2>& u,
+              const cd2* v, cd2* w) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
+        uint64_t s = h->rows[i] & kStateMask;
+        double nrm = h->nt[h->rows[i] >> kLidShift];
+        double ar = 0, ai = 0;
+        double ez = rows2_row<TC, TRIV, 4>(s, (int)fam.size(), fam.data(), foff.data(), ent.data(), ezz0, h->R,
+                                            rlo.data(), rhi, h->dir.data(), h->ph.data(), i,
+                                            [&](int64_t j) { return u[j]; },
+                                            [&](const cd2& x, double cr, double ci) {
+                                                ar += cr * x.re - ci * x.im;
+                                                ai += cr * x.im + ci * x.re;
+                                            });
+        w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -202,6 +223,24 @@ uint64_t hc_rank(void* p, uint64_t s) {
     return rank_of<uint64_t>(s, r->R, r->rt.data());
 }
 
+// Rank through the split tables of the two-row path (32-bit lower blocks,
+// 64-bit top block), N <= 40.
+uint64_t hc_rank_split(void* p, uint64_t s) {
+    Ranker* r = (Ranker*)p;
+    const size_t top = (size_t)r->R.off[r->R.nbytes - 1] * 256;
+    std::vector<uint32_t> lo(r->rt.begin(), r->rt.begin() + top);
+    uint32_t acc = 0;
+    int c = 0;
+    for (int k = 0; k < r->R.nbytes - 1; k++) {
+        const unsigned b = (unsigned)(s >> (8 * k)) & 255u;
+        acc += lo[(r->R.off[k] + c - r->R.cmin[k]) * 256 + b];
+        c += __builtin_popcount(b);
+    }
+    return (uint64_t)acc + r->rt[top + ((s >> (8 * (r->R.nbytes - 1))) & 255u)];
+}
+
+void hc_div96(uint64_t x, uint32_t* q, uint32_t* o) { div96(x, *q, *o); }
+
 uint64_t hc_unrank(void* p, uint64_t k) {
     Ranker* r = (Ranker*)p;
     return unrank(k, r->N, r->n, r->bn.data());
@@ -236,6 +275,49 @@ void hc_hv(void* p, const double* v, double* w) {
     } else {
         if (triv) hv<uint64_t, true>(h, (const cd2*)v, (cd2*)w); else hv<uint64_t, false>(h, (const cd2*)v, (cd2*)w);
     }
+}
+
+// H*v through the two-row 64-bit path (ed_torus64.h) for the compiled
+// shapes; 1 if the lattice is not a compiled two-row shape (w untouched).
+int hc_hv_rows2(void* p, const double* v_, double* w_) {
+    HC* h = (HC*)p;
+    const cd2* v = (const cd2*)v_;
+    cd2* w = (cd2*)w_;
+    std::vector<int> a(h->N), b(h->N);
+    for (int i = 0; i < h->N; i++) {
+        uint64_t s1 = apply_T<uint64_t>(1ull << i, h->G.T1), s2 = apply_T<uint64_t>(1ull << i, h->G.T2);
+        a[i] = ctz64(s1); b[i] = ctz64(s2);
+    }
+    const int N = h->N;
+    int W = (N % 2 == 0) ? N / 2 : 0;
+    for (int i = 0; i < N && W; i++)
+        if (a[i] != (i / W) * W + (i % W + 1) % W || b[i] != (i + W) % N) W = 0;
+    if (!W || !h->w.empty()) return 1;
+    std::vector<TFam64> fam;
+    std::vector<int> foff;
+    std::vector<TEntry> ent;
+    double ezz0;
+    rows2_families(h->lam, h->Ch, W, h->b, h->c, fam, foff, ent, ezz0);
+    const size_t top = (size_t)h->R.off[h->R.nbytes - 1] * 256;
+    std::vector<uint32_t> rlo(h->rt.begin(), h->rt.begin() + top);
+    const uint64_t* rhi = h->rt.data() + top;
+    std::vector<cd2> u(h->rows.size());
+    for (size_t i = 0; i < u.size(); i++) {
+        double f = h->nit[h->rows[i] >> kLidShift];
+        u[i] = cd2{v[i].re * f, v[i].im * f};
+    }
+    const bool triv = h->kx == 0 && h->ky == 0;
+#define HC_R2(n, wd)                                                                                       \
+    if (N == n && W == wd) {                                                                               \
+        if (triv) hv_rows2<Rows2C<n, wd>, true>(h, fam, foff, ent, ezz0, rlo, rhi, u, v, w);               \
+        else hv_rows2<Rows2C<n, wd>, false>(h, fam, foff, ent, ezz0, rlo, rhi, u, v, w);                   \
+        return 0;                                                                                          \
+    }
+    ED_ROWS2_SHAPES(HC_R2)
+    HC_R2(12, 6)
+    HC_R2(16, 8)
+#undef HC_R2
+    return 1;
 }
 
 // H*v through the rotation fast path (ed_torus.h).  Returns 1 if the

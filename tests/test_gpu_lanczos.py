@@ -80,3 +80,37 @@ def test_sector_lanczos_dense_and_lanczos_paths(gpu_ready):
     assert abs(e4[0] + 2.0) < 1e-12
     # odd N, Sz = 0 means n_ones = N//2
     assert ed.n_ones_for_sz(27, 0.0) == 13 and ed.n_ones_for_sz(27, 1.0) == 14
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_lean_lanczos_vs_dense(gpu_ready, prec):
+    """Three-vector Lanczos without reorthogonalisation, Ritz vector from a
+    second pass: lowest eigenvalue (FP64 Rayleigh quotient) against dense."""
+    ed = _ed()
+    model = L.triangular(4, 4, Ch=0.2, J2=0.3)
+    o = P.OracleBasis(model, 8, 1, 2)
+    e_dense = np.linalg.eigvalsh(o.make_h())[0]
+    basis = ed.Basis(ed.Lattice.of(model), 8, 1, 2)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    lz, st = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=9, precision=pr, mode=ed.ED_LZ_LEAN)
+    assert abs(lz[0] - e_dense) < 1e-9 * abs(e_dense)
+    assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-4)
+    with pytest.raises(ed.EDError):
+        op.lanczos(nev=2, iterations=50, mode=ed.ED_LZ_LEAN)
+
+
+def test_lean_lanczos_two_row_path(gpu_ready):
+    """Lean mode through the two-row 64-bit kernel (relabelled tilted 20-site
+    cluster) against the full-reorthogonalisation solver on the same sector."""
+    ed = _ed()
+    m, _ = L.regular_relabel(L.tilted_square((4, 2), (-2, 4), J2=0.5))
+    basis = ed.Basis(ed.Lattice.of(m), 10, 0, 0)
+    op = ed.Operator(basis, m)
+    op.tune(6)
+    lean, st = op.lanczos(nev=1, iterations=400, ncycles=1, tol=1e-13, seed=2, precision=ed.ED_C64,
+                          mode=ed.ED_LZ_LEAN)
+    full, _ = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=2, precision=ed.ED_C128,
+                         mode=ed.ED_LZ_FULL)
+    assert abs(lean[0] - full[0]) < 1e-9 * abs(full[0])
+    assert st["residual"][0] < 1e-4

@@ -229,3 +229,29 @@ def test_gpu_autotune_picks_identical_result(gpu_ready):
     op.tune(0, 256)                            # fixed block: only the variant is tuned
     assert torch.equal(op.hv(v), w_auto)
     assert op.tuning("c64")[2] == 256
+
+
+TWO_ROW = [("tilted20_k00", (4, 2), (-2, 4), 10, (0, 0)), ("tilted20_k71", (4, 2), (-2, 4), 10, (7, 1)),
+           ("tilted20_n9_k31", (4, 2), (-2, 4), 9, (3, 1))]
+
+
+@pytest.mark.parametrize("name,P1,P2,n,k", TWO_ROW, ids=[c[0] for c in TWO_ROW])
+def test_gpu_two_row_64bit_path(gpu_ready, name, P1, P2, n, k):
+    """Relabelled tilted 20-site cluster through the two-row 64-bit kernel
+    (forced with path 6; the 32-bit rotation kernel also applies at N=20)
+    against the oracle, both precisions."""
+    import torch
+    ed = _ed()
+    m, _ = L.regular_relabel(L.tilted_square(P1, P2, J2=0.5, lam=0.9))
+    print(f"[case] two-row {name}", flush=True)
+    o = P.OracleBasis(m, n, *k)
+    basis = ed.Basis(ed.Lattice.of(m), n, *k)
+    assert basis.D == o.D
+    op = ed.Operator(basis, m)
+    op.tune(6)
+    assert op.tuning("c128")[0] == 6
+    rng = np.random.default_rng(21)
+    v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
+    wo = o.hv(v)
+    assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo)
+    assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo)

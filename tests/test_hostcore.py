@@ -157,3 +157,57 @@ def test_batched_compiled_row_vs_oracle(model, n, k):
     assert w is not None
     wo = o.hv(v)
     assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+
+
+def _rows2_cases():
+    t20, info = L.regular_relabel(L.tilted_square((4, 2), (-2, 4), J2=0.5, lam=0.9))
+    tri, _ = L.regular_relabel(L.triangular(6, 2, Ch=0.25))
+    return [(t20, 10, info.momentum(0, 0)), (t20, 10, (7, 1)), (t20, 9, (3, 1)),
+            (L.square(8, 2, J2=0.4), 8, (3, 1)), (tri, 6, (0, 0)), (tri, 5, (2, 1)),
+            (L.chain(12, phases=[0.3] * 12), 6, (0, 0))]
+
+
+ROWS2 = _rows2_cases()
+
+
+@pytest.mark.parametrize("model,n,k", ROWS2, ids=[f"{m.name}_n{n}_k{k[0]}{k[1]}" for m, n, k in ROWS2])
+def test_two_row_64bit_row_vs_oracle(model, n, k):
+    """The two-row 64-bit row function (exact double orbit keys, split rank
+    tables, directory index through a double) against the oracle."""
+    o = P.OracleBasis(model, n, *k)
+    h = Hp.HostCore(model, n, *k)
+    assert h.D == o.D and o.D > 0
+    v = np.random.default_rng(18).normal(size=o.D) + 1j * np.random.default_rng(19).normal(size=o.D)
+    w = h.hv_rows2(v)
+    if model.name.startswith("chain"):
+        assert w is None                      # a ring is not a two-row lattice
+        return
+    assert w is not None
+    wo = o.hv(v)
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+
+
+def test_split_rank_tables_and_div96_at_40_sites():
+    """N=40, n=20: the split 32/64-bit rank tables give the combinatorial
+    number system rank (closed form) on random and extreme states, and the
+    double-based rank/96 split is exact up to C(40,20)."""
+    import ctypes as C
+    import math
+    H = Hp.hostcore()
+    r = H.hc_ranker_new(40, 20)
+    rng = np.random.default_rng(4)
+    states = [sum(1 << int(b) for b in rng.choice(40, 20, replace=False)) for _ in range(2000)]
+    states += [(1 << 20) - 1, ((1 << 20) - 1) << 20]
+    for s in states:
+        assert H.hc_rank_split(r, s) == Hp.cns_rank(s)
+    assert H.hc_rank_split(r, ((1 << 20) - 1) << 20) == math.comb(40, 20) - 1
+    H.hc_ranker_free(r)
+    q, o = C.c_uint32(), C.c_uint32()
+    top = math.comb(40, 20)
+    xs = list(rng.integers(0, top, 20000)) + [96 * k + d for k in (0, 1, 3 * 2**31 // 96, top // 96) for d in (-1, 0, 1, 95)]
+    for x in xs:
+        x = int(x)
+        if x < 0:
+            continue
+        H.hc_div96(x, C.byref(q), C.byref(o))
+        assert (q.value, o.value) == (x // 96, x % 96), x
