@@ -20,7 +20,7 @@ from typing import Optional
 import numpy as np
 
 from . import lattices
-from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_LEAN, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, EDError, LanczosParams, LanczosStats, Terms,
+from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_GRAPH, ED_LZ_LEAN, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, EDError, LanczosParams, LanczosStats, Terms,
                    check, lib)
 from .lattices import Model, n_ones_for_sz
 
@@ -192,8 +192,9 @@ class Operator:
         return (ev, vecs.reshape(self.D, self.D, order="F")) if want_vectors else ev
 
     def lanczos(self, nev=1, iterations=100, ncycles=10, tol=1e-11, seed=1234, precision=ED_C128,
-                mode=ED_LZ_REFERENCE, want_vectors=False, selective=False):
-        p = LanczosParams(nev, iterations, ncycles, precision, mode, ED_LZ_SELECTIVE if selective else 0, tol, seed)
+                mode=ED_LZ_REFERENCE, want_vectors=False, selective=False, graph=False):
+        flags = (ED_LZ_SELECTIVE if selective else 0) | (ED_LZ_GRAPH if graph else 0)
+        p = LanczosParams(nev, iterations, ncycles, precision, mode, flags, tol, seed)
         ev = np.zeros(nev)
         vecs = np.zeros((nev, self.D), np.complex128) if want_vectors else None
         st = LanczosStats()

@@ -583,6 +583,242 @@ int run_lean(Ctx& c, const ed_lanczos_params& p, cd2*& x_out, double& e0_out, in
     return ED_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Graph-captured lean Lanczos.  Every scalar of a step stays on the device:
+// alpha_k = Re<v_k, H v_k> and beta_{k+1} = ||w|| are reduced by a fixed-order
+// tree into ab[] (alpha at 2k, beta at 2k+1), the update kernels read them
+// from there, and a device step counter selects the slot.  Three steps (one
+// full rotation of the three vector buffers) are captured once as a hipGraph
+// and replayed; the host reads ab[] every kGraphCheck replays to test
+// convergence.  Nothing in the captured work allocates or memsets.
+
+constexpr int kGraphCheck = 2;   // replays (x 3 steps) between convergence checks
+
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_dot_re_partial(int64_t D, const VT* __restrict__ a,
+                                                                 const VT* __restrict__ b, double* __restrict__ part) {
+    __shared__ double red[kRedThreads / 64];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        const cd2 x = ld(a, i), y = ld(b, i);
+        acc += x.re * y.re + x.im * y.im;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int q = 0; q < kRedThreads / 64; q++) r += red[q];
+        part[blockIdx.x] = r;
+    }
+}
+
+// out = fixed-order sum of part[0..g): one block, strided partial sums then a tree
+__device__ double block_sum(const double* __restrict__ part, int g) {
+    __shared__ double red[kRedThreads];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < g; b += blockDim.x) acc += part[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// ab[2k] = alpha_k
+__global__ void __launch_bounds__(kRedThreads) k_finish_alpha(const double* part, int g, double* ab, const int* step) {
+    const double s = block_sum(part, g);
+    if (threadIdx.x == 0) ab[2 * *step] = s;
+}
+
+// ab[2k+1] = beta_{k+1} = sqrt(sum)
+__global__ void __launch_bounds__(kRedThreads) k_finish_beta(const double* part, int g, double* ab, const int* step) {
+    const double s = block_sum(part, g);
+    if (threadIdx.x == 0) ab[2 * *step + 1] = sqrt(s);
+}
+
+// w = w - alpha_k v - beta_k vp   (beta_0 = 0)
+template <typename VT>
+__global__ void k_lz_update(int64_t D, VT* __restrict__ w, const VT* __restrict__ v, const VT* __restrict__ vp,
+                            const double* ab, const int* step) {
+    const int k = *step;
+    const double al = ab[2 * k], be = k > 0 ? ab[2 * k - 1] : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        const cd2 x = ld(w, i), y = ld(v, i), z = ld(vp, i);
+        st(w, i, x.re - al * y.re - be * z.re, x.im - al * y.im - be * z.im);
+    }
+}
+
+// w = w / beta_{k+1} (0 once the Krylov space is exhausted or past nmax steps)
+template <typename VT>
+__global__ void k_lz_scale(int64_t D, VT* __restrict__ w, const double* ab, const int* step, int nmax) {
+    const int k = *step;
+    const double be = ab[2 * k + 1];
+    const double f = (k + 1 < nmax && be > 1e-300) ? 1.0 / be : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        const cd2 x = ld(w, i);
+        st(w, i, f * x.re, f * x.im);
+    }
+}
+
+// x += s_k v (k < n)
+template <typename VT>
+__global__ void k_lz_accum(int64_t D, cd2* __restrict__ x, const VT* __restrict__ v, const double* s, const int* step,
+                           int n) {
+    const int k = *step;
+    if (k >= n) return;
+    const double f = s[k];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+        const cd2 y = ld(v, i);
+        cd2 a = x[i];
+        a.re += f * y.re;
+        a.im += f * y.im;
+        x[i] = a;
+    }
+}
+
+__global__ void k_step_inc(int* step) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+template <typename VT>
+int run_lean_graph(Ctx& c, const ed_lanczos_params& p, cd2*& x_out, double& e0_out, int& steps) {
+    const int64_t D = c.D;
+    const unsigned g = grid_for(D);
+    const int dt = std::is_same<VT, cd2>::value ? ED_C128 : ED_C64;
+    const int mit = (int)std::min<int64_t>(p.iterations, D);
+    const int cap = mit + 8;   // room for the steps of the last replay
+    VT* buf[3] = {nullptr, nullptr, nullptr};
+    for (auto*& b : buf) HIPCHK(hipMalloc(&b, D * sizeof(VT)));
+    double *part = nullptr, *ab = nullptr, *sv = nullptr;
+    int* step = nullptr;
+    HIPCHK(hipMalloc(&part, g * sizeof(double)));
+    HIPCHK(hipMalloc(&ab, 2 * cap * sizeof(double)));
+    HIPCHK(hipMalloc(&sv, cap * sizeof(double)));
+    HIPCHK(hipMalloc(&step, sizeof(int)));
+    cd2* x = nullptr;
+    HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
+    auto cleanup = [&]() {
+        for (auto* b : buf) dfree(b);
+        dfree(part); dfree(ab); dfree(sv); dfree(step);
+    };
+    // start vector, normalised, in buf[1]; warm H*v outside the capture
+    // (allocates the gather stream and settles the kernel choice)
+    auto start = [&]() -> int {
+        // v_{-1} = 0 (the first update multiplies it by beta_0 = 0)
+        HIPCHK(hipMemsetAsync(buf[0], 0, D * sizeof(VT), c.st));
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed, buf[1]);
+        HIPCHK(hipGetLastError());
+        double nn;
+        RC(dnorm2(c, buf[1], nn));
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), buf[1],
+                           buf[1]);
+        HIPCHK(hipGetLastError());
+        const int zero = 0;
+        HIPCHK(hipMemcpyAsync(step, &zero, sizeof(int), hipMemcpyHostToDevice, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        return ED_OK;
+    };
+    RC(start());
+    RC(op_hv(c.op, dt, buf[1], buf[2], c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    // capture 3 steps of pass 1 (buffers rotate vp <- v <- w <- vp)
+    auto capture = [&](bool pass2, int n, hipGraphExec_t& exec) -> int {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+        int r[3] = {0, 1, 2};
+        for (int q = 0; q < 3; q++) {
+            VT *vp = buf[r[0]], *v = buf[r[1]], *w = buf[r[2]];
+            if (pass2)
+                hipLaunchKernelGGL((k_lz_accum<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, x, v, sv, step, n);
+            int rc = op_hv(c.op, dt, v, w, c.st);
+            if (rc != ED_OK) { (void)hipStreamEndCapture(c.st, &graph); return rc; }
+            if (!pass2) {
+                hipLaunchKernelGGL((k_dot_re_partial<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, v, w, part);
+                hipLaunchKernelGGL(k_finish_alpha, dim3(1), dim3(kRedThreads), 0, c.st, part, (int)g, ab, step);
+            }
+            hipLaunchKernelGGL((k_lz_update<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, w, v, vp, ab, step);
+            if (!pass2) {
+                hipLaunchKernelGGL((k_dot_re_partial<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, w, w, part);
+                hipLaunchKernelGGL(k_finish_beta, dim3(1), dim3(kRedThreads), 0, c.st, part, (int)g, ab, step);
+            }
+            hipLaunchKernelGGL((k_lz_scale<VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, w, ab, step, pass2 ? n : cap - 4);
+            hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, c.st, step);
+            const int t = r[0]; r[0] = r[1]; r[1] = r[2]; r[2] = t;
+        }
+        HIPCHK(hipStreamEndCapture(c.st, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        return ED_OK;
+    };
+    hipGraphExec_t g1;
+    RC(capture(false, 0, g1));
+    std::vector<double> hab(2 * cap);
+    std::vector<double> alphas, betas;
+    double e0 = 0.0, eprev = 1e300;
+    int done = 0;   // steps completed on the device
+    bool conv = false;
+    while (!conv && done < mit) {
+        for (int rep = 0; rep < kGraphCheck && done < mit && done + 3 <= cap - 4; rep++) {
+            HIPCHK(hipGraphLaunch(g1, c.st));
+            done += 3;
+        }
+        HIPCHK(hipMemcpyAsync(hab.data(), ab, 2 * done * sizeof(double), hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        // alpha_k = hab[2k], beta_{k+1} = hab[2k+1]; stop at an exhausted Krylov space
+        int valid = done;
+        for (int k = 0; k < done; k++)
+            if (hab[2 * k + 1] < 1e-13 * std::max(1.0, std::fabs(hab[2 * k]))) {
+                valid = k + 1;
+                conv = true;
+                break;
+            }
+        done = valid;
+        alphas.assign(done, 0.0);
+        betas.assign(done - 1, 0.0);
+        for (int k = 0; k < done; k++) {
+            alphas[k] = hab[2 * k];
+            if (k + 1 < done) betas[k] = hab[2 * k + 1];
+        }
+        if (!lowest(alphas, betas, e0, nullptr)) {
+            (void)hipGraphExecDestroy(g1);
+            cleanup();
+            return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+        }
+        if (std::fabs(e0 - eprev) < p.tol * std::max(1.0, std::fabs(e0))) conv = true;
+        eprev = e0;
+        if (std::getenv("ED_PROGRESS")) std::fprintf(stderr, "[lanczos-graph] pass 1 steps %d E0 %.15g\n", done, e0);
+    }
+    (void)hipGraphExecDestroy(g1);
+    c.hv_calls += 1 + ((done + 2) / 3) * 3;   // warm-up plus the replayed steps
+    steps = done;
+    std::vector<double> sv_h;
+    {
+        std::vector<double> al(alphas.begin(), alphas.begin() + done), off(betas.begin(), betas.begin() + done - 1);
+        if (!lowest(al, off, e0, &sv_h)) { cleanup(); return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge"); }
+    }
+    e0_out = e0;
+    // pass 2 from the same start with the stored alpha/beta
+    HIPCHK(hipMemcpyAsync(sv, sv_h.data(), done * sizeof(double), hipMemcpyHostToDevice, c.st));
+    HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+    RC(start());
+    hipGraphExec_t g2;
+    RC(capture(true, done, g2));
+    for (int k = 0; k < done; k += 3) HIPCHK(hipGraphLaunch(g2, c.st));
+    HIPCHK(hipStreamSynchronize(c.st));
+    (void)hipGraphExecDestroy(g2);
+    c.hv_calls += ((done + 2) / 3) * 3;
+    cleanup();
+    double nn;
+    RC(dnorm2(c, x, nn));
+    hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+    HIPCHK(hipGetLastError());
+    x_out = x;
+    return ED_OK;
+}
+
 // FP64 residual ||H x - (x^H H x) x|| and Rayleigh quotient of unit vector x.
 int fp64_check(Ctx& c, const cd2* x, double& rq, double& res) {
     cd2* y = nullptr;
@@ -1079,8 +1315,12 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
         } else {
             cd2* x = nullptr;
             int steps = 0;
-            rc = p->precision == ED_C128 ? run_lean<cd2>(c, *p, x, evals[0], steps)
-                                         : run_lean<cf2>(c, *p, x, evals[0], steps);
+            if (p->flags & ED_LZ_GRAPH)
+                rc = p->precision == ED_C128 ? run_lean_graph<cd2>(c, *p, x, evals[0], steps)
+                                             : run_lean_graph<cf2>(c, *p, x, evals[0], steps);
+            else
+                rc = p->precision == ED_C128 ? run_lean<cd2>(c, *p, x, evals[0], steps)
+                                             : run_lean<cf2>(c, *p, x, evals[0], steps);
             if (x) {
                 // the complex64 gather stream is no longer needed: room for the
                 // complex128 one of the FP64 check

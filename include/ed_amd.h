@@ -159,8 +159,12 @@ enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1, ED_LZ_LEAN = 2 };
 
 /* flags: ED_LZ_SELECTIVE makes the second Gram-Schmidt pass of ED_LZ_FULL
  * conditional: it runs only when the first pass removed more than
- * 1 - 1/sqrt(2) of the vector's norm (Daniel-Gragg-Kaufman-Stewart). */
-enum ed_lanczos_flags { ED_LZ_SELECTIVE = 1 };
+ * 1 - 1/sqrt(2) of the vector's norm (Daniel-Gragg-Kaufman-Stewart).
+ * ED_LZ_GRAPH runs ED_LZ_LEAN with every step's scalars on the device and
+ * three steps captured as one hipGraph, replayed without host round trips;
+ * the host reads alpha/beta every 6 steps (so up to 5 steps past the
+ * convergence point run, and are used). */
+enum ed_lanczos_flags { ED_LZ_SELECTIVE = 1, ED_LZ_GRAPH = 2 };
 
 typedef struct ed_lanczos_params {
     int32_t nev;

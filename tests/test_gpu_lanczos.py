@@ -114,3 +114,26 @@ def test_lean_lanczos_two_row_path(gpu_ready):
                          mode=ed.ED_LZ_FULL)
     assert abs(lean[0] - full[0]) < 1e-9 * abs(full[0])
     assert st["residual"][0] < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_graph_captured_lean_lanczos(gpu_ready, prec):
+    """The lean solver with three steps captured as one hipGraph and replayed
+    (device-side alpha/beta, no host round trips between checks): lowest
+    eigenvalue against dense and against the eager lean solver; the graph is
+    replayed many times in each run and the solve is run twice."""
+    ed = _ed()
+    model = L.triangular(4, 4, Ch=0.2, J2=0.3)
+    o = P.OracleBasis(model, 8, 1, 2)
+    e_dense = np.linalg.eigvalsh(o.make_h())[0]
+    basis = ed.Basis(ed.Lattice.of(model), 8, 1, 2)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    eager, _ = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=9, precision=pr, mode=ed.ED_LZ_LEAN)
+    for _ in range(2):
+        g, st = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=9, precision=pr, mode=ed.ED_LZ_LEAN,
+                           graph=True)
+        assert abs(g[0] - e_dense) < 1e-9 * abs(e_dense)
+        assert abs(g[0] - eager[0]) < 1e-9 * abs(e_dense)
+        assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-4)
+        assert st["hv_calls"] > 30
