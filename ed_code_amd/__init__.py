@@ -38,24 +38,27 @@ def init(device: int = 0):
 
 
 class Lattice:
-    def __init__(self, T1, T2):
+    def __init__(self, T1, T2, T3=None):
         init()
         self.T1 = np.ascontiguousarray(T1, np.int32)
         self.T2 = np.ascontiguousarray(T2, np.int32)
-        if self.T1.ndim != 1 or self.T1.shape != self.T2.shape:
+        self.T3 = None if T3 is None else np.ascontiguousarray(T3, np.int32)
+        if self.T1.ndim != 1 or self.T1.shape != self.T2.shape or (self.T3 is not None and self.T3.shape != self.T1.shape):
             # read_input_file.cpp:423-427 rejects T1/T2 of different lengths
             raise EDError(1, "T1 and T2 must have the same length")
         self.N = len(self.T1)
         h = C.c_void_p()
-        check(lib().ed_lattice_create(self.N, self.T1.ctypes.data, self.T2.ctypes.data, C.byref(h)))
+        check(lib().ed_lattice_create3(self.N, self.T1.ctypes.data, self.T2.ctypes.data,
+                                       None if self.T3 is None else self.T3.ctypes.data, C.byref(h)))
         self.h = h
-        a, b = C.c_int32(), C.c_int32()
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         check(lib().ed_lattice_orders(h, C.byref(a), C.byref(b)))
-        self.L1, self.L2 = a.value, b.value
+        check(lib().ed_lattice_order3(h, C.byref(c)))
+        self.L1, self.L2, self.L3 = a.value, b.value, c.value
 
     @classmethod
     def of(cls, model: Model) -> "Lattice":
-        return cls(model.T1, model.T2)
+        return cls(model.T1, model.T2, getattr(model, "T3", None))
 
     def __del__(self):
         if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
@@ -66,11 +69,11 @@ class Lattice:
 class Basis:
     """Momentum basis of (n_ones, kx, ky), resident on the GPU."""
 
-    def __init__(self, lattice: Lattice, n_ones: int, kx: int = 0, ky: int = 0):
+    def __init__(self, lattice: Lattice, n_ones: int, kx: int = 0, ky: int = 0, kz: int = 0):
         self.lattice = lattice
-        self.n_ones, self.kx, self.ky = n_ones, kx, ky
+        self.n_ones, self.kx, self.ky, self.kz = n_ones, kx, ky, kz
         h = C.c_void_p()
-        check(lib().ed_basis_build(lattice.h, n_ones, kx, ky, C.byref(h)))
+        check(lib().ed_basis_build3(lattice.h, n_ones, kx, ky, kz, C.byref(h)))
         self.h = h
         d = C.c_int64()
         check(lib().ed_basis_dim(h, C.byref(d)))

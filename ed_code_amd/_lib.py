@@ -48,6 +48,9 @@ SIGNATURES = {
     "ed_last_error": (C.c_char_p, []),
     "ed_init": (C.c_int, [C.c_int]),
     "ed_lattice_create": (C.c_int, [C.c_int32, _VP, _VP, C.POINTER(_VP)]),
+    "ed_lattice_create3": (C.c_int, [C.c_int32, _VP, _VP, _VP, C.POINTER(_VP)]),
+    "ed_lattice_order3": (C.c_int, [_VP, C.POINTER(C.c_int32)]),
+    "ed_basis_build3": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_VP)]),
     "ed_lattice_orders": (C.c_int, [_VP, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ed_lattice_destroy": (None, [_VP]),
     "ed_basis_build": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_VP)]),

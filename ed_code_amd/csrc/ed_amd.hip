@@ -5,13 +5,13 @@
 //   rows  : uint64 [D]   representative | (lid << 52), ascending; lid = number
 //                        of translation-grid points fixing it, which gives
 //                        norm = sqrt(L1*L2/lid) through a tiny table
-//   dir   : 16 B [C/96]  {base, 96 occupancy bits} over the combinatorial
+//   dir   : 12 B [C/64]  {base, 64 occupancy bits} over the combinatorial
 //                        rank of every weight-n state: index of a
 //                        representative = base + popcount of the bits below
 //   rtab  : uint64       byte-sliced rank tables (a few KB)
 //   phase : complex128 [L1*L2] exp(-i(px lx + py ly)), computed on the host
 //                        with the reference's expression (bit-equal)
-// Nothing of size C survives the build except the 1.33 bit/state directory.
+// Nothing of size C survives the build except the 1.5 bit/state directory.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <rocblas/rocblas.h>
@@ -58,29 +58,37 @@ extern "C" int ed_init(int device) {
 }
 
 // --------------------------------------------------------------- lattice --
-extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32_t* T2, ed_lattice* out) {
+extern "C" int ed_lattice_create3(int32_t n_sites, const int32_t* T1, const int32_t* T2, const int32_t* T3,
+                                  ed_lattice* out) {
     if (!out || !T1 || !T2) return ed_fail(ED_EINVAL, "null argument");
     if (n_sites < 1 || n_sites > 63) return ed_fail(ED_EINVAL, "n_sites=%d outside [1,63]", n_sites);
-    std::vector<int> a(T1, T1 + n_sites), b(T2, T2 + n_sites);
-    for (auto* T : {&a, &b}) {
+    std::vector<int> a(T1, T1 + n_sites), b(T2, T2 + n_sites), c(n_sites);
+    for (int i = 0; i < n_sites; i++) c[i] = T3 ? T3[i] : i;
+    for (auto* T : {&a, &b, &c}) {
         std::vector<char> seen(n_sites, 0);
         for (int i = 0; i < n_sites; i++) {
             int t = (*T)[i];
-            if (t < 0 || t >= n_sites || seen[t]) return ed_fail(ED_EINVAL, "T1/T2 must be permutations of 0..%d", n_sites - 1);
+            if (t < 0 || t >= n_sites || seen[t]) return ed_fail(ED_EINVAL, "T1/T2/T3 must be permutations of 0..%d", n_sites - 1);
             seen[t] = 1;
         }
     }
     for (int i = 0; i < n_sites; i++)
-        if (a[b[i]] != b[a[i]]) return ed_fail(ED_EINVAL, "T1 and T2 do not commute");
+        if (a[b[i]] != b[a[i]] || a[c[i]] != c[a[i]] || b[c[i]] != c[b[i]])
+            return ed_fail(ED_EINVAL, "the translations do not commute");
     auto* L = new ed_lattice_s();
     L->N = n_sites;
     L->T1 = a;
     L->T2 = b;
+    L->T3 = c;
     L->L1 = cycle_len0(a);
     L->L2 = cycle_len0(b);
-    L->has_torus = torus_detect(a, b, L->torus);
-    L->rows2_w = rows2_detect(a, b);
-    if (!make_prog(a, L->P1) || !make_prog(b, L->P2)) {
+    L->L3 = cycle_len0(c);
+    // the rotation fast paths cover two-generator lattices
+    if (L->L3 == 1) {
+        L->has_torus = torus_detect(a, b, L->torus);
+        L->rows2_w = rows2_detect(a, b);
+    }
+    if (!make_prog(a, L->P1) || !make_prog(b, L->P2) || !make_prog(c, L->P3)) {
         delete L;
         return ed_fail(ED_EINVAL, "translation has more than %d displacement groups", kMaxGroups);
     }
@@ -88,10 +96,20 @@ extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32
     return ED_OK;
 }
 
+extern "C" int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32_t* T2, ed_lattice* out) {
+    return ed_lattice_create3(n_sites, T1, T2, nullptr, out);
+}
+
 extern "C" int ed_lattice_orders(ed_lattice lat, int32_t* L1, int32_t* L2) {
     if (!lat) return ed_fail(ED_EINVAL, "null lattice");
     if (L1) *L1 = lat->L1;
     if (L2) *L2 = lat->L2;
+    return ED_OK;
+}
+
+extern "C" int ed_lattice_order3(ed_lattice lat, int32_t* L3) {
+    if (!lat || !L3) return ed_fail(ED_EINVAL, "null argument");
+    *L3 = lat->L3;
     return ED_OK;
 }
 
@@ -123,7 +141,7 @@ __global__ void k_classify(uint64_t C, int N, int n, Geo G, const uint64_t* bino
 
 __global__ void k_group_counts(const uint32_t* words, uint64_t ngroups, uint64_t* cnt) {
     ED_GRID_STRIDE(g, ngroups) {
-        cnt[g] = (uint64_t)(__popc(words[3 * g]) + __popc(words[3 * g + 1]) + __popc(words[3 * g + 2]));
+        cnt[g] = (uint64_t)(__popc(words[2 * g]) + __popc(words[2 * g + 1]));
     }
 }
 
@@ -131,9 +149,8 @@ __global__ void k_dir_fill(const uint32_t* words, const uint64_t* base, uint64_t
     ED_GRID_STRIDE(g, ngroups) {
         DirEntry e;
         e.base = (uint32_t)base[g];
-        e.b0 = words[3 * g];
-        e.b1 = words[3 * g + 1];
-        e.b2 = words[3 * g + 2];
+        e.b0 = words[2 * g];
+        e.b1 = words[2 * g + 1];
         dir[g] = e;
     }
 }
@@ -143,12 +160,12 @@ __global__ void k_compact(uint64_t C, int N, int n, Geo G, const uint64_t* binom
                           uint64_t ngroups, uint64_t* rows) {
     ED_GRID_STRIDE(g, ngroups) {
         DirEntry e = dir[g];
-        if ((e.b0 | e.b1 | e.b2) == 0) continue;
+        if ((e.b0 | e.b1) == 0) continue;
         uint64_t r0 = g * kDirSpan;
         uint64_t s = unrank(r0, N, n, binom);
         uint64_t j = e.base;
         for (int b = 0; b < kDirSpan && r0 + b < C; b++) {
-            uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
+            uint32_t wd = b < 32 ? e.b0 : e.b1;
             if ((wd >> (b & 31)) & 1u) {
                 int lid = classify<S>((S)s, G);
                 rows[j++] = s | ((uint64_t)lid << kLidShift);
@@ -271,43 +288,50 @@ static inline unsigned sblocks(uint64_t n, unsigned bs) { return (unsigned)std::
 // ------------------------------------------------------------------ basis --
 int basis_geo(const ed_basis_s* B, Geo& G) {
     std::memset(&G, 0, sizeof(G));
-    G.N = B->N; G.L1 = B->L1; G.L2 = B->L2; G.kx = B->kx; G.ky = B->ky;
+    G.N = B->N; G.L1 = B->L1; G.L2 = B->L2; G.L3 = B->L3; G.kx = B->kx; G.ky = B->ky; G.kz = B->kz;
     G.T1 = B->lat_P1;
     G.T2 = B->lat_P2;
+    G.T3 = B->lat_P3;
     return ED_OK;
 }
 
-extern "C" int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out) {
+extern "C" int ed_basis_build3(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, int32_t kz, ed_basis* out) {
     if (!lat || !out) return ed_fail(ED_EINVAL, "null argument");
     const int N = lat->N;
     if (n_ones < 0 || n_ones > N) return ed_fail(ED_EINVAL, "n_ones=%d outside [0,%d] (Num sites < Num_ones)", n_ones, N);
-    if (kx < 0 || ky < 0 || kx >= lat->L1 || ky >= lat->L2)
-        return ed_fail(ED_EINVAL, "Invalid kx or ky: (%d,%d) with L1=%d L2=%d", kx, ky, lat->L1, lat->L2);
+    if (kx < 0 || ky < 0 || kz < 0 || kx >= lat->L1 || ky >= lat->L2 || kz >= lat->L3)
+        return ed_fail(ED_EINVAL, "Invalid kx, ky or kz: (%d,%d,%d) with L1=%d L2=%d L3=%d", kx, ky, kz, lat->L1, lat->L2,
+                       lat->L3);
     auto* B = new ed_basis_s();
-    B->N = N; B->n = n_ones; B->kx = kx; B->ky = ky;
-    B->L1 = lat->L1; B->L2 = lat->L2;
-    B->T1 = lat->T1; B->T2 = lat->T2;
-    B->lat_P1 = lat->P1; B->lat_P2 = lat->P2;
+    B->N = N; B->n = n_ones; B->kx = kx; B->ky = ky; B->kz = kz;
+    B->L1 = lat->L1; B->L2 = lat->L2; B->L3 = lat->L3;
+    B->T1 = lat->T1; B->T2 = lat->T2; B->T3 = lat->T3;
+    B->lat_P1 = lat->P1; B->lat_P2 = lat->P2; B->lat_P3 = lat->P3;
     B->has_torus = lat->has_torus;
     B->torus = lat->torus;
     B->rows2_w = lat->rows2_w;
-    // Lanczos_sector.cpp:57
+    // Lanczos_sector.cpp:57 (three generators :240)
     B->px = 2 * kx * M_PI / B->L1;
     B->py = 2 * ky * M_PI / B->L2;
+    B->pz = 2 * kz * M_PI / B->L3;
     B->C = binom_u64(N, n_ones);
-    B->trivial_k = (kx == 0 && ky == 0);
+    B->trivial_k = (kx == 0 && ky == 0 && kz == 0);
     int rc = basis_build_device(B);
     if (rc != ED_OK) { delete_basis(B); return rc; }
     *out = B;
     return ED_OK;
 }
 
+extern "C" int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out) {
+    return ed_basis_build3(lat, n_ones, kx, ky, 0, out);
+}
+
 // Host-side tables + the device build (classify -> scan -> directory -> compact).
 int basis_build_device(ed_basis_s* B) {
     const int N = B->N, n = B->n, L1 = B->L1, L2 = B->L2;
-    std::vector<cd2> ph = phase_table(L1, L2, B->px, B->py);
+    std::vector<cd2> ph = phase_table(L1, L2, B->L3, B->px, B->py, B->pz);
     std::vector<double> nt, nit;
-    norm_tables(L1, L2, nt, nit);
+    norm_tables(L1, L2, B->L3, nt, nit);
     std::vector<uint64_t> bn = binom_table();
     std::vector<uint64_t> rt = rank_tables(N, n, B->rinfo);
     HIPCHK(hipMalloc(&B->d_phase, ph.size() * sizeof(cd2)));
@@ -336,7 +360,7 @@ int basis_build_device(ed_basis_s* B) {
     basis_geo(B, G);
     const uint64_t C = B->C;
     const uint64_t ngroups = (C + kDirSpan - 1) / kDirSpan;
-    const uint64_t nwords = ngroups * 3;
+    const uint64_t nwords = ngroups * 2;
     B->ngroups = ngroups;
     uint32_t* words = nullptr;
     uint64_t *cnt = nullptr, *base = nullptr;
@@ -352,11 +376,12 @@ int basis_build_device(ed_basis_s* B) {
     hipLaunchKernelGGL(k_group_counts, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, words, ngroups, cnt);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemset(cnt + ngroups, 0, sizeof(uint64_t)));
+    // the group count exceeds 2^31 at N=40: the scan takes a 64-bit length
     size_t tmp_bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (int)(ngroups + 1)));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
     void* tmp = nullptr;
     HIPCHK(hipMalloc(&tmp, tmp_bytes + 16));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (int)(ngroups + 1)));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
     uint64_t D = 0;
     HIPCHK(hipMemcpy(&D, base + ngroups, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (D > 0xFFFFFFFFull) {

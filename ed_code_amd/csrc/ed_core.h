@@ -26,7 +26,7 @@
 namespace edc {
 
 constexpr int kMaxGroups = 64;   // masked-shift groups per translation
-constexpr int kDirSpan = 96;     // ranks covered by one directory entry
+constexpr int kDirSpan = 64;     // ranks covered by one directory entry
 constexpr int kLidShift = 52;    // row word = state | (lid << kLidShift)
 constexpr uint64_t kStateMask = (uint64_t(1) << kLidShift) - 1;
 
@@ -39,11 +39,12 @@ struct TProg {
     int32_t sh[kMaxGroups];
 };
 
-// Translation-grid geometry of one sector.
+// Translation-grid geometry of one sector: the reference's L1 x L2 (x L3)
+// walk; two-generator lattices have L3 = 1 and T3 = identity.
 struct Geo {
-    int32_t N, L1, L2, kx, ky;
+    int32_t N, L1, L2, L3, kx, ky, kz;
     int32_t pad;
-    TProg T1, T2;
+    TProg T1, T2, T3;
 };
 
 // Byte-sliced rank tables: rank(s) = sum_k tab[(off[k] + c_k - cmin[k])*256 + byte_k(s)]
@@ -54,10 +55,11 @@ struct RankInfo {
     int32_t cmin[8];
 };
 
-// Directory entry for ranks [96g, 96g+96): base = number of valid
-// representatives of rank < 96g; bits = occupancy of the 96 ranks.
+// Directory entry for ranks [64g, 64g+64): base = number of valid
+// representatives of rank < 64g; b0, b1 = occupancy of the 64 ranks
+// (12 bytes: 1.5 bit per state).
 struct DirEntry {
-    uint32_t base, b0, b1, b2;
+    uint32_t base, b0, b1;
 };
 
 template <typename S>
@@ -71,27 +73,37 @@ ED_HD S apply_T(S s, const TProg& P) {
     return t;
 }
 
-// Representative (orbit minimum) of s; code = lx*L2 + ly of the first strict
-// minimum in the reference walk (Momentum_Basis.cpp:493-536, 745-751).
+// Representative (orbit minimum) of s; code = (lx*L2 + ly)*L3 + lz of the
+// first strict minimum in the reference walk (Momentum_Basis.cpp:493-536,
+// 745-751; three generators :566-631, 860-870).
 template <typename S, bool TRACK>
 ED_HD S find_rep(S s, const Geo& G, int& code) {
     S t = s, best = s;
-    int bx = 0, by = 0;
+    int bx = 0, by = 0, bz = 0;
     for (int x = 0; x < G.L1; x++) {
         if (x > 0) {
             t = apply_T(t, G.T1);
-            if (t < best) { best = t; if (TRACK) { bx = x; by = 0; } }
+            if (t < best) { best = t; if (TRACK) { bx = x; by = 0; bz = 0; } }
         }
-        for (int y = 1; y < G.L2; y++) {
-            t = apply_T(t, G.T2);
-            if (t < best) { best = t; if (TRACK) { bx = x; by = y; } }
+        for (int y = 0; y < G.L2; y++) {
+            if (y > 0) {
+                t = apply_T(t, G.T2);
+                if (t < best) { best = t; if (TRACK) { bx = x; by = y; bz = 0; } }
+            }
+            for (int z = 1; z < G.L3; z++) {
+                t = apply_T(t, G.T3);
+                if (t < best) { best = t; if (TRACK) { bx = x; by = y; bz = z; } }
+            }
         }
     }
     if (TRACK) {
-        int ty = bx * (G.L2 - 1) + by;
-        int lx = (G.L1 - bx) % G.L1;
-        int ly = ((G.L2 - ty) % G.L2 + G.L2) % G.L2;
-        code = lx * G.L2 + ly;
+        // ty, tz count every T2 / T3 step of the walk (they are not reset)
+        const int ty = bx * (G.L2 - 1) + by;
+        const int tz = (bx * G.L2 + by) * (G.L3 - 1) + bz;
+        const int lx = (G.L1 - bx) % G.L1;
+        const int ly = ((G.L2 - ty) % G.L2 + G.L2) % G.L2;
+        const int lz = ((G.L3 - tz) % G.L3 + G.L3) % G.L3;
+        code = (lx * G.L2 + ly) * G.L3 + lz;
     } else {
         code = 0;
     }
@@ -101,32 +113,40 @@ ED_HD S find_rep(S s, const Geo& G, int& code) {
 // Is s its own orbit minimum with a nonvanishing momentum projection?
 // Returns lid (grid points fixing s, >= 1) or 0 if s is not a valid
 // representative.  The projection test is the exact form of
-// Momentum_Basis.cpp:538-540: the sum of exp(-i(px tx + py ty)) over the
-// fixing grid points is nonzero iff every one of them has
-// kx*tx/L1 + ky*ty/L2 integral (they form a subgroup of Z_L1 x Z_L2).
+// Momentum_Basis.cpp:538-540 (3-D: :633-636): the sum of
+// exp(-i(px tx + py ty + pz tz)) over the fixing grid points is nonzero iff
+// every one of them has kx*tx/L1 + ky*ty/L2 + kz*tz/L3 integral (they form a
+// subgroup of Z_L1 x Z_L2 x Z_L3).
 template <typename S>
 ED_HD int classify(S s, const Geo& G) {
     S t = s;
     int lid = 1;
     bool ok = true;
-    const int64_t M = (int64_t)G.L1 * G.L2;
-    int ty = 0;
+    const int64_t M = (int64_t)G.L1 * G.L2 * G.L3;
+    int ty = 0, tz = 0;
+    auto fixed = [&](int x) {
+        lid++;
+        if (((int64_t)G.kx * x * G.L2 * G.L3 + (int64_t)G.ky * ty * G.L1 * G.L3 + (int64_t)G.kz * tz * G.L1 * G.L2) % M)
+            ok = false;
+    };
     for (int x = 0; x < G.L1; x++) {
         if (x > 0) {
             t = apply_T(t, G.T1);
             if (t < s) return 0;
-            if (t == s) {
-                lid++;
-                if (((int64_t)G.kx * x * G.L2 + (int64_t)G.ky * ty * G.L1) % M != 0) ok = false;
-            }
+            if (t == s) fixed(x);
         }
-        for (int y = 1; y < G.L2; y++) {
-            t = apply_T(t, G.T2);
-            ty++;
-            if (t < s) return 0;
-            if (t == s) {
-                lid++;
-                if (((int64_t)G.kx * x * G.L2 + (int64_t)G.ky * ty * G.L1) % M != 0) ok = false;
+        for (int y = 0; y < G.L2; y++) {
+            if (y > 0) {
+                t = apply_T(t, G.T2);
+                ty++;
+                if (t < s) return 0;
+                if (t == s) fixed(x);
+            }
+            for (int z = 1; z < G.L3; z++) {
+                t = apply_T(t, G.T3);
+                tz++;
+                if (t < s) return 0;
+                if (t == s) fixed(x);
             }
         }
     }
@@ -176,15 +196,13 @@ ED_HD int ctz64(uint64_t x) {
 // Index of the representative with rank r, or -1 if r is not a valid
 // representative.
 ED_HD int64_t dir_lookup(uint64_t r, const DirEntry* dir) {
-    uint64_t g = r / kDirSpan;
-    unsigned o = (unsigned)(r - g * kDirSpan);
-    DirEntry e = dir[g];
+    DirEntry e = dir[r / kDirSpan];
+    unsigned o = (unsigned)(r % kDirSpan);
     unsigned wsel = o >> 5, bit = o & 31u;
-    uint32_t bits = wsel == 0 ? e.b0 : (wsel == 1 ? e.b1 : e.b2);
+    uint32_t bits = wsel == 0 ? e.b0 : e.b1;
     if (!((bits >> bit) & 1u)) return -1;
     uint32_t j = e.base + popc32(bits & ((1u << bit) - 1u));
     if (wsel > 0) j += popc32(e.b0);
-    if (wsel > 1) j += popc32(e.b1);
     return (int64_t)j;
 }
 

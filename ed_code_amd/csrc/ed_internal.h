@@ -24,26 +24,26 @@ inline void dfree(void* p) {
     } while (0)
 
 struct ed_lattice_s {
-    int N = 0, L1 = 0, L2 = 0;
-    std::vector<int> T1, T2;
-    edc::TProg P1, P2;
+    int N = 0, L1 = 0, L2 = 0, L3 = 1;
+    std::vector<int> T1, T2, T3;     // T3 = identity for two-generator lattices
+    edc::TProg P1, P2, P3;
     bool has_torus = false;   // translations are bit rotations (rotation fast path)
     edc::Torus torus;
     int rows2_w = 0;          // > 0: two rows of rows2_w sites (64-bit rotation path)
 };
 
 struct ed_basis_s {
-    int N = 0, n = 0, kx = 0, ky = 0, L1 = 0, L2 = 0;
-    double px = 0, py = 0;
+    int N = 0, n = 0, kx = 0, ky = 0, kz = 0, L1 = 0, L2 = 0, L3 = 1;
+    double px = 0, py = 0, pz = 0;
     uint64_t C = 0;
     int64_t D = 0;
     uint64_t ngroups = 0;
     bool trivial_k = false;
-    std::vector<int> T1, T2;
-    edc::TProg lat_P1, lat_P2;
+    std::vector<int> T1, T2, T3;
+    edc::TProg lat_P1, lat_P2, lat_P3;
     edc::RankInfo rinfo;
     uint64_t* d_rows = nullptr;      // [D] representative | lid << 52
-    edc::DirEntry* d_dir = nullptr;  // [ceil(C/96)]
+    edc::DirEntry* d_dir = nullptr;  // [ceil(C/64)]
     uint64_t* d_rtab = nullptr;      // rank tables
     uint32_t* d_rtab32 = nullptr;    // same, 32-bit (N <= 32)
     int rtab32_n = 0;

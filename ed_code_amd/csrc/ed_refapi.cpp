@@ -86,12 +86,12 @@ TermsCopy terms_of(double lambda, const std::vector<coordinates>& adj, double Ch
 }
 
 struct Sector {
-    std::vector<int> T1, T2;
-    int n = -1, kx = -1, ky = -1;
+    std::vector<int> T1, T2, T3;   // T3 empty: two generators
+    int n = -1, kx = -1, ky = -1, kz = 0;
     ed_lattice lat = nullptr;
     ed_basis basis = nullptr;
     int64_t D = 0;
-    int L1 = 0, L2 = 0;
+    int L1 = 0, L2 = 0, L3 = 1;
     TermsCopy terms;
     ed_op op = nullptr;
     void* dv = nullptr;
@@ -110,30 +110,39 @@ std::vector<std::unique_ptr<Sector>>& cache() {
     return c;
 }
 
-Sector& sector(const std::vector<int>& T1, const std::vector<int>& T2, int n, double px, double py,
-               std::ofstream& out) {
+// T3 empty (and pz ignored) for two generators.
+Sector& sector(const std::vector<int>& T1, const std::vector<int>& T2, const std::vector<int>& T3, int n, double px,
+               double py, double pz, std::ofstream& out) {
     static bool init = false;
     if (!init) { check(ed_init(0), out, "ed_init"); init = true; }
-    if (T1.size() != T2.size()) fatal(out, "T1 and T2 lengths differ");
+    if (T1.size() != T2.size() || (!T3.empty() && T3.size() != T1.size())) fatal(out, "T1, T2, T3 lengths differ");
     ed_lattice lat = nullptr;
-    check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), out, "ed_lattice_create");
-    int32_t L1, L2;
+    check(ed_lattice_create3((int32_t)T1.size(), T1.data(), T2.data(), T3.empty() ? nullptr : T3.data(), &lat), out,
+          "ed_lattice_create3");
+    int32_t L1, L2, L3;
     ed_lattice_orders(lat, &L1, &L2);
-    int kx = k_from_p(px, L1), ky = k_from_p(py, L2);
+    ed_lattice_order3(lat, &L3);
+    int kx = k_from_p(px, L1), ky = k_from_p(py, L2), kz = T3.empty() ? 0 : k_from_p(pz, L3);
     auto& C = cache();
     for (auto& s : C)
-        if (s->T1 == T1 && s->T2 == T2 && s->n == n && s->kx == kx && s->ky == ky) {
+        if (s->T1 == T1 && s->T2 == T2 && s->T3 == T3 && s->n == n && s->kx == kx && s->ky == ky && s->kz == kz) {
             ed_lattice_destroy(lat);
             return *s;
         }
     std::unique_ptr<Sector> s(new Sector());
-    s->T1 = T1; s->T2 = T2; s->n = n; s->kx = kx; s->ky = ky;
-    s->lat = lat; s->L1 = L1; s->L2 = L2;
-    check(ed_basis_build(lat, n, kx, ky, &s->basis), out, "ed_basis_build");
+    s->T1 = T1; s->T2 = T2; s->T3 = T3; s->n = n; s->kx = kx; s->ky = ky; s->kz = kz;
+    s->lat = lat; s->L1 = L1; s->L2 = L2; s->L3 = L3;
+    check(ed_basis_build3(lat, n, kx, ky, kz, &s->basis), out, "ed_basis_build3");
     ed_basis_dim(s->basis, &s->D);
     if (C.size() >= 4) C.erase(C.begin());
     C.push_back(std::move(s));
     return *C.back();
+}
+
+Sector& sector(const std::vector<int>& T1, const std::vector<int>& T2, int n, double px, double py,
+               std::ofstream& out) {
+    static const std::vector<int> none;
+    return sector(T1, T2, none, n, px, py, 0.0, out);
 }
 
 ed_op op_for(Sector& S, const TermsCopy& t, std::ofstream& out) {
@@ -168,6 +177,49 @@ int64_t translate(int64_t s, const std::vector<int>& T) {
     for (size_t i = 0; i < T.size(); i++)
         if ((s >> i) & 1) t |= int64_t(1) << T[i];
     return t;
+}
+
+// Momentum_Basis.cpp:901-975 for three generators: the z walk sits inside the
+// y walk with cumulative T3 steps, the phase multiplied by exp(-i pz) per step.
+std::vector<Ids> unwrap(const std::vector<cplx>& v, const std::vector<int64_t>& reps, const std::vector<double>& norm,
+                        double px, double py, double pz, const std::vector<int>& T1, const std::vector<int>& T2,
+                        const std::vector<int>& T3, int L1, int L2, int L3) {
+    std::vector<Ids> out;
+    const cplx ex = cexp_c(-px), ey = cexp_c(-py), ez = cexp_c(-pz);
+    for (size_t i = 0; i < reps.size(); i++) {
+        int64_t id = reps[i];
+        std::vector<Ids> tmp;
+        auto add = [&](int64_t sid, cplx c) {
+            for (auto& e : tmp)
+                if (e.stateID == sid) return;
+            tmp.push_back(Ids{sid, c});
+        };
+        add(id, v[i] / norm[i]);
+        cplx ph(1.0, 0.0);
+        for (int x = 0; x < L1; x++) {
+            if (x > 0) {
+                id = translate(id, T1);
+                ph = ph * ex;
+                add(id, v[i] * ph / norm[i]);
+            }
+            for (int y = 0; y < L2; y++) {
+                if (y > 0) {
+                    id = translate(id, T2);
+                    ph = ph * ey;
+                    add(id, v[i] * ph / norm[i]);
+                }
+                for (int z = 1; z < L3; z++) {
+                    id = translate(id, T3);
+                    ph = ph * ez;
+                    add(id, v[i] * ph / norm[i]);
+                }
+            }
+        }
+        for (auto& e : tmp)
+            if (std::fabs(e.coeff.real()) > 1e-8 || std::fabs(e.coeff.imag()) > 1e-8) out.push_back(e);
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Ids& a, const Ids& b) { return a.stateID < b.stateID; });
+    return out;
 }
 
 std::vector<Ids> unwrap(const std::vector<cplx>& v, const std::vector<int64_t>& reps, const std::vector<double>& norm,
@@ -211,10 +263,12 @@ void export_basis(Sector& S, std::vector<int64_t>& reps, std::vector<double>& no
     check(ed_basis_export(S.basis, reps.data(), norm.data(), norm_inv.data()), out, "ed_basis_export");
 }
 
-// Writes the _EV<i>.txt / _pEV<i>.txt files of Lanczos_sector.cpp:136-197.
-void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, int kx, int ky, double Sz,
+// Writes the _EV<i>.txt / _pEV<i>.txt files of Lanczos_sector.cpp:136-197
+// (two generators) and :306-366 (three: kz and pz columns added).
+void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, int kx, int ky, int kz, double Sz,
                    double lambda, const std::vector<cplx>& eigs, const std::vector<std::vector<cplx>>& vecs,
-                   Sector& S, double px, double py, std::ofstream& out) {
+                   Sector& S, double px, double py, double pz, std::ofstream& out) {
+    const bool three = !S.T3.empty();
     std::vector<int64_t> reps;
     std::vector<double> norm, norm_inv;
     export_basis(S, reps, norm, norm_inv, out);
@@ -222,18 +276,29 @@ void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, i
         std::string fp = path + (unwrap_peigvec ? "_EV" : "_pEV") + std::to_string(iib) + ".txt";
         std::ofstream f(fp.c_str());
         f << "\n Eigenvector for state \n";
-        f << "Sz\t(kx,ky)\tJz/lambda\n";
-        f << Sz << "\t(" << kx << "," << ky << ")\t" << lambda << std::endl;
+        if (three) {
+            f << "Sz\t(kx,ky,kz)\tJz/lambda\n";
+            f << Sz << "\t(" << kx << "," << ky << "," << kz << ")\t" << lambda << std::endl;
+        } else {
+            f << "Sz\t(kx,ky)\tJz/lambda\n";
+            f << Sz << "\t(" << kx << "," << ky << ")\t" << lambda << std::endl;
+        }
         f << "\n\n-------------------------------------------";
         f << "\nState Energy " << eigs[iib] << std::endl;
         if (unwrap_peigvec) {
-            std::vector<Ids> psi = unwrap(vecs[iib], reps, norm, px, py, S.T1, S.T2, S.L1, S.L2);
+            std::vector<Ids> psi = three ? unwrap(vecs[iib], reps, norm, px, py, pz, S.T1, S.T2, S.T3, S.L1, S.L2, S.L3)
+                                         : unwrap(vecs[iib], reps, norm, px, py, S.T1, S.T2, S.L1, S.L2);
             f << "\nEIGENVECTOR\n";
             for (auto& e : psi) f << e.stateID << "\t" << std::fixed << std::setprecision(15) << e.coeff << std::endl;
         } else {
             f << "\nMomentum EIGENVECTOR\n";
-            f << "\nnp \t\t px \t\tpy\n";
-            f << S.D << "\t\t" << px << "\t\t" << py << std::endl;
+            if (three) {
+                f << "\nnp \t\t px \t\tpy \t\tpz\n";
+                f << S.D << "\t\t" << px << "\t\t" << py << "\t\t" << pz << std::endl;
+            } else {
+                f << "\nnp \t\t px \t\tpy\n";
+                f << S.D << "\t\t" << px << "\t\t" << py << std::endl;
+            }
             f << "\n\npblock_states \t\t Norm \t\t\t v_coeff\n\n";
             for (int64_t jb = 0; jb < S.D; jb++)
                 f << reps[jb] << "\t\t\t" << norm[jb] << "\t\t\t" << std::fixed << std::setprecision(15)
@@ -269,15 +334,14 @@ void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, dou
 }  // namespace
 
 // ------------------------------------------------------------------ sHv ----
-void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int64_t>& pblock_states,
-         const std::vector<double>&, const std::vector<double>&, const std::vector<int64_t>&,
-         const std::vector<std::complex<double> >&, const double& lambda, const std::vector<coordinates>& adj_list,
-         const double& Ch, const std::vector<triangles>& ch_list, const double& ChCh,
-         const std::vector<bowties>& chch_list, const int&, const std::vector<int64_t>&,
-         const std::vector<int64_t>&, const int&, const std::vector<int>& T1, const std::vector<int>& T2,
-         const double& kx, const double& ky, std::ofstream& outfile) {
+namespace {
+
+void shv_impl(cplx* v, cplx* w, const std::vector<int64_t>& pblock_states, double lambda,
+              const std::vector<coordinates>& adj_list, double Ch, const std::vector<triangles>& ch_list, double ChCh,
+              const std::vector<bowties>& chch_list, const std::vector<int>& T1, const std::vector<int>& T2,
+              const std::vector<int>& T3, double px, double py, double pz, std::ofstream& outfile) {
     if (pblock_states.empty()) return;
-    Sector& S = sector(T1, T2, popcount64(pblock_states[0]), kx, ky, outfile);
+    Sector& S = sector(T1, T2, T3, popcount64(pblock_states[0]), px, py, pz, outfile);
     if (S.D != (int64_t)pblock_states.size())
         fatal(outfile, "sHv: pblock_states does not match the GPU basis of this sector");
     ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
@@ -291,14 +355,39 @@ void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int
     if (hipMemcpy(w, S.dw, bytes, hipMemcpyDeviceToHost) != hipSuccess) fatal(outfile, "sHv: copy out failed");
 }
 
+}  // namespace
+
+void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int64_t>& pblock_states,
+         const std::vector<double>&, const std::vector<double>&, const std::vector<int64_t>&,
+         const std::vector<std::complex<double> >&, const double& lambda, const std::vector<coordinates>& adj_list,
+         const double& Ch, const std::vector<triangles>& ch_list, const double& ChCh,
+         const std::vector<bowties>& chch_list, const int&, const std::vector<int64_t>&,
+         const std::vector<int64_t>&, const int&, const std::vector<int>& T1, const std::vector<int>& T2,
+         const double& kx, const double& ky, std::ofstream& outfile) {
+    static const std::vector<int> none;
+    shv_impl(v, w, pblock_states, lambda, adj_list, Ch, ch_list, ChCh, chch_list, T1, T2, none, kx, ky, 0.0, outfile);
+}
+
+void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int64_t>& pblock_states,
+         const std::vector<double>&, const std::vector<double>&, const std::vector<int64_t>&,
+         const std::vector<std::complex<double> >&, const double& lambda, const std::vector<coordinates>& adj_list,
+         const double& Ch, const std::vector<triangles>& ch_list, const double& ChCh,
+         const std::vector<bowties>& chch_list, const int&, const std::vector<int64_t>&,
+         const std::vector<int64_t>&, const int&, const std::vector<int>& T1, const std::vector<int>& T2,
+         const std::vector<int>& T3, const double& kx, const double& ky, const double& kz, std::ofstream& outfile) {
+    shv_impl(v, w, pblock_states, lambda, adj_list, Ch, ch_list, ChCh, chch_list, T1, T2, T3, kx, ky, kz, outfile);
+}
+
 // ---------------------------------------------------- pConstruct_States ----
-void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
-                       std::vector<int>& T1, std::vector<int>& T2, int& num_ones, double& px, double& py,
-                       int64_t& n_states, std::vector<int64_t>& rep_loc,
-                       std::vector<std::complex<double> >& state_ph, std::vector<int64_t>& Ia,
-                       std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile) {
+namespace {
+
+void construct_impl(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
+                    const std::vector<int>& T1, const std::vector<int>& T2, const std::vector<int>& T3, int num_ones,
+                    double px, double py, double pz, int64_t& n_states, std::vector<int64_t>& rep_loc,
+                    std::vector<cplx>& state_ph, std::vector<int64_t>& Ia, std::vector<int64_t>& Ib, int& bits_right,
+                    std::ofstream& outfile) {
     if ((int)T1.size() < num_ones) fatal(outfile, "ERROR: Num sites < Num_ones");
-    Sector& S = sector(T1, T2, num_ones, px, py, outfile);
+    Sector& S = sector(T1, T2, T3, num_ones, px, py, pz, outfile);
     export_basis(S, v_sz, norm, norm_inv, outfile);
     n_states = S.D;
     int64_t C = 0, nl = 0;
@@ -311,6 +400,27 @@ void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, st
     check(ed_basis_export_full(S.basis, rep_loc.data(), (double*)state_ph.data(), Ia.data(), Ib.data(), &br), outfile,
           "ed_basis_export_full");
     bits_right = br;
+}
+
+}  // namespace
+
+void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
+                       std::vector<int>& T1, std::vector<int>& T2, int& num_ones, double& px, double& py,
+                       int64_t& n_states, std::vector<int64_t>& rep_loc,
+                       std::vector<std::complex<double> >& state_ph, std::vector<int64_t>& Ia,
+                       std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile) {
+    static const std::vector<int> none;
+    construct_impl(v_sz, norm, norm_inv, T1, T2, none, num_ones, px, py, 0.0, n_states, rep_loc, state_ph, Ia, Ib,
+                   bits_right, outfile);
+}
+
+void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
+                       std::vector<int>& T1, std::vector<int>& T2, std::vector<int>& T3, int& num_ones, double& px,
+                       double& py, double& pz, int64_t& n_states, std::vector<int64_t>& rep_loc,
+                       std::vector<std::complex<double> >& state_ph, std::vector<int64_t>& Ia,
+                       std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile) {
+    construct_impl(v_sz, norm, norm_inv, T1, T2, T3, num_ones, px, py, pz, n_states, rep_loc, state_ph, Ia, Ib,
+                   bits_right, outfile);
 }
 
 // ---------------------------------------------------------- exact_evecs ----
@@ -349,31 +459,58 @@ void lanczos_evecs(std::vector<int64_t>& pblock_states, std::vector<double>&, st
     run_lanczos(op, S.D, n_ev, iterations, ncycles, tol, ED_LZ_REFERENCE, eigs, previous_evecs, outfile);
 }
 
+// lanczos.h:38-69 (three generators)
+void lanczos_evecs(std::vector<int64_t>& pblock_states, std::vector<double>&, std::vector<double>&,
+                   std::vector<int64_t>&, std::vector<std::complex<double> >&, std::vector<std::complex<double> >& eigs,
+                   std::vector<std::vector<std::complex<double> > >& previous_evecs, bool, double& lambda, int&,
+                   double& px, double& py, double& pz, int& n_ones, int64_t&, std::vector<coordinates>& adj_list,
+                   double& Ch, std::vector<triangles>& ch_list, double& ChCh, std::vector<bowties>& chch_list,
+                   std::vector<int>& T1, std::vector<int>& T2, std::vector<int>& T3, int n_ev, double tol,
+                   int iterations, int ncycles, std::vector<int64_t>&, std::vector<int64_t>&, int,
+                   std::ofstream& outfile) {
+    Sector& S = sector(T1, T2, T3, n_ones, px, py, pz, outfile);
+    if (S.D != (int64_t)pblock_states.size()) fatal(outfile, "lanczos_evecs: basis mismatch");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    run_lanczos(op, S.D, n_ev, iterations, ncycles, tol, ED_LZ_REFERENCE, eigs, previous_evecs, outfile);
+}
+
 // ------------------------------------------------------- sector_lanczos ----
-void sector_lanczos(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
-                    std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
-                    std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2, int& n_sec_ev,
-                    bool& find_ev, bool& unwrap_peigvec, std::string& filepath_EV, double& tol, int& iterations,
-                    int& ncycles, std::ofstream& outfile) {
+namespace {
+
+// Lanczos_sector.cpp:15-198 (two generators, T3 empty) and :201-367 (three).
+void sector_lanczos_impl(int kx, int ky, int kz, double cSz, int n_spins, double lambda, double J2,
+                         std::vector<coordinates>& adj_list, double Ch, std::vector<triangles>& ch_list, double ChCh,
+                         std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
+                         const std::vector<int>& T3, int n_sec_ev, bool find_ev, bool unwrap_peigvec,
+                         const std::string& filepath_EV, double tol, int iterations, int ncycles,
+                         std::ofstream& outfile) {
+    const bool three = !T3.empty();
     int n_ones = (int)(cSz + n_spins / 2 + 1.0e-6);   // Lanczos_sector.cpp:45-46 (integer n_spins/2)
     ed_lattice lat = nullptr;
-    if (T1.size() != T2.size()) fatal(outfile, "T1 and T2 lengths differ");
-    check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), outfile, "ed_lattice_create");
-    int32_t L1, L2;
+    if (T1.size() != T2.size() || (three && T3.size() != T1.size())) fatal(outfile, "T1, T2, T3 lengths differ");
+    check(ed_lattice_create3((int32_t)T1.size(), T1.data(), T2.data(), three ? T3.data() : nullptr, &lat), outfile,
+          "ed_lattice_create3");
+    int32_t L1, L2, L3;
     ed_lattice_orders(lat, &L1, &L2);
+    ed_lattice_order3(lat, &L3);
     ed_lattice_destroy(lat);
-    if (kx >= L1 || ky >= L2 || kx < 0 || ky < 0) fatal(outfile, "Invalid kx or ky");
-    double px = 2 * kx * M_PI / L1, py = 2 * ky * M_PI / L2;
+    if (kx >= L1 || ky >= L2 || kz >= L3 || kx < 0 || ky < 0 || kz < 0) fatal(outfile, "Invalid kx or ky");
+    double px = 2 * kx * M_PI / L1, py = 2 * ky * M_PI / L2, pz = 2 * kz * M_PI / L3;
     double Sz = 0.5 * n_ones - 0.5 * (n_spins - n_ones);
     outfile << "\n--------------------------------------------------------------------\n";
-    Sector& S = sector(T1, T2, n_ones, px, py, outfile);
-    outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << ")\t" << S.D << std::endl;
+    Sector& S = sector(T1, T2, T3, n_ones, px, py, pz, outfile);
+    if (three)
+        outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << "," << pz << ")\t" << S.D << std::endl;
+    else
+        outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << ")\t" << S.D << std::endl;
     if (S.D == 0) fatal(outfile, "No states in psector");
     outfile << "------------------------------------------------------------------" << std::endl;
     ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
     std::vector<cplx> eigs;
     std::vector<std::vector<cplx>> vecs;
-    if (S.D > 10000) {   // Lanczos_sector.cpp:105
+    // Lanczos_sector.cpp:105: dense below 10000 states (two generators); the
+    // three-generator entry always runs Lanczos (:284-289)
+    if (three || S.D > 10000) {
         run_lanczos(op, S.D, n_sec_ev, iterations, ncycles, tol, ED_LZ_REFERENCE, eigs, vecs, outfile);
     } else {
         std::vector<double> ev(S.D);
@@ -385,19 +522,40 @@ void sector_lanczos(int& kx, int& ky, double& cSz, int& n_spins, double& lambda,
                 vecs.emplace_back(dv.begin() + (size_t)i * S.D, dv.begin() + (size_t)(i + 1) * S.D);
     }
     outfile << "pblock_eigs.size() = " << eigs.size() << std::endl;
-    outfile << "\n\nEnergies\t\tEnergies/site\t\tSz\tkx\tky\tJz\tJ2\n\n";
-    for (size_t i = 0; i < eigs.size(); i++)
+    if (three)
+        outfile << "\n\nEnergies\t\tEnergies/site\t\tSz\tkx\tky\tkz\tJz\tJ2\n\n";
+    else
+        outfile << "\n\nEnergies\t\tEnergies/site\t\tSz\tkx\tky\tJz\tJ2\n\n";
+    for (size_t i = 0; i < eigs.size(); i++) {
         outfile << std::fixed << std::setprecision(15) << eigs[i].real() << "\t" << eigs[i].real() / double(n_spins)
-                << "\t" << std::setprecision(3) << Sz << "\t" << px << "\t" << py << "\t" << lambda << "\t" << J2
-                << "\n";
+                << "\t" << std::setprecision(3) << Sz << "\t" << px << "\t" << py << "\t";
+        if (three) outfile << pz << "\t";
+        outfile << lambda << "\t" << J2 << "\n";
+    }
     outfile.flush();
-    if (find_ev) write_vectors(filepath_EV, unwrap_peigvec, n_sec_ev, kx, ky, Sz, lambda, eigs, vecs, S, px, py, outfile);
+    if (find_ev)
+        write_vectors(filepath_EV, unwrap_peigvec, n_sec_ev, kx, ky, kz, Sz, lambda, eigs, vecs, S, px, py, pz, outfile);
 }
 
-void sector_lanczos(int&, int&, int&, double&, int&, double&, double&, std::vector<coordinates>&, double&,
-                    std::vector<triangles>&, double&, std::vector<bowties>&, std::vector<int>&, std::vector<int>&,
-                    std::vector<int>&, int&, bool&, bool&, std::string&, double&, int&, int&, std::ofstream& outfile) {
-    fatal(outfile, "sector_lanczos: three-generator (3-D) translation groups are not supported by the GPU library");
+}  // namespace
+
+void sector_lanczos(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
+                    std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
+                    std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2, int& n_sec_ev,
+                    bool& find_ev, bool& unwrap_peigvec, std::string& filepath_EV, double& tol, int& iterations,
+                    int& ncycles, std::ofstream& outfile) {
+    static const std::vector<int> none;
+    sector_lanczos_impl(kx, ky, 0, cSz, n_spins, lambda, J2, adj_list, Ch, ch_list, ChCh, chch_list, T1, T2, none,
+                        n_sec_ev, find_ev, unwrap_peigvec, filepath_EV, tol, iterations, ncycles, outfile);
+}
+
+void sector_lanczos(int& kx, int& ky, int& kz, double& cSz, int& n_spins, double& lambda, double& J2,
+                    std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
+                    std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
+                    std::vector<int>& T3, int& n_sec_ev, bool& find_ev, bool& unwrap_peigvec,
+                    std::string& filepath_EV, double& tol, int& iterations, int& ncycles, std::ofstream& outfile) {
+    sector_lanczos_impl(kx, ky, kz, cSz, n_spins, lambda, J2, adj_list, Ch, ch_list, ChCh, chch_list, T1, T2, T3,
+                        n_sec_ev, find_ev, unwrap_peigvec, filepath_EV, tol, iterations, ncycles, outfile);
 }
 
 // ----------------------------------------------------------- ARPACK path ----

@@ -57,28 +57,33 @@ namespace edc {
 
 // exp(-i(px lx + py ly)) for lx < L1, ly < L2, the expression of
 // Momentum_Basis.cpp:753, indexed lx*L2 + ly.
-inline std::vector<cd2> phase_table(int L1, int L2, double px, double py) {
+inline std::vector<cd2> phase_table(int L1, int L2, int L3, double px, double py, double pz) {
 #if defined(__clang__)
 #pragma clang fp contract(off)   // no FMA: round -px*lx and py*ly separately, as g++ does
 #endif
-    std::vector<cd2> ph(L1 * L2);
+    std::vector<cd2> ph((size_t)L1 * L2 * L3);
     for (int lx = 0; lx < L1; lx++)
-        for (int ly = 0; ly < L2; ly++) {
-            double _Complex z;
-            __real__ z = 0.0;
-            __imag__ z = -px * lx - py * ly;
-            double _Complex e = cexp(z);
-            ph[lx * L2 + ly] = cd2{__real__ e, __imag__ e};
-        }
+        for (int ly = 0; ly < L2; ly++)
+            for (int lz = 0; lz < L3; lz++) {
+                double _Complex z;
+                __real__ z = 0.0;
+                // the two-generator expression (Momentum_Basis.cpp:753) unless a
+                // third generator is present (:872)
+                __imag__ z = L3 == 1 ? -px * lx - py * ly : -px * lx - py * ly - pz * lz;
+                double _Complex e = cexp(z);
+                ph[((size_t)lx * L2 + ly) * L3 + lz] = cd2{__real__ e, __imag__ e};
+            }
     return ph;
 }
 
-// norm = sqrt(L1*L2/lid), 1/norm as Momentum_Basis.cpp:764-766, indexed by lid.
-inline void norm_tables(int L1, int L2, std::vector<double>& nt, std::vector<double>& nit) {
-    nt.assign(L1 * L2 + 1, 0.0);
-    nit.assign(L1 * L2 + 1, 0.0);
-    for (int lid = 1; lid <= L1 * L2; lid++) {
-        int64_t orb = (int64_t)L1 * L2 / lid;
+// norm = sqrt(L1*L2*L3/lid) with an integer quotient, 1/norm, as
+// Momentum_Basis.cpp:764-766 (three generators :883-885), indexed by lid.
+inline void norm_tables(int L1, int L2, int L3, std::vector<double>& nt, std::vector<double>& nit) {
+    const int M = L1 * L2 * L3;
+    nt.assign(M + 1, 0.0);
+    nit.assign(M + 1, 0.0);
+    for (int lid = 1; lid <= M; lid++) {
+        int64_t orb = (int64_t)M / lid;
         nt[lid] = std::sqrt((double)orb);
         nit[lid] = 1 / std::sqrt((double)orb);
     }

@@ -355,16 +355,12 @@ namespace edc {
 // Decodes a directory entry fetched for rank r (see dir_lookup).
 // Branch-free (word select by masks, so the entry stays in registers).
 ED_HD int64_t dir_decode_off(const DirEntry& e, unsigned o) {
-    const unsigned wsel = o >> 5, bit = o & 31u;
+    const bool hi = (o >> 5) != 0;
+    const unsigned bit = o & 31u;
     const uint32_t low = (1u << bit) - 1u;
-    const uint32_t s0 = wsel == 0 ? 0xffffffffu : 0u, s1 = wsel == 1 ? 0xffffffffu : 0u,
-                   s2 = wsel == 2 ? 0xffffffffu : 0u;
-    const uint32_t bits = (e.b0 & s0) | (e.b1 & s1) | (e.b2 & s2);
-    // ones below position o: whole words before wsel, the low bits of word wsel
-    const uint32_t m0 = s0 ? low : 0xffffffffu;
-    const uint32_t m1 = s0 ? 0u : (s1 ? low : 0xffffffffu);
-    const uint32_t m2 = s2 ? low : 0u;
-    const int64_t idx = (int64_t)(e.base + popc32(e.b0 & m0) + popc32(e.b1 & m1) + popc32(e.b2 & m2));
+    const uint32_t bits = hi ? e.b1 : e.b0;
+    // ones below position o: all of b0 and the low bits of b1, or the low bits of b0
+    const int64_t idx = (int64_t)(e.base + popc32(e.b0 & (hi ? 0xffffffffu : low)) + popc32(e.b1 & (hi ? low : 0u)));
     return ((bits >> bit) & 1u) ? idx : int64_t(-1);
 }
 

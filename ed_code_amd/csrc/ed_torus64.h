@@ -13,8 +13,7 @@
 //
 // Ranks reach C(40,20) = 1.4e11: the lower bytes' table entries fit 32 bits
 // (their sum is a rank among 32-bit values), the top byte's entries are
-// 64-bit.  Directory index rank/96 is formed through a double and corrected
-// by one step.
+// 64-bit.
 #pragma once
 
 #include <algorithm>
@@ -110,15 +109,6 @@ ED_HD uint64_t rank64(uint64_t s, const RankInfo& R, const uint32_t* lo, const u
     return (uint64_t)r + hi[(unsigned)(s >> (8 * (TC::nbytes - 1))) & 255u];
 }
 
-// q = r / 96, o = r % 96 for r < 2^40
-ED_HD void div96(uint64_t r, uint32_t& q, uint32_t& o) {
-    const uint64_t x = r >> 5;
-    uint32_t qe = (uint32_t)((double)x * (1.0 / 3.0));   // floor(x/3) or one below
-    if (x - 3ull * qe >= 3ull) qe++;
-    q = qe;
-    o = (uint32_t)r - 96u * qe;
-}
-
 // One row with K elements in flight (same scheme as torus_row_batched).
 template <class TC, bool TRIV, int K, class LoadU, class Gf>
 ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff, const TEntry* ent, double ezz0,
@@ -172,9 +162,8 @@ ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff,
         uint32_t off[K];
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            uint32_t gi;
-            div96(rk[q], gi, off[q]);
-            e[q] = dir[gi];
+            off[q] = (uint32_t)(rk[q] % kDirSpan);
+            e[q] = dir[rk[q] / kDirSpan];
         }
         bool use[K];
         int64_t j[K];

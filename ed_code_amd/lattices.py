@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -32,6 +32,7 @@ class Model:
     ChCh: float = 0.0
     bowties: List[Tuple[int, int, int, int, int, int]] = field(default_factory=list)
     triangles: List[Tuple[int, int, int]] = field(default_factory=list)
+    T3: Optional[np.ndarray] = None       # third translation (3-D clusters), else None
 
     # ---------------------------------------------------------------- arrays
     def term_arrays(self):
@@ -67,6 +68,16 @@ class Model:
                 if s == 0:
                     return n
         return cyc(self.T1), cyc(self.T2)
+
+    def order3(self) -> int:
+        """L3 (LxLyLz, Momentum_Basis.cpp:38-64); 1 without a third translation."""
+        if self.T3 is None:
+            return 1
+        s, n = 0, 0
+        while True:
+            s = int(self.T3[s]); n += 1
+            if s == 0:
+                return n
 
     def momentum(self, kx: int, ky: int) -> Tuple[float, float]:
         """(px, py) exactly as Lanczos_sector.cpp:57 forms them."""
@@ -342,6 +353,30 @@ def triangular(Lx: int, Ly: int, J1: float = 1.0, lam: float = 1.0, Ch: float = 
     if ChCh != 0.0:
         m.bowties = bowtie_entries(tris, bonds)
     return m
+
+
+def cubic(Lx: int, Ly: int, Lz: int, J1: float = 1.0, J2: float = 0.0, lam: float = 1.0) -> Model:
+    """Lx x Ly x Lz simple-cubic torus, site x + Lx*(y + Ly*z); T1, T2, T3
+    translate by one along x, y, z.  J1 on nearest neighbours, J2 on the
+    face diagonals of the xy planes."""
+    N = Lx * Ly * Lz
+    site = lambda x, y, z: (x % Lx) + Lx * ((y % Ly) + Ly * (z % Lz))
+    T1 = np.zeros(N, np.int32)
+    T2 = np.zeros(N, np.int32)
+    T3 = np.zeros(N, np.int32)
+    bonds = []
+    for z in range(Lz):
+        for y in range(Ly):
+            for x in range(Lx):
+                s = site(x, y, z)
+                T1[s], T2[s], T3[s] = site(x + 1, y, z), site(x, y + 1, z), site(x, y, z + 1)
+                bonds.append((s, site(x + 1, y, z), J1, complex(1.0, 0.0)))
+                bonds.append((s, site(x, y + 1, z), J1, complex(1.0, 0.0)))
+                bonds.append((s, site(x, y, z + 1), J1, complex(1.0, 0.0)))
+                if J2 != 0.0:
+                    bonds.append((s, site(x + 1, y + 1, z), J2, complex(1.0, 0.0)))
+                    bonds.append((s, site(x + 1, y - 1, z), J2, complex(1.0, 0.0)))
+    return Model(f"cubic{Lx}x{Ly}x{Lz}", N, T1, T2, lam=lam, bonds=bonds, T3=T3)
 
 
 def tilted_square(P1: Tuple[int, int], P2: Tuple[int, int], J1: float = 1.0, J2: float = 0.0,

@@ -11,7 +11,9 @@ requests:
   term structure are solved together by one block operator (M = 2, 4 or 8
   instances: one H*V pass serves all of them);
 * concurrency -- several worker threads, each solve on its own HIP stream,
-  so small sectors do not leave the GPU idle between Lanczos steps;
+  so small sectors do not leave the GPU idle between Lanczos steps; a lone
+  lowest-eigenvalue request runs the graph-captured solver (no host round
+  trip per step);
 * HBM admission -- a job starts only when its footprint (basis, Krylov
   vectors, work vectors) fits next to the running jobs under the budget.
 
@@ -34,7 +36,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from . import ED_C64, Basis, BlockOperator, Lattice, init
+from . import ED_C64, ED_LZ_LEAN, Basis, BlockOperator, Lattice, Operator, init
 from . import lattices as L
 from .lattices import Model
 
@@ -165,9 +167,18 @@ class BasisCache:
 
 
 def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
+    """One job: a single request for the lowest eigenvalue runs the
+    graph-captured three-vector Lanczos; batches and nev > 1 run the block
+    full-reorthogonalisation Lanczos."""
     models = [model_for(r.lattice, r.J2, r.lam, r.Ch) for r in reqs]
-    blk = BlockOperator(basis, models)
-    ev, _ = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
+    if len(reqs) == 1 and reqs[0].nev == 1:
+        op = Operator(basis, models[0])
+        e, _ = op.lanczos(nev=1, iterations=max(iterations, 400), ncycles=1, tol=reqs[0].tol, precision=ED_C64,
+                          mode=ED_LZ_LEAN, graph=True)
+        ev = np.array([[e[0]]])
+    else:
+        blk = BlockOperator(basis, models)
+        ev, _ = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
     now = time.perf_counter()
     for m, r in enumerate(reqs):
         r.evals = [float(x) for x in ev[m]]

@@ -65,7 +65,13 @@ int ed_init(int device);
  * Fails with ED_EINVAL when T1/T2 are not permutations of the same length
  * (read_input_file.cpp:423-427) or do not commute. */
 int ed_lattice_create(int32_t n_sites, const int32_t* T1, const int32_t* T2, ed_lattice* out);
+/* Three generators (read_input_file.cpp T3= key, ED_3D_Lattice_Geometry;
+ * LxLyLz, Momentum_Basis.cpp:38-64); T3 = NULL is the identity, which is
+ * ed_lattice_create. */
+int ed_lattice_create3(int32_t n_sites, const int32_t* T1, const int32_t* T2, const int32_t* T3, ed_lattice* out);
 int ed_lattice_orders(ed_lattice lat, int32_t* L1, int32_t* L2);
+/* L3 (1 for two-generator lattices). */
+int ed_lattice_order3(ed_lattice lat, int32_t* L3);
 void ed_lattice_destroy(ed_lattice lat);
 
 /* Momentum basis of the sector (n_ones, kx, ky), built on the GPU.
@@ -74,6 +80,11 @@ void ed_lattice_destroy(ed_lattice lat);
  * directory stay resident in HBM.  kx, ky are integers in [0,L1) x [0,L2)
  * (Lanczos_sector.cpp:51-57); px = 2 pi kx / L1, py = 2 pi ky / L2. */
 int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out);
+/* Three-generator sector (pConstruct_States, Momentum_Basis.cpp:781-897,
+ * decl Momentum_Basis.h:68-100): the reference's walk over L1 x L2 x L3 with
+ * cumulative T2 and T3 counts, pz = 2 pi kz / L3 (Lanczos_sector.cpp:240).
+ * H*v, make_h and the solvers take such a basis through the generic kernel. */
+int ed_basis_build3(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, int32_t kz, ed_basis* out);
 int ed_basis_dim(ed_basis b, int64_t* D);
 /* Host copies of pblock_states, norm, norm_inv ([D] each; any may be NULL). */
 int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* norm_inv);

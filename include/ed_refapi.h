@@ -63,6 +63,23 @@ void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int
          const std::vector<int>& T1, const std::vector<int>& T2, const double& kx, const double& ky,
          std::ofstream& outfile);
 
+// H_v.h:59-83 (three generators; kz is pz in radians).
+void sHv(std::complex<double>* v, std::complex<double>* w, const std::vector<int64_t>& pblock_states,
+         const std::vector<double>& norm, const std::vector<double>& norm_inv, const std::vector<int64_t>& rep_loc,
+         const std::vector<std::complex<double> >& state_ph, const double& lambda,
+         const std::vector<coordinates>& adj_list, const double& Ch, const std::vector<triangles>& ch_list,
+         const double& ChCh, const std::vector<bowties>& chch_list, const int& n_sites,
+         const std::vector<int64_t>& Ia, const std::vector<int64_t>& Ib, const int& bits_right,
+         const std::vector<int>& T1, const std::vector<int>& T2, const std::vector<int>& T3, const double& kx,
+         const double& ky, const double& kz, std::ofstream& outfile);
+
+// Momentum_Basis.h:84-100 (three generators).
+void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
+                       std::vector<int>& T1, std::vector<int>& T2, std::vector<int>& T3, int& num_ones, double& px,
+                       double& py, double& pz, int64_t& n_states, std::vector<int64_t>& rep_loc,
+                       std::vector<std::complex<double> >& state_ph, std::vector<int64_t>& Ia,
+                       std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile);
+
 // Momentum_Basis.h:151-165 (2-D).
 void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, std::vector<double>& norm_inv,
                        std::vector<int>& T1, std::vector<int>& T2, int& num_ones, double& px, double& py,
@@ -79,6 +96,17 @@ void exact_evecs(std::vector<int64_t>& pblock_states, std::vector<double>& Norm,
                  std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2, int n_ev, double tol,
                  int iterations, int ncycles, std::vector<int64_t>& Ia, std::vector<int64_t>& Ib, int bits_right,
                  std::ofstream& outfile);
+
+// lanczos.h:38-69 (three generators)
+void lanczos_evecs(std::vector<int64_t>& pblock_states, std::vector<double>& Norm, std::vector<double>& Norm_inv,
+                   std::vector<int64_t>& rep_loc, std::vector<std::complex<double> >& state_ph,
+                   std::vector<std::complex<double> >& eigs,
+                   std::vector<std::vector<std::complex<double> > >& previous_evecs, bool find_ev, double& lambda,
+                   int& n_spins, double& px, double& py, double& pz, int& n_ones, int64_t& n_p,
+                   std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
+                   std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
+                   std::vector<int>& T3, int n_ev, double tol, int iterations, int ncycles,
+                   std::vector<int64_t>& Ia, std::vector<int64_t>& Ib, int bits_right, std::ofstream& outfile);
 
 // lanczos.h:71-99 (2-D)
 void lanczos_evecs(std::vector<int64_t>& pblock_states, std::vector<double>& Norm, std::vector<double>& Norm_inv,
@@ -98,9 +126,8 @@ void sector_lanczos(int& kx, int& ky, double& cSz, int& n_spins, double& lambda,
                     bool& find_ev, bool& unwrap_peigvec, std::string& filepath_EV, double& tol, int& iterations,
                     int& ncycles, std::ofstream& outfile);
 
-// Lanczos_sector.h:33-55 (3-D).  Three-generator translation groups are not
-// built: logs the fact and exits(1).  The reference's ED.cpp:69 fixes dim=2,
-// so its main never reaches this entry; it exists so that main links.
+// Lanczos_sector.h:33-55 (3-D): the three-generator sector, always solved by
+// Lanczos (Lanczos_sector.cpp:284-289); eigenvector files carry kz and pz.
 void sector_lanczos(int& kx, int& ky, int& kz, double& cSz, int& n_spins, double& lambda, double& J2,
                     std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list, double& ChCh,
                     std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,

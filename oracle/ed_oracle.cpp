@@ -14,6 +14,8 @@
 //   or_lin_tables/or_rank  closed-form combinatorial-number-system rank; _ref bit-exact
 //   or_translate/or_LxLy   permutation composition / orders on known lattices
 //   or_representative      brute-force orbit minimum; _ref bit-exact (rep, lx, ly, lid)
+//   or_representative3 /   three-generator forms: brute-force orbit minimum, Burnside
+//   or_basis_build3        dimensions summing to C(N,n); _ref bit-exact
 //   or_basis_*             Burnside sector dimensions, sum_k D_k = C(N,n); _ref bit-exact
 //   or_hv                  _ref (<=1e-14), Hermiticity, dense brute-force projection
 //   or_make_h              _ref bit-exact, numpy eigvalsh spectra, closed forms
@@ -158,10 +160,61 @@ OR_API int64_t or_representative(int64_t v, const int32_t* T1, const int32_t* T2
     return rep;
 }
 
+// Momentum_Basis.cpp:38-64 (LxLyLz).
+OR_API void or_LxLyLz(const int32_t* T1, const int32_t* T2, const int32_t* T3, int N, int32_t* L1, int32_t* L2,
+                      int32_t* L3) {
+    or_LxLy(T1, T2, N, L1, L2);
+    int c = 0;
+    int64_t t = 1;
+    do { t = or_translate(t, T3, N); c++; } while (t != 1);
+    *L3 = c;
+}
+
+// Momentum_Basis.cpp:545-643 (representative_p, three generators): the walk
+// nests T3 inside T2 inside T1 with the T2 and T3 counts running on; rep, lid,
+// p_id as in the two-generator form, (lx,ly,lz) = (L1-tx, L2-ty, L3-tz) at the
+// first strict new minimum.
+OR_API int64_t or_representative3(int64_t v, const int32_t* T1, const int32_t* T2, const int32_t* T3, int N,
+                                  int L1, int L2, int L3, double px, double py, double pz, int32_t* lx_out,
+                                  int32_t* ly_out, int32_t* lz_out, int32_t* pid_out, int32_t* lid_out) {
+    int lx = 0, ly = 0, lz = 0;
+    int64_t rep = v, t = v;
+    int tx = 0, ty = 0, tz = 0;
+    cd ph(1.0, 0.0), idc(1.0, 0.0);
+    int lid = 1;
+    auto visit = [&]() {
+        if (v == t) { idc = idc + ph; lid++; }
+        if (rep > t) { rep = t; lx = L1 - tx; ly = L2 - ty; lz = L3 - tz; }
+    };
+    for (int x = 0; x < L1; x++) {
+        if (x > 0) {
+            t = or_translate(t, T1, N); tx++;
+            ph = ph * std::exp(cd(0.0, -px));
+            visit();
+        }
+        for (int y = 0; y < L2; y++) {
+            if (y > 0) {
+                t = or_translate(t, T2, N); ty++;
+                ph = ph * std::exp(cd(0.0, -py));
+                visit();
+            }
+            for (int z = 1; z < L3; z++) {
+                t = or_translate(t, T3, N); tz++;
+                ph = ph * std::exp(cd(0.0, -pz));
+                visit();
+            }
+        }
+    }
+    *pid_out = (std::fabs(idc.real()) > 1e-8 || std::fabs(idc.imag()) > 1e-8) ? 1 : 0;
+    *lid_out = lid;
+    *lx_out = lx; *ly_out = ly; *lz_out = lz;
+    return rep;
+}
+
 // ---------------------------------------------------------------- basis ----
 struct OrBasis {
-    int N, n, L1, L2, half, bits_right;
-    double px, py;
+    int N, n, L1, L2, L3 = 1, half, bits_right;
+    double px, py, pz = 0;
     int64_t C, D;
     std::vector<int64_t> reps;     // pblock_states
     std::vector<double> norm, norm_inv;
@@ -235,6 +288,73 @@ OR_API void* or_basis_build(int N, int n_ones, const int32_t* T1, const int32_t*
     }
     return B;
 }
+
+// Momentum_Basis.cpp:781-897 (pConstruct_States, three generators): as the
+// two-generator form with representative_p over L1 x L2 x L3, state_ph =
+// exp(i(-px lx - py ly - pz lz)) (:872) and norm = sqrt(L1 L2 L3 / lid) (:883).
+OR_API void* or_basis_build3(int N, int n_ones, const int32_t* T1, const int32_t* T2, const int32_t* T3, double px,
+                             double py, double pz) {
+    OrBasis* B = new OrBasis();
+    B->N = N; B->n = n_ones; B->px = px; B->py = py; B->pz = pz;
+    B->half = N / 2 + (N % 2);
+    or_LxLyLz(T1, T2, T3, N, &B->L1, &B->L2, &B->L3);
+    const int L1 = B->L1, L2 = B->L2, L3 = B->L3;
+    const int64_t C = or_n_choose_k(N, n_ones);
+    B->C = C;
+    std::vector<int64_t> all(C), repid(C);
+    std::vector<char> pids(C);
+    std::vector<int> lids(C);
+    B->rep_loc.resize(C);
+    B->state_ph.resize(C);
+    B->Ia.resize(int64_t(1) << B->half);
+    B->Ib.resize(int64_t(1) << B->half);
+    or_lin_tables(B->half, B->Ia.data(), B->Ib.data());
+    B->bits_right = (int)(std::pow(2.0, B->half)) - 1;
+    or_gosper(N, n_ones, all.data());
+    int err = 0;
+    for (int64_t i = 0; i < C; i++) {
+        int64_t left, right;
+        split_id(all[i], left, right, B->bits_right, N);
+        if (B->Ib[left] == -1) B->Ib[left] = i - B->Ia[right] + 1;
+        else if (i != B->Ib[left] + B->Ia[right] - 1) err = 1;
+    }
+    B->lin_error = err;
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < C; i++) {
+        int32_t lx, ly, lz, pid, lid;
+        repid[i] = or_representative3(all[i], T1, T2, T3, N, L1, L2, L3, px, py, pz, &lx, &ly, &lz, &pid, &lid);
+        pids[i] = (char)pid; lids[i] = lid;
+        lx = lx % L1; ly = ly % L2; lz = lz % L3;
+        if (lx < 0) lx += L1;
+        if (ly < 0) ly += L2;
+        if (lz < 0) lz += L3;
+        B->state_ph[i] = std::exp(cd(0.0, -px * lx - py * ly - pz * lz));
+    }
+    for (int64_t i = 0; i < C; i++) {
+        if (repid[i] == all[i] && pids[i]) {
+            B->reps.push_back(all[i]);
+            int64_t orb = (int64_t)L1 * L2 * L3 / lids[i];
+            B->norm.push_back(std::sqrt((double)orb));
+            B->norm_inv.push_back(1 / std::sqrt((double)orb));
+        }
+    }
+    B->D = (int64_t)B->reps.size();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < C; i++) {
+        int64_t r = repid[i], j = -1;
+        int64_t lo = 0, hi = B->D - 1;
+        while (hi >= lo) {
+            int64_t m = lo + (hi - lo) / 2;
+            if (r < B->reps[m]) hi = m - 1;
+            else if (r > B->reps[m]) lo = m + 1;
+            else { j = m; break; }
+        }
+        B->rep_loc[i] = j;
+    }
+    return B;
+}
+
+OR_API int32_t or_basis_l3(void* h) { return ((OrBasis*)h)->L3; }
 
 OR_API void or_basis_free(void* h) { delete (OrBasis*)h; }
 

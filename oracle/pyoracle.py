@@ -7,6 +7,7 @@ __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -61,6 +62,13 @@ def lib():
                                         C.POINTER(C.c_int32)]
         L.or_basis_build.restype = C.c_void_p
         L.or_basis_build.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double]
+        L.or_basis_build3.restype = C.c_void_p
+        L.or_basis_build3.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_double, C.c_double, C.c_double]
+        L.or_basis_l3.restype = C.c_int32
+        L.or_basis_l3.argtypes = [C.c_void_p]
+        L.or_representative3.restype = C.c_int64
+        L.or_representative3.argtypes = [C.c_int64, i32p, i32p, i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                         C.c_double, C.c_double] + [C.POINTER(C.c_int32)] * 5
         L.or_basis_free.argtypes = [C.c_void_p]
         L.or_basis_info.argtypes = [C.c_void_p, i64p]
         L.or_basis_lin_error.restype = C.c_int32
@@ -98,6 +106,8 @@ def ref():
                                          C.POINTER(C.c_int32)]
         L.ref_basis_build.restype = C.c_void_p
         L.ref_basis_build.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double]
+        L.ref_basis_build3.restype = C.c_void_p
+        L.ref_basis_build3.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, C.c_double, C.c_double, C.c_double]
         L.ref_basis_free.argtypes = [C.c_void_p]
         L.ref_basis_info.argtypes = [C.c_void_p, i64p]
         L.ref_basis_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 7
@@ -118,13 +128,20 @@ def _ptr(a):
 class OracleBasis:
     """Oracle restatement of pConstruct_States (Momentum_Basis.cpp:645-778)."""
 
-    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0):
+    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0, kz: int = 0):
         self.model = model
         self.n_ones = n_ones
         self.px, self.py = model.momentum(kx, ky)
         L = lib()
-        self.h = L.or_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
-                                  np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
+        if getattr(model, "T3", None) is not None:
+            # three generators: Momentum_Basis.cpp:781-897, pz as Lanczos_sector.cpp:240
+            self.pz = float(2 * kz) * math.pi / model.order3()
+            self.h = L.or_basis_build3(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                       np.ascontiguousarray(model.T2, np.int32),
+                                       np.ascontiguousarray(model.T3, np.int32), self.px, self.py, self.pz)
+        else:
+            self.h = L.or_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                      np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
         info = np.zeros(8, np.int64)
         L.or_basis_info(self.h, info)
         self.N, _, self.L1, self.L2, self.half, self.bits_right, self.C, self.D = [int(x) for x in info]
@@ -196,12 +213,18 @@ def start_vector(seed: int, D: int) -> np.ndarray:
 class RefBasis:
     """The reference's own pConstruct_States / sHv / makeH (compiled slice)."""
 
-    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0):
+    def __init__(self, model, n_ones: int, kx: int = 0, ky: int = 0, kz: int = 0):
         self.model = model
         self.px, self.py = model.momentum(kx, ky)
         R = ref()
-        self.h = R.ref_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
-                                   np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
+        if getattr(model, "T3", None) is not None:
+            self.pz = float(2 * kz) * math.pi / model.order3()
+            self.h = R.ref_basis_build3(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                        np.ascontiguousarray(model.T2, np.int32),
+                                        np.ascontiguousarray(model.T3, np.int32), self.px, self.py, self.pz)
+        else:
+            self.h = R.ref_basis_build(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                       np.ascontiguousarray(model.T2, np.int32), self.px, self.py)
         info = np.zeros(4, np.int64)
         R.ref_basis_info(self.h, info)
         self.D, self.C, self.nIa, self.bits_right = [int(x) for x in info]

@@ -32,8 +32,9 @@ struct RefBasis {
     int bits_right;
     int n_sites;
     int64_t n_p;
-    std::vector<int> T1, T2;
-    double px, py;
+    std::vector<int> T1, T2, T3;
+    double px, py, pz = 0;
+    bool three = false;
 };
 
 struct Terms {
@@ -123,6 +124,26 @@ void* ref_basis_build(int N, int n_ones, const int32_t* T1, const int32_t* T2, d
     return B;
 }
 
+// The three-generator pConstruct_States (Momentum_Basis.cpp:781-897), on
+// one thread for the same reason as above (:840,860).
+void* ref_basis_build3(int N, int n_ones, const int32_t* T1, const int32_t* T2, const int32_t* T3, double px,
+                       double py, double pz) {
+    RefBasis* B = new RefBasis();
+    B->T1.assign(T1, T1 + N);
+    B->T2.assign(T2, T2 + N);
+    B->T3.assign(T3, T3 + N);
+    B->px = px; B->py = py; B->pz = pz;
+    B->three = true;
+    B->n_sites = N;
+    int nn = n_ones;
+    int nt = omp_get_max_threads();
+    omp_set_num_threads(1);
+    pConstruct_States(B->reps, B->norm, B->norm_inv, B->T1, B->T2, B->T3, nn, B->px, B->py, B->pz, B->n_p,
+                      B->rep_loc, B->state_ph, B->Ia, B->Ib, B->bits_right, devnull());
+    omp_set_num_threads(nt);
+    return B;
+}
+
 // A RefBasis whose tables are filled from outside (e.g. the GPU library's
 // full-space export, which tests show bit-equal to pConstruct_States), so that
 // the reference sHv can be timed on sectors whose single-threaded build would
@@ -187,9 +208,14 @@ void ref_hv(void* h, const void* terms, const double* v, double* w, int64_t rows
         rp = &sub;
     }
     double lam = t->lambda, Ch = t->Ch, ChCh = t->ChCh;
-    sHv((std::complex<double>*)v, (std::complex<double>*)w, *rp, B->norm, B->norm_inv, B->rep_loc,
-        B->state_ph, lam, adj, Ch, ch, ChCh, bw, B->n_sites, B->Ia, B->Ib, B->bits_right, B->T1, B->T2,
-        B->px, B->py, devnull());
+    if (B->three)   // the three-generator overload (H_v.cpp:360-527)
+        sHv((std::complex<double>*)v, (std::complex<double>*)w, *rp, B->norm, B->norm_inv, B->rep_loc,
+            B->state_ph, lam, adj, Ch, ch, ChCh, bw, B->n_sites, B->Ia, B->Ib, B->bits_right, B->T1, B->T2,
+            B->T3, B->px, B->py, B->pz, devnull());
+    else
+        sHv((std::complex<double>*)v, (std::complex<double>*)w, *rp, B->norm, B->norm_inv, B->rep_loc,
+            B->state_ph, lam, adj, Ch, ch, ChCh, bw, B->n_sites, B->Ia, B->Ib, B->bits_right, B->T1, B->T2,
+            B->px, B->py, devnull());
 }
 
 // makeH (H_v.cpp:21-190): dense column-major D x D, interleaved complex128.

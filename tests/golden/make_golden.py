@@ -40,21 +40,43 @@ def cases():
     tl = L.tilted_square((3, 1), (-1, 3), J1=1.0, J2=0.5, lam=0.7)
     out += [("tilted10_n5_k00", tl, 5, 0, 0), ("tilted10_n5_k17", tl, 5, 1, 7), ("tilted10_n4_k55", tl, 4, 5, 5),
             ("tilted10_n5_k11_empty", tl, 5, 1, 1)]
+    # three translation generators (pConstruct_States / sHv 3-D overloads)
+    cu = L.cubic(2, 2, 3, J1=1.0, J2=0.3, lam=0.8)
+    out += [("cubic2x2x3_n6_k000", cu, 6, (0, 0, 0)), ("cubic2x2x3_n6_k102", cu, 6, (1, 0, 2)),
+            ("cubic2x2x3_n5_k011", cu, 5, (0, 1, 1))]
+    cu4 = L.cubic(2, 2, 4, J1=1.0, lam=1.1)
+    out += [("cubic2x2x4_n8_k013", cu4, 8, (0, 1, 3))]
     return out
+
+
+def _case(c):
+    """(name, model, n, (kx, ky, kz)) from either case form."""
+    if len(c) == 5:
+        return c[0], c[1], c[2], (c[3], c[4], 0)
+    return c[0], c[1], c[2], c[3]
 
 
 def main():
     if not P.ref_available():
         raise SystemExit("build the reference slice first: make -C oracle ref")
+    force = "--force" in sys.argv
     rng = np.random.default_rng(20261203)
-    for name, m, n, kx, ky in cases():
-        if P.OracleBasis(m, n, kx, ky).D == 0:
+    for c in cases():
+        name, m, n, (kx, ky, kz) = _case(c)
+        path = os.path.join(HERE, f"{name}.npz")
+        if os.path.exists(path) and not force:
+            # keep the recorded file; advance the generator as writing it did
+            D = int(np.load(path, allow_pickle=False)["reps"].shape[0])
+            rng.normal(size=D); rng.normal(size=D)
+            print(f"{name}: kept")
+            continue
+        if P.OracleBasis(m, n, kx, ky, kz).D == 0:
             # pConstruct_States reads pBasis[0] of an empty list in findstate
             # (Momentum_Basis.cpp:228-230,776); the reference exits on empty
             # sectors anyway (Lanczos_sector.cpp:93-97), so none is recorded.
             print(f"{name}: empty sector, skipped")
             continue
-        R = P.RefBasis(m, n, kx, ky)
+        R = P.RefBasis(m, n, kx, ky, kz)
         t = R.tables()
         D = R.D
         v = rng.normal(size=D) + 1j * rng.normal(size=D)
@@ -62,8 +84,9 @@ def main():
         H = R.make_h() if D else np.zeros((0, 0), complex)
         ev = np.linalg.eigvalsh(H) if D else np.zeros(0)
         a = m.term_arrays()
+        extra = {} if m.T3 is None else dict(T3=m.T3, kz=kz)
         np.savez_compressed(
-            os.path.join(HERE, f"{name}.npz"),
+            path, **extra,
             n_sites=m.n_sites, n_ones=n, kx=kx, ky=ky, T1=m.T1, T2=m.T2,
             lam=a["lam"], bond_xy=a["bond_xy"][: a["nbonds"]], bond_J=a["bond_J"][: a["nbonds"]],
             bond_phase=a["bond_phase"][: a["nbonds"]], Ch=a["Ch"], chiral_ijk=a["chiral_ijk"][: a["nchiral"]],

@@ -34,6 +34,9 @@ def load_golden(path):
     m.chiral = [(int(i), int(j), int(k), complex(p[0], p[1])) for (i, j, k), p in zip(d["chiral_ijk"], d["chiral_phase"])]
     m.ChCh = float(d["ChCh"])
     m.bowties = [tuple(int(q) for q in b) for b in d["bowtie"]]
+    if "T3" in d:
+        m.T3 = d["T3"].astype(np.int32)
+    d.setdefault("kz", np.int64(0))
     return m, d
 
 
@@ -199,6 +202,30 @@ def momentum_projector(model, n, kx, ky, st):
 _hc = None
 
 
+def drive3d():
+    """tests/refapi/drive3d.cpp (a caller of the three-generator entry points
+    of include/ed_refapi.h) linked against ed_code_amd/libed_amd.so."""
+    out = os.path.join(HERE, "refapi", "_build", "drive3d")
+    src = os.path.join(HERE, "refapi", "drive3d.cpp")
+    lib = os.path.join(ROOT, "ed_code_amd", "libed_amd.so")
+    if not os.path.exists(out) or os.path.getmtime(src) > os.path.getmtime(out):
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out,
+                        "-L", os.path.dirname(lib), "-led_amd", "-Wl,-rpath,$ORIGIN/../../../ed_code_amd"], check=True)
+    return out
+
+
+def write_spec3(path, model, n, k, nev=1, iterations=100, ncycles=10, tol=1e-10):
+    """Input of drive3d: sizes and solver settings, T1, T2, T3, bonds."""
+    with open(path, "w") as f:
+        f.write(f"{model.n_sites} {n} {k[0]} {k[1]} {k[2]} {model.lam!r} {iterations} {ncycles} {nev} {tol!r}\n")
+        for T in (model.T1, model.T2, model.T3):
+            f.write(" ".join(str(int(t)) for t in T) + "\n")
+        f.write(f"{len(model.bonds)}\n")
+        for x, y, J, ph in model.bonds:
+            f.write(f"{x} {y} {J!r} {ph.real!r} {ph.imag!r}\n")
+
+
 def hostcore():
     """tests/hostcore/hostcore.cpp + ed_core.h compiled with g++ (CPU)."""
     global _hc
@@ -214,6 +241,8 @@ def hostcore():
         vp = C.c_void_p
         H.hc_build.restype = vp
         H.hc_build.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int, C.c_int]
+        H.hc_build3.restype = vp
+        H.hc_build3.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_int]
         H.hc_free.argtypes = [vp]
         H.hc_dim.restype = C.c_int64
         H.hc_dim.argtypes = [vp]
@@ -241,7 +270,6 @@ def hostcore():
         H.hc_torus_rep_c.argtypes = [vp, C.c_uint32, vp, vp]
         H.hc_rank_split.restype = C.c_uint64
         H.hc_rank_split.argtypes = [vp, C.c_uint64]
-        H.hc_div96.argtypes = [C.c_uint64, vp, vp]
         H.hc_hv_rows2.restype = C.c_int
         H.hc_hv_rows2.argtypes = [vp] * 3
         H.hc_tridiag.restype = C.c_int
@@ -251,11 +279,13 @@ def hostcore():
 
 
 class HostCore:
-    def __init__(self, model, n, kx=0, ky=0):
+    def __init__(self, model, n, kx=0, ky=0, kz=0):
         H = hostcore()
         self.T1 = np.ascontiguousarray(model.T1, np.int32)
         self.T2 = np.ascontiguousarray(model.T2, np.int32)
-        self.h = H.hc_build(model.n_sites, n, self.T1.ctypes.data, self.T2.ctypes.data, kx, ky)
+        self.T3 = None if model.T3 is None else np.ascontiguousarray(model.T3, np.int32)
+        self.h = H.hc_build3(model.n_sites, n, self.T1.ctypes.data, self.T2.ctypes.data,
+                             None if self.T3 is None else self.T3.ctypes.data, kx, ky, kz)
         self.D = H.hc_dim(self.h)
         import pyoracle
         self.terms, self._keep = pyoracle.make_terms(model)

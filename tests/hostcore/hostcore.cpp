@@ -51,8 +51,8 @@ struct Terms {
 template <typename S>
 void build(HC* h) {
     uint64_t ng = (h->C + kDirSpan - 1) / kDirSpan;
-    std::vector<uint32_t> words(ng * 3, 0);
-    for (uint64_t w = 0; w < ng * 3; w++) {
+    std::vector<uint32_t> words(ng * 2, 0);
+    for (uint64_t w = 0; w < ng * 2; w++) {
         uint64_t r0 = w * 32;
         if (r0 >= h->C) break;
         uint64_t s = unrank(r0, h->N, h->n, h->bn.data());
@@ -66,18 +66,18 @@ void build(HC* h) {
     h->dir.resize(ng);
     uint64_t base = 0;
     for (uint64_t g = 0; g < ng; g++) {
-        h->dir[g] = DirEntry{(uint32_t)base, words[3 * g], words[3 * g + 1], words[3 * g + 2]};
-        base += popc32(words[3 * g]) + popc32(words[3 * g + 1]) + popc32(words[3 * g + 2]);
+        h->dir[g] = DirEntry{(uint32_t)base, words[2 * g], words[2 * g + 1]};
+        base += popc32(words[2 * g]) + popc32(words[2 * g + 1]);
     }
     h->rows.resize(base);
     for (uint64_t g = 0; g < ng; g++) {
         DirEntry e = h->dir[g];
-        if ((e.b0 | e.b1 | e.b2) == 0) continue;
+        if ((e.b0 | e.b1) == 0) continue;
         uint64_t r0 = g * kDirSpan;
         uint64_t s = unrank(r0, h->N, h->n, h->bn.data());
         uint64_t j = e.base;
         for (int b = 0; b < kDirSpan && r0 + b < h->C; b++) {
-            uint32_t wd = b < 32 ? e.b0 : (b < 64 ? e.b1 : e.b2);
+            uint32_t wd = b < 32 ? e.b0 : e.b1;
             if ((wd >> (b & 31)) & 1u) h->rows[j++] = s | ((uint64_t)classify<S>((S)s, h->G) << kLidShift);
             s = gosper_next(s);
         }
@@ -150,25 +150,34 @@ void hv_rows2(HC* h, const std::vector<TFam64>& fam, const std::vector<int>& fof
 
 extern "C" {
 
-void* hc_build(int N, int n, const int32_t* T1, const int32_t* T2, int kx, int ky) {
+// T3 may be null: the identity (one or two generators).
+void* hc_build3(int N, int n, const int32_t* T1, const int32_t* T2, const int32_t* T3, int kx, int ky, int kz) {
     HC* h = new HC();
     h->N = N; h->n = n; h->kx = kx; h->ky = ky;
-    std::vector<int> a(T1, T1 + N), b(T2, T2 + N);
+    std::vector<int> a(T1, T1 + N), b(T2, T2 + N), c(N);
+    for (int i = 0; i < N; i++) c[i] = T3 ? T3[i] : i;
     h->L1 = cycle_len0(a);
     h->L2 = cycle_len0(b);
+    const int L3 = cycle_len0(c);
     h->px = 2 * kx * M_PI / h->L1;
     h->py = 2 * ky * M_PI / h->L2;
+    const double pz = 2 * kz * M_PI / L3;
     h->C = binom_u64(N, n);
     std::memset(&h->G, 0, sizeof(h->G));
-    h->G.N = N; h->G.L1 = h->L1; h->G.L2 = h->L2; h->G.kx = kx; h->G.ky = ky;
+    h->G.N = N; h->G.L1 = h->L1; h->G.L2 = h->L2; h->G.L3 = L3; h->G.kx = kx; h->G.ky = ky; h->G.kz = kz;
     make_prog(a, h->G.T1);
     make_prog(b, h->G.T2);
+    make_prog(c, h->G.T3);
     h->rt = rank_tables(N, n, h->R);
     h->bn = binom_table();
-    h->ph = phase_table(h->L1, h->L2, h->px, h->py);
-    norm_tables(h->L1, h->L2, h->nt, h->nit);
+    h->ph = phase_table(h->L1, h->L2, L3, h->px, h->py, pz);
+    norm_tables(h->L1, h->L2, L3, h->nt, h->nit);
     if (N <= 32) build<uint32_t>(h); else build<uint64_t>(h);
     return h;
+}
+
+void* hc_build(int N, int n, const int32_t* T1, const int32_t* T2, int kx, int ky) {
+    return hc_build3(N, n, T1, T2, nullptr, kx, ky, 0);
 }
 
 void hc_free(void* p) { delete (HC*)p; }
@@ -239,8 +248,6 @@ uint64_t hc_rank_split(void* p, uint64_t s) {
     return (uint64_t)acc + r->rt[top + ((s >> (8 * (r->R.nbytes - 1))) & 255u)];
 }
 
-void hc_div96(uint64_t x, uint32_t* q, uint32_t* o) { div96(x, *q, *o); }
-
 uint64_t hc_unrank(void* p, uint64_t k) {
     Ranker* r = (Ranker*)p;
     return unrank(k, r->N, r->n, r->bn.data());
@@ -283,6 +290,7 @@ int hc_hv_rows2(void* p, const double* v_, double* w_) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
+    if (h->G.L3 != 1) return 1;
     std::vector<int> a(h->N), b(h->N);
     for (int i = 0; i < h->N; i++) {
         uint64_t s1 = apply_T<uint64_t>(1ull << i, h->G.T1), s2 = apply_T<uint64_t>(1ull << i, h->G.T2);
@@ -326,7 +334,7 @@ int hc_hv_torus(void* p, const double* v_, double* w_) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
-    if (h->N > 32 || !h->w.empty()) return 1;
+    if (h->N > 32 || !h->w.empty() || h->G.L3 != 1) return 1;
     Torus T;
     std::vector<int> a(h->N), b(h->N);
     // recover the permutations from the translation programs
@@ -385,7 +393,7 @@ int hc_hv_torus_c(void* p, const double* v_, double* w_) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
-    if (h->N > 32 || !h->w.empty()) return 1;
+    if (h->N > 32 || !h->w.empty() || h->G.L3 != 1) return 1;
     Torus T;
     std::vector<int> a(h->N), b(h->N);
     for (int i = 0; i < h->N; i++) {

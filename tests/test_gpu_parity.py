@@ -34,7 +34,7 @@ def test_gpu_vs_golden(gpu_ready, path):
     ed = _ed()
     m, d = Hp.load_golden(path)
     print(f"[case] {path}", flush=True)
-    basis = ed.Basis(ed.Lattice.of(m), int(d["n_ones"]), int(d["kx"]), int(d["ky"]))
+    basis = ed.Basis(ed.Lattice.of(m), int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
     reps, norm, norm_inv = basis.export()
     assert np.array_equal(reps, d["reps"])
     assert np.array_equal(norm, d["norm"]) and np.array_equal(norm_inv, d["norm_inv"])
@@ -255,3 +255,56 @@ def test_gpu_two_row_64bit_path(gpu_ready, name, P1, P2, n, k):
     wo = o.hv(v)
     assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo)
     assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo)
+
+
+CUBIC = [("cubic2x3x4_k123", L.cubic(2, 3, 4, J2=0.4, lam=0.8), 12, (1, 2, 3)),
+         ("cubic3x3x2_k201", L.cubic(3, 3, 2), 9, (2, 0, 1)),
+         ("cubic2x2x5_k015", L.cubic(2, 2, 5, J2=0.2), 10, (0, 1, 4))]
+
+
+@pytest.mark.parametrize("name,model,n,k", CUBIC, ids=[c[0] for c in CUBIC])
+def test_gpu_three_generators_vs_oracle(gpu_ready, name, model, n, k):
+    """Three translation generators (Momentum_Basis.cpp:781-898, H_v.cpp:360):
+    representatives, norms, full tables and H*v against the oracle."""
+    import torch
+    ed = _ed()
+    print(f"[case] {name}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    t = o.tables()
+    basis = ed.Basis(ed.Lattice.of(model), n, *k)
+    assert basis.D == o.D > 0
+    reps, norm, _ = basis.export()
+    assert np.array_equal(reps, t["reps"]) and np.array_equal(norm, t["norm"])
+    full = basis.export_full()
+    assert np.array_equal(full["rep_loc"], t["rep_loc"])
+    assert np.array_equal(full["state_ph"].view(np.float64), t["state_ph"].view(np.float64))
+    op = ed.Operator(basis, model)
+    rng = np.random.default_rng(12)
+    v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
+    wo = o.hv(v)
+    assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo)
+    assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo)
+
+
+def test_gpu_three_generators_full_size_sum_rule(gpu_ready):
+    """4x4x2 cube, 32 sites, Sz=0: every orbit of size s lands in exactly s of
+    the 32 momentum sectors, so the sector dimensions sum to C(32,16); H is
+    Hermitian at k != 0."""
+    import math
+    import torch
+    ed = _ed()
+    model = L.cubic(4, 4, 2, J2=0.3)
+    lat = ed.Lattice.of(model)
+    total = 0
+    for kx in range(4):
+        for ky in range(4):
+            for kz in range(2):
+                total += ed.Basis(lat, 16, kx, ky, kz).D
+    assert total == math.comb(32, 16)
+    basis = ed.Basis(lat, 16, 1, 3, 1)
+    op = ed.Operator(basis, model)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = torch.randn(basis.D, dtype=torch.complex128, device="cuda", generator=g)
+    y = torch.randn(basis.D, dtype=torch.complex128, device="cuda", generator=g)
+    lhs, rhs = torch.vdot(x, op.hv(y)).item(), torch.vdot(op.hv(x), y).item()
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)

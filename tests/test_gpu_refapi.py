@@ -82,3 +82,66 @@ def test_reference_main_unwrapped_vector(gpu_ready, tmp_path):
     for s, a in zip(ids, c):
         psi[idx[s]] = a
     assert np.linalg.norm(H @ psi - e[0] * psi) < 1e-8
+
+
+# ------------------------------------------------- three generators (3-D) ----
+def _drive3d(*args):
+    r = subprocess.run([Hp.drive3d(), *args], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.mark.parametrize("model,n,k", [(L.cubic(2, 3, 4, J2=0.4, lam=0.8), 12, (1, 2, 3)),
+                                       (L.cubic(2, 2, 4), 8, (0, 0, 0)), (L.cubic(3, 2, 3, J2=0.3), 9, (2, 1, 1))],
+                         ids=["cubic2x3x4_k123", "cubic2x2x4_k000", "cubic3x2x3_k211"])
+def test_three_generator_shv_vs_oracle(gpu_ready, tmp_path, model, n, k):
+    """pConstruct_States + sHv (H_v.h:59-83, Momentum_Basis.h:84-100) through
+    the C++ interface against the oracle's three-generator basis and H*v."""
+    print(f"[case] sHv3 {model.name} n={n} k={k}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    v = np.random.default_rng(31).normal(size=o.D) + 1j * np.random.default_rng(32).normal(size=o.D)
+    spec, fv, fw, fr = (str(tmp_path / x) for x in ("spec", "v.bin", "w.bin", "r.bin"))
+    Hp.write_spec3(spec, model, n, k)
+    v.astype(np.complex128).tofile(fv)
+    out = _drive3d("hv", spec, fv, fw, fr)
+    assert f"D {o.D}" in out
+    assert np.array_equal(np.fromfile(fr, np.int64), o.tables()["reps"])
+    w = np.fromfile(fw, np.complex128)
+    wo = o.hv(v)
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+
+
+def test_three_generator_sector_lanczos_and_vectors(gpu_ready, tmp_path):
+    """sector_lanczos (Lanczos_sector.cpp:201-367): energies against the dense
+    spectrum, the momentum-basis and unwrapped eigenvector files."""
+    model, n, k = L.cubic(2, 2, 4, J2=0.3), 8, (1, 0, 2)
+    print(f"[case] sector3 {model.name} n={n} k={k}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    ev = np.linalg.eigvalsh(o.make_h())
+    spec, log, vp = str(tmp_path / "spec"), str(tmp_path / "log.txt"), str(tmp_path / "vec")
+    Hp.write_spec3(spec, model, n, k, nev=2, iterations=120, ncycles=8, tol=1e-12)
+    _drive3d("sector", spec, log, vp, "0")
+    text = open(log).read()
+    assert f"\t{o.D}" in text and "kz" in text
+    e = I.parse_energies(log)
+    assert len(e) == 2 and np.abs(np.array(e) - ev[:2]).max() < 1e-9
+    lines = open(vp + "_pEV0.txt").read().splitlines()
+    assert "(1,0,2)" in lines[3]
+    start = next(i for i, l in enumerate(lines) if l.startswith("pblock_states"))
+    rows = [l.split() for l in lines[start + 1:] if l.strip()]
+    x = np.array([complex(*map(float, r[2].strip("()").split(","))) for r in rows])
+    assert len(x) == o.D and abs(np.linalg.norm(x) - 1) < 1e-9
+    assert np.linalg.norm(o.hv(x) - e[0] * x) < 1e-6
+    # unwrapped (Momentum_Basis.cpp:901-975): an eigenvector of the full-space H
+    _drive3d("sector", spec, str(tmp_path / "log2.txt"), vp, "1")
+    rows = [l.split() for l in open(vp + "_EV0.txt") if l.strip() and l.split()[0].isdigit() and len(l.split()) == 2]
+    ids = [int(r[0]) for r in rows]
+    c = np.array([complex(*map(float, r[1].strip("()").split(","))) for r in rows])
+    assert ids == sorted(ids) and len(set(ids)) == len(ids)
+    assert abs(np.linalg.norm(c) - 1) < 1e-9
+    H, st = Hp.full_space_h(model, n)
+    idx = {s: i for i, s in enumerate(st)}
+    psi = np.zeros(len(st), complex)
+    for s, a in zip(ids, c):
+        psi[idx[s]] = a
+    assert np.linalg.norm(H @ psi - e[0] * psi) < 1e-8

@@ -48,7 +48,7 @@ def test_lin_tables_closed_form(half):
 @pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
 def test_oracle_matches_golden(path):
     m, d = Hp.load_golden(path)
-    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]))
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
     t = o.tables()
     # basis tables: bit-exact
     for k in ("reps", "norm", "norm_inv", "rep_loc", "Ia", "Ib"):
@@ -122,7 +122,7 @@ def test_momentum_hamiltonian_vs_full_space_projection(model, n, k):
 @pytest.mark.parametrize("path", GOLDEN[::3], ids=[p.split("/")[-1][:-4] for p in GOLDEN[::3]])
 def test_oracle_hv_hermitian(path):
     m, d = Hp.load_golden(path)
-    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]))
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
     rng = np.random.default_rng(3)
     x = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
     y = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
