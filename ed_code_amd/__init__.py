@@ -157,7 +157,8 @@ class Operator:
     def tune(self, path: int = 0, block: int = 0):
         """Kernel choice: path 0 auto (tuned on the first large H*v), 1 generic
         walk, 2 compiled-shape rotation kernel (K=4), 3 run-time-shape
-        rotation kernel, 4 compiled-shape K=2 at 8 waves, 5 compiled-shape K=8;
+        rotation kernel, 4 compiled-shape K=2 at 8 waves, 5 compiled-shape K=8,
+        6 two-row 64-bit kernel, 7 compiled-shape K=4 software-pipelined;
         block 0 = default."""
         check(lib().ed_op_tune(self.h, path, block))
 

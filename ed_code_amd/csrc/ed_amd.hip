@@ -200,10 +200,17 @@ __global__ void k_full_tables(uint64_t C, int N, int n, Geo G, const uint64_t* b
     }
 }
 
-// u = v / norm  (the gathered stream of the H*v kernel)
+// u = v / norm (the gathered stream of the H*v kernel), and u[D] = 0: the
+// element that the empty slots of the batched rows read.
 template <typename VT>
 __global__ void k_prescale(int64_t D, const uint64_t* rows, const double* nitab, const VT* v, VT* u) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == D) {
+        VT z;
+        z.re = 0;
+        z.im = 0;
+        u[D] = z;
+    }
     if (i >= D) return;
     double f = nitab[(int)(rows[i] >> kLidShift)];
     VT x = v[i];
@@ -559,6 +566,10 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
             op->torus = b->torus;
             op->nfam = (int)fam.size();
             op->ezz0 = ezz0;
+            bool bonds = true, real = true;
+            for (const TFam& F : fam) bonds = bonds && F.type == 0;
+            for (const TEntry& E : ent) real = real && E.cim == 0.0;
+            op->tmode = !bonds ? kModeGeneral : (real && b->trivial_k ? kModeReal : kModeBonds);
             op->tune_block = 512;
             HIPCHK(hipMalloc(&op->d_tfam, fam.size() * sizeof(TFam)));
             HIPCHK(hipMemcpy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
@@ -580,11 +591,11 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 6) return ed_fail(ED_EINVAL, "path %d not in 0..6", path);
+    if (path < 0 || path > 7) return ed_fail(ED_EINVAL, "path %d not in 0..7", path);
     if (path == 6 && !op->use_rows2) return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
-    if (path >= 2 && path <= 5 && !op->use_torus)
+    if (((path >= 2 && path <= 5) || path == 7) && !op->use_torus)
         return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
-    if ((path == 2 || path == 4 || path == 5) && !torus_shape_compiled(op->torus))
+    if ((path == 2 || path == 4 || path == 5 || path == 7) && !torus_shape_compiled(op->torus))
         return ed_fail(ED_EINVAL, "lattice shape is not one of the compiled shapes");
     if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
     op->tune_path = path;
@@ -637,8 +648,8 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     const int64_t D = B->D;
     if (D == 0) return ED_OK;
     VT*& u = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
-    if (!u) HIPCHK(hipMalloc((void**)&u, D * sizeof(VT)));
-    hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, u);
+    if (!u) HIPCHK(hipMalloc((void**)&u, (D + 1) * sizeof(VT)));
+    hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D + 1, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, u);
     HIPCHK(hipGetLastError());
     Geo G;
     basis_geo(B, G);

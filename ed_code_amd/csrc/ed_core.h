@@ -14,6 +14,7 @@
 //     Momentum_Basis.cpp:732-737), which is the combinatorial number system
 //     rank(s) = sum_t C(p_t, t) over the set-bit positions p_1 < p_2 < ....
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -237,6 +238,12 @@ struct alignas(8) cf2 { float re, im; };
 
 ED_HD cd2 load_c(const cd2* p, int64_t j) { return p[j]; }
 ED_HD cd2 load_c(const cf2* p, int64_t j) { cf2 v = p[j]; return cd2{(double)v.re, (double)v.im}; }
+// Element j through a 32-bit byte offset (a scalar base plus a 32-bit vector
+// offset on the device); the caller guarantees (j + 1) * sizeof(VT) < 2^32.
+template <typename VT>
+ED_HD cd2 load_c32(const VT* p, uint32_t j) {
+    return load_c((const VT*)((const char*)p + (uint32_t)(j * (uint32_t)sizeof(VT))), 0);
+}
 
 // Device-side Hamiltonian terms.  Terms with |J| < 1e-16 and the chiral /
 // bowtie lists when |Ch|, |ChCh| <= 1e-16 are dropped at upload, which is

@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
 // constants (fully unrolled rotation search, masks as immediates), K gathered
 // elements in flight per lane.  The order in which elements are accumulated
 // does not depend on K, so every variant returns bit-identical results.
-template <class TC, bool TRIV, int K, typename VT>
+template <class TC, bool TRIV, int MODE, int K, bool PIPE, typename VT>
 __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                 const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
                                                 double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
@@ -91,16 +91,25 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     uint32_t s = (uint32_t)(rw & kStateMask);
     double nrm = ntab[(int)(rw >> kLidShift)];
     double ar = 0.0, ai = 0.0;
-    double ez = torus_row_batched<TC, TRIV, K>(
-        s, nfam, fam, ent, ezz0, R, rtab, dir, phase, i, [&](int64_t j) { return load_c(u, j); },
-        [&](const cd2& x, double cr, double ci) {
-            ar += cr * x.re - ci * x.im;
-            ai += cr * x.im + ci * x.re;
-        });
+    auto loadu = [&](uint32_t j) { return load_c32(u, j); };   // u[D] holds zero: empty slots read it
+    // explicit FMAs, so that the accumulation rounds identically in every
+    // variant whatever the compiler contracts elsewhere
+    auto acc = [&](const cd2& x, double cr, double ci) {
+        ar = fma(cr, x.re, ar);
+        ai = fma(cr, x.im, ai);
+        if (MODE != kModeReal) {
+            ar = fma(-ci, x.im, ar);
+            ai = fma(ci, x.re, ai);
+        }
+    };
+    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, ezz0, R, rtab, dir, phase,
+                                                              (uint32_t)D, loadu, acc)
+                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, ezz0, R, rtab, dir, phase,
+                                                            (uint32_t)D, loadu, acc);
     cd2 vi = load_c(v, i);
     VT o;
-    o.re = nrm * ar + ez * vi.re;
-    o.im = nrm * ai + ez * vi.im;
+    o.re = fma(nrm, ar, ez * vi.re);
+    o.im = fma(nrm, ai, ez * vi.im);
     w[i] = o;
 }
 
@@ -112,21 +121,27 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
 #define ED_TORUS_C_PASS D, rows, nfam, gfam, gent, ezz0, R, grtab, rtab_n, dir, gphase, nphase, ntab, u, v, w
 
 // Variant A: four elements in flight, occupancy left to the register count.
-template <class TC, bool TRIV, typename VT>
+template <class TC, bool TRIV, int MODE, typename VT>
 __global__ void __launch_bounds__(512) k_hv_torus_c4(ED_TORUS_C_ARGS) {
-    hv_torus_c_body<TC, TRIV, 4, VT>(ED_TORUS_C_PASS);
+    hv_torus_c_body<TC, TRIV, MODE, 4, false, VT>(ED_TORUS_C_PASS);
+}
+
+// Variant D: four elements in flight, batches software-pipelined.
+template <class TC, bool TRIV, int MODE, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_p4(ED_TORUS_C_ARGS) {
+    hv_torus_c_body<TC, TRIV, MODE, 4, true, VT>(ED_TORUS_C_PASS);
 }
 
 // Variant C: eight elements in flight.
-template <class TC, bool TRIV, typename VT>
+template <class TC, bool TRIV, int MODE, typename VT>
 __global__ void __launch_bounds__(512) k_hv_torus_c8(ED_TORUS_C_ARGS) {
-    hv_torus_c_body<TC, TRIV, 8, VT>(ED_TORUS_C_PASS);
+    hv_torus_c_body<TC, TRIV, MODE, 8, false, VT>(ED_TORUS_C_PASS);
 }
 
 // Variant B: two elements in flight, registers capped for 8 waves per SIMD.
-template <class TC, bool TRIV, typename VT>
+template <class TC, bool TRIV, int MODE, typename VT>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) k_hv_torus_c2w8(ED_TORUS_C_ARGS) {
-    hv_torus_c_body<TC, TRIV, 2, VT>(ED_TORUS_C_PASS);
+    hv_torus_c_body<TC, TRIV, MODE, 2, false, VT>(ED_TORUS_C_PASS);
 }
 
 template <class TC, typename VT>
@@ -138,16 +153,26 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
 #define ED_GO(KER) \
     hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, op->nfam, op->d_tfam, op->d_tent, op->ezz0, B->rinfo, \
                        B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
-    if (var == 0) {
-        if (B->trivial_k) ED_GO((k_hv_torus_c4<TC, true, VT>));
-        else ED_GO((k_hv_torus_c4<TC, false, VT>));
-    } else if (var == 1) {
-        if (B->trivial_k) ED_GO((k_hv_torus_c2w8<TC, true, VT>));
-        else ED_GO((k_hv_torus_c2w8<TC, false, VT>));
-    } else {
-        if (B->trivial_k) ED_GO((k_hv_torus_c8<TC, true, VT>));
-        else ED_GO((k_hv_torus_c8<TC, false, VT>));
+    // term class: k = 0 takes the general or the real-bond kernel, k != 0 the
+    // general or the bond-only one
+#define ED_GO_MODES(KERNEL)                                                         \
+    if (B->trivial_k) {                                                             \
+        if (op->tmode == kModeReal) ED_GO((KERNEL<TC, true, kModeReal, VT>));       \
+        else ED_GO((KERNEL<TC, true, kModeGeneral, VT>));                           \
+    } else {                                                                        \
+        if (op->tmode != kModeGeneral) ED_GO((KERNEL<TC, false, kModeBonds, VT>));  \
+        else ED_GO((KERNEL<TC, false, kModeGeneral, VT>));                          \
     }
+    if (var == 0) {
+        ED_GO_MODES(k_hv_torus_c4)
+    } else if (var == 1) {
+        ED_GO_MODES(k_hv_torus_c2w8)
+    } else if (var == 2) {
+        ED_GO_MODES(k_hv_torus_c8)
+    } else {
+        ED_GO_MODES(k_hv_torus_p4)
+    }
+#undef ED_GO_MODES
 #undef ED_GO
 }
 
@@ -157,6 +182,13 @@ bool torus_shape_compiled(const Torus& T) {
     ED_TORUS_SHAPES(ED_MATCH)
 #undef ED_MATCH
     return false;
+}
+
+// The compiled-shape kernels address the gather stream with 32-bit byte
+// offsets: (D + 1) complex128 elements must span less than 4 GiB (every
+// N <= 32 momentum sector does).
+static bool compiled_fits(const ed_op_s* op) {
+    return torus_shape_compiled(op->torus) && (uint64_t)(op->b->D + 1) * sizeof(cd2) <= 0xffffffffull;
 }
 
 size_t torus_smem_bytes(const ed_op_s* op) {
@@ -179,6 +211,7 @@ static size_t torus_c_smem_bytes(const ed_op_s* op) {
 template <typename VT>
 static bool launch_compiled(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, int var, int bs) {
     const Torus& T = op->torus;
+    if (!compiled_fits(op)) return false;
     int L2 = T.kind == 2 ? 1 : T.L2;
 #define ED_LAUNCH(n, wd, l2)                                                            \
     if (T.N == n && T.w == wd && L2 == l2) {                                            \
@@ -199,7 +232,7 @@ static constexpr int dtype_slot() {
 // fastest depends on the register pressure of the shape, the vector precision
 // and the momentum (measured on the 32-site torus: K=4 at 256 threads for
 // complex64 at k=0, K=8 at 256 for complex128, K=2 at 8 waves for k != 0).
-static const int kTuneCfg[6][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256}};
+static const int kTuneCfg[8][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256}, {3, 512}, {3, 256}};
 static const int64_t kTuneMinRows = int64_t(1) << 20;
 
 // Chooses the configuration of the compiled-shape kernel for this operator
@@ -210,7 +243,7 @@ static const int64_t kTuneMinRows = int64_t(1) << 20;
 // kept.  Every candidate writes the same w.
 template <typename VT>
 int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
-    if (!op->use_torus || op->tune_path != 0 || !torus_shape_compiled(op->torus)) return ED_OK;
+    if (!op->use_torus || op->tune_path != 0 || !compiled_fits(op)) return ED_OK;
     const int k = dtype_slot<VT>();
     if (op->sel_var[k] >= 0) return ED_OK;
     const int64_t D = op->b->D;
@@ -253,7 +286,7 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
 // 3 (run-time shape); variant 0 (K=4) or 1 (K=2 at 8 waves), -1 before
 // tuning or for the run-time kernel.
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
-    const bool compiled = op->tune_path != 3 && torus_shape_compiled(op->torus);
+    const bool compiled = op->tune_path != 3 && compiled_fits(op);
     path = compiled ? 2 : 3;
     bs = op->tune_block;
     var = -1;
@@ -261,6 +294,7 @@ void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     if (op->tune_path == 2) var = 0;
     else if (op->tune_path == 4) var = 1;
     else if (op->tune_path == 5) var = 2;
+    else if (op->tune_path == 7) var = 3;
     else if (op->sel_var[dslot] >= 0) {
         var = op->sel_var[dslot];
         bs = op->sel_block[dslot];
@@ -274,7 +308,8 @@ int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st
     int path, var, bs;
     torus_config(op, dtype_slot<VT>(), path, var, bs);
     if (path == 2) {
-        launch_compiled<VT>(op, u, v, w, st, var < 0 ? 0 : var, bs);
+        if (!launch_compiled<VT>(op, u, v, w, st, var < 0 ? 0 : var, bs))
+            return ed_fail(ED_EINVAL, "compiled-shape kernel does not fit this sector");
         HIPCHK(hipGetLastError());
         return ED_OK;
     }

@@ -73,6 +73,7 @@ struct ed_op_s {
     edc::TFam* d_tfam = nullptr;
     edc::TEntry* d_tent = nullptr;
     double ezz0 = 0;
+    int tmode = 0;             // term class of the families (kModeGeneral / kModeBonds / kModeReal)
     // two-row 64-bit rotation path (ed_hv64.hip)
     bool use_rows2 = false;
     int nfam64 = 0, nterm64 = 0;

@@ -366,80 +366,160 @@ ED_HD int64_t dir_decode_off(const DirEntry& e, unsigned o) {
 
 ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) { return dir_decode_off(e, r % kDirSpan); }
 
+// Directory entry of rank block idx.  On the device the byte offset idx*12
+// is formed with a 24-bit multiply (full rate) instead of a 64-bit
+// multiply-add; idx < 2^24 holds for every N <= 32 sector (C(32,16)/64 <
+// 1e7).
+ED_HD const DirEntry& dir_at(const DirEntry* dir, uint32_t idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const DirEntry*)((const char*)dir + ((uint64_t)(uint32_t)__umul24(idx, 3u) << 2));
+#else
+    return dir[idx];
+#endif
+}
+
+// Term classes of the batched row (a compile-time switch, so the bond-only
+// rows carry no branch for chiral entries):
+//   kModeGeneral  bond and chiral families, complex coefficients;
+//   kModeBonds    bond families only;
+//   kModeReal     bond families only with real coefficients at k = 0 (the
+//                 caller accumulates with two FMAs and ignores ci).
+enum { kModeGeneral = 0, kModeBonds = 1, kModeReal = 2 };
+
+// One batch of up to K elements of a row: rank of the representative of each
+// target and its coefficient (phase applied); ok[q] false for empty slots.
+template <int K>
+struct RowBatch {
+    uint32_t rk[K];
+    double cre[K], cim[K];
+    bool ok[K];
+};
+
+// Fills the next batch of the row (the orbit search and rank of K
+// elements), advancing the family cursor (f, cur) and the diagonal ezz.
+template <class TC, bool TRIV, int MODE, int K>
+ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const RankInfo& R,
+                      const uint32_t* rtab, const cd2* phase, int& f, uint32_t& cur, double& ezz, RowBatch<K>& B) {
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        while (cur == 0u && f < nfam) {
+            if (++f < nfam) cur = fam_mask_c<TC>(s, fam[f]);
+        }
+        B.ok[q] = f < nfam;
+        B.rk[q] = 0; B.cre[q] = 0; B.cim[q] = 0;
+        if (B.ok[q]) {
+            int x = ctz32(cur);
+            cur &= cur - 1u;
+            const TEntry& E = ent[f * 32 + x];
+            uint32_t t = s ^ (1u << x) ^ (1u << E.y);
+            double cr, ci = 0.0;
+            if (MODE != kModeGeneral || fam[f].type == 0) {
+                ezz -= E.dJ2;
+                cr = E.cre;
+                if (MODE != kModeReal) ci = ((s >> x) & 1u) ? -E.cim : E.cim;
+            } else {
+                double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
+                cr = -E.cim * sg;
+                ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
+            }
+            int code;
+            uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
+            uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
+            if (!TRIV) {
+                // explicit FMAs: every variant rounds the same way
+                cd2 ph = phase[code];
+                double nr = fma(cr, ph.re, -(ci * ph.im)), ni = fma(cr, ph.im, ci * ph.re);
+                cr = nr; ci = ni;
+            }
+            B.rk[q] = r; B.cre[q] = cr; B.cim[q] = ci;
+        }
+    }
+}
+
 // torus_row_c with K elements in flight per lane: the orbit searches and
 // ranks of K elements are computed first, then their K directory loads and
 // K gathers are issued back to back, so that a lane waits for memory once
-// per K elements instead of twice per element.  The loads are unconditional
-// (an empty slot reads directory entry 0 and the row's own element `self`):
+// per K elements instead of twice per element.  The loads are unconditional:
 // a load under a lane mask would make the compiler wait for it inside the
-// masked block and serialise the K loads.  loadu(j) returns element j of u;
-// g(x, cr, ci) accumulates one element and is also called with cr = ci = 0
-// for empty slots (u[self] is non-finite only when w[self] is anyway).
-template <class TC, bool TRIV, int K, class LoadU, class Gf>
+// masked block and serialise the K loads.  An empty slot (no element left,
+// or a representative outside the sector) reads directory entry 0 and
+// element zidx of u, which holds zero, so it adds nothing whatever its
+// coefficient.  loadu(j) returns element j of u; g(x, cr, ci) accumulates.
+template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
 ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
                                const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
-                               int64_t self, LoadU&& loadu, Gf&& g) {
+                               uint32_t zidx, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
     bool more = true;
     while (more) {
-        uint32_t rk[K];
-        double cre[K], cim[K];
-        bool ok[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            while (cur == 0u && f < nfam) {
-                if (++f < nfam) cur = fam_mask_c<TC>(s, fam[f]);
-            }
-            ok[q] = f < nfam;
-            rk[q] = 0; cre[q] = 0; cim[q] = 0;
-            if (ok[q]) {
-                int x = ctz32(cur);
-                cur &= cur - 1u;
-                const TEntry E = ent[f * 32 + x];
-                uint32_t t = s ^ (1u << x) ^ (1u << E.y);
-                double cr, ci;
-                if (fam[f].type == 0) {
-                    ezz -= E.dJ2;
-                    cr = E.cre;
-                    ci = ((s >> x) & 1u) ? -E.cim : E.cim;
-                } else {
-                    double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
-                    cr = -E.cim * sg;
-                    ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
-                }
-                int code;
-                uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
-                uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
-                if (!TRIV) {
-                    cd2 ph = phase[code];
-                    double nr = cr * ph.re - ci * ph.im, ni = cr * ph.im + ci * ph.re;
-                    cr = nr; ci = ni;
-                }
-                rk[q] = r; cre[q] = cr; cim[q] = ci;
-            }
-        }
-        more = ok[K - 1];
+        RowBatch<K> B;
+        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, B);
+        more = B.ok[K - 1];
         DirEntry e[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) e[q] = dir[rk[q] / kDirSpan];
-        bool use[K];
-        int64_t j[K];
+        for (int q = 0; q < K; q++) e[q] = dir_at(dir, B.rk[q] / kDirSpan);
+        uint32_t j[K];
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            j[q] = dir_decode(e[q], rk[q]);
-            use[q] = ok[q] && j[q] >= 0;
+            int64_t jj = dir_decode_off(e[q], B.rk[q] % kDirSpan);
+            j[q] = (B.ok[q] && jj >= 0) ? (uint32_t)jj : zidx;
         }
-        cd2 x[K];
+        cd2 xv[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) x[q] = loadu(use[q] ? j[q] : self);
-        // Empty slots multiply the row's own element by a zero coefficient,
-        // so every loaded value is used on every lane and the compiler keeps
-        // the loads out of masked blocks.
+        for (int q = 0; q < K; q++) xv[q] = loadu(j[q]);
 #pragma unroll
-        for (int q = 0; q < K; q++) g(x[q], use[q] ? cre[q] : 0.0, use[q] ? cim[q] : 0.0);
+        for (int q = 0; q < K; q++) g(xv[q], B.cre[q], B.cim[q]);
     }
+    return ezz;
+}
+
+// torus_row_batched with the batches software-pipelined: while the gathers
+// of batch b are in flight the lane computes the orbit searches and ranks of
+// batch b+1 and issues its directory loads, then accumulates batch b and
+// issues the gathers of b+1.  Each wait on memory thus overlaps a batch of
+// integer work of the same lane.  Elements are accumulated in the same order
+// as torus_row_batched, so the result is bit-identical.
+template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
+ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
+                                 const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
+                                 uint32_t zidx, LoadU&& loadu, Gf&& g) {
+    double ezz = ezz0;
+    int f = 0;
+    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    RowBatch<K> A, B;
+    fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, A);
+    DirEntry e[K];
+    cd2 xa[K];
+#pragma unroll
+    for (int q = 0; q < K; q++) e[q] = dir_at(dir, A.rk[q] / kDirSpan);
+#pragma unroll
+    for (int q = 0; q < K; q++) {
+        int64_t jj = dir_decode_off(e[q], A.rk[q] % kDirSpan);
+        xa[q] = loadu((A.ok[q] && jj >= 0) ? (uint32_t)jj : zidx);
+    }
+    bool more = A.ok[K - 1];
+    while (more) {
+        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, B);
+#pragma unroll
+        for (int q = 0; q < K; q++) e[q] = dir_at(dir, B.rk[q] / kDirSpan);
+#pragma unroll
+        for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            int64_t jj = dir_decode_off(e[q], B.rk[q] % kDirSpan);
+            xa[q] = loadu((B.ok[q] && jj >= 0) ? (uint32_t)jj : zidx);
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            A.cre[q] = B.cre[q];
+            A.cim[q] = B.cim[q];
+        }
+        more = B.ok[K - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
     return ezz;
 }
 

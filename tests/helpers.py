@@ -261,6 +261,8 @@ def hostcore():
         H.hc_unrank.argtypes = [vp, C.c_uint64]
         H.hc_hv_torus.restype = C.c_int
         H.hc_hv_torus.argtypes = [vp] * 3
+        H.hc_hv_torus_p.restype = C.c_int
+        H.hc_hv_torus_p.argtypes = [vp] * 3
         H.hc_torus_rep.restype = C.c_int
         H.hc_torus_rep.argtypes = [vp, C.c_uint32, vp, vp]
         H.hc_generic_rep.argtypes = [vp, C.c_uint64, vp, vp]
@@ -321,11 +323,13 @@ class HostCore:
             return None
         return w
 
-    def hv_torus_c(self, v):
-        """Compiled-shape batched row function; None if no compiled shape fits."""
+    def hv_torus_c(self, v, pipelined=False):
+        """Compiled-shape batched row function (or its software-pipelined
+        form); None if no compiled shape fits."""
         v = np.ascontiguousarray(v, np.complex128)
         w = np.zeros(self.D, np.complex128)
-        if hostcore().hc_hv_torus_c(self.h, v.ctypes.data, w.ctypes.data):
+        fn = hostcore().hc_hv_torus_p if pipelined else hostcore().hc_hv_torus_c
+        if fn(self.h, v.ctypes.data, w.ctypes.data):
             return None
         return w
 

@@ -3,6 +3,7 @@
 // same pipeline the device code runs (classify -> directory -> compaction ->
 // visit_row) so that the kernels' logic is checked against the oracle without
 // a GPU.  It is never loaded by the product.
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -107,23 +108,42 @@ void hv(HC* h, const cd2* v, cd2* w) {
         w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
     }
 }
-template <class TC, bool TRIV>
+template <class TC, bool TRIV, int MODE, bool PIPE>
 void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
                        double ezz0, const std::vector<uint32_t>& rt32, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+    const uint32_t zidx = (uint32_t)h->rows.size();   // u[zidx] == 0
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
         uint32_t s = (uint32_t)(h->rows[i] & kStateMask);
         double nrm = h->nt[h->rows[i] >> kLidShift];
         double ar = 0, ai = 0;
-        double ez = torus_row_batched<TC, TRIV, 4>(
-            s, (int)fam.size(), fam.data(), ent.data(), ezz0, h->R, rt32.data(), h->dir.data(), h->ph.data(), i,
-            [&](int64_t j) { return u[j]; },
-            [&](const cd2& x, double cr, double ci) {
-                ar += cr * x.re - ci * x.im;
-                ai += cr * x.im + ci * x.re;
-            });
+        auto loadu = [&](uint32_t j) { return u[j]; };
+        auto acc = [&](const cd2& x, double cr, double ci) {
+            ar = std::fma(cr, x.re, ar);
+            ai = std::fma(cr, x.im, ai);
+            if (MODE != kModeReal) {
+                ar = std::fma(-ci, x.im, ar);
+                ai = std::fma(ci, x.re, ai);
+            }
+        };
+        double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), ezz0,
+                                                                  h->R, rt32.data(), h->dir.data(), h->ph.data(),
+                                                                  zidx, loadu, acc)
+                         : torus_row_batched<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), ezz0,
+                                                                h->R, rt32.data(), h->dir.data(), h->ph.data(), zidx,
+                                                                loadu, acc);
         w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
     }
+}
+
+template <class TC, bool PIPE>
+void hv_modes(HC* h, const Torus& T, bool triv, int mode, const std::vector<TFam>& fam,
+              const std::vector<TEntry>& ent, double ezz0, const std::vector<uint32_t>& rt32,
+              const std::vector<cd2>& u, const cd2* v, cd2* w) {
+    if (triv && mode == kModeReal) hv_batched<TC, true, kModeReal, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
+    else if (triv) hv_batched<TC, true, kModeGeneral, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
+    else if (mode != kModeGeneral) hv_batched<TC, false, kModeBonds, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
+    else hv_batched<TC, false, kModeGeneral, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
 }
 
 template <class TC, bool TRIV>
@@ -389,7 +409,7 @@ int hc_torus_rep(void* p, uint32_t t, uint32_t* rep, int* code) {
 
 // H*v through the compiled-shape, batched row function of the GPU kernel.
 // Returns 1 if the lattice has none of the compiled shapes.
-int hc_hv_torus_c(void* p, const double* v_, double* w_) {
+static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
@@ -406,23 +426,34 @@ int hc_hv_torus_c(void* p, const double* v_, double* w_) {
     double ezz0;
     if (!torus_families(h->lam, h->Ch, T, h->b, h->c, fam, ent, ezz0)) return 1;
     std::vector<uint32_t> rt32(h->rt.begin(), h->rt.end());
-    std::vector<cd2> u(h->rows.size());
-    for (size_t i = 0; i < u.size(); i++) {
+    std::vector<cd2> u(h->rows.size() + 1, cd2{0.0, 0.0});
+    for (size_t i = 0; i < h->rows.size(); i++) {
         double f = h->nit[h->rows[i] >> kLidShift];
         u[i] = cd2{v[i].re * f, v[i].im * f};
     }
     bool triv = h->kx == 0 && h->ky == 0;
+    // term class as the library picks it (op_create)
+    bool bonds = true, real = true;
+    for (const TFam& F : fam) bonds = bonds && F.type == 0;
+    for (const TEntry& E : ent) real = real && E.cim == 0.0;
+    const int mode = !bonds ? kModeGeneral : (real && triv ? kModeReal : kModeBonds);
     int L2 = T.kind == 2 ? 1 : T.L2;
-#define HC_SHAPE(n, wd, l2)                                                                       \
-    if (T.N == n && T.w == wd && L2 == l2) {                                                      \
-        if (triv) hv_batched<TorusC<n, wd, l2>, true>(h, T, fam, ent, ezz0, rt32, u, v, w);       \
-        else hv_batched<TorusC<n, wd, l2>, false>(h, T, fam, ent, ezz0, rt32, u, v, w);           \
-        return 0;                                                                                 \
+#define HC_SHAPE(n, wd, l2)                                                                                  \
+    if (T.N == n && T.w == wd && L2 == l2) {                                                                 \
+        using TC = TorusC<n, wd, l2>;                                                                        \
+        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, ezz0, rt32, u, v, w);                       \
+        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, ezz0, rt32, u, v, w);                           \
+        return 0;                                                                                            \
     }
     HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1) HC_SHAPE(24, 24, 1)
 #undef HC_SHAPE
     return 1;
 }
+
+int hc_hv_torus_c(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, false); }
+
+// Same through the software-pipelined row (torus_row_pipelined).
+int hc_hv_torus_p(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, true); }
 
 // Compile-time-shape search (TorusC) for the shapes the tests use; returns 1
 // if the lattice has none of them.

@@ -157,6 +157,8 @@ def test_batched_compiled_row_vs_oracle(model, n, k):
     assert w is not None
     wo = o.hv(v)
     assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+    # the software-pipelined row accumulates in the same order: bit-identical
+    assert np.array_equal(h.hv_torus_c(v, pipelined=True), w)
 
 
 def _rows2_cases():
