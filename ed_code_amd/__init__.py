@@ -101,6 +101,42 @@ class Basis:
                                          Ib.ctypes.data, C.byref(br)))
         return dict(rep_loc=rep_loc, state_ph=sph.view(np.complex128), Ia=Ia, Ib=Ib, bits_right=br.value)
 
+    def pair_correlations(self, pairs, ket, bra=None):
+        """<bra| Sz_i Sz_j |ket> and <bra| (S+_i S-_j + S-_i S+_j)/2 |ket> for each
+        site pair (i, j), from complex128 device vectors of this basis (bra
+        defaults to ket), computed in the momentum basis (no unwrapping; the
+        reference's analysis/src/pspincorr.cpp:80-145 unwraps).  Returns two
+        complex arrays (zz, pm)."""
+        import torch
+        bra = ket if bra is None else bra
+        for x in (ket, bra):
+            if x.dtype != torch.complex128 or not x.is_cuda or x.numel() != self.D:
+                raise ValueError("vectors must be complex128 CUDA tensors of length D")
+        ij = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        zz = np.zeros(2 * len(ij))
+        pm = np.zeros(2 * len(ij))
+        torch.cuda.synchronize()
+        check(lib().ed_pair_correlations(self.h, C.c_void_p(bra.data_ptr()), C.c_void_p(ket.data_ptr()), len(ij),
+                                         ij.ctypes.data, zz.ctypes.data, pm.ctypes.data))
+        return zz.view(np.complex128), pm.view(np.complex128)
+
+    def unwrap(self, v):
+        """Full-Sz-basis components of the momentum-basis vector v (complex128
+        device tensor): (states int64, coefficients complex128), ascending by
+        state, entries above 1e-8 (Unwrap_pState, Momentum_Basis.cpp:978-1037)."""
+        import torch
+        if v.dtype != torch.complex128 or not v.is_cuda or v.numel() != self.D:
+            raise ValueError("v must be a complex128 CUDA tensor of length D")
+        torch.cuda.synchronize()
+        n = C.c_int64()
+        check(lib().ed_unwrap(self.h, C.c_void_p(v.data_ptr()), 0, None, None, C.byref(n)))
+        st = np.zeros(n.value, np.int64)
+        co = np.zeros(2 * n.value)
+        if n.value:
+            check(lib().ed_unwrap(self.h, C.c_void_p(v.data_ptr()), n.value, st.ctypes.data, co.ctypes.data,
+                                  C.byref(n)))
+        return st, co.view(np.complex128)
+
     def __del__(self):
         if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_basis_destroy(self.h)

@@ -68,6 +68,9 @@ SIGNATURES = {
     "ed_hv_profile": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.POINTER(C.c_float)]),
     "ed_hv_host": (C.c_int, [_VP, _VP, _VP]),
     "ed_make_h": (C.c_int, [_VP, _VP]),
+    "ed_csr_hv_host": (C.c_int, [C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP]),
+    "ed_pair_correlations": (C.c_int, [_VP, _VP, _VP, C.c_int32, _VP, _VP, _VP]),
+    "ed_unwrap": (C.c_int, [_VP, _VP, C.c_int64, _VP, _VP, _VP]),
     "ed_dense_eigh": (C.c_int, [_VP, _VP, _VP]),
     "ed_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
     "ed_block_create": (C.c_int, [_VP, C.c_int32, _VP, C.POINTER(_VP)]),

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libed_amd.so")
-SOURCES = ["ed_amd.hip", "ed_hv_fast.hip", "ed_hv64.hip", "ed_block.hip", "ed_lanczos.hip", "ed_refapi.cpp"]
+SOURCES = ["ed_amd.hip", "ed_hv_fast.hip", "ed_hv64.hip", "ed_block.hip", "ed_lanczos.hip", "ed_obs.hip",
+           "ed_refapi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -733,6 +733,59 @@ extern "C" int ed_hv_host(ed_op op, const double* v, double* w) {
     return rc;
 }
 
+// w = A v for a CSR matrix, one lane per row, FP64 accumulation in the
+// order of the row's entries (H_v.cpp:3-19).
+__global__ void k_csr_hv(int64_t n, const int64_t* __restrict__ rowptr, const int64_t* __restrict__ col,
+                         const cd2* __restrict__ val, const cd2* __restrict__ v, cd2* __restrict__ w) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t q = rowptr[i]; q < rowptr[i + 1]; q++) {
+        const cd2 a = val[q], x = v[col[q]];
+        ar += a.re * x.re - a.im * x.im;
+        ai += a.re * x.im + a.im * x.re;
+    }
+    w[i] = cd2{ar, ai};
+}
+
+extern "C" int ed_csr_hv_host(int64_t n, int64_t ncols, const int64_t* rowptr, const int64_t* col, const double* val,
+                              const double* v, double* w) {
+    if (n < 0 || ncols < 0 || (n > 0 && (!rowptr || !v || !w))) return ed_fail(ED_EINVAL, "null argument");
+    if (n == 0) return ED_OK;
+    const int64_t nnz = rowptr[n];
+    if (rowptr[0] != 0 || nnz < 0) return ed_fail(ED_EINVAL, "row pointers must start at 0");
+    for (int64_t i = 0; i < n; i++)
+        if (rowptr[i + 1] < rowptr[i]) return ed_fail(ED_EINVAL, "row pointers decrease at row %lld", (long long)i);
+    for (int64_t q = 0; q < nnz; q++)
+        if (col[q] < 0 || col[q] >= ncols) return ed_fail(ED_EINVAL, "column %lld out of range", (long long)col[q]);
+    int64_t *drp = nullptr, *dcol = nullptr;
+    cd2 *dval = nullptr, *dv = nullptr, *dw = nullptr;
+    int rc = ED_OK;
+    auto fail = [&](hipError_t e) {
+        rc = ed_fail(e == hipErrorOutOfMemory ? ED_ENOMEM : ED_EDEVICE, "ed_csr_hv_host: %s", hipGetErrorString(e));
+    };
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&drp, (n + 1) * sizeof(int64_t))) != hipSuccess ||
+        (e = hipMalloc(&dcol, (nnz > 0 ? nnz : 1) * sizeof(int64_t))) != hipSuccess ||
+        (e = hipMalloc(&dval, (nnz > 0 ? nnz : 1) * sizeof(cd2))) != hipSuccess ||
+        (e = hipMalloc(&dv, (ncols > 0 ? ncols : 1) * sizeof(cd2))) != hipSuccess ||
+        (e = hipMalloc(&dw, n * sizeof(cd2))) != hipSuccess ||
+        (e = hipMemcpy(drp, rowptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (nnz > 0 && (e = hipMemcpy(dcol, col, nnz * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (nnz > 0 && (e = hipMemcpy(dval, val, nnz * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (ncols > 0 && (e = hipMemcpy(dv, v, ncols * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess)) {
+        fail(e);
+    } else {
+        hipLaunchKernelGGL(k_csr_hv, dim3(nblocks(n, 256)), dim3(256), 0, 0, n, drp, dcol, dval, dv, dw);
+        if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
+            (e = hipMemcpy(w, dw, n * sizeof(cd2), hipMemcpyDeviceToHost)) != hipSuccess)
+            fail(e);
+    }
+    for (void* p : {(void*)drp, (void*)dcol, (void*)dval, (void*)dv, (void*)dw})
+        if (p) dfree(p);
+    return rc;
+}
+
 extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
     ed_basis_s* B = op->b;

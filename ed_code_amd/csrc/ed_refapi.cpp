@@ -14,13 +14,13 @@
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
+#include <iostream>
 #include <memory>
 #include <vector>
 
 #include "ed_amd.h"
 #include "ed_refapi.h"
 
-extern "C" double _Complex cexp(double _Complex);
 
 namespace {
 
@@ -156,103 +156,33 @@ ed_op op_for(Sector& S, const TermsCopy& t, std::ofstream& out) {
 
 int popcount64(int64_t s) { return __builtin_popcountll((unsigned long long)s); }
 
-// Momentum_Basis.cpp:978-1037 (Unwrap_pState): every state of each orbit with
-// coefficient v_i * exp(-i k.g) / Norm_i, the first visit of a state kept
-// (idsClass.cpp:33-53), entries with |re| or |im| > 1e-8, sorted by state.
+// Momentum_Basis.cpp:978-1037 / :901-975 (Unwrap_pState), on the device
+// (ed_unwrap): every state of each orbit with coefficient v_i exp(-i k.g) /
+// Norm_i, first visit kept, entries with |re| or |im| > 1e-8, by state.
 struct Ids {
     int64_t stateID;
     cplx coeff;
 };
 
-cplx cexp_c(double im) {
-    double _Complex z;
-    __real__ z = 0.0;
-    __imag__ z = im;
-    double _Complex e = cexp(z);
-    return cplx(__real__ e, __imag__ e);
-}
-
-int64_t translate(int64_t s, const std::vector<int>& T) {
-    int64_t t = 0;
-    for (size_t i = 0; i < T.size(); i++)
-        if ((s >> i) & 1) t |= int64_t(1) << T[i];
-    return t;
-}
-
-// Momentum_Basis.cpp:901-975 for three generators: the z walk sits inside the
-// y walk with cumulative T3 steps, the phase multiplied by exp(-i pz) per step.
-std::vector<Ids> unwrap(const std::vector<cplx>& v, const std::vector<int64_t>& reps, const std::vector<double>& norm,
-                        double px, double py, double pz, const std::vector<int>& T1, const std::vector<int>& T2,
-                        const std::vector<int>& T3, int L1, int L2, int L3) {
-    std::vector<Ids> out;
-    const cplx ex = cexp_c(-px), ey = cexp_c(-py), ez = cexp_c(-pz);
-    for (size_t i = 0; i < reps.size(); i++) {
-        int64_t id = reps[i];
-        std::vector<Ids> tmp;
-        auto add = [&](int64_t sid, cplx c) {
-            for (auto& e : tmp)
-                if (e.stateID == sid) return;
-            tmp.push_back(Ids{sid, c});
-        };
-        add(id, v[i] / norm[i]);
-        cplx ph(1.0, 0.0);
-        for (int x = 0; x < L1; x++) {
-            if (x > 0) {
-                id = translate(id, T1);
-                ph = ph * ex;
-                add(id, v[i] * ph / norm[i]);
-            }
-            for (int y = 0; y < L2; y++) {
-                if (y > 0) {
-                    id = translate(id, T2);
-                    ph = ph * ey;
-                    add(id, v[i] * ph / norm[i]);
-                }
-                for (int z = 1; z < L3; z++) {
-                    id = translate(id, T3);
-                    ph = ph * ez;
-                    add(id, v[i] * ph / norm[i]);
-                }
-            }
-        }
-        for (auto& e : tmp)
-            if (std::fabs(e.coeff.real()) > 1e-8 || std::fabs(e.coeff.imag()) > 1e-8) out.push_back(e);
+std::vector<Ids> unwrap(Sector& S, const std::vector<cplx>& v, std::ofstream& out) {
+    std::vector<Ids> res;
+    if (S.D == 0) return res;
+    void* dv = nullptr;
+    if (hipMalloc(&dv, S.D * sizeof(cplx)) != hipSuccess) fatal(out, "unwrap: device allocation failed");
+    if (hipMemcpy(dv, v.data(), S.D * sizeof(cplx), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dv);
+        fatal(out, "unwrap: copy failed");
     }
-    std::stable_sort(out.begin(), out.end(), [](const Ids& a, const Ids& b) { return a.stateID < b.stateID; });
-    return out;
-}
-
-std::vector<Ids> unwrap(const std::vector<cplx>& v, const std::vector<int64_t>& reps, const std::vector<double>& norm,
-                        double px, double py, const std::vector<int>& T1, const std::vector<int>& T2, int L1, int L2) {
-    std::vector<Ids> out;
-    const cplx ex = cexp_c(-px), ey = cexp_c(-py);
-    for (size_t i = 0; i < reps.size(); i++) {
-        int64_t id = reps[i];
-        std::vector<Ids> tmp;
-        auto add = [&](int64_t sid, cplx c) {
-            for (auto& e : tmp)
-                if (e.stateID == sid) return;
-            tmp.push_back(Ids{sid, c});
-        };
-        add(id, v[i] / norm[i]);
-        cplx ph(1.0, 0.0);
-        for (int x = 0; x < L1; x++) {
-            if (x > 0) {
-                id = translate(id, T1);
-                ph = ph * ex;
-                add(id, v[i] * ph / norm[i]);
-            }
-            for (int y = 1; y < L2; y++) {
-                id = translate(id, T2);
-                ph = ph * ey;
-                add(id, v[i] * ph / norm[i]);
-            }
-        }
-        for (auto& e : tmp)
-            if (std::fabs(e.coeff.real()) > 1e-8 || std::fabs(e.coeff.imag()) > 1e-8) out.push_back(e);
-    }
-    std::stable_sort(out.begin(), out.end(), [](const Ids& a, const Ids& b) { return a.stateID < b.stateID; });
-    return out;
+    int64_t n = 0;
+    int rc = ed_unwrap(S.basis, dv, 0, nullptr, nullptr, &n);
+    std::vector<int64_t> st(n);
+    std::vector<cplx> co(n);
+    if (rc == ED_OK && n > 0) rc = ed_unwrap(S.basis, dv, n, st.data(), (double*)co.data(), &n);
+    (void)hipFree(dv);
+    check(rc, out, "ed_unwrap");
+    res.reserve(n);
+    for (int64_t q = 0; q < n; q++) res.push_back(Ids{st[q], co[q]});
+    return res;
 }
 
 void export_basis(Sector& S, std::vector<int64_t>& reps, std::vector<double>& norm, std::vector<double>& norm_inv,
@@ -286,8 +216,7 @@ void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, i
         f << "\n\n-------------------------------------------";
         f << "\nState Energy " << eigs[iib] << std::endl;
         if (unwrap_peigvec) {
-            std::vector<Ids> psi = three ? unwrap(vecs[iib], reps, norm, px, py, pz, S.T1, S.T2, S.T3, S.L1, S.L2, S.L3)
-                                         : unwrap(vecs[iib], reps, norm, px, py, S.T1, S.T2, S.L1, S.L2);
+            std::vector<Ids> psi = unwrap(S, vecs[iib], out);
             f << "\nEIGENVECTOR\n";
             for (auto& e : psi) f << e.stateID << "\t" << std::fixed << std::setprecision(15) << e.coeff << std::endl;
         } else {
@@ -421,6 +350,79 @@ void pConstruct_States(std::vector<int64_t>& v_sz, std::vector<double>& norm, st
                        std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile) {
     construct_impl(v_sz, norm, norm_inv, T1, T2, T3, num_ones, px, py, pz, n_states, rep_loc, state_ph, Ia, Ib,
                    bits_right, outfile);
+}
+
+// ---------------------------------------------------------------- makeH ----
+void makeH(const std::vector<int64_t>& pblock_states, const std::vector<double>&, const std::vector<double>&,
+           const std::vector<int64_t>&, const std::vector<std::complex<double> >&, const double& lambda,
+           const std::vector<coordinates>& adj_list, const double& Ch, const std::vector<triangles>& ch_list,
+           const double& ChCh, const std::vector<bowties>& chch_list, const int&, const std::vector<int64_t>&,
+           const std::vector<int64_t>&, const int&, const std::vector<int>& T1, const std::vector<int>& T2,
+           const double& kx, const double& ky, std::ofstream& outfile, zMatrix& H) {
+    const int64_t n = (int64_t)pblock_states.size();
+    H.resize(n, n);
+    if (n == 0) return;
+    Sector& S = sector(T1, T2, popcount64(pblock_states[0]), kx, ky, outfile);
+    if (S.D != n) fatal(outfile, "makeH: pblock_states does not match the GPU basis of this sector");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    check(ed_make_h(op, (double*)H.data()), outfile, "ed_make_h");
+}
+
+// ------------------------------------------------------- sparse sHv ----
+void sHv(sMatrix& pBij, std::complex<double>* v, std::complex<double>* w) {
+    const int64_t n = pBij.NRows();
+    std::vector<int64_t> rowptr(n + 1, 0), col;
+    std::vector<cplx> val;
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t m = pBij.RowSize(i);
+        for (int64_t j = 0; j < m; j++) {
+            ids e = pBij.me(i, j);
+            col.push_back(e.stateID);
+            val.push_back(e.coeff);
+        }
+        rowptr[i + 1] = (int64_t)col.size();
+    }
+    // the columns a row refers to: at most the largest index used
+    int64_t ncols = std::max<int64_t>(pBij.NCols(), n);
+    for (int64_t c : col) ncols = std::max(ncols, c + 1);
+    if (ed_csr_hv_host(n, ncols, rowptr.data(), col.data(), (const double*)val.data(), (const double*)v,
+                       (double*)w) != ED_OK) {
+        std::cerr << "sHv (sparse): " << ed_last_error() << std::endl;
+        std::exit(1);
+    }
+}
+
+// ------------------------------------------------- ARPACK_Iteration_EV ----
+void ARPACK_Iteration_EV(std::vector<int64_t>& pblock_states, std::vector<double>&, std::vector<double>&,
+                         std::vector<int64_t>&, std::vector<std::complex<double> >&, int&, int& n_it, double& px,
+                         double& py, std::vector<std::complex<double> >& Evals, zMatrix& Evecs, bool find_ev,
+                         double lambda, std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list,
+                         double& ChCh, std::vector<bowties>& chch_list, double tol, int maxiter,
+                         std::vector<int>& T1, std::vector<int>& T2, std::vector<int64_t>&, std::vector<int64_t>&,
+                         int&, std::ofstream& outfile) {
+    const int64_t n = (int64_t)pblock_states.size();
+    int nev = n_it;
+    if (nev >= n - 1 && n > 0) nev = (int)(n - 2);   // ARPACK_sector.cpp:34-35
+    if (n == 0 || nev <= 0) fatal(outfile, "ARPACK_Iteration_EV: sector too small for the Arnoldi iteration");
+    Sector& S = sector(T1, T2, popcount64(pblock_states[0]), px, py, outfile);
+    if (S.D != n) fatal(outfile, "ARPACK_Iteration_EV: basis mismatch");
+    ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
+    outfile << " ARPACK PARAMETERS " << std::endl;
+    outfile << " nev = " << nev << std::endl;
+    outfile << " n = " << n << std::endl;
+    outfile << " n_it = " << n_it << std::endl;
+    outfile << "----------------------\n" << std::endl;
+    const int iters = (int)std::min<int64_t>(n, std::max(4 * nev + 60, std::min(maxiter, 400)));
+    std::vector<cplx> eigs;
+    std::vector<std::vector<cplx>> vecs;
+    run_lanczos(op, S.D, nev, iters, 1, tol, ED_LZ_FULL, eigs, vecs, outfile);
+    if ((int)Evals.size() < nev) Evals.resize(nev);
+    for (int i = 0; i < nev; i++) Evals[i] = eigs[nev - 1 - i];   // largest of the wanted first
+    if (find_ev) {
+        Evecs.resize(n_it, n);                                     // ARPACK_sector.cpp:176-181
+        for (int i = 0; i < nev; i++)
+            for (int64_t j = 0; j < n; j++) Evecs(i, j) = vecs[nev - 1 - i][j];
+    }
 }
 
 // ---------------------------------------------------------- exact_evecs ----

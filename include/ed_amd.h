@@ -140,6 +140,34 @@ int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, void* stream,
 int ed_hv_host(ed_op op, const double* v, double* w);
 /* Dense D x D column-major complex128 H (makeH, H_v.cpp:21-190), D <= 20000. */
 int ed_make_h(ed_op op, double* H);
+/* w = A v for a caller-built sparse matrix in CSR form (the reference's
+ * sparse sHv, H_v.cpp:3-19, decl H_v.h:31-33): n rows, rowptr[n+1] starting
+ * at 0, col[rowptr[n]] in [0, ncols), val complex128 [rowptr[n]]; v
+ * complex128 [ncols], w complex128 [n]; HOST pointers, synchronous. */
+int ed_csr_hv_host(int64_t n, int64_t ncols, const int64_t* rowptr, const int64_t* col, const double* val,
+                   const double* v, double* w);
+
+/* Two-site spin correlations of momentum-basis vectors, on the device and
+ * without unwrapping (the reference unwraps the eigenvector files and sums
+ * over the full Sz basis: analysis/src/pspincorr.cpp:80-145 with szsz_op and
+ * sxsxsysy_op, analysis/src/chirality_exp.cpp:232-289).  For each pair p,
+ * (ij[2p], ij[2p+1]): out_zz[p] = <bra| Sz_i Sz_j |ket> and
+ * out_pm[p] = <bra| (S+_i S-_j + S-_i S+_j)/2 |ket> (complex128 each), with bra
+ * and ket DEVICE complex128 [D] vectors of basis b.  Both vectors must lie in
+ * the sector's momentum space (as every vector of the basis does); the value
+ * is that of any pair in the translation orbit of (i, j). */
+int ed_pair_correlations(ed_basis b, const void* bra, const void* ket, int32_t npairs, const int32_t* ij,
+                         double* out_zz, double* out_pm);
+
+/* Unwrap_pState (Momentum_Basis.cpp:978-1037; three generators :901-975) on
+ * the device: the full-Sz-basis components of the momentum-basis vector v
+ * (DEVICE complex128 [D]) -- every state of each orbit with coefficient
+ * v_a * exp(-i p.g) / Norm_a along the reference's walk, first visit kept,
+ * entries with |re| or |im| > 1e-8, ascending by state.  *count receives the
+ * number of entries; with capacity >= *count they are written to the HOST
+ * arrays states [count] and coeffs (complex128 [count]); capacity 0 only
+ * counts.  Needs C(N, n_ones) < 2^31. */
+int ed_unwrap(ed_basis b, const void* v, int64_t capacity, int64_t* states, double* coeffs, int64_t* count);
 
 /* All D eigenvalues (ascending) and, if evecs is non-NULL, the eigenvectors
  * (column-major D x D complex128, column k = eigenvector k) of the sector

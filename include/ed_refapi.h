@@ -14,6 +14,7 @@
 #include <complex>
 #include <cstdint>
 #include <fstream>
+#include <iostream>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,36 @@ struct bowties {                // global.h:37-40
     int p, q, r;
 };
 
+struct ids {                    // idsClass.h:8-11
+    int64_t stateID;
+    std::complex<double> coeff;
+};
+
+class sMatrix {                 // matrix.h:96-155: rows of (column, value) pairs
+    std::vector<std::vector<ids> > states;
+    int64_t Nr, Nc;
+
+public:
+    sMatrix() : Nr(0), Nc(0) {}
+    sMatrix(int64_t m, int64_t n) : states(m), Nr(m), Nc(n) {}
+    void clear() { states.clear(); }
+    void resize(int64_t m, int64_t n) {
+        states.clear();
+        states.resize(m);
+        Nr = m;
+        Nc = n;
+    }
+    int64_t NRows() { return Nr; }
+    int64_t NCols() { return Nc; }
+    int64_t RowSize(int64_t i) { return (int64_t)states[i].size(); }
+    ids me(int64_t i, int64_t j) { return states[i][j]; }
+    void push_back(int64_t id, int64_t stateID, std::complex<double> coeff) {
+        if (id < Nr) states[id].push_back(ids{stateID, coeff});
+        else std::cout << "\nError in Sparse Matrix Push_Back \n";
+    }
+    std::vector<ids> return_sVector(int64_t id) { return states[id]; }
+};
+
 class zMatrix : public std::vector<std::complex<double> > {   // matrix.h:53-91, column-major
     int64_t m;
     int64_t n;
@@ -50,6 +81,20 @@ public:
     std::complex<double>& operator()(int64_t i, int64_t j) { return operator[](i + j * m); }
     const std::complex<double>& operator()(int64_t i, int64_t j) const { return operator[](i + j * m); }
 };
+
+// H_v.h:9-28: the dense sector matrix H (D x D, D <= 20000), built on the GPU.
+// kx, ky are px, py in radians; the tables rep_loc ... bits_right are not read.
+void makeH(const std::vector<int64_t>& pblock_states, const std::vector<double>& norm,
+           const std::vector<double>& norm_inv, const std::vector<int64_t>& rep_loc,
+           const std::vector<std::complex<double> >& state_ph, const double& lambda,
+           const std::vector<coordinates>& adj_list, const double& Ch, const std::vector<triangles>& ch_list,
+           const double& ChCh, const std::vector<bowties>& chch_list, const int& n_sites,
+           const std::vector<int64_t>& Ia, const std::vector<int64_t>& Ib, const int& bits_right,
+           const std::vector<int>& T1, const std::vector<int>& T2, const double& kx, const double& ky,
+           std::ofstream& outfile, zMatrix& H);
+
+// H_v.h:31-33: w = pBij v for a caller-built sparse matrix, on the GPU.
+void sHv(sMatrix& pBij, std::complex<double>* v, std::complex<double>* w);
 
 // H_v.h:36-57.  kx, ky are px, py in radians (caller lanczos.cpp:232).  The
 // tables rep_loc, state_ph, Ia, Ib, bits_right are accepted and not read: the
@@ -133,6 +178,19 @@ void sector_lanczos(int& kx, int& ky, int& kz, double& cSz, int& n_spins, double
                     std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
                     std::vector<int>& T3, int& n_sec_ev, bool& find_ev, bool& unwrap_peigvec,
                     std::string& filepath_EV, double& tol, int& iterations, int& ncycles, std::ofstream& outfile);
+
+// ARPACK_sector.h:12-37: the n_it lowest eigenvalues ("SR"; n_it is lowered
+// to n-2 when n_it >= n-1) into Evals[0..n_it) largest first, as zneupd
+// returns them, and with find_ev the Ritz vectors as Evecs(i, j) (n_it x n).
+// Full-reorthogonalisation Lanczos on the GPU in place of znaupd/zneupd.
+void ARPACK_Iteration_EV(std::vector<int64_t>& pblock_states, std::vector<double>& norm,
+                         std::vector<double>& norm_inv, std::vector<int64_t>& rep_loc,
+                         std::vector<std::complex<double> >& state_ph, int& n_spins, int& n_it, double& px,
+                         double& py, std::vector<std::complex<double> >& Evals, zMatrix& Evecs, bool find_ev,
+                         double lambda, std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list,
+                         double& ChCh, std::vector<bowties>& chch_list, double tol, int maxiter,
+                         std::vector<int>& T1, std::vector<int>& T2, std::vector<int64_t>& Ia,
+                         std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile);
 
 // ARPACK_sector.h:38-66: lowest n_ev eigenvalues ("SR") of the sector.
 void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>& norm, std::vector<double>& norm_inv,

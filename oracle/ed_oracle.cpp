@@ -65,6 +65,14 @@ static inline int64_t flip2(int64_t v, int i, int j, bool ti, bool tj) {
 }
 
 // Simple_Math_Func.cpp:49-55
+// exp(re + i im) through the C++ library (the C library's cexp), as the
+// reference's walks form their step factors (Momentum_Basis.cpp:929,946,959).
+OR_API void or_cexp(double re, double im, double* out) {
+    std::complex<double> e = std::exp(std::complex<double>(re, im));
+    out[0] = e.real();
+    out[1] = e.imag();
+}
+
 OR_API int64_t or_n_choose_k(int n, int k) {
     if (k == 0) return 1;
     return (int64_t(n) * or_n_choose_k(n - 1, k - 1)) / k;

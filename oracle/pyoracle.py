@@ -49,6 +49,7 @@ def lib():
         if not os.path.exists(ORACLE_SO):
             raise RuntimeError(f"oracle library missing: {ORACLE_SO} (run `make -C oracle`)")
         L = C.CDLL(ORACLE_SO)
+        L.or_cexp.argtypes = [C.c_double, C.c_double, f64p]
         L.or_n_choose_k.restype = C.c_int64
         L.or_n_choose_k.argtypes = [C.c_int, C.c_int]
         L.or_gosper.argtypes = [C.c_int, C.c_int, i64p]
@@ -113,6 +114,8 @@ def ref():
         L.ref_basis_copy.argtypes = [C.c_void_p] + [C.c_void_p] * 7
         L.ref_hv.argtypes = [C.c_void_p, C.c_void_p, f64p, f64p, C.c_int64]
         L.ref_make_h.argtypes = [C.c_void_p, C.c_void_p, f64p]
+        L.ref_unwrap.restype = C.c_int64
+        L.ref_unwrap.argtypes = [C.c_void_p, f64p, C.c_int64, C.c_void_p, C.c_void_p]
         L.ref_tables_new.restype = C.c_void_p
         L.ref_tables_new.argtypes = [C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64,
                                      C.c_int]
@@ -196,6 +199,72 @@ class OracleBasis:
         lib().or_count_targets(self.h, C.byref(self.terms), C.byref(g), C.byref(t))
         return g.value, t.value
 
+    def unwrap(self, v):
+        """Unwrap_pState (Momentum_Basis.cpp:978-1037; three generators
+        :901-975): for each representative, the reference's walk (T1 steps
+        outer, T2 steps cumulative inside, T3 steps cumulative innermost), the
+        phase a running product of exp(-i px), exp(-i py), exp(-i pz) per step,
+        coefficient v_i * ph / Norm_i, the first visit of a state kept
+        (add2ids, idsClass.cpp:33-53); entries with |re| or |im| > 1e-8,
+        sorted by state.  Pure Python: small sectors only."""
+        t = self.tables(full=False)
+        m = self.model
+        T1, T2 = np.asarray(m.T1), np.asarray(m.T2)
+        three = getattr(m, "T3", None) is not None
+        T3 = np.asarray(m.T3) if three else None
+        L1, L2 = m.orders()
+        L3 = m.order3() if three else 1
+        def e(p):   # std::exp(complex(0, -p)): the C library's cexp (not (cos, sin): they differ in the last bit)
+            out = np.zeros(2)
+            lib().or_cexp(0.0, -p, out)
+            return complex(out[0], out[1])
+
+        ex, ey, ez = e(self.px), e(self.py), e(self.pz if three else 0.0)
+
+        def tr(s, T):
+            out = 0
+            i = 0
+            while s:
+                if s & 1:
+                    out |= 1 << int(T[i])
+                s >>= 1
+                i += 1
+            return out
+
+        def cmul(a, b):   # one rounding per product and per sum, as the C library's complex multiply
+            return complex(a.real * b.real - a.imag * b.imag, a.real * b.imag + a.imag * b.real)
+
+        out = []
+        for i, rep in enumerate(t["reps"]):
+            nrm = float(t["norm"][i])
+            vi = complex(v[i])
+            seen = {}
+            sid = int(rep)
+            seen[sid] = complex(vi.real / nrm, vi.imag / nrm)
+            ph = complex(1.0, 0.0)
+
+            def add(s, c):
+                if s not in seen:
+                    seen[s] = complex(c.real / nrm, c.imag / nrm)
+
+            for x in range(L1):
+                if x > 0:
+                    sid = tr(sid, T1)
+                    ph = cmul(ph, ex)
+                    add(sid, cmul(vi, ph))
+                for y in range(L2):
+                    if y > 0:
+                        sid = tr(sid, T2)
+                        ph = cmul(ph, ey)
+                        add(sid, cmul(vi, ph))
+                    for z in range(1, L3):
+                        sid = tr(sid, T3)
+                        ph = cmul(ph, ez)
+                        add(sid, cmul(vi, ph))
+            out.extend((s, c) for s, c in seen.items() if abs(c.real) > 1e-8 or abs(c.imag) > 1e-8)
+        out.sort(key=lambda e_: e_[0])
+        return np.array([s for s, _ in out], np.int64), np.array([c for _, c in out], np.complex128)
+
     def lanczos(self, nev=1, iterations=100, ncycles=10, tol=1e-11, seed=1234, want_vectors=False):
         ev = np.zeros(nev)
         vecs = np.zeros((nev, self.D), np.complex128) if want_vectors else None
@@ -259,6 +328,14 @@ class RefBasis:
         H = np.zeros(self.D * self.D, np.complex128)
         ref().ref_make_h(self.h, C.addressof(self.terms), H.view(np.float64))
         return H.reshape(self.D, self.D, order="F")
+
+    def unwrap(self, v):
+        v = np.ascontiguousarray(v, np.complex128)
+        n = ref().ref_unwrap(self.h, v.view(np.float64), 0, None, None)
+        st = np.zeros(n, np.int64)
+        co = np.zeros(2 * n)
+        ref().ref_unwrap(self.h, v.view(np.float64), n, _ptr(st), _ptr(co))
+        return st, co.view(np.complex128)
 
 
 class RefSHv:

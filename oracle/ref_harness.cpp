@@ -233,4 +233,21 @@ void ref_make_h(void* h, const void* terms, double* H) {
     std::memcpy(H, M.data(), M.size() * 16);
 }
 
+// Unwrap_pState (Momentum_Basis.cpp:978-1037, three generators :901-975) on
+// the basis' representatives and norms; returns the number of entries and
+// copies at most cap of them (states, complex128 coefficients).
+int64_t ref_unwrap(void* h, const double* v, int64_t cap, int64_t* states, double* coeffs) {
+    RefBasis* B = (RefBasis*)h;
+    std::vector<std::complex<double>> vc(B->reps.size());
+    for (size_t i = 0; i < vc.size(); i++) vc[i] = std::complex<double>(v[2 * i], v[2 * i + 1]);
+    std::vector<ids> psi = B->three ? Unwrap_pState(vc, B->reps, B->norm, B->px, B->py, B->pz, B->T1, B->T2, B->T3)
+                                    : Unwrap_pState(vc, B->reps, B->norm, B->px, B->py, B->T1, B->T2);
+    for (int64_t q = 0; q < (int64_t)psi.size() && q < cap; q++) {
+        states[q] = psi[q].stateID;
+        coeffs[2 * q] = psi[q].coeff.real();
+        coeffs[2 * q + 1] = psi[q].coeff.imag();
+    }
+    return (int64_t)psi.size();
+}
+
 }  // extern "C"

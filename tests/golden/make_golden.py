@@ -66,9 +66,16 @@ def main():
         path = os.path.join(HERE, f"{name}.npz")
         if os.path.exists(path) and not force:
             # keep the recorded file; advance the generator as writing it did
-            D = int(np.load(path, allow_pickle=False)["reps"].shape[0])
+            d = dict(np.load(path, allow_pickle=False))
+            D = int(d["reps"].shape[0])
             rng.normal(size=D); rng.normal(size=D)
-            print(f"{name}: kept")
+            if "unwrap_states" not in d:
+                # the reference's Unwrap_pState of the recorded v, added to the file
+                st, co = P.RefBasis(m, n, kx, ky, kz).unwrap(d["v"])
+                np.savez_compressed(path, **d, unwrap_states=st, unwrap_coef=co)
+                print(f"{name}: kept, unwrap added ({len(st)} entries)")
+            else:
+                print(f"{name}: kept")
             continue
         if P.OracleBasis(m, n, kx, ky, kz).D == 0:
             # pConstruct_States reads pBasis[0] of an empty list in findstate
@@ -83,6 +90,7 @@ def main():
         w = R.hv(v) if D else np.zeros(0, complex)
         H = R.make_h() if D else np.zeros((0, 0), complex)
         ev = np.linalg.eigvalsh(H) if D else np.zeros(0)
+        ust, uco = R.unwrap(v)
         a = m.term_arrays()
         extra = {} if m.T3 is None else dict(T3=m.T3, kz=kz)
         np.savez_compressed(
@@ -92,7 +100,8 @@ def main():
             bond_phase=a["bond_phase"][: a["nbonds"]], Ch=a["Ch"], chiral_ijk=a["chiral_ijk"][: a["nchiral"]],
             chiral_phase=a["chiral_phase"][: a["nchiral"]], ChCh=a["ChCh"], bowtie=a["bowtie"][: a["nbowtie"]],
             reps=t["reps"], norm=t["norm"], norm_inv=t["norm_inv"], rep_loc=t["rep_loc"], state_ph=t["state_ph"],
-            Ia=t["Ia"], Ib=t["Ib"], bits_right=R.bits_right, v=v, w=w, eigvals=ev,
+            Ia=t["Ia"], Ib=t["Ib"], bits_right=R.bits_right, v=v, w=w, eigvals=ev, unwrap_states=ust,
+            unwrap_coef=uco,
         )
         print(f"{name}: D={D} C={R.C}")
 

@@ -202,24 +202,29 @@ def momentum_projector(model, n, kx, ky, st):
 _hc = None
 
 
-def drive3d():
-    """tests/refapi/drive3d.cpp (a caller of the three-generator entry points
-    of include/ed_refapi.h) linked against ed_code_amd/libed_amd.so."""
-    out = os.path.join(HERE, "refapi", "_build", "drive3d")
-    src = os.path.join(HERE, "refapi", "drive3d.cpp")
+def refapi_driver():
+    """tests/refapi/driver.cpp (a caller of the entry points of
+    include/ed_refapi.h that the reference's main does not reach) linked
+    against ed_code_amd/libed_amd.so."""
+    out = os.path.join(HERE, "refapi", "_build", "driver")
+    src = os.path.join(HERE, "refapi", "driver.cpp")
     lib = os.path.join(ROOT, "ed_code_amd", "libed_amd.so")
-    if not os.path.exists(out) or os.path.getmtime(src) > os.path.getmtime(out):
+    hdr = os.path.join(ROOT, "include", "ed_refapi.h")
+    if not os.path.exists(out) or max(os.path.getmtime(src), os.path.getmtime(hdr)) > os.path.getmtime(out):
         os.makedirs(os.path.dirname(out), exist_ok=True)
         subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out,
                         "-L", os.path.dirname(lib), "-led_amd", "-Wl,-rpath,$ORIGIN/../../../ed_code_amd"], check=True)
     return out
 
 
-def write_spec3(path, model, n, k, nev=1, iterations=100, ncycles=10, tol=1e-10):
-    """Input of drive3d: sizes and solver settings, T1, T2, T3, bonds."""
+def write_spec(path, model, n, k, nev=1, iterations=100, ncycles=10, tol=1e-10):
+    """Input of the refapi driver: sizes and solver settings, T1, T2, T3 (the
+    identity for two generators), bonds."""
+    k = tuple(k) + (0,) * (3 - len(k))
+    T3 = model.T3 if model.T3 is not None else np.arange(model.n_sites)
     with open(path, "w") as f:
         f.write(f"{model.n_sites} {n} {k[0]} {k[1]} {k[2]} {model.lam!r} {iterations} {ncycles} {nev} {tol!r}\n")
-        for T in (model.T1, model.T2, model.T3):
+        for T in (model.T1, model.T2, T3):
             f.write(" ".join(str(int(t)) for t in T) + "\n")
         f.write(f"{len(model.bonds)}\n")
         for x, y, J, ph in model.bonds:

@@ -1,0 +1,78 @@
+"""Two-site correlations of momentum-basis vectors (ed_pair_correlations)
+against the same expectation values taken in the full Sz basis, as the
+reference's analysis programs take them (analysis/src/pspincorr.cpp:80-145,
+szsz_op / sxsxsysy_op at analysis/src/chirality_exp.cpp:232-289), on vectors
+mapped to the full basis by an independent projector."""
+import numpy as np
+import pytest
+
+import helpers as Hp
+import pyoracle as P
+from ed_code_amd import lattices as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _full_ops(st, i, j):
+    """Sz_i Sz_j (diagonal) and (S+_i S-_j + S-_i S+_j)/2 on the state list."""
+    idx = {s: a for a, s in enumerate(st)}
+    zz = np.array([((s >> i) & 1) - 0.5 for s in st]) * np.array([((s >> j) & 1) - 0.5 for s in st])
+    pm = np.zeros((len(st), len(st)))
+    for a, s in enumerate(st):
+        if ((s >> i) & 1) != ((s >> j) & 1):
+            pm[idx[s ^ (1 << i) ^ (1 << j)], a] = 0.5
+    return np.diag(zz), pm
+
+
+CASES = [(L.square(4, 4, J2=0.4), 8, (1, 2), [(0, 1), (0, 5), (0, 10), (3, 12)]),
+         (L.triangular(4, 3, Ch=0.2), 6, (0, 0), [(0, 1), (2, 7), (0, 6)]),
+         (L.chain(12), 6, (3, 0), [(0, 1), (0, 6), (5, 2)])]
+
+
+@pytest.mark.parametrize("model,n,k,pairs", CASES, ids=[f"{c[0].name}_k{c[2][0]}{c[2][1]}" for c in CASES])
+def test_pair_correlations_vs_full_basis(gpu_ready, model, n, k, pairs):
+    import torch
+    import ed_code_amd as ed
+    print(f"[case] correlations {model.name} k={k}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    basis = ed.Basis(ed.Lattice.of(model), n, *k)
+    H, st = Hp.full_space_h(model, n)
+    Pm = Hp.momentum_projector(model, n, *k, st)
+    assert Pm.shape[1] == o.D == basis.D
+    # the projector's convention is the library's: an eigenvector of the sector
+    # maps to an eigenvector of the full H
+    ev, vec = np.linalg.eigh(o.make_h())
+    psi = Pm @ vec[:, 0]
+    assert np.linalg.norm(H @ psi - ev[0] * psi) < 1e-9
+    rng = np.random.default_rng(3)
+    bra = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    kets = [vec[:, 0], vec[:, 1] if o.D > 1 else vec[:, 0], rng.normal(size=o.D) + 1j * rng.normal(size=o.D)]
+    for ket, b in ((kets[0], kets[0]), (kets[1], kets[0]), (kets[2], bra)):
+        tk = torch.tensor(ket, dtype=torch.complex128, device="cuda")
+        tb = torch.tensor(b, dtype=torch.complex128, device="cuda")
+        zz, pm = basis.pair_correlations(pairs, tk, tb)
+        for p, (i, j) in enumerate(pairs):
+            Zf, Pf = _full_ops(st, i, j)
+            fk, fb = Pm @ ket, Pm @ b
+            ref_zz, ref_pm = np.vdot(fb, Zf @ fk), np.vdot(fb, Pf @ fk)
+            assert abs(zz[p] - ref_zz) < 1e-12 * max(1.0, np.linalg.norm(fk) * np.linalg.norm(fb)), (i, j)
+            assert abs(pm[p] - ref_pm) < 1e-12 * max(1.0, np.linalg.norm(fk) * np.linalg.norm(fb)), (i, j)
+
+
+def test_pair_correlations_sum_rules(gpu_ready):
+    """Ground state of the 4x4 Heisenberg torus (Sz=0, k=0, a singlet):
+    sum_{j != 0} <S_0.S_j> = -3/4 (total spin zero), and the energy is
+    E0 = 16 (<S_0.S_1> + <S_0.S_4>) (16 bonds along x and 16 along y)."""
+    import torch
+    import ed_code_amd as ed
+    model = L.square(4, 4)
+    basis = ed.Basis(ed.Lattice.of(model), 8, 0, 0)
+    op = ed.Operator(basis, model)
+    ev, vecs, _ = op.lanczos(nev=1, iterations=200, ncycles=4, tol=1e-12, want_vectors=True)
+    psi = torch.tensor(vecs[0], dtype=torch.complex128, device="cuda")
+    pairs = [(0, j) for j in range(1, 16)]
+    zz, pm = basis.pair_correlations(pairs, psi)
+    sdots = zz.real + pm.real              # S_i.S_j = Sz Sz + (S+S- + S-S+)/2
+    assert abs(sdots.sum() + 0.75) < 1e-9
+    assert abs(16 * (sdots[0] + sdots[3]) - ev[0]) < 1e-9
+    assert np.abs(zz.imag).max() < 1e-12 and np.abs(pm.imag).max() < 1e-12

@@ -308,3 +308,36 @@ def test_gpu_three_generators_full_size_sum_rule(gpu_ready):
     y = torch.randn(basis.D, dtype=torch.complex128, device="cuda", generator=g)
     lhs, rhs = torch.vdot(x, op.hv(y)).item(), torch.vdot(op.hv(x), y).item()
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
+def test_gpu_unwrap_vs_golden(gpu_ready, path):
+    """ed_unwrap against the reference's own Unwrap_pState output recorded in
+    the fixture (states and coefficients bit for bit)."""
+    import torch
+    ed = _ed()
+    m, d = Hp.load_golden(path)
+    print(f"[case] unwrap {path.split('/')[-1]}", flush=True)
+    basis = ed.Basis(ed.Lattice.of(m), int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
+    v = torch.tensor(d["v"], dtype=torch.complex128, device="cuda")
+    st, co = basis.unwrap(v)
+    assert np.array_equal(st, d["unwrap_states"])
+    assert np.array_equal(co.view(np.float64), d["unwrap_coef"].view(np.float64))
+
+
+@pytest.mark.parametrize("name,model,n,k", [("chain20_k3", L.chain(20, phases=[0.1] * 20), 10, (3, 0)),
+                                            ("tri4x4_k13", L.triangular(4, 4, Ch=0.2), 8, (1, 3)),
+                                            ("tilted20_k71", L.tilted_square((4, 2), (-2, 4)), 10, (7, 1)),
+                                            ("cubic2x2x4_k013", L.cubic(2, 2, 4), 8, (0, 1, 3))])
+def test_gpu_unwrap_vs_oracle(gpu_ready, name, model, n, k):
+    import torch
+    ed = _ed()
+    print(f"[case] unwrap {name}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    basis = ed.Basis(ed.Lattice.of(model), n, *k)
+    rng = np.random.default_rng(21)
+    v = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    st, co = basis.unwrap(torch.tensor(v, dtype=torch.complex128, device="cuda"))
+    so, cob = o.unwrap(v)
+    assert np.array_equal(st, so)
+    assert np.array_equal(co.view(np.float64), cob.view(np.float64))

@@ -85,8 +85,8 @@ def test_reference_main_unwrapped_vector(gpu_ready, tmp_path):
 
 
 # ------------------------------------------------- three generators (3-D) ----
-def _drive3d(*args):
-    r = subprocess.run([Hp.drive3d(), *args], capture_output=True, text=True, timeout=180)
+def _driver(*args):
+    r = subprocess.run([Hp.refapi_driver(), *args], capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr[-2000:]
     return r.stdout
 
@@ -101,9 +101,9 @@ def test_three_generator_shv_vs_oracle(gpu_ready, tmp_path, model, n, k):
     o = P.OracleBasis(model, n, *k)
     v = np.random.default_rng(31).normal(size=o.D) + 1j * np.random.default_rng(32).normal(size=o.D)
     spec, fv, fw, fr = (str(tmp_path / x) for x in ("spec", "v.bin", "w.bin", "r.bin"))
-    Hp.write_spec3(spec, model, n, k)
+    Hp.write_spec(spec, model, n, k)
     v.astype(np.complex128).tofile(fv)
-    out = _drive3d("hv", spec, fv, fw, fr)
+    out = _driver("hv", spec, fv, fw, fr)
     assert f"D {o.D}" in out
     assert np.array_equal(np.fromfile(fr, np.int64), o.tables()["reps"])
     w = np.fromfile(fw, np.complex128)
@@ -119,8 +119,8 @@ def test_three_generator_sector_lanczos_and_vectors(gpu_ready, tmp_path):
     o = P.OracleBasis(model, n, *k)
     ev = np.linalg.eigvalsh(o.make_h())
     spec, log, vp = str(tmp_path / "spec"), str(tmp_path / "log.txt"), str(tmp_path / "vec")
-    Hp.write_spec3(spec, model, n, k, nev=2, iterations=120, ncycles=8, tol=1e-12)
-    _drive3d("sector", spec, log, vp, "0")
+    Hp.write_spec(spec, model, n, k, nev=2, iterations=120, ncycles=8, tol=1e-12)
+    _driver("sector", spec, log, vp, "0")
     text = open(log).read()
     assert f"\t{o.D}" in text and "kz" in text
     e = I.parse_energies(log)
@@ -133,7 +133,7 @@ def test_three_generator_sector_lanczos_and_vectors(gpu_ready, tmp_path):
     assert len(x) == o.D and abs(np.linalg.norm(x) - 1) < 1e-9
     assert np.linalg.norm(o.hv(x) - e[0] * x) < 1e-6
     # unwrapped (Momentum_Basis.cpp:901-975): an eigenvector of the full-space H
-    _drive3d("sector", spec, str(tmp_path / "log2.txt"), vp, "1")
+    _driver("sector", spec, str(tmp_path / "log2.txt"), vp, "1")
     rows = [l.split() for l in open(vp + "_EV0.txt") if l.strip() and l.split()[0].isdigit() and len(l.split()) == 2]
     ids = [int(r[0]) for r in rows]
     c = np.array([complex(*map(float, r[1].strip("()").split(","))) for r in rows])
@@ -145,3 +145,49 @@ def test_three_generator_sector_lanczos_and_vectors(gpu_ready, tmp_path):
     for s, a in zip(ids, c):
         psi[idx[s]] = a
     assert np.linalg.norm(H @ psi - e[0] * psi) < 1e-8
+
+
+# --------------------------- makeH, sparse sHv, ARPACK_Iteration_EV (2-D) ----
+# bond terms only: the driver's input carries bonds
+MAKEH = [(L.tilted_square((3, 1), (-1, 3), J2=0.5), 5, (1, 7)), (L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 2)),
+         (L.chain(14, phases=[0.2] * 14), 7, (3, 0))]
+
+
+@pytest.mark.parametrize("model,n,k", MAKEH, ids=[f"{m.name}_k{k[0]}{k[1]}" for m, n, k in MAKEH])
+def test_makeh_and_sparse_shv(gpu_ready, tmp_path, model, n, k):
+    """makeH (H_v.cpp:21-190) through the C++ interface against the oracle's
+    dense matrix; the sparse sHv (H_v.cpp:3-19) of the same matrix against
+    H v."""
+    print(f"[case] makeH {model.name} n={n} k={k}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    spec, fh, fv, fw = (str(tmp_path / x) for x in ("spec", "H.bin", "v.bin", "w.bin"))
+    Hp.write_spec(spec, model, n, k)
+    assert f"D {o.D}" in _driver("makeh", spec, fh)
+    H = np.fromfile(fh, np.complex128).reshape(o.D, o.D, order="F")
+    Ho = o.make_h()
+    assert np.abs(H - Ho).max() <= 1e-12 * max(1.0, np.abs(Ho).max())
+    v = np.random.default_rng(41).normal(size=o.D) + 1j * np.random.default_rng(42).normal(size=o.D)
+    v.astype(np.complex128).tofile(fv)
+    _driver("sparse", spec, fv, fw)
+    w = np.fromfile(fw, np.complex128)
+    assert np.linalg.norm(w - H @ v) <= 1e-12 * np.linalg.norm(H @ v)
+
+
+def test_arpack_iteration_ev(gpu_ready, tmp_path):
+    """ARPACK_Iteration_EV (ARPACK_sector.cpp:4-187): the n_it lowest
+    eigenvalues largest first, Evecs(i, j) the matching Ritz vectors."""
+    model, n, k = L.square(4, 4, J2=0.3), 8, (0, 0)
+    print("[case] ARPACK_Iteration_EV sq4x4", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    ev = np.linalg.eigvalsh(o.make_h())
+    spec, fe, fv = str(tmp_path / "spec"), str(tmp_path / "e.bin"), str(tmp_path / "V.bin")
+    Hp.write_spec(spec, model, n, k, nev=3, iterations=300, tol=1e-12)
+    out = _driver("arpack", spec, fe, fv)
+    assert f"D {o.D} nev 3 rows 3" in out
+    e = np.fromfile(fe, np.complex128)
+    assert np.abs(e.real - ev[:3][::-1]).max() < 1e-9 and np.abs(e.imag).max() == 0
+    V = np.fromfile(fv, np.complex128).reshape(3, o.D, order="F")
+    for i in range(3):
+        x = V[i]
+        assert abs(np.linalg.norm(x) - 1) < 1e-9
+        assert np.linalg.norm(o.hv(x) - e[i].real * x) < 1e-6

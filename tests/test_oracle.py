@@ -208,3 +208,31 @@ def test_oracle_vs_reference_slice(model, n, k):
     assert np.array_equal(to["state_ph"].view(np.float64), tr["state_ph"].view(np.float64))
     v = np.random.default_rng(1).normal(size=o.D) + 0j
     assert np.linalg.norm(o.hv(v) - r.hv(v)) <= 1e-14 * np.linalg.norm(r.hv(v))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
+def test_oracle_unwrap_matches_golden(path):
+    """The oracle's Unwrap_pState restatement against the reference's own
+    Unwrap_pState output recorded in the fixture, bit for bit."""
+    m, d = Hp.load_golden(path)
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
+    st, co = o.unwrap(d["v"])
+    assert np.array_equal(st, d["unwrap_states"])
+    assert np.array_equal(co.view(np.float64), d["unwrap_coef"].view(np.float64))
+
+
+def test_unwrap_is_the_projected_vector():
+    """Closed form: at a regular translation grid the unwrapped vector is the
+    momentum state sum_g exp(-i k.g) g|r> / (|G| / sqrt|orbit|) expanded in the
+    full basis, i.e. the projector's column times the coefficient."""
+    m, n, k = L.square(4, 4, J2=0.3), 8, (1, 2)
+    o = P.OracleBasis(m, n, *k)
+    rng = np.random.default_rng(9)
+    v = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    st, co = o.unwrap(v)
+    _, full = Hp.full_space_h(m, n)
+    Pm = Hp.momentum_projector(m, n, *k, full)
+    psi = Pm @ v
+    ref = {s: c for s, c in zip(full, psi) if abs(c.real) > 1e-8 or abs(c.imag) > 1e-8}
+    assert list(st) == sorted(ref)
+    assert np.abs(co - np.array([ref[s] for s in st])).max() < 1e-13
