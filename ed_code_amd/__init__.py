@@ -15,6 +15,7 @@ All compute runs in ed_code_amd/libed_amd.so through its C ABI
 from __future__ import annotations
 
 import ctypes as C
+import math
 from typing import Optional
 
 import numpy as np
@@ -78,6 +79,16 @@ class Basis:
         d = C.c_int64()
         check(lib().ed_basis_dim(h, C.byref(d)))
         self.D = d.value
+
+    def momenta(self):
+        """(px, py) = 2 pi (kx/L1, ky/L2) as Lanczos_sector.cpp:57 forms them;
+        (px, py, pz) on three-generator lattices (:240)."""
+        L = self.lattice
+        px = float(2 * self.kx) * math.pi / L.L1
+        py = float(2 * self.ky) * math.pi / L.L2
+        if L.T3 is None:
+            return px, py
+        return px, py, float(2 * self.kz) * math.pi / L.L3
 
     def export(self):
         """(pblock_states, norm, norm_inv) as host arrays."""

@@ -76,3 +76,29 @@ def test_pair_correlations_sum_rules(gpu_ready):
     assert abs(sdots.sum() + 0.75) < 1e-9
     assert abs(16 * (sdots[0] + sdots[3]) - ev[0]) < 1e-9
     assert np.abs(zz.imag).max() < 1e-12 and np.abs(pm.imag).max() < 1e-12
+
+
+def test_unwrap_to_files(gpu_ready, tmp_path):
+    """Eigenvector files of a solved sector: the binary twins hold exactly the
+    GPU unwrap and the momentum vector, the text files parse back to them."""
+    import torch
+    import ed_code_amd as ed
+    from ed_code_amd import evfiles as E
+    model = L.triangular(4, 3, Ch=0.2)
+    basis = ed.Basis(ed.Lattice.of(model), 6, 1, 1)
+    op = ed.Operator(basis, model)
+    ev, vecs, _ = op.lanczos(nev=1, iterations=150, ncycles=4, tol=1e-12, want_vectors=True)
+    psi = torch.tensor(vecs[0], dtype=torch.complex128, device="cuda")
+    prefix = str(tmp_path / "tri")
+    E.unwrap_to_files(basis, psi, prefix, 0, 0.0, (1, 1), 1.0, ev[0])
+    st, co = basis.unwrap(psi)
+    b = E.read_bin(prefix + "_EV0.bin")
+    assert np.array_equal(b["states"], st) and np.array_equal(b["coeffs"], co)
+    ts, tc = E.read_ev(prefix + "_EV0.txt")
+    assert np.array_equal(ts, st) and np.abs(tc - co).max() < 1e-15
+    assert abs(np.linalg.norm(co) - 1) < 1e-9
+    pb = E.read_bin(prefix + "_pEV0.bin")
+    reps, norm, _ = basis.export()
+    assert np.array_equal(pb["states"], reps) and np.array_equal(pb["coeffs"], vecs[0])
+    n_p, p, r, nm, c = E.read_pev(prefix + "_pEV0.txt")
+    assert n_p == basis.D and np.array_equal(r, reps) and np.abs(c - vecs[0]).max() < 1e-15
