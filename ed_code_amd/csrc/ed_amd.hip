@@ -566,10 +566,10 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
             op->torus = b->torus;
             op->nfam = (int)fam.size();
             op->ezz0 = ezz0;
-            bool bonds = true, real = true;
-            for (const TFam& F : fam) bonds = bonds && F.type == 0;
+            bool bond_only = true, real = true;
+            for (const TFam& F : fam) bond_only = bond_only && F.type == 0;
             for (const TEntry& E : ent) real = real && E.cim == 0.0;
-            op->tmode = !bonds ? kModeGeneral : (real && b->trivial_k ? kModeReal : kModeBonds);
+            op->tmode = !bond_only ? kModeGeneral : (real && b->trivial_k ? kModeReal : kModeBonds);
             op->tune_block = 512;
             HIPCHK(hipMalloc(&op->d_tfam, fam.size() * sizeof(TFam)));
             HIPCHK(hipMemcpy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
