@@ -39,6 +39,37 @@ def test_last_error_without_device_is_safe():
     assert lib.ed_last_error() is not None
 
 
+def test_argument_checks_before_any_device_call():
+    """Entry points reject bad arguments with ED_EINVAL before they touch the
+    GPU (so these run without one)."""
+    from ed_code_amd import _lib
+    lib = _lib.lib()
+    EINVAL = 1
+    n = C.c_int64(-1)
+    assert lib.ed_unwrap(None, None, 0, None, None, C.byref(n)) == EINVAL
+    assert lib.ed_pair_correlations(None, None, None, 1, None, None, None) == EINVAL
+    # CSR product: negative size, row pointers not starting at 0, decreasing,
+    # and a column out of range
+    v = np.zeros(4)
+    w = np.zeros(4)
+    assert lib.ed_csr_hv_host(-1, 2, None, None, None, None, None) == EINVAL
+    rp = np.array([1, 1, 2], np.int64)
+    col = np.array([0, 1], np.int64)
+    val = np.zeros(4)
+    assert lib.ed_csr_hv_host(2, 2, rp.ctypes.data, col.ctypes.data, val.ctypes.data, v.ctypes.data,
+                              w.ctypes.data) == EINVAL
+    rp = np.array([0, 2, 1], np.int64)
+    assert lib.ed_csr_hv_host(2, 2, rp.ctypes.data, col.ctypes.data, val.ctypes.data, v.ctypes.data,
+                              w.ctypes.data) == EINVAL
+    rp = np.array([0, 1, 2], np.int64)
+    bad = np.array([0, 5], np.int64)
+    assert lib.ed_csr_hv_host(2, 2, rp.ctypes.data, bad.ctypes.data, val.ctypes.data, v.ctypes.data,
+                              w.ctypes.data) == EINVAL
+    assert b"column 5" in lib.ed_last_error()
+    # an empty matrix is a no-op
+    assert lib.ed_csr_hv_host(0, 0, None, None, None, None, None) == 0
+
+
 def test_lattice_definitions():
     sq = L.square(4, 8, J2=0.5)
     assert sq.n_sites == 32 and len(sq.bonds) == 128 and sq.orders() == (4, 8)
