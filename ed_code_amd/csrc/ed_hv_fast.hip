@@ -4,7 +4,7 @@
 // One lane per row.  Per lane the antiparallel bonds of each family are the
 // set bits of one XOR; each set bit is one gathered element: flip, orbit
 // minimum by rotations (alignbit + min3 tree), byte-sliced rank from LDS,
-// 16-byte directory entry, then the gather of u = v/norm.  The rank tables,
+// 12-byte directory entry, then the gather of u = v/norm.  The rank tables,
 // family entries and phase table are staged in LDS once per block.
 #include <hip/hip_runtime.h>
 

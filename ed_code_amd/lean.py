@@ -6,7 +6,7 @@ The reference cannot hold this sector (rep_loc + state_ph over C(40,20) =
 1.4e11 states would take 3.3 TB, and its Lanczos indexes vectors with int,
 lanczos.cpp:188,313).  Here the cluster is relabelled so its translations are
 row rotations (lattices.regular_relabel), the basis keeps 8 B per
-representative plus 1.33 bit per state of the directory (50.6 GB), H*v runs
+representative plus 1.5 bit per state of the directory (53.4 GB), H*v runs
 the two-row 64-bit kernel, and the eigensolver keeps three complex64 Krylov
 vectors (ED_LZ_LEAN).
 
