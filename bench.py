@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD published
+VALU_CLOCK_HZ = 2.4e9   # MI355X peak engine clock, AMD published
 
 
 def log(*a):
@@ -120,6 +121,23 @@ def measure_traffic(workload, dtype, kernel_key, tune=None):
     shutil.rmtree(tmp, ignore_errors=True)
     return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write, "fetch_scale": fr,
             "write_scale": fw, "dispatches": len(f_k)}
+
+
+def measure_valu(workload, dtype, kernel_key, tune=None):
+    """VALU wave-instructions per launch of the dominant kernel (one
+    rocprofv3 pass of SQ_INSTS_VALU), for the integer-issue bound."""
+    import shutil
+    import statistics
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None
+    tmp = tempfile.mkdtemp(prefix="edprof_", dir="/tmp")
+    inner = ["--inner", "--workload", workload, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
+    if tune is not None:
+        inner += ["--tune", f"{tune[0]},{tune[1]}"]
+    vals = [v for n, v in _pmc_run("SQ_INSTS_VALU", inner, "valu", tmp) if kernel_key in n]
+    shutil.rmtree(tmp, ignore_errors=True)
+    return statistics.mean(vals) if vals else None
 
 
 def main():
@@ -270,6 +288,22 @@ def main():
             log(f"[bench] traffic {traffic}")
         except Exception as e:  # counters are reported, never required
             log(f"[bench] traffic measurement failed: {e!r}")
+    # secondary bound: the integer issue rate of the representative search
+    # (every distinct target is the orbit minimum over |G| translations)
+    secondary = {"group_order": basis.lattice.L1 * basis.lattice.L2,
+                 "candidate_translations_per_s": G * basis.lattice.L1 * basis.lattice.L2 / (r["kernel_ms"] * 1e-3)}
+    if not args.no_traffic:
+        try:
+            insts = measure_valu(args.workload, main_dt, r["kernel"], r["tune"])
+            if insts:
+                cus = torch.cuda.get_device_properties(0).multi_processor_count
+                # one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs per CU, 2.4 GHz
+                peak = cus * 4 * VALU_CLOCK_HZ / 4
+                secondary.update(valu_wave_insts=insts, valu_issue_peak_per_s=peak,
+                                 valu_issue_frac=insts / (r["kernel_ms"] * 1e-3) / peak)
+            log(f"[bench] secondary {secondary}")
+        except Exception as e:  # counters are reported, never required
+            log(f"[bench] VALU measurement failed: {e!r}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -295,7 +329,8 @@ def main():
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic["bytes"],
                      "traffic_detail": traffic,
-                     "kernel": r["kernel"], "block": r["block"], "kernel_ms": r["kernel_ms"], "alg_bytes_per_launch": r["alg_bytes"]},
+                     "kernel": r["kernel"], "block": r["block"], "kernel_ms": r["kernel_ms"], "alg_bytes_per_launch": r["alg_bytes"],
+                     "secondary": secondary},
         "cpu_baseline": cpu,
         "lanczos": lanczos,
         "other_precision": {k: v for k, v in results.items() if k != main_dt},
