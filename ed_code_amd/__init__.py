@@ -21,12 +21,12 @@ from typing import Optional
 import numpy as np
 
 from . import lattices
-from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_GRAPH, ED_LZ_LEAN, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, EDError, LanczosParams, LanczosStats, Terms,
+from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_GRAPH, ED_LZ_LEAN, ED_LZ_REFERENCE, ED_LZ_SELECTIVE, ED_LZ_THICK, EDError, LanczosParams, LanczosStats, Terms,
                    check, lib)
 from .lattices import Model, n_ones_for_sz
 
 __all__ = ["Lattice", "Basis", "Operator", "BlockOperator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
-           "ED_LZ_FULL", "ED_LZ_REFERENCE", "ED_LZ_LEAN", "ED_LZ_SELECTIVE", "n_ones_for_sz"]
+           "ED_LZ_FULL", "ED_LZ_REFERENCE", "ED_LZ_LEAN", "ED_LZ_SELECTIVE", "ED_LZ_THICK", "n_ones_for_sz"]
 
 _initialised = False
 

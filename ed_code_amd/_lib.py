@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("ED_AMD_LIB", os.path.join(HERE, "libed_amd.so"))
 
 ED_OK, ED_EINVAL, ED_ENOMEM, ED_EDEVICE, ED_EEMPTY, ED_ENOTCONV = range(6)
 ED_C64, ED_C128 = 0, 1
-ED_LZ_REFERENCE, ED_LZ_FULL, ED_LZ_LEAN = 0, 1, 2
+ED_LZ_REFERENCE, ED_LZ_FULL, ED_LZ_LEAN, ED_LZ_THICK = 0, 1, 2, 3
 ED_LZ_SELECTIVE, ED_LZ_GRAPH = 1, 2
 
 

@@ -488,6 +488,113 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
     return ED_OK;
 }
 
+// Thick-restart Lanczos (Wu and Simon, SIAM J. Matrix Anal. Appl. 22 (2000)
+// 602) for the nev lowest eigenpairs with at most m = iterations Krylov
+// vectors: the ARPACK-like solver of the "SR" drivers (ARPACK_sector.cpp:
+// 4-187 runs znaupd with ncv Arnoldi vectors).  Each expansion step
+// reorthogonalises against all kept vectors (two classical Gram-Schmidt
+// passes); the projected matrix is the Lanczos tridiagonal bordered by the
+// arrowhead row of the kept Ritz pairs.  After m vectors the Rayleigh-Ritz
+// pairs are formed (symeig_jacobi); if the nev lowest have residual
+// beta_m |y_m,i| <= tol max(1, |theta_i|) the run stops, else the
+// nev + (m - nev)/2 lowest Ritz vectors and the normalised residual start
+// the next cycle (at most ncycles cycles).
+template <typename VT>
+int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D;
+    const int nev = (int)std::min<int64_t>(p.nev, D);
+    const int m = (int)std::min<int64_t>(p.iterations, D);
+    if (m < nev + 1 && m < D) return ed_fail(ED_EINVAL, "thick restart needs iterations > nev (%d <= %d)", m, nev);
+    unsigned g = grid_for(D);
+    VT *V = nullptr, *w = nullptr;
+    cd2* W = nullptr;
+    HIPCHK(hipMalloc(&V, (size_t)(m + 1) * D * sizeof(VT)));
+    HIPCHK(hipMalloc(&w, D * sizeof(VT)));
+    HIPCHK(hipMalloc(&W, (size_t)m * D * sizeof(cd2)));
+    auto Vk = [&](int k) { return V + (int64_t)k * D; };
+    std::vector<std::complex<double>> q, q2;
+    hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed, Vk(0));
+    double nn;
+    RC(dnorm2(c, Vk(0), nn));
+    hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), Vk(0), Vk(0));
+    std::vector<double> T((size_t)m * m, 0.0), theta, Y;
+    int start = 0, meff = m, ncyc = 0;
+    double beta_m = 0.0;
+    bool done = false;
+    while (!done) {
+        meff = m;
+        for (int j = start; j < m; j++) {
+            RC(hv<VT>(c, Vk(j), w));
+            RC((multidot<VT, VT>(c, j + 1, V, D, w, q)));
+            RC((multiaxpy<VT, VT>(c, j + 1, V, D, q, 1.0, w)));
+            RC((multidot<VT, VT>(c, j + 1, V, D, w, q2)));
+            RC((multiaxpy<VT, VT>(c, j + 1, V, D, q2, 1.0, w)));
+            T[(size_t)j * m + j] = q[j].real() + q2[j].real();
+            double bb;
+            RC(dnorm2(c, w, bb));
+            const double beta = std::sqrt(bb);
+            if (beta < 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * m + j]))) {   // invariant subspace
+                meff = j + 1;
+                beta_m = 0.0;
+                break;
+            }
+            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, Vk(j + 1));
+            if (j + 1 < m) {
+                T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = beta;
+            } else {
+                beta_m = beta;   // V[m] holds the normalised residual
+            }
+        }
+        std::vector<double> Tm((size_t)meff * meff);
+        for (int i = 0; i < meff; i++)
+            for (int k = 0; k < meff; k++) Tm[(size_t)i * meff + k] = T[(size_t)i * m + k];
+        if (!symeig_jacobi(Tm, meff, theta, Y)) return ed_fail(ED_ENOTCONV, "Rayleigh-Ritz eigensolver did not converge");
+        bool conv = true;
+        for (int i = 0; i < std::min(nev, meff); i++)
+            if (beta_m * std::fabs(Y[(meff - 1) + (size_t)i * meff]) > p.tol * std::max(1.0, std::fabs(theta[i])))
+                conv = false;
+        ncyc++;
+        if (conv || beta_m == 0.0 || ncyc >= p.ncycles || meff < m) {
+            done = true;
+            break;
+        }
+        // restart with the kk lowest Ritz vectors and the residual
+        const int kk = std::min(nev + (meff - nev) / 2, meff - 1);
+        std::vector<double> y(meff);
+        for (int i = 0; i < kk; i++) {
+            cd2* Wi = W + (int64_t)i * D;
+            HIPCHK(hipMemsetAsync(Wi, 0, D * sizeof(cd2), c.st));
+            for (int l = 0; l < meff; l++) y[l] = Y[l + (size_t)i * meff];
+            RC((accumulate<VT>(c, meff, V, D, y, Wi)));
+        }
+        for (int i = 0; i < kk; i++)
+            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, W + (int64_t)i * D, Vk(i));
+        HIPCHK(hipMemcpyAsync(Vk(kk), Vk(m), D * sizeof(VT), hipMemcpyDeviceToDevice, c.st));
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < kk; i++) {
+            T[(size_t)i * m + i] = theta[i];
+            T[(size_t)i * m + kk] = T[(size_t)kk * m + i] = beta_m * Y[(meff - 1) + (size_t)i * meff];
+        }
+        start = kk;
+    }
+    // Ritz vectors of the nev lowest pairs
+    std::vector<double> y(meff);
+    for (int i = 0; i < std::min(nev, meff); i++) {
+        cd2* x = nullptr;
+        HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
+        HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+        for (int l = 0; l < meff; l++) y[l] = Y[l + (size_t)i * meff];
+        RC((accumulate<VT>(c, meff, V, D, y, x)));
+        RC(dnorm2(c, x, nn));
+        hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+        prev.push_back(x);
+        evals[i] = theta[i];
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    dfree(V); dfree(w); dfree(W);
+    return ED_OK;
+}
+
 // Memory-lean Lanczos: three Krylov vectors, no reorthogonalisation, lowest
 // eigenvalue; a second pass regenerates the vectors from the stored alpha and
 // beta to assemble the FP64 Ritz vector x (returned, unit norm).
@@ -1330,6 +1437,8 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
         }
     } else if (p->mode == ED_LZ_REFERENCE)
         rc = p->precision == ED_C128 ? run_reference<cd2>(c, *p, prev, evals) : run_reference<cf2>(c, *p, prev, evals);
+    else if (p->mode == ED_LZ_THICK)
+        rc = p->precision == ED_C128 ? run_trl<cd2>(c, *p, prev, evals) : run_trl<cf2>(c, *p, prev, evals);
     else
         rc = p->precision == ED_C128 ? run_full<cd2>(c, *p, prev, evals) : run_full<cf2>(c, *p, prev, evals);
     if (rc == ED_OK) {

@@ -412,10 +412,12 @@ void ARPACK_Iteration_EV(std::vector<int64_t>& pblock_states, std::vector<double
     outfile << " n = " << n << std::endl;
     outfile << " n_it = " << n_it << std::endl;
     outfile << "----------------------\n" << std::endl;
-    const int iters = (int)std::min<int64_t>(n, std::max(4 * nev + 60, std::min(maxiter, 400)));
+    // ncv = 4 nev Arnoldi vectors (ARPACK_sector.cpp:42), at least 2 nev + 40
+    // for the thick restart, maxiter restarts
+    const int ncv = (int)std::min<int64_t>(n, std::max(4 * nev, 2 * nev + 40));
     std::vector<cplx> eigs;
     std::vector<std::vector<cplx>> vecs;
-    run_lanczos(op, S.D, nev, iters, 1, tol, ED_LZ_FULL, eigs, vecs, outfile);
+    run_lanczos(op, S.D, nev, ncv, std::max(maxiter, 1), tol, ED_LZ_THICK, eigs, vecs, outfile);
     if ((int)Evals.size() < nev) Evals.resize(nev);
     for (int i = 0; i < nev; i++) Evals[i] = eigs[nev - 1 - i];   // largest of the wanted first
     if (find_ev) {
@@ -577,8 +579,8 @@ void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>&, st
     std::vector<cplx> eigs;
     std::vector<std::vector<cplx>> vecs;
     if (n_p > n_ev + 1) {
-        int iters = (int)std::min<int64_t>(n_p, std::max(4 * it + 60, std::min(maxiter, 400)));
-        run_lanczos(op, S.D, it, iters, 1, tol, ED_LZ_FULL, eigs, vecs, outfile);
+        const int ncv = (int)std::min<int64_t>(n_p, std::max(4 * it, 2 * it + 40));
+        run_lanczos(op, S.D, it, ncv, std::max(maxiter, 1), tol, ED_LZ_THICK, eigs, vecs, outfile);
     } else {
         std::vector<double> ev(S.D);
         std::vector<cplx> dv((size_t)S.D * S.D);

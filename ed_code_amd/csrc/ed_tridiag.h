@@ -88,4 +88,82 @@ inline bool tridiag_ql(std::vector<double>& d, std::vector<double> e, std::vecto
     return true;
 }
 
+// Eigenpairs of a dense real symmetric matrix A (m x m, row-major, read
+// symmetric from its upper triangle) by the cyclic Jacobi method with the
+// threshold strategy of Rutishauser (Handbook for Automatic Computation II,
+// contribution II/1): rotations zero the off-diagonal entries sweep by sweep
+// until their sum is below the rounding level.  d receives the eigenvalues
+// ascending, z (column-major, z[i + k*m]) the eigenvectors.  For the
+// thick-restart Lanczos matrices (arrowhead + tridiagonal, m up to a few
+// hundred).  Returns false if 100 sweeps did not converge.
+inline bool symeig_jacobi(std::vector<double> A, int m, std::vector<double>& d, std::vector<double>& z) {
+    d.assign(m, 0.0);
+    z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; i++) {
+        z[i + (size_t)i * m] = 1.0;
+        d[i] = A[(size_t)i * m + i];
+        for (int j = 0; j < i; j++) A[(size_t)i * m + j] = A[(size_t)j * m + i];
+    }
+    std::vector<double> b(d), zz(m, 0.0);
+    bool ok = false;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) off += std::fabs(A[(size_t)p * m + q]);
+        if (off == 0.0) { ok = true; break; }
+        const double tresh = sweep < 3 ? 0.2 * off / ((double)m * m) : 0.0;
+        for (int p = 0; p < m; p++) {
+            for (int q = p + 1; q < m; q++) {
+                double& apq = A[(size_t)p * m + q];
+                const double g = 100.0 * std::fabs(apq);
+                if (sweep > 3 && std::fabs(d[p]) + g == std::fabs(d[p]) && std::fabs(d[q]) + g == std::fabs(d[q])) {
+                    apq = 0.0;
+                    continue;
+                }
+                if (std::fabs(apq) <= tresh) continue;
+                double h = d[q] - d[p];
+                double t;
+                if (std::fabs(h) + g == std::fabs(h)) {
+                    t = apq / h;
+                } else {
+                    const double theta = 0.5 * h / apq;
+                    t = 1.0 / (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
+                    if (theta < 0.0) t = -t;
+                }
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c, tau = s / (1.0 + c);
+                h = t * apq;
+                zz[p] -= h; zz[q] += h;
+                d[p] -= h; d[q] += h;
+                apq = 0.0;
+                auto rot = [&](double& x, double& y) {
+                    const double gx = x, hy = y;
+                    x = gx - s * (hy + gx * tau);
+                    y = hy + s * (gx - hy * tau);
+                };
+                for (int j = 0; j < p; j++) rot(A[(size_t)j * m + p], A[(size_t)j * m + q]);
+                for (int j = p + 1; j < q; j++) rot(A[(size_t)p * m + j], A[(size_t)j * m + q]);
+                for (int j = q + 1; j < m; j++) rot(A[(size_t)p * m + j], A[(size_t)q * m + j]);
+                for (int j = 0; j < m; j++) rot(z[j + (size_t)p * m], z[j + (size_t)q * m]);
+            }
+        }
+        for (int p = 0; p < m; p++) {
+            b[p] += zz[p];
+            d[p] = b[p];
+            zz[p] = 0.0;
+        }
+    }
+    // ascending order
+    std::vector<int> idx(m);
+    for (int i = 0; i < m; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int c) { return d[a] < d[c]; });
+    std::vector<double> d2(m), z2((size_t)m * m);
+    for (int k = 0; k < m; k++) {
+        d2[k] = d[idx[k]];
+        for (int i = 0; i < m; i++) z2[i + (size_t)k * m] = z[i + (size_t)idx[k] * m];
+    }
+    d.swap(d2);
+    z.swap(z2);
+    return ok;
+}
+
 }  // namespace edc

@@ -196,7 +196,13 @@ int ed_dense_eigh(ed_op op, double* evals, double* evecs);
  * residual and Rayleigh quotient go to stats (the eigenvalue reported is the
  * Rayleigh quotient).  Setting the environment variable ED_PROGRESS prints
  * one line per step to stderr. */
-enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1, ED_LZ_LEAN = 2 };
+/* ED_LZ_THICK is thick-restart Lanczos, the ARPACK-like solver of the "SR"
+ * drivers (ARPACK_sector.cpp:4-187, znaupd with ncv Arnoldi vectors): the
+ * nev lowest eigenpairs with at most iterations (= ncv) Krylov vectors, each
+ * step reorthogonalised against all kept vectors, restarted from the
+ * nev + (ncv - nev)/2 lowest Ritz vectors and the residual until every
+ * wanted residual |beta y| <= tol max(1, |theta|) or ncycles restarts. */
+enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1, ED_LZ_LEAN = 2, ED_LZ_THICK = 3 };
 
 /* flags: ED_LZ_SELECTIVE makes the second Gram-Schmidt pass of ED_LZ_FULL
  * conditional: it runs only when the first pass removed more than

@@ -182,7 +182,8 @@ void sector_lanczos(int& kx, int& ky, int& kz, double& cSz, int& n_spins, double
 // ARPACK_sector.h:12-37: the n_it lowest eigenvalues ("SR"; n_it is lowered
 // to n-2 when n_it >= n-1) into Evals[0..n_it) largest first, as zneupd
 // returns them, and with find_ev the Ritz vectors as Evecs(i, j) (n_it x n).
-// Full-reorthogonalisation Lanczos on the GPU in place of znaupd/zneupd.
+// Thick-restart Lanczos on the GPU (ED_LZ_THICK, ncv = max(4 n_it, 2 n_it + 40)
+// vectors, maxiter restarts) in place of znaupd/zneupd.
 void ARPACK_Iteration_EV(std::vector<int64_t>& pblock_states, std::vector<double>& norm,
                          std::vector<double>& norm_inv, std::vector<int64_t>& rep_loc,
                          std::vector<std::complex<double> >& state_ph, int& n_spins, int& n_it, double& px,

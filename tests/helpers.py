@@ -266,6 +266,8 @@ def hostcore():
         H.hc_unrank.argtypes = [vp, C.c_uint64]
         H.hc_hv_torus.restype = C.c_int
         H.hc_hv_torus.argtypes = [vp] * 3
+        H.hc_symeig.restype = C.c_int
+        H.hc_symeig.argtypes = [C.c_int, vp, vp, vp]
         H.hc_hv_torus_p.restype = C.c_int
         H.hc_hv_torus_p.argtypes = [vp] * 3
         H.hc_torus_rep.restype = C.c_int

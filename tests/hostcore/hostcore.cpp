@@ -481,6 +481,16 @@ void hc_generic_rep(void* p, uint64_t t, uint64_t* rep, int* code) {
     *rep = find_rep<uint64_t, true>(t, h->G, *code);
 }
 
+// Eigenpairs of a dense symmetric matrix (row-major a[m*m]) with the
+// library's Jacobi solver: d <- ascending eigenvalues, z <- column-major vectors.
+int hc_symeig(int m, const double* a, double* d, double* z) {
+    std::vector<double> A(a, a + (size_t)m * m), dd, zz;
+    bool ok = symeig_jacobi(A, m, dd, zz);
+    std::memcpy(d, dd.data(), m * sizeof(double));
+    std::memcpy(z, zz.data(), (size_t)m * m * sizeof(double));
+    return ok ? 0 : 1;
+}
+
 // Eigenpairs of the symmetric tridiagonal (d[m], e[m-1]) with the library's QL
 // solver: d <- ascending eigenvalues, z <- column-major eigenvectors.
 int hc_tridiag(int m, double* d, const double* e, double* z) {

@@ -137,3 +137,26 @@ def test_graph_captured_lean_lanczos(gpu_ready, prec):
         assert abs(g[0] - eager[0]) < 1e-9 * abs(e_dense)
         assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-4)
         assert st["hv_calls"] > 30
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_thick_restart_lanczos_vs_dense(gpu_ready, prec):
+    """ED_LZ_THICK with a Krylov space too small to hold the answer without
+    restarts (m = 16 for 4 eigenpairs): the 4 lowest eigenvalues against dense
+    diagonalisation, unit Ritz vectors with small FP64 residuals."""
+    import ed_code_amd as ed
+    model, n, k = L.triangular(4, 4, Ch=0.25, J2=0.1), 8, (1, 3)
+    print(f"[case] thick restart {prec}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    ref = np.linalg.eigvalsh(o.make_h())[:4]
+    basis = ed.Basis(ed.Lattice.of(model), n, *k)
+    op = ed.Operator(basis, model)
+    ev, vecs, st = op.lanczos(nev=4, iterations=16, ncycles=400, tol=1e-11, seed=3, mode=ed.ED_LZ_THICK,
+                              precision=ed.ED_C128 if prec == "c128" else ed.ED_C64, want_vectors=True)
+    assert np.abs(ev - ref).max() < 1e-9 * max(1.0, np.abs(ref).max())
+    for i in range(4):
+        x = vecs[i]
+        assert abs(np.linalg.norm(x) - 1) < 1e-9
+        tol = 1e-6 if prec == "c128" else 1e-4
+        assert np.linalg.norm(o.hv(x) - ev[i] * x) < tol
+    assert st["hv_calls"] > 16                     # it did restart

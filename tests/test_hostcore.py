@@ -83,6 +83,35 @@ def test_tridiagonal_ql_vs_dense():
         assert np.abs(T @ Z - Z * dd).max() < 1e-11 * max(1, np.abs(ref).max())
 
 
+def test_dense_symmetric_jacobi_vs_numpy():
+    """The thick-restart Lanczos's Rayleigh-Ritz solver (ed_tridiag.h
+    symeig_jacobi) on random symmetric, arrowhead and degenerate matrices."""
+    rng = np.random.default_rng(7)
+    cases = []
+    for m in (1, 2, 5, 40, 120):
+        a = rng.normal(size=(m, m))
+        cases.append((a + a.T) / 2)
+    m = 30                                   # arrowhead + tridiagonal, as after a restart
+    a = np.diag(np.sort(rng.normal(size=m)))
+    k = 12
+    a[:k, k] = a[k, :k] = rng.normal(size=k)
+    for j in range(k, m - 1):
+        a[j, j + 1] = a[j + 1, j] = abs(rng.normal()) + 0.1
+    cases.append(a)
+    cases.append(np.diag([1.0, 1.0, 2.0, 2.0, 2.0, -3.0]))   # degenerate
+    for A in cases:
+        m = A.shape[0]
+        d = np.zeros(m)
+        z = np.zeros(m * m)
+        assert Hp.hostcore().hc_symeig(m, np.ascontiguousarray(A).ctypes.data, d.ctypes.data, z.ctypes.data) == 0
+        ref = np.linalg.eigvalsh(A)
+        scale = max(1.0, np.abs(ref).max())
+        assert np.abs(d - ref).max() < 1e-12 * scale
+        Z = z.reshape(m, m, order="F")
+        assert np.abs(A @ Z - Z * d).max() < 1e-11 * scale
+        assert np.abs(Z.T @ Z - np.eye(m)).max() < 1e-12
+
+
 TORUS = [
     (L.chain(16), 8, (0, 0)), (L.chain(16, phases=[0.3] * 16), 8, (5, 0)), (L.chain(13), 6, (2, 0)),
     (L.square(4, 4, J2=0.5), 8, (0, 0)), (L.square(4, 4, J2=0.5, lam=0.7), 8, (1, 3)),
