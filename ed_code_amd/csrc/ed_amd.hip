@@ -575,6 +575,22 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
             HIPCHK(hipMemcpy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
             HIPCHK(hipMalloc(&op->d_tent, ent.size() * sizeof(TEntry)));
             HIPCHK(hipMemcpy(op->d_tent, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+            // chiral terms on a built-in shape: merge each pair's terms into one element
+            std::vector<TFam> mfam;
+            std::vector<TEntry> ment;
+            std::vector<TChir> chx;
+            double mezz0 = 0;
+            if (!ch.empty() && torus_shape_compiled(b->torus) &&
+                torus_families_merged(op->lam, op->Ch, b->torus, bonds, ch, mfam, ment, chx, mezz0)) {
+                op->tmode = kModeMerged;
+                op->nmfam = (int)mfam.size();
+                HIPCHK(hipMalloc(&op->d_mfam, mfam.size() * sizeof(TFam)));
+                HIPCHK(hipMemcpy(op->d_mfam, mfam.data(), mfam.size() * sizeof(TFam), hipMemcpyHostToDevice));
+                HIPCHK(hipMalloc(&op->d_ment, ment.size() * sizeof(TEntry)));
+                HIPCHK(hipMemcpy(op->d_ment, ment.data(), ment.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+                HIPCHK(hipMalloc(&op->d_tchx, chx.size() * sizeof(TChir)));
+                HIPCHK(hipMemcpy(op->d_tchx, chx.data(), chx.size() * sizeof(TChir), hipMemcpyHostToDevice));
+            }
         }
     }
     // two-row 64-bit rotation path: no bowtie terms, compiled shape
@@ -632,7 +648,8 @@ extern "C" int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* var
 extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
-                    (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64})
+                    (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
+                    (void*)op->d_ment, (void*)op->d_tchx})
         if (p) dfree(p);
     delete op;
 }

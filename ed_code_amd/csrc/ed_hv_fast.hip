@@ -64,16 +64,20 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
 template <class TC, bool TRIV, int MODE, int K, bool PIPE, typename VT>
 __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                 const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
+                                                const TChir* __restrict__ gchx,
                                                 double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
                                                 int rtab_n, const DirEntry* __restrict__ dir,
                                                 const cd2* __restrict__ gphase, int nphase,
                                                 const double* __restrict__ ntab, const VT* __restrict__ u,
                                                 const VT* __restrict__ v, VT* __restrict__ w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    TEntry* ent = (TEntry*)smem;                     // nfam*32
+    TChir* chx = (TChir*)smem;                                   // nfam*32 (kModeMerged)
+    TEntry* ent = (TEntry*)(chx + (MODE == kModeMerged ? nfam * 32 : 0));   // nfam*32
     TFam* fam = (TFam*)(ent + nfam * 32);            // nfam
     cd2* phase = (cd2*)(fam + nfam + (nfam & 1));    // nphase (16-byte aligned)
     uint32_t* rtab = (uint32_t*)(phase + nphase);    // rtab_n
+    if (MODE == kModeMerged)
+        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) chx[q] = gchx[q];
     for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
@@ -102,9 +106,9 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
             ai = fma(ci, x.re, ai);
         }
     };
-    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, ezz0, R, rtab, dir, phase,
+    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, dir, phase,
                                                               (uint32_t)D, loadu, acc)
-                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, ezz0, R, rtab, dir, phase,
+                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, dir, phase,
                                                             (uint32_t)D, loadu, acc);
     cd2 vi = load_c(v, i);
     VT o;
@@ -115,10 +119,11 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
 
 #define ED_TORUS_C_ARGS                                                                                   \
     int64_t D, const uint64_t *__restrict__ rows, int nfam, const TFam *__restrict__ gfam,               \
-        const TEntry *__restrict__ gent, double ezz0, RankInfo R, const uint32_t *__restrict__ grtab,     \
+        const TEntry *__restrict__ gent, const TChir *__restrict__ gchx, double ezz0, RankInfo R,        \
+        const uint32_t *__restrict__ grtab,                                                               \
         int rtab_n, const DirEntry *__restrict__ dir, const cd2 *__restrict__ gphase, int nphase,         \
         const double *__restrict__ ntab, const VT *__restrict__ u, const VT *__restrict__ v, VT *__restrict__ w
-#define ED_TORUS_C_PASS D, rows, nfam, gfam, gent, ezz0, R, grtab, rtab_n, dir, gphase, nphase, ntab, u, v, w
+#define ED_TORUS_C_PASS D, rows, nfam, gfam, gent, gchx, ezz0, R, grtab, rtab_n, dir, gphase, nphase, ntab, u, v, w
 
 // Variant A: four elements in flight, occupancy left to the register count.
 template <class TC, bool TRIV, int MODE, typename VT>
@@ -150,17 +155,22 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
     const int64_t D = B->D;
     dim3 grid((unsigned)((D + bs - 1) / bs)), blk(bs);
     int nph = B->L1 * B->L2;
-#define ED_GO(KER) \
-    hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, op->nfam, op->d_tfam, op->d_tent, op->ezz0, B->rinfo, \
-                       B->d_rtab32, B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
+    // the merged families (bond pairs carrying their chiral entries) when built
+    const bool mg = op->tmode == kModeMerged;
+#define ED_GO(KER)                                                                                            \
+    hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, mg ? op->nmfam : op->nfam, mg ? op->d_mfam : op->d_tfam, \
+                       mg ? op->d_ment : op->d_tent, op->d_tchx, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,    \
+                       B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
     // term class: k = 0 takes the general or the real-bond kernel, k != 0 the
     // general or the bond-only one
 #define ED_GO_MODES(KERNEL)                                                         \
     if (B->trivial_k) {                                                             \
         if (op->tmode == kModeReal) ED_GO((KERNEL<TC, true, kModeReal, VT>));       \
+        else if (mg) ED_GO((KERNEL<TC, true, kModeMerged, VT>));                    \
         else ED_GO((KERNEL<TC, true, kModeGeneral, VT>));                           \
     } else {                                                                        \
-        if (op->tmode != kModeGeneral) ED_GO((KERNEL<TC, false, kModeBonds, VT>));  \
+        if (op->tmode == kModeBonds) ED_GO((KERNEL<TC, false, kModeBonds, VT>));    \
+        else if (mg) ED_GO((KERNEL<TC, false, kModeMerged, VT>));                   \
         else ED_GO((KERNEL<TC, false, kModeGeneral, VT>));                          \
     }
     if (var == 0) {
@@ -201,9 +211,11 @@ size_t torus_smem_bytes(const ed_op_s* op) {
 // operator's family count.
 static size_t torus_c_smem_bytes(const ed_op_s* op) {
     const ed_basis_s* B = op->b;
-    const int nf = op->nfam;
-    return (size_t)nf * 32 * sizeof(TEntry) + (size_t)(nf + (nf & 1)) * sizeof(TFam) +
-           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)B->rtab32_n * sizeof(uint32_t);
+    const bool mg = op->tmode == kModeMerged;
+    const int nf = mg ? op->nmfam : op->nfam;
+    return (mg ? (size_t)nf * 32 * sizeof(TChir) : 0) + (size_t)nf * 32 * sizeof(TEntry) +
+           (size_t)(nf + (nf & 1)) * sizeof(TFam) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
+           (size_t)B->rtab32_n * sizeof(uint32_t);
 }
 
 // Launches the compiled-shape kernel for the operator's lattice; false when

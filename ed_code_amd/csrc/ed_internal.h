@@ -73,7 +73,13 @@ struct ed_op_s {
     edc::TFam* d_tfam = nullptr;
     edc::TEntry* d_tent = nullptr;
     double ezz0 = 0;
-    int tmode = 0;             // term class of the families (kModeGeneral / kModeBonds / kModeReal)
+    int tmode = 0;             // term class of the families (kModeGeneral / kModeBonds / kModeReal / kModeMerged)
+    // kModeMerged: the compiled-shape kernels' bond-keyed families with the
+    // chiral entries of each pair (d_tfam / d_tent stay for the other kernels)
+    int nmfam = 0;
+    edc::TFam* d_mfam = nullptr;
+    edc::TEntry* d_ment = nullptr;
+    edc::TChir* d_tchx = nullptr;
     // two-row 64-bit rotation path (ed_hv64.hip)
     bool use_rows2 = false;
     int nfam64 = 0, nterm64 = 0;

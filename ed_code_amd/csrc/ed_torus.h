@@ -34,6 +34,16 @@ struct alignas(16) TEntry {
     int32_t y, k;       // flip partner, chiral z-site
 };
 
+// The chiral entries that flip the same pair (x, y) as a bond family member
+// (kModeMerged): at most two, each with its coefficient Ch/4 exp(i theta),
+// its z-site k, and whether its j-site is x (the entry lists the pair as
+// (y, x)) rather than y.  flags: bit 2c = entry c present, bit 2c+1 = j is x.
+struct alignas(16) TChir {
+    double cre[2], cim[2];
+    int32_t k[2];
+    int32_t flags, pad;
+};
+
 ED_HD uint32_t alignbit32(uint32_t hi, uint32_t lo, uint32_t sh) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __builtin_amdgcn_alignbit(hi, lo, sh);
@@ -383,8 +393,11 @@ ED_HD const DirEntry& dir_at(const DirEntry* dir, uint32_t idx) {
 //   kModeGeneral  bond and chiral families, complex coefficients;
 //   kModeBonds    bond families only;
 //   kModeReal     bond families only with real coefficients at k = 0 (the
-//                 caller accumulates with two FMAs and ignores ci).
-enum { kModeGeneral = 0, kModeBonds = 1, kModeReal = 2 };
+//                 caller accumulates with two FMAs and ignores ci);
+//   kModeMerged   bond-keyed families whose members also carry the chiral
+//                 entries of the same flipped pair (TChir): one orbit search,
+//                 rank and gather per pair instead of one per term.
+enum { kModeGeneral = 0, kModeBonds = 1, kModeReal = 2, kModeMerged = 3 };
 
 // One batch of up to K elements of a row: rank of the representative of each
 // target and its coefficient (phase applied); ok[q] false for empty slots.
@@ -398,7 +411,7 @@ struct RowBatch {
 // Fills the next batch of the row (the orbit search and rank of K
 // elements), advancing the family cursor (f, cur) and the diagonal ezz.
 template <class TC, bool TRIV, int MODE, int K>
-ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const RankInfo& R,
+ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx, const RankInfo& R,
                       const uint32_t* rtab, const cd2* phase, int& f, uint32_t& cur, double& ezz, RowBatch<K>& B) {
 #pragma unroll
     for (int q = 0; q < K; q++) {
@@ -417,6 +430,18 @@ ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, 
                 ezz -= E.dJ2;
                 cr = E.cre;
                 if (MODE != kModeReal) ci = ((s >> x) & 1u) ? -E.cim : E.cim;
+                if (MODE == kModeMerged) {
+                    // the chiral entries of the pair, in a fixed order
+                    const TChir& X = chx[f * 32 + x];
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        if (!((X.flags >> (2 * c)) & 1)) continue;
+                        const double sg = ((s >> X.k[c]) & 1u) ? 1.0 : -1.0;
+                        const uint32_t jbit = ((X.flags >> (2 * c + 1)) & 1) ? (s >> x) & 1u : (s >> E.y) & 1u;
+                        cr += -X.cim[c] * sg;
+                        ci += (jbit ? X.cre[c] : -X.cre[c]) * sg;
+                    }
+                }
             } else {
                 double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
                 cr = -E.cim * sg;
@@ -446,16 +471,16 @@ ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, 
 // element zidx of u, which holds zero, so it adds nothing whatever its
 // coefficient.  loadu(j) returns element j of u; g(x, cr, ci) accumulates.
 template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
-ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
-                               const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
-                               uint32_t zidx, LoadU&& loadu, Gf&& g) {
+ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
+                               double ezz0, const RankInfo& R, const uint32_t* rtab, const DirEntry* dir,
+                               const cd2* phase, uint32_t zidx, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
     bool more = true;
     while (more) {
         RowBatch<K> B;
-        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, B);
+        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
         more = B.ok[K - 1];
         DirEntry e[K];
 #pragma unroll
@@ -482,14 +507,14 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
 // integer work of the same lane.  Elements are accumulated in the same order
 // as torus_row_batched, so the result is bit-identical.
 template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
-ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, double ezz0,
-                                 const RankInfo& R, const uint32_t* rtab, const DirEntry* dir, const cd2* phase,
-                                 uint32_t zidx, LoadU&& loadu, Gf&& g) {
+ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
+                                 double ezz0, const RankInfo& R, const uint32_t* rtab, const DirEntry* dir,
+                                 const cd2* phase, uint32_t zidx, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
     RowBatch<K> A, B;
-    fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, A);
+    fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, A);
     DirEntry e[K];
     cd2 xa[K];
 #pragma unroll
@@ -501,7 +526,7 @@ ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TE
     }
     bool more = A.ok[K - 1];
     while (more) {
-        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, R, rtab, phase, f, cur, ezz, B);
+        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
 #pragma unroll
         for (int q = 0; q < K; q++) e[q] = dir_at(dir, B.rk[q] / kDirSpan);
 #pragma unroll

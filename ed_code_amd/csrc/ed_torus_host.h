@@ -96,4 +96,64 @@ inline bool torus_families(double lam, double Ch, const Torus& T, const std::vec
     return true;
 }
 
+// Families for kModeMerged: keyed by the bonds (pair x -> y = h(x)); each
+// chiral entry (i, j, k) joins the member whose pair is {i, j} (listed either
+// way), or opens a member with a zero bond part if no bond flips that pair.
+// False if a pair carries more than two chiral entries or the families do
+// not fit.
+inline bool torus_families_merged(double lam, double Ch, const Torus& T, const std::vector<Bond>& bonds,
+                                  const std::vector<Chiral>& ch, std::vector<TFam>& fam, std::vector<TEntry>& ent,
+                                  std::vector<TChir>& chx, double& ezz0) {
+    fam.clear();
+    ent.assign(kMaxFam * 32, TEntry{0, 0, 0, 0, 0});
+    TChir none;
+    std::memset(&none, 0, sizeof(none));
+    chx.assign(kMaxFam * 32, none);
+    ezz0 = 0;
+    auto find = [&](int x, int y) -> int {   // family whose member x flips (x, y)
+        int a, q;
+        rel_element(T, x, y, a, q);
+        for (size_t f = 0; f < fam.size(); f++)
+            if (fam[f].a == a && fam[f].q == q && ((fam[f].S >> x) & 1u) && ent[f * 32 + x].y == y) return (int)f;
+        return -1;
+    };
+    auto add = [&](int x, int y) -> int {    // new member x of a family of pair (x, y)'s key
+        int a, q;
+        rel_element(T, x, y, a, q);
+        for (size_t f = 0; f < fam.size(); f++)
+            if (fam[f].a == a && fam[f].q == q && !((fam[f].S >> x) & 1u)) {
+                fam[f].S |= 1u << x;
+                ent[f * 32 + x].y = y;
+                return (int)f;
+            }
+        if ((int)fam.size() >= kMaxFam) return -1;
+        fam.push_back(TFam{0, a, q, 1u << x});
+        ent[(fam.size() - 1) * 32 + x].y = y;
+        return (int)fam.size() - 1;
+    };
+    for (const Bond& b : bonds) {
+        const int f = add(b.x, b.y);
+        if (f < 0) return false;
+        ent[f * 32 + b.x] = TEntry{0.5 * b.J * b.re, 0.5 * b.J * b.im, 0.5 * lam * b.J, b.y, 0};
+        ezz0 += 0.25 * lam * b.J;
+    }
+    for (const Chiral& c : ch) {
+        int x = c.i, f = find(c.i, c.j);
+        bool rev = false;
+        if (f < 0 && (f = find(c.j, c.i)) >= 0) {
+            rev = true;
+            x = c.j;
+        }
+        if (f < 0 && (f = add(c.i, c.j)) < 0) return false;
+        TChir& X = chx[f * 32 + x];
+        const int slot = !(X.flags & 1) ? 0 : !(X.flags & 4) ? 1 : -1;
+        if (slot < 0) return false;
+        X.cre[slot] = 0.25 * Ch * c.re;
+        X.cim[slot] = 0.25 * Ch * c.im;
+        X.k[slot] = c.k;
+        X.flags |= (1 << (2 * slot)) | (rev ? 1 << (2 * slot + 1) : 0);
+    }
+    return true;
+}
+
 }  // namespace edc

@@ -268,6 +268,8 @@ def hostcore():
         H.hc_hv_torus.argtypes = [vp] * 3
         H.hc_symeig.restype = C.c_int
         H.hc_symeig.argtypes = [C.c_int, vp, vp, vp]
+        H.hc_torus_mode.restype = C.c_int
+        H.hc_torus_mode.argtypes = [vp]
         H.hc_hv_torus_p.restype = C.c_int
         H.hc_hv_torus_p.argtypes = [vp] * 3
         H.hc_torus_rep.restype = C.c_int

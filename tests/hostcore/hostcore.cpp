@@ -110,7 +110,8 @@ void hv(HC* h, const cd2* v, cd2* w) {
 }
 template <class TC, bool TRIV, int MODE, bool PIPE>
 void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
-                       double ezz0, const std::vector<uint32_t>& rt32, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+                const std::vector<TChir>& chx, double ezz0, const std::vector<uint32_t>& rt32,
+                const std::vector<cd2>& u, const cd2* v, cd2* w) {
     const uint32_t zidx = (uint32_t)h->rows.size();   // u[zidx] == 0
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
@@ -126,24 +127,30 @@ void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::
                 ai = std::fma(ci, x.re, ai);
             }
         };
-        double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), ezz0,
+        const TChir* cx = chx.empty() ? nullptr : chx.data();
+        double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
                                                                   h->R, rt32.data(), h->dir.data(), h->ph.data(),
                                                                   zidx, loadu, acc)
-                         : torus_row_batched<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), ezz0,
+                         : torus_row_batched<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
                                                                 h->R, rt32.data(), h->dir.data(), h->ph.data(), zidx,
                                                                 loadu, acc);
-        w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
+        w[i] = cd2{std::fma(nrm, ar, ez * v[i].re), std::fma(nrm, ai, ez * v[i].im)};
     }
 }
 
 template <class TC, bool PIPE>
 void hv_modes(HC* h, const Torus& T, bool triv, int mode, const std::vector<TFam>& fam,
-              const std::vector<TEntry>& ent, double ezz0, const std::vector<uint32_t>& rt32,
-              const std::vector<cd2>& u, const cd2* v, cd2* w) {
-    if (triv && mode == kModeReal) hv_batched<TC, true, kModeReal, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
-    else if (triv) hv_batched<TC, true, kModeGeneral, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
-    else if (mode != kModeGeneral) hv_batched<TC, false, kModeBonds, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
-    else hv_batched<TC, false, kModeGeneral, PIPE>(h, T, fam, ent, ezz0, rt32, u, v, w);
+              const std::vector<TEntry>& ent, const std::vector<TChir>& chx, double ezz0,
+              const std::vector<uint32_t>& rt32, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+#define HC_RUN(TRIV, MODE) hv_batched<TC, TRIV, MODE, PIPE>(h, T, fam, ent, chx, ezz0, rt32, u, v, w)
+    if (mode == kModeMerged) {
+        if (triv) HC_RUN(true, kModeMerged);
+        else HC_RUN(false, kModeMerged);
+    } else if (triv && mode == kModeReal) HC_RUN(true, kModeReal);
+    else if (triv) HC_RUN(true, kModeGeneral);
+    else if (mode != kModeGeneral) HC_RUN(false, kModeBonds);
+    else HC_RUN(false, kModeGeneral);
+#undef HC_RUN
 }
 
 template <class TC, bool TRIV>
@@ -436,18 +443,55 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
     bool bonds = true, real = true;
     for (const TFam& F : fam) bonds = bonds && F.type == 0;
     for (const TEntry& E : ent) real = real && E.cim == 0.0;
-    const int mode = !bonds ? kModeGeneral : (real && triv ? kModeReal : kModeBonds);
+    int mode = !bonds ? kModeGeneral : (real && triv ? kModeReal : kModeBonds);
+    std::vector<TChir> chx;
+    if (!h->c.empty()) {
+        std::vector<TFam> mf;
+        std::vector<TEntry> me;
+        double mz;
+        if (torus_families_merged(h->lam, h->Ch, T, h->b, h->c, mf, me, chx, mz)) {
+            fam.swap(mf);
+            ent.swap(me);
+            mode = kModeMerged;
+        } else {
+            chx.clear();
+        }
+    }
     int L2 = T.kind == 2 ? 1 : T.L2;
 #define HC_SHAPE(n, wd, l2)                                                                                  \
     if (T.N == n && T.w == wd && L2 == l2) {                                                                 \
         using TC = TorusC<n, wd, l2>;                                                                        \
-        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, ezz0, rt32, u, v, w);                       \
-        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, ezz0, rt32, u, v, w);                           \
+        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, u, v, w);                  \
+        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, u, v, w);                      \
         return 0;                                                                                            \
     }
     HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1) HC_SHAPE(24, 24, 1)
 #undef HC_SHAPE
     return 1;
+}
+
+// Term class the rotation fast path takes for this handle's terms (as
+// op_create picks it), -1 if the lattice does not take that path.
+int hc_torus_mode(void* p) {
+    HC* h = (HC*)p;
+    if (h->N > 32 || !h->w.empty() || h->G.L3 != 1) return -1;
+    Torus T;
+    std::vector<int> a(h->N), b(h->N);
+    for (int i = 0; i < h->N; i++) {
+        a[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T1));
+        b[i] = ctz64(apply_T<uint64_t>(1ull << i, h->G.T2));
+    }
+    if (!torus_detect(a, b, T)) return -1;
+    std::vector<TFam> fam, mf;
+    std::vector<TEntry> ent, me;
+    std::vector<TChir> chx;
+    double ezz0, mz;
+    if (!torus_families(h->lam, h->Ch, T, h->b, h->c, fam, ent, ezz0)) return -1;
+    if (!h->c.empty() && torus_families_merged(h->lam, h->Ch, T, h->b, h->c, mf, me, chx, mz)) return kModeMerged;
+    bool bonds = true, real = true;
+    for (const TFam& F : fam) bonds = bonds && F.type == 0;
+    for (const TEntry& E : ent) real = real && E.cim == 0.0;
+    return !bonds ? kModeGeneral : (real && h->kx == 0 && h->ky == 0 ? kModeReal : kModeBonds);
 }
 
 int hc_hv_torus_c(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, false); }

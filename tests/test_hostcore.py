@@ -174,7 +174,8 @@ def test_compiled_shape_search_matches_generic_walk(model, n, k):
 
 BATCHED = [(L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 3)), (L.square(4, 4, J2=0.5), 8, (0, 0)),
            (L.square(4, 5, J2=0.3), 10, (2, 1)), (L.triangular(6, 3, Ch=0.2, J2=0.1), 9, (3, 1)),
-           (L.chain(16, phases=[0.4] * 16), 8, (5, 0)), (L.triangular(4, 4, Ch=0.3), 7, (2, 3))]
+           (L.chain(16, phases=[0.4] * 16), 8, (5, 0)), (L.triangular(4, 4, Ch=0.3), 7, (2, 3)),
+           (L.triangular(4, 4, Ch=0.3), 8, (0, 0)), (L.triangular(4, 4, J1=0.0, Ch=0.3), 8, (1, 1))]
 
 
 @pytest.mark.parametrize("model,n,k", BATCHED, ids=[f"{m.name}_n{n}_k{k[0]}{k[1]}" for m, n, k in BATCHED])
@@ -188,6 +189,9 @@ def test_batched_compiled_row_vs_oracle(model, n, k):
     assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
     # the software-pipelined row accumulates in the same order: bit-identical
     assert np.array_equal(h.hv_torus_c(v, pipelined=True), w)
+    # lattices with chiral terms take the merged class (one element per pair)
+    mode = Hp.hostcore().hc_torus_mode(h.h)
+    assert mode == (3 if model.chiral and model.Ch else mode)
 
 
 def _rows2_cases():
