@@ -29,11 +29,15 @@ using namespace edc;
 
 namespace {
 
-struct BEnt {            // per term: flip partner y, chiral z-site k (-1 for a bond)
-    int32_t y, k;
+struct BEnt {            // per term: flip partner y, chiral z-site k0 (-1 for a bond)
+    int32_t y, k0;
+    int32_t k1, flags;   // merged terms: second chiral z-site; TChir flags
 };
-struct BCoef {           // per (term, instance)
+struct BCoef {           // per (term, instance): bond part (or the chiral entry)
     double cre, cim, dJ2;
+};
+struct BChir {           // per (term, instance), merged terms: the two chiral entries
+    double cre[2], cim[2];
 };
 
 inline unsigned nblk(int64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -68,24 +72,28 @@ __device__ __forceinline__ void st_c(cf2* p, double re, double im) { *p = cf2{(f
 // Same element order, coefficient expressions and accumulation order as the
 // single-instance rotation kernel (torus_row_batched).  Terms are numbered
 // densely: term = foff[f] + (number of sites of family f below x).
-template <class TC, bool TRIV, int M, int K, typename VT>
+template <class TC, bool TRIV, bool MG, int M, int K, typename VT>
 __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                     const TFam* __restrict__ gfam, const int* __restrict__ gfoff,
                                                     int nterm, const BEnt* __restrict__ gent,
-                                                    const BCoef* __restrict__ gcoef, const double* __restrict__ gezz0,
+                                                    const BCoef* __restrict__ gcoef, const BChir* __restrict__ gchir,
+                                                    const double* __restrict__ gezz0,
                                                     RankInfo R, const uint32_t* __restrict__ grtab, int rtab_n,
                                                     const DirEntry* __restrict__ dir, const cd2* __restrict__ gphase,
                                                     int nphase, const double* __restrict__ ntab,
                                                     const VT* __restrict__ u, const VT* __restrict__ v,
                                                     VT* __restrict__ w) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cd2* phase = (cd2*)smem;                                   // nphase
+    BChir* chir = (BChir*)smem;                                // nterm*M (merged terms)
+    cd2* phase = (cd2*)(chir + (MG ? nterm * M : 0));          // nphase
     BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M
     TFam* fam = (TFam*)(coef + nterm * M);                     // nfam
     int* foff = (int*)(fam + nfam);                            // nfam (+1 pad)
     BEnt* ent = (BEnt*)(foff + nfam + (nfam & 1));             // nterm
     double* ezz0 = (double*)(ent + nterm);                     // M
     uint32_t* rtab = (uint32_t*)(ezz0 + M);                    // rtab_n
+    if (MG)
+        for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) chir[q] = gchir[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
     for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) { fam[q] = gfam[q]; foff[q] = gfoff[q]; }
@@ -153,8 +161,11 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
 #pragma unroll
         for (int q = 0; q < K; q++) {
             const BEnt E = ent[ti[q]];
-            const bool bond = E.k < 0;
-            const bool sx = (s >> xs[q]) & 1u, sy = (s >> E.y) & 1u, sk = (s >> (E.k & 31)) & 1u;
+            const bool bond = MG || E.k0 < 0;
+            const bool sx = (s >> xs[q]) & 1u, sy = (s >> E.y) & 1u, sk = (s >> (E.k0 & 31)) & 1u;
+            // merged terms: signs and j-site bits of the two chiral entries
+            const double sg0 = ((s >> (E.k0 & 31)) & 1u) ? 1.0 : -1.0, sg1 = ((s >> (E.k1 & 31)) & 1u) ? 1.0 : -1.0;
+            const bool j0 = (E.flags & 2) ? sx : sy, j1 = (E.flags & 8) ? sx : sy;
             cd2 ph{1.0, 0.0};
             if (!TRIV) ph = phase[code[q]];
 #pragma unroll
@@ -165,6 +176,17 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
                     if (ok[q]) ez[m] -= C.dJ2;
                     cr = C.cre;
                     ci = sx ? -C.cim : C.cim;
+                    if (MG) {
+                        const BChir X = chir[ti[q] * M + m];
+                        if (E.flags & 1) {
+                            cr += -X.cim[0] * sg0;
+                            ci += (j0 ? X.cre[0] : -X.cre[0]) * sg0;
+                        }
+                        if (E.flags & 4) {
+                            cr += -X.cim[1] * sg1;
+                            ci += (j1 ? X.cre[1] : -X.cre[1]) * sg1;
+                        }
+                    }
                 } else {
                     const double sg = sk ? 1.0 : -1.0;
                     cr = -C.cim * sg;
@@ -201,6 +223,8 @@ struct ed_block_s {
     int* d_foff = nullptr;
     BEnt* d_ent = nullptr;
     BCoef* d_coef = nullptr;
+    BChir* d_chir = nullptr;        // merged terms (chiral entries on the bond pairs)
+    bool mg = false;
     double* d_ezz0 = nullptr;
     void* d_u64 = nullptr;          // prescaled interleaved gather stream, complex64 [D*M]
     void* d_u128 = nullptr;         // complex128 [D*M]
@@ -235,7 +259,8 @@ void union_lists(const ed_terms* T, int M, int m, std::vector<Bond>& bonds, std:
 size_t block_smem(const ed_block_s* K) {
     const ed_basis_s* B = K->b;
     const int nf = K->nfam;
-    return (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
+    return (K->mg ? (size_t)K->nterm * K->M * sizeof(BChir) : 0) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
+           (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
            (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)K->nterm * sizeof(BEnt) + (size_t)K->M * sizeof(double) +
            (size_t)B->rtab32_n * sizeof(uint32_t);
 }
@@ -248,14 +273,18 @@ void launch_block_c(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t 
     dim3 grid(nblk(B->D, bs)), blk(bs);
     size_t sm = block_smem(K);
     int nph = B->L1 * B->L2;
-    if (B->trivial_k)
-        hipLaunchKernelGGL((k_hv_block_c<TC, true, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam, K->d_fam,
-                           K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,
-                           B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
-    else
-        hipLaunchKernelGGL((k_hv_block_c<TC, false, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam,
-                           K->d_fam, K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_ezz0, B->rinfo, B->d_rtab32,
-                           B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
+#define ED_BGO(TRIV, MG)                                                                                         \
+    hipLaunchKernelGGL((k_hv_block_c<TC, TRIV, MG, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam, K->d_fam, \
+                       K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_chir, K->d_ezz0, B->rinfo, B->d_rtab32,          \
+                       B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
+    if (B->trivial_k) {
+        if (K->mg) ED_BGO(true, true);
+        else ED_BGO(true, false);
+    } else {
+        if (K->mg) ED_BGO(false, true);
+        else ED_BGO(false, false);
+    }
+#undef ED_BGO
 }
 
 template <int M, typename VT>
@@ -324,8 +353,8 @@ int block_size(const ed_block_s* K) { return K->M; }
 extern "C" void ed_block_destroy(ed_block blk) {
     if (!blk) return;
     for (auto* op : blk->ops) ed_op_destroy(op);
-    for (void* p : {(void*)blk->d_fam, (void*)blk->d_foff, (void*)blk->d_ent, (void*)blk->d_coef, (void*)blk->d_ezz0, blk->d_u64,
-                    blk->d_u128, blk->d_tmp})
+    for (void* p : {(void*)blk->d_fam, (void*)blk->d_foff, (void*)blk->d_ent, (void*)blk->d_coef, (void*)blk->d_chir,
+                    (void*)blk->d_ezz0, blk->d_u64, blk->d_u128, blk->d_tmp})
         if (p) dfree(p);
     delete blk;
 }
@@ -366,35 +395,63 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
         torus_shape_compiled(b->torus)) {
         std::vector<std::vector<TFam>> fams(M);
         std::vector<std::vector<TEntry>> ents(M);
+        std::vector<std::vector<TChir>> chxs(M);
         std::vector<double> ezz0(M);
-        bool ok = true;
-        for (int m = 0; m < M && ok; m++) {
-            std::vector<Bond> bonds;
-            std::vector<Chiral> ch;
-            union_lists(terms, M, m, bonds, ch);
-            ok = torus_families(terms[m].lambda, terms[m].Ch, b->torus, bonds, ch, fams[m], ents[m], ezz0[m]) &&
-                 !fams[m].empty();
-        }
-        for (int m = 1; m < M && ok; m++) {
-            ok = fams[m].size() == fams[0].size();
-            for (size_t f = 0; ok && f < fams[0].size(); f++)
-                ok = fams[m][f].type == fams[0][f].type && fams[m][f].a == fams[0][f].a &&
-                     fams[m][f].q == fams[0][f].q && fams[m][f].S == fams[0][f].S;
+        // with chiral terms, first the merged families (one term per flipped
+        // pair, as the single-instance kernels); else one term per entry
+        bool anych = false;
+        for (int m = 0; m < M; m++) anych |= terms[m].nchiral > 0 && std::fabs(terms[m].Ch) > 1e-16;
+        bool ok = false;
+        for (int attempt = anych ? 0 : 1; attempt < 2 && !ok; attempt++) {
+            const bool merged = attempt == 0;
+            ok = true;
+            for (int m = 0; m < M && ok; m++) {
+                std::vector<Bond> bonds;
+                std::vector<Chiral> ch;
+                union_lists(terms, M, m, bonds, ch);
+                ok = (merged ? torus_families_merged(terms[m].lambda, terms[m].Ch, b->torus, bonds, ch, fams[m],
+                                                     ents[m], chxs[m], ezz0[m])
+                             : torus_families(terms[m].lambda, terms[m].Ch, b->torus, bonds, ch, fams[m], ents[m],
+                                              ezz0[m])) &&
+                     !fams[m].empty();
+            }
+            for (int m = 1; m < M && ok; m++) {
+                ok = fams[m].size() == fams[0].size();
+                for (size_t f = 0; ok && f < fams[0].size(); f++) {
+                    ok = fams[m][f].type == fams[0][f].type && fams[m][f].a == fams[0][f].a &&
+                         fams[m][f].q == fams[0][f].q && fams[m][f].S == fams[0][f].S;
+                    for (int x = 0; ok && merged && x < 32; x++) {
+                        const TChir &A = chxs[m][f * 32 + x], &C0 = chxs[0][f * 32 + x];
+                        ok = A.flags == C0.flags && A.k[0] == C0.k[0] && A.k[1] == C0.k[1];
+                    }
+                }
+            }
+            K->mg = ok && merged;
         }
         if (ok) {
             const int nf = (int)fams[0].size();
             std::vector<int> foff(nf);
             std::vector<BEnt> be;
             std::vector<BCoef> bc;
+            std::vector<BChir> bx;
             for (int f = 0; f < nf; f++) {
                 foff[f] = (int)be.size();
                 for (int x = 0; x < 32; x++) {
                     if (!((fams[0][f].S >> x) & 1u)) continue;
                     const TEntry& E0 = ents[0][f * 32 + x];
-                    be.push_back(BEnt{E0.y, fams[0][f].type == 0 ? -1 : E0.k});
+                    if (K->mg) {
+                        const TChir& X0 = chxs[0][f * 32 + x];
+                        be.push_back(BEnt{E0.y, X0.k[0], X0.k[1], X0.flags});
+                    } else {
+                        be.push_back(BEnt{E0.y, fams[0][f].type == 0 ? -1 : E0.k, 0, 0});
+                    }
                     for (int m = 0; m < M; m++) {
                         const TEntry& E = ents[m][f * 32 + x];
                         bc.push_back(BCoef{E.cre, E.cim, E.dJ2});
+                        if (K->mg) {
+                            const TChir& X = chxs[m][f * 32 + x];
+                            bx.push_back(BChir{{X.cre[0], X.cre[1]}, {X.cim[0], X.cim[1]}});
+                        }
                     }
                 }
             }
@@ -409,6 +466,7 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
             if (rc == ED_OK) rc = up((void**)&K->d_foff, foff.data(), nf * sizeof(int));
             if (rc == ED_OK) rc = up((void**)&K->d_ent, be.data(), be.size() * sizeof(BEnt));
             if (rc == ED_OK) rc = up((void**)&K->d_coef, bc.data(), bc.size() * sizeof(BCoef));
+            if (rc == ED_OK && K->mg) rc = up((void**)&K->d_chir, bx.data(), bx.size() * sizeof(BChir));
             if (rc == ED_OK) rc = up((void**)&K->d_ezz0, ezz0.data(), M * sizeof(double));
             if (rc != ED_OK) {
                 ed_block_destroy(K);
