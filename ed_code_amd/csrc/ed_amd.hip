@@ -155,6 +155,25 @@ __global__ void k_dir_fill(const uint32_t* words, const uint64_t* base, uint64_t
     }
 }
 
+// Marks the gather-image chunks (2^(6 + shift) ranks each) that hold a
+// representative of the sector.  Threads of one chunk store the same byte.
+__global__ void k_image_touch(const DirEntry* dir, uint64_t ngroups, int shift, uint8_t* touched) {
+    ED_GRID_STRIDE(g, ngroups) {
+        const DirEntry e = dir[g];
+        if (e.b0 | e.b1) touched[g >> shift] = 1;
+    }
+}
+
+// Image element of every row: the rank of its representative through the
+// chunk map.
+__global__ void k_image_pos(uint64_t D, const uint64_t* rows, RankInfo R, const uint64_t* rtab, const uint16_t* map,
+                            int cb, uint32_t* pos) {
+    ED_GRID_STRIDE(i, D) {
+        const uint32_t r = (uint32_t)rank_of<uint32_t>((uint32_t)(rows[i] & kStateMask), R, rtab);
+        pos[i] = ((uint32_t)map[r >> cb] << cb) | (r & ((1u << cb) - 1u));
+    }
+}
+
 template <typename S>
 __global__ void k_compact(uint64_t C, int N, int n, Geo G, const uint64_t* binom, const DirEntry* dir,
                           uint64_t ngroups, uint64_t* rows) {
@@ -200,23 +219,42 @@ __global__ void k_full_tables(uint64_t C, int N, int n, Geo G, const uint64_t* b
     }
 }
 
-// u = v / norm (the gathered stream of the H*v kernel), and u[D] = 0: the
-// element that the empty slots of the batched rows read.
+// u = v / norm, the gathered stream of the H*v kernels that address u by
+// row through the directory.
 template <typename VT>
 __global__ void k_prescale(int64_t D, const uint64_t* rows, const double* nitab, const VT* v, VT* u) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == D) {
-        VT z;
-        z.re = 0;
-        z.im = 0;
-        u[D] = z;
-    }
     if (i >= D) return;
     double f = nitab[(int)(rows[i] >> kLidShift)];
     VT x = v[i];
     x.re = x.re * f;
     x.im = x.im * f;
     u[i] = x;
+}
+
+// The same values written into the gather image (element pos[i] for row i;
+// ascending with i, so the stores of a wave fall into few lines).
+template <typename VT>
+__global__ void k_prescale_img(int64_t D, const uint64_t* rows, const double* nitab, const uint32_t* pos, const VT* v,
+                               VT* img) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    double f = nitab[(int)(rows[i] >> kLidShift)];
+    VT x = v[i];
+    x.re = x.re * f;
+    x.im = x.im * f;
+    img[pos[i]] = x;
+}
+
+// Zero fill with a kernel (no memset node if the caller captures the stream).
+template <typename VT>
+__global__ void k_zero(uint64_t n, VT* p) {
+    ED_GRID_STRIDE(i, n) {
+        VT z;
+        z.re = 0;
+        z.im = 0;
+        p[i] = z;
+    }
 }
 
 template <typename VT>
@@ -406,13 +444,51 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     dfree(tmp); dfree(words); dfree(cnt); dfree(base);
+    return basis_image_build(B);
+}
+
+// Gather image of the sector (ImageLayout): the smallest chunk size whose
+// map fits in LDS and whose image stays within the size bounds.  Sectors of
+// shapes without compiled kernels, or whose image would be too large, get
+// none and run the directory kernels.
+int basis_image_build(ed_basis_s* B) {
+    B->img_cb = -1;
+    if (B->N > 32 || B->D == 0 || !B->has_torus || !torus_shape_compiled(B->torus)) return ED_OK;
+    const uint64_t C = B->C;
+    const int cb0 = image_min_bits(C);
+    ImageLayout L;
+    uint8_t* d_t = nullptr;
+    HIPCHK(hipMalloc(&d_t, ((C - 1) >> cb0) + 1));
+    for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
+        const uint64_t nmap = ((C - 1) >> cb) + 1;
+        std::vector<uint8_t> t(nmap);
+        HIPCHK(hipMemset(d_t, 0, nmap));
+        hipLaunchKernelGGL(k_image_touch, dim3(sblocks(B->ngroups, 256)), dim3(256), 0, 0, B->d_dir, B->ngroups,
+                           cb - 6, d_t);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(t.data(), d_t, nmap, hipMemcpyDeviceToHost));
+        image_layout(C, B->D, cb, t, L);
+    }
+    dfree(d_t);
+    if (L.cb < 0) return ED_OK;
+    HIPCHK(hipMalloc(&B->d_img_map, L.map.size() * sizeof(uint16_t)));
+    HIPCHK(hipMemcpy(B->d_img_map, L.map.data(), L.map.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&B->d_img_pos, (size_t)B->D * sizeof(uint32_t)));
+    hipLaunchKernelGGL(k_image_pos, dim3(sblocks((uint64_t)B->D, 256)), dim3(256), 0, 0, (uint64_t)B->D, B->d_rows,
+                       B->rinfo, B->d_rtab, B->d_img_map, L.cb, B->d_img_pos);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    B->img_cb = L.cb;
+    B->img_nslot = L.nslot;
+    B->img_nmap = (uint32_t)L.map.size();
     return ED_OK;
 }
 
 void delete_basis(ed_basis_s* B) {
     if (!B) return;
     for (void* p : {(void*)B->d_rows, (void*)B->d_dir, (void*)B->d_rtab, (void*)B->d_binom, (void*)B->d_phase,
-                    (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32, (void*)B->d_rlo, (void*)B->d_rhi})
+                    (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32, (void*)B->d_rlo, (void*)B->d_rhi,
+                    (void*)B->d_img_map, (void*)B->d_img_pos})
         if (p) dfree(p);
     delete B;
 }
@@ -649,7 +725,7 @@ extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
                     (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
-                    (void*)op->d_ment, (void*)op->d_tchx})
+                    (void*)op->d_ment, (void*)op->d_tchx, op->d_img64, op->d_img128})
         if (p) dfree(p);
     delete op;
 }
@@ -664,9 +740,27 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
     if (D == 0) return ED_OK;
-    VT*& u = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
-    if (!u) HIPCHK(hipMalloc((void**)&u, (D + 1) * sizeof(VT)));
-    hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D + 1, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, u);
+    const bool rows2 = op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus));
+    const bool img = !rows2 && torus_uses_image(op);
+    VT* u = nullptr;
+    if (img) {
+        // the compiled rotation kernels gather from the image; its zero
+        // positions are written once, at allocation
+        VT*& im = std::is_same<VT, cd2>::value ? (VT*&)op->d_img128 : (VT*&)op->d_img64;
+        const uint64_t ne = (uint64_t)(B->img_nslot + 1) << B->img_cb;
+        if (!im) {
+            HIPCHK(hipMalloc((void**)&im, ne * sizeof(VT)));
+            hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, im);
+        }
+        hipLaunchKernelGGL(k_prescale_img<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab,
+                           B->d_img_pos, v, im);
+        u = im;
+    } else {
+        VT*& uu = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
+        if (!uu) HIPCHK(hipMalloc((void**)&uu, D * sizeof(VT)));
+        hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, uu);
+        u = uu;
+    }
     HIPCHK(hipGetLastError());
     Geo G;
     basis_geo(B, G);
@@ -678,7 +772,7 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
         if (rc != ED_OK) return rc;
     }
     if (ev0) HIPCHK(hipEventRecord(ev0, st));
-    if (op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus))) {
+    if (rows2) {
         int rc = launch_hv_rows2<VT>(op, u, v, w, st);
         if (rc != ED_OK) return rc;
     } else if (op->use_torus && op->tune_path != 1) {

@@ -66,8 +66,8 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
                                                 const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
                                                 const TChir* __restrict__ gchx,
                                                 double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
-                                                int rtab_n, const DirEntry* __restrict__ dir,
-                                                const cd2* __restrict__ gphase, int nphase,
+                                                int rtab_n, const uint16_t* __restrict__ gmap, int nmap,
+                                                int cb, uint32_t zidx, const cd2* __restrict__ gphase, int nphase,
                                                 const double* __restrict__ ntab, const VT* __restrict__ u,
                                                 const VT* __restrict__ v, VT* __restrict__ w) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -76,12 +76,14 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     TFam* fam = (TFam*)(ent + nfam * 32);            // nfam
     cd2* phase = (cd2*)(fam + nfam + (nfam & 1));    // nphase (16-byte aligned)
     uint32_t* rtab = (uint32_t*)(phase + nphase);    // rtab_n
+    uint16_t* map = (uint16_t*)(rtab + rtab_n);      // nmap: gather-image chunk -> slot
     if (MODE == kModeMerged)
         for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) chx[q] = gchx[q];
     for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
     for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
+    for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
     __syncthreads();
     // Workgroups are dispatched round-robin over the 8 XCDs; remap so that
     // each XCD sweeps one contiguous range of rows and its L2 serves the
@@ -95,7 +97,10 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     uint32_t s = (uint32_t)(rw & kStateMask);
     double nrm = ntab[(int)(rw >> kLidShift)];
     double ar = 0.0, ai = 0.0;
-    auto loadu = [&](uint32_t j) { return load_c32(u, j); };   // u[D] holds zero: empty slots read it
+    // u is the gather image: the rank of a target and the LDS chunk map give
+    // its element, with no directory load
+    auto jof = [&](uint32_t r, bool ok) { return image_index(map, cb, zidx, r, ok); };
+    auto loadu = [&](uint32_t j) { return load_c32(u, j); };
     // explicit FMAs, so that the accumulation rounds identically in every
     // variant whatever the compiler contracts elsewhere
     auto acc = [&](const cd2& x, double cr, double ci) {
@@ -106,10 +111,10 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
             ai = fma(ci, x.re, ai);
         }
     };
-    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, dir, phase,
-                                                              (uint32_t)D, loadu, acc)
-                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, dir, phase,
-                                                            (uint32_t)D, loadu, acc);
+    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, phase, jof,
+                                                              loadu, acc)
+                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, phase, jof, loadu,
+                                                            acc);
     cd2 vi = load_c(v, i);
     VT o;
     o.re = fma(nrm, ar, ez * vi.re);
@@ -121,9 +126,11 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     int64_t D, const uint64_t *__restrict__ rows, int nfam, const TFam *__restrict__ gfam,               \
         const TEntry *__restrict__ gent, const TChir *__restrict__ gchx, double ezz0, RankInfo R,        \
         const uint32_t *__restrict__ grtab,                                                               \
-        int rtab_n, const DirEntry *__restrict__ dir, const cd2 *__restrict__ gphase, int nphase,         \
-        const double *__restrict__ ntab, const VT *__restrict__ u, const VT *__restrict__ v, VT *__restrict__ w
-#define ED_TORUS_C_PASS D, rows, nfam, gfam, gent, gchx, ezz0, R, grtab, rtab_n, dir, gphase, nphase, ntab, u, v, w
+        int rtab_n, const uint16_t *__restrict__ gmap, int nmap, int cb, uint32_t zidx,                   \
+        const cd2 *__restrict__ gphase, int nphase, const double *__restrict__ ntab, const VT *__restrict__ u, \
+        const VT *__restrict__ v, VT *__restrict__ w
+#define ED_TORUS_C_PASS \
+    D, rows, nfam, gfam, gent, gchx, ezz0, R, grtab, rtab_n, gmap, nmap, cb, zidx, gphase, nphase, ntab, u, v, w
 
 // Variant A: four elements in flight, occupancy left to the register count.
 template <class TC, bool TRIV, int MODE, typename VT>
@@ -155,12 +162,13 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
     const int64_t D = B->D;
     dim3 grid((unsigned)((D + bs - 1) / bs)), blk(bs);
     int nph = B->L1 * B->L2;
+    const uint32_t zidx = B->img_nslot << B->img_cb;   // first element of the zero slot
     // the merged families (bond pairs carrying their chiral entries) when built
     const bool mg = op->tmode == kModeMerged;
 #define ED_GO(KER)                                                                                            \
     hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, mg ? op->nmfam : op->nfam, mg ? op->d_mfam : op->d_tfam, \
                        mg ? op->d_ment : op->d_tent, op->d_tchx, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,    \
-                       B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
+                       B->d_img_map, (int)B->img_nmap, B->img_cb, zidx, B->d_phase, nph, B->d_ntab, u, v, w)
     // term class: k = 0 takes the general or the real-bond kernel, k != 0 the
     // general or the bond-only one
 #define ED_GO_MODES(KERNEL)                                                         \
@@ -194,11 +202,16 @@ bool torus_shape_compiled(const Torus& T) {
     return false;
 }
 
-// The compiled-shape kernels address the gather stream with 32-bit byte
-// offsets: (D + 1) complex128 elements must span less than 4 GiB (every
-// N <= 32 momentum sector does).
+// The compiled-shape kernels gather from the sector's image (built with the
+// basis when it fits the LDS map and the 32-bit byte offsets of the loads).
 static bool compiled_fits(const ed_op_s* op) {
-    return torus_shape_compiled(op->torus) && (uint64_t)(op->b->D + 1) * sizeof(cd2) <= 0xffffffffull;
+    return torus_shape_compiled(op->torus) && op->b->img_cb >= 0;
+}
+
+// Whether the next H*v of this operator runs a compiled-shape kernel, whose
+// gathered vector is the image rather than u by row.
+bool torus_uses_image(const ed_op_s* op) {
+    return op->use_torus && op->tune_path != 1 && op->tune_path != 3 && op->tune_path != 6 && compiled_fits(op);
 }
 
 size_t torus_smem_bytes(const ed_op_s* op) {
@@ -215,7 +228,7 @@ static size_t torus_c_smem_bytes(const ed_op_s* op) {
     const int nf = mg ? op->nmfam : op->nfam;
     return (mg ? (size_t)nf * 32 * sizeof(TChir) : 0) + (size_t)nf * 32 * sizeof(TEntry) +
            (size_t)(nf + (nf & 1)) * sizeof(TFam) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
-           (size_t)B->rtab32_n * sizeof(uint32_t);
+           (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)B->img_nmap * sizeof(uint16_t);
 }
 
 // Launches the compiled-shape kernel for the operator's lattice; false when

@@ -57,6 +57,12 @@ struct ed_basis_s {
     edc::cd2* d_phase = nullptr;     // [L1*L2]
     double* d_ntab = nullptr;        // norm by lid
     double* d_nitab = nullptr;       // 1/norm by lid
+    // gather image of u for the compiled rotation kernels (ImageLayout in
+    // ed_tables.h); img_cb = -1 when the sector has none
+    int img_cb = -1;
+    uint32_t img_nslot = 0, img_nmap = 0;
+    uint16_t* d_img_map = nullptr;   // [img_nmap] chunk -> slot
+    uint32_t* d_img_pos = nullptr;   // [D] image element of row i
 };
 
 struct ed_op_s {
@@ -95,6 +101,8 @@ struct ed_op_s {
     int sel_block[2] = {0, 0};
     void* d_u64 = nullptr;    // prescaled gather stream, complex64 [D]
     void* d_u128 = nullptr;   // prescaled gather stream, complex128 [D]
+    void* d_img64 = nullptr;  // prescaled gather image (compiled rotation kernels), complex64
+    void* d_img128 = nullptr; // complex128
     std::vector<edc::Bond> bonds;
 };
 
@@ -110,6 +118,8 @@ template <typename VT>
 int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs);
 bool torus_shape_compiled(const edc::Torus& T);
+bool torus_uses_image(const ed_op_s* op);
+int basis_image_build(ed_basis_s* B);
 int block_op_hv(ed_block_s* K, int dtype, const void* v, void* w, hipStream_t st);
 int rows2_detect(const std::vector<int>& T1, const std::vector<int>& T2);
 int rows2_tables(ed_basis_s* B, const std::vector<uint64_t>& rt);

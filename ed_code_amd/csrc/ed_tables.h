@@ -140,4 +140,52 @@ inline std::vector<uint64_t> rank_tables(int N, int n, RankInfo& R) {
     return rt;
 }
 
+// Gather image of the prescaled vector u for the compiled rotation kernels.
+// u is laid out by rank instead of by row: the rank space [0, C) is cut into
+// chunks of 2^cb ranks, each chunk that holds a representative of the
+// sector gets a slot of 2^cb elements (slots in rank order), and one more
+// slot, index nslot, holds zeros.  The target of rank r is element
+//     (map[r >> cb] << cb) | (r & (2^cb - 1)),
+// map sending every chunk without representatives to the zero slot; inside a
+// slot the positions of ranks that are no representative of the sector
+// (orbits outside the sector at k != 0, non-minimal words) are never written
+// and stay zero.  The map takes 2 bytes per chunk and fits in LDS, so a
+// gather needs no directory load: the rank alone addresses u.
+struct ImageLayout {
+    int cb = -1;                 // chunk bits; -1: no image
+    uint32_t nslot = 0;          // slots holding representatives; the zero slot
+    std::vector<uint16_t> map;   // chunk -> slot, (C - 1) >> cb + 1 entries
+    uint64_t elems() const { return cb < 0 ? 0 : (uint64_t)(nslot + 1) << cb; }
+};
+
+constexpr uint32_t kImageMapMax = 3072;   // map entries (6 KiB of LDS)
+
+// Smallest chunk size whose map fits kImageMapMax entries (at least one
+// 64-rank directory group per chunk).
+inline int image_min_bits(uint64_t C) {
+    int cb = 6;
+    while (((C > 0 ? C - 1 : 0) >> cb) + 1 > kImageMapMax) cb++;
+    return cb;
+}
+
+// Builds the layout for chunk bits cb from touched[c] != 0 (chunk c holds a
+// representative).  False when the image would exceed 6 (D + 1) elements (and
+// 2^22) or 4 GiB of complex128, the reach of the kernels' 32-bit offsets.
+inline bool image_layout(uint64_t C, int64_t D, int cb, const std::vector<uint8_t>& touched, ImageLayout& L) {
+    const uint64_t nmap = ((C > 0 ? C - 1 : 0) >> cb) + 1;
+    if (touched.size() < nmap) return false;
+    uint32_t n = 0;
+    for (uint64_t c = 0; c < nmap; c++) n += touched[c] ? 1u : 0u;
+    const uint64_t elems = (uint64_t)(n + 1) << cb;
+    if (n >= 0xffffu || elems * 16 > 0xffffffffull || elems > std::max<uint64_t>(6 * (uint64_t)(D + 1), 1ull << 22))
+        return false;
+    L.cb = cb;
+    L.nslot = n;
+    L.map.assign(nmap, (uint16_t)n);
+    uint32_t s = 0;
+    for (uint64_t c = 0; c < nmap; c++)
+        if (touched[c]) L.map[c] = (uint16_t)s++;
+    return true;
+}
+
 }  // namespace edc

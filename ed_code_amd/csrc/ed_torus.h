@@ -376,18 +376,6 @@ ED_HD int64_t dir_decode_off(const DirEntry& e, unsigned o) {
 
 ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) { return dir_decode_off(e, r % kDirSpan); }
 
-// Directory entry of rank block idx.  On the device the byte offset idx*12
-// is formed with a 24-bit multiply (full rate) instead of a 64-bit
-// multiply-add; idx < 2^24 holds for every N <= 32 sector (C(32,16)/64 <
-// 1e7).
-ED_HD const DirEntry& dir_at(const DirEntry* dir, uint32_t idx) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const DirEntry*)((const char*)dir + ((uint64_t)(uint32_t)__umul24(idx, 3u) << 2));
-#else
-    return dir[idx];
-#endif
-}
-
 // Term classes of the batched row (a compile-time switch, so the bond-only
 // rows carry no branch for chiral entries):
 //   kModeGeneral  bond and chiral families, complex coefficients;
@@ -462,18 +450,19 @@ ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, 
 }
 
 // torus_row_c with K elements in flight per lane: the orbit searches and
-// ranks of K elements are computed first, then their K directory loads and
-// K gathers are issued back to back, so that a lane waits for memory once
-// per K elements instead of twice per element.  The loads are unconditional:
-// a load under a lane mask would make the compiler wait for it inside the
-// masked block and serialise the K loads.  An empty slot (no element left,
-// or a representative outside the sector) reads directory entry 0 and
-// element zidx of u, which holds zero, so it adds nothing whatever its
-// coefficient.  loadu(j) returns element j of u; g(x, cr, ci) accumulates.
-template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
+// ranks of K elements are computed first, then their K gathers are issued
+// back to back, so that a lane waits for memory once per K elements.  The
+// loads are unconditional: a load under a lane mask would make the compiler
+// wait for it inside the masked block and serialise the K loads.
+// jof(r, ok) gives the element of u that holds the target of rank r; for an
+// empty slot (ok false: no element left) it gives an element that holds
+// zero, and so does every rank outside the sector, so such slots add nothing
+// whatever their coefficient.  loadu(j) returns element j of u; g(x, cr, ci)
+// accumulates.
+template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
 ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
-                               double ezz0, const RankInfo& R, const uint32_t* rtab, const DirEntry* dir,
-                               const cd2* phase, uint32_t zidx, LoadU&& loadu, Gf&& g) {
+                               double ezz0, const RankInfo& R, const uint32_t* rtab, const cd2* phase, JOf&& jof,
+                               LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
@@ -482,18 +471,9 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
         RowBatch<K> B;
         fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
         more = B.ok[K - 1];
-        DirEntry e[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) e[q] = dir_at(dir, B.rk[q] / kDirSpan);
-        uint32_t j[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            int64_t jj = dir_decode_off(e[q], B.rk[q] % kDirSpan);
-            j[q] = (B.ok[q] && jj >= 0) ? (uint32_t)jj : zidx;
-        }
         cd2 xv[K];
 #pragma unroll
-        for (int q = 0; q < K; q++) xv[q] = loadu(j[q]);
+        for (int q = 0; q < K; q++) xv[q] = loadu(jof(B.rk[q], B.ok[q]));
 #pragma unroll
         for (int q = 0; q < K; q++) g(xv[q], B.cre[q], B.cim[q]);
     }
@@ -502,40 +482,29 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
 
 // torus_row_batched with the batches software-pipelined: while the gathers
 // of batch b are in flight the lane computes the orbit searches and ranks of
-// batch b+1 and issues its directory loads, then accumulates batch b and
-// issues the gathers of b+1.  Each wait on memory thus overlaps a batch of
-// integer work of the same lane.  Elements are accumulated in the same order
-// as torus_row_batched, so the result is bit-identical.
-template <class TC, bool TRIV, int MODE, int K, class LoadU, class Gf>
+// batch b+1, then accumulates batch b and issues the gathers of b+1.  Each
+// wait on memory thus overlaps a batch of integer work of the same lane.
+// Elements are accumulated in the same order as torus_row_batched, so the
+// result is bit-identical.
+template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
 ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
-                                 double ezz0, const RankInfo& R, const uint32_t* rtab, const DirEntry* dir,
-                                 const cd2* phase, uint32_t zidx, LoadU&& loadu, Gf&& g) {
+                                 double ezz0, const RankInfo& R, const uint32_t* rtab, const cd2* phase, JOf&& jof,
+                                 LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
     RowBatch<K> A, B;
     fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, A);
-    DirEntry e[K];
     cd2 xa[K];
 #pragma unroll
-    for (int q = 0; q < K; q++) e[q] = dir_at(dir, A.rk[q] / kDirSpan);
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        int64_t jj = dir_decode_off(e[q], A.rk[q] % kDirSpan);
-        xa[q] = loadu((A.ok[q] && jj >= 0) ? (uint32_t)jj : zidx);
-    }
+    for (int q = 0; q < K; q++) xa[q] = loadu(jof(A.rk[q], A.ok[q]));
     bool more = A.ok[K - 1];
     while (more) {
         fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
 #pragma unroll
-        for (int q = 0; q < K; q++) e[q] = dir_at(dir, B.rk[q] / kDirSpan);
-#pragma unroll
         for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
 #pragma unroll
-        for (int q = 0; q < K; q++) {
-            int64_t jj = dir_decode_off(e[q], B.rk[q] % kDirSpan);
-            xa[q] = loadu((B.ok[q] && jj >= 0) ? (uint32_t)jj : zidx);
-        }
+        for (int q = 0; q < K; q++) xa[q] = loadu(jof(B.rk[q], B.ok[q]));
 #pragma unroll
         for (int q = 0; q < K; q++) {
             A.cre[q] = B.cre[q];
@@ -546,6 +515,12 @@ ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TE
 #pragma unroll
     for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
     return ezz;
+}
+
+// Element of the gather image (ImageLayout) holding the target of rank r;
+// zidx (an element of the zero slot) for an empty slot.
+ED_HD uint32_t image_index(const uint16_t* map, int cb, uint32_t zidx, uint32_t r, bool ok) {
+    return ok ? (((uint32_t)map[r >> cb] << cb) | (r & ((1u << cb) - 1u))) : zidx;
 }
 
 }  // namespace edc

@@ -111,14 +111,15 @@ void hv(HC* h, const cd2* v, cd2* w) {
 template <class TC, bool TRIV, int MODE, bool PIPE>
 void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
                 const std::vector<TChir>& chx, double ezz0, const std::vector<uint32_t>& rt32,
-                const std::vector<cd2>& u, const cd2* v, cd2* w) {
-    const uint32_t zidx = (uint32_t)h->rows.size();   // u[zidx] == 0
+                const ImageLayout& L, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+    const uint32_t zidx = L.nslot << L.cb;   // the zero slot
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
         uint32_t s = (uint32_t)(h->rows[i] & kStateMask);
         double nrm = h->nt[h->rows[i] >> kLidShift];
         double ar = 0, ai = 0;
-        auto loadu = [&](uint32_t j) { return u[j]; };
+        auto loadu = [&](uint32_t j) { return u.at(j); };
+        auto jof = [&](uint32_t r, bool ok) { return image_index(L.map.data(), L.cb, zidx, r, ok); };
         auto acc = [&](const cd2& x, double cr, double ci) {
             ar = std::fma(cr, x.re, ar);
             ai = std::fma(cr, x.im, ai);
@@ -129,11 +130,9 @@ void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::
         };
         const TChir* cx = chx.empty() ? nullptr : chx.data();
         double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
-                                                                  h->R, rt32.data(), h->dir.data(), h->ph.data(),
-                                                                  zidx, loadu, acc)
+                                                                  h->R, rt32.data(), h->ph.data(), jof, loadu, acc)
                          : torus_row_batched<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
-                                                                h->R, rt32.data(), h->dir.data(), h->ph.data(), zidx,
-                                                                loadu, acc);
+                                                                h->R, rt32.data(), h->ph.data(), jof, loadu, acc);
         w[i] = cd2{std::fma(nrm, ar, ez * v[i].re), std::fma(nrm, ai, ez * v[i].im)};
     }
 }
@@ -141,8 +140,9 @@ void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::
 template <class TC, bool PIPE>
 void hv_modes(HC* h, const Torus& T, bool triv, int mode, const std::vector<TFam>& fam,
               const std::vector<TEntry>& ent, const std::vector<TChir>& chx, double ezz0,
-              const std::vector<uint32_t>& rt32, const std::vector<cd2>& u, const cd2* v, cd2* w) {
-#define HC_RUN(TRIV, MODE) hv_batched<TC, TRIV, MODE, PIPE>(h, T, fam, ent, chx, ezz0, rt32, u, v, w)
+              const std::vector<uint32_t>& rt32, const ImageLayout& L, const std::vector<cd2>& u, const cd2* v,
+              cd2* w) {
+#define HC_RUN(TRIV, MODE) hv_batched<TC, TRIV, MODE, PIPE>(h, T, fam, ent, chx, ezz0, rt32, L, u, v, w)
     if (mode == kModeMerged) {
         if (triv) HC_RUN(true, kModeMerged);
         else HC_RUN(false, kModeMerged);
@@ -433,10 +433,22 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
     double ezz0;
     if (!torus_families(h->lam, h->Ch, T, h->b, h->c, fam, ent, ezz0)) return 1;
     std::vector<uint32_t> rt32(h->rt.begin(), h->rt.end());
-    std::vector<cd2> u(h->rows.size() + 1, cd2{0.0, 0.0});
+    // the gather image as the library lays it out (basis_image_build):
+    // chunks marked from the directory, u written at the rows' positions
+    ImageLayout L;
+    const int cb0 = image_min_bits(h->C);
+    for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
+        std::vector<uint8_t> t(((h->C - 1) >> cb) + 1, 0);
+        for (size_t g = 0; g < h->dir.size(); g++)
+            if (h->dir[g].b0 | h->dir[g].b1) t[g >> (cb - 6)] = 1;
+        image_layout(h->C, (int64_t)h->rows.size(), cb, t, L);
+    }
+    if (L.cb < 0) return 1;
+    std::vector<cd2> u(L.elems(), cd2{0.0, 0.0});
     for (size_t i = 0; i < h->rows.size(); i++) {
         double f = h->nit[h->rows[i] >> kLidShift];
-        u[i] = cd2{v[i].re * f, v[i].im * f};
+        const uint32_t r = (uint32_t)rank_of<uint32_t>((uint32_t)(h->rows[i] & kStateMask), h->R, h->rt.data());
+        u[image_index(L.map.data(), L.cb, 0, r, true)] = cd2{v[i].re * f, v[i].im * f};
     }
     bool triv = h->kx == 0 && h->ky == 0;
     // term class as the library picks it (op_create)
@@ -461,8 +473,8 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
 #define HC_SHAPE(n, wd, l2)                                                                                  \
     if (T.N == n && T.w == wd && L2 == l2) {                                                                 \
         using TC = TorusC<n, wd, l2>;                                                                        \
-        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, u, v, w);                  \
-        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, u, v, w);                      \
+        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, L, u, v, w);               \
+        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, L, u, v, w);                   \
         return 0;                                                                                            \
     }
     HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1) HC_SHAPE(24, 24, 1)
