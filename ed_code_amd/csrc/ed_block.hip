@@ -42,15 +42,29 @@ struct BChir {           // per (term, instance), merged terms: the two chiral e
 
 inline unsigned nblk(int64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// The M instances' u = v / norm written into the interleaved gather image:
+// image element pos[i] of row i holds its M values.
 template <typename VT>
-__global__ void k_prescale_block(int64_t D, int M, const uint64_t* rows, const double* nitab, const VT* v, VT* u) {
+__global__ void k_prescale_block(int64_t D, int M, const uint64_t* rows, const double* nitab, const uint32_t* pos,
+                                 const VT* v, VT* img) {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= D * M) return;
-    double f = nitab[(int)(rows[e / M] >> kLidShift)];
+    const int64_t i = e / M;
+    double f = nitab[(int)(rows[i] >> kLidShift)];
     VT x = v[e];
     x.re = x.re * f;
     x.im = x.im * f;
-    u[e] = x;
+    img[(int64_t)pos[i] * M + (e - i * M)] = x;
+}
+
+template <typename VT>
+__global__ void k_zero_block(uint64_t n, VT* p) {
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (uint64_t)gridDim.x * blockDim.x) {
+        VT z;
+        z.re = 0;
+        z.im = 0;
+        p[e] = z;
+    }
 }
 
 template <typename VT>
@@ -79,7 +93,8 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
                                                     const BCoef* __restrict__ gcoef, const BChir* __restrict__ gchir,
                                                     const double* __restrict__ gezz0,
                                                     RankInfo R, const uint32_t* __restrict__ grtab, int rtab_n,
-                                                    const DirEntry* __restrict__ dir, const cd2* __restrict__ gphase,
+                                                    const uint16_t* __restrict__ gmap, int nmap, int cb,
+                                                    uint32_t zidx, const cd2* __restrict__ gphase,
                                                     int nphase, const double* __restrict__ ntab,
                                                     const VT* __restrict__ u, const VT* __restrict__ v,
                                                     VT* __restrict__ w) {
@@ -92,6 +107,7 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
     BEnt* ent = (BEnt*)(foff + nfam + (nfam & 1));             // nterm
     double* ezz0 = (double*)(ent + nterm);                     // M
     uint32_t* rtab = (uint32_t*)(ezz0 + M);                    // rtab_n
+    uint16_t* map = (uint16_t*)(rtab + rtab_n);                // nmap: gather-image chunk -> slot
     if (MG)
         for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) chir[q] = gchir[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
@@ -100,6 +116,7 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
     for (int q = threadIdx.x; q < nterm; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < M; q += blockDim.x) ezz0[q] = gezz0[q];
     for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
+    for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
     __syncthreads();
     // XCD-contiguous row ranges (see the single-instance kernel)
     const unsigned nb = gridDim.x, qq = nb / 8, rem = nb % 8;
@@ -140,21 +157,12 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
             }
         }
         more = ok[K - 1];
-        DirEntry e[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) e[q] = dir[rk[q] / kDirSpan];
-        bool use[K];
-        int64_t j[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            j[q] = dir_decode(e[q], rk[q]);
-            use[q] = ok[q] && j[q] >= 0;
-        }
-        // unconditional loads (an empty slot reads the row's own values)
+        // unconditional loads from the gather image (an empty slot reads the
+        // zero slot; so does a target outside the sector)
         VT xv[K][M];
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            const VT* p = u + (use[q] ? j[q] : i) * M;
+            const VT* p = u + (int64_t)image_index(map, cb, zidx, rk[q], ok[q]) * M;
 #pragma unroll
             for (int m = 0; m < M; m++) xv[q][m] = p[m];
         }
@@ -197,7 +205,7 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
                     cr = nr;
                     ci = ni;
                 }
-                if (!use[q]) { cr = 0.0; ci = 0.0; }
+                if (!ok[q]) { cr = 0.0; ci = 0.0; }
                 const cd2 xx = load_c(&xv[q][m], 0);
                 ar[m] += cr * xx.re - ci * xx.im;
                 ai[m] += cr * xx.im + ci * xx.re;
@@ -226,8 +234,8 @@ struct ed_block_s {
     BChir* d_chir = nullptr;        // merged terms (chiral entries on the bond pairs)
     bool mg = false;
     double* d_ezz0 = nullptr;
-    void* d_u64 = nullptr;          // prescaled interleaved gather stream, complex64 [D*M]
-    void* d_u128 = nullptr;         // complex128 [D*M]
+    void* d_u64 = nullptr;          // prescaled interleaved gather image, complex64 [image elements * M]
+    void* d_u128 = nullptr;         // complex128
     void* d_tmp = nullptr;          // fallback: two de-interleaved vectors, complex128-sized
 };
 
@@ -262,7 +270,7 @@ size_t block_smem(const ed_block_s* K) {
     return (K->mg ? (size_t)K->nterm * K->M * sizeof(BChir) : 0) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
            (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
            (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)K->nterm * sizeof(BEnt) + (size_t)K->M * sizeof(double) +
-           (size_t)B->rtab32_n * sizeof(uint32_t);
+           (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)B->img_nmap * sizeof(uint16_t);
 }
 
 template <class TC, int M, typename VT>
@@ -273,10 +281,11 @@ void launch_block_c(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t 
     dim3 grid(nblk(B->D, bs)), blk(bs);
     size_t sm = block_smem(K);
     int nph = B->L1 * B->L2;
+    const uint32_t zidx = B->img_nslot << B->img_cb;   // first element of the zero slot
 #define ED_BGO(TRIV, MG)                                                                                         \
     hipLaunchKernelGGL((k_hv_block_c<TC, TRIV, MG, M, KB, VT>), grid, blk, sm, st, B->D, B->d_rows, K->nfam, K->d_fam, \
                        K->d_foff, K->nterm, K->d_ent, K->d_coef, K->d_chir, K->d_ezz0, B->rinfo, B->d_rtab32,          \
-                       B->rtab32_n, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w)
+                       B->rtab32_n, B->d_img_map, (int)B->img_nmap, B->img_cb, zidx, B->d_phase, nph, B->d_ntab, u, v, w)
     if (B->trivial_k) {
         if (K->mg) ED_BGO(true, true);
         else ED_BGO(true, false);
@@ -323,9 +332,15 @@ int block_hv(ed_block_s* K, const VT* v, VT* w, hipStream_t st) {
         return ED_OK;
     }
     VT*& u = std::is_same<VT, cd2>::value ? (VT*&)K->d_u128 : (VT*&)K->d_u64;
-    if (!u) HIPCHK(hipMalloc((void**)&u, D * M * sizeof(VT)));
+    if (!u) {
+        // the zero positions of the image are written once, at allocation
+        const uint64_t ne = ((uint64_t)(B->img_nslot + 1) << B->img_cb) * M;
+        HIPCHK(hipMalloc((void**)&u, ne * sizeof(VT)));
+        hipLaunchKernelGGL(k_zero_block<VT>, dim3((unsigned)std::min<uint64_t>((ne + 255) / 256, 1u << 20)), dim3(256),
+                           0, st, ne, u);
+    }
     hipLaunchKernelGGL(k_prescale_block<VT>, dim3(nblk(D * M, 256)), dim3(256), 0, st, D, (int)M, B->d_rows,
-                       B->d_nitab, v, u);
+                       B->d_nitab, B->d_img_pos, v, u);
     HIPCHK(hipGetLastError());
     bool done = false;
     switch (M) {
@@ -392,7 +407,7 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
     for (int m = 0; m < M; m++)
         if (terms[m].nbowtie && std::fabs(terms[m].ChCh) > 1e-16) nobow = false;
     if (b->has_torus && b->N <= 32 && b->d_rtab32 && nobow && (M == 2 || M == 4 || M == 8) &&
-        torus_shape_compiled(b->torus)) {
+        torus_shape_compiled(b->torus) && b->img_cb >= 0) {
         std::vector<std::vector<TFam>> fams(M);
         std::vector<std::vector<TEntry>> ents(M);
         std::vector<std::vector<TChir>> chxs(M);

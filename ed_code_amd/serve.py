@@ -132,10 +132,12 @@ def estimate_rows(lattice: str, n_up: int) -> int:
 
 def footprint(rows: int, M: int, iterations: int, nev: int) -> int:
     """Device bytes of one block solve in complex64 with a complex128 polish:
-    Krylov vectors, work vectors, FP64 Ritz vectors, basis and directory."""
+    Krylov vectors, work vectors, the gather images of both precisions (at
+    most 6 elements per row), FP64 Ritz vectors, basis, row positions and
+    directory."""
     krylov = (iterations + 1) * rows * M * 8 + (min(iterations, 60) + 1) * rows * M * 16
-    work = rows * M * (8 + 16 * 3) + nev * rows * M * 16
-    basis = rows * 8 * 2
+    work = rows * M * (6 * (8 + 16) + 2 * 16) + nev * rows * M * 16
+    basis = rows * (8 * 2 + 4)
     return krylov + work + basis
 
 
