@@ -246,6 +246,23 @@ __global__ void k_prescale_img(int64_t D, const uint64_t* rows, const double* ni
     img[pos[i]] = x;
 }
 
+// The same into the wide image of the two-row kernels: the rank of each
+// row's representative through the global chunk map (64-bit positions).
+template <typename VT>
+__global__ void k_prescale_wimg(uint64_t D, const uint64_t* rows, const double* nitab, RankInfo R,
+                                const uint64_t* rtab, const uint32_t* map, int cb, const VT* v, VT* img) {
+    ED_GRID_STRIDE(i, D) {
+        const uint64_t rw = rows[i];
+        const uint64_t r = rank_of<uint64_t>(rw & kStateMask, R, rtab);
+        const uint64_t e = ((uint64_t)map[r >> cb] << cb) | (r & ((uint64_t(1) << cb) - 1));
+        double f = nitab[(int)(rw >> kLidShift)];
+        VT x = v[i];
+        x.re = x.re * f;
+        x.im = x.im * f;
+        img[e] = x;
+    }
+}
+
 // Zero fill with a kernel (no memset node if the caller captures the stream).
 template <typename VT>
 __global__ void k_zero(uint64_t n, VT* p) {
@@ -451,12 +468,13 @@ int basis_build_device(ed_basis_s* B) {
 // map fits in LDS and whose image stays within the size bounds.  Sectors of
 // shapes without compiled kernels, or whose image would be too large, get
 // none and run the directory kernels.
-int basis_image_build(ed_basis_s* B) {
-    B->img_cb = -1;
-    if (B->N > 32 || B->D == 0 || !B->has_torus || !torus_shape_compiled(B->torus)) return ED_OK;
+// Chooses the chunk size of a gather image: the smallest whose map fits
+// max_entries and whose image stays within max_elems (up to three doublings
+// tried), chunks marked from the directory on the device.
+template <typename M>
+static int image_choose(ed_basis_s* B, uint64_t max_entries, uint64_t max_elems, ImageLayoutT<M>& L) {
     const uint64_t C = B->C;
-    const int cb0 = image_min_bits(C);
-    ImageLayout L;
+    const int cb0 = image_min_bits(C, max_entries);
     uint8_t* d_t = nullptr;
     HIPCHK(hipMalloc(&d_t, ((C - 1) >> cb0) + 1));
     for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
@@ -467,9 +485,32 @@ int basis_image_build(ed_basis_s* B) {
                            cb - 6, d_t);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(t.data(), d_t, nmap, hipMemcpyDeviceToHost));
-        image_layout(C, B->D, cb, t, L);
+        image_layout(C, cb, t, max_elems, L);
     }
     dfree(d_t);
+    return ED_OK;
+}
+
+int basis_image_build(ed_basis_s* B) {
+    B->img_cb = -1;
+    B->wimg_cb = -1;
+    if (B->D == 0) return ED_OK;
+    if (B->rows2_w && B->N <= 40) {
+        WideImageLayout L;
+        int rc = image_choose(B, kWideMapMax, wide_image_max_elems(B->D), L);
+        if (rc != ED_OK) return rc;
+        if (L.cb >= 0) {
+            HIPCHK(hipMalloc(&B->d_wimg_map, L.map.size() * sizeof(uint32_t)));
+            HIPCHK(hipMemcpy(B->d_wimg_map, L.map.data(), L.map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            B->wimg_cb = L.cb;
+            B->wimg_nslot = L.nslot;
+            B->wimg_nmap = L.map.size();
+        }
+    }
+    if (B->N > 32 || !B->has_torus || !torus_shape_compiled(B->torus)) return ED_OK;
+    ImageLayout L;
+    int rc = image_choose(B, kImageMapMax, image_max_elems(B->D), L);
+    if (rc != ED_OK) return rc;
     if (L.cb < 0) return ED_OK;
     HIPCHK(hipMalloc(&B->d_img_map, L.map.size() * sizeof(uint16_t)));
     HIPCHK(hipMemcpy(B->d_img_map, L.map.data(), L.map.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -479,7 +520,7 @@ int basis_image_build(ed_basis_s* B) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     B->img_cb = L.cb;
-    B->img_nslot = L.nslot;
+    B->img_nslot = (uint32_t)L.nslot;
     B->img_nmap = (uint32_t)L.map.size();
     return ED_OK;
 }
@@ -488,7 +529,7 @@ void delete_basis(ed_basis_s* B) {
     if (!B) return;
     for (void* p : {(void*)B->d_rows, (void*)B->d_dir, (void*)B->d_rtab, (void*)B->d_binom, (void*)B->d_phase,
                     (void*)B->d_ntab, (void*)B->d_nitab, (void*)B->d_rtab32, (void*)B->d_rlo, (void*)B->d_rhi,
-                    (void*)B->d_img_map, (void*)B->d_img_pos})
+                    (void*)B->d_img_map, (void*)B->d_img_pos, (void*)B->d_wimg_map})
         if (p) dfree(p);
     delete B;
 }
@@ -725,7 +766,8 @@ extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
                     (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
-                    (void*)op->d_ment, (void*)op->d_tchx, op->d_img64, op->d_img128})
+                    (void*)op->d_ment, (void*)op->d_tchx, op->d_img64, op->d_img128, op->d_wimg64,
+                    op->d_wimg128})
         if (p) dfree(p);
     delete op;
 }
@@ -755,7 +797,29 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
         hipLaunchKernelGGL(k_prescale_img<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab,
                            B->d_img_pos, v, im);
         u = im;
-    } else {
+    }
+    bool wimg = false;
+    if (rows2 && B->wimg_cb >= 0) {
+        // the two-row kernels gather from the wide image when it fits the
+        // free device memory, else from u by row through the directory
+        VT*& im = std::is_same<VT, cd2>::value ? (VT*&)op->d_wimg128 : (VT*&)op->d_wimg64;
+        const uint64_t ne = (B->wimg_nslot + 1) << B->wimg_cb;
+        if (!im) {
+            if (hipMalloc((void**)&im, ne * sizeof(VT)) != hipSuccess) {
+                im = nullptr;
+                (void)hipGetLastError();
+            } else {
+                hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, im);
+            }
+        }
+        if (im) {
+            hipLaunchKernelGGL(k_prescale_wimg<VT>, dim3(sblocks((uint64_t)D, 256)), dim3(256), 0, st, (uint64_t)D,
+                               B->d_rows, B->d_nitab, B->rinfo, B->d_rtab, B->d_wimg_map, B->wimg_cb, v, im);
+            u = im;
+            wimg = true;
+        }
+    }
+    if (!img && !wimg) {
         VT*& uu = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
         if (!uu) HIPCHK(hipMalloc((void**)&uu, D * sizeof(VT)));
         hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, uu);
@@ -773,7 +837,7 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     }
     if (ev0) HIPCHK(hipEventRecord(ev0, st));
     if (rows2) {
-        int rc = launch_hv_rows2<VT>(op, u, v, w, st);
+        int rc = launch_hv_rows2<VT>(op, u, v, w, st, wimg);
         if (rc != ED_OK) return rc;
     } else if (op->use_torus && op->tune_path != 1) {
         int rc = launch_hv_torus<VT>(op, u, v, w, st);

@@ -78,12 +78,13 @@ namespace {
 
 constexpr int kRows2K = 4;   // elements in flight per lane
 
-template <class TC, bool TRIV, typename VT>
+template <class TC, bool TRIV, bool WIMG, typename VT>
 __global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                   const TFam64* __restrict__ gfam, const int* __restrict__ gfoff,
                                                   int nterm, const TEntry* __restrict__ gent, double ezz0, RankInfo R,
                                                   const uint32_t* __restrict__ grlo, int rlo_n,
                                                   const uint64_t* __restrict__ grhi, const DirEntry* __restrict__ dir,
+                                                  const uint32_t* __restrict__ wmap, int cb,
                                                   const cd2* __restrict__ gphase, int nphase,
                                                   const double* __restrict__ ntab, const VT* __restrict__ u,
                                                   const VT* __restrict__ v, VT* __restrict__ w) {
@@ -110,8 +111,14 @@ __global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __r
     const uint64_t s = rw & kStateMask;
     const double nrm = ntab[(int)(rw >> kLidShift)];
     double ar = 0.0, ai = 0.0;
+    // WIMG: u is the wide gather image, addressed by rank through the chunk
+    // map (cache resident); else u by row through the directory
+    auto jof = [&](uint64_t r) -> int64_t {
+        if (WIMG) return (int64_t)(((uint64_t)wmap[r >> cb] << cb) | (r & ((uint64_t(1) << cb) - 1)));
+        return dir_decode_off(dir[r / kDirSpan], (uint32_t)(r % kDirSpan));
+    };
     const double ez = rows2_row<TC, TRIV, kRows2K>(
-        s, nfam, fam, foff, ent, ezz0, R, rlo, rhi, dir, phase, i, [&](int64_t j) { return load_c(u, j); },
+        s, nfam, fam, foff, ent, ezz0, R, rlo, rhi, phase, i, jof, [&](int64_t j) { return load_c(u, j); },
         [&](const cd2& x, double cr, double ci) {
             ar += cr * x.re - ci * x.im;
             ai += cr * x.im + ci * x.re;
@@ -131,31 +138,36 @@ size_t rows2_smem(const ed_op_s* op) {
 }
 
 template <class TC, typename VT>
-void launch_r2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
+void launch_r2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wimg) {
     ed_basis_s* B = op->b;
     const unsigned bs = 256;
     dim3 grid((unsigned)((B->D + bs - 1) / bs)), blk(bs);
     const size_t sm = rows2_smem(op);
     const int nph = B->L1 * B->L2;
-    if (B->trivial_k)
-        hipLaunchKernelGGL((k_hv_rows2<TC, true, VT>), grid, blk, sm, st, B->D, B->d_rows, op->nfam64,
-                           (const TFam64*)op->d_fam64, op->d_foff64, op->nterm64, op->d_ent64, op->ezz0_64, B->rinfo,
-                           B->d_rlo, B->rlo_n, B->d_rhi, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
-    else
-        hipLaunchKernelGGL((k_hv_rows2<TC, false, VT>), grid, blk, sm, st, B->D, B->d_rows, op->nfam64,
-                           (const TFam64*)op->d_fam64, op->d_foff64, op->nterm64, op->d_ent64, op->ezz0_64, B->rinfo,
-                           B->d_rlo, B->rlo_n, B->d_rhi, B->d_dir, B->d_phase, nph, B->d_ntab, u, v, w);
+#define ED_R2GO(TRIV, WIMG)                                                                                        \
+    hipLaunchKernelGGL((k_hv_rows2<TC, TRIV, WIMG, VT>), grid, blk, sm, st, B->D, B->d_rows, op->nfam64,            \
+                       (const TFam64*)op->d_fam64, op->d_foff64, op->nterm64, op->d_ent64, op->ezz0_64, B->rinfo,   \
+                       B->d_rlo, B->rlo_n, B->d_rhi, B->d_dir, B->d_wimg_map, B->wimg_cb, B->d_phase, nph,          \
+                       B->d_ntab, u, v, w)
+    if (B->trivial_k) {
+        if (wimg) ED_R2GO(true, true);
+        else ED_R2GO(true, false);
+    } else {
+        if (wimg) ED_R2GO(false, true);
+        else ED_R2GO(false, false);
+    }
+#undef ED_R2GO
 }
 
 }  // namespace
 
 template <typename VT>
-int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
+int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wide_image) {
     const ed_basis_s* B = op->b;
     if (rows2_smem(op) > 64 * 1024) return ed_fail(ED_EINVAL, "two-row kernel tables exceed 64 KiB of LDS");
 #define ED_R2LAUNCH(n, wd)                                   \
     if (B->N == n && B->rows2_w == wd) {                     \
-        launch_r2<Rows2C<n, wd>, VT>(op, u, v, w, st);       \
+        launch_r2<Rows2C<n, wd>, VT>(op, u, v, w, st, wide_image); \
         HIPCHK(hipGetLastError());                           \
         return ED_OK;                                        \
     }
@@ -164,5 +176,5 @@ int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st
     return ed_fail(ED_EINVAL, "two-row shape (%d, %d) is not compiled", B->N, B->rows2_w);
 }
 
-template int launch_hv_rows2<cd2>(ed_op_s*, const cd2*, const cd2*, cd2*, hipStream_t);
-template int launch_hv_rows2<cf2>(ed_op_s*, const cf2*, const cf2*, cf2*, hipStream_t);
+template int launch_hv_rows2<cd2>(ed_op_s*, const cd2*, const cd2*, cd2*, hipStream_t, bool);
+template int launch_hv_rows2<cf2>(ed_op_s*, const cf2*, const cf2*, cf2*, hipStream_t, bool);

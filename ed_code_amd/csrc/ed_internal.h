@@ -63,6 +63,11 @@ struct ed_basis_s {
     uint32_t img_nslot = 0, img_nmap = 0;
     uint16_t* d_img_map = nullptr;   // [img_nmap] chunk -> slot
     uint32_t* d_img_pos = nullptr;   // [D] image element of row i
+    // wide gather image of the two-row 64-bit kernels (WideImageLayout);
+    // wimg_cb = -1 when the sector has none
+    int wimg_cb = -1;
+    uint64_t wimg_nslot = 0, wimg_nmap = 0;
+    uint32_t* d_wimg_map = nullptr;  // [wimg_nmap] chunk -> slot
 };
 
 struct ed_op_s {
@@ -103,6 +108,8 @@ struct ed_op_s {
     void* d_u128 = nullptr;   // prescaled gather stream, complex128 [D]
     void* d_img64 = nullptr;  // prescaled gather image (compiled rotation kernels), complex64
     void* d_img128 = nullptr; // complex128
+    void* d_wimg64 = nullptr; // prescaled wide gather image (two-row 64-bit kernels), complex64
+    void* d_wimg128 = nullptr;
     std::vector<edc::Bond> bonds;
 };
 
@@ -126,6 +133,6 @@ int rows2_tables(ed_basis_s* B, const std::vector<uint64_t>& rt);
 int rows2_op_setup(ed_op_s* op, const std::vector<edc::Bond>& bonds, const std::vector<edc::Chiral>& ch);
 bool rows2_shape_compiled(int N, int W);
 template <typename VT>
-int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
+int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wide_image);
 ed_basis_s* block_basis(const ed_block_s* K);
 int block_size(const ed_block_s* K);

@@ -1433,6 +1433,7 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
                 // complex128 one of the FP64 check
                 if (op->d_u64) { dfree(op->d_u64); op->d_u64 = nullptr; }
                 if (op->d_img64) { dfree(op->d_img64); op->d_img64 = nullptr; }
+                if (op->d_wimg64) { dfree(op->d_wimg64); op->d_wimg64 = nullptr; }
                 prev.push_back(x);
             }
         }

@@ -149,43 +149,59 @@ inline std::vector<uint64_t> rank_tables(int N, int n, RankInfo& R) {
 // map sending every chunk without representatives to the zero slot; inside a
 // slot the positions of ranks that are no representative of the sector
 // (orbits outside the sector at k != 0, non-minimal words) are never written
-// and stay zero.  The map takes 2 bytes per chunk and fits in LDS, so a
-// gather needs no directory load: the rank alone addresses u.
-struct ImageLayout {
+// and stay zero.  The rank alone addresses u: a gather needs the small map
+// (LDS or cache resident) instead of a directory entry.
+template <typename M>
+struct ImageLayoutT {
     int cb = -1;                 // chunk bits; -1: no image
-    uint32_t nslot = 0;          // slots holding representatives; the zero slot
-    std::vector<uint16_t> map;   // chunk -> slot, (C - 1) >> cb + 1 entries
-    uint64_t elems() const { return cb < 0 ? 0 : (uint64_t)(nslot + 1) << cb; }
+    uint64_t nslot = 0;          // slots holding representatives; the zero slot
+    std::vector<M> map;          // chunk -> slot, (C - 1) >> cb + 1 entries
+    uint64_t elems() const { return cb < 0 ? 0 : (nslot + 1) << cb; }
 };
+// The compiled rotation kernels keep a u16 map of at most kImageMapMax
+// entries (6 KiB) in LDS; the two-row 64-bit kernels (N <= 40) read a u32 map
+// of at most kWideMapMax entries (16 MiB) from global memory, where it stays
+// in L2 and the Infinity Cache.
+using ImageLayout = ImageLayoutT<uint16_t>;
+using WideImageLayout = ImageLayoutT<uint32_t>;
+constexpr uint64_t kImageMapMax = 3072;
+constexpr uint64_t kWideMapMax = uint64_t(1) << 22;
 
-constexpr uint32_t kImageMapMax = 3072;   // map entries (6 KiB of LDS)
-
-// Smallest chunk size whose map fits kImageMapMax entries (at least one
-// 64-rank directory group per chunk).
-inline int image_min_bits(uint64_t C) {
+// Smallest chunk size whose map fits max_entries (at least one 64-rank
+// directory group per chunk).
+inline int image_min_bits(uint64_t C, uint64_t max_entries) {
     int cb = 6;
-    while (((C > 0 ? C - 1 : 0) >> cb) + 1 > kImageMapMax) cb++;
+    while (((C > 0 ? C - 1 : 0) >> cb) + 1 > max_entries) cb++;
     return cb;
 }
 
 // Builds the layout for chunk bits cb from touched[c] != 0 (chunk c holds a
-// representative).  False when the image would exceed 6 (D + 1) elements (and
-// 2^22) or 4 GiB of complex128, the reach of the kernels' 32-bit offsets.
-inline bool image_layout(uint64_t C, int64_t D, int cb, const std::vector<uint8_t>& touched, ImageLayout& L) {
+// representative).  False when the image would exceed max_elems elements or
+// the slot count the map type holds.
+template <typename M>
+inline bool image_layout(uint64_t C, int cb, const std::vector<uint8_t>& touched, uint64_t max_elems,
+                         ImageLayoutT<M>& L) {
     const uint64_t nmap = ((C > 0 ? C - 1 : 0) >> cb) + 1;
     if (touched.size() < nmap) return false;
-    uint32_t n = 0;
+    uint64_t n = 0;
     for (uint64_t c = 0; c < nmap; c++) n += touched[c] ? 1u : 0u;
-    const uint64_t elems = (uint64_t)(n + 1) << cb;
-    if (n >= 0xffffu || elems * 16 > 0xffffffffull || elems > std::max<uint64_t>(6 * (uint64_t)(D + 1), 1ull << 22))
-        return false;
+    if (n >= (uint64_t)(M)~M(0) || ((n + 1) << cb) > max_elems) return false;
     L.cb = cb;
     L.nslot = n;
-    L.map.assign(nmap, (uint16_t)n);
-    uint32_t s = 0;
+    L.map.assign(nmap, (M)n);
+    M sl = 0;
     for (uint64_t c = 0; c < nmap; c++)
-        if (touched[c]) L.map[c] = (uint16_t)s++;
+        if (touched[c]) L.map[c] = sl++;
     return true;
 }
+
+// Element bound of the compiled kernels' image: 6 (D + 1) (at least 2^22) and
+// 4 GiB of complex128, the reach of their 32-bit load offsets.
+inline uint64_t image_max_elems(int64_t D) {
+    return std::min<uint64_t>(std::max<uint64_t>(6 * (uint64_t)(D + 1), uint64_t(1) << 22), 0xffffffffull / 16);
+}
+
+// Element bound of the two-row kernels' image: 4 (D + 1) + 2^22.
+inline uint64_t wide_image_max_elems(int64_t D) { return 4 * (uint64_t)(D + 1) + (uint64_t(1) << 22); }
 
 }  // namespace edc

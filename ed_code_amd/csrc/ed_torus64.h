@@ -110,10 +110,13 @@ ED_HD uint64_t rank64(uint64_t s, const RankInfo& R, const uint32_t* lo, const u
 }
 
 // One row with K elements in flight (same scheme as torus_row_batched).
-template <class TC, bool TRIV, int K, class LoadU, class Gf>
+// jof(r) gives the element of u holding the target of rank r, or -1 for a
+// rank outside the sector; such elements, and the empty slots, load the
+// row's own element (self) and add it with a zero coefficient.
+template <class TC, bool TRIV, int K, class JOf, class LoadU, class Gf>
 ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff, const TEntry* ent, double ezz0,
-                       const RankInfo& R, const uint32_t* rlo, const uint64_t* rhi, const DirEntry* dir,
-                       const cd2* phase, int64_t self, LoadU&& loadu, Gf&& g) {
+                       const RankInfo& R, const uint32_t* rlo, const uint64_t* rhi, const cd2* phase, int64_t self,
+                       JOf&& jof, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint64_t cur = nfam > 0 ? fam_mask64<TC>(s, fam[0]) : 0ull;
@@ -158,18 +161,11 @@ ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff,
             }
         }
         more = ok[K - 1];
-        DirEntry e[K];
-        uint32_t off[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            off[q] = (uint32_t)(rk[q] % kDirSpan);
-            e[q] = dir[rk[q] / kDirSpan];
-        }
         bool use[K];
         int64_t j[K];
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            j[q] = dir_decode_off(e[q], off[q]);
+            j[q] = jof(rk[q]);
             use[q] = ok[q] && j[q] >= 0;
         }
         cd2 xv[K];

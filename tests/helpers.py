@@ -283,6 +283,8 @@ def hostcore():
         H.hc_rank_split.argtypes = [vp, C.c_uint64]
         H.hc_hv_rows2.restype = C.c_int
         H.hc_hv_rows2.argtypes = [vp] * 3
+        H.hc_hv_rows2w.restype = C.c_int
+        H.hc_hv_rows2w.argtypes = [vp, vp, vp, C.c_int]
         H.hc_tridiag.restype = C.c_int
         H.hc_tridiag.argtypes = [C.c_int, vp, vp, vp]
         _hc = H
@@ -342,12 +344,13 @@ class HostCore:
             return None
         return w
 
-    def hv_rows2(self, v):
-        """Two-row 64-bit row function (ed_torus64.h); None if the lattice is
-        not a compiled two-row shape."""
+    def hv_rows2(self, v, wide=True):
+        """Two-row 64-bit row function (ed_torus64.h), gathering from the wide
+        image (or by row through the directory); None if the lattice is not a
+        compiled two-row shape."""
         v = np.ascontiguousarray(v, np.complex128)
         w = np.zeros(self.D, np.complex128)
-        if hostcore().hc_hv_rows2(self.h, v.ctypes.data, w.ctypes.data):
+        if hostcore().hc_hv_rows2w(self.h, v.ctypes.data, w.ctypes.data, int(wide)):
             return None
         return w
 

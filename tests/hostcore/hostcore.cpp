@@ -153,18 +153,37 @@ void hv_modes(HC* h, const Torus& T, bool triv, int mode, const std::vector<TFam
 #undef HC_RUN
 }
 
+// Gather image of the sector as the library chooses it (basis_image_build):
+// chunks marked from the directory.
+template <typename M>
+void host_image(HC* h, uint64_t max_entries, uint64_t max_elems, ImageLayoutT<M>& L) {
+    const int cb0 = image_min_bits(h->C, max_entries);
+    for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
+        std::vector<uint8_t> t(((h->C - 1) >> cb) + 1, 0);
+        for (size_t g = 0; g < h->dir.size(); g++)
+            if (h->dir[g].b0 | h->dir[g].b1) t[g >> (cb - 6)] = 1;
+        image_layout(h->C, cb, t, max_elems, L);
+    }
+}
+
+// u gathered from the wide image (the two-row kernels' default) when wide,
+// else by row through the directory.
 template <class TC, bool TRIV>
 void hv_rows2(HC* h, const std::vector<TFam64>& fam, const std::vector<int>& foff, const std::vector<TEntry>& ent,
-              double ezz0, const std::vector<uint32_t>& rlo, const uint64_t* rhi, const std::vector<cd2>& u,
-              const cd2* v, cd2* w) {
+              double ezz0, const std::vector<uint32_t>& rlo, const uint64_t* rhi, const WideImageLayout& L, bool wide,
+              const std::vector<cd2>& u, const cd2* v, cd2* w) {
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
         uint64_t s = h->rows[i] & kStateMask;
         double nrm = h->nt[h->rows[i] >> kLidShift];
         double ar = 0, ai = 0;
+        auto jof = [&](uint64_t r) -> int64_t {
+            if (wide) return (int64_t)(((uint64_t)L.map.at(r >> L.cb) << L.cb) | (r & ((uint64_t(1) << L.cb) - 1)));
+            return dir_decode_off(h->dir[r / kDirSpan], (uint32_t)(r % kDirSpan));
+        };
         double ez = rows2_row<TC, TRIV, 4>(s, (int)fam.size(), fam.data(), foff.data(), ent.data(), ezz0, h->R,
-                                            rlo.data(), rhi, h->dir.data(), h->ph.data(), i,
-                                            [&](int64_t j) { return u[j]; },
+                                            rlo.data(), rhi, h->ph.data(), i, jof,
+                                            [&](int64_t j) { return u.at(j); },
                                             [&](const cd2& x, double cr, double ci) {
                                                 ar += cr * x.re - ci * x.im;
                                                 ai += cr * x.im + ci * x.re;
@@ -312,8 +331,9 @@ void hc_hv(void* p, const double* v, double* w) {
 }
 
 // H*v through the two-row 64-bit path (ed_torus64.h) for the compiled
-// shapes; 1 if the lattice is not a compiled two-row shape (w untouched).
-int hc_hv_rows2(void* p, const double* v_, double* w_) {
+// shapes, gathering from the wide image (wide != 0) or by row through the
+// directory; 1 if the lattice is not a compiled two-row shape (w untouched).
+int hc_hv_rows2w(void* p, const double* v_, double* w_, int wide) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
@@ -336,17 +356,27 @@ int hc_hv_rows2(void* p, const double* v_, double* w_) {
     const size_t top = (size_t)h->R.off[h->R.nbytes - 1] * 256;
     std::vector<uint32_t> rlo(h->rt.begin(), h->rt.begin() + top);
     const uint64_t* rhi = h->rt.data() + top;
-    std::vector<cd2> u(h->rows.size());
-    for (size_t i = 0; i < u.size(); i++) {
+    WideImageLayout L;
+    if (wide) {
+        host_image(h, kWideMapMax, wide_image_max_elems((int64_t)h->rows.size()), L);
+        if (L.cb < 0) return 1;
+    }
+    std::vector<cd2> u(wide ? L.elems() : h->rows.size(), cd2{0.0, 0.0});
+    for (size_t i = 0; i < h->rows.size(); i++) {
         double f = h->nit[h->rows[i] >> kLidShift];
-        u[i] = cd2{v[i].re * f, v[i].im * f};
+        size_t e = i;
+        if (wide) {
+            const uint64_t r = rank_of<uint64_t>(h->rows[i] & kStateMask, h->R, h->rt.data());
+            e = ((uint64_t)L.map[r >> L.cb] << L.cb) | (r & ((uint64_t(1) << L.cb) - 1));
+        }
+        u.at(e) = cd2{v[i].re * f, v[i].im * f};
     }
     const bool triv = h->kx == 0 && h->ky == 0;
-#define HC_R2(n, wd)                                                                                       \
-    if (N == n && W == wd) {                                                                               \
-        if (triv) hv_rows2<Rows2C<n, wd>, true>(h, fam, foff, ent, ezz0, rlo, rhi, u, v, w);               \
-        else hv_rows2<Rows2C<n, wd>, false>(h, fam, foff, ent, ezz0, rlo, rhi, u, v, w);                   \
-        return 0;                                                                                          \
+#define HC_R2(n, wd)                                                                                           \
+    if (N == n && W == wd) {                                                                                   \
+        if (triv) hv_rows2<Rows2C<n, wd>, true>(h, fam, foff, ent, ezz0, rlo, rhi, L, wide, u, v, w);          \
+        else hv_rows2<Rows2C<n, wd>, false>(h, fam, foff, ent, ezz0, rlo, rhi, L, wide, u, v, w);              \
+        return 0;                                                                                              \
     }
     ED_ROWS2_SHAPES(HC_R2)
     HC_R2(12, 6)
@@ -354,6 +384,8 @@ int hc_hv_rows2(void* p, const double* v_, double* w_) {
 #undef HC_R2
     return 1;
 }
+
+int hc_hv_rows2(void* p, const double* v, double* w) { return hc_hv_rows2w(p, v, w, 1); }
 
 // H*v through the rotation fast path (ed_torus.h).  Returns 1 if the
 // lattice/terms do not fit the fast path (w untouched).
@@ -436,13 +468,7 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
     // the gather image as the library lays it out (basis_image_build):
     // chunks marked from the directory, u written at the rows' positions
     ImageLayout L;
-    const int cb0 = image_min_bits(h->C);
-    for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
-        std::vector<uint8_t> t(((h->C - 1) >> cb) + 1, 0);
-        for (size_t g = 0; g < h->dir.size(); g++)
-            if (h->dir[g].b0 | h->dir[g].b1) t[g >> (cb - 6)] = 1;
-        image_layout(h->C, (int64_t)h->rows.size(), cb, t, L);
-    }
+    host_image(h, kImageMapMax, image_max_elems((int64_t)h->rows.size()), L);
     if (L.cb < 0) return 1;
     std::vector<cd2> u(L.elems(), cd2{0.0, 0.0});
     for (size_t i = 0; i < h->rows.size(); i++) {

@@ -220,6 +220,8 @@ def test_two_row_64bit_row_vs_oracle(model, n, k):
     assert w is not None
     wo = o.hv(v)
     assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+    # the wide gather image and the directory give the same sums
+    assert np.array_equal(w, h.hv_rows2(v, wide=False))
 
 
 def test_split_rank_tables_at_40_sites():
