@@ -76,8 +76,9 @@ void ed_lattice_destroy(ed_lattice lat);
 
 /* Momentum basis of the sector (n_ones, kx, ky), built on the GPU.
  * Replaces pConstruct_States (Momentum_Basis.cpp:645-778, decl
- * Momentum_Basis.h:151-165): representatives, orbit norms and the index
- * directory stay resident in HBM.  kx, ky are integers in [0,L1) x [0,L2)
+ * Momentum_Basis.h:151-165): representatives, orbit norms, the index
+ * directory and the gather-image layout (chunk map, row positions) stay
+ * resident in HBM.  kx, ky are integers in [0,L1) x [0,L2)
  * (Lanczos_sector.cpp:51-57); px = 2 pi kx / L1, py = 2 pi ky / L2. */
 int ed_basis_build(ed_lattice lat, int32_t n_ones, int32_t kx, int32_t ky, ed_basis* out);
 /* Three-generator sector (pConstruct_States, Momentum_Basis.cpp:781-897,
@@ -240,7 +241,7 @@ int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evec
  * sector_lanczos) and sweeps sectors one at a time (ARPACK_full_eigs.cpp:
  * 169-277); a block holds M term sets on one basis and applies all M
  * Hamiltonians in one pass over the rows, sharing the orbit search, rank and
- * directory work of every gathered element.  The term sets must have the same
+ * gather addressing of every gathered element.  The term sets must have the same
  * bond, chiral and bowtie site lists (same order); J, Jz, Ch, ChCh and phases
  * may differ per instance.  M in 1..8; M in {2, 4, 8} on a built-in torus
  * shape without bowtie terms runs the shared kernel, other cases apply the
