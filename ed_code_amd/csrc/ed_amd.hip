@@ -475,19 +475,21 @@ template <typename M>
 static int image_choose(ed_basis_s* B, uint64_t max_entries, uint64_t max_elems, ImageLayoutT<M>& L) {
     const uint64_t C = B->C;
     const int cb0 = image_min_bits(C, max_entries);
-    uint8_t* d_t = nullptr;
-    HIPCHK(hipMalloc(&d_t, ((C - 1) >> cb0) + 1));
+    struct Flags {
+        uint8_t* p = nullptr;
+        ~Flags() { if (p) dfree(p); }
+    } d_t;
+    HIPCHK(hipMalloc(&d_t.p, ((C - 1) >> cb0) + 1));
     for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
         const uint64_t nmap = ((C - 1) >> cb) + 1;
         std::vector<uint8_t> t(nmap);
-        HIPCHK(hipMemset(d_t, 0, nmap));
+        HIPCHK(hipMemset(d_t.p, 0, nmap));
         hipLaunchKernelGGL(k_image_touch, dim3(sblocks(B->ngroups, 256)), dim3(256), 0, 0, B->d_dir, B->ngroups,
-                           cb - 6, d_t);
+                           cb - 6, d_t.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy(t.data(), d_t, nmap, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(t.data(), d_t.p, nmap, hipMemcpyDeviceToHost));
         image_layout(C, cb, t, max_elems, L);
     }
-    dfree(d_t);
     return ED_OK;
 }
 
@@ -630,6 +632,9 @@ extern "C" int ed_basis_export_full(ed_basis b, int64_t* rep_loc, double* state_
 }
 
 // -------------------------------------------------------------- operator --
+static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vector<Chiral>& ch,
+                    const std::vector<Bowtie>& bw);
+
 extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
     if (!b || !t || !out) return ed_fail(ED_EINVAL, "null argument");
     std::vector<Bond> bonds;
@@ -660,6 +665,20 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
     op->b = b;
     op->lam = t->lambda; op->Ch = t->Ch; op->ChCh = t->ChCh;
     op->nb = (int)bonds.size(); op->nc = (int)ch.size(); op->nw = (int)bw.size();
+    const int rc = op_setup(op, bonds, ch, bw);
+    if (rc != ED_OK) {
+        ed_op_destroy(op);   // frees whatever the setup allocated
+        return rc;
+    }
+    *out = op;
+    return ED_OK;
+}
+
+// Device copies of the terms and the tables of the kernel paths the
+// operator can take.  On failure the caller destroys the operator.
+static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vector<Chiral>& ch,
+                    const std::vector<Bowtie>& bw) {
+    ed_basis_s* b = op->b;
     if (!bonds.empty()) {
         HIPCHK(hipMalloc(&op->d_b, bonds.size() * sizeof(Bond)));
         HIPCHK(hipMemcpy(op->d_b, bonds.data(), bonds.size() * sizeof(Bond), hipMemcpyHostToDevice));
@@ -711,14 +730,8 @@ extern "C" int ed_op_create(ed_basis b, const ed_terms* t, ed_op* out) {
         }
     }
     // two-row 64-bit rotation path: no bowtie terms, compiled shape
-    if (b->rows2_w && bw.empty() && b->d_rlo && rows2_shape_compiled(b->N, b->rows2_w)) {
-        int rc = rows2_op_setup(op, bonds, ch);
-        if (rc != ED_OK) {
-            ed_op_destroy(op);
-            return rc;
-        }
-    }
-    *out = op;
+    if (b->rows2_w && bw.empty() && b->d_rlo && rows2_shape_compiled(b->N, b->rows2_w))
+        return rows2_op_setup(op, bonds, ch);
     return ED_OK;
 }
 
