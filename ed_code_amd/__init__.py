@@ -131,6 +131,58 @@ class Basis:
                                          ij.ctypes.data, zz.ctypes.data, pm.ctypes.data))
         return zz.view(np.complex128), pm.view(np.complex128)
 
+    def triangle_orbit(self, tri):
+        """Distinct translates g(i), g(j), g(k) of the oriented triangle tri
+        under the lattice's translations (cyclic rotations of a triple are the
+        same triangle)."""
+        L = self.lattice
+        gens = [L.T1, L.T2] + ([L.T3] if L.T3 is not None else [])
+        seen, todo, out = set(), [tuple(int(s) for s in tri)], []
+        while todo:
+            t = todo.pop()
+            key = min(t[r:] + t[:r] for r in range(3))
+            if key in seen:
+                continue
+            seen.add(key)
+            out.append(t)
+            todo.extend(tuple(int(T[s]) for s in t) for T in gens)
+        return out
+
+    def scalar_chirality(self, triangles, ket, bra=None):
+        """<bra| S_i . (S_j x S_k) |ket> for each oriented triangle (i, j, k),
+        from complex128 device vectors of this basis (bra defaults to ket), in
+        the momentum basis without unwrapping (the reference unwraps the
+        vectors first: analysis/src/chirality_exp.cpp:22-72 applies one cyclic
+        term per call).  Both vectors have the sector's momentum, so every
+        translate of a triangle has the same matrix element: each value is
+        that of the operator averaged over the triangle's translation orbit,
+        one H*v per triangle of the operator made of the three cyclic chiral
+        entries (read_input_file.cpp:262-273, phases 1) of every translate.
+        Those entries of one triangle (i, j, k) with Ch = 1 form
+        -S_i . (S_j x S_k) (bit 1 = spin up, the gather convention of
+        H_v.cpp), so the operator takes Ch = -1 / orbit size; the chiral part
+        of a Hamiltonian with coupling Ch is -Ch * sum of these values."""
+        import torch
+        bra = ket if bra is None else bra
+        for x in (ket, bra):
+            if x.dtype != torch.complex128 or not x.is_cuda or x.numel() != self.D:
+                raise ValueError("vectors must be complex128 CUDA tensors of length D")
+        L = self.lattice
+        out = np.zeros(len(triangles), np.complex128)
+        done = {}   # triangles of one orbit share the value
+        for q, tri in enumerate(triangles):
+            orb = self.triangle_orbit(tri)
+            key = frozenset(min(t[r:] + t[:r] for r in range(3)) for t in orb)
+            if key in done:
+                out[q] = out[done[key]]
+                continue
+            done[key] = q
+            model = Model(name="chirality", n_sites=L.N, T1=L.T1, T2=L.T2, T3=L.T3, lam=0.0, bonds=[],
+                          Ch=-1.0 / len(orb), chiral=lattices.chiral_entries(orb, []))
+            w = Operator(self, model).hv(ket)
+            out[q] = complex(torch.vdot(bra, w).item())
+        return out
+
     def unwrap(self, v):
         """Full-Sz-basis components of the momentum-basis vector v (complex128
         device tensor): (states int64, coefficients complex128), ascending by

@@ -167,6 +167,29 @@ def full_space_h(model, n):
     return H, st
 
 
+def scalar_chirality_matrix(N, st, tri):
+    """S_i . (S_j x S_k) = sum_abc eps_abc S^a_i S^b_j S^c_k on the weight-n
+    states st, built from the spin-1/2 matrices (bit 1 = up) with no use of
+    the Hamiltonian's term lists: column t is the operator applied to |t>."""
+    import itertools
+    sx = np.array([[0, 1], [1, 0]]) / 2
+    sy = np.array([[0, 1j], [-1j, 0]]) / 2
+    sz = np.array([[-1, 0], [0, 1]]) / 2
+    S = [sx, sy, sz]
+    M = np.zeros((8, 8), complex)
+    for a, b, c in itertools.permutations(range(3)):
+        M += np.linalg.det(np.eye(3)[[a, b, c]]) * np.kron(np.kron(S[a], S[b]), S[c])
+    M = M.reshape((2,) * 6)
+    ax = [N - 1 - tri[0], N - 1 - tri[1], N - 1 - tri[2]]   # axis of site s in the (2,)*N view
+    X = np.zeros((len(st), len(st)), complex)
+    for col, t in enumerate(st):
+        e = np.zeros((2,) * N, complex)
+        e.reshape(-1)[t] = 1
+        y = np.moveaxis(np.tensordot(M, np.moveaxis(e, ax, [0, 1, 2]), axes=([3, 4, 5], [0, 1, 2])), [0, 1, 2], ax)
+        X[:, col] = y.reshape(-1)[st]
+    return X
+
+
 def momentum_projector(model, n, kx, ky, st):
     """Orthonormal momentum states as columns over the full weight-n space:
     |r(k)> ~ sum_g exp(-i k.g) g|r> over the translation grid."""

@@ -102,3 +102,42 @@ def test_unwrap_to_files(gpu_ready, tmp_path):
     assert np.array_equal(pb["states"], reps) and np.array_equal(pb["coeffs"], vecs[0])
     n_p, p, r, nm, c = E.read_pev(prefix + "_pEV0.txt")
     assert n_p == basis.D and np.array_equal(r, reps) and np.abs(c - vecs[0]).max() < 1e-15
+
+
+@pytest.mark.parametrize("k", [(0, 0), (1, 1)])
+def test_scalar_chirality_vs_full_basis(gpu_ready, k):
+    """<bra| S_i.(S_j x S_k) |ket> of momentum-basis vectors (orbit-averaged
+    operator, no unwrapping) against the spin-matrix operator in the full
+    basis, on the ground state, an excited state and random vectors of a
+    chiral triangular torus; the ground-state energy then splits into the bond
+    and chiral parts: E0 = sum_b J (lam <SzSz> + <S+S-+h.c.>/2) - Ch sum_t chi_t."""
+    import torch
+    import ed_code_amd as ed
+    model, n = L.triangular(4, 3, Ch=0.3), 6
+    print(f"[case] chirality {model.name} k={k}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    basis = ed.Basis(ed.Lattice.of(model), n, *k)
+    H, st = Hp.full_space_h(model, n)
+    Pm = Hp.momentum_projector(model, n, *k, st)
+    ev, vec = np.linalg.eigh(o.make_h())
+    rng = np.random.default_rng(5)
+    tris = [model.triangles[0], model.triangles[1], model.triangles[7]]
+    X = [Hp.scalar_chirality_matrix(model.n_sites, st, t) for t in tris]
+    r1 = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    r2 = rng.normal(size=o.D) + 1j * rng.normal(size=o.D)
+    for ket, bra in ((vec[:, 0], vec[:, 0]), (vec[:, 1], vec[:, 0]), (r1, r2)):
+        tk = torch.tensor(ket, dtype=torch.complex128, device="cuda")
+        tb = torch.tensor(bra, dtype=torch.complex128, device="cuda")
+        chi = basis.scalar_chirality(tris, tk, tb)
+        fk, fb = Pm @ ket, Pm @ bra
+        scale = max(1.0, np.linalg.norm(fk) * np.linalg.norm(fb))
+        for q in range(len(tris)):
+            assert abs(chi[q] - np.vdot(fb, X[q] @ fk)) < 1e-12 * scale, (k, q)
+    # energy split of the ground state
+    psi = torch.tensor(vec[:, 0], dtype=torch.complex128, device="cuda")
+    pairs = [(x, y) for (x, y, _J, _ph) in model.bonds]
+    zz, pm = basis.pair_correlations(pairs, psi)
+    e_bonds = sum(J * (model.lam * zz[b].real + pm[b].real) for b, (_x, _y, J, _ph) in enumerate(model.bonds))
+    chi = basis.scalar_chirality(model.triangles, psi)
+    assert abs(e_bonds - model.Ch * chi.real.sum() - ev[0]) < 1e-9
+    assert np.abs(chi.imag).max() < 1e-12

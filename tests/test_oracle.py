@@ -236,3 +236,21 @@ def test_unwrap_is_the_projected_vector():
     ref = {s: c for s, c in zip(full, psi) if abs(c.real) > 1e-8 or abs(c.imag) > 1e-8}
     assert list(st) == sorted(ref)
     assert np.abs(co - np.array([ref[s] for s in st])).max() < 1e-13
+
+
+def test_chiral_entries_are_minus_the_scalar_chirality():
+    """The three cyclic chiral entries of one triangle (read_input_file.cpp:
+    262-273) with Ch = 1, as the full-space H of H_v.cpp's gather convention
+    applies them, equal -S_i . (S_j x S_k) built from spin matrices; the
+    other orientation gives +."""
+    m0 = L.triangular(4, 3)
+    N, n = m0.n_sites, 6
+    for tri in (m0.triangles[0], m0.triangles[5]):
+        m = L.Model(name="t", n_sites=N, T1=m0.T1, T2=m0.T2, lam=0.0, bonds=[], Ch=1.0,
+                    chiral=L.chiral_entries([tri], []))
+        H, st = Hp.full_space_h(m, n)
+        X = Hp.scalar_chirality_matrix(N, st, tri)
+        assert np.abs(X).max() > 0.2
+        assert np.abs(H + X).max() < 1e-15
+        i, j, k = tri
+        assert np.abs(Hp.scalar_chirality_matrix(N, st, (i, k, j)) - H).max() < 1e-15
