@@ -189,13 +189,13 @@ def main():
         raise SystemExit("no GPU visible")
     ed.init(0)
     if args.workload == "sq4x8":
-        model = L.square(4, 8, J1=1.0, J2=0.5, lam=1.0)
+        model, couplings = L.square(4, 8, J1=1.0, J2=0.5, lam=1.0), "J1=1 J2=0.5 Delta=1"
     elif args.workload == "chain24":
-        model = L.chain(24)
+        model, couplings = L.chain(24), "J1=1 Delta=1"
     elif args.workload == "tri6x5":
-        model = L.triangular(6, 5, Ch=0.15)
+        model, couplings = L.triangular(6, 5, Ch=0.15), "J1=1 Delta=1 Jchi=0.15"
     else:
-        model = L.triangular(3, 9, Ch=0.15)
+        model, couplings = L.triangular(3, 9, Ch=0.15), "J1=1 Delta=1 Jchi=0.15"
     n_ones = L.n_ones_for_sz(model.n_sites, 0.0)
     kx = ky = 0
     t0 = time.time()
@@ -323,7 +323,7 @@ def main():
         "vs_baseline": None,
         "dtype": main_dt,
         "data": "synthetic (seeded complex normal v; lattice and couplings fixed)",
-        "config": {"workload": f"{model.name} J1=1 J2=0.5 Delta=1 Sz=0 k=(0,0) single H·v", "n_sites": model.n_sites,
+        "config": {"workload": f"{model.name} {couplings} Sz=0 k=(0,0) single H·v", "n_sites": model.n_sites,
                    "n_up": n_ones, "D": D, "distinct_targets": G, "vector": main_dt},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS,
