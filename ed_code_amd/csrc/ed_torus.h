@@ -255,56 +255,85 @@ ED_HD uint32_t fam_mask_c(uint32_t s, const TFam& F) {
     return (s ^ rot_c<TC>(hi, lo, F.q)) & F.S;
 }
 
-// torus_rep for a compile-time shape; same result and phase code.
+// Top-aligned doubled word for the orbit minimum of an N-bit field:
+// Y = s 2^(64-N) + s 2^(64-2N), hi:lo = Y.  The left rotation of s by q is the
+// top N bits of alignbit(hi, lo, 32 - q) (0 < q < 32) and of hi (q = 0), the
+// same rotation rot_c gives.  The low 32 - N bits of such a candidate continue
+// the cycle; they only order candidates whose top N bits are equal, so the
+// minimum's top N bits are the exact minimum and no candidate needs a mask.
+template <class TC>
+ED_HD void cat_top(uint32_t s, uint32_t& hi, uint32_t& lo) {
+    if (TC::N == 32) { hi = s; lo = s; return; }
+    constexpr int S = 32 - TC::N;
+    if (2 * TC::N > 32) {
+        hi = (s << S) | (s >> (2 * TC::N - 32));
+        lo = s << (64 - 2 * TC::N);
+    } else {
+        hi = (s << S) | (s << (32 - 2 * TC::N));
+        lo = 0u;
+    }
+}
+
+// top-aligned candidate of rotation q (see cat_top)
+template <class TC>
+ED_HD uint32_t rot_top(uint32_t hi, uint32_t lo, int q) {
+    return q == 0 ? hi : alignbit32(hi, lo, (uint32_t)(32 - q));
+}
+
+// torus_rep for a compile-time shape; same result and phase code.  The
+// candidates are compared top-aligned (cat_top), one instruction each.
 template <class TC, bool TRACK>
 ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
+    constexpr int S = 32 - TC::N;
     uint32_t best = 0xffffffffu;
     if (TC::ring) {
         uint32_t hi, lo;
-        cat_c<TC>(t, hi, lo);
+        cat_top<TC>(t, hi, lo);
 #pragma unroll
         for (int a = 0; a < TC::L1; a += 2) {
-            uint32_t c0 = rot_c<TC>(hi, lo, a);
-            uint32_t c1 = (a + 1 < TC::L1) ? rot_c<TC>(hi, lo, a + 1) : 0xffffffffu;
+            uint32_t c0 = rot_top<TC>(hi, lo, a);
+            uint32_t c1 = (a + 1 < TC::L1) ? rot_top<TC>(hi, lo, a + 1) : 0xffffffffu;
             best = min3u(best, c0, c1);
         }
+        const uint32_t rep = best >> S;
         if (TRACK) {
             int bx = 0;
             for (int a = TC::L1 - 1; a >= 0; a--)
-                if (rot_c<TC>(hi, lo, a) == best) bx = a;
+                if ((rot_top<TC>(hi, lo, a) >> S) == rep) bx = a;
             code = (TC::L1 - bx) % TC::L1;
         } else {
             code = 0;
         }
-        return best;
+        return rep;
     }
     uint32_t mins[TC::L1];
 #pragma unroll
     for (int a = 0; a < TC::L1; a++) {
         uint32_t sa = field_rot_rt<TC>(t, a);   // a is a constant after unrolling
         uint32_t hi, lo;
-        cat_c<TC>(sa, hi, lo);
+        cat_top<TC>(sa, hi, lo);
         uint32_t m = 0xffffffffu;
 #pragma unroll
         for (int b = 0; b < TC::L2; b += 2) {
-            uint32_t c0 = rot_c<TC>(hi, lo, b * TC::W);
-            uint32_t c1 = (b + 1 < TC::L2) ? rot_c<TC>(hi, lo, (b + 1) * TC::W) : 0xffffffffu;
+            uint32_t c0 = rot_top<TC>(hi, lo, b * TC::W);
+            uint32_t c1 = (b + 1 < TC::L2) ? rot_top<TC>(hi, lo, (b + 1) * TC::W) : 0xffffffffu;
             m = min3u(m, c0, c1);
         }
         mins[a] = m;
         best = m < best ? m : best;
     }
+    const uint32_t rep = best >> S;
     if (TRACK) {
         int bx = 0;
         for (int a = TC::L1 - 1; a >= 0; a--)
-            if (mins[a] == best) bx = a;
+            if ((mins[a] >> S) == rep) bx = a;
         uint32_t sa = field_rot_rt<TC>(t, bx);
         uint32_t hi, lo;
         cat_c<TC>(sa, hi, lo);
         int by = 0;
         for (int y = TC::L2 - 1; y >= 0; y--) {
             int b = ((y - bx) % TC::L2 + TC::L2) % TC::L2;
-            if (rot_c<TC>(hi, lo, b * TC::W) == best) by = y;
+            if (rot_c<TC>(hi, lo, b * TC::W) == rep) by = y;
         }
         int ty = bx * (TC::L2 - 1) + by;
         int lx = (TC::L1 - bx) % TC::L1;
@@ -313,7 +342,7 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
     } else {
         code = 0;
     }
-    return best;
+    return rep;
 }
 
 // torus_row for a compile-time shape.

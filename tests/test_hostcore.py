@@ -170,6 +170,9 @@ def test_compiled_shape_search_matches_generic_walk(model, n, k):
         row = sum(1 << q for q in range(w // 2))
         t = sum(row << (w * y) for y in range(L2))
         assert h.torus_rep_c(t) == h.generic_rep(t)
+    else:   # rings: period-2 and period-4 patterns
+        for t in (sum(1 << q for q in range(0, model.n_sites, 2)), sum(3 << q for q in range(0, model.n_sites, 4))):
+            assert h.torus_rep_c(t) == h.generic_rep(t)
 
 
 BATCHED = [(L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 3)), (L.square(4, 4, J2=0.5), 8, (0, 0)),
