@@ -159,11 +159,11 @@ struct Ctx {
 
 unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D + kRedThreads - 1) / kRedThreads); }
 
-constexpr int kChunk = 8;   // vectors per pass of the multi-vector kernels
 
 template <typename XT, typename YT>
 void launch_multidot(Ctx& c, int mm, const XT* X, int64_t ldx, const YT* y, unsigned g, int kofs) {
     switch (mm) {
+        case 16: hipLaunchKernelGGL((k_multidot<16, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
         case 8: hipLaunchKernelGGL((k_multidot<8, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
         case 4: hipLaunchKernelGGL((k_multidot<4, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
         case 2: hipLaunchKernelGGL((k_multidot<2, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
@@ -180,7 +180,7 @@ int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<s
         int mm = std::min(kMaxMulti, m - k0);
         // passes of 8, 4, 2, 1 vectors fill columns [0, mm) of the partials
         for (int q = 0; q < mm;) {
-            int step = (mm - q) >= 8 ? 8 : (mm - q) >= 4 ? 4 : (mm - q) >= 2 ? 2 : 1;
+            int step = (mm - q) >= 16 ? 16 : (mm - q) >= 8 ? 8 : (mm - q) >= 4 ? 4 : (mm - q) >= 2 ? 2 : 1;
             launch_multidot<XT, YT>(c, step, X + (int64_t)(k0 + q) * ldx, ldx, y, g, q);
             HIPCHK(hipGetLastError());
             q += step;
@@ -219,10 +219,11 @@ int multiaxpy(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<std::co
     HIPCHK(hipMemcpyAsync(dc, cc.data(), m * sizeof(cd2), hipMemcpyHostToDevice, c.st));
     bool first = true;
     for (int q = 0; q < m;) {
-        int step = (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
+        int step = (m - q) >= 16 ? 16 : (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
         double aa = first ? a : 1.0;
         const XT* Xq = X + (int64_t)q * ldx;
         switch (step) {
+            case 16: hipLaunchKernelGGL((k_multiaxpy<16, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
             case 8: hipLaunchKernelGGL((k_multiaxpy<8, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
             case 4: hipLaunchKernelGGL((k_multiaxpy<4, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
             case 2: hipLaunchKernelGGL((k_multiaxpy<2, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, dc + q, aa, y); break;
@@ -243,9 +244,10 @@ int accumulate(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<double
     if (m > c.coef_cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", m);
     HIPCHK(hipMemcpyAsync(c.d_t_big, t.data(), m * sizeof(double), hipMemcpyHostToDevice, c.st));
     for (int q = 0; q < m;) {
-        int step = (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
+        int step = (m - q) >= 16 ? 16 : (m - q) >= 8 ? 8 : (m - q) >= 4 ? 4 : (m - q) >= 2 ? 2 : 1;
         const XT* Xq = X + (int64_t)q * ldx;
         switch (step) {
+            case 16: hipLaunchKernelGGL((k_accumulate<16, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
             case 8: hipLaunchKernelGGL((k_accumulate<8, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
             case 4: hipLaunchKernelGGL((k_accumulate<4, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
             case 2: hipLaunchKernelGGL((k_accumulate<2, XT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, Xq, ldx, c.d_t_big + q, y); break;
@@ -1101,9 +1103,10 @@ int bdot(BCtx& c, int mv, const XT* X, int64_t ldx, const YT* y, std::vector<std
     for (int k0 = 0; k0 < mv; k0 += kMaxMulti) {
         const int mm = std::min(kMaxMulti, mv - k0);
         for (int q = 0; q < mm;) {
-            const int step = (mm - q) >= 8 ? 8 : (mm - q) >= 4 ? 4 : (mm - q) >= 2 ? 2 : 1;
+            const int step = (mm - q) >= 16 ? 16 : (mm - q) >= 8 ? 8 : (mm - q) >= 4 ? 4 : (mm - q) >= 2 ? 2 : 1;
             const XT* Xq = X + (int64_t)(k0 + q) * ldx;
             switch (step) {
+                case 16: launch_bdot<16, XT, YT>(c, Xq, ldx, y, q); break;
                 case 8: launch_bdot<8, XT, YT>(c, Xq, ldx, y, q); break;
                 case 4: launch_bdot<4, XT, YT>(c, Xq, ldx, y, q); break;
                 case 2: launch_bdot<2, XT, YT>(c, Xq, ldx, y, q); break;
@@ -1142,11 +1145,12 @@ int baxpy(BCtx& c, int mv, const XT* X, int64_t ldx, const std::vector<std::comp
         return ED_OK;
     }
     for (int q = 0; q < mv;) {
-        const int step = (mv - q) >= 8 ? 8 : (mv - q) >= 4 ? 4 : (mv - q) >= 2 ? 2 : 1;
+        const int step = (mv - q) >= 16 ? 16 : (mv - q) >= 8 ? 8 : (mv - q) >= 4 ? 4 : (mv - q) >= 2 ? 2 : 1;
         const XT* Xq = X + (int64_t)q * ldx;
         const cd2* cq = c.d_coef + (size_t)q * c.M;
         double* aq = q == 0 ? da : nullptr;
         switch (step) {
+            case 16: hipLaunchKernelGGL((k_baxpy<16, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
             case 8: hipLaunchKernelGGL((k_baxpy<8, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
             case 4: hipLaunchKernelGGL((k_baxpy<4, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
             case 2: hipLaunchKernelGGL((k_baxpy<2, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
@@ -1173,10 +1177,11 @@ int baccumulate(BCtx& c, int mv, const XT* X, int64_t ldx, const std::vector<dou
     if (mv > c.cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", mv);
     HIPCHK(hipMemcpyAsync(c.d_t, t.data(), (size_t)mv * c.M * sizeof(double), hipMemcpyHostToDevice, c.st));
     for (int q = 0; q < mv;) {
-        const int step = (mv - q) >= 8 ? 8 : (mv - q) >= 4 ? 4 : (mv - q) >= 2 ? 2 : 1;
+        const int step = (mv - q) >= 16 ? 16 : (mv - q) >= 8 ? 8 : (mv - q) >= 4 ? 4 : (mv - q) >= 2 ? 2 : 1;
         const XT* Xq = X + (int64_t)q * ldx;
         const double* tq = c.d_t + (size_t)q * c.M;
         switch (step) {
+            case 16: hipLaunchKernelGGL((k_baccumulate<16, XT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, tq, y); break;
             case 8: hipLaunchKernelGGL((k_baccumulate<8, XT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, tq, y); break;
             case 4: hipLaunchKernelGGL((k_baccumulate<4, XT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, tq, y); break;
             case 2: hipLaunchKernelGGL((k_baccumulate<2, XT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, tq, y); break;
