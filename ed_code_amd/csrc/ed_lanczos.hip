@@ -989,19 +989,34 @@ __global__ void __launch_bounds__(kRedThreads) k_bdot(int64_t n, int M, const XT
     }
 }
 
-// out[col] = compensated sum of partial[col * g + b] over b in order
-__global__ void k_colsum(const cd2* __restrict__ partial, int g, int ncols, cd2* __restrict__ out) {
-    int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncols) return;
+// out[col] = compensated sum of partial[col * g + b], one wave per column:
+// lane l sums b = l, l + 64, ... in order (Kahan), then lane 0 sums the 64
+// lane totals in lane order (Kahan) -- a fixed order, so the result does not
+// depend on scheduling.
+__global__ void __launch_bounds__(64) k_colsum(const cd2* __restrict__ partial, int g, cd2* __restrict__ out) {
+    __shared__ double red[64][2];
+    const int col = blockIdx.x, lane = threadIdx.x;
     double sr = 0, cr = 0, si = 0, ci = 0;
-    for (int b = 0; b < g; b++) {
+    for (int b = lane; b < g; b += 64) {
         const cd2 p = partial[(int64_t)col * g + b];
         double yr = p.re - cr, tr = sr + yr;
         cr = (tr - sr) - yr; sr = tr;
         double yi = p.im - ci, tj = si + yi;
         ci = (tj - si) - yi; si = tj;
     }
-    out[col] = cd2{sr, si};
+    red[lane][0] = sr - cr;
+    red[lane][1] = si - ci;
+    __syncthreads();
+    if (lane == 0) {
+        sr = 0; cr = 0; si = 0; ci = 0;
+        for (int l = 0; l < 64; l++) {
+            double yr = red[l][0] - cr, tr = sr + yr;
+            cr = (tr - sr) - yr; sr = tr;
+            double yi = red[l][1] - ci, tj = si + yi;
+            ci = (tj - si) - yi; si = tj;
+        }
+        out[col] = cd2{sr, si};
+    }
 }
 
 // y[e] = a[m] * y[e] - sum_k c[k*M + m] * X_k[e]
@@ -1116,7 +1131,7 @@ int bdot(BCtx& c, int mv, const XT* X, int64_t ldx, const YT* y, std::vector<std
             q += step;
         }
         const int ncols = mm * c.M;
-        hipLaunchKernelGGL(k_colsum, dim3((ncols + 63) / 64), dim3(64), 0, c.st, c.d_partial, (int)c.g, ncols, c.d_sum);
+        hipLaunchKernelGGL(k_colsum, dim3(ncols), dim3(64), 0, c.st, c.d_partial, (int)c.g, c.d_sum);
         HIPCHK(hipGetLastError());
         h.resize(ncols);
         HIPCHK(hipMemcpyAsync(h.data(), c.d_sum, ncols * sizeof(cd2), hipMemcpyDeviceToHost, c.st));
