@@ -797,6 +797,16 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     if (D == 0) return ED_OK;
     const bool rows2 = op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus));
     const bool img = !rows2 && torus_uses_image(op);
+    // a gather buffer is allocated by the first H*v that needs it, which
+    // therefore cannot be part of a stream capture
+    auto may_alloc = [&]() -> int {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return ed_fail(ED_EINVAL, "the first H*v of a precision and kernel path allocates its gather buffer: "
+                                      "run it before capturing");
+        return ED_OK;
+    };
     VT* u = nullptr;
     if (img) {
         // the compiled rotation kernels gather from the image; its zero
@@ -804,6 +814,8 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
         VT*& im = std::is_same<VT, cd2>::value ? (VT*&)op->d_img128 : (VT*&)op->d_img64;
         const uint64_t ne = (uint64_t)(B->img_nslot + 1) << B->img_cb;
         if (!im) {
+            int rc = may_alloc();
+            if (rc != ED_OK) return rc;
             HIPCHK(hipMalloc((void**)&im, ne * sizeof(VT)));
             hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, im);
         }
@@ -812,14 +824,19 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
         u = im;
     }
     bool wimg = false;
-    if (rows2 && B->wimg_cb >= 0) {
+    bool& wimg_off = op->wimg_off[std::is_same<VT, cd2>::value ? 0 : 1];
+    if (rows2 && B->wimg_cb >= 0 && !wimg_off) {
         // the two-row kernels gather from the wide image when it fits the
-        // free device memory, else from u by row through the directory
+        // free device memory, else (for the operator's lifetime) from u by
+        // row through the directory
         VT*& im = std::is_same<VT, cd2>::value ? (VT*&)op->d_wimg128 : (VT*&)op->d_wimg64;
         const uint64_t ne = (B->wimg_nslot + 1) << B->wimg_cb;
         if (!im) {
+            int rc = may_alloc();
+            if (rc != ED_OK) return rc;
             if (hipMalloc((void**)&im, ne * sizeof(VT)) != hipSuccess) {
                 im = nullptr;
+                wimg_off = true;
                 (void)hipGetLastError();
             } else {
                 hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, im);
@@ -834,7 +851,11 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     }
     if (!img && !wimg) {
         VT*& uu = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
-        if (!uu) HIPCHK(hipMalloc((void**)&uu, D * sizeof(VT)));
+        if (!uu) {
+            int rc = may_alloc();
+            if (rc != ED_OK) return rc;
+            HIPCHK(hipMalloc((void**)&uu, D * sizeof(VT)));
+        }
         hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, uu);
         u = uu;
     }

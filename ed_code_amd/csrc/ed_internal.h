@@ -110,6 +110,7 @@ struct ed_op_s {
     void* d_img128 = nullptr; // complex128
     void* d_wimg64 = nullptr; // prescaled wide gather image (two-row 64-bit kernels), complex64
     void* d_wimg128 = nullptr;
+    bool wimg_off[2] = {false, false};   // wide image did not fit (per precision: 0 complex128, 1 complex64)
     std::vector<edc::Bond> bonds;
 };
 
