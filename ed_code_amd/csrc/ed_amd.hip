@@ -426,6 +426,13 @@ int basis_build_device(ed_basis_s* B) {
     B->ngroups = ngroups;
     uint32_t* words = nullptr;
     uint64_t *cnt = nullptr, *base = nullptr;
+    void* tmp = nullptr;
+    // the build's temporaries are freed on every exit path
+    struct Temps {
+        void** p[4];
+        void release() { for (void** q : p) if (*q) { dfree(*q); *q = nullptr; } }
+        ~Temps() { release(); }
+    } temps{{(void**)&words, (void**)&cnt, (void**)&base, &tmp}};
     HIPCHK(hipMalloc(&words, nwords * sizeof(uint32_t)));
     HIPCHK(hipMalloc(&cnt, (ngroups + 1) * sizeof(uint64_t)));
     HIPCHK(hipMalloc(&base, (ngroups + 1) * sizeof(uint64_t)));
@@ -441,13 +448,11 @@ int basis_build_device(ed_basis_s* B) {
     // the group count exceeds 2^31 at N=40: the scan takes a 64-bit length
     size_t tmp_bytes = 0;
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
-    void* tmp = nullptr;
     HIPCHK(hipMalloc(&tmp, tmp_bytes + 16));
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
     uint64_t D = 0;
     HIPCHK(hipMemcpy(&D, base + ngroups, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (D > 0xFFFFFFFFull) {
-        dfree(tmp); dfree(words); dfree(cnt); dfree(base);
         return ed_fail(ED_EINVAL, "sector dimension %llu exceeds 2^32", (unsigned long long)D);
     }
     B->D = (int64_t)D;
@@ -460,7 +465,7 @@ int basis_build_device(ed_basis_s* B) {
         hipLaunchKernelGGL(k_compact<uint64_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
-    dfree(tmp); dfree(words); dfree(cnt); dfree(base);
+    temps.release();   // before the image layout: at N=40 the temporaries hold ~50 GB
     return basis_image_build(B);
 }
 
