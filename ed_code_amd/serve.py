@@ -29,6 +29,7 @@ import argparse
 import collections
 import json
 import math
+import sys
 import threading
 import time
 from dataclasses import dataclass
@@ -186,6 +187,8 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
         r.evals = [float(x) for x in ev[m]]
         r.done = now
         r.batch = len(reqs)
+    print(f"[serve] answered {[r.rid for r in reqs]} {reqs[0].lattice} D={basis.D} batch {len(reqs)}",
+          file=sys.stderr, flush=True)
 
 
 class Server:
