@@ -16,6 +16,9 @@ requests:
   trip per step);
 * HBM admission -- a job starts only when its footprint (basis, Krylov
   vectors, work vectors) fits next to the running jobs under the budget.
+* Exclusive large jobs -- a job on a sector of at least --exclusive-rows rows
+  (the 32-site torus) runs with the GPU to itself: its H*v already fills the
+  device, and sharing only stretches its latency.
 
 Latency is completion time minus arrival time.  The baseline is the plain
 loop: first come first served, one request at a time, basis built per
@@ -193,8 +196,12 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
 
 class Server:
     def __init__(self, workers: int = 4, hbm_budget: int = 200 * GiB, cache_budget: int = 24 * GiB,
-                 max_batch: int = 8, iterations: int = 300):
+                 max_batch: int = 8, iterations: int = 300, exclusive_rows: int = 1 << 24):
         self.workers = workers
+        # a job of at least this many rows runs with the GPU to itself (its
+        # H*v saturates the device; sharing only stretches its latency)
+        self.exclusive_rows = exclusive_rows
+        self.exclusive = False
         self.budget = hbm_budget
         self.max_batch = max_batch
         self.iterations = iterations
@@ -214,6 +221,8 @@ class Server:
             with self.cv:
                 self.in_use -= fp
                 self.running -= 1
+                if self.running == 0:
+                    self.exclusive = False
                 self.cv.notify_all()
 
     def run(self, requests: List[Request]) -> Tuple[float, float]:
@@ -230,7 +239,7 @@ class Server:
                 while future and future[0].arrival_abs <= now:
                     pending.append(future.pop(0))
                 started = False
-                while pending and self.running < self.workers:
+                while pending and self.running < self.workers and not self.exclusive:
                     head = pending[0]
                     group = [p for p in pending if batch_key(p) == batch_key(head)]
                     M = 8 if len(group) >= 8 else 4 if len(group) >= 4 else 2 if len(group) >= 2 else 1
@@ -238,12 +247,14 @@ class Server:
                     group = group[:M]
                     rows = estimate_rows(head.lattice, sector_key(head)[1])
                     fp = footprint(rows, M, self.iterations, head.nev)
-                    if self.in_use + fp > self.budget and self.running:
+                    big = rows >= self.exclusive_rows
+                    if (self.in_use + fp > self.budget or big) and self.running:
                         break
                     for g in group:
                         pending.remove(g)
                     self.in_use += fp
                     self.running += 1
+                    self.exclusive = big
                     th = threading.Thread(target=self._job, args=(group, fp), daemon=True)
                     th.start()
                     threads.append(th)
@@ -293,16 +304,19 @@ def main(argv=None):
     ap.add_argument("--iterations", type=int, default=300)
     ap.add_argument("--k-zero", action="store_true", help="every request at k=0 (coupling scans)")
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--exclusive-rows", type=int, default=1 << 24,
+                    help="jobs on sectors of at least this many rows run alone (0: never)")
     a = ap.parse_args(argv)
     init(0)
     lats = a.lattices.split(",")
     reqs = make_requests(a.requests, a.rate, a.seed, lats, a.k_zero)
-    srv = Server(workers=a.workers, iterations=a.iterations)
+    srv = Server(workers=a.workers, iterations=a.iterations, exclusive_rows=a.exclusive_rows or (1 << 62))
     # warm-up outside the timed runs (GPU context, library load, first kernels)
     warm = Request(-1, "chain24", 0.0, 0, 0, 0.1, 1.0, 0.0, 1, 1e-10, 0.0)
     solve(srv.cache.get(sector_key(warm)), [warm], a.iterations)
     t0, wall = srv.run(reqs)
-    out = dict(config=dict(requests=a.requests, rate=a.rate, lattices=lats, workers=a.workers, k_zero=a.k_zero),
+    out = dict(config=dict(requests=a.requests, rate=a.rate, lattices=lats, workers=a.workers, k_zero=a.k_zero,
+                           exclusive_rows=a.exclusive_rows),
                server=summarize(reqs, wall, t0), cache=dict(hits=srv.cache.hits, misses=srv.cache.misses),
                errors=srv.errors)
     served = {r.rid: r.evals for r in reqs}
