@@ -202,10 +202,14 @@ bool torus_shape_compiled(const Torus& T) {
     return false;
 }
 
+static size_t torus_c_smem_bytes(const ed_op_s* op);
+
 // The compiled-shape kernels gather from the sector's image (built with the
-// basis when it fits the LDS map and the 32-bit byte offsets of the loads).
+// basis when it fits the LDS map and the 32-bit byte offsets of the loads)
+// and keep their tables in at most 64 KiB of LDS; other operators run the
+// run-time-shape kernel.
 static bool compiled_fits(const ed_op_s* op) {
-    return torus_shape_compiled(op->torus) && op->b->img_cb >= 0;
+    return torus_shape_compiled(op->torus) && op->b->img_cb >= 0 && torus_c_smem_bytes(op) <= 64 * 1024;
 }
 
 // Whether the next H*v of this operator runs a compiled-shape kernel, whose
