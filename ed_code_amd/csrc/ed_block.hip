@@ -118,10 +118,8 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
     for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
     for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
     __syncthreads();
-    // XCD-contiguous row ranges (see the single-instance kernel)
-    const unsigned nb = gridDim.x, qq = nb / 8, rem = nb % 8;
-    const unsigned xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const unsigned lb = xcd * qq + (xcd < rem ? xcd : rem) + slot;
+    // XCD tiles of consecutive rows, the 8 XCDs on adjacent tiles
+    const unsigned lb = xcd_tile_block();
     const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
     if (i >= D) return;
     const uint64_t rw = rows[i];

@@ -101,10 +101,8 @@ __global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __r
     for (int q = threadIdx.x; q < 256; q += blockDim.x) rhi[q] = grhi[q];
     for (int q = threadIdx.x; q < rlo_n; q += blockDim.x) rlo[q] = grlo[q];
     __syncthreads();
-    // XCD-contiguous row ranges (see the 32-bit rotation kernel)
-    const unsigned nb = gridDim.x, qq = nb / 8, rem = nb % 8;
-    const unsigned xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const unsigned lb = xcd * qq + (xcd < rem ? xcd : rem) + slot;
+    // XCD tiles of consecutive rows, the 8 XCDs on adjacent tiles
+    const unsigned lb = xcd_tile_block();
     const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
     if (i >= D) return;
     const uint64_t rw = rows[i];

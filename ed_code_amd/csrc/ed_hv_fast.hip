@@ -85,12 +85,8 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
     for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
     __syncthreads();
-    // Workgroups are dispatched round-robin over the 8 XCDs; remap so that
-    // each XCD sweeps one contiguous range of rows and its L2 serves the
-    // near-diagonal gathers of neighbouring rows.
-    const unsigned nb = gridDim.x, q = nb / 8, rem = nb % 8;
-    const unsigned xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const unsigned lb = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    // XCD tiles of consecutive rows, the 8 XCDs on adjacent tiles
+    const unsigned lb = xcd_tile_block();
     int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
     if (i >= D) return;
     uint64_t rw = rows[i];

@@ -11,6 +11,21 @@
 
 int ed_fail(int code, const char* fmt, ...);
 
+// Logical block index of a row kernel's workgroup.  Workgroups are dispatched
+// round-robin over the 8 XCDs; this mapping hands each XCD a sequence of
+// tiles of kXcdTileRows rows (tile t to XCD t mod 8), so that the 8 XCDs
+// sweep adjacent tiles at the same time and their gathers meet in the shared
+// Infinity Cache, while each XCD's L2 still sees consecutive rows.  The tail
+// that does not fill a round of 8 tiles keeps the dispatch order.  A
+// bijection of [0, gridDim.x) for any grid.
+constexpr unsigned kXcdTileRows = 32768;
+__device__ inline unsigned xcd_tile_block() {
+    const unsigned nb = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const unsigned T = blockDim.x >= kXcdTileRows ? 1u : kXcdTileRows / blockDim.x;
+    const unsigned full = nb / (8 * T);
+    return blockIdx.x < full * 8 * T ? ((slot / T) * 8 + xcd) * T + (slot % T) : blockIdx.x;
+}
+
 inline void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
