@@ -742,8 +742,9 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 7) return ed_fail(ED_EINVAL, "path %d not in 0..7", path);
-    if (path == 6 && !op->use_rows2) return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
+    if (path < 0 || path > 8) return ed_fail(ED_EINVAL, "path %d not in 0..8", path);
+    if ((path == 6 || path == 8) && !op->use_rows2)
+        return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
     if (((path >= 2 && path <= 5) || path == 7) && !op->use_torus)
         return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
     if ((path == 2 || path == 4 || path == 5 || path == 7) && !torus_shape_compiled(op->torus))
@@ -760,8 +761,8 @@ extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
 extern "C" int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block) {
     if (!op || !path || !variant || !block) return ed_fail(ED_EINVAL, "null argument");
     if (dtype != ED_C128 && dtype != ED_C64) return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
-    if (op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus))) {
-        *path = 6;
+    if (op->use_rows2 && (op->tune_path == 6 || op->tune_path == 8 || (op->tune_path == 0 && !op->use_torus))) {
+        *path = op->tune_path == 8 ? 8 : 6;
         *variant = -1;
         *block = 256;
         return ED_OK;
@@ -800,7 +801,8 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
     if (D == 0) return ED_OK;
-    const bool rows2 = op->use_rows2 && (op->tune_path == 6 || (op->tune_path == 0 && !op->use_torus));
+    const bool rows2 =
+        op->use_rows2 && (op->tune_path == 6 || op->tune_path == 8 || (op->tune_path == 0 && !op->use_torus));
     const bool img = !rows2 && torus_uses_image(op);
     // a gather buffer is allocated by the first H*v that needs it, which
     // therefore cannot be part of a stream capture
@@ -830,7 +832,7 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     }
     bool wimg = false;
     bool& wimg_off = op->wimg_off[std::is_same<VT, cd2>::value ? 0 : 1];
-    if (rows2 && B->wimg_cb >= 0 && !wimg_off) {
+    if (rows2 && B->wimg_cb >= 0 && !wimg_off && op->tune_path != 8) {
         // the two-row kernels gather from the wide image when it fits the
         // free device memory, else (for the operator's lifetime) from u by
         // row through the directory
@@ -870,7 +872,7 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 256)), blk(256);
-    if (op->use_torus && op->tune_path != 1 && op->tune_path != 6) {
+    if (op->use_torus && op->tune_path != 1 && op->tune_path != 6 && op->tune_path != 8) {
         int rc = torus_prepare<VT>(op, u, v, w, st);
         if (rc != ED_OK) return rc;
     }

@@ -211,7 +211,8 @@ static bool compiled_fits(const ed_op_s* op) {
 // Whether the next H*v of this operator runs a compiled-shape kernel, whose
 // gathered vector is the image rather than u by row.
 bool torus_uses_image(const ed_op_s* op) {
-    return op->use_torus && op->tune_path != 1 && op->tune_path != 3 && op->tune_path != 6 && compiled_fits(op);
+    return op->use_torus && op->tune_path != 1 && op->tune_path != 3 && op->tune_path != 6 && op->tune_path != 8 &&
+           compiled_fits(op);
 }
 
 size_t torus_smem_bytes(const ed_op_s* op) {

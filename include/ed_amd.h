@@ -112,14 +112,16 @@ void ed_op_destroy(ed_op op);
  * compiled in (variant K=4 elements in flight), 3 = rotation fast path with
  * the shape as run-time parameters, 4 = as 2 with K=2 at 8 waves per SIMD,
  * 5 = as 2 with K=8, 6 = two-row 64-bit kernel (lattices relabelled to two
- * rows of W sites), 7 = as 2 with K=4 and the batches software-pipelined.
+ * rows of W sites), 7 = as 2 with K=4 and the batches software-pipelined,
+ * 8 = as 6 gathering u by row through the directory (the layout the kernel
+ * falls back to when its wide gather image does not fit in device memory).
  * Paths 2-5 and 7 need a lattice whose translations are bit rotations, N <= 32
  * and no bowtie terms; 2, 4, 5 and 7 also a built-in shape.
  * block = threads per block (64..512, multiple of 64; 0 = default, tuned
  * under path 0).  Resets any earlier tuning. */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
-/* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3 or
- * 6 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
+/* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3, 6
+ * or 8 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
  * pipelined, -1 = not yet tuned or not applicable) and threads per block. */
 int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the

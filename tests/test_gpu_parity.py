@@ -253,7 +253,15 @@ def test_gpu_two_row_64bit_path(gpu_ready, name, P1, P2, n, k):
     rng = np.random.default_rng(21)
     v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
     wo = o.hv(v)
-    assert np.linalg.norm(_hv(op, v, torch.complex128) - wo) <= 1e-12 * np.linalg.norm(wo)
+    w6 = _hv(op, v, torch.complex128)
+    assert np.linalg.norm(w6 - wo) <= 1e-12 * np.linalg.norm(wo)
+    assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo)
+    # path 8: the same kernel gathering u by row through the directory (the
+    # layout it falls back to when the wide gather image does not fit)
+    op.tune(8)
+    assert op.tuning("c128")[0] == 8
+    w8 = _hv(op, v, torch.complex128)
+    assert np.array_equal(w8, w6)
     assert np.linalg.norm(_hv(op, v, torch.complex64) - wo) <= 1e-6 * np.linalg.norm(wo)
 
 
