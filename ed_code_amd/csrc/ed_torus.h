@@ -306,10 +306,11 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
         }
         return rep;
     }
-    uint32_t mins[TC::L1];
+    uint32_t mins[TC::L1], fr[TC::L1];
 #pragma unroll
     for (int a = 0; a < TC::L1; a++) {
         uint32_t sa = field_rot_rt<TC>(t, a);   // a is a constant after unrolling
+        fr[a] = sa;
         uint32_t hi, lo;
         cat_top<TC>(sa, hi, lo);
         uint32_t m = 0xffffffffu;
@@ -324,20 +325,28 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
     }
     const uint32_t rep = best >> S;
     if (TRACK) {
+        // the reference's first minimum: the lowest field rotation bx whose
+        // row holds the minimum, then the lowest step y of the walk's
+        // cumulative T2 count, y = (b + bx) mod L2 for the word rotation b
         int bx = 0;
+        uint32_t sa = fr[0];
+#pragma unroll
         for (int a = TC::L1 - 1; a >= 0; a--)
-            if ((mins[a] >> S) == rep) bx = a;
-        uint32_t sa = field_rot_rt<TC>(t, bx);
+            if ((mins[a] >> S) == rep) { bx = a; sa = fr[a]; }
         uint32_t hi, lo;
-        cat_c<TC>(sa, hi, lo);
-        int by = 0;
-        for (int y = TC::L2 - 1; y >= 0; y--) {
-            int b = ((y - bx) % TC::L2 + TC::L2) % TC::L2;
-            if (rot_c<TC>(hi, lo, b * TC::W) == rep) by = y;
+        cat_top<TC>(sa, hi, lo);
+        const int bxm = bx % TC::L2;
+        int by = TC::L2;
+#pragma unroll
+        for (int b = 0; b < TC::L2; b++) {
+            int y = b + bxm;
+            y -= y >= TC::L2 ? TC::L2 : 0;
+            if ((rot_top<TC>(hi, lo, b * TC::W) >> S) == rep && y < by) by = y;
         }
-        int ty = bx * (TC::L2 - 1) + by;
-        int lx = (TC::L1 - bx) % TC::L1;
-        int ly = ((TC::L2 - ty) % TC::L2 + TC::L2) % TC::L2;
+        // lx = -bx mod L1; ly = -(bx (L2 - 1) + by) mod L2 = (bx - by) mod L2
+        const int lx = bx == 0 ? 0 : TC::L1 - bx;
+        const int d = bxm - by;
+        const int ly = d < 0 ? d + TC::L2 : d;
         code = lx * TC::L2 + ly;
     } else {
         code = 0;

@@ -173,6 +173,24 @@ def test_compiled_shape_search_matches_generic_walk(model, n, k):
     else:   # rings: period-2 and period-4 patterns
         for t in (sum(1 << q for q in range(0, model.n_sites, 2)), sum(3 << q for q in range(0, model.n_sites, 4))):
             assert h.torus_rep_c(t) == h.generic_rep(t)
+    # states invariant under a subgroup (ties between several minimising
+    # translations): the union of the images of a random pattern under the
+    # powers of one translation T1^p T2^q
+    T1, T2 = np.asarray(model.T1), np.asarray(model.T2)
+    for _ in range(300):
+        p, q = int(rng.integers(0, L1)), int(rng.integers(0, max(L2, 1)))
+        g = np.arange(model.n_sites)
+        for _ in range(p):
+            g = T1[g]
+        for _ in range(q):
+            g = T2[g]
+        sites = set(int(x) for x in rng.choice(model.n_sites, size=int(rng.integers(1, 5)), replace=False))
+        orbit, cur = set(sites), set(sites)
+        for _ in range(2 * model.n_sites):
+            cur = set(int(g[x]) for x in cur)
+            orbit |= cur
+        t = sum(1 << x for x in orbit)
+        assert h.torus_rep_c(t) == h.generic_rep(t), (p, q, sorted(orbit))
 
 
 BATCHED = [(L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 3)), (L.square(4, 4, J2=0.5), 8, (0, 0)),
