@@ -116,7 +116,8 @@ void ed_op_destroy(ed_op op);
  * 8 = as 6 gathering u by row through the directory (the layout the kernel
  * falls back to when its wide gather image does not fit in device memory).
  * Paths 2-5 and 7 need a lattice whose translations are bit rotations, N <= 32
- * and no bowtie terms; 2, 4, 5 and 7 also a built-in shape.
+ * and no bowtie terms; 2, 4, 5 and 7 also a built-in shape; 6 and 8 a
+ * compiled two-row shape and no bowtie terms.
  * block = threads per block (64..512, multiple of 64; 0 = default, tuned
  * under path 0).  Resets any earlier tuning. */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
