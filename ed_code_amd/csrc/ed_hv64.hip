@@ -76,7 +76,7 @@ int rows2_op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vecto
 
 namespace {
 
-constexpr int kRows2K = 4;   // elements in flight per lane
+constexpr int kRows2K = 8;   // elements in flight per lane
 
 template <class TC, bool TRIV, bool WIMG, typename VT>
 __global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __restrict__ rows, int nfam,
