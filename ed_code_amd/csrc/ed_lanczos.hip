@@ -159,6 +159,12 @@ struct Ctx {
 
 unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D + kRedThreads - 1) / kRedThreads); }
 
+// One element per thread for the multi-vector updates: a wave loads its
+// elements, writes y and ends.  A grid-stride loop would make every wave wait
+// for its own store (stores and loads share one counter) before the loads of
+// its next element; the results are the same either way.
+unsigned grid_stream(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads); }
+
 
 template <typename XT, typename YT>
 void launch_multidot(Ctx& c, int mm, const XT* X, int64_t ldx, const YT* y, unsigned g, int kofs) {
@@ -206,7 +212,7 @@ int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<s
 // y = a*y - sum_k coef[k] X_k
 template <typename XT, typename YT>
 int multiaxpy(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<std::complex<double>>& coef, double a, YT* y) {
-    unsigned g = grid_for(c.D);
+    unsigned g = grid_stream(c.D);
     if (m == 0) {
         if (a != 1.0)
             hipLaunchKernelGGL((k_multiaxpy<1, YT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, y, 0, c.d_coef, a, y);
@@ -1165,11 +1171,11 @@ int baxpy(BCtx& c, int mv, const XT* X, int64_t ldx, const std::vector<std::comp
         const cd2* cq = c.d_coef + (size_t)q * c.M;
         double* aq = q == 0 ? da : nullptr;
         switch (step) {
-            case 16: hipLaunchKernelGGL((k_baxpy<16, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
-            case 8: hipLaunchKernelGGL((k_baxpy<8, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
-            case 4: hipLaunchKernelGGL((k_baxpy<4, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
-            case 2: hipLaunchKernelGGL((k_baxpy<2, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
-            default: hipLaunchKernelGGL((k_baxpy<1, XT, YT>), dim3(c.g), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
+            case 16: hipLaunchKernelGGL((k_baxpy<16, XT, YT>), dim3(grid_stream(c.n)), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
+            case 8: hipLaunchKernelGGL((k_baxpy<8, XT, YT>), dim3(grid_stream(c.n)), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
+            case 4: hipLaunchKernelGGL((k_baxpy<4, XT, YT>), dim3(grid_stream(c.n)), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
+            case 2: hipLaunchKernelGGL((k_baxpy<2, XT, YT>), dim3(grid_stream(c.n)), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
+            default: hipLaunchKernelGGL((k_baxpy<1, XT, YT>), dim3(grid_stream(c.n)), dim3(kRedThreads), 0, c.st, c.n, c.M, Xq, ldx, cq, aq, y); break;
         }
         HIPCHK(hipGetLastError());
         q += step;
