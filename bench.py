@@ -140,6 +140,32 @@ def measure_valu(workload, dtype, kernel_key, tune=None):
     return statistics.mean(vals) if vals else None
 
 
+def gather_ceiling(n_elem, elem_bytes, n_gather, group=1, threads=512, inflight=8):
+    """Random-gather rate (elements/s) of ed_gather_calibrate: n_gather loads
+    of elem_bytes from random 128-byte lines of an n_elem-element buffer."""
+    import ctypes as C
+    from ed_code_amd._lib import check, lib
+    ms = C.c_float()
+    check(lib().ed_gather_calibrate(int(n_elem), elem_bytes, int(n_gather), group, threads, inflight, C.byref(ms)))
+    return n_gather / (ms.value * 1e-3), ms.value
+
+
+def gather_sweep():
+    import ed_code_amd as ed
+    ed.init(0)
+    n_gather = 1_241_024_576          # the 32-site torus sector's off-diagonal elements
+    for mb in (8, 32, 150, 405, 4096):
+        for eb in (8, 16):
+            for group in (1, 2, 4, 16):
+                if group * eb > 128:
+                    continue
+                for inflight in (4, 8):
+                    rate, ms = gather_ceiling(mb * 2**20 // eb, eb, n_gather, group, 512, inflight)
+                    print(json.dumps({"buffer_MiB": mb, "elem_bytes": eb, "lanes_per_line": group,
+                                      "inflight": inflight, "n_gather": n_gather, "ms": ms,
+                                      "gathers_per_s": rate, "line_fills_per_s": rate / group}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,7 +184,13 @@ def main():
     ap.add_argument("--tune", default="", help="fixed kernel 'path,block' (ed_op_tune) instead of the autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="time a plain device copy of 2 GiB (reference for rocprof byte counters)")
+    ap.add_argument("--gather-calibration", action="store_true",
+                    help="sweep the random-gather ceiling (ed_gather_calibrate) over buffer sizes, element sizes "
+                         "and line sharing, one JSON line per point")
     args = ap.parse_args()
+    if args.gather_calibration:
+        gather_sweep()
+        return
     if args.copy_calibration:
         import torch
         n = 1 << 31

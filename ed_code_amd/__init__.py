@@ -232,10 +232,28 @@ class Operator:
         or host numpy complex128 arrays (copied through the device)."""
         if isinstance(v, np.ndarray):
             v = np.ascontiguousarray(v, np.complex128)
+            if v.ndim != 1 or v.size != self.D:
+                raise ValueError(f"v must hold D = {self.D} values, got shape {v.shape}")
             if w is None:
                 w = np.zeros(self.D, np.complex128)
+            elif (not isinstance(w, np.ndarray) or w.dtype != np.complex128 or w.ndim != 1 or w.size != self.D
+                  or not w.flags.c_contiguous or not w.flags.writeable):
+                raise ValueError("w must be a writeable C-contiguous complex128 array of length D")
+            if self.D and np.shares_memory(v, w):
+                raise ValueError("v and w must not alias")
             check(lib().ed_hv_host(self.h, v.ctypes.data, w.ctypes.data))
             return w
+        dt = self._check_device(v, w)
+        if w is None:
+            w = v.new_empty(v.shape)
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().ed_hv(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st)))
+        return w
+
+    def _check_device(self, v, w):
+        """dtype code of a device H*v: v (and w when given) contiguous CUDA
+        tensors of D complex64 or complex128 values, w distinct from v."""
         import torch
         if v.dtype == torch.complex128:
             dt = ED_C128
@@ -245,13 +263,12 @@ class Operator:
             raise TypeError(f"unsupported dtype {v.dtype}")
         if not v.is_cuda or not v.is_contiguous() or v.numel() != self.D:
             raise ValueError("v must be a contiguous device tensor of length D")
-        if w is None:
-            w = torch.empty_like(v)
-        if w.dtype != v.dtype or not w.is_contiguous() or w.numel() != self.D:
-            raise ValueError("w must match v")
-        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
-        check(lib().ed_hv(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st)))
-        return w
+        if w is not None:
+            if w.dtype != v.dtype or not w.is_cuda or not w.is_contiguous() or w.numel() != self.D:
+                raise ValueError("w must match v")
+            if self.D and w.data_ptr() == v.data_ptr():
+                raise ValueError("v and w must not alias")
+        return dt
 
     def tune(self, path: int = 0, block: int = 0):
         """Kernel choice: path 0 auto (tuned on the first large H*v), 1 generic
@@ -271,7 +288,9 @@ class Operator:
     def hv_profile(self, v, w, stream=None) -> float:
         """ed_hv with HIP events around the row kernel; returns its ms."""
         import torch
-        dt = ED_C128 if v.dtype == torch.complex128 else ED_C64
+        if w is None:
+            raise ValueError("hv_profile needs an output tensor w")
+        dt = self._check_device(v, w)
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         ms = C.c_float()
         check(lib().ed_hv_profile(self.h, dt, C.c_void_p(v.data_ptr()), C.c_void_p(w.data_ptr()), C.c_void_p(st),

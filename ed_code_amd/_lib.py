@@ -78,6 +78,8 @@ SIGNATURES = {
     "ed_block_fast": (C.c_int, [_VP]),
     "ed_block_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
     "ed_block_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, C.POINTER(LanczosStats)]),
+    "ed_gather_calibrate": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_float)]),
 }
 
 _lib = None

@@ -302,24 +302,42 @@ __global__ void __launch_bounds__(256) k_hv_rows(int64_t D, const uint64_t* rows
     store_c<VT>(w, i, nrm * ar + ez * vi.re, nrm * ai + ez * vi.im);
 }
 
+// Distinct targets of a row are collected in a per-lane list of at most
+// kMaxRowTargets; a row with more (bowtie terms give hundreds) is counted
+// exactly by extracting its distinct targets in ascending order, one pass over
+// the row per value.
+constexpr int kMaxRowTargets = 384;
+
 template <typename S, bool TRIV>
 __global__ void k_count_targets(int64_t D, const uint64_t* rows, Geo G, RankInfo R, TablesView tv, TermsView T,
                                 unsigned long long* distinct, unsigned long long* total) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long dcount = 0, tcount = 0;
     if (i < D) {
-        constexpr int MAXJ = 384;
-        int64_t js[MAXJ];
+        int64_t js[kMaxRowTargets];
         int nj = 0;
-        bool overflow = false;
+        bool over = false;
         S s = (S)(rows[i] & kStateMask);
         visit_row<S, TRIV>(s, G, R, tv, T, [&](int64_t j, double, double) {
             tcount++;
             for (int q = 0; q < nj; q++)
                 if (js[q] == j) return;
-            if (nj < MAXJ) js[nj++] = j; else overflow = true;
+            if (nj < kMaxRowTargets) js[nj++] = j; else over = true;
         });
-        dcount = overflow ? 0 : (unsigned long long)nj;
+        dcount = (unsigned long long)nj;
+        if (over) {
+            dcount = 0;
+            int64_t cur = -1;
+            while (true) {
+                int64_t nxt = INT64_MAX;
+                visit_row<S, TRIV>(s, G, R, tv, T, [&](int64_t j, double, double) {
+                    if (j > cur && j < nxt) nxt = j;
+                });
+                if (nxt == INT64_MAX) break;
+                dcount++;
+                cur = nxt;
+            }
+        }
     }
     atomicAdd(distinct, dcount);
     atomicAdd(total, tcount);
@@ -554,19 +572,27 @@ extern "C" int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* 
     const int64_t D = b->D;
     if (D == 0) return ED_OK;
     if (reps) {
-        std::vector<uint64_t> rw(D);
-        HIPCHK(hipMemcpy(rw.data(), b->d_rows, D * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < D; i++) reps[i] = (int64_t)(rw[i] & kStateMask);
+        // the row words land in the caller's array and are masked in place
+        // (no host staging copy: at 40 sites the array is 27.6 GB)
+        HIPCHK(hipMemcpy(reps, b->d_rows, D * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < D; i++) reps[i] = (int64_t)((uint64_t)reps[i] & kStateMask);
     }
     if (norm || norm_inv) {
         double *dn = nullptr, *dni = nullptr;
-        HIPCHK(hipMalloc(&dn, D * sizeof(double)));
-        HIPCHK(hipMalloc(&dni, D * sizeof(double)));
-        hipLaunchKernelGGL(k_norms, dim3(nblocks(D, 256)), dim3(256), 0, 0, b->d_rows, D, b->d_ntab, b->d_nitab, dn, dni);
-        HIPCHK(hipGetLastError());
-        if (norm) HIPCHK(hipMemcpy(norm, dn, D * sizeof(double), hipMemcpyDeviceToHost));
-        if (norm_inv) HIPCHK(hipMemcpy(norm_inv, dni, D * sizeof(double), hipMemcpyDeviceToHost));
-        dfree(dn); dfree(dni);
+        auto body = [&]() -> int {
+            HIPCHK(hipMalloc(&dn, D * sizeof(double)));
+            HIPCHK(hipMalloc(&dni, D * sizeof(double)));
+            hipLaunchKernelGGL(k_norms, dim3(nblocks(D, 256)), dim3(256), 0, 0, b->d_rows, D, b->d_ntab, b->d_nitab,
+                               dn, dni);
+            HIPCHK(hipGetLastError());
+            if (norm) HIPCHK(hipMemcpy(norm, dn, D * sizeof(double), hipMemcpyDeviceToHost));
+            if (norm_inv) HIPCHK(hipMemcpy(norm_inv, dni, D * sizeof(double), hipMemcpyDeviceToHost));
+            return ED_OK;
+        };
+        const int rc = body();
+        dfree(dn);
+        dfree(dni);
+        if (rc != ED_OK) return rc;
     }
     return ED_OK;
 }
@@ -915,20 +941,25 @@ extern "C" int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, vo
     if (!op || !kernel_ms) return ed_fail(ED_EINVAL, "null argument");
     if (v == w && op->b->D) return ed_fail(ED_EINVAL, "v and w must not alias");
     hipStream_t st = (hipStream_t)stream;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    int rc;
-    if (dtype == ED_C128) rc = launch_hv<cd2>(op, (const cd2*)v, (cd2*)w, st, e0, e1);
-    else if (dtype == ED_C64) rc = launch_hv<cf2>(op, (const cf2*)v, (cf2*)w, st, e0, e1);
-    else rc = ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     *kernel_ms = 0.f;
-    if (rc == ED_OK && op->b->D) {
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(kernel_ms, e0, e1));
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    // every exit after the events exist destroys them
+    auto body = [&]() -> int {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        int rc;
+        if (dtype == ED_C128) rc = launch_hv<cd2>(op, (const cd2*)v, (cd2*)w, st, e0, e1);
+        else if (dtype == ED_C64) rc = launch_hv<cf2>(op, (const cf2*)v, (cf2*)w, st, e0, e1);
+        else rc = ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+        if (rc == ED_OK && op->b->D) {
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(kernel_ms, e0, e1));
+        }
+        return rc;
+    };
+    const int rc = body();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -936,16 +967,23 @@ extern "C" int ed_hv_host(ed_op op, const double* v, double* w) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
     const int64_t D = op->b->D;
     if (D == 0) return ED_OK;
+    if (!v || !w) return ed_fail(ED_EINVAL, "null argument");
+    if (v == w) return ed_fail(ED_EINVAL, "v and w must not alias");
     void *dv = nullptr, *dw = nullptr;
-    HIPCHK(hipMalloc(&dv, D * sizeof(cd2)));
-    HIPCHK(hipMalloc(&dw, D * sizeof(cd2)));
-    HIPCHK(hipMemcpy(dv, v, D * sizeof(cd2), hipMemcpyHostToDevice));
-    int rc = op_hv(op, ED_C128, dv, dw, 0);
-    if (rc == ED_OK) {
+    // every exit after the first allocation frees both buffers
+    auto body = [&]() -> int {
+        HIPCHK(hipMalloc(&dv, D * sizeof(cd2)));
+        HIPCHK(hipMalloc(&dw, D * sizeof(cd2)));
+        HIPCHK(hipMemcpy(dv, v, D * sizeof(cd2), hipMemcpyHostToDevice));
+        int rc = op_hv(op, ED_C128, dv, dw, 0);
+        if (rc != ED_OK) return rc;
         HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(w, dw, D * sizeof(cd2), hipMemcpyDeviceToHost));
-    }
-    dfree(dv); dfree(dw);
+        return ED_OK;
+    };
+    const int rc = body();
+    dfree(dv);
+    dfree(dw);
     return rc;
 }
 
@@ -1008,22 +1046,29 @@ extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) 
     const int64_t D = B->D;
     unsigned long long* d = nullptr;
     HIPCHK(hipMalloc(&d, 2 * sizeof(unsigned long long)));
-    HIPCHK(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
-    Geo G;
-    basis_geo(B, G);
-    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
-    TermsView T = op_terms(op);
-    if (D > 0) {
-        dim3 grid(nblocks(D, 128)), blk(128);
-        if (B->N <= 32)
-            hipLaunchKernelGGL((k_count_targets<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, d, d + 1);
-        else
-            hipLaunchKernelGGL((k_count_targets<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, d, d + 1);
-        HIPCHK(hipGetLastError());
-    }
-    unsigned long long h[2];
-    HIPCHK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+    unsigned long long h[2] = {0, 0};
+    auto run = [&]() -> int {
+        HIPCHK(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
+        Geo G;
+        basis_geo(B, G);
+        TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
+        TermsView T = op_terms(op);
+        if (D > 0) {
+            dim3 grid(nblocks(D, 128)), blk(128);
+            if (B->N <= 32)
+                hipLaunchKernelGGL((k_count_targets<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T,
+                                   d, d + 1);
+            else
+                hipLaunchKernelGGL((k_count_targets<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T,
+                                   d, d + 1);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+        return ED_OK;
+    };
+    const int rc = run();
     dfree(d);
+    if (rc != ED_OK) return rc;
     if (distinct) *distinct = (int64_t)h[0];
     if (total) *total = (int64_t)h[1];
     return ED_OK;

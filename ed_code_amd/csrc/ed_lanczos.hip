@@ -213,11 +213,7 @@ int multidot(Ctx& c, int m, const XT* X, int64_t ldx, const YT* y, std::vector<s
 template <typename XT, typename YT>
 int multiaxpy(Ctx& c, int m, const XT* X, int64_t ldx, const std::vector<std::complex<double>>& coef, double a, YT* y) {
     unsigned g = grid_stream(c.D);
-    if (m == 0) {
-        if (a != 1.0)
-            hipLaunchKernelGGL((k_multiaxpy<1, YT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, y, 0, c.d_coef, a, y);
-        return ED_OK;
-    }
+    if (m <= 0) return ed_fail(ED_EINVAL, "multiaxpy needs at least one vector (m = %d)", m);
     std::vector<cd2> cc(m);
     for (int k = 0; k < m; k++) cc[k] = cd2{coef[k].real(), coef[k].imag()};
     cd2* dc = c.d_coef_big;

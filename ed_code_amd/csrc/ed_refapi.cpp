@@ -236,12 +236,24 @@ void write_vectors(const std::string& path, bool unwrap_peigvec, int n_sec_ev, i
     }
 }
 
+// Krylov storage precision of the drop-in solvers: complex128 unless the
+// environment variable ED_PRECISION is "c64", which stores the Krylov vectors
+// in complex64 and keeps FP64 for alpha/beta, the reorthogonalisation
+// coefficients, the tridiagonal eigensolve and a final FP64 residual /
+// Rayleigh-quotient correction of every eigenvalue.
+int krylov_precision(std::ofstream& out) {
+    const char* e = std::getenv("ED_PRECISION");
+    if (!e || !*e || !std::strcmp(e, "c128")) return ED_C128;
+    if (!std::strcmp(e, "c64")) return ED_C64;
+    fatal(out, std::string("ED_PRECISION must be c64 or c128, got ") + e);
+}
+
 void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, double tol, int mode,
                  std::vector<cplx>& eigs, std::vector<std::vector<cplx>>& evecs, std::ofstream& out) {
     int nev = (int)std::min<int64_t>(n_ev, D);
     ed_lanczos_params p;
     std::memset(&p, 0, sizeof(p));
-    p.nev = nev; p.iterations = iterations; p.ncycles = ncycles; p.precision = ED_C128; p.mode = mode;
+    p.nev = nev; p.iterations = iterations; p.ncycles = ncycles; p.precision = krylov_precision(out); p.mode = mode;
     p.tol = tol; p.seed = 1;
     std::vector<double> ev(nev);
     std::vector<cplx> vecs((size_t)nev * D);
@@ -258,6 +270,12 @@ void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, dou
     out << "\n Lanczos SUMMARY \n" << std::endl;
     out << " Hv calls\t\t\t" << st.hv_calls << std::endl;
     out << " Hv total time \t\t\t" << st.hv_seconds / 60.0 << " minutes" << std::endl;
+    // FP64 residual ||H x - E x|| of each returned Ritz vector: the error
+    // bound of its eigenvalue (|E - E_exact| <= residual for Hermitian H)
+    out << " Krylov storage\t\t\t" << (p.precision == ED_C64 ? "complex64" : "complex128") << std::endl;
+    for (int k = 0; k < std::min(nev, 16); k++)
+        out << " FP64 residual " << k << "\t\t" << std::scientific << std::setprecision(3) << st.residual[k]
+            << std::defaultfloat << std::endl;
 }
 
 }  // namespace
@@ -574,29 +592,51 @@ void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>&, st
     Sector& S = sector(T1, T2, n_ones, px, py, outfile);
     if (S.D != (int64_t)pblock_states.size() || S.D != n_p) fatal(outfile, "ARPACK_sector: basis mismatch");
     ed_op op = op_for(S, terms_of(lambda, adj_list, Ch, ch_list, ChCh, chch_list), outfile);
-    // ARPACK_sector.cpp:213-217: n_ev eigenvalues, all of them when n_p <= n_ev + 1
+    // ARPACK_sector.cpp:213-217: n_ev eigenvalues; n_p <= n_ev + 1 takes the
+    // dense branch, which returns all n_p of them (:232-245)
     int it = n_ev > n_p - 1 ? (int)n_p : n_ev;
     std::vector<cplx> eigs;
     std::vector<std::vector<cplx>> vecs;
     if (n_p > n_ev + 1) {
         const int ncv = (int)std::min<int64_t>(n_p, std::max(4 * it, 2 * it + 40));
         run_lanczos(op, S.D, it, ncv, std::max(maxiter, 1), tol, ED_LZ_THICK, eigs, vecs, outfile);
-    } else {
-        std::vector<double> ev(S.D);
-        std::vector<cplx> dv((size_t)S.D * S.D);
-        check(ed_dense_eigh(op, ev.data(), (double*)dv.data()), outfile, "ed_dense_eigh");
-        for (int k = 0; k < it; k++) {
-            eigs.push_back(ev[k]);
-            vecs.emplace_back(dv.begin() + (size_t)k * S.D, dv.begin() + (size_t)(k + 1) * S.D);
+        // ARPACK returns the wanted Ritz values largest first; the caller prints
+        // them in reverse (ARPACK_sector.cpp:339-340), i.e. ascending.
+        pblock_eigs.assign(eigs.rbegin(), eigs.rend());
+        if (find_ev) {
+            pblock_eigvecs.resize(it, S.D);   // Evecs(i, j) = component j of vector i (ARPACK_sector.cpp:176-181)
+            for (int i = 0; i < it; i++)
+                for (int64_t j = 0; j < S.D; j++) pblock_eigvecs(i, j) = vecs[it - 1 - i][j];
         }
-    }
-    // ARPACK returns the wanted Ritz values largest first; the caller prints
-    // them in reverse (ARPACK_sector.cpp:339-340), i.e. ascending.
-    pblock_eigs.assign(eigs.rbegin(), eigs.rend());
-    if (find_ev) {
-        pblock_eigvecs.resize(it, S.D);   // Evecs(i, j) = component j of vector i (ARPACK_sector.cpp:176-181)
-        for (int i = 0; i < it; i++)
-            for (int64_t j = 0; j < S.D; j++) pblock_eigvecs(i, j) = vecs[it - 1 - i][j];
+    } else {
+        // Dense branch (ARPACK_sector.cpp:232-251): it = n_p, every eigenpair
+        // of the n_p x n_p sector matrix, the eigenvectors as the columns of
+        // pblock_eigvecs (zgeev's VR: element (c, k) = component c of vector
+        // k), the matrix printed to the log.  The reference fills this matrix
+        // with its legacy builder (zXXZ_Heisenberg,
+        // Heisenberg_Hamiltonian.cpp:92-200), whose diagonal omits J and which
+        // drops bond phases and bowtie terms; here it is the sector matrix of
+        // sHv, so that both branches diagonalise the same H.  zgeev leaves the
+        // order unspecified; the values are stored largest first, as the
+        // Arnoldi branch does, so that the caller prints them ascending.
+        outfile << " Here \n\n";
+        it = (int)n_p;
+        std::vector<double> ev(S.D);
+        std::vector<cplx> dv((size_t)S.D * S.D), H((size_t)S.D * S.D);
+        check(ed_dense_eigh(op, ev.data(), (double*)dv.data()), outfile, "ed_dense_eigh");
+        check(ed_make_h(op, (double*)H.data()), outfile, "ed_make_h");
+        pblock_eigs.assign(it, cplx(0, 0));
+        pblock_eigvecs.resize(n_p, n_p);
+        for (int k = 0; k < it; k++) {
+            const int src = it - 1 - k;
+            pblock_eigs[k] = ev[src];
+            for (int64_t c = 0; c < n_p; c++) pblock_eigvecs(c, k) = dv[(size_t)src * S.D + c];
+        }
+        for (int64_t ii = 0; ii < n_p; ii++) {
+            for (int64_t jj = 0; jj < n_p; jj++)
+                outfile << std::fixed << std::setprecision(5) << H[ii + jj * n_p] << "\t\t";
+            outfile << std::endl;
+        }
     }
     outfile << "\nEigenvalues\n";
     for (int kb = 0; kb < it; kb++) outfile << std::setprecision(10) << pblock_eigs[kb] << "\t";
@@ -605,9 +645,11 @@ void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>&, st
 void sector_eigs_eigvecs(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
                          std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list,
                          double& ChCh, std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,
-                         int& n_sec_ev, bool& find_ev, bool&, std::string&, double& tol, int& maxiter,
-                         std::ofstream& outfile) {
-    int n_ones = (int)(cSz + n_spins / 2 + 1.0e-6);
+                         int& n_sec_ev, bool& find_ev, bool& unwrap_peigvec, std::string& filepath_EV, double& tol,
+                         int& maxiter, std::ofstream& outfile) {
+    // ARPACK_sector.cpp:284-287: n_spins/2 plus one for every step of 1 below cSz
+    int n_ones = n_spins / 2;
+    for (double i = 0.0; i < cSz; i += 1) n_ones++;
     ed_lattice lat = nullptr;
     if (T1.size() != T2.size()) fatal(outfile, "T1 and T2 lengths differ");
     check(ed_lattice_create((int32_t)T1.size(), T1.data(), T2.data(), &lat), outfile, "ed_lattice_create");
@@ -617,9 +659,11 @@ void sector_eigs_eigvecs(int& kx, int& ky, double& cSz, int& n_spins, double& la
     if (kx >= L1 || ky >= L2 || kx < 0 || ky < 0) fatal(outfile, "Invalid kx or ky");
     double px = 2 * kx * M_PI / L1, py = 2 * ky * M_PI / L2;
     double Sz = 0.5 * n_ones - 0.5 * (n_spins - n_ones);
+    outfile << "\n--------------------------------------------------------------------\n";
     Sector& S = sector(T1, T2, n_ones, px, py, outfile);
     outfile << " SECTOR : " << Sz << "\t(" << px << "," << py << ")\t" << S.D << std::endl;
     if (S.D == 0) fatal(outfile, "No states in psector");
+    outfile << "-------------------------------------------------------\n" << std::endl;
     std::vector<int64_t> reps;
     std::vector<double> norm, norm_inv;
     export_basis(S, reps, norm, norm_inv, outfile);
@@ -636,4 +680,33 @@ void sector_eigs_eigvecs(int& kx, int& ky, double& cSz, int& n_spins, double& la
         outfile << std::fixed << std::setprecision(15) << pblock_eigs[i].real() << "\t"
                 << pblock_eigs[i].real() / double(n_spins) << "\t" << std::setprecision(3) << Sz << "\t" << px << "\t"
                 << py << "\t" << lambda << "\t" << J2 << "\n";
+    if (!find_ev) return;
+    // ARPACK_sector.cpp:345-397: files numbered from 1 by one character,
+    // vector iib read as row iib of pblock_eigvecs; the reference reads past
+    // the returned values when n_sec_ev exceeds them, the shim stops there
+    for (int iib = 0; iib < std::min<int>(n_sec_ev, (int)pblock_eigs.size()); iib++) {
+        const char tag = (char)(iib + 1 + (int)'0');
+        std::string fp = filepath_EV + (unwrap_peigvec ? "_EV" : "_pEV") + tag + ".txt";
+        std::ofstream f(fp.c_str());
+        f << "\n Eigenvector for state \n";
+        f << "Sz\t(kx,ky)\tJz/lambda\n";
+        f << Sz << "\t(" << kx << "," << ky << ")\t" << lambda << std::endl;
+        f << "\n\n-------------------------------------------";
+        f << "\nState Energy " << pblock_eigs[iib] << std::endl;
+        std::vector<cplx> v(n_p);
+        for (int64_t jb = 0; jb < n_p; jb++) v[jb] = pblock_eigvecs(iib, jb);
+        if (unwrap_peigvec) {
+            std::vector<Ids> psi = unwrap(S, v, outfile);
+            f << "\nEIGENVECTOR\n";
+            for (auto& e : psi) f << e.stateID << "\t" << std::fixed << std::setprecision(15) << e.coeff << std::endl;
+        } else {
+            f << "\nMomentum EIGENVECTOR\n";
+            f << "\nnp \t\t px \t\tpy\n";
+            f << n_p << "\t\t" << px << "\t\t" << py << std::endl;
+            f << "\n\npblock_states \t\t Norm \t\t\t v_coeff\n\n";
+            for (int64_t jb = 0; jb < n_p; jb++)
+                f << reps[jb] << "\t\t\t" << norm[jb] << "\t\t\t" << std::fixed << std::setprecision(15) << v[jb]
+                  << std::endl;
+        }
+    }
 }

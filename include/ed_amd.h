@@ -126,19 +126,34 @@ int ed_op_tune(ed_op op, int32_t path, int32_t block);
  * pipelined, -1 = not yet tuned or not applicable) and threads per block. */
 int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
- * total number of off-diagonal elements (one counting pass on the GPU). */
+ * total number of off-diagonal elements (one counting pass on the GPU; no
+ * reference counterpart: the G of the algorithmic-bytes model). */
 int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total);
 
 /* w = H v (sHv, H_v.cpp:192-357, decl H_v.h:36-57).  v, w are DEVICE
  * pointers to D complex numbers of type dtype; w is overwritten; v and w must
  * not alias.  stream is a hipStream_t (NULL = default stream).  The call is
- * asynchronous with respect to the host. */
+ * asynchronous with respect to the host.  The operator owns one gather buffer
+ * per precision (u = v/norm, written by each call before its row kernel):
+ * calls on one operator and precision that may run at the same time on
+ * different streams race on it, so order them (one stream, or events), or
+ * give each stream its own operator. */
 int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stream);
 /* ed_hv with the dominant kernel bracketed by HIP events on `stream`:
  * *kernel_ms receives the duration of the row kernel alone (the v/norm
  * prescale pass is excluded).  Synchronises the stream.  No reference
  * counterpart: measurement helper for roofline accounting. */
 int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, void* stream, float* kernel_ms);
+/* Random-gather calibration (no reference counterpart: a measurement helper
+ * that gives the H*v roofline a measured ceiling).  One launch gathers
+ * n_gather elements of elem_bytes (8 = complex64, 16 = complex128) from
+ * random 128-byte lines of a device buffer of n_elem elements, allocated and
+ * freed inside the call; lanes in groups of `group` (a power of 2, 1..64)
+ * share each line, inflight (4 or 8) loads per lane are in flight, threads
+ * per block 64..512.  *ms = the best of three timed launches (HIP events).
+ * Synchronous. */
+int ed_gather_calibrate(int64_t n_elem, int32_t elem_bytes, int64_t n_gather, int32_t group, int32_t threads,
+                        int32_t inflight, float* ms);
 /* Same with HOST complex128 pointers (the reference sHv calling convention);
  * copies in and out, synchronous. */
 int ed_hv_host(ed_op op, const double* v, double* w);

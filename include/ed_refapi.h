@@ -8,6 +8,13 @@
 // objects, compiled against its own global.h, share the mangled names and the
 // memory layout; a translation unit must include either this header or the
 // reference's global.h, not both.
+//
+// The Lanczos and ARPACK-like solvers behind these entries keep every Krylov
+// vector in complex128 unless the environment variable ED_PRECISION is "c64":
+// then the vectors are stored in complex64, alpha/beta, the
+// reorthogonalisation coefficients and the tridiagonal eigensolve stay FP64,
+// and each eigenvalue is replaced by the FP64 Rayleigh quotient of its Ritz
+// vector, whose FP64 residual is written to the log.
 #ifndef ED_REFAPI_H
 #define ED_REFAPI_H
 
@@ -193,7 +200,12 @@ void ARPACK_Iteration_EV(std::vector<int64_t>& pblock_states, std::vector<double
                          std::vector<int>& T1, std::vector<int>& T2, std::vector<int64_t>& Ia,
                          std::vector<int64_t>& Ib, int& bits_right, std::ofstream& outfile);
 
-// ARPACK_sector.h:38-66: lowest n_ev eigenvalues ("SR") of the sector.
+// ARPACK_sector.h:38-66: lowest n_ev eigenvalues ("SR") of the sector,
+// largest first, with find_ev the vectors as rows of pblock_eigvecs
+// (n_ev x n_p).  When n_p <= n_ev + 1 (ARPACK_sector.cpp:232-251) all n_p
+// eigenpairs of the dense sector matrix, largest first, the vectors as the
+// COLUMNS of pblock_eigvecs (n_p x n_p, zgeev's layout), and the matrix is
+// printed to the log.
 void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>& norm, std::vector<double>& norm_inv,
                    std::vector<int64_t>& rep_loc, std::vector<std::complex<double> >& state_ph,
                    std::vector<std::complex<double> >& pblock_eigs, zMatrix& pblock_eigvecs, bool find_ev,
@@ -203,7 +215,10 @@ void ARPACK_sector(std::vector<int64_t>& pblock_states, std::vector<double>& nor
                    double tol, int maxiter, std::vector<int64_t>& Ia, std::vector<int64_t>& Ib, int& bits_right,
                    std::ofstream& outfile);
 
-// ARPACK_sector.h:67-86
+// ARPACK_sector.h:67-86: n_ones = n_spins/2 plus one per unit step below cSz
+// (ARPACK_sector.cpp:284-287), basis, ARPACK_sector, the energy table
+// ascending, with find_ev the _EV<c>/_pEV<c> files numbered '1', '2', ...
+// (:345-397).
 void sector_eigs_eigvecs(int& kx, int& ky, double& cSz, int& n_spins, double& lambda, double& J2,
                          std::vector<coordinates>& adj_list, double& Ch, std::vector<triangles>& ch_list,
                          double& ChCh, std::vector<bowties>& chch_list, std::vector<int>& T1, std::vector<int>& T2,

@@ -19,6 +19,11 @@
 //   or_basis_*             Burnside sector dimensions, sum_k D_k = C(N,n); _ref bit-exact
 //   or_hv                  _ref (<=1e-14), Hermiticity, dense brute-force projection
 //   or_make_h              _ref bit-exact, numpy eigvalsh spectra, closed forms
+//   or_hv_rows             or_hv on the same rows, zero elsewhere
+//   or_count_targets       brute-force count over the reference's rep_loc (fixtures)
+//   or_tables_new          or_hv on adopted reference tables = the fixtures' w
+//   or_hv_rows_search /    the fixtures' w on the same rows (2-D sectors); the
+//   or_rows_targets_search full-table oracle on the relabelled tilted cluster
 //   or_lanczos             closed-form spectra (4-site ring, XY chain free fermions,
 //                          Ising limit), numpy eigvalsh on dense H
 #include <cstdint>
@@ -389,15 +394,17 @@ OR_API void or_basis_copy(void* h, int64_t* reps, double* norm, double* norm_inv
 }
 
 // ------------------------------------------------------------------- H*v ---
-// One row of H_v.cpp:218-355 (sHv, on-the-fly 2-D).  Calls emit(j, coef, kind)
-// for every element in the reference's order: bond terms (kind 0), then the
-// diagonal Ezz (kind 3, j = i, coef = Ezz), then chiral (1) and bowtie (2).
-template <class Emit>
-static inline void row_terms(const OrBasis* B, const or_terms* t, int64_t i, Emit emit) {
-    const int N = B->N;
-    const int64_t idi = B->reps[i];
+// One row of H_v.cpp:218-355 (sHv, on-the-fly 2-D) for the state idi with
+// norm norm_i.  look(tmp, j, ph) gives rep_loc and state_ph of the flipped
+// state tmp; ninv(j) = norm_inv[j].  Calls emit(j, coef, kind) for every
+// element in the reference's order: bond terms (kind 0), then the diagonal
+// Ezz (kind 3, j = -2, coef = Ezz), then chiral (1) and bowtie (2).
+template <class Look, class NInv, class Emit>
+static inline void row_terms_g(int64_t idi, double norm_i, const or_terms* t, Look look, NInv ninv, Emit emit) {
     const double Ezzfac = 0.25 * t->lambda;
     double Ezz = 0.0;
+    int64_t j;
+    cd ph;
     for (int k = 0; k < t->nbonds; k++) {
         int x = t->bond_xy[2 * k], y = t->bond_xy[2 * k + 1];
         double J = t->bond_J[k];
@@ -409,14 +416,11 @@ static inline void row_terms(const OrBasis* B, const or_terms* t, int64_t i, Emi
             if (tx) ep = std::conj(ep);
             Ezz += -(Ezzfac * J);
             int64_t tmp = flip2(idi, x, y, tx, ty);
-            int64_t left, right;
-            split_id(tmp, left, right, B->bits_right, N);
-            int64_t loc = B->Ia[right] + B->Ib[left] - 1;
-            int64_t j = B->rep_loc[loc];
-            if (j >= 0) emit(j, ep * J * 0.5 * B->state_ph[loc] * B->norm[i] * B->norm_inv[j], 0);
+            look(tmp, j, ph);
+            if (j >= 0) emit(j, ep * J * 0.5 * ph * norm_i * ninv(j), 0);
         }
     }
-    emit(i, cd(Ezz, 0.0), 3);
+    emit(int64_t(-2), cd(Ezz, 0.0), 3);
     if (std::fabs(t->Ch) > 1e-16) {
         for (int k = 0; k < t->nchiral; k++) {
             int x = t->chiral_ijk[3 * k], y = t->chiral_ijk[3 * k + 1], z = t->chiral_ijk[3 * k + 2];
@@ -424,14 +428,11 @@ static inline void row_terms(const OrBasis* B, const or_terms* t, int64_t i, Emi
             bool tx = btest(idi, x), ty = btest(idi, y);
             if (tx != ty) {
                 int64_t tmp = flip2(idi, x, y, tx, ty);
-                int64_t left, right;
-                split_id(tmp, left, right, B->bits_right, N);
-                int64_t loc = B->Ia[right] + B->Ib[left] - 1;
-                int64_t j = B->rep_loc[loc];
+                look(tmp, j, ph);
                 if (j >= 0) {
                     cd sg = ty ? cd(0, 1) * ep : cd(0, -1) * std::conj(ep);
                     if (!btest(idi, z)) sg = sg * cd(-1, 0);
-                    emit(j, t->Ch * 0.25 * sg * B->state_ph[loc] * B->norm[i] * B->norm_inv[j], 1);
+                    emit(j, t->Ch * 0.25 * sg * ph * norm_i * ninv(j), 1);
                 }
             }
         }
@@ -450,18 +451,33 @@ static inline void row_terms(const OrBasis* B, const or_terms* t, int64_t i, Emi
                 if (tp != tq) {
                     int64_t tmp2 = flip2(tmp, p, q, tp, tq);
                     bool tr = btest(tmp2, r);
-                    int64_t left, right;
-                    split_id(tmp2, left, right, B->bits_right, N);
-                    int64_t loc = B->Ia[right] + B->Ib[left] - 1;
-                    int64_t j = B->rep_loc[loc];
+                    look(tmp2, j, ph);
                     if (j >= 0) {
                         if (!tr) s2 = s2 * cd(-1, 0);
-                        emit(j, t->ChCh * 0.25 * 0.25 * s1 * s2 * B->state_ph[loc] * B->norm[i] * B->norm_inv[j], 2);
+                        emit(j, t->ChCh * 0.25 * 0.25 * s1 * s2 * ph * norm_i * ninv(j), 2);
                     }
                 }
             }
         }
     }
+}
+
+// Row i of the basis B through its full-space tables: rep_loc / state_ph at
+// the Lin-table position Ia[right] + Ib[left] - 1 (H_v.cpp:229-234).  The
+// diagonal element is emitted with j = i.
+template <class Emit>
+static inline void row_terms(const OrBasis* B, const or_terms* t, int64_t i, Emit emit) {
+    const int N = B->N;
+    auto look = [&](int64_t tmp, int64_t& j, cd& ph) {
+        int64_t left, right;
+        split_id(tmp, left, right, B->bits_right, N);
+        int64_t loc = B->Ia[right] + B->Ib[left] - 1;
+        j = B->rep_loc[loc];
+        ph = B->state_ph[loc];
+    };
+    auto ninv = [&](int64_t j) { return B->norm_inv[j]; };
+    row_terms_g(B->reps[i], B->norm[i], t, look, ninv,
+                [&](int64_t j, cd c, int kind) { emit(kind == 3 ? i : j, c, kind); });
 }
 
 // H_v.cpp:192-357 (sHv).  v, w: interleaved complex128 [D].  w is overwritten.
@@ -521,6 +537,113 @@ OR_API void or_make_h(void* h, const or_terms* t, double* H_) {
             if (kind == 3) H[i + i * D] += c.real(); else H[i + j * D] += c;
         });
     }
+}
+
+// ------------------------------------------- tables supplied from outside ---
+// An OrBasis over caller-supplied copies of pConstruct_States' outputs (reps,
+// norm, norm_inv, rep_loc, state_ph, Ia, Ib, bits_right), so that or_hv,
+// or_hv_rows and or_count_targets run on sectors whose oracle build would be
+// too slow for a test.  The caller fills the arrays through or_tables_ptrs
+// (order: reps, norm, norm_inv, rep_loc, state_ph, Ia, Ib) and pins the
+// tables it supplies on its own (or_representative on sampled states).
+OR_API void* or_tables_new(int N, int n_ones, const int32_t* T1, const int32_t* T2, double px, double py, int64_t C,
+                           int64_t D, int64_t nlin, int bits_right) {
+    OrBasis* B = new OrBasis();
+    B->N = N; B->n = n_ones; B->px = px; B->py = py;
+    B->half = N / 2 + (N % 2);
+    or_LxLy(T1, T2, N, &B->L1, &B->L2);
+    B->C = C; B->D = D;
+    B->bits_right = bits_right;
+    B->lin_error = 0;
+    B->reps.resize(D); B->norm.resize(D); B->norm_inv.resize(D);
+    B->rep_loc.resize(C); B->state_ph.resize(C);
+    B->Ia.resize(nlin); B->Ib.resize(nlin);
+    return B;
+}
+
+OR_API void or_tables_ptrs(void* h, void** out) {
+    OrBasis* B = (OrBasis*)h;
+    out[0] = B->reps.data(); out[1] = B->norm.data(); out[2] = B->norm_inv.data();
+    out[3] = B->rep_loc.data(); out[4] = B->state_ph.data(); out[5] = B->Ia.data(); out[6] = B->Ib.data();
+}
+
+// ------------------------------------------------ rows without the tables ---
+// H_v.cpp:218-355 for selected rows of a two-generator sector, without the
+// full-space tables: each target's rep_loc / state_ph entry is recomputed the
+// way pConstruct_States defines it (Momentum_Basis.cpp:645-778):
+// representative_p of the target (:472-543; lx, ly reduced mod L1, L2 and
+// state_ph = exp(i(-px lx - py ly)), :745-753), findstate of its
+// representative in reps (:220-240, -1 when absent, which includes orbits
+// whose momentum projection vanishes), and norm = sqrt(L1 L2 / lid) of each
+// representative (:764-766; lid is the same for every state of an orbit).
+// For sectors too large for the full-space tables (40 sites: C = 1.4e11).
+template <class Fn>
+static void rows_search(int N, const int32_t* T1, const int32_t* T2, double px, double py, int64_t D,
+                        const int64_t* reps, const or_terms* t, int64_t nrows, const int64_t* rows, Fn fn) {
+    int32_t L1, L2;
+    or_LxLy(T1, T2, N, &L1, &L2);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t q = 0; q < nrows; q++) {
+        const int64_t i = rows[q];
+        int32_t lx, ly, pid, lid;
+        or_representative(reps[i], T1, T2, N, L1, L2, px, py, &lx, &ly, &pid, &lid);
+        const double ni = std::sqrt((double)((int64_t)L1 * L2 / lid));
+        double last_ninv = 0.0;   // norm_inv of the target looked up last (row_terms_g asks right after)
+        auto look = [&](int64_t tmp, int64_t& j, cd& ph) {
+            int32_t tx, ty, tpid, tlid;
+            int64_t r = or_representative(tmp, T1, T2, N, L1, L2, px, py, &tx, &ty, &tpid, &tlid);
+            tx = tx % L1; ty = ty % L2;
+            if (tx < 0) tx += L1;
+            if (ty < 0) ty += L2;
+            ph = std::exp(cd(0.0, -px * tx - py * ty));
+            const int64_t* it = std::lower_bound(reps, reps + D, r);
+            j = (it != reps + D && *it == r) ? (int64_t)(it - reps) : -1;
+            if (j >= 0) last_ninv = 1 / std::sqrt((double)((int64_t)L1 * L2 / tlid));
+        };
+        auto ninv = [&](int64_t) { return last_ninv; };
+        fn(q, i, ni, look, ninv);
+    }
+}
+
+// The indices that rows[0..nrows) gather from (targets with j >= 0 and the
+// rows themselves), sorted and unique: returns their number and writes the
+// first cap of them to out.  Lets a caller supply only those entries of v.
+OR_API int64_t or_rows_targets_search(int N, const int32_t* T1, const int32_t* T2, double px, double py, int64_t D,
+                                      const int64_t* reps, const or_terms* t, int64_t nrows, const int64_t* rows,
+                                      int64_t cap, int64_t* out) {
+    std::vector<std::vector<int64_t>> per(nrows);
+    rows_search(N, T1, T2, px, py, D, reps, t, nrows, rows, [&](int64_t q, int64_t i, double ni, auto& look, auto& ninv) {
+        per[q].push_back(i);
+        row_terms_g(reps[i], ni, t, look, ninv, [&](int64_t j, cd, int kind) { if (kind != 3) per[q].push_back(j); });
+    });
+    std::vector<int64_t> all;
+    for (auto& p : per) all.insert(all.end(), p.begin(), p.end());
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    for (int64_t k = 0; k < (int64_t)all.size() && k < cap; k++) out[k] = all[k];
+    return (int64_t)all.size();
+}
+
+// w_out[q] = (H v)[rows[q]], complex128 [nrows].  v is given by its entries
+// at the sorted indices vidx[0..nv) (values vval, complex128 [nv]); an entry
+// a row needs and vidx lacks makes that row NaN.
+OR_API void or_hv_rows_search(int N, const int32_t* T1, const int32_t* T2, double px, double py, int64_t D,
+                              const int64_t* reps, const or_terms* t, const int64_t* vidx, const double* vval_,
+                              int64_t nv, int64_t nrows, const int64_t* rows, double* w_out) {
+    const cd* vval = (const cd*)vval_;
+    cd* w = (cd*)w_out;
+    auto vat = [&](int64_t j) {
+        const int64_t* it = std::lower_bound(vidx, vidx + nv, j);
+        return (it != vidx + nv && *it == j) ? vval[it - vidx] : cd(NAN, NAN);
+    };
+    rows_search(N, T1, T2, px, py, D, reps, t, nrows, rows, [&](int64_t q, int64_t i, double ni, auto& look, auto& ninv) {
+        cd acc(0.0, 0.0);
+        const cd vi = vat(i);
+        row_terms_g(reps[i], ni, t, look, ninv, [&](int64_t j, cd c, int kind) {
+            if (kind == 3) acc += vi * c.real(); else acc += c * vat(j);
+        });
+        w[q] = acc;
+    });
 }
 
 // --------------------------------------------------------------- Lanczos ---

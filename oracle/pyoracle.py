@@ -80,6 +80,15 @@ def lib():
         L.or_count_targets.argtypes = [C.c_void_p, C.POINTER(Terms), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.or_make_h.argtypes = [C.c_void_p, C.POINTER(Terms), f64p]
         L.or_start_vector.argtypes = [C.c_uint64, C.c_int64, f64p]
+        L.or_tables_new.restype = C.c_void_p
+        L.or_tables_new.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_int64,
+                                    C.c_int64, C.c_int]
+        L.or_tables_ptrs.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.or_hv_rows_search.argtypes = [C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_void_p,
+                                        C.POINTER(Terms), i64p, f64p, C.c_int64, C.c_int64, i64p, f64p]
+        L.or_rows_targets_search.restype = C.c_int64
+        L.or_rows_targets_search.argtypes = [C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_void_p,
+                                             C.POINTER(Terms), C.c_int64, i64p, C.c_int64, C.c_void_p]
         L.or_lanczos.restype = C.c_int64
         L.or_lanczos.argtypes = [C.c_void_p, C.POINTER(Terms), C.c_int, C.c_int, C.c_int, C.c_double, C.c_uint64,
                                  f64p, C.c_void_p]
@@ -358,8 +367,98 @@ class RefSHv:
         except Exception:
             pass
 
+    def set_model(self, model):
+        self.model = model
+        self.terms, self._keep = make_terms(model)
+
     def hv_rows(self, v, rows):
         v = np.ascontiguousarray(v, np.complex128)
         w = np.zeros(self.D, np.complex128)
         ref().ref_hv(self.h, C.addressof(self.terms), v.view(np.float64), w.view(np.float64), rows)
         return w
+
+
+class OracleTables:
+    """The oracle's row routines (or_hv, or_hv_rows, or_count_targets) over
+    pConstruct_States tables supplied from outside (or_tables_new): fill the
+    arrays through .ptrs (reps, norm, norm_inv, rep_loc, state_ph, Ia, Ib),
+    e.g. with the product's export, whose tables a test pins separately."""
+
+    def __init__(self, model, n_ones, px, py, C_, D, nlin, bits_right):
+        self.model = model
+        self.D = D
+        self.h = lib().or_tables_new(model.n_sites, n_ones, np.ascontiguousarray(model.T1, np.int32),
+                                     np.ascontiguousarray(model.T2, np.int32), px, py, C_, D, nlin, bits_right)
+        ptrs = (C.c_void_p * 7)()
+        lib().or_tables_ptrs(self.h, ptrs)
+        self.ptrs = [p for p in ptrs]
+        self.terms, self._keep = make_terms(model)
+
+    def __del__(self):
+        try:
+            lib().or_basis_free(self.h)
+        except Exception:
+            pass
+
+    def set_model(self, model):
+        self.model = model
+        self.terms, self._keep = make_terms(model)
+
+    def hv(self, v):
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        lib().or_hv(self.h, C.byref(self.terms), v.view(np.float64), w.view(np.float64))
+        return w
+
+    def count_targets(self):
+        g = C.c_int64(); t = C.c_int64()
+        lib().or_count_targets(self.h, C.byref(self.terms), C.byref(g), C.byref(t))
+        return g.value, t.value
+
+
+def rows_targets_search(model, px, py, reps, rows):
+    """Sorted indices of v that the rows gather from (their targets and the
+    rows themselves), without full-space tables (or_rows_targets_search)."""
+    rows = np.ascontiguousarray(rows, np.int64)
+    t, keep = make_terms(model)
+    args = (model.n_sites, np.ascontiguousarray(model.T1, np.int32), np.ascontiguousarray(model.T2, np.int32), px, py,
+            len(reps), reps.ctypes.data, C.byref(t), len(rows), rows)
+    n = lib().or_rows_targets_search(*args, 0, None)
+    out = np.zeros(n, np.int64)
+    lib().or_rows_targets_search(*args, n, out.ctypes.data)
+    return out
+
+
+def hv_rows_search(model, px, py, reps, v, rows, vidx=None):
+    """(H v)[rows] of a two-generator sector without full-space tables
+    (or_hv_rows_search): representative_p + findstate per target.  reps:
+    int64 array (may be a large memory view); v: the whole vector, or with
+    vidx its entries at those sorted indices."""
+    assert reps.dtype == np.int64 and reps.flags.c_contiguous
+    rows = np.ascontiguousarray(rows, np.int64)
+    v = np.ascontiguousarray(v, np.complex128)
+    vidx = np.arange(len(v), dtype=np.int64) if vidx is None else np.ascontiguousarray(vidx, np.int64)
+    t, keep = make_terms(model)
+    w = np.zeros(len(rows), np.complex128)
+    lib().or_hv_rows_search(model.n_sites, np.ascontiguousarray(model.T1, np.int32),
+                            np.ascontiguousarray(model.T2, np.int32), px, py, len(reps), reps.ctypes.data, C.byref(t),
+                            vidx, v.view(np.float64), len(vidx), len(rows), rows, w.view(np.float64))
+    return w
+
+
+def representative(model, s, px, py):
+    """or_representative (representative_p, Momentum_Basis.cpp:472-543):
+    (rep, lx, ly, p_id, lid)."""
+    L1, L2 = model.orders()
+    lx, ly, pid, lid = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    r = lib().or_representative(int(s), np.ascontiguousarray(model.T1, np.int32),
+                                np.ascontiguousarray(model.T2, np.int32), model.n_sites, L1, L2, px, py,
+                                C.byref(lx), C.byref(ly), C.byref(pid), C.byref(lid))
+    return int(r), lx.value, ly.value, pid.value, lid.value
+
+
+def cexp(re, im):
+    """std::exp of complex(re, im) as the reference computes state_ph (glibc cexp)."""
+    out = np.zeros(2)
+    lib().or_cexp(re, im, out)
+    return complex(out[0], out[1])

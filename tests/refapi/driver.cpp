@@ -11,6 +11,8 @@
 //   driver makeh  <spec> <H.bin>                      2-D pConstruct_States + makeH
 //   driver sparse <spec> <v.bin> <w.bin>              makeH -> sMatrix -> sparse sHv
 //   driver arpack <spec> <evals.bin> <evecs.bin>      2-D ARPACK_Iteration_EV
+//   driver arpsec <spec> <evals.bin> <evecs.bin> <log> 2-D ARPACK_sector (both branches)
+//   driver eigs   <spec> <log> <vecpath> <unwrap>     sector_eigs_eigvecs
 //
 // spec (whitespace separated): N n_ones kx ky kz lambda iterations ncycles
 // n_ev tol, then T1[N], T2[N], T3[N], then nbonds and per bond x y J re im.
@@ -115,7 +117,19 @@ int main(int argc, char** argv) {
                        find_ev, unwrap, vecpath, s.tol, s.iterations, s.ncycles, log);
         return 0;
     }
-    if ((mode == "makeh" && argc == 4) || (mode == "sparse" && argc == 5) || (mode == "arpack" && argc == 5)) {
+    if (mode == "eigs" && argc == 6) {
+        // sector_eigs_eigvecs (ARPACK_sector.cpp:262-401): maxiter = ncycles
+        std::ofstream log(argv[3]);
+        double cSz = 0.5 * s.n - 0.5 * (s.N - s.n), J2 = 0;
+        std::string vecpath = argv[4];
+        bool find_ev = !vecpath.empty() && vecpath != "-", unwrap = std::atoi(argv[5]) != 0;
+        int maxiter = s.ncycles;
+        sector_eigs_eigvecs(s.kx, s.ky, cSz, s.N, s.lambda, J2, s.adj, Ch, ch, ChCh, chch, s.T1, s.T2, s.nev, find_ev,
+                            unwrap, vecpath, s.tol, maxiter, log);
+        return 0;
+    }
+    if ((mode == "makeh" && argc == 4) || (mode == "sparse" && argc == 5) || (mode == "arpack" && argc == 5) ||
+        (mode == "arpsec" && argc == 6)) {
         std::ofstream log("/dev/null");
         const int L1 = orbit_len(s.T1), L2 = orbit_len(s.T2);
         double px = 2 * s.kx * M_PI / L1, py = 2 * s.ky * M_PI / L2;
@@ -126,6 +140,24 @@ int main(int argc, char** argv) {
         int bits_right = 0, N = s.N;
         pConstruct_States(reps, norm, norm_inv, s.T1, s.T2, s.n, px, py, D, rep_loc, state_ph, Ia, Ib, bits_right,
                           log);
+        if (mode == "arpsec") {
+            // ARPACK_sector as sector_eigs_eigvecs calls it (ARPACK_sector.cpp:324-327)
+            std::ofstream alog(argv[5]);
+            std::vector<std::complex<double> > evals;
+            zMatrix evecs;
+            int n_ones = s.n;
+            ARPACK_sector(reps, norm, norm_inv, rep_loc, state_ph, evals, evecs, true, s.lambda, N, px, py, n_ones, D,
+                          s.adj, Ch, ch, ChCh, chch, s.T1, s.T2, s.nev, s.tol, s.ncycles, Ia, Ib, bits_right, alog);
+            FILE* fe = std::fopen(argv[3], "wb");
+            std::fwrite(evals.data(), sizeof(evals[0]), evals.size(), fe);
+            std::fclose(fe);
+            FILE* fv = std::fopen(argv[4], "wb");
+            std::fwrite(evecs.data(), sizeof(evecs[0]), evecs.size(), fv);
+            std::fclose(fv);
+            std::printf("D %lld nev %zu rows %lld cols %lld\n", (long long)D, evals.size(), (long long)evecs.NRows(),
+                        (long long)evecs.NCols());
+            return 0;
+        }
         if (mode == "arpack") {
             std::vector<std::complex<double> > evals;
             zMatrix evecs;

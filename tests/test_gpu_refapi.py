@@ -191,3 +191,112 @@ def test_arpack_iteration_ev(gpu_ready, tmp_path):
         x = V[i]
         assert abs(np.linalg.norm(x) - 1) < 1e-9
         assert np.linalg.norm(o.hv(x) - e[i].real * x) < 1e-6
+
+
+# ------------------------------------- ARPACK_sector, sector_eigs_eigvecs ----
+ARPSEC = [("sq4x4_nev3", L.square(4, 4, J2=0.3), 8, (0, 0), 3),      # n_p > n_ev + 1: Arnoldi branch
+          ("chain6_n2_nev2", L.chain(6, J=0.7), 2, (0, 0), 2),        # n_p = 3 = n_ev + 1: dense branch
+          ("chain6_n2_nev5", L.chain(6, J=0.7), 2, (0, 0), 5)]        # n_ev > n_p - 1: dense, it = n_p
+
+
+@pytest.mark.parametrize("name,model,n,k,nev", ARPSEC, ids=[c[0] for c in ARPSEC])
+def test_arpack_sector_branches(gpu_ready, tmp_path, name, model, n, k, nev):
+    """ARPACK_sector (ARPACK_sector.cpp:186-259): with n_p > n_ev + 1 the n_ev
+    lowest eigenvalues largest first and the vectors as rows of
+    pblock_eigvecs; otherwise (:232-251) it = n_p, all n_p eigenpairs of the
+    dense matrix, the vectors as its columns, the matrix in the log."""
+    print(f"[case] ARPACK_sector {name}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    Ho = o.make_h()
+    ev = np.linalg.eigvalsh(Ho)
+    spec, fe, fv, log = (str(tmp_path / x) for x in ("spec", "e.bin", "V.bin", "log.txt"))
+    Hp.write_spec(spec, model, n, k, nev=nev, ncycles=300, tol=1e-12)
+    out = _driver("arpsec", spec, fe, fv, log)
+    e = np.fromfile(fe, np.complex128)
+    V = np.fromfile(fv, np.complex128)
+    dense = o.D <= nev + 1
+    it = o.D if dense else nev
+    assert f"D {o.D} nev {it} " in out
+    assert np.abs(e.imag).max() == 0
+    assert np.abs(e.real - ev[:it][::-1]).max() < 1e-10
+    text = open(log).read()
+    if dense:
+        assert " Here " in text and f"rows {o.D} cols {o.D}" in out
+        V = V.reshape(o.D, o.D, order="F")
+        vecs = [V[:, q] for q in range(it)]          # columns, as zgeev's VR
+        mat = [l for l in text.split(" Here ")[1].split("Eigenvalues")[0].splitlines() if l.startswith("(")]
+        assert len(mat) == o.D
+        for ii, l in enumerate(mat):
+            vals = [complex(*map(float, t.strip("()").split(","))) for t in l.split()]
+            assert np.abs(np.array(vals) - Ho[ii]).max() < 1e-5
+    else:
+        V = V.reshape(it, o.D, order="F")
+        vecs = [V[q] for q in range(it)]
+    assert "Eigenvalues" in text
+    for q in range(it):
+        x = vecs[q]
+        assert abs(np.linalg.norm(x) - 1) < 1e-9
+        assert np.linalg.norm(o.hv(x) - e[q].real * x) < 1e-6
+
+
+@pytest.mark.parametrize("unwrap", [0, 1], ids=["pEV", "EV"])
+def test_sector_eigs_eigvecs(gpu_ready, tmp_path, unwrap):
+    """sector_eigs_eigvecs (ARPACK_sector.cpp:262-401): the energy table
+    ascending, eigenvector files numbered from '1', file c holding the vector
+    of the c-th returned value (largest of the wanted first)."""
+    model, n, k, nev = L.square(4, 4, J2=0.4, lam=0.9), 9, (1, 2), 2
+    print(f"[case] sector_eigs_eigvecs unwrap={unwrap}", flush=True)
+    o = P.OracleBasis(model, n, *k)
+    ev = np.linalg.eigvalsh(o.make_h())
+    spec, log, vp = str(tmp_path / "spec"), str(tmp_path / "log.txt"), str(tmp_path / "vec")
+    Hp.write_spec(spec, model, n, k, nev=nev, ncycles=300, tol=1e-12)
+    _driver("eigs", spec, log, vp, str(unwrap))
+    e = I.parse_energies(log)
+    assert len(e) == nev and np.abs(np.array(e) - ev[:nev]).max() < 1e-10
+    for c in range(nev):
+        f = vp + ("_EV" if unwrap else "_pEV") + str(c + 1) + ".txt"
+        lines = open(f).read().splitlines()
+        E = float(next(l for l in lines if l.startswith("State Energy")).split()[2].strip("()").split(",")[0])
+        # the file prints the energy with the stream's default 6 digits
+        assert abs(E - ev[nev - 1 - c]) < 1e-5 * max(1.0, abs(E))
+        E = ev[nev - 1 - c]
+        if unwrap:
+            rows = [l.split() for l in lines if l.strip() and l.split()[0].isdigit() and len(l.split()) == 2]
+            ids = [int(r[0]) for r in rows]
+            amp = np.array([complex(*map(float, r[1].strip("()").split(","))) for r in rows])
+            H, st = Hp.full_space_h(model, n)
+            idx = {s: i for i, s in enumerate(st)}
+            psi = np.zeros(len(st), complex)
+            for s, a in zip(ids, amp):
+                psi[idx[s]] = a
+            assert abs(np.linalg.norm(psi) - 1) < 1e-9
+            assert np.linalg.norm(H @ psi - E * psi) < 1e-8
+        else:
+            start = next(i for i, l in enumerate(lines) if l.startswith("pblock_states"))
+            rows = [l.split() for l in lines[start + 1:] if l.strip()]
+            x = np.array([complex(*map(float, r[2].strip("()").split(","))) for r in rows])
+            assert len(x) == o.D and abs(np.linalg.norm(x) - 1) < 1e-9
+            assert np.linalg.norm(o.hv(x) - E * x) < 1e-6
+    assert not os.path.exists(vp + ("_EV0.txt" if unwrap else "_pEV0.txt"))
+
+
+@needs_ed2
+def test_reference_main_complex64_krylov(gpu_ready, tmp_path):
+    """ED_PRECISION=c64: the reference's own main on the Lanczos branch with
+    complex64 Krylov storage, FP64 alpha/beta and the FP64 Rayleigh-quotient
+    correction; energies within 1e-10 of the complex128 run, FP64 residuals
+    in the log."""
+    model = L.chain(22)
+    out = {}
+    for prec in ("c128", "c64"):
+        f, log = str(tmp_path / f"in_{prec}.txt"), str(tmp_path / f"log_{prec}.txt")
+        I.write_input(f, model, 0.0, 0, 0, num_eigenvalues=2, lanczos_iterations=80, lanczos_ncycles=4, tol=1e-12)
+        r = subprocess.run([ED2, f, log], capture_output=True, text=True, timeout=180,
+                           env=dict(os.environ, ED_PRECISION=prec))
+        assert r.returncode == 0, open(log).read()[-2000:]
+        out[prec] = I.parse_energies(log)
+        text = open(log).read()
+        assert ("complex64" if prec == "c64" else "complex128") in text
+        res = [float(l.split()[-1]) for l in text.splitlines() if l.startswith(" FP64 residual")]
+        assert len(res) == 2 and max(res) < 1e-4
+    assert np.abs(np.array(out["c64"]) - np.array(out["c128"])).max() < 1e-10 * abs(out["c128"][0])

@@ -1,5 +1,6 @@
 """Pins the CPU oracle (oracle/ed_oracle.cpp) against the reference's golden
 vectors, the compiled reference slice (when built), and closed forms."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -254,3 +255,109 @@ def test_chiral_entries_are_minus_the_scalar_chirality():
         assert np.abs(H + X).max() < 1e-15
         i, j, k = tri
         assert np.abs(Hp.scalar_chirality_matrix(N, st, (i, k, j)) - H).max() < 1e-15
+
+
+# ------------------------------------------------ row routines of the oracle ----
+def _brute_targets(m, d):
+    """Distinct (row, target) pairs with j >= 0 and the element count, from
+    the fixture's reference tables alone: every off-diagonal term of H_v.cpp
+    (antiparallel bond, chiral pair, bowtie double flip), its flipped state's
+    combinatorial-number-system rank into the reference's rep_loc."""
+    rep_loc = d["rep_loc"]
+    G = T = 0
+    for s in (int(x) for x in d["reps"]):
+        js = []
+
+        def add(t):
+            j = int(rep_loc[Hp.cns_rank(t)])
+            if j >= 0:
+                js.append(j)
+
+        for x, y, J, _ in m.bonds:
+            if abs(J) >= 1e-16 and ((s >> x) & 1) != ((s >> y) & 1):
+                add(s ^ (1 << x) ^ (1 << y))
+        if abs(m.Ch) > 1e-16:
+            for i, j, _k, _ in m.chiral:
+                if ((s >> i) & 1) != ((s >> j) & 1):
+                    add(s ^ (1 << i) ^ (1 << j))
+        if abs(m.ChCh) > 1e-16:
+            for x, y, _z, p, q, _r in m.bowties:
+                if ((s >> x) & 1) != ((s >> y) & 1):
+                    t = s ^ (1 << x) ^ (1 << y)
+                    if ((t >> p) & 1) != ((t >> q) & 1):
+                        add(t ^ (1 << p) ^ (1 << q))
+        T += len(js)
+        G += len(set(js))
+    return G, T
+
+
+TWO_GEN = [p for p in GOLDEN if "cubic" not in p]
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[p.split("/")[-1][:-4] for p in GOLDEN])
+def test_count_targets_brute_force(path):
+    """or_count_targets (the G of the algorithmic-bytes model) against a
+    brute-force count over the reference's own rep_loc in the fixture."""
+    m, d = Hp.load_golden(path)
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
+    G, T = o.count_targets()
+    assert (G, T) == _brute_targets(m, d)
+    assert 0 < G <= T
+
+
+@pytest.mark.parametrize("path", GOLDEN[::3], ids=[p.split("/")[-1][:-4] for p in GOLDEN[::3]])
+def test_hv_rows_is_or_hv_on_those_rows(path):
+    m, d = Hp.load_golden(path)
+    o = P.OracleBasis(m, int(d["n_ones"]), int(d["kx"]), int(d["ky"]), int(d["kz"]))
+    w = o.hv(d["v"])
+    r0, r1 = o.D // 3, (2 * o.D) // 3 + 1
+    wr = o.hv_rows(d["v"], r0, r1)
+    assert np.array_equal(wr[r0:r1], w[r0:r1])
+    assert not wr[:r0].any() and not wr[r1:].any()
+
+
+@pytest.mark.parametrize("path", TWO_GEN, ids=[p.split("/")[-1][:-4] for p in TWO_GEN])
+def test_adopted_reference_tables(path):
+    """or_tables_new filled with the reference's tables from the fixture:
+    H v equals the reference's output, the target count equals the oracle's."""
+    m, d = Hp.load_golden(path)
+    n, kx, ky = int(d["n_ones"]), int(d["kx"]), int(d["ky"])
+    px, py = m.momentum(kx, ky)
+    D, Cn = len(d["reps"]), len(d["rep_loc"])
+    t = P.OracleTables(m, n, px, py, Cn, D, len(d["Ia"]), int(d["bits_right"]))
+    for ptr, key, dt in zip(t.ptrs, ("reps", "norm", "norm_inv", "rep_loc", "state_ph", "Ia", "Ib"),
+                            (np.int64, np.float64, np.float64, np.int64, np.complex128, np.int64, np.int64)):
+        a = np.ascontiguousarray(d[key], dt)
+        C.memmove(ptr, a.ctypes.data, a.nbytes)
+    w = t.hv(d["v"])
+    assert np.linalg.norm(w - d["w"]) <= 1e-14 * np.linalg.norm(d["w"])
+    assert t.count_targets() == P.OracleBasis(m, n, kx, ky).count_targets()
+
+
+@pytest.mark.parametrize("path", TWO_GEN, ids=[p.split("/")[-1][:-4] for p in TWO_GEN])
+def test_hv_rows_search_vs_reference(path):
+    """or_hv_rows_search (no full-space tables: representative_p and
+    findstate per target) against the reference's H v in the fixture."""
+    m, d = Hp.load_golden(path)
+    px, py = m.momentum(int(d["kx"]), int(d["ky"]))
+    D = len(d["reps"])
+    rows = np.unique(np.random.default_rng(5).integers(0, D, size=min(D, 40)))
+    w = P.hv_rows_search(m, px, py, np.ascontiguousarray(d["reps"], np.int64), d["v"], rows)
+    assert np.linalg.norm(w - d["w"][rows]) <= 1e-14 * np.linalg.norm(d["w"][rows])
+
+
+def test_hv_rows_search_tilted_relabelled():
+    """The 20-site tilted cluster relabelled to two rows (the 40-site mode's
+    layout): or_hv_rows_search against the full-table oracle."""
+    m, _ = L.regular_relabel(L.tilted_square((4, 2), (-2, 4), J2=0.5))
+    for n, k in ((10, (0, 0)), (9, (3, 1))):
+        o = P.OracleBasis(m, n, *k)
+        reps = o.tables(full=False)["reps"]
+        v = P.start_vector(2, o.D)
+        rows = np.random.default_rng(3).choice(o.D, size=64, replace=False)
+        w = P.hv_rows_search(m, o.px, o.py, reps, v, rows)
+        need = P.rows_targets_search(m, o.px, o.py, reps, rows)
+        ws = P.hv_rows_search(m, o.px, o.py, reps, v[need], rows, vidx=need)
+        assert np.array_equal(ws, w)
+        wo = o.hv(v)[rows]
+        assert np.linalg.norm(w - wo) <= 1e-14 * np.linalg.norm(wo)
