@@ -66,6 +66,7 @@ SIGNATURES = {
     "ed_op_count_targets": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ed_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
     "ed_hv_profile": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.POINTER(C.c_float)]),
+    "ed_hv_noalloc": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
     "ed_hv_host": (C.c_int, [_VP, _VP, _VP]),
     "ed_make_h": (C.c_int, [_VP, _VP]),
     "ed_csr_hv_host": (C.c_int, [C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP]),

@@ -13,6 +13,8 @@
 //                        with the reference's expression (bit-equal)
 // Nothing of size C survives the build except the 1.5 bit/state directory.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <hipcub/hipcub.hpp>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -48,13 +50,45 @@ int ed_fail(int code, const char* fmt, ...) {
 
 extern "C" const char* ed_last_error(void) { return g_err.c_str(); }
 
+// Gather stream of ed_hv_noalloc (u = v/norm by row, or the sector's gather
+// image) in a file-scope device array: each GPU the library runs on holds its
+// own copy until the process ends.  Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in for the symbol): it uses the
+// device address that hipGetSymbolAddress returned for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+int scratch_ptr(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return ed_fail(ED_EINVAL, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ed_fail(ED_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ED_OK;
+}
+}  // namespace
+
 extern "C" int ed_init(int device) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n == 0) return ed_fail(ED_EDEVICE, "no HIP device: %s", hipGetErrorString(e));
     if (device < 0 || device >= n) return ed_fail(ED_EINVAL, "device %d out of range (%d devices)", device, n);
     HIPCHK(hipSetDevice(device));
-    return ED_OK;
+    void* p = nullptr;
+    return scratch_ptr(&p);
 }
 
 // --------------------------------------------------------------- lattice --
@@ -821,9 +855,13 @@ TermsView op_terms(const ed_op_s* op) {
     return TermsView{op->lam, op->Ch, op->ChCh, op->nb, op->nc, op->nw, op->d_b, op->d_c, op->d_w};
 }
 
+// ext (with ext_bytes) replaces the operator's gather buffers: the gather
+// stream is built there by every call (zero fill where the layout has holes,
+// then the prescale), nothing is allocated and the kernel variant is not
+// timed (the tuned one, else the default).
 template <typename VT>
 static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t ev0 = nullptr,
-                     hipEvent_t ev1 = nullptr) {
+                     hipEvent_t ev1 = nullptr, VT* ext = nullptr, size_t ext_bytes = 0) {
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
     if (D == 0) return ED_OK;
@@ -841,7 +879,17 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
         return ED_OK;
     };
     VT* u = nullptr;
-    if (img) {
+    if (img && ext) {
+        const uint64_t ne = (uint64_t)(B->img_nslot + 1) << B->img_cb;
+        if (ne * sizeof(VT) > ext_bytes)
+            return ed_fail(ED_ENOMEM, "gather image of %llu elements (%llu bytes) exceeds the %llu-byte scratch",
+                           (unsigned long long)ne, (unsigned long long)(ne * sizeof(VT)),
+                           (unsigned long long)ext_bytes);
+        hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, ext);
+        hipLaunchKernelGGL(k_prescale_img<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab,
+                           B->d_img_pos, v, ext);
+        u = ext;
+    } else if (img) {
         // the compiled rotation kernels gather from the image; its zero
         // positions are written once, at allocation
         VT*& im = std::is_same<VT, cd2>::value ? (VT*&)op->d_img128 : (VT*&)op->d_img64;
@@ -858,7 +906,17 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     }
     bool wimg = false;
     bool& wimg_off = op->wimg_off[std::is_same<VT, cd2>::value ? 0 : 1];
-    if (rows2 && B->wimg_cb >= 0 && !wimg_off && op->tune_path != 8) {
+    if (ext && rows2 && B->wimg_cb >= 0 && op->tune_path != 8 &&
+        ((B->wimg_nslot + 1) << B->wimg_cb) * sizeof(VT) <= ext_bytes) {
+        // the wide image when it fits the scratch, else u by row below
+        // (both layouts give the same sums)
+        const uint64_t ne = (B->wimg_nslot + 1) << B->wimg_cb;
+        hipLaunchKernelGGL(k_zero<VT>, dim3(sblocks(ne, 256)), dim3(256), 0, st, ne, ext);
+        hipLaunchKernelGGL(k_prescale_wimg<VT>, dim3(sblocks((uint64_t)D, 256)), dim3(256), 0, st, (uint64_t)D,
+                           B->d_rows, B->d_nitab, B->rinfo, B->d_rtab, B->d_wimg_map, B->wimg_cb, v, ext);
+        u = ext;
+        wimg = true;
+    } else if (!ext && rows2 && B->wimg_cb >= 0 && !wimg_off && op->tune_path != 8) {
         // the two-row kernels gather from the wide image when it fits the
         // free device memory, else (for the operator's lifetime) from u by
         // row through the directory
@@ -882,7 +940,13 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
             wimg = true;
         }
     }
-    if (!img && !wimg) {
+    if (!img && !wimg && ext) {
+        if ((uint64_t)D * sizeof(VT) > ext_bytes)
+            return ed_fail(ED_ENOMEM, "gather stream of %lld elements exceeds the %llu-byte scratch", (long long)D,
+                           (unsigned long long)ext_bytes);
+        hipLaunchKernelGGL(k_prescale<VT>, dim3(nblocks(D, 256)), dim3(256), 0, st, D, B->d_rows, B->d_nitab, v, ext);
+        u = ext;
+    } else if (!img && !wimg) {
         VT*& uu = std::is_same<VT, cd2>::value ? (VT*&)op->d_u128 : (VT*&)op->d_u64;
         if (!uu) {
             int rc = may_alloc();
@@ -898,7 +962,7 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 256)), blk(256);
-    if (op->use_torus && op->tune_path != 1 && op->tune_path != 6 && op->tune_path != 8) {
+    if (!ext && op->use_torus && op->tune_path != 1 && op->tune_path != 6 && op->tune_path != 8) {
         int rc = torus_prepare<VT>(op, u, v, w, st);
         if (rc != ED_OK) return rc;
     }
@@ -935,6 +999,20 @@ extern "C" int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stre
     if (!op || (!v && op->b->D) || (!w && op->b->D)) return ed_fail(ED_EINVAL, "null argument");
     if (v == w && op->b->D) return ed_fail(ED_EINVAL, "v and w must not alias");
     return op_hv(op, dtype, v, w, (hipStream_t)stream);
+}
+
+extern "C" int ed_hv_noalloc(ed_op op, int32_t dtype, const void* v, void* w, void* stream) {
+    if (!op || (!v && op->b->D) || (!w && op->b->D)) return ed_fail(ED_EINVAL, "null argument");
+    if (v == w && op->b->D) return ed_fail(ED_EINVAL, "v and w must not alias");
+    if (dtype != ED_C128 && dtype != ED_C64) return ed_fail(ED_EINVAL, "unknown dtype %d", dtype);
+    if (op->b->D == 0) return ED_OK;
+    void* ptr = nullptr;
+    int rc = scratch_ptr(&ptr);
+    if (rc != ED_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == ED_C128)
+        return launch_hv<cd2>(op, (const cd2*)v, (cd2*)w, st, nullptr, nullptr, (cd2*)ptr, kNoAllocScratchBytes);
+    return launch_hv<cf2>(op, (const cf2*)v, (cf2*)w, st, nullptr, nullptr, (cf2*)ptr, kNoAllocScratchBytes);
 }
 
 extern "C" int ed_hv_profile(ed_op op, int32_t dtype, const void* v, void* w, void* stream, float* kernel_ms) {

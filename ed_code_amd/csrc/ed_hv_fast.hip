@@ -204,8 +204,11 @@ static size_t torus_c_smem_bytes(const ed_op_s* op);
 // basis when it fits the LDS map and the 32-bit byte offsets of the loads)
 // and keep their tables in at most 64 KiB of LDS; other operators run the
 // run-time-shape kernel.
+#ifndef ED_TORUS_LDS_MAX
+#define ED_TORUS_LDS_MAX (64 * 1024)
+#endif
 static bool compiled_fits(const ed_op_s* op) {
-    return torus_shape_compiled(op->torus) && op->b->img_cb >= 0 && torus_c_smem_bytes(op) <= 64 * 1024;
+    return torus_shape_compiled(op->torus) && op->b->img_cb >= 0 && torus_c_smem_bytes(op) <= ED_TORUS_LDS_MAX;
 }
 
 // Whether the next H*v of this operator runs a compiled-shape kernel, whose

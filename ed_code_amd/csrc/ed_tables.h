@@ -164,7 +164,10 @@ struct ImageLayoutT {
 // in L2 and the Infinity Cache.
 using ImageLayout = ImageLayoutT<uint16_t>;
 using WideImageLayout = ImageLayoutT<uint32_t>;
-constexpr uint64_t kImageMapMax = 3072;
+#ifndef ED_IMAGE_MAP_MAX
+#define ED_IMAGE_MAP_MAX 3072
+#endif
+constexpr uint64_t kImageMapMax = ED_IMAGE_MAP_MAX;
 constexpr uint64_t kWideMapMax = uint64_t(1) << 22;
 
 // Smallest chunk size whose map fits max_entries (at least one 64-rank

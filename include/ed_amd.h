@@ -139,6 +139,19 @@ int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total);
  * different streams race on it, so order them (one stream, or events), or
  * give each stream its own operator. */
 int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stream);
+/* ed_hv that allocates nothing (no counterpart in the reference: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and result as ed_hv, but the gather stream (u = v/norm by row, or the
+ * sector's gather image) is built by every call in a fixed 64 MiB device
+ * array of the library (one per GPU, held until the process ends) instead
+ * of the operator's buffers, and the kernel variant is not timed (the one an
+ * earlier ed_hv tuned, else the default).  Fails with ED_ENOMEM, before
+ * launching anything, when the stream exceeds 64 MiB: 4,194,304 complex128
+ * or 8,388,608 complex64 elements of the layout (the gather image holds
+ * about 2-3 D elements).  All calls on one GPU share the array: calls that
+ * may overlap on one GPU race, so serialise them (one stream, or events). */
+int ed_hv_noalloc(ed_op op, int32_t dtype, const void* v, void* w, void* stream);
 /* ed_hv with the dominant kernel bracketed by HIP events on `stream`:
  * *kernel_ms receives the duration of the row kernel alone (the v/norm
  * prescale pass is excluded).  Synchronises the stream.  No reference
