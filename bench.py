@@ -67,14 +67,37 @@ def cpu_baseline(model, basis, kx, ky, budget_s):
     per_row = max(dt / rows, 1e-9)
     rows = int(min(D, max(rows, budget_s / per_row)))
     t1 = time.time()
-    R.hv_rows(v, rows)
+    w_ref = R.hv_rows(v, rows)
     dt = time.time() - t1
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     hv_per_s = rows / dt / D
+    # the GPU's complex128 H*v of the same vector against the reference's, on
+    # the sampled rows (the same kernel path the timed region runs)
+    import torch
+    from ed_code_amd import Operator
+    op = Operator(basis, model)
+    w_gpu = op.hv(torch.from_numpy(v).to("cuda")).cpu().numpy()
+    relerr = float(np.linalg.norm(w_gpu[:rows] - w_ref[:rows]) / np.linalg.norm(w_ref[:rows]))
+    w64 = op.hv(torch.from_numpy(v).to("cuda").to(torch.complex64)).to(torch.complex128).cpu().numpy()
+    relerr64 = float(np.linalg.norm(w64[:rows] - w_ref[:rows]) / np.linalg.norm(w_ref[:rows]))
+    del op
     return {"value": hv_per_s, "unit": "H·v/s", "cores": threads, "kind": "reference",
+            "cpu_model": cpu_model(),
             "sample": f"reference sHv (H_v.cpp:192-357, built from the reference sources) on rows [0,{rows}) of "
                       f"{D} (complex128, OpenMP {threads} threads, {dt:.1f}s); tables from the GPU export, "
-                      f"bit-equal to pConstruct_States"}
+                      f"bit-equal to pConstruct_States",
+            "gpu_vs_reference_relerr": {"c128": relerr, "c64": relerr64, "rows": rows}}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def _pmc_run(counter, extra, tag, tmp):
@@ -180,6 +203,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 traffic passes")
     ap.add_argument("--lanczos-iters", type=int, default=300,
                     help="c128 vs c64 full-reorthogonalisation Lanczos length (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the seeded serving stream run beside the plain loop (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean arrivals per second of that stream")
     ap.add_argument("--inner", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--tune", default="", help="fixed kernel 'path,block' (ed_op_tune) instead of the autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -304,12 +330,18 @@ def main():
                                 mode=ed.ED_LZ_FULL, selective=sel)
             el = time.perf_counter() - t1
             lanczos[name] = {"seconds": el, "E0": float(ev[0]), "hv_seconds": st["hv_seconds"],
-                             "hv_calls": st["hv_calls"], "residual": st["residual"][0], "rayleigh": st["rayleigh"][0]}
+                             "hv_calls": st["hv_calls"], "residual": st["residual"][0], "rayleigh": st["rayleigh"][0],
+                             "E0_error_bound": st["eval_error"][0]}
             log(f"[bench] lanczos {name}: {lanczos[name]}")
         lanczos["speedup_c64_vs_c128"] = lanczos["c128"]["seconds"] / lanczos["c64"]["seconds"]
         lanczos["speedup_c64_vs_c128_selective"] = (lanczos["c128_selective"]["seconds"] /
                                                     lanczos["c64_selective"]["seconds"])
         lanczos["E0_rel_diff_c64_vs_c128"] = abs(lanczos["c64"]["E0"] - lanczos["c128"]["E0"]) / abs(lanczos["c128"]["E0"])
+        # each complex64 eigenvalue against the FP64 (complex128) solve of the same schedule
+        lanczos["E0_rel_err_c64_vs_fp64"] = {
+            "two_pass": abs(lanczos["c64"]["E0"] - lanczos["c128"]["E0"]) / abs(lanczos["c128"]["E0"]),
+            "selective": abs(lanczos["c64_selective"]["E0"] - lanczos["c128_selective"]["E0"]) /
+                         abs(lanczos["c128_selective"]["E0"])}
 
     main_dt = args.dtype if args.dtype in results else "c64"
     r = results[main_dt]
@@ -324,6 +356,19 @@ def main():
     # (every distinct target is the orbit minimum over |G| translations)
     secondary = {"group_order": basis.lattice.L1 * basis.lattice.L2,
                  "candidate_translations_per_s": G * basis.lattice.L1 * basis.lattice.L2 / (r["kernel_ms"] * 1e-3)}
+    # measured ceiling of the gathers: random 128-byte line fills from a
+    # buffer the size of the gather image (2.7 D elements at the 32-site
+    # torus), one element per lane, the kernel's element count
+    try:
+        eb = 8 if main_dt == "c64" else 16
+        rate, cms = gather_ceiling(max(int(2.7 * D), 1 << 22), eb, T, group=1)
+        ceil = {"random_line_fills_per_s": rate, "bytes_per_s": rate * 128, "elements": T, "ms": cms}
+        if traffic is not None:
+            kl = traffic["fetch_bytes"] / 128 / (r["kernel_ms"] * 1e-3)
+            ceil.update(kernel_line_fills_per_s=kl, kernel_frac_of_ceiling=kl / rate)
+        secondary["gather_ceiling"] = ceil
+    except Exception as e:  # reported, never required
+        log(f"[bench] gather ceiling failed: {e!r}")
     if not args.no_traffic:
         try:
             insts = measure_valu(args.workload, main_dt, r["kernel"], r["tune"])
@@ -336,6 +381,20 @@ def main():
             log(f"[bench] secondary {secondary}")
         except Exception as e:  # counters are reported, never required
             log(f"[bench] VALU measurement failed: {e!r}")
+    serving = None
+    if args.serve_requests > 0:
+        # the serving half of the metric: a seeded Poisson stream over the
+        # 24-site chain, the 27- and 30-site triangular tori and the 32-site
+        # square torus (random Sz, random allowed momentum, couplings drawn,
+        # nev in {1, 4}, tol 1e-10) through the server, and the same stream
+        # through the one-at-a-time plain loop
+        from ed_code_amd import serve as S
+        torch.cuda.empty_cache()
+        t1 = time.perf_counter()
+        serving = S.run_stream(args.serve_requests, args.serve_rate, seed=11,
+                               lattices=["chain24", "tri3x9", "tri6x5", "sq4x8"], workers=4)
+        serving["seconds"] = time.perf_counter() - t1
+        log(f"[bench] serving {serving}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -365,6 +424,7 @@ def main():
                      "secondary": secondary},
         "cpu_baseline": cpu,
         "lanczos": lanczos,
+        "serving": serving,
         "other_precision": {k: v for k, v in results.items() if k != main_dt},
         "basis_build_s": t_basis,
     }

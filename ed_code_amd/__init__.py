@@ -325,7 +325,8 @@ class Operator:
                                C.byref(st)))
         n = min(nev, self.D)
         stats = dict(hv_calls=st.hv_calls, hv_seconds=st.hv_seconds, total_seconds=st.total_seconds,
-                     residual=list(st.residual[:min(n, 16)]), rayleigh=list(st.rayleigh[:min(n, 16)]))
+                     residual=list(st.residual[:min(n, 16)]), rayleigh=list(st.rayleigh[:min(n, 16)]),
+                     eval_error=list(st.eval_error[:min(n, 64)]))
         return (ev[:n], vecs, stats) if want_vectors else (ev[:n], stats)
 
     def __del__(self):
@@ -384,8 +385,12 @@ class BlockOperator:
         ev = np.zeros((self.M, n))
         st = LanczosStats()
         check(lib().ed_block_lanczos(self.h, C.byref(p), ev.ctypes.data, C.byref(st)))
+        err = np.zeros(self.M * n)
+        k = min(self.M * n, 64)
+        err[:k] = st.eval_error[:k]
         stats = dict(hv_calls=st.hv_calls, hv_seconds=st.hv_seconds, total_seconds=st.total_seconds,
-                     residual=list(st.residual[:min(self.M, 16)]), rayleigh=list(st.rayleigh[:min(self.M, 16)]))
+                     residual=list(st.residual[:min(self.M, 16)]), rayleigh=list(st.rayleigh[:min(self.M, 16)]),
+                     eval_error=err.reshape(self.M, n))
         return ev, stats
 
     def __del__(self):

@@ -39,7 +39,7 @@ class LanczosParams(C.Structure):
 
 class LanczosStats(C.Structure):
     _fields_ = [("hv_calls", C.c_int64), ("hv_seconds", C.c_double), ("total_seconds", C.c_double),
-                ("residual", C.c_double * 16), ("rayleigh", C.c_double * 16)]
+                ("residual", C.c_double * 16), ("rayleigh", C.c_double * 16), ("eval_error", C.c_double * 64)]
 
 
 # Every symbol include/ed_amd.h declares, with (restype, argtypes).

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -397,7 +398,21 @@ __global__ void k_make_h(int64_t D, const uint64_t* rows, Geo G, RankInfo R, Tab
 
 static inline unsigned nblocks(uint64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 // grid for the grid-stride kernels: at most 2^22 blocks (2^30 work-items)
-static inline unsigned sblocks(uint64_t n, unsigned bs) { return (unsigned)std::min<uint64_t>((n + bs - 1) / bs, 1u << 22); }
+// Block count of a grid-stride launch: one element per lane up to the cap,
+// then every lane loops (at 40 sites the basis kernels see 2^31 work-items).
+// The cap is 2^22 blocks; the environment variable ED_GRID_CAP lowers it
+// (read once), so that small sectors run the same loops in tests.
+static unsigned grid_cap() {
+    static const unsigned cap = [] {
+        const char* e = std::getenv("ED_GRID_CAP");
+        unsigned long v = e ? std::strtoul(e, nullptr, 10) : 0ul;
+        return (v >= 1 && v < (1ul << 22)) ? (unsigned)v : (1u << 22);
+    }();
+    return cap;
+}
+static inline unsigned sblocks(uint64_t n, unsigned bs) {
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + bs - 1) / bs, grid_cap()));
+}
 
 // ------------------------------------------------------------------ basis --
 int basis_geo(const ed_basis_s* B, Geo& G) {

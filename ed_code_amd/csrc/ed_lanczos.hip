@@ -1465,14 +1465,16 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
         rc = p->precision == ED_C128 ? run_trl<cd2>(c, *p, prev, evals) : run_trl<cf2>(c, *p, prev, evals);
     else
         rc = p->precision == ED_C128 ? run_full<cd2>(c, *p, prev, evals) : run_full<cf2>(c, *p, prev, evals);
+    if (stats) std::memset(stats->eval_error, 0, sizeof(stats->eval_error));
     if (rc == ED_OK) {
         for (size_t k = 0; k < prev.size(); k++) {
             double rq = 0, res = 0;
-            if (p->precision == ED_C64 || p->mode == ED_LZ_LEAN || (stats && k < 16)) {
+            if (p->precision == ED_C64 || p->mode == ED_LZ_LEAN || (stats && k < 64)) {
                 rc = fp64_check(c, prev[k], rq, res);
                 if (rc != ED_OK) break;
                 if (p->precision == ED_C64 || p->mode == ED_LZ_LEAN) evals[k] = rq;   // FP64 Rayleigh-quotient correction
                 if (stats && k < 16) { stats->residual[k] = res; stats->rayleigh[k] = rq; }
+                if (stats && k < 64) stats->eval_error[k] = res;
             }
             if (evecs) {
                 hipError_t e = hipMemcpy(evecs + 2 * D * k, prev[k], D * sizeof(cd2), hipMemcpyDeviceToHost);
@@ -1537,19 +1539,33 @@ extern "C" int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double
         }
     }
     const int how_many = (int)std::min<int64_t>(p->nev, D);
+    std::vector<double> err((size_t)M * how_many, 0.0);
     if (rc == ED_OK) {
         for (size_t k = 0; k < prev.size(); k++) {
-            if (p->precision != ED_C64 && !(stats && k == 0)) continue;
+            if (p->precision != ED_C64 && !stats) continue;
             std::vector<double> rq, res;
             rc = bfp64_check(c, prev[k], rq, res);
             if (rc != ED_OK) break;
             for (int m = 0; m < M; m++) {
                 if (p->precision == ED_C64) evals[(size_t)m * how_many + k] = rq[m];
                 if (stats && k == 0 && m < 16) { stats->residual[m] = res[m]; stats->rayleigh[m] = rq[m]; }
+                if (k < (size_t)how_many) err[(size_t)m * how_many + k] = res[m];
             }
         }
-        // each instance's eigenvalues in ascending order
-        for (int m = 0; m < M; m++) std::sort(evals + (size_t)m * how_many, evals + (size_t)(m + 1) * how_many);
+        // each instance's eigenvalues in ascending order, their error bounds with them
+        for (int m = 0; m < M; m++) {
+            std::vector<std::pair<double, double>> pr(how_many);
+            for (int k = 0; k < how_many; k++) pr[k] = {evals[(size_t)m * how_many + k], err[(size_t)m * how_many + k]};
+            std::sort(pr.begin(), pr.end());
+            for (int k = 0; k < how_many; k++) {
+                evals[(size_t)m * how_many + k] = pr[k].first;
+                err[(size_t)m * how_many + k] = pr[k].second;
+            }
+        }
+    }
+    if (stats) {
+        std::memset(stats->eval_error, 0, sizeof(stats->eval_error));
+        for (size_t q = 0; q < err.size() && q < 64; q++) stats->eval_error[q] = err[q];
     }
     for (auto* v : prev) dfree(v);
     dfree(c.d_partial); dfree(c.d_sum); dfree(c.d_coef); dfree(c.d_t); dfree(c.d_s);

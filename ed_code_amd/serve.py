@@ -15,7 +15,9 @@ requests:
   lowest-eigenvalue request runs the graph-captured solver (no host round
   trip per step);
 * HBM admission -- a job starts only when its footprint (basis, Krylov
-  vectors, work vectors) fits next to the running jobs under the budget.
+  vectors, work vectors) fits next to the running jobs under the budget, by
+  default 85 % of the device memory free when the server starts
+  (hipMemGetInfo, through torch).
 * Exclusive large jobs -- a job on a sector of at least --exclusive-rows rows
   (the 32-site torus) runs with the GPU to itself: its H*v already fills the
   device, and sharing only stretches its latency.
@@ -62,6 +64,7 @@ class Request:
     arrival: float
     done: Optional[float] = None
     evals: Optional[List[float]] = None
+    errors: Optional[List[float]] = None   # FP64 residual bound of each eigenvalue
     batch: int = 0
 
 
@@ -179,24 +182,37 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
     models = [model_for(r.lattice, r.J2, r.lam, r.Ch) for r in reqs]
     if len(reqs) == 1 and reqs[0].nev == 1:
         op = Operator(basis, models[0])
-        e, _ = op.lanczos(nev=1, iterations=max(iterations, 400), ncycles=1, tol=reqs[0].tol, precision=ED_C64,
-                          mode=ED_LZ_LEAN, graph=True)
+        e, st = op.lanczos(nev=1, iterations=max(iterations, 400), ncycles=1, tol=reqs[0].tol, precision=ED_C64,
+                           mode=ED_LZ_LEAN, graph=True)
         ev = np.array([[e[0]]])
+        err = np.array([[st["eval_error"][0]]])
     else:
         blk = BlockOperator(basis, models)
-        ev, _ = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
+        ev, st = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
+        err = st["eval_error"]
     now = time.perf_counter()
     for m, r in enumerate(reqs):
         r.evals = [float(x) for x in ev[m]]
+        r.errors = [float(x) for x in err[m]]
         r.done = now
         r.batch = len(reqs)
     print(f"[serve] answered {[r.rid for r in reqs]} {reqs[0].lattice} D={basis.D} batch {len(reqs)}",
           file=sys.stderr, flush=True)
 
 
+def device_budget(fraction: float = 0.85) -> int:
+    """Bytes of device memory the server may plan with: the given fraction of
+    what the device reports free now (hipMemGetInfo)."""
+    import torch
+    free, _total = torch.cuda.mem_get_info()
+    return int(free * fraction)
+
+
 class Server:
-    def __init__(self, workers: int = 4, hbm_budget: int = 200 * GiB, cache_budget: int = 24 * GiB,
+    def __init__(self, workers: int = 4, hbm_budget: Optional[int] = None, cache_budget: int = 24 * GiB,
                  max_batch: int = 8, iterations: int = 300, exclusive_rows: int = 1 << 24):
+        if hbm_budget is None:
+            hbm_budget = device_budget()
         self.workers = workers
         # a job of at least this many rows runs with the GPU to itself (its
         # H*v saturates the device; sharing only stretches its latency)
@@ -287,11 +303,42 @@ def run_baseline(requests: List[Request], iterations: int = 300) -> Tuple[float,
 def summarize(requests: List[Request], wall: float, t_base: float) -> dict:
     lat = np.array([(r.done - t_base) - r.arrival for r in requests if r.done is not None])
     batches = collections.Counter(r.batch for r in requests if r.done is not None)
+    errs = [e for r in requests if r.errors for e in r.errors]
     return dict(answered=int(len(lat)), wall_seconds=wall, solves_per_s=len(lat) / wall if wall else 0.0,
                 p50_s=float(np.percentile(lat, 50)) if len(lat) else None,
                 p99_s=float(np.percentile(lat, 99)) if len(lat) else None,
                 mean_s=float(lat.mean()) if len(lat) else None,
+                max_eval_error_bound=max(errs) if errs else None,
                 batch_sizes={int(k): int(v) for k, v in sorted(batches.items())})
+
+
+def run_stream(requests: int, rate: float, seed: int, lattices: List[str], workers: int = 4,
+               iterations: int = 300, k_zero: bool = False, exclusive_rows: int = 1 << 24,
+               baseline: bool = True) -> dict:
+    """One seeded request stream through the server and, on the same stream,
+    the plain loop: latency percentiles, throughput, per-eigenvalue FP64
+    error bounds, and the largest difference between the two answers."""
+    reqs = make_requests(requests, rate, seed, lattices, k_zero)
+    srv = Server(workers=workers, iterations=iterations, exclusive_rows=exclusive_rows or (1 << 62))
+    # warm-up outside the timed runs (GPU context, library load, first kernels)
+    warm = Request(-1, "chain24", 0.0, 0, 0, 0.1, 1.0, 0.0, 1, 1e-10, 0.0)
+    solve(srv.cache.get(sector_key(warm)), [warm], iterations)
+    t0, wall = srv.run(reqs)
+    out = dict(config=dict(requests=requests, rate=rate, seed=seed, lattices=lattices, workers=workers,
+                           k_zero=k_zero, exclusive_rows=exclusive_rows, hbm_budget_bytes=srv.budget),
+               server=summarize(reqs, wall, t0), cache=dict(hits=srv.cache.hits, misses=srv.cache.misses),
+               errors=srv.errors)
+    served = {r.rid: r.evals for r in reqs}
+    if baseline:
+        base = make_requests(requests, rate, seed, lattices, k_zero)
+        t0b, wall_b = run_baseline(base, iterations)
+        out["baseline"] = summarize(base, wall_b, t0b)
+        diff = 0.0
+        for r in base:
+            if r.evals is not None and served.get(r.rid) is not None:
+                diff = max(diff, max(abs(x - y) for x, y in zip(r.evals, served[r.rid])))
+        out["max_abs_eval_diff_vs_baseline"] = diff
+    return out
 
 
 def main(argv=None):
@@ -308,27 +355,8 @@ def main(argv=None):
                     help="jobs on sectors of at least this many rows run alone (0: never)")
     a = ap.parse_args(argv)
     init(0)
-    lats = a.lattices.split(",")
-    reqs = make_requests(a.requests, a.rate, a.seed, lats, a.k_zero)
-    srv = Server(workers=a.workers, iterations=a.iterations, exclusive_rows=a.exclusive_rows or (1 << 62))
-    # warm-up outside the timed runs (GPU context, library load, first kernels)
-    warm = Request(-1, "chain24", 0.0, 0, 0, 0.1, 1.0, 0.0, 1, 1e-10, 0.0)
-    solve(srv.cache.get(sector_key(warm)), [warm], a.iterations)
-    t0, wall = srv.run(reqs)
-    out = dict(config=dict(requests=a.requests, rate=a.rate, lattices=lats, workers=a.workers, k_zero=a.k_zero,
-                           exclusive_rows=a.exclusive_rows),
-               server=summarize(reqs, wall, t0), cache=dict(hits=srv.cache.hits, misses=srv.cache.misses),
-               errors=srv.errors)
-    served = {r.rid: r.evals for r in reqs}
-    if not a.no_baseline:
-        base = make_requests(a.requests, a.rate, a.seed, lats, a.k_zero)
-        t0b, wall_b = run_baseline(base, a.iterations)
-        out["baseline"] = summarize(base, wall_b, t0b)
-        diff = 0.0
-        for r in base:
-            if r.evals is not None and served.get(r.rid) is not None:
-                diff = max(diff, max(abs(x - y) for x, y in zip(r.evals, served[r.rid])))
-        out["max_abs_eval_diff_vs_baseline"] = diff
+    out = run_stream(a.requests, a.rate, a.seed, a.lattices.split(","), a.workers, a.iterations, a.k_zero,
+                     a.exclusive_rows, baseline=not a.no_baseline)
     print(json.dumps(out), flush=True)
 
 

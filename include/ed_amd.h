@@ -262,6 +262,12 @@ typedef struct ed_lanczos_stats {
     double total_seconds;
     double residual[16];    /* FP64 ||H x - E x|| of the first 16 eigenvectors */
     double rayleigh[16];    /* FP64 Rayleigh quotient x^H H x / x^H x */
+    /* Error bound of every returned eigenvalue against the exact one, in
+     * FP64: the FP64 residual ||H x - E x|| of its unit Ritz vector x (for
+     * Hermitian H some eigenvalue lies within that distance of E).  ed_lanczos:
+     * [k] for k < min(nev, 64); ed_block_lanczos: [m*nev + k] in the order of
+     * evals, for m*nev + k < 64.  0 where nothing is returned. */
+    double eval_error[64];
 } ed_lanczos_stats;
 
 /* evals [nev]; evecs (host, complex128 [nev][D]) may be NULL; stats may be NULL. */

@@ -1,4 +1,8 @@
-"""GPU Lanczos against dense diagonalisation, closed forms and the oracle."""
+"""GPU Lanczos against dense diagonalisation, closed forms and the oracle.
+Bar: complex128 Krylov storage within 1e-10 relative of the exact
+eigenvalues (the drop-in contract), complex64 storage with the FP64
+correction within 1e-9 relative (the precision target); every eigenvalue
+within the FP64 error bound the solver reports for it."""
 import math
 
 import numpy as np
@@ -17,6 +21,19 @@ def _ed():
     return ed
 
 
+def _bar(prec):
+    return 1e-10 if prec == "c128" else 1e-9
+
+
+def _within_bounds(lz, exact, st):
+    """Each returned eigenvalue lies within its reported FP64 error bound of
+    the exact one (the bound is the residual of its unit Ritz vector)."""
+    err = np.abs(np.asarray(lz) - np.asarray(exact))
+    bound = np.array(st["eval_error"][:len(err)])
+    assert np.all(bound > 0)
+    assert np.all(err <= bound + 1e-12), (err, bound)
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 @pytest.mark.parametrize("mode", ["reference", "full"])
 def test_lanczos_vs_dense(gpu_ready, prec, mode):
@@ -29,11 +46,10 @@ def test_lanczos_vs_dense(gpu_ready, prec, mode):
     op = ed.Operator(basis, model)
     pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
     md = ed.ED_LZ_REFERENCE if mode == "reference" else ed.ED_LZ_FULL
-    lz, st = op.lanczos(nev=3, iterations=80, ncycles=4, tol=1e-12, seed=7, precision=pr, mode=md)
-    tol = 1e-10
-    assert abs(lz[0] - ev[0]) < tol
-    assert abs(lz[1] - ev[1]) < 1e-8 and abs(lz[2] - ev[2]) < 1e-8
+    lz, st = op.lanczos(nev=3, iterations=120, ncycles=6, tol=1e-12, seed=7, precision=pr, mode=md)
+    assert np.abs(lz - ev[:3]).max() < _bar(prec) * np.abs(ev[:3]).max()
     assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-3)
+    _within_bounds(lz, ev[:3], st)
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
@@ -49,8 +65,9 @@ def test_lanczos_selective_reorthogonalisation(gpu_ready, prec):
     pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
     lz, st = op.lanczos(nev=3, iterations=200, ncycles=1, tol=1e-13, seed=5, precision=pr, mode=ed.ED_LZ_FULL,
                         selective=True)
-    assert np.abs(lz - ev[:3]).max() < 1e-9
+    assert np.abs(lz - ev[:3]).max() < _bar(prec) * np.abs(ev[:3]).max()
     assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-3)
+    _within_bounds(lz, ev[:3], st)
 
 
 def test_lanczos_chain24_ground_state(gpu_ready):
@@ -94,7 +111,8 @@ def test_lean_lanczos_vs_dense(gpu_ready, prec):
     op = ed.Operator(basis, model)
     pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
     lz, st = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=9, precision=pr, mode=ed.ED_LZ_LEAN)
-    assert abs(lz[0] - e_dense) < 1e-9 * abs(e_dense)
+    assert abs(lz[0] - e_dense) < _bar(prec) * abs(e_dense)
+    _within_bounds(lz, [e_dense], st)
     assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-4)
     with pytest.raises(ed.EDError):
         op.lanczos(nev=2, iterations=50, mode=ed.ED_LZ_LEAN)
@@ -133,8 +151,8 @@ def test_graph_captured_lean_lanczos(gpu_ready, prec):
     for _ in range(2):
         g, st = op.lanczos(nev=1, iterations=300, ncycles=1, tol=1e-13, seed=9, precision=pr, mode=ed.ED_LZ_LEAN,
                            graph=True)
-        assert abs(g[0] - e_dense) < 1e-9 * abs(e_dense)
-        assert abs(g[0] - eager[0]) < 1e-9 * abs(e_dense)
+        assert abs(g[0] - e_dense) < _bar(prec) * abs(e_dense)
+        assert abs(g[0] - eager[0]) < _bar(prec) * abs(e_dense)
         assert st["residual"][0] < (1e-6 if prec == "c128" else 1e-4)
         assert st["hv_calls"] > 30
 
@@ -153,7 +171,8 @@ def test_thick_restart_lanczos_vs_dense(gpu_ready, prec):
     op = ed.Operator(basis, model)
     ev, vecs, st = op.lanczos(nev=4, iterations=16, ncycles=400, tol=1e-11, seed=3, mode=ed.ED_LZ_THICK,
                               precision=ed.ED_C128 if prec == "c128" else ed.ED_C64, want_vectors=True)
-    assert np.abs(ev - ref).max() < 1e-9 * max(1.0, np.abs(ref).max())
+    assert np.abs(ev - ref).max() < _bar(prec) * max(1.0, np.abs(ref).max())
+    _within_bounds(ev, ref, st)
     for i in range(4):
         x = vecs[i]
         assert abs(np.linalg.norm(x) - 1) < 1e-9

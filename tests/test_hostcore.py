@@ -2,6 +2,7 @@
 checked against the oracle and the golden fixtures.  These run the same
 functions the gfx950 kernels run, so logic errors surface without a GPU."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -258,3 +259,20 @@ def test_split_rank_tables_at_40_sites():
         assert H.hc_rank_split(r, s) == Hp.cns_rank(s)
     assert H.hc_rank_split(r, ((1 << 20) - 1) << 20) == math.comb(40, 20) - 1
     H.hc_ranker_free(r)
+
+
+def test_kernel_arithmetic_under_address_and_ub_sanitizers(tmp_path):
+    """tests/hostcore/sancheck.cpp: the host build of the kernel arithmetic
+    compiled with -fsanitize=address,undefined (aborting on the first report)
+    and run on small lattices, every code path checked against the others."""
+    import subprocess
+    src = os.path.join(Hp.HERE, "hostcore", "sancheck.cpp")
+    exe = str(tmp_path / "sancheck")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-I", os.path.join(Hp.ROOT, "ed_code_amd", "csrc"), "-I",
+                    os.path.join(Hp.HERE, "hostcore"), src, "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+                                UBSAN_OPTIONS="print_stacktrace=1"))
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "sancheck ok" in r.stdout
