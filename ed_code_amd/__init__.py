@@ -25,15 +25,25 @@ from ._lib import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_GRAPH, ED_LZ_LEAN, ED_LZ_R
                    check, lib)
 from .lattices import Model, n_ones_for_sz
 
-__all__ = ["Lattice", "Basis", "Operator", "BlockOperator", "sector_lanczos", "lattices", "Model", "EDError", "ED_C64", "ED_C128",
+__all__ = ["Lattice", "Basis", "Operator", "BlockOperator", "MultiBlockOperator", "sector_lanczos", "entanglement_entropies", "lattices", "Model",
+           "EDError", "ED_C64", "ED_C128",
            "ED_LZ_FULL", "ED_LZ_REFERENCE", "ED_LZ_LEAN", "ED_LZ_SELECTIVE", "ED_LZ_THICK", "n_ones_for_sz"]
 
 _initialised = False
 
 
 def init(device: int = 0):
+    """Selects the device.  torch's HIP runtime is started first when torch
+    sees a GPU: the library and torch each load a HIP runtime, and torch's
+    must initialise before the library places anything on the device."""
     global _initialised
     if not _initialised:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
         check(lib().ed_init(device))
         _initialised = True
 
@@ -200,10 +210,44 @@ class Basis:
                                   C.byref(n)))
         return st, co.view(np.complex128)
 
+    def density_matrix(self, cut, ket, bra=None, n_cut_ones=None):
+        """Reduced density matrix of the cut (a list of sites) from complex128
+        device vectors of this basis: dm[j, k] = sum_e ket(j, e) conj(bra(k, e))
+        over the configurations e of the other sites (bra defaults to ket),
+        computed on the GPU (density_matrixij, analysis/src/Entropy.cpp:170-205).
+        j, k are configurations of the cut, bit l = site cut[l]; with
+        n_cut_ones, the block of cut configurations with that many up spins,
+        in ascending order (density_matrixij_Sz, :207-265)."""
+        import torch
+        bra = ket if bra is None else bra
+        for x in (ket, bra):
+            if x.dtype != torch.complex128 or not x.is_cuda or not x.is_contiguous() or x.numel() != self.D:
+                raise ValueError("vectors must be contiguous complex128 CUDA tensors of length D")
+        cut = np.ascontiguousarray(np.asarray(cut, np.int32).ravel())
+        nco = -1 if n_cut_ones is None else int(n_cut_ones)
+        m = (1 << len(cut)) if nco < 0 else math.comb(len(cut), nco)
+        dm = np.zeros(2 * m * m)
+        torch.cuda.synchronize()
+        check(lib().ed_density_matrix(self.h, C.c_void_p(ket.data_ptr()), C.c_void_p(bra.data_ptr()), len(cut),
+                                      cut.ctypes.data, nco, dm.ctypes.data))
+        return dm.view(np.complex128).reshape(m, m, order="F")
+
     def __del__(self):
         if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_basis_destroy(self.h)
             self.h = None
+
+
+def entanglement_entropies(dm):
+    """(Renyi-2, von Neumann) entropies in bits of a density matrix
+    (Renyi_Entropy and Von_Neumann_Entropy, analysis/src/Entropy.cpp:307-355),
+    the eigenvalues from rocSOLVER on the GPU."""
+    a = np.asfortranarray(np.asarray(dm, np.complex128))
+    if a.ndim != 2 or a.shape[0] != a.shape[1]:
+        raise ValueError("dm must be a square matrix")
+    r2, vn = C.c_double(), C.c_double()
+    check(lib().ed_entanglement_entropies(a.ctypes.data, a.shape[0], C.byref(r2), C.byref(vn)))
+    return r2.value, vn.value
 
 
 def _terms(model: Model):
@@ -396,6 +440,82 @@ class BlockOperator:
     def __del__(self):
         if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
             lib().ed_block_destroy(self.h)
+            self.h = None
+
+
+class MultiBlockOperator:
+    """M coupling instances (models with one term structure) on each of
+    several momentum sectors (bases of one lattice and one n_ones), applied
+    in one launch over all sectors' row tiles.  Vectors are (rows, M)
+    row-major torch tensors on the GPU: the sectors' (D_s, M) blocks
+    stacked in sector order (rows = sum of D_s); sector s occupies rows
+    [row0[s], row0[s] + D_s)."""
+
+    def __init__(self, bases, models):
+        self.bases = list(bases)
+        self.models = list(models)
+        self.M = len(self.models)
+        ts, self._keep = [], []
+        for m in self.models:
+            t, keep = _terms(m)
+            ts.append(t)
+            self._keep.append(keep)
+        arr = (Terms * self.M)(*ts)
+        hb = (C.c_void_p * len(self.bases))(*[b.h.value for b in self.bases])
+        h = C.c_void_p()
+        check(lib().ed_mblock_create(hb, len(self.bases), self.M, arr, C.byref(h)))
+        self.h = h
+        self.row0 = []
+        for s in range(len(self.bases) + 1):
+            r0, d = C.c_int64(), C.c_int64()
+            check(lib().ed_mblock_rows(h, s, C.byref(r0), C.byref(d)))
+            self.row0.append(r0.value if s < len(self.bases) else d.value)
+        self.rows = self.row0[-1]
+
+    @property
+    def fast(self) -> bool:
+        return bool(lib().ed_mblock_fast(self.h))
+
+    def split(self, V):
+        """The per-sector (D_s, M) views of a stacked vector."""
+        return [V[self.row0[s]:self.row0[s + 1]] for s in range(len(self.bases))]
+
+    def hv(self, V, W=None, stream=None):
+        import torch
+        if V.dtype == torch.complex128:
+            dt = ED_C128
+        elif V.dtype == torch.complex64:
+            dt = ED_C64
+        else:
+            raise TypeError(f"unsupported dtype {V.dtype}")
+        if not V.is_cuda or not V.is_contiguous() or tuple(V.shape) != (self.rows, self.M):
+            raise ValueError("V must be a contiguous (rows, M) device tensor")
+        if W is None:
+            W = torch.empty_like(V)
+        if W.dtype != V.dtype or not W.is_contiguous() or W.shape != V.shape or W.data_ptr() == V.data_ptr():
+            raise ValueError("W must match V and not alias it")
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(lib().ed_mblock_hv(self.h, dt, C.c_void_p(V.data_ptr()), C.c_void_p(W.data_ptr()), C.c_void_p(st)))
+        return W
+
+    def lanczos(self, nev=1, iterations=200, tol=1e-10, seed=1234, precision=ED_C64):
+        """Lowest nev eigenvalues of every (sector, instance), all advanced
+        together: arrays (nsec, M, nev) of eigenvalues and of their FP64
+        error bounds, and stats."""
+        S = len(self.bases)
+        n = min(nev, min(b.D for b in self.bases))
+        p = LanczosParams(nev, iterations, 1, precision, ED_LZ_FULL, 0, tol, seed)
+        ev = np.zeros((S, self.M, n))
+        err = np.zeros((S, self.M, n))
+        st = LanczosStats()
+        check(lib().ed_mblock_lanczos(self.h, C.byref(p), ev.ctypes.data, err.ctypes.data, C.byref(st)))
+        stats = dict(hv_calls=st.hv_calls, hv_seconds=st.hv_seconds, total_seconds=st.total_seconds,
+                     eval_error=err)
+        return ev, stats
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:   # module globals are gone at interpreter exit
+            lib().ed_mblock_destroy(self.h)
             self.h = None
 
 

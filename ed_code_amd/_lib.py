@@ -72,6 +72,8 @@ SIGNATURES = {
     "ed_csr_hv_host": (C.c_int, [C.c_int64, C.c_int64, _VP, _VP, _VP, _VP, _VP]),
     "ed_pair_correlations": (C.c_int, [_VP, _VP, _VP, C.c_int32, _VP, _VP, _VP]),
     "ed_unwrap": (C.c_int, [_VP, _VP, C.c_int64, _VP, _VP, _VP]),
+    "ed_density_matrix": (C.c_int, [_VP, _VP, _VP, C.c_int32, _VP, C.c_int32, _VP]),
+    "ed_entanglement_entropies": (C.c_int, [_VP, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "ed_dense_eigh": (C.c_int, [_VP, _VP, _VP]),
     "ed_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
     "ed_block_create": (C.c_int, [_VP, C.c_int32, _VP, C.POINTER(_VP)]),
@@ -79,6 +81,12 @@ SIGNATURES = {
     "ed_block_fast": (C.c_int, [_VP]),
     "ed_block_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
     "ed_block_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, C.POINTER(LanczosStats)]),
+    "ed_mblock_create": (C.c_int, [_VP, C.c_int32, C.c_int32, _VP, C.POINTER(_VP)]),
+    "ed_mblock_destroy": (None, [_VP]),
+    "ed_mblock_rows": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "ed_mblock_fast": (C.c_int, [_VP]),
+    "ed_mblock_hv": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP]),
+    "ed_mblock_lanczos": (C.c_int, [_VP, C.POINTER(LanczosParams), _VP, _VP, C.POINTER(LanczosStats)]),
     "ed_gather_calibrate": (C.c_int, [C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_float)]),
 }

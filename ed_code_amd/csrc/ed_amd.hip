@@ -65,7 +65,9 @@ std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxScratchDevices] = {};
 
 // Device address of `scratch` on the current device, looked up once per
-// device (with that device current).
+// device (with that device current) by the first ed_hv_noalloc there: the
+// lookup loads the code object and places the array, which processes that
+// never call ed_hv_noalloc do not pay for.
 int scratch_ptr(void** out) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -88,8 +90,7 @@ extern "C" int ed_init(int device) {
     if (e != hipSuccess || n == 0) return ed_fail(ED_EDEVICE, "no HIP device: %s", hipGetErrorString(e));
     if (device < 0 || device >= n) return ed_fail(ED_EINVAL, "device %d out of range (%d devices)", device, n);
     HIPCHK(hipSetDevice(device));
-    void* p = nullptr;
-    return scratch_ptr(&p);
+    return ED_OK;
 }
 
 // --------------------------------------------------------------- lattice --

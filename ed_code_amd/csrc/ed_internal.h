@@ -153,3 +153,9 @@ template <typename VT>
 int launch_hv_rows2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wide_image);
 ed_basis_s* block_basis(const ed_block_s* K);
 int block_size(const ed_block_s* K);
+int mblock_op_hv(ed_mblock_s* MB, int dtype, const void* v, void* w, hipStream_t st);
+int mblock_nsec(const ed_mblock_s* MB);
+int mblock_set_live(ed_mblock_s* MB, const std::vector<char>& live);
+int mblock_M(const ed_mblock_s* MB);
+int64_t mblock_row0(const ed_mblock_s* MB, int s);
+ed_block_s* mblock_block(const ed_mblock_s* MB, int s);

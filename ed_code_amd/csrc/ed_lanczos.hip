@@ -1412,6 +1412,537 @@ int bfp64_check(BCtx& c, const cd2* x, std::vector<double>& rq, std::vector<doub
     return ED_OK;
 }
 
+// ---- multi-sector block Lanczos ---------------------------------------
+// run_block_full for every (sector, instance) of a multi-sector block in lock
+// step.  Krylov vector k of all sectors is one contiguous n_all-element
+// array (sector s at element offset off_s = row0_s * M, interleaved D_s x M).
+// Each step is one H*V launch over the live sectors' row tiles, then the
+// step's reductions and updates as segmented kernels: the live sectors' rows
+// cut into chunks of kSegElems elements, one workgroup per chunk, per-chunk
+// partial sums per (vector, instance) column, and one compensated sum per
+// (sector, column) -- so a step costs the same number of launches and host
+// round trips for 30 sectors as for one.  A sector leaves the launches once
+// all its instances have stopped.  Same stopping rule and Ritz vectors as
+// run_block_full; the second Gram-Schmidt pass runs for every live sector
+// when one of them needs it.
+constexpr int64_t kSegElems = int64_t(1) << 16;   // elements per chunk (a multiple of 64)
+
+struct SegChunk {
+    int32_t sec, pad;
+    int64_t e0, e1;    // elements [e0, e1) of sector sec
+};
+
+template <int MB, typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_sdot(const SegChunk* __restrict__ ch, int M,
+                                                       const XT* __restrict__ X, int64_t ldx,
+                                                       const YT* __restrict__ y, cd2* __restrict__ partial, int nch,
+                                                       int kofs) {
+    __shared__ double red[kRedThreads / 64][MB][8][2];
+    const SegChunk c = ch[blockIdx.x];
+    double are[MB], aim[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) { are[k] = 0.0; aim[k] = 0.0; }
+    for (int64_t e = c.e0 + threadIdx.x; e < c.e1; e += blockDim.x) {
+        cd2 yv = ld(y, e);
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
+            cd2 x = ld(X + k * ldx, e);
+            are[k] += x.re * yv.re + x.im * yv.im;
+            aim[k] += x.re * yv.im - x.im * yv.re;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < MB; k++) {
+        for (int off = 32; off >= M; off >>= 1) {
+            are[k] += __shfl_xor(are[k], off, 64);
+            aim[k] += __shfl_xor(aim[k], off, 64);
+        }
+        if (lane < M) { red[wv][k][lane][0] = are[k]; red[wv][k][lane][1] = aim[k]; }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < MB * M) {
+        const int k = threadIdx.x / M, m = threadIdx.x % M;
+        double r = 0.0, im = 0.0;
+        for (int q = 0; q < kRedThreads / 64; q++) { r += red[q][k][m][0]; im += red[q][k][m][1]; }
+        partial[(int64_t)((kofs + k) * M + m) * nch + blockIdx.x] = cd2{r, im};
+    }
+}
+
+// out[q * ncols + col] = compensated sum of partial[col * nch + c] over the
+// chunks c0[q] <= c < c0[q + 1] of the q-th live sector (fixed order, as
+// k_colsum).  Grid (ncols, live sectors), one wave each.
+__global__ void __launch_bounds__(64) k_scolsum(const cd2* __restrict__ partial, int nch, const int* __restrict__ c0,
+                                                int ncols, cd2* __restrict__ out) {
+    __shared__ double red[64][2];
+    const int col = blockIdx.x, q = blockIdx.y, lane = threadIdx.x;
+    const int b0 = c0[q], b1 = c0[q + 1];
+    double sr = 0, cr = 0, si = 0, ci = 0;
+    for (int b = b0 + lane; b < b1; b += 64) {
+        const cd2 p = partial[(int64_t)col * nch + b];
+        double yr = p.re - cr, tr = sr + yr;
+        cr = (tr - sr) - yr; sr = tr;
+        double yi = p.im - ci, tj = si + yi;
+        ci = (tj - si) - yi; si = tj;
+    }
+    red[lane][0] = sr - cr;
+    red[lane][1] = si - ci;
+    __syncthreads();
+    if (lane == 0) {
+        sr = 0; cr = 0; si = 0; ci = 0;
+        for (int l = 0; l < 64; l++) {
+            double yr = red[l][0] - cr, tr = sr + yr;
+            cr = (tr - sr) - yr; sr = tr;
+            double yi = red[l][1] - ci, tj = si + yi;
+            ci = (tj - si) - yi; si = tj;
+        }
+        out[(int64_t)q * ncols + col] = cd2{sr, si};
+    }
+}
+
+// y[e] = a[sec][m] y[e] - sum_k c[sec * cs + k*M + m] X_k[e] over the chunk
+template <int MB, typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_saxpy(const SegChunk* __restrict__ ch, int M,
+                                                        const XT* __restrict__ X, int64_t ldx,
+                                                        const cd2* __restrict__ c, int cs,
+                                                        const double* __restrict__ a, YT* __restrict__ y) {
+    const SegChunk sc = ch[blockIdx.x];
+    const int m = threadIdx.x % M;
+    cd2 cf[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) cf[k] = c[(int64_t)sc.sec * cs + k * M + m];
+    const double am = a ? a[sc.sec * M + m] : 1.0;
+    for (int64_t e = sc.e0 + threadIdx.x; e < sc.e1; e += blockDim.x) {
+        cd2 yv = ld(y, e);
+        double re = am * yv.re, im = am * yv.im;
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
+            cd2 x = ld(X + k * ldx, e);
+            re -= cf[k].re * x.re - cf[k].im * x.im;
+            im -= cf[k].re * x.im + cf[k].im * x.re;
+        }
+        st(y, e, re, im);
+    }
+}
+
+// y[e] = s[sec][m] x[e]
+template <typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_sscale(const SegChunk* __restrict__ ch, int M,
+                                                         const double* __restrict__ sc, const XT* x, YT* y) {
+    const SegChunk c = ch[blockIdx.x];
+    const double f = sc[c.sec * M + threadIdx.x % M];
+    for (int64_t e = c.e0 + threadIdx.x; e < c.e1; e += blockDim.x) {
+        cd2 v = ld(x, e);
+        st(y, e, f * v.re, f * v.im);
+    }
+}
+
+// y[e] += sum_k t[sec * ts + k*M + m] X_k[e]   (Ritz vectors, y FP64)
+template <int MB, typename XT>
+__global__ void __launch_bounds__(kRedThreads) k_saccum(const SegChunk* __restrict__ ch, int M,
+                                                         const XT* __restrict__ X, int64_t ldx,
+                                                         const double* __restrict__ t, int ts, cd2* __restrict__ y) {
+    const SegChunk c = ch[blockIdx.x];
+    const int m = threadIdx.x % M;
+    double tt[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) tt[k] = t[(int64_t)c.sec * ts + k * M + m];
+    for (int64_t e = c.e0 + threadIdx.x; e < c.e1; e += blockDim.x) {
+        cd2 yv = y[e];
+#pragma unroll
+        for (int k = 0; k < MB; k++) {
+            // a sector that stopped early never wrote its later Krylov
+            // vectors: their coefficient is 0 and their memory is not read
+            if (tt[k] == 0.0) continue;
+            cd2 x = ld(X + k * ldx, e);
+            yv.re += tt[k] * x.re;
+            yv.im += tt[k] * x.im;
+        }
+        y[e] = yv;
+    }
+}
+
+// The segmented context: chunks of the live sectors (or of all sectors) and
+// the buffers of the reductions.
+struct SCtx {
+    ed_mblock_s* mb;
+    int S, M;
+    std::vector<int64_t> off;        // [S + 1] element offsets
+    int64_t n_all = 0;
+    hipStream_t st;
+    std::vector<SegChunk> ch;        // current chunk list (live sectors, in sector order)
+    std::vector<int> c0;             // first chunk of each listed sector, [nlist + 1]
+    std::vector<int> list;           // the listed sectors
+    SegChunk* d_ch = nullptr;
+    int* d_c0 = nullptr;
+    int max_ch = 0;
+    cd2* d_partial = nullptr;        // [kBCols * max_ch]
+    cd2* d_sum = nullptr;            // [kBCols * S]
+    cd2* d_coef = nullptr;           // [S * cap * M]
+    double* d_t = nullptr;           // [S * cap * M]
+    double* d_s = nullptr;           // [S * M]
+    int cap = 0;
+    double hv_ms = 0.0;
+    int64_t hv_calls = 0;
+    hipEvent_t e0, e1;
+};
+
+int s_list(SCtx& c, const std::vector<char>& live) {
+    c.ch.clear();
+    c.c0.clear();
+    c.list.clear();
+    for (int s = 0; s < c.S; s++) {
+        if (!live[s]) continue;
+        c.list.push_back(s);
+        c.c0.push_back((int)c.ch.size());
+        for (int64_t e = c.off[s]; e < c.off[s + 1]; e += kSegElems)
+            c.ch.push_back(SegChunk{s, 0, e, std::min(e + kSegElems, c.off[s + 1])});
+    }
+    c.c0.push_back((int)c.ch.size());
+    if ((int)c.ch.size() > c.max_ch) return ed_fail(ED_EINVAL, "chunk list exceeds its buffer");
+    if (!c.ch.empty()) {
+        HIPCHK(hipMemcpyAsync(c.d_ch, c.ch.data(), c.ch.size() * sizeof(SegChunk), hipMemcpyHostToDevice, c.st));
+        HIPCHK(hipMemcpyAsync(c.d_c0, c.c0.data(), c.c0.size() * sizeof(int), hipMemcpyHostToDevice, c.st));
+    }
+    return ED_OK;
+}
+
+int pass_step(int left) { return left >= 16 ? 16 : left >= 8 ? 8 : left >= 4 ? 4 : left >= 2 ? 2 : 1; }
+
+// out[s][k*M + m] = <X_k, y>_(s, m) for k < mv and the listed sectors s
+template <typename XT, typename YT>
+int sdot(SCtx& c, int mv, const XT* X, int64_t ldx, const YT* y, std::vector<std::vector<std::complex<double>>>& out) {
+    out.assign(c.S, std::vector<std::complex<double>>());
+    for (int s : c.list) out[s].assign((size_t)mv * c.M, 0.0);
+    const int nch = (int)c.ch.size();
+    if (nch == 0) return ED_OK;
+    std::vector<cd2> h;
+    for (int k0 = 0; k0 < mv; k0 += kMaxMulti) {
+        const int mm = std::min(kMaxMulti, mv - k0);
+        for (int q = 0; q < mm;) {
+            const int step = pass_step(mm - q);
+            const XT* Xq = X + (int64_t)(k0 + q) * ldx;
+#define ED_SDOT(MBV) hipLaunchKernelGGL((k_sdot<MBV, XT, YT>), dim3(nch), dim3(kRedThreads), 0, c.st, c.d_ch, c.M, Xq, ldx, y, c.d_partial, nch, q)
+            switch (step) {
+                case 16: ED_SDOT(16); break;
+                case 8: ED_SDOT(8); break;
+                case 4: ED_SDOT(4); break;
+                case 2: ED_SDOT(2); break;
+                default: ED_SDOT(1); break;
+            }
+#undef ED_SDOT
+            HIPCHK(hipGetLastError());
+            q += step;
+        }
+        const int ncols = mm * c.M;
+        hipLaunchKernelGGL(k_scolsum, dim3(ncols, (unsigned)c.list.size()), dim3(64), 0, c.st, c.d_partial, nch, c.d_c0,
+                           ncols, c.d_sum);
+        HIPCHK(hipGetLastError());
+        h.resize((size_t)ncols * c.list.size());
+        HIPCHK(hipMemcpyAsync(h.data(), c.d_sum, h.size() * sizeof(cd2), hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        for (size_t q = 0; q < c.list.size(); q++)
+            for (int k = 0; k < ncols; k++)
+                out[c.list[q]][(size_t)k0 * c.M + k] = std::complex<double>(h[q * ncols + k].re, h[q * ncols + k].im);
+    }
+    return ED_OK;
+}
+
+// y = a .* y - sum_k coef[s][k*M + m] X_k on the listed sectors (a per
+// (sector, instance); nullptr = 1)
+template <typename XT, typename YT>
+int saxpy(SCtx& c, int mv, const XT* X, int64_t ldx, const std::vector<std::vector<std::complex<double>>>& coef,
+          const std::vector<std::vector<double>>* a, YT* y) {
+    const int nch = (int)c.ch.size();
+    if (nch == 0 || mv == 0) return ED_OK;
+    if (mv > c.cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", mv);
+    const int cs = mv * c.M;
+    std::vector<cd2> cc((size_t)c.S * cs, cd2{0.0, 0.0});
+    for (int s : c.list)
+        for (int k = 0; k < cs; k++) cc[(size_t)s * cs + k] = cd2{coef[s][k].real(), coef[s][k].imag()};
+    HIPCHK(hipMemcpyAsync(c.d_coef, cc.data(), cc.size() * sizeof(cd2), hipMemcpyHostToDevice, c.st));
+    double* da = nullptr;
+    std::vector<double> ha;
+    if (a) {
+        ha.assign((size_t)c.S * c.M, 1.0);
+        for (int s : c.list)
+            for (int m = 0; m < c.M; m++) ha[(size_t)s * c.M + m] = (*a)[s][m];
+        HIPCHK(hipMemcpyAsync(c.d_s, ha.data(), ha.size() * sizeof(double), hipMemcpyHostToDevice, c.st));
+        da = c.d_s;
+    }
+    for (int q = 0; q < mv;) {
+        const int step = pass_step(mv - q);
+        const XT* Xq = X + (int64_t)q * ldx;
+        const cd2* cq = c.d_coef + (size_t)q * c.M;
+        double* aq = q == 0 ? da : nullptr;
+#define ED_SAXPY(MBV) hipLaunchKernelGGL((k_saxpy<MBV, XT, YT>), dim3(nch), dim3(kRedThreads), 0, c.st, c.d_ch, c.M, Xq, ldx, cq, cs, aq, y)
+        switch (step) {
+            case 16: ED_SAXPY(16); break;
+            case 8: ED_SAXPY(8); break;
+            case 4: ED_SAXPY(4); break;
+            case 2: ED_SAXPY(2); break;
+            default: ED_SAXPY(1); break;
+        }
+#undef ED_SAXPY
+        HIPCHK(hipGetLastError());
+        q += step;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    return ED_OK;
+}
+
+template <typename XT, typename YT>
+int sscale(SCtx& c, const std::vector<std::vector<double>>& sc, const XT* x, YT* y) {
+    const int nch = (int)c.ch.size();
+    if (nch == 0) return ED_OK;
+    std::vector<double> hs((size_t)c.S * c.M, 0.0);
+    for (int s : c.list)
+        for (int m = 0; m < c.M; m++) hs[(size_t)s * c.M + m] = sc[s][m];
+    HIPCHK(hipMemcpyAsync(c.d_s, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice, c.st));
+    hipLaunchKernelGGL((k_sscale<XT, YT>), dim3(nch), dim3(kRedThreads), 0, c.st, c.d_ch, c.M, c.d_s, x, y);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c.st));
+    return ED_OK;
+}
+
+template <typename XT>
+int saccum(SCtx& c, int mv, const XT* X, int64_t ldx, const std::vector<std::vector<double>>& t, cd2* y) {
+    const int nch = (int)c.ch.size();
+    if (nch == 0 || mv == 0) return ED_OK;
+    if (mv > c.cap) return ed_fail(ED_EINVAL, "too many vectors (%d) in one update", mv);
+    const int ts = mv * c.M;
+    std::vector<double> ht((size_t)c.S * ts, 0.0);
+    for (int s : c.list)
+        for (int k = 0; k < ts && k < (int)t[s].size(); k++) ht[(size_t)s * ts + k] = t[s][k];
+    HIPCHK(hipMemcpyAsync(c.d_t, ht.data(), ht.size() * sizeof(double), hipMemcpyHostToDevice, c.st));
+    for (int q = 0; q < mv;) {
+        const int step = pass_step(mv - q);
+        const XT* Xq = X + (int64_t)q * ldx;
+        const double* tq = c.d_t + (size_t)q * c.M;
+#define ED_SACC(MBV) hipLaunchKernelGGL((k_saccum<MBV, XT>), dim3(nch), dim3(kRedThreads), 0, c.st, c.d_ch, c.M, Xq, ldx, tq, ts, y)
+        switch (step) {
+            case 16: ED_SACC(16); break;
+            case 8: ED_SACC(8); break;
+            case 4: ED_SACC(4); break;
+            case 2: ED_SACC(2); break;
+            default: ED_SACC(1); break;
+        }
+#undef ED_SACC
+        HIPCHK(hipGetLastError());
+        q += step;
+    }
+    HIPCHK(hipStreamSynchronize(c.st));
+    return ED_OK;
+}
+
+template <typename T>
+int snorm2(SCtx& c, const T* y, std::vector<std::vector<double>>& out) {
+    std::vector<std::vector<std::complex<double>>> r;
+    RC((sdot<T, T>(c, 1, y, 0, y, r)));
+    out.assign(c.S, std::vector<double>(c.M, 0.0));
+    for (int s : c.list)
+        for (int m = 0; m < c.M; m++) out[s][m] = r[s][m].real();
+    return ED_OK;
+}
+
+// x -= projections on the earlier eigenvectors (two passes), listed sectors
+template <typename VT>
+int sdeflate(SCtx& c, const std::vector<cd2*>& prev, VT* x) {
+    std::vector<std::vector<std::complex<double>>> q;
+    for (int pass = 0; pass < 2; pass++)
+        for (size_t e = 0; e < prev.size(); e++) {
+            RC((sdot<cd2, VT>(c, 1, prev[e], 0, x, q)));
+            RC((saxpy<cd2, VT>(c, 1, prev[e], 0, q, nullptr, x)));
+        }
+    return ED_OK;
+}
+
+template <typename VT>
+int shv(SCtx& c, const std::vector<char>& live, const VT* v, VT* w) {
+    int rc = mblock_set_live(c.mb, live);
+    if (rc != ED_OK) return rc;
+    HIPCHK(hipEventRecord(c.e0, c.st));
+    rc = mblock_op_hv(c.mb, std::is_same<VT, cd2>::value ? ED_C128 : ED_C64, v, w, c.st);
+    if (rc != ED_OK) return rc;
+    HIPCHK(hipEventRecord(c.e1, c.st));
+    HIPCHK(hipEventSynchronize(c.e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.e0, c.e1));
+    c.hv_ms += ms;
+    c.hv_calls++;
+    return ED_OK;
+}
+
+template <typename VT>
+int run_mblock_full(SCtx& c, const std::vector<int64_t>& Dsec, const ed_lanczos_params& p, double tol,
+                    const std::vector<cd2*>* starts, std::vector<cd2*>& prev, double* evals) {
+    const int S = c.S, M = c.M;
+    const int64_t n_all = c.n_all;
+    const int64_t Dmin = *std::min_element(Dsec.begin(), Dsec.end());
+    const int64_t Dmax = *std::max_element(Dsec.begin(), Dsec.end());
+    const int how_many = (int)std::min<int64_t>(p.nev, Dmin);
+    const int m_it = (int)std::min<int64_t>(p.iterations, Dmax);
+    const std::vector<char> all(S, 1);
+    VT *V = nullptr, *w = nullptr;
+    cd2* x = nullptr;
+    HIPCHK(hipMalloc(&V, (size_t)(m_it + 1) * n_all * sizeof(VT)));
+    HIPCHK(hipMalloc(&w, n_all * sizeof(VT)));
+    HIPCHK(hipMalloc(&x, n_all * sizeof(cd2)));
+    auto cleanup = [&]() { dfree(V); dfree(w); dfree(x); };
+#define SRC(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != ED_OK) {            \
+            cleanup();                 \
+            return rc_;                \
+        }                              \
+    } while (0)
+    std::vector<std::vector<std::complex<double>>> q;
+    for (int num = 0; num < how_many; num++) {
+        std::vector<int> mit(S);
+        std::vector<std::vector<std::vector<double>>> alphas(S, std::vector<std::vector<double>>(M)),
+            betas(S, std::vector<std::vector<double>>(M));
+        std::vector<std::vector<int>> nvs(S, std::vector<int>(M, 0));
+        std::vector<std::vector<char>> active(S, std::vector<char>(M, 1));
+        std::vector<char> live(S, 1);
+        SRC(s_list(c, all));
+        for (int s = 0; s < S; s++) {
+            mit[s] = (int)std::min<int64_t>(m_it, Dsec[s] - num);
+            VT* v0 = V + c.off[s];
+            const int64_t ns = c.off[s + 1] - c.off[s];
+            if (starts && num < (int)starts->size())
+                hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3((unsigned)std::min<int64_t>(kRedBlocks, (ns + 255) / 256)),
+                                   dim3(kRedThreads), 0, c.st, ns, 1.0, (*starts)[num] + c.off[s], v0);
+            else
+                hipLaunchKernelGGL(k_bstart<VT>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c.st, Dsec[s], M,
+                                   p.seed + (uint64_t)num * 1000003ull, v0);
+            SRC(hipGetLastError() == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "start vector launch failed"));
+        }
+        SRC(sdeflate<VT>(c, prev, V));
+        std::vector<std::vector<double>> nn, sc(S, std::vector<double>(M));
+        SRC(snorm2(c, V, nn));
+        for (int s = 0; s < S; s++)
+            for (int m = 0; m < M; m++) sc[s][m] = 1.0 / std::sqrt(nn[s][m]);
+        SRC((sscale<VT, VT>(c, sc, V, V)));
+        for (int k = 0; k < m_it; k++) {
+            if (std::find(live.begin(), live.end(), 1) == live.end()) break;
+            SRC(s_list(c, live));
+            VT* vk = V + (int64_t)k * n_all;
+            SRC(shv<VT>(c, live, vk, w));
+            SRC((sdot<VT, VT>(c, 1, vk, 0, w, q)));
+            std::vector<std::vector<double>> alpha(S, std::vector<double>(M, 0.0));
+            std::vector<std::vector<std::complex<double>>> cf(S);
+            for (int s : c.list) {
+                for (int m = 0; m < M; m++) alpha[s][m] = q[s][m].real();
+                if (k > 0) {
+                    cf[s].resize(2 * M);
+                    for (int m = 0; m < M; m++) {
+                        cf[s][m] = betas[s][m].size() == (size_t)k ? betas[s][m].back() : 0.0;   // v_{k-1}
+                        cf[s][M + m] = alpha[s][m];                                                // v_k
+                    }
+                } else {
+                    cf[s].resize(M);
+                    for (int m = 0; m < M; m++) cf[s][m] = alpha[s][m];
+                }
+            }
+            if (k > 0) SRC((saxpy<VT, VT>(c, 2, V + (int64_t)(k - 1) * n_all, n_all, cf, nullptr, w)));
+            else SRC((saxpy<VT, VT>(c, 1, vk, 0, cf, nullptr, w)));
+            // two Gram-Schmidt passes; the second only when the first removed
+            // more than 1 - 1/sqrt(2) of some live instance's norm
+            std::vector<std::vector<double>> nb2, bb;
+            bool have_bb = false;
+            SRC(snorm2(c, w, nb2));
+            for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1) {
+                    SRC(snorm2(c, w, bb));
+                    bool need = false;
+                    for (int s : c.list)
+                        for (int m = 0; m < M; m++) need |= active[s][m] && bb[s][m] <= 0.5 * nb2[s][m];
+                    if (!need) { have_bb = true; break; }
+                }
+                SRC((sdot<VT, VT>(c, k + 1, V, n_all, w, q)));
+                if (pass == 0)
+                    for (int s : c.list)
+                        for (int m = 0; m < M; m++) alpha[s][m] += q[s][(size_t)k * M + m].real();
+                SRC((saxpy<VT, VT>(c, k + 1, V, n_all, q, nullptr, w)));
+                std::vector<std::vector<std::complex<double>>> qe;
+                for (size_t e = 0; e < prev.size(); e++) {
+                    SRC((sdot<cd2, VT>(c, 1, prev[e], 0, w, qe)));
+                    SRC((saxpy<cd2, VT>(c, 1, prev[e], 0, qe, nullptr, w)));
+                }
+            }
+            if (!have_bb) SRC(snorm2(c, w, bb));
+            std::vector<std::vector<double>> scale(S, std::vector<double>(M, 0.0));
+            for (int s : c.list) {
+                bool any = false;
+                for (int m = 0; m < M; m++) {
+                    if (!active[s][m]) continue;
+                    alphas[s][m].push_back(alpha[s][m]);
+                    nvs[s][m] = k + 1;
+                    const double beta = std::sqrt(bb[s][m]);
+                    bool stop = false;
+                    if ((k + 1) % 5 == 0 || k + 1 == mit[s]) {
+                        double ee;
+                        std::vector<double> z;
+                        if (!lowest(alphas[s][m], betas[s][m], ee, &z)) {
+                            cleanup();
+                            return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+                        }
+                        if (beta * std::fabs(z.back()) < tol * std::max(1.0, std::fabs(ee))) stop = true;
+                    }
+                    if (beta < 1e-13 * std::max(1.0, std::fabs(alpha[s][m]))) stop = true;   // invariant subspace
+                    if (stop || k + 1 == mit[s]) {
+                        active[s][m] = 0;
+                        continue;
+                    }
+                    betas[s][m].push_back(beta);
+                    scale[s][m] = 1.0 / beta;
+                    any = true;
+                }
+                if (!any) live[s] = 0;
+            }
+            // the next Krylov vector of every sector still listed (zero for
+            // its stopped instances)
+            SRC((sscale<VT, VT>(c, scale, w, V + (int64_t)(k + 1) * n_all)));
+        }
+        // Ritz vectors: t[s][j*M + m] = lowest eigenvector of (s, m)'s T
+        SRC(s_list(c, all));
+        int kmax = 0;
+        std::vector<std::vector<double>> t(S);
+        for (int s = 0; s < S; s++) kmax = std::max(kmax, *std::max_element(nvs[s].begin(), nvs[s].end()));
+        for (int s = 0; s < S; s++) {
+            t[s].assign((size_t)kmax * M, 0.0);
+            for (int m = 0; m < M; m++) {
+                std::vector<double> tv, off(betas[s][m].begin(), betas[s][m].begin() + std::max(0, nvs[s][m] - 1));
+                std::vector<double> al(alphas[s][m].begin(), alphas[s][m].begin() + nvs[s][m]);
+                double ee;
+                if (!lowest(al, off, ee, &tv)) {
+                    cleanup();
+                    return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
+                }
+                for (int j = 0; j < nvs[s][m]; j++) t[s][(size_t)j * M + m] = tv[j];
+                evals[((size_t)s * M + m) * how_many + num] = ee;
+            }
+        }
+        SRC(hipMemsetAsync(x, 0, n_all * sizeof(cd2), c.st) == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "memset failed"));
+        SRC(saccum<VT>(c, kmax, V, n_all, t, x));
+        SRC(sdeflate<cd2>(c, prev, x));
+        SRC(snorm2(c, x, nn));
+        for (int s = 0; s < S; s++)
+            for (int m = 0; m < M; m++) sc[s][m] = 1.0 / std::sqrt(nn[s][m]);
+        SRC((sscale<cd2, cd2>(c, sc, x, x)));
+        cd2* keep = nullptr;
+        SRC(hipMalloc(&keep, n_all * sizeof(cd2)) == hipSuccess ? ED_OK : ed_fail(ED_ENOMEM, "Ritz vector allocation"));
+        prev.push_back(keep);
+        SRC(hipMemcpyAsync(keep, x, n_all * sizeof(cd2), hipMemcpyDeviceToDevice, c.st) == hipSuccess
+                ? ED_OK : ed_fail(ED_EDEVICE, "copy failed"));
+    }
+#undef SRC
+    HIPCHK(hipStreamSynchronize(c.st));
+    cleanup();
+    return ED_OK;
+}
+
 }  // namespace
 
 extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evecs, ed_lanczos_stats* stats) {
@@ -1577,5 +2108,121 @@ extern "C" int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double
         stats->hv_seconds = c.hv_ms * 1e-3;
         stats->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    return rc;
+}
+
+extern "C" int ed_mblock_lanczos(ed_mblock mb, const ed_lanczos_params* p, double* evals, double* errors,
+                                 ed_lanczos_stats* stats) {
+    if (!mb || !p || !evals) return ed_fail(ED_EINVAL, "null argument");
+    if (p->nev < 1 || p->iterations < 1) return ed_fail(ED_EINVAL, "nev and iterations must be >= 1");
+    if (p->precision != ED_C64 && p->precision != ED_C128) return ed_fail(ED_EINVAL, "bad precision %d", p->precision);
+    const int S = mblock_nsec(mb), M = mblock_M(mb);
+    if (64 % M) return ed_fail(ED_EINVAL, "block Lanczos needs M in {1, 2, 4, 8}, got %d", M);
+    std::vector<int64_t> Dsec(S);
+    for (int s = 0; s < S; s++) {
+        Dsec[s] = block_basis(mblock_block(mb, s))->D;
+        if (Dsec[s] == 0) return ed_fail(ED_EEMPTY, "sector %d has no states", s);
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    SCtx c;
+    c.mb = mb;
+    c.S = S;
+    c.M = M;
+    c.off.resize(S + 1);
+    for (int s = 0; s <= S; s++) c.off[s] = mblock_row0(mb, s) * M;
+    c.n_all = c.off[S];
+    for (int s = 0; s < S; s++) c.max_ch += (int)((c.off[s + 1] - c.off[s] + kSegElems - 1) / kSegElems);
+    c.cap = std::max(p->iterations, 1) + 64;
+    HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c.e0));
+    HIPCHK(hipEventCreate(&c.e1));
+    int rc = ED_OK;
+    std::vector<cd2*> prev;
+    auto alloc = [&]() -> int {
+        HIPCHK(hipMalloc(&c.d_ch, std::max(c.max_ch, 1) * sizeof(SegChunk)));
+        HIPCHK(hipMalloc(&c.d_c0, (S + 1) * sizeof(int)));
+        // also the partials of the per-sector FP64 checks (kRedBlocks workgroups)
+        HIPCHK(hipMalloc(&c.d_partial, (size_t)kBCols * std::max(c.max_ch, kRedBlocks) * sizeof(cd2)));
+        HIPCHK(hipMalloc(&c.d_sum, (size_t)kBCols * S * sizeof(cd2)));
+        HIPCHK(hipMalloc(&c.d_coef, (size_t)S * c.cap * M * sizeof(cd2)));
+        HIPCHK(hipMalloc(&c.d_t, (size_t)S * c.cap * M * sizeof(double)));
+        HIPCHK(hipMalloc(&c.d_s, (size_t)S * M * sizeof(double)));
+        return ED_OK;
+    };
+    rc = alloc();
+    if (rc == ED_OK) {
+        if (p->precision == ED_C128) {
+            rc = run_mblock_full<cd2>(c, Dsec, *p, p->tol, nullptr, prev, evals);
+        } else {
+            // as ed_block_lanczos: complex64 to residual 1e-6, then a
+            // complex128 run from its Ritz vectors when tol asks for less
+            const double floor64 = 1e-6;
+            rc = run_mblock_full<cf2>(c, Dsec, *p, std::max(p->tol, floor64), nullptr, prev, evals);
+            if (rc == ED_OK && p->tol < floor64) {
+                std::vector<cd2*> coarse;
+                coarse.swap(prev);
+                rc = run_mblock_full<cd2>(c, Dsec, *p, p->tol, &coarse, prev, evals);
+                for (auto* v : coarse) dfree(v);
+            }
+        }
+    }
+    const int how_many = (int)std::min<int64_t>(p->nev, *std::min_element(Dsec.begin(), Dsec.end()));
+    std::vector<double> err((size_t)S * M * how_many, 0.0);
+    if (rc == ED_OK) {
+        // the FP64 Rayleigh quotient and residual of every Ritz vector, sector
+        // by sector through its block operator
+        for (int s = 0; s < S && rc == ED_OK; s++) {
+            BCtx b;
+            b.blk = mblock_block(mb, s);
+            b.D = Dsec[s];
+            b.M = M;
+            b.n = b.D * M;
+            b.st = c.st;
+            b.g = (unsigned)std::min<int64_t>(kRedBlocks, (b.n + kRedThreads - 1) / kRedThreads);
+            b.cap = c.cap;
+            b.d_partial = c.d_partial; b.d_sum = c.d_sum; b.d_coef = c.d_coef; b.d_t = c.d_t; b.d_s = c.d_s;
+            b.e0 = c.e0; b.e1 = c.e1;
+            for (size_t k = 0; k < prev.size(); k++) {
+                std::vector<double> rq, res;
+                rc = bfp64_check(b, prev[k] + c.off[s], rq, res);
+                if (rc != ED_OK) break;
+                for (int m = 0; m < M; m++) {
+                    const size_t at = ((size_t)s * M + m) * how_many + k;
+                    if (p->precision == ED_C64) evals[at] = rq[m];
+                    err[at] = res[m];
+                    const size_t sm = (size_t)s * M + m;
+                    if (stats && k == 0 && sm < 16) { stats->residual[sm] = res[m]; stats->rayleigh[sm] = rq[m]; }
+                }
+            }
+            c.hv_calls += b.hv_calls;
+            c.hv_ms += b.hv_ms;
+        }
+        // each instance's eigenvalues ascending, their bounds with them
+        for (size_t im = 0; rc == ED_OK && im < (size_t)S * M; im++) {
+            std::vector<std::pair<double, double>> pr(how_many);
+            for (int k = 0; k < how_many; k++) pr[k] = {evals[im * how_many + k], err[im * how_many + k]};
+            std::sort(pr.begin(), pr.end());
+            for (int k = 0; k < how_many; k++) {
+                evals[im * how_many + k] = pr[k].first;
+                err[im * how_many + k] = pr[k].second;
+            }
+        }
+        if (errors) std::memcpy(errors, err.data(), err.size() * sizeof(double));
+    }
+    (void)mblock_set_live(mb, std::vector<char>(S, 1));
+    for (auto* v : prev) dfree(v);
+    for (void* q : {(void*)c.d_ch, (void*)c.d_c0, (void*)c.d_partial, (void*)c.d_sum, (void*)c.d_coef, (void*)c.d_t,
+                    (void*)c.d_s})
+        dfree(q);
+    if (stats) {
+        std::memset(stats->eval_error, 0, sizeof(stats->eval_error));
+        for (size_t q = 0; q < err.size() && q < 64; q++) stats->eval_error[q] = err[q];
+        stats->hv_calls = c.hv_calls;
+        stats->hv_seconds = c.hv_ms * 1e-3;
+        stats->total_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    (void)hipEventDestroy(c.e0);
+    (void)hipEventDestroy(c.e1);
+    (void)hipStreamDestroy(c.st);
     return rc;
 }

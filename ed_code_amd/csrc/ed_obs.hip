@@ -16,8 +16,11 @@
 // <bra|O|ket> for a pair.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <set>
 #include <utility>
@@ -416,4 +419,237 @@ extern "C" int ed_unwrap(ed_basis b, const void* v, int64_t capacity, int64_t* s
 #undef ED_UW
     cleanup();
     return rc;
+}
+
+// ------------------------------------------------------- density matrices ----
+// Reduced density matrix of a cut A (sites cut[0..ncut)) from two
+// momentum-basis vectors of one basis, the quantity of density_matrixij
+// (analysis/src/Entropy.cpp:170-205, driven by analysis/src/dm.cpp:10-88):
+//   dm(j, k) = sum_e psi1(j, e) conj(psi2(k, e)),
+// j, k configurations of A (bit l of j = site cut[l]), e of the other sites,
+// psi the unwrapped full-basis vectors with the unwrap threshold applied
+// (entries with |re| and |im| <= 1e-8 are absent from the reference's EV
+// files, so they count as zero).  The reference loops over 2^ncut x (unwrapped
+// entries) with a binary search per term; here the unwrapped vector is laid
+// out once as one matrix per count a of up spins in A,
+//   Psi_a[row, col]:  row = rank of j among the C(ncut, a) weight-a
+//                     configurations of A, col = rank of e among the
+//                     C(N-ncut, n-a) configurations of the rest,
+// (every weight-n state is one entry of one Psi_a), and each diagonal block
+// of the matrix (states of equal Sz in A; the others vanish) is one complex
+// GEMM, rho_a = Psi1_a Psi2_a^H, on rocBLAS.
+namespace {
+
+struct DmCut {
+    int32_t ncut, nenv;
+    int32_t cut[64];      // site of cut bit l
+    int32_t env[64];      // the other sites, ascending
+};
+
+// combinatorial-number-system rank of the nb-bit word x: sum_t C(p_t, t)
+__device__ __forceinline__ uint64_t cns_rank(uint64_t x, const uint64_t* binom) {
+    uint64_t r = 0;
+    int t = 0;
+    while (x) {
+        const int p = __ffsll((unsigned long long)x) - 1;
+        t++;
+        r += binom[p * 65 + t];
+        x &= x - 1;
+    }
+    return r;
+}
+
+__global__ void k_dm_layout(uint64_t C, int N, int n, const uint64_t* __restrict__ binom, DmCut cut,
+                            const uint64_t* __restrict__ off, const uint64_t* __restrict__ nrow,
+                            const cd2* __restrict__ coef, const int* __restrict__ keep, cd2* __restrict__ psi) {
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < C; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = unrank(r, N, n, binom);
+        uint64_t j = 0, e = 0;
+        for (int l = 0; l < cut.ncut; l++) j |= ((s >> cut.cut[l]) & 1ull) << l;
+        for (int l = 0; l < cut.nenv; l++) e |= ((s >> cut.env[l]) & 1ull) << l;
+        const int a = __popcll(j);
+        const uint64_t row = cns_rank(j, binom), col = cns_rank(e, binom);
+        psi[off[a] + row + col * nrow[a]] = keep[r] ? coef[r] : cd2{0.0, 0.0};
+    }
+}
+
+__global__ void k_zero_cd2(uint64_t n, cd2* p) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = cd2{0.0, 0.0};
+}
+
+// The unwrapped vector by state rank (coef and the threshold flags keep, [C]
+// each): the first two passes of ed_unwrap.  States of orbits outside the
+// sector keep coef 0, keep 0.
+int unwrap_dense(const ed_basis_s* B, const cd2* v, int* d_first, cd2* d_coef, int* d_keep) {
+    const uint64_t C = B->C;
+    Walk W{B->lat_P1, B->lat_P2, B->lat_P3, B->L1, B->L2, B->L3};
+    const cd2 ex = cexp_neg(B->px), ey = cexp_neg(B->py), ez = cexp_neg(B->pz);
+    const unsigned gC = (unsigned)std::min<uint64_t>((C + 255) / 256, 1u << 20);
+    const unsigned gD = (unsigned)std::min<int64_t>((B->D + 255) / 256, 1 << 20);
+    hipLaunchKernelGGL(k_fill_int, dim3(gC), dim3(256), 0, 0, C, d_first, kNoVisit);
+    hipLaunchKernelGGL(k_fill_int, dim3(gC), dim3(256), 0, 0, C, d_keep, 0);
+    hipLaunchKernelGGL(k_zero_cd2, dim3(gC), dim3(256), 0, 0, C, d_coef);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_unwrap_first, dim3(gD), dim3(256), 0, 0, B->D, B->d_rows, W, B->rinfo, B->d_rtab, d_first);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_unwrap_fill, dim3(gD), dim3(256), 0, 0, B->D, B->d_rows, B->d_ntab, W, B->rinfo, B->d_rtab,
+                       ex, ey, ez, v, d_first, d_coef, d_keep);
+    HIPCHK(hipGetLastError());
+    return ED_OK;
+}
+
+}  // namespace
+
+extern "C" int ed_density_matrix(ed_basis b, const void* v1, const void* v2, int32_t ncut, const int32_t* cut,
+                                 int32_t n_cut_ones, double* dm) {
+    if (!b || !cut || !dm) return ed_fail(ED_EINVAL, "null argument");
+    const ed_basis_s* B = b;
+    if (B->D > 0 && (!v1 || !v2)) return ed_fail(ED_EINVAL, "null vector");
+    if (B->L3 != 1) return ed_fail(ED_EINVAL, "density matrices of three-generator sectors are not supported");
+    const int N = B->N, n = B->n;
+    if (ncut < 1 || ncut > N || ncut > 30) return ed_fail(ED_EINVAL, "cut of %d sites (1..min(N, 30))", ncut);
+    DmCut dc;
+    std::memset(&dc, 0, sizeof(dc));
+    dc.ncut = ncut;
+    std::vector<int> in(N, 0);
+    for (int l = 0; l < ncut; l++) {
+        if (cut[l] < 0 || cut[l] >= N || in[cut[l]]) return ed_fail(ED_EINVAL, "cut site %d out of range or repeated", cut[l]);
+        in[cut[l]] = 1;
+        dc.cut[l] = cut[l];
+    }
+    for (int s = 0; s < N; s++)
+        if (!in[s]) dc.env[dc.nenv++] = s;
+    if (n_cut_ones < -1 || n_cut_ones > ncut) return ed_fail(ED_EINVAL, "n_cut_ones %d out of range", n_cut_ones);
+    // blocks: a up spins in A, rows C(ncut, a), columns C(N - ncut, n - a)
+    std::vector<uint64_t> off(ncut + 2, 0), nrow(ncut + 1, 0), ncol(ncut + 1, 0);
+    for (int a = 0; a <= ncut; a++) {
+        nrow[a] = binom_u64(ncut, a);
+        ncol[a] = (n - a >= 0 && n - a <= dc.nenv) ? binom_u64(dc.nenv, n - a) : 0;
+        off[a + 1] = off[a] + nrow[a] * ncol[a];
+    }
+    const uint64_t C = B->C;
+    if (off[ncut + 1] != C) return ed_fail(ED_EDEVICE, "density-matrix layout covers %llu of %llu states",
+                                           (unsigned long long)off[ncut + 1], (unsigned long long)C);
+    const uint64_t ndm = n_cut_ones < 0 ? (uint64_t(1) << ncut) : nrow[n_cut_ones];
+    if (ndm > 32768) return ed_fail(ED_EINVAL, "density matrix of %llu x %llu exceeds 32768 rows", (unsigned long long)ndm,
+                                    (unsigned long long)ndm);
+    std::memset(dm, 0, ndm * ndm * 2 * sizeof(double));
+    if (B->D == 0) return ED_OK;
+    if (C > 0x7fffffffull) return ed_fail(ED_EINVAL, "density matrix of a sector with C(N,n) = %llu > 2^31 states",
+                                          (unsigned long long)C);
+    if ((uint64_t)B->L1 * B->L2 >= (uint64_t)kNoVisit) return ed_fail(ED_EINVAL, "walk too long");
+    uint64_t maxblk = 0;
+    for (int a = 0; a <= ncut; a++)
+        if (n_cut_ones < 0 || a == n_cut_ones) maxblk = std::max(maxblk, nrow[a] * nrow[a]);
+    int *d_first = nullptr, *d_keep = nullptr;
+    cd2 *d_coef = nullptr, *d_psi1 = nullptr, *d_psi2 = nullptr, *d_rho = nullptr;
+    uint64_t *d_off = nullptr, *d_nrow = nullptr;
+    rocblas_handle h = nullptr;
+    std::vector<cd2> blk;
+    auto body = [&]() -> int {
+        HIPCHK(hipMalloc(&d_first, C * sizeof(int)));
+        HIPCHK(hipMalloc(&d_keep, C * sizeof(int)));
+        HIPCHK(hipMalloc(&d_coef, C * sizeof(cd2)));
+        HIPCHK(hipMalloc(&d_psi1, C * sizeof(cd2)));
+        HIPCHK(hipMalloc(&d_psi2, C * sizeof(cd2)));
+        HIPCHK(hipMalloc(&d_rho, std::max<uint64_t>(maxblk, 1) * sizeof(cd2)));
+        HIPCHK(hipMalloc(&d_off, (ncut + 2) * sizeof(uint64_t)));
+        HIPCHK(hipMalloc(&d_nrow, (ncut + 1) * sizeof(uint64_t)));
+        HIPCHK(hipMemcpy(d_off, off.data(), (ncut + 2) * sizeof(uint64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_nrow, nrow.data(), (ncut + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+        const unsigned gC = (unsigned)std::min<uint64_t>((C + 255) / 256, 1u << 20);
+        for (int q = 0; q < 2; q++) {
+            int rc = unwrap_dense(B, (const cd2*)(q == 0 ? v1 : v2), d_first, d_coef, d_keep);
+            if (rc != ED_OK) return rc;
+            hipLaunchKernelGGL(k_dm_layout, dim3(gC), dim3(256), 0, 0, C, N, n, B->d_binom, dc, d_off, d_nrow, d_coef,
+                               d_keep, q == 0 ? d_psi1 : d_psi2);
+            HIPCHK(hipGetLastError());
+        }
+        if (rocblas_create_handle(&h) != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
+        // the rank order of each weight's configurations (row r of block a = cfg[a][r])
+        std::vector<std::vector<uint32_t>> cfg(ncut + 1);
+        if (n_cut_ones < 0)
+            for (uint64_t x = 0; x < (uint64_t(1) << ncut); x++) cfg[__builtin_popcountll(x)].push_back((uint32_t)x);
+        const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+        for (int a = 0; a <= ncut; a++) {
+            if (n_cut_ones >= 0 && a != n_cut_ones) continue;
+            const uint64_t m = nrow[a], kk = ncol[a];
+            if (m == 0 || kk == 0) continue;
+            if (m > 0x7fffffffull || kk > 0x7fffffffull) return ed_fail(ED_EINVAL, "density-matrix block too large");
+            // rho_a = Psi1_a Psi2_a^H (column-major, leading dimension m)
+            rocblas_status rs = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_conjugate_transpose,
+                                              (rocblas_int)m, (rocblas_int)m, (rocblas_int)kk, &one,
+                                              (const rocblas_double_complex*)(d_psi1 + off[a]), (rocblas_int)m,
+                                              (const rocblas_double_complex*)(d_psi2 + off[a]), (rocblas_int)m, &zero,
+                                              (rocblas_double_complex*)d_rho, (rocblas_int)m);
+            if (rs != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_zgemm failed (%d)", (int)rs);
+            blk.resize(m * m);
+            HIPCHK(hipMemcpy(blk.data(), d_rho, m * m * sizeof(cd2), hipMemcpyDeviceToHost));
+            if (n_cut_ones >= 0) {
+                std::memcpy(dm, blk.data(), m * m * sizeof(cd2));
+            } else {
+                const uint64_t M2 = uint64_t(1) << ncut;
+                for (uint64_t c = 0; c < m; c++)
+                    for (uint64_t r = 0; r < m; r++) {
+                        const uint64_t at = (uint64_t)cfg[a][r] + (uint64_t)cfg[a][c] * M2;
+                        dm[2 * at] = blk[r + c * m].re;
+                        dm[2 * at + 1] = blk[r + c * m].im;
+                    }
+            }
+        }
+        return ED_OK;
+    };
+    const int rc = body();
+    if (h) rocblas_destroy_handle(h);
+    for (void* p : {(void*)d_first, (void*)d_keep, (void*)d_coef, (void*)d_psi1, (void*)d_psi2, (void*)d_rho,
+                    (void*)d_off, (void*)d_nrow})
+        dfree(p);
+    return rc;
+}
+
+extern "C" int ed_entanglement_entropies(const double* dm, int64_t n, double* renyi2, double* von_neumann) {
+    if (!dm || n < 1 || !renyi2 || !von_neumann) return ed_fail(ED_EINVAL, "null argument or empty matrix");
+    if (n > 32768) return ed_fail(ED_EINVAL, "matrix of %lld rows exceeds 32768", (long long)n);
+    // Renyi_Entropy (Entropy.cpp:334-355): -log2 sum_ij |dm_ij|^2, in the
+    // reference's summation order
+    double sr = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = 0; j < n; j++) {
+            const double re = dm[2 * (i + j * n)], im = dm[2 * (i + j * n) + 1];
+            sr += re * re + im * im;
+        }
+    *renyi2 = -std::log2(sr);
+    // Von_Neumann_Entropy (Entropy.cpp:307-331): -sum lambda log2 lambda over
+    // the eigenvalues above 1e-8, here from rocSOLVER's Hermitian solver
+    cd2* dA = nullptr;
+    double *dW = nullptr, *dE = nullptr;
+    int* dinfo = nullptr;
+    rocblas_handle h = nullptr;
+    std::vector<double> w(n);
+    auto body = [&]() -> int {
+        HIPCHK(hipMalloc(&dA, n * n * sizeof(cd2)));
+        HIPCHK(hipMalloc(&dW, n * sizeof(double)));
+        HIPCHK(hipMalloc(&dE, n * sizeof(double)));
+        HIPCHK(hipMalloc(&dinfo, sizeof(int)));
+        HIPCHK(hipMemcpy(dA, dm, n * n * sizeof(cd2), hipMemcpyHostToDevice));
+        if (rocblas_create_handle(&h) != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
+        rocblas_status rs = rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_upper, (rocblas_int)n,
+                                             (rocblas_double_complex*)dA, (rocblas_int)n, dW, dE, dinfo);
+        if (rs != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocsolver_zheevd failed (%d)", (int)rs);
+        int info = 0;
+        HIPCHK(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+        if (info != 0) return ed_fail(ED_ENOTCONV, "zheevd info=%d", info);
+        HIPCHK(hipMemcpy(w.data(), dW, n * sizeof(double), hipMemcpyDeviceToHost));
+        return ED_OK;
+    };
+    const int rc = body();
+    if (h) rocblas_destroy_handle(h);
+    for (void* p : {(void*)dA, (void*)dW, (void*)dE, (void*)dinfo}) dfree(p);
+    if (rc != ED_OK) return rc;
+    double s = 0.0;
+    for (double l : w)
+        if (std::fabs(l) > 1e-8) s += l * std::log2(l);
+    *von_neumann = -s;
+    return ED_OK;
 }

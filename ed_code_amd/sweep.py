@@ -18,7 +18,7 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from . import (ED_C64, ED_C128, ED_LZ_FULL, Basis, BlockOperator, Lattice, Operator, init)
+from . import (ED_C64, ED_C128, ED_LZ_FULL, Basis, BlockOperator, Lattice, MultiBlockOperator, Operator, init)
 from . import lattices as L
 from .lattices import Model
 
@@ -74,6 +74,67 @@ def sweep_sectors(models: Sequence[Model], n_up: int, nev: int = 2, tol: float =
                 hv_block_calls=hv_calls)
 
 
+def krylov_bytes(D: int, M: int, iterations: int, precision: int) -> int:
+    """Device bytes one sector adds to a multi-sector Lanczos: Krylov vectors
+    (the complex128 stage that follows a complex64 one when tol asks for it
+    holds as many vectors at twice the size, so complex128 decides), two work
+    vectors, FP64 Ritz vectors and the gather images (at most 6 elements per
+    row) of both precisions."""
+    rows = D * M
+    krylov = (iterations + 1) * rows * 16
+    return krylov + rows * (2 * 16 + 4 * 16 + 6 * (8 + 16)) + D * 20
+
+
+def sweep_sectors_multi(models: Sequence[Model], n_up: int, nev: int = 2, tol: float = 1e-8, iterations: int = 120,
+                        precision: int = ED_C64, sectors: Optional[Sequence[tuple]] = None, seed: int = 1234,
+                        budget: Optional[int] = None, log: Optional[Callable[[str], None]] = None) -> dict:
+    """sweep_sectors with the sectors in as few multi-sector block operators
+    as device memory allows: each group's every Lanczos step is one H*V
+    launch over all its sectors and instances (ed_mblock_lanczos).  budget:
+    device bytes for the groups' Krylov storage (default 60 % of the free
+    memory)."""
+    import torch
+    lat = Lattice.of(models[0])
+    if sectors is None:
+        sectors = [(kx, ky) for kx in range(lat.L1) for ky in range(lat.L2)]
+    if budget is None:
+        budget = int(0.6 * torch.cuda.mem_get_info()[0])
+    t_all = time.perf_counter()
+    bases = []
+    for (kx, ky) in sectors:
+        b = Basis(lat, n_up, kx, ky)
+        if b.D > 0:
+            bases.append(((kx, ky), b))
+    groups, cur, used = [], [], 0
+    for item in bases:
+        need = krylov_bytes(item[1].D, len(models), iterations, precision)
+        if cur and used + need > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(item)
+        used += need
+    if cur:
+        groups.append(cur)
+    res, hv_calls, launches = [], 0, 0
+    for g in groups:
+        t0 = time.perf_counter()
+        mb = MultiBlockOperator([b for _, b in g], models)
+        ev, st = mb.lanczos(nev=nev, iterations=iterations, tol=tol, seed=seed, precision=precision)
+        dt = time.perf_counter() - t0
+        hv_calls += st["hv_calls"]
+        for s, ((kx, ky), b) in enumerate(g):
+            res.append(dict(kx=kx, ky=ky, D=b.D, evals=ev[s].tolist(), eval_error=st["eval_error"][s].tolist(),
+                            seconds=dt / len(g), group=len(groups) and groups.index(g), fast=mb.fast))
+        if log:
+            log(f"[sweep] group of {len(g)} sectors ({sum(b.D for _, b in g)} rows) {dt:.2f}s "
+                f"hv={st['hv_calls']} fast={mb.fast} max err bound={st['eval_error'].max():.1e}")
+        del mb
+    total = time.perf_counter() - t_all
+    solves = len(res) * len(models)
+    return dict(sectors=res, seconds=total, solves=solves, solves_per_s=solves / total if total else 0.0,
+                hv_calls=hv_calls, groups=[len(g) for g in groups])
+
+
 def one_at_a_time(models: Sequence[Model], n_up: int, sectors: Sequence[tuple], nev: int = 2, tol: float = 1e-8,
                   iterations: int = 300, precision: int = ED_C64, seed: int = 1234) -> dict:
     """The plain loop the block sweep replaces: one single-instance Lanczos
@@ -107,6 +168,8 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--sectors", type=int, default=0, help="only the first n sectors (0 = all)")
     ap.add_argument("--compare", type=int, default=0, help="also run the one-at-a-time loop on n sectors")
+    ap.add_argument("--multi", action="store_true",
+                    help="multi-sector block operators: every Lanczos step one H*V launch over all sectors of a group")
     a = ap.parse_args(argv)
     init(0)
     models = coupling_instances(a.lattice, a.instances)
@@ -117,8 +180,9 @@ def main(argv=None):
         secs = secs[:a.sectors]
     prec = ED_C64 if a.precision == "c64" else ED_C128
     import sys
-    out = sweep_sectors(models, n_up, a.nev, a.tol, a.iterations, prec, secs,
-                        log=lambda s: print(s, file=sys.stderr, flush=True))
+    sweep = sweep_sectors_multi if a.multi else sweep_sectors
+    out = sweep(models, n_up, a.nev, a.tol, a.iterations, prec, secs,
+                log=lambda s: print(s, file=sys.stderr, flush=True))
     out["lattice"], out["instances"], out["n_up"] = a.lattice, a.instances, n_up
     if a.compare:
         cmp_secs = [(r["kx"], r["ky"]) for r in out["sectors"][:a.compare]]

@@ -145,7 +145,8 @@ int ed_hv(ed_op op, int32_t dtype, const void* v, void* w, void* stream);
  * and result as ed_hv, but the gather stream (u = v/norm by row, or the
  * sector's gather image) is built by every call in a fixed 64 MiB device
  * array of the library (one per GPU, held until the process ends) instead
- * of the operator's buffers, and the kernel variant is not timed (the one an
+ * of the operator's buffers (its device address is looked up by the first
+ * call on each device), and the kernel variant is not timed (the one an
  * earlier ed_hv tuned, else the default).  Fails with ED_ENOMEM, before
  * launching anything, when the stream exceeds 64 MiB: 4,194,304 complex128
  * or 8,388,608 complex64 elements of the layout (the gather image holds
@@ -200,6 +201,26 @@ int ed_pair_correlations(ed_basis b, const void* bra, const void* ket, int32_t n
  * arrays states [count] and coeffs (complex128 [count]); capacity 0 only
  * counts.  Needs C(N, n_ones) < 2^31. */
 int ed_unwrap(ed_basis b, const void* v, int64_t capacity, int64_t* states, double* coeffs, int64_t* count);
+
+/* Reduced density matrix of the cut A = sites cut[0..ncut) from two
+ * momentum-basis vectors (DEVICE complex128 [D]) of basis b: dm(j, k) =
+ * sum_e psi1(j, e) conj(psi2(k, e)), j, k configurations of A (bit l = site
+ * cut[l]), e of the other sites, psi1, psi2 the unwrapped vectors with the
+ * unwrap threshold (entries with |re| and |im| <= 1e-8 count as zero) --
+ * density_matrixij (analysis/src/Entropy.cpp:170-205, driver
+ * analysis/src/dm.cpp:10-88; with psi1 = psi2 density_matrix, :135-168).
+ * n_cut_ones = -1: the whole 2^ncut x 2^ncut matrix; n_cut_ones >= 0: its
+ * block of configurations of A with that many up spins, rows and columns by
+ * ascending configuration (density_matrixij_Sz, :207-265).  dm is HOST
+ * complex128, column-major (element (j, k) at j + k rows), at most 32768
+ * rows.  Needs C(N, n_ones) < 2^31 and two translation generators. */
+int ed_density_matrix(ed_basis b, const void* v1, const void* v2, int32_t ncut, const int32_t* cut,
+                      int32_t n_cut_ones, double* dm);
+/* Entropies of a HOST complex128 density matrix (column-major, n x n):
+ * *renyi2 = -log2 sum_ij |dm_ij|^2 (Renyi_Entropy, Entropy.cpp:334-355),
+ * *von_neumann = -sum lambda log2 lambda over its eigenvalues with |lambda| >
+ * 1e-8 (Von_Neumann_Entropy, Entropy.cpp:307-331; eigenvalues from rocSOLVER). */
+int ed_entanglement_entropies(const double* dm, int64_t n, double* renyi2, double* von_neumann);
 
 /* All D eigenvalues (ascending) and, if evecs is non-NULL, the eigenvectors
  * (column-major D x D complex128, column k = eigenvector k) of the sector
@@ -306,6 +327,38 @@ int ed_block_hv(ed_block blk, int32_t dtype, const void* V, void* W, void* strea
  * lowest Ritz vector of each instance (residual[m], rayleigh[m]).
  * p->mode and p->ncycles are ignored. */
 int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double* evals, ed_lanczos_stats* stats);
+
+/* ---- Multi-sector block operators: M coupling instances on each of nsec
+ * sectors of one lattice and one n_ones in one launch ----------------------
+ * The reference sweeps the sectors one ARPACK run at a time
+ * (ARPACK_full_eigs.cpp:169-277, one coupling set per run).  A multi-sector
+ * block applies the M Hamiltonians of every sector in one launch over a
+ * queue of (sector, 256-row tile) items (uneven dimensions are uneven tile
+ * counts); terms as ed_block_create (same site lists, couplings per
+ * instance, the same for every sector).  Vectors: the sectors' interleaved
+ * D_s x M blocks (element (i, m) at i*M + m) concatenated in sector order;
+ * ed_mblock_rows gives each sector's first row and D_s (sector = nsec: the
+ * total).  When a sector cannot take the shared block kernel (no compiled
+ * torus shape, bowtie terms, M not in {2, 4, 8}) the sectors run one
+ * launch each; ed_mblock_fast says which. */
+typedef struct ed_mblock_s* ed_mblock;
+int ed_mblock_create(const ed_basis* bases, int32_t nsec, int32_t M, const ed_terms* terms, ed_mblock* out);
+void ed_mblock_destroy(ed_mblock mb);
+int ed_mblock_rows(ed_mblock mb, int32_t sector, int64_t* row0, int64_t* D);
+int ed_mblock_fast(ed_mblock mb);
+/* W = H V for all sectors and instances (DEVICE vectors, layout above).
+ * Asynchronous on `stream`. */
+int ed_mblock_hv(ed_mblock mb, int32_t dtype, const void* V, void* W, void* stream);
+/* ed_block_lanczos for every (sector, instance) at once, in lock step: each
+ * Lanczos step is one multi-sector H*V launch, then every sector's
+ * reorthogonalisation.  Krylov storage is (p->iterations + 1) vectors of all
+ * sectors (the complex128 polish of a complex64 run at most 61): choose the
+ * sector set to fit device memory.  evals[(s*M + m)*nev
+ * + k] ascending per instance; errors (optional, same layout) the FP64
+ * residual bound of each eigenvalue; stats as ed_block_lanczos (residual and
+ * rayleigh of the lowest pair of the first 16 (sector, instance) pairs). */
+int ed_mblock_lanczos(ed_mblock mb, const ed_lanczos_params* p, double* evals, double* errors,
+                      ed_lanczos_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,9 @@
 //   or_tables_new          or_hv on adopted reference tables = the fixtures' w
 //   or_hv_rows_search /    the fixtures' w on the same rows (2-D sectors); the
 //   or_rows_targets_search full-table oracle on the relabelled tilted cluster
+//   or_density_matrix      numpy partial trace of random full-basis vectors, dimer
+//                          product states (entropy = number of cut singlets),
+//                          the Sz block = the block of the full matrix
 //   or_lanczos             closed-form spectra (4-site ring, XY chain free fermions,
 //                          Ising limit), numpy eigvalsh on dense H
 #include <cstdint>
@@ -644,6 +647,68 @@ OR_API void or_hv_rows_search(int N, const int32_t* T1, const int32_t* T2, doubl
         });
         w[q] = acc;
     });
+}
+
+// ------------------------------------------------------- density matrices ---
+// analysis/src/Entropy.cpp:57-82 (set_bits_Cut) and :84-106 (findstate_psi):
+// the cut bits of a state replaced by the bits of a cut configuration; binary
+// search of a state in an unwrapped vector sorted by state.
+static int64_t set_bits_cut(int64_t big, int64_t small, int ncut, const int32_t* cut) {
+    for (int i = 0; i < ncut; i++) big = btest(small, i) ? ibset(big, cut[i]) : ibclr(big, cut[i]);
+    return big;
+}
+
+static int64_t findstate_psi(int64_t s, int64_t n, const int64_t* st) {
+    int64_t lo = 0, hi = n - 1;
+    while (hi >= lo) {
+        int64_t m = lo + (hi - lo) / 2;
+        if (s < st[m]) hi = m - 1;
+        else if (s > st[m]) lo = m + 1;
+        else return m;
+    }
+    return -1;
+}
+
+// Entropy.cpp:170-205 (density_matrixij): for every configuration k of the
+// cut and every entry i of psi1, j = the cut bits of state i, the state with
+// its cut bits set to k looked up in psi2, dm(j, k) += c1_i conj(c2).  psi1,
+// psi2: unwrapped vectors sorted by state (states int64, coefficients
+// interleaved complex128).  dm: column-major complex128 [2^ncut x 2^ncut].
+// With n_cut_ones >= 0, Entropy.cpp:207-265 (density_matrixij_Sz): only
+// entries whose cut has n_cut_ones up spins, k over those configurations in
+// ascending order (constrained_dets), rows by the rank of j among them
+// (findstate_basis); dm is [C(ncut, n_cut_ones)]^2.
+OR_API void or_density_matrix(int64_t n1, const int64_t* st1, const double* c1_, int64_t n2, const int64_t* st2,
+                              const double* c2_, int ncut, const int32_t* cut, int n_cut_ones, double* dm_) {
+    const cd* c1 = (const cd*)c1_;
+    const cd* c2 = (const cd*)c2_;
+    cd* dm = (cd*)dm_;
+    std::vector<int64_t> basis;
+    if (n_cut_ones < 0) {
+        for (int64_t k = 0; k < (int64_t(1) << ncut); k++) basis.push_back(k);
+    } else {
+        std::vector<int64_t> b(or_n_choose_k(ncut, n_cut_ones));
+        or_gosper(ncut, n_cut_ones, b.data());
+        basis = b;
+    }
+    const int64_t ndm = (int64_t)basis.size();
+    for (int64_t q = 0; q < ndm * ndm; q++) dm[q] = 0.0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t k = 0; k < ndm; k++) {
+        const int64_t sector_k = basis[k];
+        for (int64_t i = 0; i < n1; i++) {
+            const int64_t big = st1[i];
+            int64_t j = 0;
+            int ones = 0;
+            for (int l = 0; l < ncut; l++)
+                if (btest(big, cut[l])) { j = ibset(j, l); ones++; }
+            if (n_cut_ones >= 0 && ones != n_cut_ones) continue;
+            const int64_t sk = findstate_psi(set_bits_cut(big, sector_k, ncut, cut), n2, st2);
+            if (sk < 0) continue;
+            const int64_t row = n_cut_ones < 0 ? j : findstate_psi(j, ndm, basis.data());
+            dm[row + k * ndm] += c1[i] * std::conj(c2[sk]);
+        }
+    }
 }
 
 // --------------------------------------------------------------- Lanczos ---

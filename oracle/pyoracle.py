@@ -80,6 +80,7 @@ def lib():
         L.or_count_targets.argtypes = [C.c_void_p, C.POINTER(Terms), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.or_make_h.argtypes = [C.c_void_p, C.POINTER(Terms), f64p]
         L.or_start_vector.argtypes = [C.c_uint64, C.c_int64, f64p]
+        L.or_density_matrix.argtypes = [C.c_int64, i64p, f64p, C.c_int64, i64p, f64p, C.c_int, i32p, C.c_int, f64p]
         L.or_tables_new.restype = C.c_void_p
         L.or_tables_new.argtypes = [C.c_int, C.c_int, i32p, i32p, C.c_double, C.c_double, C.c_int64, C.c_int64,
                                     C.c_int64, C.c_int]
@@ -462,3 +463,19 @@ def cexp(re, im):
     out = np.zeros(2)
     lib().or_cexp(re, im, out)
     return complex(out[0], out[1])
+
+
+def density_matrix(st1, c1, st2, c2, cut, n_cut_ones=None):
+    """density_matrixij / density_matrixij_Sz (analysis/src/Entropy.cpp:170-265)
+    of two unwrapped vectors (states sorted, complex coefficients)."""
+    st1 = np.ascontiguousarray(st1, np.int64)
+    st2 = np.ascontiguousarray(st2, np.int64)
+    c1 = np.ascontiguousarray(c1, np.complex128)
+    c2 = np.ascontiguousarray(c2, np.complex128)
+    cut = np.ascontiguousarray(cut, np.int32)
+    nco = -1 if n_cut_ones is None else int(n_cut_ones)
+    m = (1 << len(cut)) if nco < 0 else math.comb(len(cut), nco)
+    dm = np.zeros(2 * m * m)
+    lib().or_density_matrix(len(st1), st1, c1.view(np.float64), len(st2), st2, c2.view(np.float64), len(cut), cut,
+                            nco, dm)
+    return dm.view(np.complex128).reshape(m, m, order="F")

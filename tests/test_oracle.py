@@ -361,3 +361,63 @@ def test_hv_rows_search_tilted_relabelled():
         assert np.array_equal(ws, w)
         wo = o.hv(v)[rows]
         assert np.linalg.norm(w - wo) <= 1e-14 * np.linalg.norm(wo)
+
+
+# ------------------------------------------------------ density matrices ----
+def _partial_trace(N, n, states, coef1, coef2, cut):
+    """Brute force: the full-basis vectors as 2^N arrays, the cut sites moved
+    to the front, rho = Psi1 Psi2^H over the remaining sites."""
+    full1 = np.zeros(1 << N, complex)
+    full2 = np.zeros(1 << N, complex)
+    full1[states] = coef1
+    full2[states] = coef2
+    rest = [s for s in range(N) if s not in cut]
+    # axes of the reshaped tensor: bit s of the index is axis N-1-s
+    t1 = full1.reshape([2] * N)
+    t2 = full2.reshape([2] * N)
+    order = [N - 1 - s for s in reversed(cut)] + [N - 1 - s for s in reversed(rest)]
+    a1 = np.transpose(t1, order).reshape(1 << len(cut), -1)
+    a2 = np.transpose(t2, order).reshape(1 << len(cut), -1)
+    return a1 @ a2.conj().T
+
+
+@pytest.mark.parametrize("N,n,cut", [(8, 4, [0, 1, 2]), (10, 5, [7, 2, 4, 9]), (11, 6, [0, 3, 5, 6, 8]),
+                                     (12, 6, [11, 0, 5, 6, 1, 2])])
+def test_density_matrix_vs_partial_trace(N, n, cut):
+    rng = np.random.default_rng(N)
+    st = np.array(Hp.states(N, n), np.int64)
+    c1 = rng.normal(size=len(st)) + 1j * rng.normal(size=len(st))
+    c2 = rng.normal(size=len(st)) + 1j * rng.normal(size=len(st))
+    dm = P.density_matrix(st, c1, st, c2, cut)
+    ref = _partial_trace(N, n, st, c1, c2, cut)
+    assert np.abs(dm - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert abs(np.trace(dm) - np.vdot(c2, c1)) <= 1e-12 * abs(np.vdot(c2, c1))
+    # the Sz blocks are the blocks of the full matrix
+    for a in range(len(cut) + 1):
+        rows = [x for x in range(1 << len(cut)) if bin(x).count("1") == a]
+        blk = P.density_matrix(st, c1, st, c2, cut, n_cut_ones=a)
+        assert np.array_equal(blk, dm[np.ix_(rows, rows)])
+
+
+def test_density_matrix_of_dimer_products():
+    """A product of singlets on pairs (0,1), (2,3), ...: the cut's reduced
+    density matrix is that of the singlets it splits (1/2 each), so the
+    Renyi-2 and von Neumann entropies are the number of split pairs, in bits."""
+    N = 8
+    pairs = [(0, 1), (2, 3), (4, 5), (6, 7)]
+    amp = {}
+    for bits in range(16):
+        s, sign = 0, 1.0
+        for q, (a, b) in enumerate(pairs):
+            up = a if (bits >> q) & 1 else b
+            s |= 1 << up
+            sign *= 1.0 if (bits >> q) & 1 else -1.0
+        amp[s] = sign / 4.0
+    st = np.array(sorted(amp), np.int64)
+    c = np.array([amp[s] for s in st], complex)
+    for cut, split in (([0], 1), ([0, 1], 0), ([0, 2], 2), ([1, 2, 4, 7], 4), ([0, 1, 2], 1)):
+        dm = P.density_matrix(st, c, st, c, cut)
+        w = np.linalg.eigvalsh(dm)
+        w = w[np.abs(w) > 1e-8]
+        assert abs(-np.log2(np.sum(np.abs(dm) ** 2)) - split) < 1e-12
+        assert abs(-np.sum(w * np.log2(w)) - split) < 1e-12
