@@ -33,6 +33,13 @@ __device__ __forceinline__ cd2 ld(const cd2* p, int64_t i) { return p[i]; }
 __device__ __forceinline__ cd2 ld(const cf2* p, int64_t i) { cf2 v = p[i]; return cd2{(double)v.re, (double)v.im}; }
 __device__ __forceinline__ void st(cd2* p, int64_t i, double re, double im) { p[i] = cd2{re, im}; }
 __device__ __forceinline__ void st(cf2* p, int64_t i, double re, double im) { p[i] = cf2{(float)re, (float)im}; }
+// elements 2t and 2t+1 (p 16-byte aligned): one 16-byte load for complex64
+__device__ __forceinline__ void ld2(const cf2* p, int64_t t, cd2& a, cd2& b) {
+    const float4 q = ((const float4*)p)[t];
+    a = cd2{(double)q.x, (double)q.y};
+    b = cd2{(double)q.z, (double)q.w};
+}
+__device__ __forceinline__ void ld2(const cd2* p, int64_t t, cd2& a, cd2& b) { a = p[2 * t]; b = p[2 * t + 1]; }
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
@@ -56,8 +63,10 @@ __global__ void k_start(int64_t D, uint64_t seed, VT* v) {
 // partial[blk][k] = sum over the block's rows of conj(X_k[i]) * y[i] for the
 // MB vectors X_k = X + k*ldx (MB a compile-time chunk, so the accumulators
 // stay in registers).  Wave reduction by shuffles, then across the block's
-// waves through LDS: one barrier per chunk.
-template <int MB, typename XT, typename YT>
+// waves through LDS: one barrier per chunk.  PAIR: each thread takes element
+// pairs (2t, 2t+1), 16-byte loads for complex64 (X, y and ldx keep every
+// vector 16-byte aligned); an odd last element goes to thread 0 of block 0.
+template <int MB, typename XT, typename YT, bool PAIR>
 __global__ void __launch_bounds__(kRedThreads) k_multidot(int64_t D, const XT* __restrict__ X, int64_t ldx,
                                                            const YT* __restrict__ y, cd2* __restrict__ partial,
                                                            int kofs) {
@@ -65,13 +74,36 @@ __global__ void __launch_bounds__(kRedThreads) k_multidot(int64_t D, const XT* _
     double are[MB], aim[MB];
 #pragma unroll
     for (int k = 0; k < MB; k++) { are[k] = 0.0; aim[k] = 0.0; }
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
-        cd2 yv = ld(y, i);
+    if (PAIR) {
+        const int64_t np = D >> 1;
+        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x) {
+            cd2 y0, y1;
+            ld2(y, t, y0, y1);
 #pragma unroll
-        for (int k = 0; k < MB; k++) {
-            cd2 x = ld(X + k * ldx, i);
-            are[k] += x.re * yv.re + x.im * yv.im;
-            aim[k] += x.re * yv.im - x.im * yv.re;
+            for (int k = 0; k < MB; k++) {
+                cd2 x0, x1;
+                ld2(X + k * ldx, t, x0, x1);
+                are[k] += x0.re * y0.re + x0.im * y0.im + x1.re * y1.re + x1.im * y1.im;
+                aim[k] += x0.re * y0.im - x0.im * y0.re + x1.re * y1.im - x1.im * y1.re;
+            }
+        }
+        if ((D & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+            const cd2 yv = ld(y, D - 1);
+            for (int k = 0; k < MB; k++) {
+                const cd2 x = ld(X + k * ldx, D - 1);
+                are[k] += x.re * yv.re + x.im * yv.im;
+                aim[k] += x.re * yv.im - x.im * yv.re;
+            }
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += (int64_t)gridDim.x * blockDim.x) {
+            cd2 yv = ld(y, i);
+#pragma unroll
+            for (int k = 0; k < MB; k++) {
+                cd2 x = ld(X + k * ldx, i);
+                are[k] += x.re * yv.re + x.im * yv.im;
+                aim[k] += x.re * yv.im - x.im * yv.re;
+            }
         }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -166,15 +198,24 @@ unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D
 unsigned grid_stream(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads); }
 
 
+template <typename XT, typename YT, bool PAIR>
+void launch_multidot_p(Ctx& c, int mm, const XT* X, int64_t ldx, const YT* y, unsigned g, int kofs) {
+    switch (mm) {
+        case 16: hipLaunchKernelGGL((k_multidot<16, XT, YT, PAIR>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        case 8: hipLaunchKernelGGL((k_multidot<8, XT, YT, PAIR>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        case 4: hipLaunchKernelGGL((k_multidot<4, XT, YT, PAIR>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        case 2: hipLaunchKernelGGL((k_multidot<2, XT, YT, PAIR>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+        default: hipLaunchKernelGGL((k_multidot<1, XT, YT, PAIR>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
+    }
+}
+
+// paired loads when every vector of the pass is 16-byte aligned
 template <typename XT, typename YT>
 void launch_multidot(Ctx& c, int mm, const XT* X, int64_t ldx, const YT* y, unsigned g, int kofs) {
-    switch (mm) {
-        case 16: hipLaunchKernelGGL((k_multidot<16, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
-        case 8: hipLaunchKernelGGL((k_multidot<8, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
-        case 4: hipLaunchKernelGGL((k_multidot<4, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
-        case 2: hipLaunchKernelGGL((k_multidot<2, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
-        default: hipLaunchKernelGGL((k_multidot<1, XT, YT>), dim3(g), dim3(kRedThreads), 0, c.st, c.D, X, ldx, y, c.d_partial, kofs); break;
-    }
+    const bool aligned = ((uintptr_t)X % 16 == 0) && ((uintptr_t)y % 16 == 0) &&
+                         (mm == 1 || ((size_t)ldx * sizeof(XT)) % 16 == 0);
+    if (aligned) launch_multidot_p<XT, YT, true>(c, mm, X, ldx, y, g, kofs);
+    else launch_multidot_p<XT, YT, false>(c, mm, X, ldx, y, g, kofs);
 }
 
 // out[k] = X_k^H y for k < m (any m), FP64, fixed reduction order.
@@ -404,7 +445,8 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
     unsigned g = grid_for(D);
     int m = (int)std::min<int64_t>(p.iterations, D);
     VT *V = nullptr, *w = nullptr;
-    HIPCHK(hipMalloc(&V, (size_t)(m + 1) * D * sizeof(VT)));
+    const int64_t ldv = (D + 63) / 64 * 64;   // 16-byte aligned Krylov vectors (paired loads)
+    HIPCHK(hipMalloc(&V, (size_t)(m + 1) * ldv * sizeof(VT)));
     HIPCHK(hipMalloc(&w, D * sizeof(VT)));
     cd2* x = nullptr;
     HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
@@ -425,14 +467,14 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
         double e0 = 0.0, eprev = 1e300;
         int k = 0;
         for (; k < mit; k++) {
-            VT* vk = V + (int64_t)k * D;
+            VT* vk = V + (int64_t)k * ldv;
             RC(hv<VT>(c, vk, w));
             RC((multidot<VT, VT>(c, 1, vk, 0, w, q)));
             double alpha = q[0].real();
             std::vector<std::complex<double>> cf{alpha};
             if (k > 0) {
                 cf.insert(cf.begin(), betas.back());
-                RC((multiaxpy<VT, VT>(c, 2, V + (int64_t)(k - 1) * D, D, cf, 1.0, w)));
+                RC((multiaxpy<VT, VT>(c, 2, V + (int64_t)(k - 1) * ldv, ldv, cf, 1.0, w)));
             } else {
                 RC((multiaxpy<VT, VT>(c, 1, vk, 0, cf, 1.0, w)));
             }
@@ -445,9 +487,9 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
                     RC(dnorm2(c, w, bb));
                     if (bb > 0.5 * nb2) { have_bb = true; break; }
                 }
-                RC((multidot<VT, VT>(c, k + 1, V, D, w, q)));
+                RC((multidot<VT, VT>(c, k + 1, V, ldv, w, q)));
                 if (pass == 0) alpha += q[k].real();
-                RC((multiaxpy<VT, VT>(c, k + 1, V, D, q, 1.0, w)));
+                RC((multiaxpy<VT, VT>(c, k + 1, V, ldv, q, 1.0, w)));
                 for (size_t e = 0; e < prev.size(); e++) {
                     RC((multidot<cd2, VT>(c, 1, prev[e], 0, w, q)));
                     RC((multiaxpy<cd2, VT>(c, 1, prev[e], 0, q, 1.0, w)));
@@ -466,14 +508,14 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
             if (beta < 1e-13 * std::max(1.0, std::fabs(alpha))) stop = true;   // invariant subspace
             if (stop || k + 1 == mit) { k++; break; }
             betas.push_back(beta);
-            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, V + (int64_t)(k + 1) * D);
+            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / beta, w, V + (int64_t)(k + 1) * ldv);
         }
         int nvs = k;
         std::vector<double> tvec, off(betas.begin(), betas.begin() + std::max(0, nvs - 1));
         std::vector<double> al(alphas.begin(), alphas.begin() + nvs);
         if (!lowest(al, off, e0, &tvec)) return ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge");
         HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
-        RC((accumulate<VT>(c, nvs, V, D, tvec, x)));
+        RC((accumulate<VT>(c, nvs, V, ldv, tvec, x)));
         for (int pass = 0; pass < 2; pass++)
             for (size_t e = 0; e < prev.size(); e++) {
                 RC((multidot<cd2, cd2>(c, 1, prev[e], 0, x, q)));
@@ -1412,6 +1454,358 @@ int bfp64_check(BCtx& c, const cd2* x, std::vector<double>& rq, std::vector<doub
     return ED_OK;
 }
 
+// ---- graph-captured full-reorthogonalisation Lanczos --------------------
+// run_full with every scalar of a step on the device and the step captured
+// once as a hipGraph, replayed for every k.  The step index k lives in device
+// memory: kernels locate v_k = V + k D themselves and bound their loops by k,
+// so one graph serves all steps.  H*v reads the fixed buffer vc, which the
+// step's last kernel fills with v_{k+1} beside V[k+1].  Device scalars:
+// ab[2k] = alpha_k, ab[2k+1] = beta_{k+1}; hc[0 .. nv) and hc[nv .. 2 nv) the
+// two Gram-Schmidt passes' coefficients; hs[0] = <v_k, H v_k>, hs[1] = |w|^2
+// before pass 1, hs[2] = |w|^2 after it, hs[3] = |w|^2 at the end, hs[4 + e]
+// and hs[4 + np + e] the deflation coefficients of earlier eigenvector e in
+// either pass.  With ED_LZ_SELECTIVE the second pass's kernels read hs[1],
+// hs[2] and return at once unless hs[2] <= hs[1] / 2, the rule of run_full.
+// The host reads ab[] every 5 steps, where run_full tests convergence, so the
+// steps, stopping points and Ritz vectors are run_full's.
+constexpr int kGChunk = 16;        // Krylov vectors per workgroup of k_gdot
+constexpr int kFixed = INT32_MIN;  // a vector argument that does not move with k
+
+__device__ __forceinline__ bool g_skip(const cd2* gate) { return gate && !(gate[2].re <= 0.5 * gate[1].re); }
+
+// part[j * g + blk] = the block's share of <X_j, y>, j < n, X_j = X0 + j ldx,
+// X0 = X + (k + xrel) ldx (xrel != kFixed), n = (nk ? k + 1 : 0) + nadd.
+// Grid (g, chunks of kGChunk vectors); chunks at or past n exit at once.
+// Each thread takes element pairs (2t, 2t+1): complex64 vectors then load
+// 16 bytes per access, which streams at the HBM read rate where 8-byte
+// accesses do not; X, y and ldx keep every vector 16-byte aligned.  An odd
+// last element is added by thread 0 of workgroup column 0.
+template <typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_gdot(int64_t D, const XT* __restrict__ X, int64_t ldx, int xrel,
+                                                       const YT* __restrict__ y, cd2* __restrict__ part,
+                                                       const int* __restrict__ step, int nk, int nadd,
+                                                       const cd2* gate) {
+    if (g_skip(gate)) return;
+    const int k = *step;
+    const int n = (nk ? k + 1 : 0) + nadd;
+    const int j0 = blockIdx.y * kGChunk;
+    if (j0 >= n) return;
+    const XT* X0 = (xrel == kFixed ? X : X + (int64_t)(k + xrel) * ldx) + (int64_t)j0 * ldx;
+    const int nb = min(kGChunk, n - j0);
+    __shared__ double red[kRedThreads / 64][kGChunk][2];
+    double are[kGChunk], aim[kGChunk];
+#pragma unroll
+    for (int j = 0; j < kGChunk; j++) { are[j] = 0.0; aim[j] = 0.0; }
+    const int64_t np = D >> 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np; t += (int64_t)gridDim.x * blockDim.x) {
+        cd2 y0, y1;
+        ld2(y, t, y0, y1);
+#pragma unroll
+        for (int j = 0; j < kGChunk; j++) {
+            if (j < nb) {
+                cd2 x0, x1;
+                ld2(X0 + j * ldx, t, x0, x1);
+                are[j] += x0.re * y0.re + x0.im * y0.im + x1.re * y1.re + x1.im * y1.im;
+                aim[j] += x0.re * y0.im - x0.im * y0.re + x1.re * y1.im - x1.im * y1.re;
+            }
+        }
+    }
+    if ((D & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const cd2 yv = ld(y, D - 1);
+        for (int j = 0; j < nb; j++) {
+            const cd2 x = ld(X0 + j * ldx, D - 1);
+            are[j] += x.re * yv.re + x.im * yv.im;
+            aim[j] += x.re * yv.im - x.im * yv.re;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kGChunk; j++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            are[j] += __shfl_xor(are[j], off, 64);
+            aim[j] += __shfl_xor(aim[j], off, 64);
+        }
+        if (lane == 0) { red[wv][j][0] = are[j]; red[wv][j][1] = aim[j]; }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nb) {
+        double r = 0.0, im = 0.0;
+        for (int q = 0; q < kRedThreads / 64; q++) { r += red[q][threadIdx.x][0]; im += red[q][threadIdx.x][1]; }
+        part[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = cd2{r, im};
+    }
+}
+
+// out[j] = compensated sum of part[j * g + b] over b (k_colsum's order),
+// j < n as in k_gdot; one wave per column, columns at or past n exit.
+__global__ void __launch_bounds__(64) k_gcolsum(const cd2* __restrict__ part, int g, cd2* __restrict__ out,
+                                                const int* __restrict__ step, int nk, int nadd) {
+    const int n = (nk ? *step + 1 : 0) + nadd;
+    if ((int)blockIdx.x >= n) return;
+    __shared__ double red[64][2];
+    const int col = blockIdx.x, lane = threadIdx.x;
+    double sr = 0, cr = 0, si = 0, ci = 0;
+    for (int b = lane; b < g; b += 64) {
+        const cd2 p = part[(int64_t)col * g + b];
+        double yr = p.re - cr, tr = sr + yr;
+        cr = (tr - sr) - yr; sr = tr;
+        double yi = p.im - ci, tj = si + yi;
+        ci = (tj - si) - yi; si = tj;
+    }
+    red[lane][0] = sr - cr;
+    red[lane][1] = si - ci;
+    __syncthreads();
+    if (lane == 0) {
+        sr = 0; cr = 0; si = 0; ci = 0;
+        for (int l = 0; l < 64; l++) {
+            double yr = red[l][0] - cr, tr = sr + yr;
+            cr = (tr - sr) - yr; sr = tr;
+            double yi = red[l][1] - ci, tj = si + yi;
+            ci = (tj - si) - yi; si = tj;
+        }
+        out[col] = cd2{sr, si};
+    }
+}
+
+// y -= sum_{j < n} c[j] X_j (n and X_j as in k_gdot), one element per thread
+template <typename XT, typename YT>
+__global__ void __launch_bounds__(kRedThreads) k_gaxpy(int64_t D, const XT* __restrict__ X, int64_t ldx, int xrel,
+                                                        const cd2* __restrict__ c, YT* __restrict__ y,
+                                                        const int* __restrict__ step, int nk, int nadd,
+                                                        const cd2* gate) {
+    if (g_skip(gate)) return;
+    const int k = *step;
+    const int n = (nk ? k + 1 : 0) + nadd;
+    const XT* X0 = xrel == kFixed ? X : X + (int64_t)(k + xrel) * ldx;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 yv = ld(y, i);
+    double re = yv.re, im = yv.im;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        cd2 x[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[q] = ld(X0 + (int64_t)(j + q) * ldx, i);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const cd2 cf = c[j + q];
+            re -= cf.re * x[q].re - cf.im * x[q].im;
+            im -= cf.re * x[q].im + cf.im * x[q].re;
+        }
+    }
+    for (; j < n; j++) {
+        const cd2 x = ld(X0 + (int64_t)j * ldx, i), cf = c[j];
+        re -= cf.re * x.re - cf.im * x.im;
+        im -= cf.re * x.im + cf.im * x.re;
+    }
+    st(y, i, re, im);
+}
+
+// w -= alpha0 v_k + beta_k v_{k-1}, alpha0 = Re hs[0], beta_0 = 0
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_gthree(int64_t D, const VT* __restrict__ V, int64_t ldv,
+                                                         VT* __restrict__ w, const cd2* __restrict__ hs,
+                                                         const double* __restrict__ ab, const int* __restrict__ step) {
+    const int k = *step;
+    const double al = hs[0].re, be = k > 0 ? ab[2 * k - 1] : 0.0;
+    const VT* vk = V + (int64_t)k * ldv;
+    const VT* vm = V + (int64_t)(k > 0 ? k - 1 : 0) * ldv;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 x = ld(w, i), y = ld(vk, i), z = ld(vm, i);
+    st(w, i, x.re - al * y.re - be * z.re, x.im - al * y.im - be * z.im);
+}
+
+// ab[2k] = alpha0 + Re(pass-1 coefficient of v_k), ab[2k+1] = |w| at the end
+__global__ void k_gfinish(const cd2* __restrict__ hs, const cd2* __restrict__ h1, double* __restrict__ ab,
+                          const int* __restrict__ step) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int k = *step;
+    ab[2 * k] = hs[0].re + h1[k].re;
+    ab[2 * k + 1] = sqrt(fmax(hs[3].re, 0.0));
+}
+
+// V[k+1] = vc = w / beta_{k+1}; zero once the Krylov space is invariant
+// (beta < 1e-13 max(1, |alpha|), the stopping rule of run_full) or past the
+// last vector slot (k + 1 >= nmax)
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_gnext(int64_t D, const VT* __restrict__ w, VT* __restrict__ V,
+                                                        int64_t ldv, VT* __restrict__ vc, const double* __restrict__ ab,
+                                                        const int* __restrict__ step, int nmax) {
+    const int k = *step;
+    if (k + 1 >= nmax) return;
+    const double al = ab[2 * k], be = ab[2 * k + 1];
+    const double f = be >= 1e-13 * fmax(1.0, fabs(al)) ? 1.0 / be : 0.0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 x = ld(w, i);
+    st(V + (int64_t)(k + 1) * ldv, i, f * x.re, f * x.im);
+    st(vc, i, f * x.re, f * x.im);
+}
+
+template <typename VT>
+int run_full_graph(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D;
+    const int how_many = (int)std::min<int64_t>(p.nev, D);
+    const unsigned g = grid_for(D), gs = grid_stream(D);
+    const int m = (int)std::min<int64_t>(p.iterations, D);
+    const int nv = ((m + 1 + kGChunk - 1) / kGChunk) * kGChunk;   // Krylov vector slots, rounded to chunks
+    const int nchunk = nv / kGChunk;
+    const int dt = std::is_same<VT, cd2>::value ? ED_C128 : ED_C64;
+    const bool sel = (p.flags & ED_LZ_SELECTIVE) != 0;
+    VT *V = nullptr, *w = nullptr, *vc = nullptr;
+    cd2 *x = nullptr, *part = nullptr, *hc = nullptr, *hs = nullptr;
+    double* ab = nullptr;
+    int* step = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        dfree(V); dfree(w); dfree(vc); dfree(x); dfree(part); dfree(hc); dfree(hs); dfree(ab); dfree(step);
+    };
+#define GRC(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != ED_OK) {            \
+            cleanup();                 \
+            return rc_;                \
+        }                              \
+    } while (0)
+#define GHIP(expr) GRC((expr) == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "%s failed", #expr))
+    // Krylov vector j at V + j ldv, ldv a multiple of 64 elements: every
+    // vector 16-byte aligned for the paired loads of k_gdot
+    const int64_t ldv = (D + 63) / 64 * 64;
+    GHIP(hipMalloc(&V, (size_t)(m + 1) * ldv * sizeof(VT)));
+    GHIP(hipMalloc(&w, D * sizeof(VT)));
+    GHIP(hipMalloc(&vc, D * sizeof(VT)));
+    GHIP(hipMalloc(&x, D * sizeof(cd2)));
+    GHIP(hipMalloc(&part, (size_t)nv * g * sizeof(cd2)));
+    GHIP(hipMalloc(&hc, 2 * (size_t)nv * sizeof(cd2)));
+    GHIP(hipMalloc(&hs, (4 + 2 * (size_t)how_many) * sizeof(cd2)));
+    GHIP(hipMalloc(&ab, 2 * (size_t)(m + 1) * sizeof(double)));
+    GHIP(hipMalloc(&step, sizeof(int)));
+    std::vector<std::complex<double>> q;
+    std::vector<double> hab;
+    for (int num = 0; num < how_many; num++) {
+        const int mit = (int)std::min<int64_t>(m, D - num);
+        const int np = (int)prev.size();
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed + (uint64_t)num, V);
+        GHIP(hipGetLastError());
+        for (int pass = 0; pass < 2; pass++)
+            for (int k = 0; k < np; k++) {
+                GRC((multidot<cd2, VT>(c, 1, prev[k], 0, V, q)));
+                GRC((multiaxpy<cd2, VT>(c, 1, prev[k], 0, q, 1.0, V)));
+            }
+        double nn;
+        GRC(dnorm2(c, V, nn));
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), V, V);
+        hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, V, vc);
+        GHIP(hipGetLastError());
+        // one eager H*v: allocates the gather stream outside the capture and
+        // times the product (the graph's H*v share is this time per step)
+        const double ms_before = c.hv_ms;
+        GRC(hv<VT>(c, vc, w));
+        const double hv_once = c.hv_ms - ms_before;
+        const int zero = 0;
+        GHIP(hipMemcpyAsync(step, &zero, sizeof(int), hipMemcpyHostToDevice, c.st));
+        GHIP(hipStreamSynchronize(c.st));
+        // capture one step
+        hipGraph_t graph;
+        GHIP(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+        {
+            int rc = op_hv(c.op, dt, vc, w, c.st);
+            auto dot1 = [&](const auto* X, int xrel, const auto* y, cd2* out, const cd2* gate) {
+                using XT = std::remove_cv_t<std::remove_pointer_t<decltype(X)>>;
+                using YT = std::remove_cv_t<std::remove_pointer_t<decltype(y)>>;
+                hipLaunchKernelGGL((k_gdot<XT, YT>), dim3(g, 1), dim3(kRedThreads), 0, c.st, D, X, ldv, xrel, y, part,
+                                   step, 0, 1, gate);
+                hipLaunchKernelGGL(k_gcolsum, dim3(1), dim3(64), 0, c.st, part, (int)g, out, step, 0, 1);
+            };
+            // alpha0 = <v_k, w>, three-term update, |w|^2
+            dot1(V, 0, w, hs + 0, nullptr);
+            hipLaunchKernelGGL((k_gthree<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, V, ldv, w, hs, ab, step);
+            dot1(w, kFixed, w, hs + 1, nullptr);
+            for (int pass = 0; pass < 2; pass++) {
+                const cd2* gate = pass == 1 && sel ? hs : nullptr;
+                cd2* h = hc + (size_t)pass * nv;
+                hipLaunchKernelGGL((k_gdot<VT, VT>), dim3(g, nchunk), dim3(kRedThreads), 0, c.st, D, V, ldv, kFixed, w,
+                                   part, step, 1, 0, gate);
+                hipLaunchKernelGGL(k_gcolsum, dim3(nv), dim3(64), 0, c.st, part, (int)g, h, step, 1, 0);
+                hipLaunchKernelGGL((k_gaxpy<VT, VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, V, ldv, kFixed, h, w, step,
+                                   1, 0, gate);
+                for (int e = 0; e < np; e++) {
+                    cd2* he = hs + 4 + (size_t)pass * how_many + e;
+                    dot1(prev[e], kFixed, w, he, gate);
+                    hipLaunchKernelGGL((k_gaxpy<cd2, VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, prev[e], ldv, kFixed,
+                                       he, w, step, 0, 1, gate);
+                }
+                if (pass == 0) dot1(w, kFixed, w, hs + 2, nullptr);
+            }
+            dot1(w, kFixed, w, hs + 3, nullptr);
+            hipLaunchKernelGGL(k_gfinish, dim3(1), dim3(64), 0, c.st, hs, hc, ab, step);
+            hipLaunchKernelGGL((k_gnext<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, w, V, ldv, vc, ab, step, m + 1);
+            hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, c.st, step);
+            const hipError_t le = hipGetLastError();
+            GHIP(hipStreamEndCapture(c.st, &graph));
+            if (rc != ED_OK) { (void)hipGraphDestroy(graph); GRC(rc); }
+            if (le != hipSuccess) { (void)hipGraphDestroy(graph); GRC(ed_fail(ED_EDEVICE, "launch in the captured step failed")); }
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        GHIP(ie);
+        std::vector<double> alphas, betas;
+        double e0 = 0.0, eprev = 1e300;
+        int done = 0, nvs = 0;
+        while (nvs == 0) {
+            // replay up to the next multiple of 5 (or mit), where run_full checks
+            const int from = done;
+            const int upto = std::min(mit, (done / 5 + 1) * 5);
+            for (; done < upto; done++) GHIP(hipGraphLaunch(exec, c.st));
+            hab.resize(2 * (size_t)done);
+            GHIP(hipMemcpyAsync(hab.data(), ab, hab.size() * sizeof(double), hipMemcpyDeviceToHost, c.st));
+            GHIP(hipStreamSynchronize(c.st));
+            alphas.assign(done, 0.0);
+            betas.assign(done, 0.0);
+            for (int k = 0; k < done; k++) { alphas[k] = hab[2 * k]; betas[k] = hab[2 * k + 1]; }
+            for (int k = from; k < done && nvs == 0; k++)
+                if (betas[k] < 1e-13 * std::max(1.0, std::fabs(alphas[k]))) nvs = k + 1;   // invariant subspace
+            if (nvs == 0) {
+                std::vector<double> off(betas.begin(), betas.begin() + done - 1);
+                if (!lowest(alphas, off, e0, nullptr)) GRC(ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge"));
+                if (std::fabs(e0 - eprev) < p.tol || done == mit) nvs = done;
+                eprev = e0;
+            }
+            if (std::getenv("ED_PROGRESS")) std::fprintf(stderr, "[lanczos-graph-full] steps %d E0 %.15g\n", done, e0);
+        }
+        (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        c.hv_calls += done;
+        c.hv_ms += hv_once * done;   // the replayed products, at the eager product's time
+        std::vector<double> tvec, off(betas.begin(), betas.begin() + std::max(0, nvs - 1));
+        std::vector<double> al(alphas.begin(), alphas.begin() + nvs);
+        if (!lowest(al, off, e0, &tvec)) GRC(ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge"));
+        GHIP(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+        GRC((accumulate<VT>(c, nvs, V, ldv, tvec, x)));
+        for (int pass = 0; pass < 2; pass++)
+            for (int e = 0; e < np; e++) {
+                GRC((multidot<cd2, cd2>(c, 1, prev[e], 0, x, q)));
+                GRC((multiaxpy<cd2, cd2>(c, 1, prev[e], 0, q, 1.0, x)));
+            }
+        GRC(dnorm2(c, x, nn));
+        hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+        cd2* keep = nullptr;
+        GHIP(hipMalloc(&keep, D * sizeof(cd2)));
+        prev.push_back(keep);
+        GHIP(hipMemcpyAsync(keep, x, D * sizeof(cd2), hipMemcpyDeviceToDevice, c.st));
+        evals[num] = e0;
+    }
+    GHIP(hipStreamSynchronize(c.st));
+#undef GHIP
+#undef GRC
+    cleanup();
+    return ED_OK;
+}
+
 // ---- multi-sector block Lanczos ---------------------------------------
 // run_block_full for every (sector, instance) of a multi-sector block in lock
 // step.  Krylov vector k of all sectors is one contiguous n_all-element
@@ -1994,6 +2388,9 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
         rc = p->precision == ED_C128 ? run_reference<cd2>(c, *p, prev, evals) : run_reference<cf2>(c, *p, prev, evals);
     else if (p->mode == ED_LZ_THICK)
         rc = p->precision == ED_C128 ? run_trl<cd2>(c, *p, prev, evals) : run_trl<cf2>(c, *p, prev, evals);
+    else if (p->flags & ED_LZ_GRAPH)
+        rc = p->precision == ED_C128 ? run_full_graph<cd2>(c, *p, prev, evals)
+                                     : run_full_graph<cf2>(c, *p, prev, evals);
     else
         rc = p->precision == ED_C128 ? run_full<cd2>(c, *p, prev, evals) : run_full<cf2>(c, *p, prev, evals);
     if (stats) std::memset(stats->eval_error, 0, sizeof(stats->eval_error));

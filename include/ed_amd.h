@@ -260,10 +260,18 @@ enum ed_lanczos_mode { ED_LZ_REFERENCE = 0, ED_LZ_FULL = 1, ED_LZ_LEAN = 2, ED_L
 /* flags: ED_LZ_SELECTIVE makes the second Gram-Schmidt pass of ED_LZ_FULL
  * conditional: it runs only when the first pass removed more than
  * 1 - 1/sqrt(2) of the vector's norm (Daniel-Gragg-Kaufman-Stewart).
- * ED_LZ_GRAPH runs ED_LZ_LEAN with every step's scalars on the device and
- * three steps captured as one hipGraph, replayed without host round trips;
- * the host reads alpha/beta every 6 steps (so up to 5 steps past the
- * convergence point run, and are used). */
+ * ED_LZ_GRAPH keeps every step's scalars on the device and replays captured
+ * hipGraphs instead of launching each step's kernels from the host:
+ * - with ED_LZ_LEAN, three steps are one graph; the host reads alpha/beta
+ *   every 6 steps (so up to 5 steps past the convergence point run, and are
+ *   used);
+ * - with ED_LZ_FULL, one step (H*v, the three-term update, both Gram-Schmidt
+ *   passes over all Krylov vectors and earlier eigenvectors, alpha, beta,
+ *   the next vector) is one graph replayed for every k, the step index read
+ *   by the kernels from device memory; the host reads alpha/beta every 5
+ *   steps, where the eager solver tests convergence, so the steps, stopping
+ *   point and eigenpairs are the eager solver's (to reduction rounding);
+ *   with ED_LZ_SELECTIVE the second pass is skipped on the device. */
 enum ed_lanczos_flags { ED_LZ_SELECTIVE = 1, ED_LZ_GRAPH = 2 };
 
 typedef struct ed_lanczos_params {

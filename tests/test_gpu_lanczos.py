@@ -157,6 +157,52 @@ def test_graph_captured_lean_lanczos(gpu_ready, prec):
         assert st["hv_calls"] > 30
 
 
+@pytest.mark.parametrize("selective", [False, True])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_graph_captured_full_lanczos(gpu_ready, prec, selective):
+    """Full reorthogonalisation with one step captured as a hipGraph and
+    replayed for every k (device-side step index, coefficients, alpha and
+    beta; the selective second pass gated on the device): three eigenvalues
+    of a complex sector against dense and against the eager solver, and two
+    runs identical to the last bit (fixed-order reductions, replays that
+    reuse one graph)."""
+    ed = _ed()
+    print(f"[case] graph full lanczos {prec} selective={selective}", flush=True)
+    model = L.triangular(4, 4, Ch=0.2, J2=0.3)
+    o = P.OracleBasis(model, 8, 1, 2)
+    ev = np.linalg.eigvalsh(o.make_h())[:3]
+    basis = ed.Basis(ed.Lattice.of(model), 8, 1, 2)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    kw = dict(nev=3, iterations=150, ncycles=1, tol=1e-12, seed=5, precision=pr, mode=ed.ED_LZ_FULL,
+              selective=selective)
+    eager, _ = op.lanczos(**kw)
+    runs = [op.lanczos(graph=True, **kw) for _ in range(2)]
+    for g, st in runs:
+        assert np.abs(g - ev).max() < _bar(prec) * np.abs(ev).max(), (g, ev)
+        assert np.abs(g - eager).max() < _bar(prec) * np.abs(ev).max(), (g, eager)
+        assert st["hv_calls"] > 30
+        _within_bounds(g, ev, st)
+    assert np.array_equal(runs[0][0], runs[1][0])
+
+
+def test_graph_captured_full_lanczos_exhausts_small_sector(gpu_ready):
+    """A 245-state sector with more iterations than states: the Krylov space
+    becomes invariant inside a replay batch; the host truncates at the first
+    vanishing beta and every eigenvalue matches dense diagonalisation."""
+    ed = _ed()
+    model = L.chain(14)
+    o = P.OracleBasis(model, 7, 0, 0)
+    ev = np.linalg.eigvalsh(o.make_h())[:2]
+    basis = ed.Basis(ed.Lattice.of(model), 7, 0, 0)
+    assert basis.D < 400
+    op = ed.Operator(basis, model)
+    g, st = op.lanczos(nev=2, iterations=400, ncycles=1, tol=0.0, seed=3, precision=ed.ED_C128,
+                       mode=ed.ED_LZ_FULL, graph=True)
+    assert np.abs(g - ev).max() < 1e-10 * np.abs(ev).max(), (g, ev)
+    _within_bounds(g, ev, st)
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_thick_restart_lanczos_vs_dense(gpu_ready, prec):
     """ED_LZ_THICK with a Krylov space too small to hold the answer without
