@@ -1806,6 +1806,248 @@ int run_full_graph(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, 
     return ED_OK;
 }
 
+// ---- graph-captured reference-schedule Lanczos --------------------------
+// run_reference with each step's scalars on the device and one step captured
+// as a hipGraph (two graphs: the cycles without and the last cycle with the
+// Gram-Schmidt pass over the cycle's vectors), replayed for every step.
+// Device scalars: ab[2k] = Re alpha_k, ab[2k+1] = beta_k (|w| before the
+// division); hs[0] = alpha_k (complex, as the reference subtracts it),
+// hs[1] = |w|^2, hs[2] = |vp|^2 after the orthogonalisations, hs[4 + e] the
+// deflation coefficient of earlier eigenvector e.  The first and the last
+// cycle run all their steps without a host read; the other cycles are read
+// every 5 steps, and the host applies the reference's test |E0 - E0_prev| <
+// tol step by step to find where the cycle ends (the steps replayed past it
+// are not used: the cycle's Ritz vector is built from the vectors up to that
+// step and becomes the next start vector).
+
+// w -= beta_{k-1} V[k-1]   (k > 0)
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_rprev(int64_t D, const VT* __restrict__ V, int64_t ldv,
+                                                        VT* __restrict__ w, const double* __restrict__ ab,
+                                                        const int* __restrict__ step) {
+    const int k = *step;
+    if (k == 0) return;
+    const double be = ab[2 * k - 1];
+    const VT* vm = V + (int64_t)(k - 1) * ldv;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 x = ld(w, i), y = ld(vm, i);
+    st(w, i, x.re - be * y.re, x.im - be * y.im);
+}
+
+// w -= alpha vp (alpha = hs[0], complex), V[k] = vp
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_ralpha(int64_t D, const VT* __restrict__ vp, VT* __restrict__ w,
+                                                         VT* __restrict__ V, int64_t ldv, const cd2* __restrict__ hs,
+                                                         const int* __restrict__ step) {
+    const int k = *step;
+    const cd2 a = hs[0];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 x = ld(vp, i), y = ld(w, i);
+    st(w, i, y.re - (a.re * x.re - a.im * x.im), y.im - (a.re * x.im + a.im * x.re));
+    st(V + (int64_t)k * ldv, i, x.re, x.im);
+}
+
+// ab[2k] = Re alpha, ab[2k+1] = beta = |w|, vp = w / beta (0 if beta = 0)
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_rbeta(int64_t D, const VT* __restrict__ w, VT* __restrict__ vp,
+                                                        const cd2* __restrict__ hs, double* __restrict__ ab,
+                                                        const int* __restrict__ step) {
+    const double be = sqrt(fmax(hs[1].re, 0.0));
+    const double f = be > 1e-300 ? 1.0 / be : 0.0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        const int k = *step;
+        ab[2 * k] = hs[0].re;
+        ab[2 * k + 1] = be;
+    }
+    if (i >= D) return;
+    const cd2 x = ld(w, i);
+    st(vp, i, f * x.re, f * x.im);
+}
+
+// vp /= |vp| (|vp|^2 = hs[2])
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_rnorm(int64_t D, VT* __restrict__ vp, const cd2* __restrict__ hs) {
+    const double nn = hs[2].re;
+    const double f = nn > 1e-300 ? 1.0 / sqrt(nn) : 0.0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const cd2 x = ld(vp, i);
+    st(vp, i, f * x.re, f * x.im);
+}
+
+template <typename VT>
+int run_reference_graph(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D;
+    const int how_many = (int)std::min<int64_t>(p.nev, D);
+    int iterations = p.iterations;
+    const unsigned g = grid_for(D), gs = grid_stream(D);
+    const int maxit = (int)std::min<int64_t>(iterations, D);
+    const int nv = ((std::max(maxit, 1) + kGChunk - 1) / kGChunk) * kGChunk;
+    const int nchunk = nv / kGChunk;
+    const int64_t ldv = (D + 63) / 64 * 64;   // 16-byte aligned vectors for the paired loads
+    const int dt = std::is_same<VT, cd2>::value ? ED_C128 : ED_C64;
+    VT *V = nullptr, *vp = nullptr, *w = nullptr;
+    cd2 *x = nullptr, *part = nullptr, *hc = nullptr, *hs = nullptr;
+    double* ab = nullptr;
+    int* step = nullptr;
+    hipGraphExec_t gx[2] = {nullptr, nullptr};   // [0] no Gram-Schmidt pass, [1] the last cycle's
+    auto cleanup = [&]() {
+        for (auto& e : gx)
+            if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+        dfree(V); dfree(vp); dfree(w); dfree(x); dfree(part); dfree(hc); dfree(hs); dfree(ab); dfree(step);
+    };
+#define GRC(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != ED_OK) {            \
+            cleanup();                 \
+            return rc_;                \
+        }                              \
+    } while (0)
+#define GHIP(expr) GRC((expr) == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "%s failed", #expr))
+    GHIP(hipMalloc(&V, (size_t)std::max(maxit, 1) * ldv * sizeof(VT)));
+    GHIP(hipMalloc(&vp, D * sizeof(VT)));
+    GHIP(hipMalloc(&w, D * sizeof(VT)));
+    GHIP(hipMalloc(&x, D * sizeof(cd2)));
+    GHIP(hipMalloc(&part, (size_t)nv * g * sizeof(cd2)));
+    GHIP(hipMalloc(&hc, (size_t)nv * sizeof(cd2)));
+    GHIP(hipMalloc(&hs, (4 + (size_t)how_many) * sizeof(cd2)));
+    GHIP(hipMalloc(&ab, 2 * (size_t)std::max(maxit, 1) * sizeof(double)));
+    GHIP(hipMalloc(&step, sizeof(int)));
+    std::vector<std::complex<double>> q;
+    std::vector<double> hab;
+    auto dot1 = [&](const auto* X, const auto* y, cd2* out) {
+        using XT = std::remove_cv_t<std::remove_pointer_t<decltype(X)>>;
+        using YT = std::remove_cv_t<std::remove_pointer_t<decltype(y)>>;
+        hipLaunchKernelGGL((k_gdot<XT, YT>), dim3(g, 1), dim3(kRedThreads), 0, c.st, D, X, ldv, kFixed, y, part, step,
+                           0, 1, (const cd2*)nullptr);
+        hipLaunchKernelGGL(k_gcolsum, dim3(1), dim3(64), 0, c.st, part, (int)g, out, step, 0, 1);
+    };
+    // T of the first n steps: diagonal ab[2j], off-diagonal ab[2j+1], j < n - 1
+    auto tri = [&](int n, std::vector<double>& al, std::vector<double>& off) {
+        al.resize(n);
+        off.resize(std::max(0, n - 1));
+        for (int j = 0; j < n; j++) al[j] = hab[2 * j];
+        for (int j = 0; j + 1 < n; j++) off[j] = hab[2 * j + 1];
+    };
+    for (int num = 0; num < how_many; num++) {
+        iterations = (int)std::min<int64_t>(iterations, D - num);
+        const int np = (int)prev.size();
+        hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed + (uint64_t)num, vp);
+        GHIP(hipGetLastError());
+        // one eager H*v: allocates the gather stream outside the captures and
+        // times the product
+        const double ms_before = c.hv_ms;
+        GRC(hv<VT>(c, vp, w));
+        const double hv_once = c.hv_ms - ms_before;
+        for (int cg = 0; cg < 2; cg++) {
+            if (cg == 0 && p.ncycles < 2) continue;
+            hipGraph_t graph;
+            GHIP(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+            const int rc = op_hv(c.op, dt, vp, w, c.st);
+            hipLaunchKernelGGL((k_rprev<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, V, ldv, w, ab, step);
+            dot1(vp, w, hs + 0);
+            hipLaunchKernelGGL((k_ralpha<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, vp, w, V, ldv, hs, step);
+            dot1(w, w, hs + 1);
+            hipLaunchKernelGGL((k_rbeta<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, w, vp, hs, ab, step);
+            for (int e = 0; e < np; e++) {
+                dot1(prev[e], vp, hs + 4 + e);
+                hipLaunchKernelGGL((k_gaxpy<cd2, VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, prev[e], ldv, kFixed,
+                                   hs + 4 + e, vp, step, 0, 1, (const cd2*)nullptr);
+            }
+            if (cg == 1) {
+                // classical Gram-Schmidt against this cycle's vectors V[0..k]
+                hipLaunchKernelGGL((k_gdot<VT, VT>), dim3(g, nchunk), dim3(kRedThreads), 0, c.st, D, V, ldv, kFixed, vp,
+                                   part, step, 1, 0, (const cd2*)nullptr);
+                hipLaunchKernelGGL(k_gcolsum, dim3(nv), dim3(64), 0, c.st, part, (int)g, hc, step, 1, 0);
+                hipLaunchKernelGGL((k_gaxpy<VT, VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, V, ldv, kFixed, hc, vp,
+                                   step, 1, 0, (const cd2*)nullptr);
+            }
+            dot1(vp, vp, hs + 2);
+            hipLaunchKernelGGL((k_rnorm<VT>), dim3(gs), dim3(kRedThreads), 0, c.st, D, vp, hs);
+            hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, c.st, step);
+            const hipError_t le = hipGetLastError();
+            GHIP(hipStreamEndCapture(c.st, &graph));
+            if (rc != ED_OK) { (void)hipGraphDestroy(graph); GRC(rc); }
+            if (le != hipSuccess) { (void)hipGraphDestroy(graph); GRC(ed_fail(ED_EDEVICE, "launch in the captured step failed")); }
+            const hipError_t ie = hipGraphInstantiate(&gx[cg], graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            GHIP(ie);
+        }
+        double previous = 1000.0, e0 = 0.0, nn = 0.0;
+        std::vector<double> tvec, al, off;
+        for (int cycle = 0; cycle < p.ncycles; cycle++) {
+            // orth_wrt_previous_evecs + normalize (lanczos.cpp:199-200)
+            for (int k = 0; k < np; k++) {
+                GRC((multidot<cd2, VT>(c, 1, prev[k], 0, vp, q)));
+                GRC((multiaxpy<cd2, VT>(c, 1, prev[k], 0, q, 1.0, vp)));
+            }
+            GRC(dnorm2(c, vp, nn));
+            hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), vp, vp);
+            GHIP(hipGetLastError());
+            const int zero = 0;
+            GHIP(hipMemcpyAsync(step, &zero, sizeof(int), hipMemcpyHostToDevice, c.st));
+            const bool last = cycle == p.ncycles - 1;
+            hipGraphExec_t ex = gx[last ? 1 : 0];
+            int done = 0, nvs = 0;
+            bool converged = false;
+            while (nvs == 0) {
+                const int from = done;
+                const int upto = (last || cycle == 0) ? iterations : std::min(iterations, done + 5);
+                for (; done < upto; done++) GHIP(hipGraphLaunch(ex, c.st));
+                hab.resize(2 * (size_t)done);
+                GHIP(hipMemcpyAsync(hab.data(), ab, hab.size() * sizeof(double), hipMemcpyDeviceToHost, c.st));
+                GHIP(hipStreamSynchronize(c.st));
+                if (last || cycle == 0) {
+                    nvs = done;
+                    break;
+                }
+                // the reference's step-by-step test in a middle cycle
+                for (int it = from; it < done && nvs == 0; it++) {
+                    tri(it + 1, al, off);
+                    if (!lowest(al, off, e0, nullptr)) GRC(ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge"));
+                    if (it > 1 && std::fabs(e0 - previous) < p.tol) { nvs = it + 1; converged = true; }
+                    previous = e0;
+                }
+                if (nvs == 0 && done == iterations) nvs = done;
+            }
+            c.hv_calls += done;
+            c.hv_ms += hv_once * done;   // the replayed products, at the eager product's time
+            if (std::getenv("ED_PROGRESS"))
+                std::fprintf(stderr, "[lanczos-graph-reference] cycle %d steps %d (used %d)\n", cycle, done, nvs);
+            // Ritz vector (lanczos.cpp:312-320), assembled in FP64
+            tri(nvs, al, off);
+            if (!lowest(al, off, e0, &tvec)) GRC(ed_fail(ED_ENOTCONV, "tridiagonal QL did not converge"));
+            GHIP(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+            GRC((accumulate<VT>(c, nvs, V, ldv, tvec, x)));
+            for (int k = 0; k < np; k++) {
+                GRC((multidot<cd2, cd2>(c, 1, prev[k], 0, x, q)));
+                GRC((multiaxpy<cd2, cd2>(c, 1, prev[k], 0, q, 1.0, x)));
+            }
+            GRC(dnorm2(c, x, nn));
+            hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
+            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, x, vp);
+            GHIP(hipGetLastError());
+            if (converged) cycle = p.ncycles - 2;   // next: the last cycle
+        }
+        for (auto& e : gx)
+            if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+        cd2* keep = nullptr;
+        GHIP(hipMalloc(&keep, D * sizeof(cd2)));
+        prev.push_back(keep);
+        GHIP(hipMemcpyAsync(keep, x, D * sizeof(cd2), hipMemcpyDeviceToDevice, c.st));
+        evals[num] = e0;
+    }
+    GHIP(hipStreamSynchronize(c.st));
+#undef GHIP
+#undef GRC
+    cleanup();
+    return ED_OK;
+}
+
 // ---- multi-sector block Lanczos ---------------------------------------
 // run_block_full for every (sector, instance) of a multi-sector block in lock
 // step.  Krylov vector k of all sectors is one contiguous n_all-element
@@ -2384,7 +2626,10 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
                 prev.push_back(x);
             }
         }
-    } else if (p->mode == ED_LZ_REFERENCE)
+    } else if (p->mode == ED_LZ_REFERENCE && (p->flags & ED_LZ_GRAPH))
+        rc = p->precision == ED_C128 ? run_reference_graph<cd2>(c, *p, prev, evals)
+                                     : run_reference_graph<cf2>(c, *p, prev, evals);
+    else if (p->mode == ED_LZ_REFERENCE)
         rc = p->precision == ED_C128 ? run_reference<cd2>(c, *p, prev, evals) : run_reference<cf2>(c, *p, prev, evals);
     else if (p->mode == ED_LZ_THICK)
         rc = p->precision == ED_C128 ? run_trl<cd2>(c, *p, prev, evals) : run_trl<cf2>(c, *p, prev, evals);

@@ -248,12 +248,23 @@ int krylov_precision(std::ofstream& out) {
     fatal(out, std::string("ED_PRECISION must be c64 or c128, got ") + e);
 }
 
+// Lanczos steps as replayed hipGraphs (ED_LZ_GRAPH, the reference schedule's
+// steps with their scalars on the device) unless the environment variable
+// ED_GRAPH is "0"; either way the same steps, stopping points and eigenpairs.
+int step_flags(std::ofstream& out) {
+    const char* e = std::getenv("ED_GRAPH");
+    if (!e || !*e || !std::strcmp(e, "1")) return ED_LZ_GRAPH;
+    if (!std::strcmp(e, "0")) return 0;
+    fatal(out, std::string("ED_GRAPH must be 0 or 1, got ") + e);
+}
+
 void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, double tol, int mode,
                  std::vector<cplx>& eigs, std::vector<std::vector<cplx>>& evecs, std::ofstream& out) {
     int nev = (int)std::min<int64_t>(n_ev, D);
     ed_lanczos_params p;
     std::memset(&p, 0, sizeof(p));
     p.nev = nev; p.iterations = iterations; p.ncycles = ncycles; p.precision = krylov_precision(out); p.mode = mode;
+    p.flags = mode == ED_LZ_REFERENCE ? step_flags(out) : 0;
     p.tol = tol; p.seed = 1;
     std::vector<double> ev(nev);
     std::vector<cplx> vecs((size_t)nev * D);
@@ -273,6 +284,8 @@ void run_lanczos(ed_op op, int64_t D, int n_ev, int iterations, int ncycles, dou
     // FP64 residual ||H x - E x|| of each returned Ritz vector: the error
     // bound of its eigenvalue (|E - E_exact| <= residual for Hermitian H)
     out << " Krylov storage\t\t\t" << (p.precision == ED_C64 ? "complex64" : "complex128") << std::endl;
+    if (mode == ED_LZ_REFERENCE)
+        out << " Lanczos steps\t\t\t" << ((p.flags & ED_LZ_GRAPH) ? "hipGraph replays" : "eager launches") << std::endl;
     for (int k = 0; k < std::min(nev, 16); k++)
         out << " FP64 residual " << k << "\t\t" << std::scientific << std::setprecision(3) << st.residual[k]
             << std::defaultfloat << std::endl;

@@ -186,6 +186,33 @@ def test_graph_captured_full_lanczos(gpu_ready, prec, selective):
     assert np.array_equal(runs[0][0], runs[1][0])
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("ncycles", [1, 6])
+def test_graph_captured_reference_schedule(gpu_ready, prec, ncycles):
+    """The reference schedule (restart cycles, deflation every step, the
+    Gram-Schmidt pass in the last cycle, the jump to the last cycle once
+    |E0 - E0_prev| < tol) with one step captured per kind of cycle and
+    replayed: three eigenvalues of a complex sector against dense and the
+    eager solver; two runs identical to the last bit."""
+    ed = _ed()
+    print(f"[case] graph reference lanczos {prec} ncycles={ncycles}", flush=True)
+    model = L.triangular(4, 4, Ch=0.2, J2=0.3)
+    o = P.OracleBasis(model, 8, 1, 2)
+    ev = np.linalg.eigvalsh(o.make_h())[:3]
+    basis = ed.Basis(ed.Lattice.of(model), 8, 1, 2)
+    op = ed.Operator(basis, model)
+    pr = ed.ED_C128 if prec == "c128" else ed.ED_C64
+    kw = dict(nev=3, iterations=140 if ncycles == 1 else 60, ncycles=ncycles, tol=1e-12, seed=4, precision=pr,
+              mode=ed.ED_LZ_REFERENCE)
+    eager, _ = op.lanczos(**kw)
+    runs = [op.lanczos(graph=True, **kw) for _ in range(2)]
+    for g, st in runs:
+        assert np.abs(g - ev).max() < _bar(prec) * np.abs(ev).max(), (g, ev)
+        assert np.abs(g - eager).max() < _bar(prec) * np.abs(ev).max(), (g, eager)
+        _within_bounds(g, ev, st)
+    assert np.array_equal(runs[0][0], runs[1][0])
+
+
 def test_graph_captured_full_lanczos_exhausts_small_sector(gpu_ready):
     """A 245-state sector with more iterations than states: the Krylov space
     becomes invariant inside a replay batch; the host truncates at the first

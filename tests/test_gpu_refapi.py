@@ -300,3 +300,23 @@ def test_reference_main_complex64_krylov(gpu_ready, tmp_path):
         res = [float(l.split()[-1]) for l in text.splitlines() if l.startswith(" FP64 residual")]
         assert len(res) == 2 and max(res) < 1e-4
     assert np.abs(np.array(out["c64"]) - np.array(out["c128"])).max() < 1e-10 * abs(out["c128"][0])
+
+
+@needs_ed2
+def test_reference_main_graph_and_eager_steps(gpu_ready, tmp_path):
+    """The drop-in runs the reference schedule's steps as replayed hipGraphs
+    by default; ED_GRAPH=0 launches them one by one.  The reference's own
+    main on both: the same energies (within 1e-12 relative), each log saying
+    which ran (22 sites: above the dense-path size)."""
+    model = L.chain(22)
+    out = {}
+    for mode in ("1", "0"):
+        f, log = str(tmp_path / f"in_{mode}.txt"), str(tmp_path / f"log_{mode}.txt")
+        I.write_input(f, model, 0.0, 0, 0, num_eigenvalues=3, lanczos_iterations=60, lanczos_ncycles=5, tol=1e-12)
+        r = subprocess.run([ED2, f, log], capture_output=True, text=True, timeout=180,
+                           env=dict(os.environ, ED_GRAPH=mode))
+        assert r.returncode == 0, open(log).read()[-2000:]
+        out[mode] = I.parse_energies(log)
+        assert ("hipGraph replays" if mode == "1" else "eager launches") in open(log).read()
+    assert len(out["1"]) == 3
+    assert np.abs(np.array(out["1"]) - np.array(out["0"])).max() < 1e-12 * abs(out["0"][0])
