@@ -109,18 +109,26 @@ __global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __r
     const uint64_t s = rw & kStateMask;
     const double nrm = ntab[(int)(rw >> kLidShift)];
     double ar = 0.0, ai = 0.0;
-    // WIMG: u is the wide gather image, addressed by rank through the chunk
-    // map (cache resident); else u by row through the directory
-    auto jof = [&](uint64_t r) -> int64_t {
-        if (WIMG) return (int64_t)(((uint64_t)wmap[r >> cb] << cb) | (r & ((uint64_t(1) << cb) - 1)));
-        return dir_decode_off(dir[r / kDirSpan], (uint32_t)(r % kDirSpan));
+    auto loadu = [&](int64_t j) { return load_c(u, j); };
+    auto acc = [&](const cd2& x, double cr, double ci) {
+        ar += cr * x.re - ci * x.im;
+        ai += cr * x.im + ci * x.re;
     };
-    const double ez = rows2_row<TC, TRIV, kRows2K>(
-        s, nfam, fam, foff, ent, ezz0, R, rlo, rhi, phase, i, jof, [&](int64_t j) { return load_c(u, j); },
-        [&](const cd2& x, double cr, double ci) {
-            ar += cr * x.re - ci * x.im;
-            ai += cr * x.im + ci * x.re;
-        });
+    double ez;
+    if constexpr (WIMG) {
+        // u is the wide gather image, addressed by rank through the chunk map
+        // (cache resident)
+        auto jget = [&](uint64_t r) -> uint32_t { return wmap[r >> cb]; };
+        auto jres = [&](uint64_t r, uint32_t m) -> int64_t {
+            return (int64_t)(((uint64_t)m << cb) | (r & ((uint64_t(1) << cb) - 1)));
+        };
+        ez = rows2_row<TC, TRIV, kRows2K>(s, nfam, fam, foff, ent, ezz0, R, rlo, rhi, phase, i, jget, jres, loadu, acc);
+    } else {
+        // u by row through the directory
+        auto jget = [&](uint64_t r) -> DirEntry { return dir[r / kDirSpan]; };
+        auto jres = [&](uint64_t r, const DirEntry& e) -> int64_t { return dir_decode_off(e, (uint32_t)(r % kDirSpan)); };
+        ez = rows2_row<TC, TRIV, kRows2K>(s, nfam, fam, foff, ent, ezz0, R, rlo, rhi, phase, i, jget, jres, loadu, acc);
+    }
     const cd2 vi = load_c(v, i);
     VT o;
     o.re = nrm * ar + ez * vi.re;

@@ -94,6 +94,65 @@ ED_HD uint64_t rep64(uint64_t t, int& code) {
     return ((uint64_t)hi << TC::W) | lo;
 }
 
+// The same orbit minimum (and, with TRACK, the same code) from a few
+// candidate translations.  The minimum's high row is the smallest of the 2W
+// row rotations, so it starts with the longest cyclic run of zeros found in
+// either row: a translation whose high row starts with fewer leading zeros
+// cannot win.  Candidate translations are those whose high row starts with
+// at least L = min(kPruneMax, that longest run) zeros -- the start positions
+// p of zero runs of length >= L in f1 (no row exchange) or f0 (exchange),
+// rotation a = W - 1 - p -- plus the identity; the minimum is taken over
+// them as 64-bit keys (high row << 32 | low row, the same order as
+// high * 2^W + low).  Random 40-site states keep 1.8 candidates on average
+// instead of 40.  TRACK: among equal keys the smallest a, and at equal a no
+// exchange first -- the translation rep64 picks.
+constexpr int kPruneMax = 5;
+
+template <class TC, bool TRACK>
+ED_HD uint64_t rep64p(uint64_t t, int& code) {
+    constexpr int W = TC::W;
+    const uint32_t f0 = (uint32_t)t & TC::wmask, f1 = (uint32_t)(t >> W) & TC::wmask;
+    const uint64_t X0 = (uint64_t)f0 | ((uint64_t)f0 << W), X1 = (uint64_t)f1 | ((uint64_t)f1 << W);
+    // R_k[p] = 1 when bits p, p-1, ..., p-k+1 (cyclic) of the row are zero
+    const uint32_t z0 = f0 ^ TC::wmask, z1 = f1 ^ TC::wmask;
+    const uint64_t Z0 = (uint64_t)z0 | ((uint64_t)z0 << W), Z1 = (uint64_t)z1 | ((uint64_t)z1 << W);
+    uint32_t r0 = z0, r1 = z1, m0 = z0, m1 = z1;
+#pragma unroll
+    for (int k = 1; k < kPruneMax; k++) {
+        // bit p of (Z >> (W - k)) is z[p - k]; bits at W and above are cut by r
+        r0 &= (uint32_t)(Z0 >> (W - k));
+        r1 &= (uint32_t)(Z1 >> (W - k));
+        if ((r0 | r1) != 0u) { m0 = r0; m1 = r1; }
+    }
+    // the identity translation first: key (f1, f0), a = 0, no exchange
+    uint64_t best = ((uint64_t)f1 << 32) | f0;
+    int bcode = 0;   // a * 2 + exchange
+    auto consider = [&](uint32_t mask, bool swap) {
+        while (mask) {
+            const int p = ctz32(mask);
+            mask &= mask - 1u;
+            const int a = W - 1 - p;
+            const uint32_t A = (uint32_t)(X0 >> (W - a)) & TC::wmask, B = (uint32_t)(X1 >> (W - a)) & TC::wmask;
+            const uint64_t key = swap ? (((uint64_t)A << 32) | B) : (((uint64_t)B << 32) | A);
+            if (TRACK) {
+                const int sc = a * 2 + (swap ? 1 : 0);
+                if (key < best || (key == best && sc < bcode)) { best = key; bcode = sc; }
+            } else if (key < best) {
+                best = key;
+            }
+        }
+    };
+    consider(m1, false);
+    consider(m0, true);
+    if (TRACK) {
+        const int ba = bcode >> 1, bb = bcode & 1;
+        code = ((W - ba) % W) * 2 + (2 - bb) % 2;
+    } else {
+        code = 0;
+    }
+    return ((best >> 32) << W) | (best & 0xffffffffull);
+}
+
 // Rank: lower bytes from 32-bit blocks (layout of rank_tables without its
 // last block), top byte from its 64-bit block.
 template <class TC>
@@ -109,14 +168,25 @@ ED_HD uint64_t rank64(uint64_t s, const RankInfo& R, const uint32_t* lo, const u
     return (uint64_t)r + hi[(unsigned)(s >> (8 * (TC::nbytes - 1))) & 255u];
 }
 
+// Orders the scheduler not to move instructions across this point (device
+// code); a no-op in the host build of the same header.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ED_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
+#else
+#define ED_SCHED_BARRIER() ((void)0)
+#endif
+
 // One row with K elements in flight (same scheme as torus_row_batched).
-// jof(r) gives the element of u holding the target of rank r, or -1 for a
-// rank outside the sector; such elements, and the empty slots, load the
-// row's own element (self) and add it with a zero coefficient.
-template <class TC, bool TRIV, int K, class JOf, class LoadU, class Gf>
+// The element of u holding the target of rank r is jres(r, jget(r)), or -1
+// for a rank outside the sector; such elements, and the empty slots, load
+// the row's own element (self) and add it with a zero coefficient.  jget
+// does the lookup's memory access (a chunk-map or directory entry); the K
+// lookups of a batch are all issued before any is used, so they are in
+// flight together instead of one after the other.
+template <class TC, bool TRIV, int K, class JGet, class JRes, class LoadU, class Gf>
 ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff, const TEntry* ent, double ezz0,
                        const RankInfo& R, const uint32_t* rlo, const uint64_t* rhi, const cd2* phase, int64_t self,
-                       JOf&& jof, LoadU&& loadu, Gf&& g) {
+                       JGet&& jget, JRes&& jres, LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
     int f = 0;
     uint64_t cur = nfam > 0 ? fam_mask64<TC>(s, fam[0]) : 0ull;
@@ -148,7 +218,7 @@ ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff,
                     ci = (((s >> E.y) & 1ull) ? E.cre : -E.cre) * sg;
                 }
                 int code;
-                const uint64_t rep = rep64<TC, !TRIV>(t, code);
+                const uint64_t rep = rep64p<TC, !TRIV>(t, code);
                 rk[q] = rank64<TC>(rep, R, rlo, rhi);
                 if (!TRIV) {
                     const cd2 ph = phase[code];
@@ -161,11 +231,16 @@ ED_HD double rows2_row(uint64_t s, int nfam, const TFam64* fam, const int* foff,
             }
         }
         more = ok[K - 1];
+        using Tok = decltype(jget(uint64_t(0)));
+        Tok tk[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) tk[q] = jget(rk[q]);
+        ED_SCHED_BARRIER();
         bool use[K];
         int64_t j[K];
 #pragma unroll
         for (int q = 0; q < K; q++) {
-            j[q] = jof(rk[q]);
+            j[q] = jres(rk[q], tk[q]);
             use[q] = ok[q] && j[q] >= 0;
         }
         cd2 xv[K];

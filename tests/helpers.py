@@ -278,6 +278,8 @@ def hostcore():
         H.hc_full_tables.argtypes = [vp] * 3
         H.hc_set_terms.argtypes = [vp, vp]
         H.hc_hv.argtypes = [vp] * 3
+        H.hc_rep64_check.restype = C.c_int
+        H.hc_rep64_check.argtypes = [C.c_int64, C.c_uint64]
         H.hc_ranker_new.restype = vp
         H.hc_ranker_new.argtypes = [C.c_int, C.c_int]
         H.hc_ranker_free.argtypes = [vp]

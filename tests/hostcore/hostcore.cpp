@@ -181,8 +181,9 @@ void hv_rows2(HC* h, const std::vector<TFam64>& fam, const std::vector<int>& fof
             if (wide) return (int64_t)(((uint64_t)L.map.at(r >> L.cb) << L.cb) | (r & ((uint64_t(1) << L.cb) - 1)));
             return dir_decode_off(h->dir[r / kDirSpan], (uint32_t)(r % kDirSpan));
         };
+        auto jres = [](uint64_t, int64_t j) { return j; };
         double ez = rows2_row<TC, TRIV, 4>(s, (int)fam.size(), fam.data(), foff.data(), ent.data(), ezz0, h->R,
-                                            rlo.data(), rhi, h->ph.data(), i, jof,
+                                            rlo.data(), rhi, h->ph.data(), i, jof, jres,
                                             [&](int64_t j) { return u.at(j); },
                                             [&](const cd2& x, double cr, double ci) {
                                                 ar += cr * x.re - ci * x.im;
@@ -565,6 +566,52 @@ void hc_generic_rep(void* p, uint64_t t, uint64_t* rep, int* code) {
 
 // Eigenpairs of a dense symmetric matrix (row-major a[m*m]) with the
 // library's Jacobi solver: d <- ascending eigenvalues, z <- column-major vectors.
+// rep64p (pruned candidates) against rep64 (all 2W translations): the
+// representative and, with TRACK, the translation code, on every state of
+// the 20-site two-row cluster and on n random and structured 40-site
+// states; returns the number of disagreements.
+int hc_rep64_check(int64_t n, uint64_t seed) {
+    int bad = 0;
+    auto one = [&](auto tc, uint64_t t) {
+        using TC = decltype(tc);
+        int c0 = 0, c1 = 0;
+        const uint64_t a = rep64<TC, true>(t, c0), b = rep64p<TC, true>(t, c1);
+        int d0 = 0, d1 = 0;
+        const uint64_t e = rep64<TC, false>(t, d0), f = rep64p<TC, false>(t, d1);
+        if (a != b || c0 != c1 || e != f || a != e) bad++;
+    };
+    for (uint64_t t = 0; t < (1ull << 20); t++) one(Rows2C<20, 10>{}, t);
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto next = [&]() {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        return x;
+    };
+    const uint64_t m40 = (1ull << 40) - 1;
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t t = next() & m40;
+        switch (i % 4) {
+            case 1: t &= next() & next(); break;                                       // sparse: long zero runs
+            case 2: t |= next() | next(); t &= m40; break;                              // dense: short zero runs
+            case 3: {                                                                   // periodic rows
+                const int per = (int)(next() % 5) + 1;
+                const int P[5] = {1, 2, 4, 5, 10};
+                const int q = P[per - 1];
+                uint64_t r0 = next() & ((1ull << q) - 1), r1 = next() & ((1ull << q) - 1), f0 = 0, f1 = 0;
+                for (int j = 0; j < 20; j += q) { f0 |= r0 << j; f1 |= r1 << j; }
+                t = (f0 & 0xfffffull) | ((next() & 1 ? f1 : f0) & 0xfffffull) << 20;
+                break;
+            }
+            default: break;
+        }
+        one(Rows2C<40, 20>{}, t);
+    }
+    one(Rows2C<40, 20>{}, 0);
+    one(Rows2C<40, 20>{}, m40);
+    one(Rows2C<40, 20>{}, 0xfffffull);
+    one(Rows2C<40, 20>{}, 0xfffffull << 20);
+    return bad;
+}
+
 int hc_symeig(int m, const double* a, double* d, double* z) {
     std::vector<double> A(a, a + (size_t)m * m), dd, zz;
     bool ok = symeig_jacobi(A, m, dd, zz);

@@ -246,6 +246,15 @@ def test_two_row_64bit_row_vs_oracle(model, n, k):
     assert np.array_equal(w, h.hv_rows2(v, wide=False))
 
 
+def test_pruned_orbit_minimum_matches_full_scan():
+    """The two-row orbit minimum from the zero-run candidates gives the
+    representative and translation code of the scan over all 2W
+    translations: every state of the 20-site cluster, 400k random, sparse,
+    dense and row-periodic 40-site states, and the all-zero/all-one rows."""
+    H = Hp.hostcore()
+    assert H.hc_rep64_check(400_000, 7) == 0
+
+
 def test_split_rank_tables_at_40_sites():
     """N=40, n=20: the split 32/64-bit rank tables give the combinatorial
     number system rank (closed form) on random and extreme states."""
