@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -78,8 +79,8 @@ namespace {
 
 constexpr int kRows2K = 8;   // elements in flight per lane
 
-template <class TC, bool TRIV, bool WIMG, typename VT>
-__global__ void __launch_bounds__(256) k_hv_rows2(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+template <class TC, bool TRIV, bool WIMG, typename VT, int BS>
+__global__ void __launch_bounds__(BS) k_hv_rows2(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                   const TFam64* __restrict__ gfam, const int* __restrict__ gfoff,
                                                   int nterm, const TEntry* __restrict__ gent, double ezz0, RankInfo R,
                                                   const uint32_t* __restrict__ grlo, int rlo_n,
@@ -143,15 +144,22 @@ size_t rows2_smem(const ed_op_s* op) {
            256 * sizeof(uint64_t) + (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)B->rlo_n * sizeof(uint32_t);
 }
 
-template <class TC, typename VT>
-void launch_r2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wimg) {
+// Workgroup size: the rank tables in LDS (about 53 KB for N = 40) are one
+// copy per workgroup, so larger workgroups put more waves on a CU.
+int rows2_block() {
+    const char* e = std::getenv("ED_ROWS2_BLOCK");
+    const int v = e ? std::atoi(e) : 0;
+    return (v == 256 || v == 512 || v == 1024) ? v : 512;
+}
+
+template <class TC, typename VT, int BS>
+void launch_r2_bs(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wimg) {
     ed_basis_s* B = op->b;
-    const unsigned bs = 256;
-    dim3 grid((unsigned)((B->D + bs - 1) / bs)), blk(bs);
+    dim3 grid((unsigned)((B->D + BS - 1) / BS)), blk(BS);
     const size_t sm = rows2_smem(op);
     const int nph = B->L1 * B->L2;
 #define ED_R2GO(TRIV, WIMG)                                                                                        \
-    hipLaunchKernelGGL((k_hv_rows2<TC, TRIV, WIMG, VT>), grid, blk, sm, st, B->D, B->d_rows, op->nfam64,            \
+    hipLaunchKernelGGL((k_hv_rows2<TC, TRIV, WIMG, VT, BS>), grid, blk, sm, st, B->D, B->d_rows, op->nfam64,        \
                        (const TFam64*)op->d_fam64, op->d_foff64, op->nterm64, op->d_ent64, op->ezz0_64, B->rinfo,   \
                        B->d_rlo, B->rlo_n, B->d_rhi, B->d_dir, B->d_wimg_map, B->wimg_cb, B->d_phase, nph,          \
                        B->d_ntab, u, v, w)
@@ -163,6 +171,15 @@ void launch_r2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, boo
         else ED_R2GO(false, false);
     }
 #undef ED_R2GO
+}
+
+template <class TC, typename VT>
+void launch_r2(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, bool wimg) {
+    switch (rows2_block()) {
+        case 256: launch_r2_bs<TC, VT, 256>(op, u, v, w, st, wimg); break;
+        case 1024: launch_r2_bs<TC, VT, 1024>(op, u, v, w, st, wimg); break;
+        default: launch_r2_bs<TC, VT, 512>(op, u, v, w, st, wimg); break;
+    }
 }
 
 }  // namespace
