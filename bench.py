@@ -177,7 +177,7 @@ def gather_sweep():
     import ed_code_amd as ed
     ed.init(0)
     n_gather = 1_241_024_576          # the 32-site torus sector's off-diagonal elements
-    for mb in (8, 32, 150, 405, 4096):
+    for mb in (8, 32, 150, 405, 4096, 16384, 66560):   # up to the 65 GiB image of the 40-site sector
         for eb in (8, 16):
             for group in (1, 2, 4, 16):
                 if group * eb > 128:
