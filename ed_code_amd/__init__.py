@@ -319,7 +319,8 @@ class Operator:
         walk, 2 compiled-shape rotation kernel (K=4), 3 run-time-shape
         rotation kernel, 4 compiled-shape K=2 at 8 waves, 5 compiled-shape K=8,
         6 two-row 64-bit kernel, 7 compiled-shape K=4 software-pipelined,
-        8 two-row kernel gathering u by row through the directory; block 0 =
+        8 two-row kernel gathering u by row through the directory, 9 / 10
+        compiled-shape with every row split over two / four lanes; block 0 =
         default."""
         check(lib().ed_op_tune(self.h, path, block))
 

@@ -39,6 +39,16 @@ using namespace edc;
 // ----------------------------------------------------------------- errors --
 static thread_local std::string g_err;
 
+hipStream_t setup_stream() {
+    thread_local hipStream_t st[16] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    // created once per thread and device and kept for the thread's life; a
+    // failed creation leaves the legacy stream (0)
+    if (!st[dev] && hipStreamCreateWithFlags(&st[dev], hipStreamNonBlocking) != hipSuccess) st[dev] = 0;
+    return st[dev];
+}
+
 int ed_fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -469,11 +479,11 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipMalloc(&B->d_nitab, nit.size() * sizeof(double)));
     HIPCHK(hipMalloc(&B->d_binom, bn.size() * sizeof(uint64_t)));
     HIPCHK(hipMalloc(&B->d_rtab, rt.size() * sizeof(uint64_t)));
-    HIPCHK(hipMemcpy(B->d_phase, ph.data(), ph.size() * sizeof(cd2), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B->d_ntab, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B->d_nitab, nit.data(), nit.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B->d_binom, bn.data(), bn.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(B->d_rtab, rt.data(), rt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_phase, ph.data(), ph.size() * sizeof(cd2), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_ntab, nt.data(), nt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_nitab, nit.data(), nit.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_binom, bn.data(), bn.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_rtab, rt.data(), rt.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
     if (B->rows2_w) {
         int rc = rows2_tables(B, rt);
         if (rc != ED_OK) return rc;
@@ -483,7 +493,7 @@ int basis_build_device(ed_basis_s* B) {
         for (size_t q = 0; q < rt.size(); q++) rt32[q] = (uint32_t)rt[q];   // C(N,n) < 2^32
         B->rtab32_n = (int)rt32.size();
         HIPCHK(hipMalloc(&B->d_rtab32, rt32.size() * sizeof(uint32_t)));
-        HIPCHK(hipMemcpy(B->d_rtab32, rt32.data(), rt32.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(scopy(B->d_rtab32, rt32.data(), rt32.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
 
     Geo G;
@@ -506,33 +516,33 @@ int basis_build_device(ed_basis_s* B) {
     HIPCHK(hipMalloc(&base, (ngroups + 1) * sizeof(uint64_t)));
     HIPCHK(hipMalloc(&B->d_dir, ngroups * sizeof(DirEntry)));
     if (N <= 32)
-        hipLaunchKernelGGL(k_classify<uint32_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+        hipLaunchKernelGGL(k_classify<uint32_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, setup_stream(), C, N, n, G, B->d_binom, words, nwords);
     else
-        hipLaunchKernelGGL(k_classify<uint64_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, words, nwords);
+        hipLaunchKernelGGL(k_classify<uint64_t>, dim3(sblocks(nwords, 256)), dim3(256), 0, setup_stream(), C, N, n, G, B->d_binom, words, nwords);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_group_counts, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, words, ngroups, cnt);
+    hipLaunchKernelGGL(k_group_counts, dim3(sblocks(ngroups, 256)), dim3(256), 0, setup_stream(), words, ngroups, cnt);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemset(cnt + ngroups, 0, sizeof(uint64_t)));
+    HIPCHK(szero(cnt + ngroups, sizeof(uint64_t)));
     // the group count exceeds 2^31 at N=40: the scan takes a 64-bit length
     size_t tmp_bytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, cnt, base, (size_t)(ngroups + 1), setup_stream()));
     HIPCHK(hipMalloc(&tmp, tmp_bytes + 16));
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (size_t)(ngroups + 1)));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cnt, base, (size_t)(ngroups + 1), setup_stream()));
     uint64_t D = 0;
-    HIPCHK(hipMemcpy(&D, base + ngroups, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(&D, base + ngroups, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (D > 0xFFFFFFFFull) {
         return ed_fail(ED_EINVAL, "sector dimension %llu exceeds 2^32", (unsigned long long)D);
     }
     B->D = (int64_t)D;
-    hipLaunchKernelGGL(k_dir_fill, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, words, base, ngroups, B->d_dir);
+    hipLaunchKernelGGL(k_dir_fill, dim3(sblocks(ngroups, 256)), dim3(256), 0, setup_stream(), words, base, ngroups, B->d_dir);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMalloc(&B->d_rows, std::max<uint64_t>(D, 1) * sizeof(uint64_t)));
     if (N <= 32)
-        hipLaunchKernelGGL(k_compact<uint32_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+        hipLaunchKernelGGL(k_compact<uint32_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, setup_stream(), C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
     else
-        hipLaunchKernelGGL(k_compact<uint64_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, 0, C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
+        hipLaunchKernelGGL(k_compact<uint64_t>, dim3(sblocks(ngroups, 256)), dim3(256), 0, setup_stream(), C, N, n, G, B->d_binom, B->d_dir, ngroups, B->d_rows);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(ssync());
     temps.release();   // before the image layout: at N=40 the temporaries hold ~50 GB
     return basis_image_build(B);
 }
@@ -556,11 +566,11 @@ static int image_choose(ed_basis_s* B, uint64_t max_entries, uint64_t max_elems,
     for (int cb = cb0; cb <= cb0 + 3 && L.cb < 0; cb++) {
         const uint64_t nmap = ((C - 1) >> cb) + 1;
         std::vector<uint8_t> t(nmap);
-        HIPCHK(hipMemset(d_t.p, 0, nmap));
-        hipLaunchKernelGGL(k_image_touch, dim3(sblocks(B->ngroups, 256)), dim3(256), 0, 0, B->d_dir, B->ngroups,
+        HIPCHK(szero(d_t.p, nmap));
+        hipLaunchKernelGGL(k_image_touch, dim3(sblocks(B->ngroups, 256)), dim3(256), 0, setup_stream(), B->d_dir, B->ngroups,
                            cb - 6, d_t.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy(t.data(), d_t.p, nmap, hipMemcpyDeviceToHost));
+        HIPCHK(scopy(t.data(), d_t.p, nmap, hipMemcpyDeviceToHost));
         image_layout(C, cb, t, max_elems, L);
     }
     return ED_OK;
@@ -576,7 +586,7 @@ int basis_image_build(ed_basis_s* B) {
         if (rc != ED_OK) return rc;
         if (L.cb >= 0) {
             HIPCHK(hipMalloc(&B->d_wimg_map, L.map.size() * sizeof(uint32_t)));
-            HIPCHK(hipMemcpy(B->d_wimg_map, L.map.data(), L.map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            HIPCHK(scopy(B->d_wimg_map, L.map.data(), L.map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             B->wimg_cb = L.cb;
             B->wimg_nslot = L.nslot;
             B->wimg_nmap = L.map.size();
@@ -588,12 +598,12 @@ int basis_image_build(ed_basis_s* B) {
     if (rc != ED_OK) return rc;
     if (L.cb < 0) return ED_OK;
     HIPCHK(hipMalloc(&B->d_img_map, L.map.size() * sizeof(uint16_t)));
-    HIPCHK(hipMemcpy(B->d_img_map, L.map.data(), L.map.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_img_map, L.map.data(), L.map.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&B->d_img_pos, (size_t)B->D * sizeof(uint32_t)));
-    hipLaunchKernelGGL(k_image_pos, dim3(sblocks((uint64_t)B->D, 256)), dim3(256), 0, 0, (uint64_t)B->D, B->d_rows,
+    hipLaunchKernelGGL(k_image_pos, dim3(sblocks((uint64_t)B->D, 256)), dim3(256), 0, setup_stream(), (uint64_t)B->D, B->d_rows,
                        B->rinfo, B->d_rtab, B->d_img_map, L.cb, B->d_img_pos);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(ssync());
     B->img_cb = L.cb;
     B->img_nslot = (uint32_t)L.nslot;
     B->img_nmap = (uint32_t)L.map.size();
@@ -624,7 +634,7 @@ extern "C" int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* 
     if (reps) {
         // the row words land in the caller's array and are masked in place
         // (no host staging copy: at 40 sites the array is 27.6 GB)
-        HIPCHK(hipMemcpy(reps, b->d_rows, D * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        HIPCHK(scopy(reps, b->d_rows, D * sizeof(uint64_t), hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < D; i++) reps[i] = (int64_t)((uint64_t)reps[i] & kStateMask);
     }
     if (norm || norm_inv) {
@@ -632,11 +642,11 @@ extern "C" int ed_basis_export(ed_basis b, int64_t* reps, double* norm, double* 
         auto body = [&]() -> int {
             HIPCHK(hipMalloc(&dn, D * sizeof(double)));
             HIPCHK(hipMalloc(&dni, D * sizeof(double)));
-            hipLaunchKernelGGL(k_norms, dim3(nblocks(D, 256)), dim3(256), 0, 0, b->d_rows, D, b->d_ntab, b->d_nitab,
+            hipLaunchKernelGGL(k_norms, dim3(nblocks(D, 256)), dim3(256), 0, setup_stream(), b->d_rows, D, b->d_ntab, b->d_nitab,
                                dn, dni);
             HIPCHK(hipGetLastError());
-            if (norm) HIPCHK(hipMemcpy(norm, dn, D * sizeof(double), hipMemcpyDeviceToHost));
-            if (norm_inv) HIPCHK(hipMemcpy(norm_inv, dni, D * sizeof(double), hipMemcpyDeviceToHost));
+            if (norm) HIPCHK(scopy(norm, dn, D * sizeof(double), hipMemcpyDeviceToHost));
+            if (norm_inv) HIPCHK(scopy(norm_inv, dni, D * sizeof(double), hipMemcpyDeviceToHost));
             return ED_OK;
         };
         const int rc = body();
@@ -670,12 +680,12 @@ extern "C" int ed_basis_export_full(ed_basis b, int64_t* rep_loc, double* state_
         TablesView tv{b->d_rtab, b->d_dir, b->d_phase};
         uint64_t nth = (C + 31) / 32;
         if (N <= 32)
-            hipLaunchKernelGGL(k_full_tables<uint32_t>, dim3(nblocks(nth, 256)), dim3(256), 0, 0, C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
+            hipLaunchKernelGGL(k_full_tables<uint32_t>, dim3(nblocks(nth, 256)), dim3(256), 0, setup_stream(), C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
         else
-            hipLaunchKernelGGL(k_full_tables<uint64_t>, dim3(nblocks(nth, 256)), dim3(256), 0, 0, C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
+            hipLaunchKernelGGL(k_full_tables<uint64_t>, dim3(nblocks(nth, 256)), dim3(256), 0, setup_stream(), C, N, n, G, b->d_binom, b->rinfo, tv, d_loc, d_ph);
         HIPCHK(hipGetLastError());
-        if (rep_loc) HIPCHK(hipMemcpy(rep_loc, d_loc, C * sizeof(int64_t), hipMemcpyDeviceToHost));
-        if (state_ph) HIPCHK(hipMemcpy(state_ph, d_ph, C * sizeof(cd2), hipMemcpyDeviceToHost));
+        if (rep_loc) HIPCHK(scopy(rep_loc, d_loc, C * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (state_ph) HIPCHK(scopy(state_ph, d_ph, C * sizeof(cd2), hipMemcpyDeviceToHost));
         dfree(d_loc); dfree(d_ph);
     }
     // Lin tables (Lin_Tables.cpp:57-78) and their Ib completion
@@ -762,15 +772,15 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
     ed_basis_s* b = op->b;
     if (!bonds.empty()) {
         HIPCHK(hipMalloc(&op->d_b, bonds.size() * sizeof(Bond)));
-        HIPCHK(hipMemcpy(op->d_b, bonds.data(), bonds.size() * sizeof(Bond), hipMemcpyHostToDevice));
+        HIPCHK(scopy(op->d_b, bonds.data(), bonds.size() * sizeof(Bond), hipMemcpyHostToDevice));
     }
     if (!ch.empty()) {
         HIPCHK(hipMalloc(&op->d_c, ch.size() * sizeof(Chiral)));
-        HIPCHK(hipMemcpy(op->d_c, ch.data(), ch.size() * sizeof(Chiral), hipMemcpyHostToDevice));
+        HIPCHK(scopy(op->d_c, ch.data(), ch.size() * sizeof(Chiral), hipMemcpyHostToDevice));
     }
     if (!bw.empty()) {
         HIPCHK(hipMalloc(&op->d_w, bw.size() * sizeof(Bowtie)));
-        HIPCHK(hipMemcpy(op->d_w, bw.data(), bw.size() * sizeof(Bowtie), hipMemcpyHostToDevice));
+        HIPCHK(scopy(op->d_w, bw.data(), bw.size() * sizeof(Bowtie), hipMemcpyHostToDevice));
     }
     op->bonds = bonds;
     // rotation fast path: rotational lattice, no bowtie terms, families fit
@@ -787,11 +797,18 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
             for (const TFam& F : fam) bond_only = bond_only && F.type == 0;
             for (const TEntry& E : ent) real = real && E.cim == 0.0;
             op->tmode = !bond_only ? kModeGeneral : (real && b->trivial_k ? kModeReal : kModeBonds);
+            std::vector<uint32_t> fmk;
+            std::vector<cd2> fco;
+            if (torus_fam_extras(fam, ent, fmk, fco) && op->tmode == kModeReal) op->tmode = kModeUniform;
             op->tune_block = 512;
             HIPCHK(hipMalloc(&op->d_tfam, fam.size() * sizeof(TFam)));
-            HIPCHK(hipMemcpy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
+            HIPCHK(scopy(op->d_tfam, fam.data(), fam.size() * sizeof(TFam), hipMemcpyHostToDevice));
             HIPCHK(hipMalloc(&op->d_tent, ent.size() * sizeof(TEntry)));
-            HIPCHK(hipMemcpy(op->d_tent, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+            HIPCHK(scopy(op->d_tent, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&op->d_tfmk, fmk.size() * sizeof(uint32_t)));
+            HIPCHK(scopy(op->d_tfmk, fmk.data(), fmk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&op->d_tfco, fco.size() * sizeof(cd2)));
+            HIPCHK(scopy(op->d_tfco, fco.data(), fco.size() * sizeof(cd2), hipMemcpyHostToDevice));
             // chiral terms on a built-in shape: merge each pair's terms into one element
             std::vector<TFam> mfam;
             std::vector<TEntry> ment;
@@ -802,11 +819,16 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
                 op->tmode = kModeMerged;
                 op->nmfam = (int)mfam.size();
                 HIPCHK(hipMalloc(&op->d_mfam, mfam.size() * sizeof(TFam)));
-                HIPCHK(hipMemcpy(op->d_mfam, mfam.data(), mfam.size() * sizeof(TFam), hipMemcpyHostToDevice));
+                HIPCHK(scopy(op->d_mfam, mfam.data(), mfam.size() * sizeof(TFam), hipMemcpyHostToDevice));
                 HIPCHK(hipMalloc(&op->d_ment, ment.size() * sizeof(TEntry)));
-                HIPCHK(hipMemcpy(op->d_ment, ment.data(), ment.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+                HIPCHK(scopy(op->d_ment, ment.data(), ment.size() * sizeof(TEntry), hipMemcpyHostToDevice));
                 HIPCHK(hipMalloc(&op->d_tchx, chx.size() * sizeof(TChir)));
-                HIPCHK(hipMemcpy(op->d_tchx, chx.data(), chx.size() * sizeof(TChir), hipMemcpyHostToDevice));
+                HIPCHK(scopy(op->d_tchx, chx.data(), chx.size() * sizeof(TChir), hipMemcpyHostToDevice));
+                std::vector<uint32_t> mfmk;
+                std::vector<cd2> mfco;
+                (void)torus_fam_extras(mfam, ment, mfmk, mfco);
+                HIPCHK(hipMalloc(&op->d_mfmk, mfmk.size() * sizeof(uint32_t)));
+                HIPCHK(scopy(op->d_mfmk, mfmk.data(), mfmk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
             }
         }
     }
@@ -818,12 +840,12 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 8) return ed_fail(ED_EINVAL, "path %d not in 0..8", path);
+    if (path < 0 || path > 10) return ed_fail(ED_EINVAL, "path %d not in 0..10", path);
     if ((path == 6 || path == 8) && !op->use_rows2)
         return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
-    if (((path >= 2 && path <= 5) || path == 7) && !op->use_torus)
+    if (((path >= 2 && path <= 5) || path == 7 || path >= 9) && !op->use_torus)
         return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
-    if ((path == 2 || path == 4 || path == 5 || path == 7) && !torus_shape_compiled(op->torus))
+    if ((path == 2 || path == 4 || path == 5 || path == 7 || path >= 9) && !torus_shape_compiled(op->torus))
         return ed_fail(ED_EINVAL, "lattice shape is not one of the compiled shapes");
     if (block != 0 && (block < 64 || block > 512 || block % 64)) return ed_fail(ED_EINVAL, "block %d", block);
     op->tune_path = path;
@@ -861,8 +883,8 @@ extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
                     (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
-                    (void*)op->d_ment, (void*)op->d_tchx, op->d_img64, op->d_img128, op->d_wimg64,
-                    op->d_wimg128})
+                    (void*)op->d_ment, (void*)op->d_tchx, (void*)op->d_tfmk, (void*)op->d_tfco,
+                    (void*)op->d_mfmk, op->d_img64, op->d_img128, op->d_wimg64, op->d_wimg128})
         if (p) dfree(p);
     delete op;
 }
@@ -1068,11 +1090,11 @@ extern "C" int ed_hv_host(ed_op op, const double* v, double* w) {
     auto body = [&]() -> int {
         HIPCHK(hipMalloc(&dv, D * sizeof(cd2)));
         HIPCHK(hipMalloc(&dw, D * sizeof(cd2)));
-        HIPCHK(hipMemcpy(dv, v, D * sizeof(cd2), hipMemcpyHostToDevice));
-        int rc = op_hv(op, ED_C128, dv, dw, 0);
+        HIPCHK(scopy(dv, v, D * sizeof(cd2), hipMemcpyHostToDevice));
+        int rc = op_hv(op, ED_C128, dv, dw, setup_stream());
         if (rc != ED_OK) return rc;
-        HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMemcpy(w, dw, D * sizeof(cd2), hipMemcpyDeviceToHost));
+        HIPCHK(ssync());
+        HIPCHK(scopy(w, dw, D * sizeof(cd2), hipMemcpyDeviceToHost));
         return ED_OK;
     };
     const int rc = body();
@@ -1118,15 +1140,15 @@ extern "C" int ed_csr_hv_host(int64_t n, int64_t ncols, const int64_t* rowptr, c
         (e = hipMalloc(&dval, (nnz > 0 ? nnz : 1) * sizeof(cd2))) != hipSuccess ||
         (e = hipMalloc(&dv, (ncols > 0 ? ncols : 1) * sizeof(cd2))) != hipSuccess ||
         (e = hipMalloc(&dw, n * sizeof(cd2))) != hipSuccess ||
-        (e = hipMemcpy(drp, rowptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
-        (nnz > 0 && (e = hipMemcpy(dcol, col, nnz * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) ||
-        (nnz > 0 && (e = hipMemcpy(dval, val, nnz * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess) ||
-        (ncols > 0 && (e = hipMemcpy(dv, v, ncols * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess)) {
+        (e = scopy(drp, rowptr, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (nnz > 0 && (e = scopy(dcol, col, nnz * sizeof(int64_t), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (nnz > 0 && (e = scopy(dval, val, nnz * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (ncols > 0 && (e = scopy(dv, v, ncols * sizeof(cd2), hipMemcpyHostToDevice)) != hipSuccess)) {
         fail(e);
     } else {
-        hipLaunchKernelGGL(k_csr_hv, dim3(nblocks(n, 256)), dim3(256), 0, 0, n, drp, dcol, dval, dv, dw);
-        if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
-            (e = hipMemcpy(w, dw, n * sizeof(cd2), hipMemcpyDeviceToHost)) != hipSuccess)
+        hipLaunchKernelGGL(k_csr_hv, dim3(nblocks(n, 256)), dim3(256), 0, setup_stream(), n, drp, dcol, dval, dv, dw);
+        if ((e = hipGetLastError()) != hipSuccess || (e = ssync()) != hipSuccess ||
+            (e = scopy(w, dw, n * sizeof(cd2), hipMemcpyDeviceToHost)) != hipSuccess)
             fail(e);
     }
     for (void* p : {(void*)drp, (void*)dcol, (void*)dval, (void*)dv, (void*)dw})
@@ -1142,7 +1164,7 @@ extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) 
     HIPCHK(hipMalloc(&d, 2 * sizeof(unsigned long long)));
     unsigned long long h[2] = {0, 0};
     auto run = [&]() -> int {
-        HIPCHK(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
+        HIPCHK(szero(d, 2 * sizeof(unsigned long long)));
         Geo G;
         basis_geo(B, G);
         TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
@@ -1150,14 +1172,14 @@ extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) 
         if (D > 0) {
             dim3 grid(nblocks(D, 128)), blk(128);
             if (B->N <= 32)
-                hipLaunchKernelGGL((k_count_targets<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T,
+                hipLaunchKernelGGL((k_count_targets<uint32_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T,
                                    d, d + 1);
             else
-                hipLaunchKernelGGL((k_count_targets<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T,
+                hipLaunchKernelGGL((k_count_targets<uint64_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T,
                                    d, d + 1);
             HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+        HIPCHK(scopy(h, d, sizeof(h), hipMemcpyDeviceToHost));
         return ED_OK;
     };
     const int rc = run();
@@ -1176,7 +1198,7 @@ extern "C" int ed_make_h(ed_op op, double* H) {
     if (D == 0) return ED_OK;
     cd2* dH = nullptr;
     HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
-    HIPCHK(hipMemset(dH, 0, D * D * sizeof(cd2)));
+    HIPCHK(szero(dH, D * D * sizeof(cd2)));
     Geo G;
     basis_geo(B, G);
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
@@ -1184,17 +1206,17 @@ extern "C" int ed_make_h(ed_op op, double* H) {
     dim3 grid(nblocks(D, 64)), blk(64);
     if (B->N <= 32) {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     } else {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
     dfree(dH);
     return ED_OK;
 }
@@ -1215,7 +1237,7 @@ extern "C" int ed_dense_eigh(ed_op op, double* evals, double* evecs) {
     HIPCHK(hipMalloc(&dW, D * sizeof(double)));
     HIPCHK(hipMalloc(&dE, D * sizeof(double)));
     HIPCHK(hipMalloc(&dinfo, sizeof(int)));
-    HIPCHK(hipMemset(dH, 0, D * D * sizeof(cd2)));
+    HIPCHK(szero(dH, D * D * sizeof(cd2)));
     Geo G;
     basis_geo(B, G);
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
@@ -1223,28 +1245,33 @@ extern "C" int ed_dense_eigh(ed_op op, double* evals, double* evecs) {
     dim3 grid(nblocks(D, 64)), blk(64);
     if (B->N <= 32) {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     } else {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, 0, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     }
     HIPCHK(hipGetLastError());
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
+    // the solver follows the matrix kernels on the setup stream
+    if (rocblas_set_stream(h, setup_stream()) != rocblas_status_success) {
+        rocblas_destroy_handle(h);
+        return ed_fail(ED_EDEVICE, "rocblas_set_stream failed");
+    }
     // column-major H, upper triangle referenced (the matrix is Hermitian)
     rocblas_status rs = rocsolver_zheevd(h, evecs ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
                                          (rocblas_int)D, (rocblas_double_complex*)dH, (rocblas_int)D, dW, dE, dinfo);
     int info = 0;
-    hipError_t e1 = hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
+    hipError_t e1 = scopy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
     rocblas_destroy_handle(h);
     if (rs != rocblas_status_success || e1 != hipSuccess) return ed_fail(ED_EDEVICE, "rocsolver_zheevd failed (%d)", (int)rs);
     if (info != 0) return ed_fail(ED_ENOTCONV, "zheevd info=%d", info);
-    HIPCHK(hipMemcpy(evals, dW, D * sizeof(double), hipMemcpyDeviceToHost));
-    if (evecs) HIPCHK(hipMemcpy(evecs, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+    HIPCHK(scopy(evals, dW, D * sizeof(double), hipMemcpyDeviceToHost));
+    if (evecs) HIPCHK(scopy(evecs, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
     dfree(dH); dfree(dW); dfree(dE); dfree(dinfo);
     return ED_OK;
 }

@@ -472,7 +472,7 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
             K->nterm = (int)be.size();
             auto up = [&](void** d, const void* h, size_t n) -> int {
                 HIPCHK(hipMalloc(d, n));
-                HIPCHK(hipMemcpy(*d, h, n, hipMemcpyHostToDevice));
+                HIPCHK(scopy(*d, h, n, hipMemcpyHostToDevice));
                 return ED_OK;
             };
             int rc = up((void**)&K->d_fam, fams[0].data(), nf * sizeof(TFam));
@@ -764,7 +764,7 @@ int mblock_upload_queue(ed_mblock_s* MB) {
     }
     MB->ntiles = tiles;
     if (!MB->secs.empty())
-        HIPCHK(hipMemcpy(MB->d_secs, MB->secs.data(), MB->secs.size() * sizeof(MSec), hipMemcpyHostToDevice));
+        HIPCHK(scopy(MB->d_secs, MB->secs.data(), MB->secs.size() * sizeof(MSec), hipMemcpyHostToDevice));
     return ED_OK;
 }
 

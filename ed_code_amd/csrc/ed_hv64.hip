@@ -48,9 +48,9 @@ int rows2_tables(ed_basis_s* B, const std::vector<uint64_t>& rt) {
     }
     B->rlo_n = (int)top;
     HIPCHK(hipMalloc(&B->d_rlo, std::max<size_t>(top, 1) * sizeof(uint32_t)));
-    if (top) HIPCHK(hipMemcpy(B->d_rlo, lo.data(), top * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (top) HIPCHK(scopy(B->d_rlo, lo.data(), top * sizeof(uint32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&B->d_rhi, 256 * sizeof(uint64_t)));
-    HIPCHK(hipMemcpy(B->d_rhi, rt.data() + top, 256 * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIPCHK(scopy(B->d_rhi, rt.data() + top, 256 * sizeof(uint64_t), hipMemcpyHostToDevice));
     return ED_OK;
 }
 
@@ -66,11 +66,11 @@ int rows2_op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vecto
     op->nterm64 = (int)ent.size();
     op->ezz0_64 = ezz0;
     HIPCHK(hipMalloc(&op->d_fam64, fam.size() * sizeof(TFam64)));
-    HIPCHK(hipMemcpy(op->d_fam64, fam.data(), fam.size() * sizeof(TFam64), hipMemcpyHostToDevice));
+    HIPCHK(scopy(op->d_fam64, fam.data(), fam.size() * sizeof(TFam64), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&op->d_foff64, foff.size() * sizeof(int)));
-    HIPCHK(hipMemcpy(op->d_foff64, foff.data(), foff.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(scopy(op->d_foff64, foff.data(), foff.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&op->d_ent64, ent.size() * sizeof(TEntry)));
-    HIPCHK(hipMemcpy(op->d_ent64, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
+    HIPCHK(scopy(op->d_ent64, ent.data(), ent.size() * sizeof(TEntry), hipMemcpyHostToDevice));
     op->use_rows2 = true;
     return ED_OK;
 }

@@ -57,6 +57,42 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
     w[i] = o;
 }
 
+// LDS of the compiled-shape kernels, in this order: [TChir x nfam*32
+// (kModeMerged)] [TEntry x nfam*32 (not kModeUniform)] [TFam x nfam, padded
+// to even] [cd2 x nfam (kModeUniform: J/2, Delta J/2)] [cd2 phase x nphase]
+// [u32 flip mask x nfam*32] [u32 rank tables x rtab_n] [u16 chunk map x nmap].
+// Stages the tables with the whole block and returns them as RowTabs.
+template <int MODE>
+__device__ __forceinline__ RowTabs stage_row_tabs(int nfam, const TFam* __restrict__ gfam,
+                                                  const TEntry* __restrict__ gent, const TChir* __restrict__ gchx,
+                                                  const uint32_t* __restrict__ gfmk, const cd2* __restrict__ gfco,
+                                                  const uint32_t* __restrict__ grtab, int rtab_n,
+                                                  const uint16_t* __restrict__ gmap, int nmap,
+                                                  const cd2* __restrict__ gphase, int nphase, uint16_t*& map) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    TChir* chx = (TChir*)smem;
+    TEntry* ent = (TEntry*)(chx + (MODE == kModeMerged ? nfam * 32 : 0));
+    TFam* fam = (TFam*)(ent + (MODE == kModeUniform ? 0 : nfam * 32));
+    cd2* fco = (cd2*)(fam + nfam + (nfam & 1));
+    cd2* phase = fco + (MODE == kModeUniform ? nfam : 0);
+    uint32_t* fmk = (uint32_t*)(phase + nphase);
+    uint32_t* rtab = fmk + nfam * 32;
+    map = (uint16_t*)(rtab + rtab_n);
+    if (MODE == kModeMerged)
+        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) chx[q] = gchx[q];
+    if (MODE != kModeUniform)
+        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
+    for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
+    if (MODE == kModeUniform)
+        for (int q = threadIdx.x; q < nfam; q += blockDim.x) fco[q] = gfco[q];
+    for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
+    for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) fmk[q] = gfmk[q];
+    for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
+    for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
+    __syncthreads();
+    return RowTabs{nfam, fam, ent, chx, fmk, fco, rtab, phase};
+}
+
 // Row body of the compiled-shape kernels: the lattice shape as compile-time
 // constants (fully unrolled rotation search, masks as immediates), K gathered
 // elements in flight per lane.  The order in which elements are accumulated
@@ -64,27 +100,16 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
 template <class TC, bool TRIV, int MODE, int K, bool PIPE, typename VT>
 __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                 const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
-                                                const TChir* __restrict__ gchx,
+                                                const TChir* __restrict__ gchx, const uint32_t* __restrict__ gfmk,
+                                                const cd2* __restrict__ gfco,
                                                 double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
                                                 int rtab_n, const uint16_t* __restrict__ gmap, int nmap,
                                                 int cb, uint32_t zidx, const cd2* __restrict__ gphase, int nphase,
                                                 const double* __restrict__ ntab, const VT* __restrict__ u,
                                                 const VT* __restrict__ v, VT* __restrict__ w) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    TChir* chx = (TChir*)smem;                                   // nfam*32 (kModeMerged)
-    TEntry* ent = (TEntry*)(chx + (MODE == kModeMerged ? nfam * 32 : 0));   // nfam*32
-    TFam* fam = (TFam*)(ent + nfam * 32);            // nfam
-    cd2* phase = (cd2*)(fam + nfam + (nfam & 1));    // nphase (16-byte aligned)
-    uint32_t* rtab = (uint32_t*)(phase + nphase);    // rtab_n
-    uint16_t* map = (uint16_t*)(rtab + rtab_n);      // nmap: gather-image chunk -> slot
-    if (MODE == kModeMerged)
-        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) chx[q] = gchx[q];
-    for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
-    for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
-    for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
-    for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
-    for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
-    __syncthreads();
+    uint16_t* map;
+    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gchx, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
+                                           nphase, map);
     // XCD tiles of consecutive rows, the 8 XCDs on adjacent tiles
     const unsigned lb = xcd_tile_block();
     int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
@@ -102,15 +127,13 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     auto acc = [&](const cd2& x, double cr, double ci) {
         ar = fma(cr, x.re, ar);
         ai = fma(cr, x.im, ai);
-        if (MODE != kModeReal) {
+        if (MODE != kModeReal && MODE != kModeUniform) {
             ar = fma(-ci, x.im, ar);
             ai = fma(ci, x.re, ai);
         }
     };
-    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, phase, jof,
-                                                              loadu, acc)
-                     : torus_row_batched<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, ezz0, R, rtab, phase, jof, loadu,
-                                                            acc);
+    double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, K>(s, T, ezz0, R, jof, loadu, acc)
+                     : torus_row_batched<TC, TRIV, MODE, K>(s, T, ezz0, R, jof, loadu, acc);
     cd2 vi = load_c(v, i);
     VT o;
     o.re = fma(nrm, ar, ez * vi.re);
@@ -118,15 +141,87 @@ __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __res
     w[i] = o;
 }
 
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)x, o), hi = (uint32_t)__shfl_xor((int)(uint32_t)(x >> 32), o);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Row body of the split kernels: 2^LPRL consecutive lanes share a row, each
+// taking one consecutive share of its elements (torus_row_share), K in
+// flight; the lanes' partial sums meet in a fixed tree (lane q with q ^ 1,
+// then q ^ 2, ...).  A workgroup then covers blockDim / 2^LPRL rows, so the
+// rows in flight on an XCD -- whose targets compete for its L2 -- are fewer
+// for the same number of waves.  Rounds differently from the one-lane
+// variants (the shares are summed separately); all split variants with the
+// same 2^LPRL agree bit for bit.
+template <class TC, bool TRIV, int MODE, int K, int LPRL, typename VT>
+__device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+                                                    const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
+                                                    const TChir* __restrict__ gchx,
+                                                    const uint32_t* __restrict__ gfmk, const cd2* __restrict__ gfco,
+                                                    double ezz0, const RankInfo& R,
+                                                    const uint32_t* __restrict__ grtab, int rtab_n,
+                                                    const uint16_t* __restrict__ gmap, int nmap, int cb,
+                                                    uint32_t zidx, const cd2* __restrict__ gphase, int nphase,
+                                                    const double* __restrict__ ntab, const VT* __restrict__ u,
+                                                    const VT* __restrict__ v, VT* __restrict__ w) {
+    constexpr int LPR = 1 << LPRL;
+    uint16_t* map;
+    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gchx, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
+                                           nphase, map);
+    const unsigned rpb = blockDim.x >> LPRL;
+    const unsigned lb = xcd_tile_block_rows(rpb);
+    const int q = (int)(threadIdx.x & (LPR - 1));
+    const int64_t i = (int64_t)lb * rpb + (threadIdx.x >> LPRL);
+    const bool live = i < D;
+    uint32_t s = 0;
+    double nrm = 0.0;
+    if (live) {
+        const uint64_t rw = rows[i];
+        s = (uint32_t)(rw & kStateMask);
+        nrm = ntab[(int)(rw >> kLidShift)];
+    }
+    // antiparallel count of every family: lane q counts families q, q + LPR, ...
+    uint64_t counts = live ? split_counts<TC>(s, T, q, LPR) : 0u;
+#pragma unroll
+    for (int o = 1; o < LPR; o <<= 1) counts |= shfl_xor_u64(counts, o);
+    double ar = 0.0, ai = 0.0, ez = 0.0;
+    auto jof = [&](uint32_t r, bool ok) { return image_index(map, cb, zidx, r, ok); };
+    auto loadu = [&](uint32_t j) { return load_c32(u, j); };
+    auto acc = [&](const cd2& x, double cr, double ci) {
+        ar = fma(cr, x.re, ar);
+        ai = fma(cr, x.im, ai);
+        if (MODE != kModeReal && MODE != kModeUniform) {
+            ar = fma(-ci, x.im, ar);
+            ai = fma(ci, x.re, ai);
+        }
+    };
+    if (live) ez = torus_row_share<TC, TRIV, MODE, K>(s, q, LPRL, counts, T, ezz0, R, jof, loadu, acc);
+#pragma unroll
+    for (int o = 1; o < LPR; o <<= 1) {
+        ar += __shfl_xor(ar, o);
+        ai += __shfl_xor(ai, o);
+        ez += __shfl_xor(ez, o);
+    }
+    if (live && q == 0) {
+        const cd2 vi = load_c(v, i);
+        VT o;
+        o.re = fma(nrm, ar, ez * vi.re);
+        o.im = fma(nrm, ai, ez * vi.im);
+        w[i] = o;
+    }
+}
+
 #define ED_TORUS_C_ARGS                                                                                   \
     int64_t D, const uint64_t *__restrict__ rows, int nfam, const TFam *__restrict__ gfam,               \
-        const TEntry *__restrict__ gent, const TChir *__restrict__ gchx, double ezz0, RankInfo R,        \
+        const TEntry *__restrict__ gent, const TChir *__restrict__ gchx, const uint32_t *__restrict__ gfmk, \
+        const cd2 *__restrict__ gfco, double ezz0, RankInfo R,                                             \
         const uint32_t *__restrict__ grtab,                                                               \
         int rtab_n, const uint16_t *__restrict__ gmap, int nmap, int cb, uint32_t zidx,                   \
         const cd2 *__restrict__ gphase, int nphase, const double *__restrict__ ntab, const VT *__restrict__ u, \
         const VT *__restrict__ v, VT *__restrict__ w
 #define ED_TORUS_C_PASS \
-    D, rows, nfam, gfam, gent, gchx, ezz0, R, grtab, rtab_n, gmap, nmap, cb, zidx, gphase, nphase, ntab, u, v, w
+    D, rows, nfam, gfam, gent, gchx, gfmk, gfco, ezz0, R, grtab, rtab_n, gmap, nmap, cb, zidx, gphase, nphase, ntab, u, v, w
 
 // Variant A: four elements in flight, occupancy left to the register count.
 template <class TC, bool TRIV, int MODE, typename VT>
@@ -152,24 +247,42 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8))) k
     hv_torus_c_body<TC, TRIV, MODE, 2, false, VT>(ED_TORUS_C_PASS);
 }
 
+// Variant E: rows split over two lanes, eight elements in flight per lane.
+template <class TC, bool TRIV, int MODE, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_s2(ED_TORUS_C_ARGS) {
+    hv_torus_split_body<TC, TRIV, MODE, 8, 1, VT>(ED_TORUS_C_PASS);
+}
+
+// Variant F: rows split over four lanes, four elements in flight per lane.
+template <class TC, bool TRIV, int MODE, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_s4(ED_TORUS_C_ARGS) {
+    hv_torus_split_body<TC, TRIV, MODE, 4, 2, VT>(ED_TORUS_C_PASS);
+}
+
+// Lanes per row of a variant (1 for the one-lane variants 0-3).
+static int variant_lanes(int var) { return var == 4 ? 2 : var == 5 ? 4 : 1; }
+
 template <class TC, typename VT>
 static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, size_t sm, int var, int bs) {
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
-    dim3 grid((unsigned)((D + bs - 1) / bs)), blk(bs);
+    const int64_t rpb = bs / variant_lanes(var);   // rows per workgroup
+    dim3 grid((unsigned)((D + rpb - 1) / rpb)), blk(bs);
     int nph = B->L1 * B->L2;
     const uint32_t zidx = B->img_nslot << B->img_cb;   // first element of the zero slot
     // the merged families (bond pairs carrying their chiral entries) when built
     const bool mg = op->tmode == kModeMerged;
 #define ED_GO(KER)                                                                                            \
     hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, mg ? op->nmfam : op->nfam, mg ? op->d_mfam : op->d_tfam, \
-                       mg ? op->d_ment : op->d_tent, op->d_tchx, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,    \
-                       B->d_img_map, (int)B->img_nmap, B->img_cb, zidx, B->d_phase, nph, B->d_ntab, u, v, w)
-    // term class: k = 0 takes the general or the real-bond kernel, k != 0 the
-    // general or the bond-only one
+                       mg ? op->d_ment : op->d_tent, op->d_tchx, mg ? op->d_mfmk : op->d_tfmk, op->d_tfco, op->ezz0,  \
+                       B->rinfo, B->d_rtab32, B->rtab32_n, B->d_img_map, (int)B->img_nmap, B->img_cb, zidx,       \
+                       B->d_phase, nph, B->d_ntab, u, v, w)
+    // term class: k = 0 takes the general, real-bond or uniform-family kernel,
+    // k != 0 the general or the bond-only one
 #define ED_GO_MODES(KERNEL)                                                         \
     if (B->trivial_k) {                                                             \
-        if (op->tmode == kModeReal) ED_GO((KERNEL<TC, true, kModeReal, VT>));       \
+        if (op->tmode == kModeUniform) ED_GO((KERNEL<TC, true, kModeUniform, VT>)); \
+        else if (op->tmode == kModeReal) ED_GO((KERNEL<TC, true, kModeReal, VT>));  \
         else if (mg) ED_GO((KERNEL<TC, true, kModeMerged, VT>));                    \
         else ED_GO((KERNEL<TC, true, kModeGeneral, VT>));                           \
     } else {                                                                        \
@@ -183,8 +296,12 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
         ED_GO_MODES(k_hv_torus_c2w8)
     } else if (var == 2) {
         ED_GO_MODES(k_hv_torus_c8)
-    } else {
+    } else if (var == 3) {
         ED_GO_MODES(k_hv_torus_p4)
+    } else if (var == 4) {
+        ED_GO_MODES(k_hv_torus_s2)
+    } else {
+        ED_GO_MODES(k_hv_torus_s4)
     }
 #undef ED_GO_MODES
 #undef ED_GO
@@ -228,10 +345,11 @@ size_t torus_smem_bytes(const ed_op_s* op) {
 // operator's family count.
 static size_t torus_c_smem_bytes(const ed_op_s* op) {
     const ed_basis_s* B = op->b;
-    const bool mg = op->tmode == kModeMerged;
+    const bool mg = op->tmode == kModeMerged, un = op->tmode == kModeUniform;
     const int nf = mg ? op->nmfam : op->nfam;
-    return (mg ? (size_t)nf * 32 * sizeof(TChir) : 0) + (size_t)nf * 32 * sizeof(TEntry) +
-           (size_t)(nf + (nf & 1)) * sizeof(TFam) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
+    return (mg ? (size_t)nf * 32 * sizeof(TChir) : 0) + (un ? 0 : (size_t)nf * 32 * sizeof(TEntry)) +
+           (size_t)(nf + (nf & 1)) * sizeof(TFam) + (un ? (size_t)nf * sizeof(cd2) : 0) +
+           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)nf * 32 * sizeof(uint32_t) +
            (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)B->img_nmap * sizeof(uint16_t);
 }
 
@@ -261,7 +379,14 @@ static constexpr int dtype_slot() {
 // fastest depends on the register pressure of the shape, the vector precision
 // and the momentum (measured on the 32-site torus: K=4 at 256 threads for
 // complex64 at k=0, K=8 at 256 for complex128, K=2 at 8 waves for k != 0).
-static const int kTuneCfg[8][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256}, {3, 512}, {3, 256}};
+// The split variants (4, 5) serve families of at most kSplitMaxFam.
+static const int kTuneCfg[12][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256},
+                                    {3, 512}, {3, 256}, {4, 512}, {4, 256}, {5, 512}, {5, 256}};
+
+static bool variant_fits(const ed_op_s* op, int var) {
+    const int nf = op->tmode == kModeMerged ? op->nmfam : op->nfam;
+    return variant_lanes(var) == 1 || nf <= kSplitMaxFam;
+}
 static const int64_t kTuneMinRows = int64_t(1) << 20;
 
 // Chooses the configuration of the compiled-shape kernel for this operator
@@ -286,6 +411,7 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
         float best = 1e30f;
         for (const auto& c : kTuneCfg) {
             if (op->block_fixed && c[1] != op->tune_block) continue;
+            if (!variant_fits(op, c[0])) continue;
             float t = 1e30f;
             for (int rep = 0; rep < 3; rep++) {
                 HIPCHK(hipEventRecord(e0, st));
@@ -312,8 +438,9 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
 }
 
 // Kernel configuration the next launch uses: path 2 (compiled shape),
-// 3 (run-time shape); variant 0 (K=4) or 1 (K=2 at 8 waves), -1 before
-// tuning or for the run-time kernel.
+// 3 (run-time shape); variant 0 (K=4), 1 (K=2 at 8 waves), 2 (K=8),
+// 3 (K=4 pipelined), 4 (two lanes per row, K=8), 5 (four lanes per row,
+// K=4), -1 before tuning or for the run-time kernel.
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     const bool compiled = op->tune_path != 3 && compiled_fits(op);
     path = compiled ? 2 : 3;
@@ -324,6 +451,8 @@ void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     else if (op->tune_path == 4) var = 1;
     else if (op->tune_path == 5) var = 2;
     else if (op->tune_path == 7) var = 3;
+    else if (op->tune_path == 9) var = variant_fits(op, 4) ? 4 : 0;
+    else if (op->tune_path == 10) var = variant_fits(op, 5) ? 5 : 0;
     else if (op->sel_var[dslot] >= 0) {
         var = op->sel_var[dslot];
         bs = op->sel_block[dslot];

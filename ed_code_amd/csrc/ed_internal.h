@@ -19,16 +19,36 @@ int ed_fail(int code, const char* fmt, ...);
 // that does not fill a round of 8 tiles keeps the dispatch order.  A
 // bijection of [0, gridDim.x) for any grid.
 constexpr unsigned kXcdTileRows = 32768;
-__device__ inline unsigned xcd_tile_block() {
+__device__ inline unsigned xcd_tile_block_rows(unsigned rows_per_block) {
     const unsigned nb = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const unsigned T = blockDim.x >= kXcdTileRows ? 1u : kXcdTileRows / blockDim.x;
+    const unsigned T = rows_per_block >= kXcdTileRows ? 1u : kXcdTileRows / rows_per_block;
     const unsigned full = nb / (8 * T);
     return blockIdx.x < full * 8 * T ? ((slot / T) * 8 + xcd) * T + (slot % T) : blockIdx.x;
 }
+// one row per thread
+__device__ inline unsigned xcd_tile_block() { return xcd_tile_block_rows(blockDim.x); }
 
 inline void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
+
+// The calling host thread's non-blocking stream (one per device) for the
+// library's own setup work: basis builds, operator and image tables, host
+// copies of tables, dense matrices.  Work on the legacy default stream would
+// wait on other streams and is refused while another thread captures a
+// graph (concurrent solves in the serving loop); a private non-blocking
+// stream keeps the setup of one solve independent of the captures of others.
+// None of this work reads device data a caller produced.
+hipStream_t setup_stream();
+// copy on the setup stream, then wait for it
+inline hipError_t scopy(void* dst, const void* src, size_t n, hipMemcpyKind kind) {
+    hipStream_t s = setup_stream();
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+// zero n bytes, ordered before later work on the setup stream
+inline hipError_t szero(void* p, size_t n) { return hipMemsetAsync(p, 0, n, setup_stream()); }
+inline hipError_t ssync() { return hipStreamSynchronize(setup_stream()); }
 
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
@@ -99,13 +119,17 @@ struct ed_op_s {
     edc::TFam* d_tfam = nullptr;
     edc::TEntry* d_tent = nullptr;
     double ezz0 = 0;
-    int tmode = 0;             // term class of the families (kModeGeneral / kModeBonds / kModeReal / kModeMerged)
+    int tmode = 0;             // term class of the families (kModeGeneral / kModeBonds / kModeReal / kModeMerged /
+                               // kModeUniform)
+    uint32_t* d_tfmk = nullptr;   // flip mask per family member [nfam * 32]
+    edc::cd2* d_tfco = nullptr;   // kModeUniform: {J/2, Delta J/2} per family
     // kModeMerged: the compiled-shape kernels' bond-keyed families with the
     // chiral entries of each pair (d_tfam / d_tent stay for the other kernels)
     int nmfam = 0;
     edc::TFam* d_mfam = nullptr;
     edc::TEntry* d_ment = nullptr;
     edc::TChir* d_tchx = nullptr;
+    uint32_t* d_mfmk = nullptr;   // flip mask per merged family member [nmfam * 32]
     // two-row 64-bit rotation path (ed_hv64.hip)
     bool use_rows2 = false;
     int nfam64 = 0, nterm64 = 0;
@@ -115,7 +139,8 @@ struct ed_op_s {
     double ezz0_64 = 0;
     int tune_path = 0;        // 0 auto, 1 generic walk, 2/4/5 compiled shape K=4 / K=2 / K=8, 3 run-time shape,
                               // 6 two-row 64-bit kernel, 7 compiled shape K=4 pipelined, 8 as 6 with u by
-                              // row through the directory
+                              // row through the directory, 9/10 compiled shape with rows split over 2 / 4
+                              // lanes
     int tune_block = 256;
     bool block_fixed = false; // block set by ed_op_tune
     int sel_var[2] = {-1, -1};   // tuned variant per precision (0 complex128, 1 complex64)

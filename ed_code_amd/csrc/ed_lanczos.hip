@@ -2653,7 +2653,9 @@ extern "C" int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, d
                 if (stats && k < 64) stats->eval_error[k] = res;
             }
             if (evecs) {
-                hipError_t e = hipMemcpy(evecs + 2 * D * k, prev[k], D * sizeof(cd2), hipMemcpyDeviceToHost);
+                // on the solve's stream, after the kernels that formed the vector
+                hipError_t e = hipMemcpyAsync(evecs + 2 * D * k, prev[k], D * sizeof(cd2), hipMemcpyDeviceToHost, c.st);
+                if (e == hipSuccess) e = hipStreamSynchronize(c.st);
                 if (e != hipSuccess) { rc = ed_fail(ED_EDEVICE, "copy of eigenvector: %s", hipGetErrorString(e)); break; }
             }
         }

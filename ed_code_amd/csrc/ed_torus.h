@@ -52,13 +52,9 @@ ED_HD uint32_t alignbit32(uint32_t hi, uint32_t lo, uint32_t sh) {
 #endif
 }
 
-ED_HD int ctz32(uint32_t x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __ffs(x) - 1;
-#else
-    return __builtin_ctz(x);
-#endif
-}
+// Index of the lowest set bit; x != 0 at every call (one v_ffbl, no guard
+// for a zero argument).
+ED_HD int ctz32(uint32_t x) { return __builtin_ctz(x); }
 
 ED_HD uint32_t min3u(uint32_t a, uint32_t b, uint32_t c) {
     uint32_t m = a < b ? a : b;
@@ -422,8 +418,43 @@ ED_HD int64_t dir_decode(const DirEntry& e, uint32_t r) { return dir_decode_off(
 //                 caller accumulates with two FMAs and ignores ci);
 //   kModeMerged   bond-keyed families whose members also carry the chiral
 //                 entries of the same flipped pair (TChir): one orbit search,
-//                 rank and gather per pair instead of one per term.
-enum { kModeGeneral = 0, kModeBonds = 1, kModeReal = 2, kModeMerged = 3 };
+//                 rank and gather per pair instead of one per term;
+//   kModeUniform  kModeReal where all members of a family share J/2 and
+//                 Delta J/2: the coefficient is read once per family and the
+//                 diagonal takes each family's antiparallel count at once, so
+//                 an element costs no coefficient load and no FP64 add.
+enum { kModeGeneral = 0, kModeBonds = 1, kModeReal = 2, kModeMerged = 3, kModeUniform = 4 };
+
+// Tables of a compiled-shape row (staged in LDS by the kernels).
+struct RowTabs {
+    int nfam;
+    const TFam* fam;
+    const TEntry* ent;     // per member, [f * 32 + x] (not read under kModeUniform)
+    const TChir* chx;      // kModeMerged: chiral entries of each member's pair
+    const uint32_t* fmk;   // per member, [f * 32 + x]: the flip mask (1 << x) | (1 << y)
+    const cd2* fco;        // kModeUniform: {J/2, Delta J/2} per family
+    const uint32_t* rtab;  // 32-bit byte-sliced rank tables
+    const cd2* phase;      // exp(-i k.r) per phase code
+};
+
+// Cursor over a row's elements: the current family f, its antiparallel
+// members not yet visited (cur), and its coefficient under kModeUniform (cu).
+struct RowCur {
+    int f;
+    uint32_t cur;
+    double cu;
+};
+
+// Moves the cursor onto family c.f.  Under kModeUniform with EZ the diagonal
+// takes the whole family at once: ezz -= popcount * Delta J / 2.
+template <class TC, int MODE, bool EZ>
+ED_HD void fam_enter(uint32_t s, const RowTabs& T, RowCur& c, double& ezz) {
+    c.cur = fam_mask_c<TC>(s, T.fam[c.f]);
+    if (MODE == kModeUniform) {
+        c.cu = T.fco[c.f].re;
+        if (EZ) ezz = fma(-(double)popc32(c.cur), T.fco[c.f].im, ezz);
+    }
+}
 
 // One batch of up to K elements of a row: rank of the representative of each
 // target and its coefficient (phase applied); ok[q] false for empty slots.
@@ -435,50 +466,56 @@ struct RowBatch {
 };
 
 // Fills the next batch of the row (the orbit search and rank of K
-// elements), advancing the family cursor (f, cur) and the diagonal ezz.
-template <class TC, bool TRIV, int MODE, int K>
-ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx, const RankInfo& R,
-                      const uint32_t* rtab, const cd2* phase, int& f, uint32_t& cur, double& ezz, RowBatch<K>& B) {
+// elements), advancing the cursor and the diagonal ezz.  SPLIT: the row is
+// shared by several lanes; at most `left` more elements belong to this lane,
+// and the kModeUniform diagonal is not taken here (one lane computes it).
+template <class TC, bool TRIV, int MODE, int K, bool SPLIT>
+ED_HD void fill_batch(uint32_t s, const RowTabs& T, const RankInfo& R, RowCur& c, int& left, double& ezz,
+                      RowBatch<K>& B) {
 #pragma unroll
     for (int q = 0; q < K; q++) {
-        while (cur == 0u && f < nfam) {
-            if (++f < nfam) cur = fam_mask_c<TC>(s, fam[f]);
+        while (c.cur == 0u && c.f < T.nfam && (!SPLIT || left > 0)) {
+            if (++c.f < T.nfam) fam_enter<TC, MODE, !SPLIT>(s, T, c, ezz);
         }
-        B.ok[q] = f < nfam;
+        B.ok[q] = c.f < T.nfam && (!SPLIT || left > 0);
         B.rk[q] = 0; B.cre[q] = 0; B.cim[q] = 0;
         if (B.ok[q]) {
-            int x = ctz32(cur);
-            cur &= cur - 1u;
-            const TEntry& E = ent[f * 32 + x];
-            uint32_t t = s ^ (1u << x) ^ (1u << E.y);
+            if (SPLIT) left--;
+            const int x = ctz32(c.cur);
+            c.cur &= c.cur - 1u;
+            const uint32_t t = s ^ T.fmk[c.f * 32 + x];
             double cr, ci = 0.0;
-            if (MODE != kModeGeneral || fam[f].type == 0) {
+            if (MODE == kModeUniform) {
+                cr = c.cu;
+            } else if (MODE != kModeGeneral || T.fam[c.f].type == 0) {
+                const TEntry& E = T.ent[c.f * 32 + x];
                 ezz -= E.dJ2;
                 cr = E.cre;
                 if (MODE != kModeReal) ci = ((s >> x) & 1u) ? -E.cim : E.cim;
                 if (MODE == kModeMerged) {
                     // the chiral entries of the pair, in a fixed order
-                    const TChir& X = chx[f * 32 + x];
+                    const TChir& X = T.chx[c.f * 32 + x];
 #pragma unroll
-                    for (int c = 0; c < 2; c++) {
-                        if (!((X.flags >> (2 * c)) & 1)) continue;
-                        const double sg = ((s >> X.k[c]) & 1u) ? 1.0 : -1.0;
-                        const uint32_t jbit = ((X.flags >> (2 * c + 1)) & 1) ? (s >> x) & 1u : (s >> E.y) & 1u;
-                        cr += -X.cim[c] * sg;
-                        ci += (jbit ? X.cre[c] : -X.cre[c]) * sg;
+                    for (int h = 0; h < 2; h++) {
+                        if (!((X.flags >> (2 * h)) & 1)) continue;
+                        const double sg = ((s >> X.k[h]) & 1u) ? 1.0 : -1.0;
+                        const uint32_t jbit = ((X.flags >> (2 * h + 1)) & 1) ? (s >> x) & 1u : (s >> E.y) & 1u;
+                        cr += -X.cim[h] * sg;
+                        ci += (jbit ? X.cre[h] : -X.cre[h]) * sg;
                     }
                 }
             } else {
+                const TEntry& E = T.ent[c.f * 32 + x];
                 double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;
                 cr = -E.cim * sg;
                 ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
             }
             int code;
             uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
-            uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
+            uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, T.rtab, (TC::N + 7) / 8);
             if (!TRIV) {
                 // explicit FMAs: every variant rounds the same way
-                cd2 ph = phase[code];
+                cd2 ph = T.phase[code];
                 double nr = fma(cr, ph.re, -(ci * ph.im)), ni = fma(cr, ph.im, ci * ph.re);
                 cr = nr; ci = ni;
             }
@@ -487,27 +524,27 @@ ED_HD void fill_batch(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, 
     }
 }
 
-// torus_row_c with K elements in flight per lane: the orbit searches and
-// ranks of K elements are computed first, then their K gathers are issued
-// back to back, so that a lane waits for memory once per K elements.  The
-// loads are unconditional: a load under a lane mask would make the compiler
-// wait for it inside the masked block and serialise the K loads.
+// The row with K elements in flight per lane: the orbit searches and ranks of
+// K elements are computed first, then their K gathers are issued back to
+// back, so that a lane waits for memory once per K elements.  The loads are
+// unconditional: a load under a lane mask would make the compiler wait for it
+// inside the masked block and serialise the K loads.
 // jof(r, ok) gives the element of u that holds the target of rank r; for an
 // empty slot (ok false: no element left) it gives an element that holds
 // zero, and so does every rank outside the sector, so such slots add nothing
 // whatever their coefficient.  loadu(j) returns element j of u; g(x, cr, ci)
-// accumulates.
+// accumulates.  Returns the diagonal.
 template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
-ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
-                               double ezz0, const RankInfo& R, const uint32_t* rtab, const cd2* phase, JOf&& jof,
+ED_HD double torus_row_batched(uint32_t s, const RowTabs& T, double ezz0, const RankInfo& R, JOf&& jof,
                                LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
-    int f = 0;
-    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    RowCur c{0, 0u, 0.0};
+    if (T.nfam > 0) fam_enter<TC, MODE, true>(s, T, c, ezz);
+    int left = 0;
     bool more = true;
     while (more) {
         RowBatch<K> B;
-        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
+        fill_batch<TC, TRIV, MODE, K, false>(s, T, R, c, left, ezz, B);
         more = B.ok[K - 1];
         cd2 xv[K];
 #pragma unroll
@@ -525,20 +562,20 @@ ED_HD double torus_row_batched(uint32_t s, int nfam, const TFam* fam, const TEnt
 // Elements are accumulated in the same order as torus_row_batched, so the
 // result is bit-identical.
 template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
-ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TEntry* ent, const TChir* chx,
-                                 double ezz0, const RankInfo& R, const uint32_t* rtab, const cd2* phase, JOf&& jof,
+ED_HD double torus_row_pipelined(uint32_t s, const RowTabs& T, double ezz0, const RankInfo& R, JOf&& jof,
                                  LoadU&& loadu, Gf&& g) {
     double ezz = ezz0;
-    int f = 0;
-    uint32_t cur = nfam > 0 ? fam_mask_c<TC>(s, fam[0]) : 0u;
+    RowCur c{0, 0u, 0.0};
+    if (T.nfam > 0) fam_enter<TC, MODE, true>(s, T, c, ezz);
+    int left = 0;
     RowBatch<K> A, B;
-    fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, A);
+    fill_batch<TC, TRIV, MODE, K, false>(s, T, R, c, left, ezz, A);
     cd2 xa[K];
 #pragma unroll
     for (int q = 0; q < K; q++) xa[q] = loadu(jof(A.rk[q], A.ok[q]));
     bool more = A.ok[K - 1];
     while (more) {
-        fill_batch<TC, TRIV, MODE, K>(s, nfam, fam, ent, chx, R, rtab, phase, f, cur, ezz, B);
+        fill_batch<TC, TRIV, MODE, K, false>(s, T, R, c, left, ezz, B);
 #pragma unroll
         for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
 #pragma unroll
@@ -552,6 +589,60 @@ ED_HD double torus_row_pipelined(uint32_t s, int nfam, const TFam* fam, const TE
     }
 #pragma unroll
     for (int q = 0; q < K; q++) g(xa[q], A.cre[q], A.cim[q]);
+    return ezz;
+}
+
+// A row shared by LPR lanes (the split kernels): the n antiparallel elements
+// of the row, in the order torus_row_batched visits them, are cut into LPR
+// consecutive shares, elements [q n / LPR, (q + 1) n / LPR) to lane q.
+// counts holds the antiparallel count of family f in bits [6f, 6f + 6)
+// (nfam <= kSplitMaxFam).  The lane accumulates its share through g (K in
+// flight) and returns its part of the diagonal: the per-element Delta J / 2
+// of its share, plus, for lane 0, ezz0 (less every family's count times
+// Delta J / 2 under kModeUniform, as torus_row_batched takes it).  The
+// caller adds the lanes' sums as a tree, lane q with q ^ 1, then q ^ 2, ...
+constexpr int kSplitMaxFam = 10;
+
+ED_HD int split_count(uint64_t counts, int f) { return (int)((counts >> (6 * f)) & 63u); }
+
+template <class TC>
+ED_HD uint64_t split_counts(uint32_t s, const RowTabs& T, int first, int step) {
+    uint64_t c = 0;
+    for (int f = first; f < T.nfam; f += step) c |= (uint64_t)popc32(fam_mask_c<TC>(s, T.fam[f])) << (6 * f);
+    return c;
+}
+
+template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
+ED_HD double torus_row_share(uint32_t s, int q, int lpr_log2, uint64_t counts, const RowTabs& T, double ezz0,
+                             const RankInfo& R, JOf&& jof, LoadU&& loadu, Gf&& g) {
+    int n = 0;
+    for (int f = 0; f < T.nfam; f++) n += split_count(counts, f);
+    const int st = (q * n) >> lpr_log2, en = ((q + 1) * n) >> lpr_log2;
+    double ezz = 0.0;
+    if (q == 0) {
+        ezz = ezz0;
+        if (MODE == kModeUniform)
+            for (int f = 0; f < T.nfam; f++) ezz = fma(-(double)split_count(counts, f), T.fco[f].im, ezz);
+    }
+    int left = en - st;
+    if (left <= 0) return ezz;
+    // the family holding element st, then its (st - cum) earlier members skipped
+    RowCur c{0, 0u, 0.0};
+    int cum = 0;
+    while (cum + split_count(counts, c.f) <= st) cum += split_count(counts, c.f++);
+    fam_enter<TC, MODE, false>(s, T, c, ezz);
+    for (int k = st - cum; k > 0; k--) c.cur &= c.cur - 1u;
+    bool more = true;
+    while (more) {
+        RowBatch<K> B;
+        fill_batch<TC, TRIV, MODE, K, true>(s, T, R, c, left, ezz, B);
+        more = B.ok[K - 1] && left > 0;
+        cd2 xv[K];
+#pragma unroll
+        for (int h = 0; h < K; h++) xv[h] = loadu(jof(B.rk[h], B.ok[h]));
+#pragma unroll
+        for (int h = 0; h < K; h++) g(xv[h], B.cre[h], B.cim[h]);
+    }
     return ezz;
 }
 

@@ -96,6 +96,32 @@ inline bool torus_families(double lam, double Ch, const Torus& T, const std::vec
     return true;
 }
 
+// Flip mask (1 << x) | (1 << y) of every family member ([f * 32 + x]) and
+// each family's {J/2, Delta J/2}; true when every family is a bond family
+// whose members all carry the same real J/2 and Delta J/2 (kModeUniform).
+inline bool torus_fam_extras(const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
+                             std::vector<uint32_t>& fmk, std::vector<cd2>& fco) {
+    fmk.assign(kMaxFam * 32, 0u);
+    fco.assign(fam.size(), cd2{0.0, 0.0});
+    bool uniform = true;
+    for (size_t f = 0; f < fam.size(); f++) {
+        bool first = true;
+        for (int x = 0; x < 32; x++) {
+            if (!((fam[f].S >> x) & 1u)) continue;
+            const TEntry& E = ent[f * 32 + x];
+            fmk[f * 32 + x] = (1u << x) | (1u << E.y);
+            if (fam[f].type != 0 || E.cim != 0.0) uniform = false;
+            if (first) {
+                fco[f] = cd2{E.cre, E.dJ2};
+                first = false;
+            } else if (E.cre != fco[f].re || E.dJ2 != fco[f].im) {
+                uniform = false;
+            }
+        }
+    }
+    return uniform;
+}
+
 // Families for kModeMerged: keyed by the bonds (pair x -> y = h(x)); each
 // chiral entry (i, j, k) joins the member whose pair is {i, j} (listed either
 // way), or opens a member with a zero bond part if no bond flips that pair.

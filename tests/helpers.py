@@ -297,6 +297,8 @@ def hostcore():
         H.hc_torus_mode.argtypes = [vp]
         H.hc_hv_torus_p.restype = C.c_int
         H.hc_hv_torus_p.argtypes = [vp] * 3
+        H.hc_hv_torus_s.restype = C.c_int
+        H.hc_hv_torus_s.argtypes = [vp, vp, vp, C.c_int]
         H.hc_torus_rep.restype = C.c_int
         H.hc_torus_rep.argtypes = [vp, C.c_uint32, vp, vp]
         H.hc_generic_rep.argtypes = [vp, C.c_uint64, vp, vp]
@@ -366,6 +368,15 @@ class HostCore:
         w = np.zeros(self.D, np.complex128)
         fn = hostcore().hc_hv_torus_p if pipelined else hostcore().hc_hv_torus_c
         if fn(self.h, v.ctypes.data, w.ctypes.data):
+            return None
+        return w
+
+    def hv_torus_split(self, v, lanes):
+        """Compiled-shape row split over `lanes` lanes, summed in the split
+        kernels' tree order; None if no compiled shape fits."""
+        v = np.ascontiguousarray(v, np.complex128)
+        w = np.zeros(self.D, np.complex128)
+        if hostcore().hc_hv_torus_s(self.h, v.ctypes.data, w.ctypes.data, lanes):
             return None
         return w
 

@@ -108,45 +108,69 @@ void hv(HC* h, const cd2* v, cd2* w) {
         w[i] = cd2{nrm * ar + ez * v[i].re, nrm * ai + ez * v[i].im};
     }
 }
-template <class TC, bool TRIV, int MODE, bool PIPE>
-void hv_batched(HC* h, const Torus& T, const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
-                const std::vector<TChir>& chx, double ezz0, const std::vector<uint32_t>& rt32,
-                const ImageLayout& L, const std::vector<cd2>& u, const cd2* v, cd2* w) {
+// layout: 0 = torus_row_batched, 1 = torus_row_pipelined, 2/4/8 = the row
+// split over that many lanes (torus_row_share), the lanes' sums added in the
+// split kernels' tree order (lane q with q ^ 1, then q ^ 2, ...).
+template <class TC, bool TRIV, int MODE>
+void hv_batched(HC* h, int layout, const RowTabs& T, double ezz0, const ImageLayout& L, const std::vector<cd2>& u,
+                const cd2* v, cd2* w) {
     const uint32_t zidx = L.nslot << L.cb;   // the zero slot
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t i = 0; i < (int64_t)h->rows.size(); i++) {
         uint32_t s = (uint32_t)(h->rows[i] & kStateMask);
         double nrm = h->nt[h->rows[i] >> kLidShift];
-        double ar = 0, ai = 0;
         auto loadu = [&](uint32_t j) { return u.at(j); };
         auto jof = [&](uint32_t r, bool ok) { return image_index(L.map.data(), L.cb, zidx, r, ok); };
+        double ar = 0, ai = 0;
         auto acc = [&](const cd2& x, double cr, double ci) {
             ar = std::fma(cr, x.re, ar);
             ai = std::fma(cr, x.im, ai);
-            if (MODE != kModeReal) {
+            if (MODE != kModeReal && MODE != kModeUniform) {
                 ar = std::fma(-ci, x.im, ar);
                 ai = std::fma(ci, x.re, ai);
             }
         };
-        const TChir* cx = chx.empty() ? nullptr : chx.data();
-        double ez = PIPE ? torus_row_pipelined<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
-                                                                  h->R, rt32.data(), h->ph.data(), jof, loadu, acc)
-                         : torus_row_batched<TC, TRIV, MODE, 4>(s, (int)fam.size(), fam.data(), ent.data(), cx, ezz0,
-                                                                h->R, rt32.data(), h->ph.data(), jof, loadu, acc);
+        double ez;
+        if (layout == 0) {
+            ez = torus_row_batched<TC, TRIV, MODE, 4>(s, T, ezz0, h->R, jof, loadu, acc);
+        } else if (layout == 1) {
+            ez = torus_row_pipelined<TC, TRIV, MODE, 4>(s, T, ezz0, h->R, jof, loadu, acc);
+        } else {
+            const int lg = layout == 2 ? 1 : layout == 4 ? 2 : 3, n = 1 << lg;
+            const uint64_t counts = split_counts<TC>(s, T, 0, 1);
+            double pr[8], pi[8], pe[8];
+            for (int q = 0; q < n; q++) {
+                ar = ai = 0;
+                pe[q] = torus_row_share<TC, TRIV, MODE, 4>(s, q, lg, counts, T, ezz0, h->R, jof, loadu, acc);
+                pr[q] = ar;
+                pi[q] = ai;
+            }
+            for (int o = 1; o < n; o <<= 1) {
+                double tr[8], ti[8], te[8];
+                for (int q = 0; q < n; q++) {
+                    tr[q] = pr[q] + pr[q ^ o];
+                    ti[q] = pi[q] + pi[q ^ o];
+                    te[q] = pe[q] + pe[q ^ o];
+                }
+                for (int q = 0; q < n; q++) { pr[q] = tr[q]; pi[q] = ti[q]; pe[q] = te[q]; }
+            }
+            ar = pr[0];
+            ai = pi[0];
+            ez = pe[0];
+        }
         w[i] = cd2{std::fma(nrm, ar, ez * v[i].re), std::fma(nrm, ai, ez * v[i].im)};
     }
 }
 
-template <class TC, bool PIPE>
-void hv_modes(HC* h, const Torus& T, bool triv, int mode, const std::vector<TFam>& fam,
-              const std::vector<TEntry>& ent, const std::vector<TChir>& chx, double ezz0,
-              const std::vector<uint32_t>& rt32, const ImageLayout& L, const std::vector<cd2>& u, const cd2* v,
-              cd2* w) {
-#define HC_RUN(TRIV, MODE) hv_batched<TC, TRIV, MODE, PIPE>(h, T, fam, ent, chx, ezz0, rt32, L, u, v, w)
+template <class TC>
+void hv_modes(HC* h, int layout, bool triv, int mode, const RowTabs& T, double ezz0, const ImageLayout& L,
+              const std::vector<cd2>& u, const cd2* v, cd2* w) {
+#define HC_RUN(TRIV, MODE) hv_batched<TC, TRIV, MODE>(h, layout, T, ezz0, L, u, v, w)
     if (mode == kModeMerged) {
         if (triv) HC_RUN(true, kModeMerged);
         else HC_RUN(false, kModeMerged);
-    } else if (triv && mode == kModeReal) HC_RUN(true, kModeReal);
+    } else if (triv && mode == kModeUniform) HC_RUN(true, kModeUniform);
+    else if (triv && mode == kModeReal) HC_RUN(true, kModeReal);
     else if (triv) HC_RUN(true, kModeGeneral);
     else if (mode != kModeGeneral) HC_RUN(false, kModeBonds);
     else HC_RUN(false, kModeGeneral);
@@ -449,7 +473,7 @@ int hc_torus_rep(void* p, uint32_t t, uint32_t* rep, int* code) {
 
 // H*v through the compiled-shape, batched row function of the GPU kernel.
 // Returns 1 if the lattice has none of the compiled shapes.
-static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
+static int hv_torus_c_impl(void* p, const double* v_, double* w_, int layout) {
     HC* h = (HC*)p;
     const cd2* v = (const cd2*)v_;
     cd2* w = (cd2*)w_;
@@ -496,12 +520,16 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, bool pipe) {
             chx.clear();
         }
     }
+    std::vector<uint32_t> fmk;
+    std::vector<cd2> fco;
+    if (torus_fam_extras(fam, ent, fmk, fco) && mode == kModeReal) mode = kModeUniform;
+    if (layout >= 2 && (int)fam.size() > kSplitMaxFam) return 1;
+    const RowTabs RT{(int)fam.size(), fam.data(), ent.data(), chx.empty() ? nullptr : chx.data(), fmk.data(),
+                     fco.data(), rt32.data(), h->ph.data()};
     int L2 = T.kind == 2 ? 1 : T.L2;
 #define HC_SHAPE(n, wd, l2)                                                                                  \
     if (T.N == n && T.w == wd && L2 == l2) {                                                                 \
-        using TC = TorusC<n, wd, l2>;                                                                        \
-        if (pipe) hv_modes<TC, true>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, L, u, v, w);               \
-        else hv_modes<TC, false>(h, T, triv, mode, fam, ent, chx, ezz0, rt32, L, u, v, w);                   \
+        hv_modes<TorusC<n, wd, l2>>(h, layout, triv, mode, RT, ezz0, L, u, v, w);                            \
         return 0;                                                                                            \
     }
     HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1) HC_SHAPE(24, 24, 1)
@@ -530,13 +558,23 @@ int hc_torus_mode(void* p) {
     bool bonds = true, real = true;
     for (const TFam& F : fam) bonds = bonds && F.type == 0;
     for (const TEntry& E : ent) real = real && E.cim == 0.0;
-    return !bonds ? kModeGeneral : (real && h->kx == 0 && h->ky == 0 ? kModeReal : kModeBonds);
+    const int mode = !bonds ? kModeGeneral : (real && h->kx == 0 && h->ky == 0 ? kModeReal : kModeBonds);
+    std::vector<uint32_t> fmk;
+    std::vector<cd2> fco;
+    return torus_fam_extras(fam, ent, fmk, fco) && mode == kModeReal ? kModeUniform : mode;
 }
 
-int hc_hv_torus_c(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, false); }
+int hc_hv_torus_c(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, 0); }
 
 // Same through the software-pipelined row (torus_row_pipelined).
-int hc_hv_torus_p(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, true); }
+int hc_hv_torus_p(void* p, const double* v, double* w) { return hv_torus_c_impl(p, v, w, 1); }
+
+// Same with every row split over `lanes` (2, 4 or 8) lanes as the split
+// kernels do (torus_row_share, tree sum).
+int hc_hv_torus_s(void* p, const double* v, double* w, int lanes) {
+    if (lanes != 2 && lanes != 4 && lanes != 8) return 1;
+    return hv_torus_c_impl(p, v, w, lanes);
+}
 
 // Compile-time-shape search (TorusC) for the shapes the tests use; returns 1
 // if the lattice has none of them.

@@ -214,6 +214,42 @@ def test_batched_compiled_row_vs_oracle(model, n, k):
     # lattices with chiral terms take the merged class (one element per pair)
     mode = Hp.hostcore().hc_torus_mode(h.h)
     assert mode == (3 if model.chiral and model.Ch else mode)
+    # rows split over 2, 4 and 8 lanes: the shares cover every element once
+    for lanes in (2, 4, 8):
+        ws = h.hv_torus_split(v, lanes)
+        assert ws is not None
+        assert np.linalg.norm(ws - wo) <= 1e-12 * np.linalg.norm(wo), lanes
+        assert np.linalg.norm(ws - w) <= 1e-14 * np.linalg.norm(w), lanes
+
+
+def _nonuniform_square():
+    m = L.square(4, 4, J2=0.5)
+    x, y, J, ph = m.bonds[3]
+    m.bonds[3] = (x, y, 0.7, ph)          # one J1 bond differs: its family is not uniform
+    return m
+
+
+UNIFORM = [(L.square(4, 4, J2=0.5), 8, (0, 0), 4), (L.chain(16), 8, (0, 0), 4), (L.square(4, 4, J2=0.5), 8, (1, 2), 1),
+           (_nonuniform_square(), 8, (0, 0), 2), (L.triangular(6, 3, J2=0.1, lam=0.7), 9, (0, 0), 4)]
+
+
+@pytest.mark.parametrize("model,n,k,mode", UNIFORM, ids=[f"{m.name}_n{n}_k{k[0]}{k[1]}_m{c}" for m, n, k, c in UNIFORM])
+def test_uniform_family_class(model, n, k, mode):
+    """Real bond families whose members share J/2 and Delta J/2 take the
+    uniform class at k = 0 (coefficient per family, diagonal from the
+    families' antiparallel counts); a family with one different J keeps the
+    per-member class.  Every layout matches the oracle."""
+    o = P.OracleBasis(model, n, *k)
+    h = Hp.HostCore(model, n, *k)
+    assert Hp.hostcore().hc_torus_mode(h.h) == mode
+    v = np.random.default_rng(21).normal(size=o.D) + 1j * np.random.default_rng(22).normal(size=o.D)
+    wo = o.hv(v)
+    w = h.hv_torus_c(v)
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+    assert np.array_equal(h.hv_torus_c(v, pipelined=True), w)
+    for lanes in (2, 4, 8):
+        ws = h.hv_torus_split(v, lanes)
+        assert np.linalg.norm(ws - wo) <= 1e-12 * np.linalg.norm(wo), lanes
 
 
 def _rows2_cases():
