@@ -822,13 +822,18 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
                 HIPCHK(scopy(op->d_mfam, mfam.data(), mfam.size() * sizeof(TFam), hipMemcpyHostToDevice));
                 HIPCHK(hipMalloc(&op->d_ment, ment.size() * sizeof(TEntry)));
                 HIPCHK(scopy(op->d_ment, ment.data(), ment.size() * sizeof(TEntry), hipMemcpyHostToDevice));
-                HIPCHK(hipMalloc(&op->d_tchx, chx.size() * sizeof(TChir)));
-                HIPCHK(scopy(op->d_tchx, chx.data(), chx.size() * sizeof(TChir), hipMemcpyHostToDevice));
                 std::vector<uint32_t> mfmk;
                 std::vector<cd2> mfco;
                 (void)torus_fam_extras(mfam, ment, mfmk, mfco);
                 HIPCHK(hipMalloc(&op->d_mfmk, mfmk.size() * sizeof(uint32_t)));
                 HIPCHK(scopy(op->d_mfmk, mfmk.data(), mfmk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+                std::vector<uint32_t> mki;
+                std::vector<cd2> mco;
+                torus_merged_tables(mfam, ment, chx, mki, mco);
+                HIPCHK(hipMalloc(&op->d_mki, mki.size() * sizeof(uint32_t)));
+                HIPCHK(scopy(op->d_mki, mki.data(), mki.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+                HIPCHK(hipMalloc(&op->d_mco, mco.size() * sizeof(cd2)));
+                HIPCHK(scopy(op->d_mco, mco.data(), mco.size() * sizeof(cd2), hipMemcpyHostToDevice));
             }
         }
     }
@@ -883,8 +888,9 @@ extern "C" void ed_op_destroy(ed_op op) {
     if (!op) return;
     for (void* p : {(void*)op->d_b, (void*)op->d_c, (void*)op->d_w, op->d_u64, op->d_u128, (void*)op->d_tfam,
                     (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
-                    (void*)op->d_ment, (void*)op->d_tchx, (void*)op->d_tfmk, (void*)op->d_tfco,
-                    (void*)op->d_mfmk, op->d_img64, op->d_img128, op->d_wimg64, op->d_wimg128})
+                    (void*)op->d_ment, (void*)op->d_tfmk, (void*)op->d_tfco,
+                    (void*)op->d_mfmk, (void*)op->d_mki, (void*)op->d_mco, op->d_img64, op->d_img128, op->d_wimg64,
+                    op->d_wimg128})
         if (p) dfree(p);
     delete op;
 }
@@ -1190,35 +1196,50 @@ extern "C" int ed_op_count_targets(ed_op op, int64_t* distinct, int64_t* total) 
     return ED_OK;
 }
 
-extern "C" int ed_make_h(ed_op op, double* H) {
-    if (!op || !H) return ed_fail(ED_EINVAL, "null argument");
+// The sector matrix, column-major D x D complex128, into dH on the setup
+// stream (k_make_h, one row per lane).
+static int fill_dense_h(ed_op_s* op, cd2* dH) {
     ed_basis_s* B = op->b;
     const int64_t D = B->D;
-    if (D > 20000) return ed_fail(ED_EINVAL, "dense H requested for D=%lld > 20000", (long long)D);
-    if (D == 0) return ED_OK;
-    cd2* dH = nullptr;
-    HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
     HIPCHK(szero(dH, D * D * sizeof(cd2)));
     Geo G;
     basis_geo(B, G);
     TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
     TermsView T = op_terms(op);
     dim3 grid(nblocks(D, 64)), blk(64);
+    hipStream_t st = setup_stream();
     if (B->N <= 32) {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     } else {
         if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
         else
-            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
+            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, st, D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(scopy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
-    dfree(dH);
     return ED_OK;
+}
+
+extern "C" int ed_make_h(ed_op op, double* H) {
+    if (!op || !H) return ed_fail(ED_EINVAL, "null argument");
+    const int64_t D = op->b->D;
+    if (D > 20000) return ed_fail(ED_EINVAL, "dense H requested for D=%lld > 20000", (long long)D);
+    if (D == 0) return ED_OK;
+    cd2* dH = nullptr;
+    // every exit after the allocation frees it
+    auto body = [&]() -> int {
+        HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
+        const int frc = fill_dense_h(op, dH);
+        if (frc != ED_OK) return frc;
+        HIPCHK(scopy(H, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+        return ED_OK;
+    };
+    const int rc = body();
+    dfree(dH);
+    return rc;
 }
 
 // Dense Hermitian eigensolve of the sector matrix on the GPU (rocSOLVER
@@ -1226,52 +1247,42 @@ extern "C" int ed_make_h(ed_op op, double* H) {
 // the reference runs through LAPACK zheev (Matrix_Functions.cpp:28-48).
 extern "C" int ed_dense_eigh(ed_op op, double* evals, double* evecs) {
     if (!op || !evals) return ed_fail(ED_EINVAL, "null argument");
-    ed_basis_s* B = op->b;
-    const int64_t D = B->D;
+    const int64_t D = op->b->D;
     if (D == 0) return ed_fail(ED_EEMPTY, "No states in psector");
     if (D > 20000) return ed_fail(ED_EINVAL, "dense eigensolve requested for D=%lld > 20000", (long long)D);
     cd2* dH = nullptr;
     double *dW = nullptr, *dE = nullptr;
     int* dinfo = nullptr;
-    HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
-    HIPCHK(hipMalloc(&dW, D * sizeof(double)));
-    HIPCHK(hipMalloc(&dE, D * sizeof(double)));
-    HIPCHK(hipMalloc(&dinfo, sizeof(int)));
-    HIPCHK(szero(dH, D * D * sizeof(cd2)));
-    Geo G;
-    basis_geo(B, G);
-    TablesView tv{B->d_rtab, B->d_dir, B->d_phase};
-    TermsView T = op_terms(op);
-    dim3 grid(nblocks(D, 64)), blk(64);
-    if (B->N <= 32) {
-        if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint32_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
-        else
-            hipLaunchKernelGGL((k_make_h<uint32_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
-    } else {
-        if (B->trivial_k)
-            hipLaunchKernelGGL((k_make_h<uint64_t, true>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
-        else
-            hipLaunchKernelGGL((k_make_h<uint64_t, false>), grid, blk, 0, setup_stream(), D, B->d_rows, G, B->rinfo, tv, T, B->d_ntab, B->d_nitab, dH);
-    }
-    HIPCHK(hipGetLastError());
     rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
-    // the solver follows the matrix kernels on the setup stream
-    if (rocblas_set_stream(h, setup_stream()) != rocblas_status_success) {
-        rocblas_destroy_handle(h);
-        return ed_fail(ED_EDEVICE, "rocblas_set_stream failed");
-    }
-    // column-major H, upper triangle referenced (the matrix is Hermitian)
-    rocblas_status rs = rocsolver_zheevd(h, evecs ? rocblas_evect_original : rocblas_evect_none, rocblas_fill_upper,
-                                         (rocblas_int)D, (rocblas_double_complex*)dH, (rocblas_int)D, dW, dE, dinfo);
-    int info = 0;
-    hipError_t e1 = scopy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
-    rocblas_destroy_handle(h);
-    if (rs != rocblas_status_success || e1 != hipSuccess) return ed_fail(ED_EDEVICE, "rocsolver_zheevd failed (%d)", (int)rs);
-    if (info != 0) return ed_fail(ED_ENOTCONV, "zheevd info=%d", info);
-    HIPCHK(scopy(evals, dW, D * sizeof(double), hipMemcpyDeviceToHost));
-    if (evecs) HIPCHK(scopy(evecs, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+    // every exit frees the buffers and the handle
+    auto body = [&]() -> int {
+        HIPCHK(hipMalloc(&dH, D * D * sizeof(cd2)));
+        HIPCHK(hipMalloc(&dW, D * sizeof(double)));
+        HIPCHK(hipMalloc(&dE, D * sizeof(double)));
+        HIPCHK(hipMalloc(&dinfo, sizeof(int)));
+        const int frc = fill_dense_h(op, dH);
+        if (frc != ED_OK) return frc;
+        if (rocblas_create_handle(&h) != rocblas_status_success) {
+            h = nullptr;
+            return ed_fail(ED_EDEVICE, "rocblas_create_handle failed");
+        }
+        // the solver follows the matrix kernels on the setup stream
+        if (rocblas_set_stream(h, setup_stream()) != rocblas_status_success)
+            return ed_fail(ED_EDEVICE, "rocblas_set_stream failed");
+        // column-major H, upper triangle referenced (the matrix is Hermitian)
+        const rocblas_status rs = rocsolver_zheevd(h, evecs ? rocblas_evect_original : rocblas_evect_none,
+                                                   rocblas_fill_upper, (rocblas_int)D, (rocblas_double_complex*)dH,
+                                                   (rocblas_int)D, dW, dE, dinfo);
+        if (rs != rocblas_status_success) return ed_fail(ED_EDEVICE, "rocsolver_zheevd failed (%d)", (int)rs);
+        int info = 0;
+        HIPCHK(scopy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost));
+        if (info != 0) return ed_fail(ED_ENOTCONV, "zheevd info=%d", info);
+        HIPCHK(scopy(evals, dW, D * sizeof(double), hipMemcpyDeviceToHost));
+        if (evecs) HIPCHK(scopy(evecs, dH, D * D * sizeof(cd2), hipMemcpyDeviceToHost));
+        return ED_OK;
+    };
+    const int rc = body();
+    if (h) rocblas_destroy_handle(h);
     dfree(dH); dfree(dW); dfree(dE); dfree(dinfo);
-    return ED_OK;
+    return rc;
 }

@@ -57,29 +57,34 @@ __global__ void __launch_bounds__(512) k_hv_torus(int64_t D, const uint64_t* __r
     w[i] = o;
 }
 
-// LDS of the compiled-shape kernels, in this order: [TChir x nfam*32
-// (kModeMerged)] [TEntry x nfam*32 (not kModeUniform)] [TFam x nfam, padded
-// to even] [cd2 x nfam (kModeUniform: J/2, Delta J/2)] [cd2 phase x nphase]
-// [u32 flip mask x nfam*32] [u32 rank tables x rtab_n] [u16 chunk map x nmap].
-// Stages the tables with the whole block and returns them as RowTabs.
+// LDS of the compiled-shape kernels, in this order: [cd2 x nfam*32*4
+// (kModeMerged: coefficients)] [TEntry x nfam*32 (not kModeUniform)] [TFam x
+// nfam, padded to even] [cd2 x nfam (kModeUniform: J/2, Delta J/2)] [cd2 phase
+// x nphase] [u32 flip mask x nfam*32] [u32 z-sites x nfam*32 (kModeMerged)]
+// [u32 rank tables x rtab_n] [u16 chunk map x nmap].  Stages the tables with
+// the whole block and returns them as RowTabs.
 template <int MODE>
 __device__ __forceinline__ RowTabs stage_row_tabs(int nfam, const TFam* __restrict__ gfam,
-                                                  const TEntry* __restrict__ gent, const TChir* __restrict__ gchx,
-                                                  const uint32_t* __restrict__ gfmk, const cd2* __restrict__ gfco,
-                                                  const uint32_t* __restrict__ grtab, int rtab_n,
-                                                  const uint16_t* __restrict__ gmap, int nmap,
+                                                  const TEntry* __restrict__ gent, const uint32_t* __restrict__ gmki,
+                                                  const cd2* __restrict__ gmco, const uint32_t* __restrict__ gfmk,
+                                                  const cd2* __restrict__ gfco, const uint32_t* __restrict__ grtab,
+                                                  int rtab_n, const uint16_t* __restrict__ gmap, int nmap,
                                                   const cd2* __restrict__ gphase, int nphase, uint16_t*& map) {
     extern __shared__ __align__(16) unsigned char smem[];
-    TChir* chx = (TChir*)smem;
-    TEntry* ent = (TEntry*)(chx + (MODE == kModeMerged ? nfam * 32 : 0));
+    constexpr bool MG = MODE == kModeMerged;
+    cd2* mco = (cd2*)smem;
+    TEntry* ent = (TEntry*)(mco + (MG ? nfam * 32 * 4 : 0));
     TFam* fam = (TFam*)(ent + (MODE == kModeUniform ? 0 : nfam * 32));
     cd2* fco = (cd2*)(fam + nfam + (nfam & 1));
     cd2* phase = fco + (MODE == kModeUniform ? nfam : 0);
     uint32_t* fmk = (uint32_t*)(phase + nphase);
-    uint32_t* rtab = fmk + nfam * 32;
+    uint32_t* mki = fmk + nfam * 32;
+    uint32_t* rtab = mki + (MG ? nfam * 32 : 0);
     map = (uint16_t*)(rtab + rtab_n);
-    if (MODE == kModeMerged)
-        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) chx[q] = gchx[q];
+    if (MG) {
+        for (int q = threadIdx.x; q < nfam * 32 * 4; q += blockDim.x) mco[q] = gmco[q];
+        for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) mki[q] = gmki[q];
+    }
     if (MODE != kModeUniform)
         for (int q = threadIdx.x; q < nfam * 32; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) fam[q] = gfam[q];
@@ -90,7 +95,7 @@ __device__ __forceinline__ RowTabs stage_row_tabs(int nfam, const TFam* __restri
     for (int q = threadIdx.x; q < rtab_n; q += blockDim.x) rtab[q] = grtab[q];
     for (int q = threadIdx.x; q < nmap; q += blockDim.x) map[q] = gmap[q];
     __syncthreads();
-    return RowTabs{nfam, fam, ent, chx, fmk, fco, rtab, phase};
+    return RowTabs{nfam, fam, ent, mki, mco, fmk, fco, rtab, phase};
 }
 
 // Row body of the compiled-shape kernels: the lattice shape as compile-time
@@ -100,15 +105,15 @@ __device__ __forceinline__ RowTabs stage_row_tabs(int nfam, const TFam* __restri
 template <class TC, bool TRIV, int MODE, int K, bool PIPE, typename VT>
 __device__ __forceinline__ void hv_torus_c_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                 const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
-                                                const TChir* __restrict__ gchx, const uint32_t* __restrict__ gfmk,
-                                                const cd2* __restrict__ gfco,
+                                                const uint32_t* __restrict__ gmki, const cd2* __restrict__ gmco,
+                                                const uint32_t* __restrict__ gfmk, const cd2* __restrict__ gfco,
                                                 double ezz0, const RankInfo& R, const uint32_t* __restrict__ grtab,
                                                 int rtab_n, const uint16_t* __restrict__ gmap, int nmap,
                                                 int cb, uint32_t zidx, const cd2* __restrict__ gphase, int nphase,
                                                 const double* __restrict__ ntab, const VT* __restrict__ u,
                                                 const VT* __restrict__ v, VT* __restrict__ w) {
     uint16_t* map;
-    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gchx, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
+    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gmki, gmco, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
                                            nphase, map);
     // XCD tiles of consecutive rows, the 8 XCDs on adjacent tiles
     const unsigned lb = xcd_tile_block();
@@ -157,7 +162,7 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
 template <class TC, bool TRIV, int MODE, int K, int LPRL, typename VT>
 __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                     const TFam* __restrict__ gfam, const TEntry* __restrict__ gent,
-                                                    const TChir* __restrict__ gchx,
+                                                    const uint32_t* __restrict__ gmki, const cd2* __restrict__ gmco,
                                                     const uint32_t* __restrict__ gfmk, const cd2* __restrict__ gfco,
                                                     double ezz0, const RankInfo& R,
                                                     const uint32_t* __restrict__ grtab, int rtab_n,
@@ -167,7 +172,7 @@ __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* _
                                                     const VT* __restrict__ v, VT* __restrict__ w) {
     constexpr int LPR = 1 << LPRL;
     uint16_t* map;
-    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gchx, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
+    const RowTabs T = stage_row_tabs<MODE>(nfam, gfam, gent, gmki, gmco, gfmk, gfco, grtab, rtab_n, gmap, nmap, gphase,
                                            nphase, map);
     const unsigned rpb = blockDim.x >> LPRL;
     const unsigned lb = xcd_tile_block_rows(rpb);
@@ -214,14 +219,15 @@ __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* _
 
 #define ED_TORUS_C_ARGS                                                                                   \
     int64_t D, const uint64_t *__restrict__ rows, int nfam, const TFam *__restrict__ gfam,               \
-        const TEntry *__restrict__ gent, const TChir *__restrict__ gchx, const uint32_t *__restrict__ gfmk, \
+        const TEntry *__restrict__ gent, const uint32_t *__restrict__ gmki, const cd2 *__restrict__ gmco,  \
+        const uint32_t *__restrict__ gfmk,                                                                 \
         const cd2 *__restrict__ gfco, double ezz0, RankInfo R,                                             \
         const uint32_t *__restrict__ grtab,                                                               \
         int rtab_n, const uint16_t *__restrict__ gmap, int nmap, int cb, uint32_t zidx,                   \
         const cd2 *__restrict__ gphase, int nphase, const double *__restrict__ ntab, const VT *__restrict__ u, \
         const VT *__restrict__ v, VT *__restrict__ w
 #define ED_TORUS_C_PASS \
-    D, rows, nfam, gfam, gent, gchx, gfmk, gfco, ezz0, R, grtab, rtab_n, gmap, nmap, cb, zidx, gphase, nphase, ntab, u, v, w
+    D, rows, nfam, gfam, gent, gmki, gmco, gfmk, gfco, ezz0, R, grtab, rtab_n, gmap, nmap, cb, zidx, gphase, nphase, ntab, u, v, w
 
 // Variant A: four elements in flight, occupancy left to the register count.
 template <class TC, bool TRIV, int MODE, typename VT>
@@ -274,7 +280,8 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
     const bool mg = op->tmode == kModeMerged;
 #define ED_GO(KER)                                                                                            \
     hipLaunchKernelGGL(KER, grid, blk, sm, st, D, B->d_rows, mg ? op->nmfam : op->nfam, mg ? op->d_mfam : op->d_tfam, \
-                       mg ? op->d_ment : op->d_tent, op->d_tchx, mg ? op->d_mfmk : op->d_tfmk, op->d_tfco, op->ezz0,  \
+                       mg ? op->d_ment : op->d_tent, op->d_mki, op->d_mco, mg ? op->d_mfmk : op->d_tfmk, op->d_tfco,  \
+                       op->ezz0,                                                                                  \
                        B->rinfo, B->d_rtab32, B->rtab32_n, B->d_img_map, (int)B->img_nmap, B->img_cb, zidx,       \
                        B->d_phase, nph, B->d_ntab, u, v, w)
     // term class: k = 0 takes the general, real-bond or uniform-family kernel,
@@ -347,9 +354,9 @@ static size_t torus_c_smem_bytes(const ed_op_s* op) {
     const ed_basis_s* B = op->b;
     const bool mg = op->tmode == kModeMerged, un = op->tmode == kModeUniform;
     const int nf = mg ? op->nmfam : op->nfam;
-    return (mg ? (size_t)nf * 32 * sizeof(TChir) : 0) + (un ? 0 : (size_t)nf * 32 * sizeof(TEntry)) +
+    return (mg ? (size_t)nf * 32 * 4 * sizeof(cd2) : 0) + (un ? 0 : (size_t)nf * 32 * sizeof(TEntry)) +
            (size_t)(nf + (nf & 1)) * sizeof(TFam) + (un ? (size_t)nf * sizeof(cd2) : 0) +
-           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)nf * 32 * sizeof(uint32_t) +
+           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)nf * 32 * sizeof(uint32_t) * (mg ? 2 : 1) +
            (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)B->img_nmap * sizeof(uint16_t);
 }
 

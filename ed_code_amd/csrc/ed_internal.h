@@ -128,8 +128,9 @@ struct ed_op_s {
     int nmfam = 0;
     edc::TFam* d_mfam = nullptr;
     edc::TEntry* d_ment = nullptr;
-    edc::TChir* d_tchx = nullptr;
     uint32_t* d_mfmk = nullptr;   // flip mask per merged family member [nmfam * 32]
+    uint32_t* d_mki = nullptr;    // z-sites of each merged member's chiral entries [nmfam * 32]
+    edc::cd2* d_mco = nullptr;    // merged member coefficients by z-site bits [nmfam * 32 * 4]
     // two-row 64-bit rotation path (ed_hv64.hip)
     bool use_rows2 = false;
     int nfam64 = 0, nterm64 = 0;

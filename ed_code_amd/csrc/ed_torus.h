@@ -430,7 +430,8 @@ struct RowTabs {
     int nfam;
     const TFam* fam;
     const TEntry* ent;     // per member, [f * 32 + x] (not read under kModeUniform)
-    const TChir* chx;      // kModeMerged: chiral entries of each member's pair
+    const uint32_t* mki;   // kModeMerged: z-sites k0 | k1 << 8 of each member's chiral entries
+    const cd2* mco;        // kModeMerged: coefficient per member at s_x = 0 and (s_k0, s_k1), [(f * 32 + x) * 4 + idx]
     const uint32_t* fmk;   // per member, [f * 32 + x]: the flip mask (1 << x) | (1 << y)
     const cd2* fco;        // kModeUniform: {J/2, Delta J/2} per family
     const uint32_t* rtab;  // 32-bit byte-sliced rank tables
@@ -487,23 +488,22 @@ ED_HD void fill_batch(uint32_t s, const RowTabs& T, const RankInfo& R, RowCur& c
             double cr, ci = 0.0;
             if (MODE == kModeUniform) {
                 cr = c.cu;
+            } else if (MODE == kModeMerged) {
+                // the bond and chiral terms of the pair, from the member's
+                // table of its z-site bits, conjugated for s_x = 1
+                // (torus_merged_tables)
+                const int m = c.f * 32 + x;
+                const uint32_t ki = T.mki[m];
+                const uint32_t idx = ((s >> (ki & 31u)) & 1u) | (((s >> (ki >> 8)) & 1u) << 1);
+                const cd2 co = T.mco[m * 4 + (int)idx];
+                ezz -= T.ent[m].dJ2;
+                cr = co.re;
+                ci = ((s >> x) & 1u) ? -co.im : co.im;
             } else if (MODE != kModeGeneral || T.fam[c.f].type == 0) {
                 const TEntry& E = T.ent[c.f * 32 + x];
                 ezz -= E.dJ2;
                 cr = E.cre;
                 if (MODE != kModeReal) ci = ((s >> x) & 1u) ? -E.cim : E.cim;
-                if (MODE == kModeMerged) {
-                    // the chiral entries of the pair, in a fixed order
-                    const TChir& X = T.chx[c.f * 32 + x];
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        if (!((X.flags >> (2 * h)) & 1)) continue;
-                        const double sg = ((s >> X.k[h]) & 1u) ? 1.0 : -1.0;
-                        const uint32_t jbit = ((X.flags >> (2 * h + 1)) & 1) ? (s >> x) & 1u : (s >> E.y) & 1u;
-                        cr += -X.cim[h] * sg;
-                        ci += (jbit ? X.cre[h] : -X.cre[h]) * sg;
-                    }
-                }
             } else {
                 const TEntry& E = T.ent[c.f * 32 + x];
                 double sg = ((s >> E.k) & 1u) ? 1.0 : -1.0;

@@ -122,6 +122,43 @@ inline bool torus_fam_extras(const std::vector<TFam>& fam, const std::vector<TEn
     return uniform;
 }
 
+// kModeMerged coefficient tables: a member (f, x) flipping the pair (x, y)
+// carries its bond term and up to two chiral entries with z-sites k0, k1
+// (TChir), so its coefficient is a function of the bits s_x, s_k0, s_k1 alone
+// (s_y = 1 - s_x for an antiparallel pair).  Flipping s_x conjugates it: the
+// real parts (J/2 cos phi, -cim sg) do not depend on s_x and every imaginary
+// term changes sign, exactly in IEEE arithmetic.  mco[(f * 32 + x) * 4 + idx],
+// idx = s_k0 | s_k1 << 1, holds the coefficient at s_x = 0, computed with the
+// operations and order of the per-element formula (the bond part, then each
+// chiral entry's -cim sg and +-cre sg); mki[f * 32 + x] = k0 | k1 << 8, an
+// absent entry's z-site set to x.
+inline void torus_merged_tables(const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
+                                const std::vector<TChir>& chx, std::vector<uint32_t>& mki, std::vector<cd2>& mco) {
+    mki.assign(kMaxFam * 32, 0u);
+    mco.assign((size_t)kMaxFam * 32 * 4, cd2{0.0, 0.0});
+    for (size_t f = 0; f < fam.size(); f++)
+        for (int x = 0; x < 32; x++) {
+            if (!((fam[f].S >> x) & 1u)) continue;
+            const size_t m = f * 32 + x;
+            const TEntry& E = ent[m];
+            const TChir& X = chx[m];
+            const int k0 = (X.flags & 1) ? X.k[0] : x, k1 = (X.flags & 4) ? X.k[1] : x;
+            mki[m] = (uint32_t)k0 | ((uint32_t)k1 << 8);
+            for (int idx = 0; idx < 4; idx++) {
+                const int sx = 0, sk[2] = {idx & 1, (idx >> 1) & 1};
+                double cr = E.cre, ci = sx ? -E.cim : E.cim;
+                for (int h = 0; h < 2; h++) {
+                    if (!((X.flags >> (2 * h)) & 1)) continue;
+                    const double sg = sk[h] ? 1.0 : -1.0;
+                    const int jbit = ((X.flags >> (2 * h + 1)) & 1) ? sx : 1 - sx;
+                    cr += -X.cim[h] * sg;
+                    ci += (jbit ? X.cre[h] : -X.cre[h]) * sg;
+                }
+                mco[m * 4 + idx] = cd2{cr, ci};
+            }
+        }
+}
+
 // Families for kModeMerged: keyed by the bonds (pair x -> y = h(x)); each
 // chiral entry (i, j, k) joins the member whose pair is {i, j} (listed either
 // way), or opens a member with a zero bond part if no bond flips that pair.

@@ -524,8 +524,11 @@ static int hv_torus_c_impl(void* p, const double* v_, double* w_, int layout) {
     std::vector<cd2> fco;
     if (torus_fam_extras(fam, ent, fmk, fco) && mode == kModeReal) mode = kModeUniform;
     if (layout >= 2 && (int)fam.size() > kSplitMaxFam) return 1;
-    const RowTabs RT{(int)fam.size(), fam.data(), ent.data(), chx.empty() ? nullptr : chx.data(), fmk.data(),
-                     fco.data(), rt32.data(), h->ph.data()};
+    std::vector<uint32_t> mki;
+    std::vector<cd2> mco;
+    if (mode == kModeMerged) torus_merged_tables(fam, ent, chx, mki, mco);
+    const RowTabs RT{(int)fam.size(), fam.data(), ent.data(), mki.empty() ? nullptr : mki.data(),
+                     mco.empty() ? nullptr : mco.data(), fmk.data(), fco.data(), rt32.data(), h->ph.data()};
     int L2 = T.kind == 2 ? 1 : T.L2;
 #define HC_SHAPE(n, wd, l2)                                                                                  \
     if (T.N == n && T.w == wd && L2 == l2) {                                                                 \
