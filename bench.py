@@ -196,6 +196,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--dtype", default="c64", choices=["c64", "c128"])
     ap.add_argument("--workload", default="sq4x8", choices=["sq4x8", "chain24", "tri6x5", "tri3x9"])
+    ap.add_argument("--k", default="0,0", help="momentum indices kx,ky of the sector (profiling other sectors)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c128", action="store_true")
@@ -255,7 +256,7 @@ def main():
     else:
         model, couplings = L.triangular(3, 9, Ch=0.15), "J1=1 Delta=1 Jchi=0.15"
     n_ones = L.n_ones_for_sz(model.n_sites, 0.0)
-    kx = ky = 0
+    kx, ky = (int(x) for x in args.k.split(","))
     t0 = time.time()
     lat = ed.Lattice.of(model)
     basis = ed.Basis(lat, n_ones, kx, ky)
@@ -422,7 +423,7 @@ def main():
         "vs_baseline": None,
         "dtype": main_dt,
         "data": "synthetic (seeded complex normal v; lattice and couplings fixed)",
-        "config": {"workload": f"{model.name} {couplings} Sz=0 k=(0,0) single H·v", "n_sites": model.n_sites,
+        "config": {"workload": f"{model.name} {couplings} Sz=0 k=({kx},{ky}) single H·v", "n_sites": model.n_sites,
                    "n_up": n_ones, "D": D, "distinct_targets": G, "vector": main_dt},
         "roofline": {"bound": "hbm", "achieved": r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r["achieved_gbs"] / HBM_PEAK_GBS,

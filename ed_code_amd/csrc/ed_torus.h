@@ -276,6 +276,29 @@ ED_HD uint32_t rot_top(uint32_t hi, uint32_t lo, int q) {
     return q == 0 ? hi : alignbit32(hi, lo, (uint32_t)(32 - q));
 }
 
+// Whether two top-aligned candidates (rot_top) hold the same state.  A
+// candidate's low 32 - N bits continue the doubled word: they repeat the
+// top bits of the same rotated state when the rotation is at most 2N - 32
+// (the window stays inside the two copies of the state), so then equal
+// states are equal words; larger rotations of the shape reach into the
+// zero padding and only the top N bits are compared.
+template <class TC>
+ED_HD bool top_eq(uint32_t a, uint32_t b) {
+    constexpr int S = 32 - TC::N;
+    constexpr int qmax = TC::ring ? TC::N - 1 : TC::N - TC::W;
+    if (TC::N == 32 || qmax <= 2 * TC::N - 32) return a == b;
+    return (a >> S) == (b >> S);
+}
+
+// The N-bit state rotated left by q bits (0 <= q < N, run-time q).
+template <class TC>
+ED_HD uint32_t rot_rt(uint32_t s, uint32_t q) {
+    uint32_t hi, lo;
+    cat_c<TC>(s, hi, lo);
+    if (TC::N == 32) return alignbit32(lo, lo, (32u - q) & 31u);
+    return alignbit32(hi, lo, (uint32_t)TC::N - q) & TC::nmask;
+}
+
 // torus_rep for a compile-time shape; same result and phase code.  The
 // candidates are compared top-aligned (cat_top), one instruction each.
 template <class TC, bool TRACK>
@@ -295,7 +318,7 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
         if (TRACK) {
             int bx = 0;
             for (int a = TC::L1 - 1; a >= 0; a--)
-                if ((rot_top<TC>(hi, lo, a) >> S) == rep) bx = a;
+                if (top_eq<TC>(rot_top<TC>(hi, lo, a), best)) bx = a;
             code = (TC::L1 - bx) % TC::L1;
         } else {
             code = 0;
@@ -328,17 +351,18 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
         uint32_t sa = fr[0];
 #pragma unroll
         for (int a = TC::L1 - 1; a >= 0; a--)
-            if ((mins[a] >> S) == rep) { bx = a; sa = fr[a]; }
-        uint32_t hi, lo;
-        cat_top<TC>(sa, hi, lo);
+            if (top_eq<TC>(mins[a], best)) { bx = a; sa = fr[a]; }
         const int bxm = bx % TC::L2;
-        int by = TC::L2;
+        // step y is the word rotation b = (y - bxm) mod L2: with sa rotated
+        // back by bxm rows once, step y is the rotation by y rows, a
+        // compile-time amount
+        const uint32_t sw = rot_rt<TC>(sa, (uint32_t)((bxm == 0 ? 0 : TC::L2 - bxm) * TC::W));
+        uint32_t hi, lo;
+        cat_top<TC>(sw, hi, lo);
+        int by = 0;
 #pragma unroll
-        for (int b = 0; b < TC::L2; b++) {
-            int y = b + bxm;
-            y -= y >= TC::L2 ? TC::L2 : 0;
-            if ((rot_top<TC>(hi, lo, b * TC::W) >> S) == rep && y < by) by = y;
-        }
+        for (int y = TC::L2 - 1; y >= 0; y--)
+            if (top_eq<TC>(rot_top<TC>(hi, lo, y * TC::W), best)) by = y;
         // lx = -bx mod L1; ly = -(bx (L2 - 1) + by) mod L2 = (bx - by) mod L2
         const int lx = bx == 0 ? 0 : TC::L1 - bx;
         const int d = bxm - by;
