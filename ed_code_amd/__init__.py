@@ -423,9 +423,12 @@ class BlockOperator:
         check(lib().ed_block_hv(self.h, dt, C.c_void_p(V.data_ptr()), C.c_void_p(W.data_ptr()), C.c_void_p(st)))
         return W
 
-    def lanczos(self, nev=1, iterations=200, tol=1e-10, seed=1234, precision=ED_C64):
-        """Lowest nev eigenvalues of every instance: array (M, nev), stats."""
-        p = LanczosParams(nev, iterations, 1, precision, ED_LZ_FULL, 0, tol, seed)
+    def lanczos(self, nev=1, iterations=200, tol=1e-10, seed=1234, precision=ED_C64, mode=ED_LZ_FULL, ncycles=1):
+        """Lowest nev eigenvalues of every instance: array (M, nev), stats.
+        mode ED_LZ_FULL: one run per eigenvalue with deflation (iterations
+        steps each); ED_LZ_THICK: thick restart with iterations Krylov
+        vectors and at most ncycles restarts."""
+        p = LanczosParams(nev, iterations, ncycles, precision, mode, 0, tol, seed)
         n = min(nev, self.D)
         ev = np.zeros((self.M, n))
         st = LanczosStats()

@@ -1434,6 +1434,166 @@ int run_block_full(BCtx& c, const ed_lanczos_params& p, double tol, const std::v
     return ED_OK;
 }
 
+// Thick-restart Lanczos (run_trl) for the M instances of a block operator in
+// lock step: one block H*V per expansion step for all instances, two
+// classical Gram-Schmidt passes against the instance's kept and new vectors
+// (per-instance coefficients), the Rayleigh-Ritz of each instance's
+// arrowhead-bordered tridiagonal, restart of every instance from its
+// nev + (m - nev)/2 lowest Ritz vectors and its normalised residual, until
+// every instance's nev lowest pairs have residual estimate
+// beta_m |y_m,i| <= tol max(1, |theta_i|) or ncycles cycles have run.  An
+// instance whose Krylov space becomes invariant (beta below the rounding
+// level) keeps its projected matrix at that size: its later block columns
+// are zero and add nothing.  With starts, instance m's start vector is the
+// sum of the start vectors' columns m (the complex128 polish of complex64
+// Ritz vectors); otherwise a seeded random vector.  Ritz vectors of the nev
+// lowest pairs go to prev (interleaved, unit per instance), their values to
+// evals[m * nev + i].
+template <typename VT>
+int run_block_trl(BCtx& c, const ed_lanczos_params& p, double tol, const std::vector<cd2*>* starts,
+                  std::vector<cd2*>& prev, double* evals) {
+    const int64_t D = c.D, n = c.n;
+    const int M = c.M;
+    const int nev = (int)std::min<int64_t>(p.nev, D);
+    const int m = (int)std::min<int64_t>(p.iterations, D);
+    if (m < nev + 1 && m < D) return ed_fail(ED_EINVAL, "thick restart needs iterations > nev (%d <= %d)", m, nev);
+    if (m > c.cap) return ed_fail(ED_EINVAL, "too many Krylov vectors (%d)", m);
+    const int kk = std::max(nev, std::min(nev + (m - nev) / 2, m - 1));
+    VT *V = nullptr, *w = nullptr;
+    cd2* W = nullptr;
+    auto cleanup = [&]() { dfree(V); dfree(w); dfree(W); };
+#define TRC(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != ED_OK) {        \
+            cleanup();             \
+            return rc_;            \
+        }                          \
+    } while (0)
+#define THIP(expr) TRC((expr) == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "%s failed", #expr))
+    THIP(hipMalloc(&V, (size_t)(m + 1) * n * sizeof(VT)));
+    THIP(hipMalloc(&w, n * sizeof(VT)));
+    THIP(hipMalloc(&W, (size_t)std::max(kk, nev) * n * sizeof(cd2)));
+    auto Vk = [&](int k) { return V + (int64_t)k * n; };
+    std::vector<std::complex<double>> q, q2;
+    std::vector<double> nn, sc(M);
+    // start vectors
+    if (starts && !starts->empty()) {
+        hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(c.g), dim3(kRedThreads), 0, c.st, n, 1.0, (*starts)[0], Vk(0));
+        THIP(hipGetLastError());
+        std::vector<std::complex<double>> minus1(M, std::complex<double>(-1.0, 0.0));
+        for (size_t i = 1; i < starts->size(); i++) TRC((baxpy<cd2, VT>(c, 1, (*starts)[i], 0, minus1, nullptr, Vk(0))));
+    } else {
+        hipLaunchKernelGGL(k_bstart<VT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.st, D, M, p.seed, Vk(0));
+        THIP(hipGetLastError());
+    }
+    TRC(bnorm2(c, Vk(0), nn));
+    for (int i = 0; i < M; i++) sc[i] = nn[i] > 0.0 ? 1.0 / std::sqrt(nn[i]) : 0.0;
+    TRC((bscale<VT, VT>(c, sc, Vk(0), Vk(0))));
+    // per instance: projected matrix (m x m, row-major), its valid size, whether
+    // steps still extend it, the residual norm after the last step
+    std::vector<std::vector<double>> T(M, std::vector<double>((size_t)m * m, 0.0));
+    std::vector<int> meff(M, 0);
+    std::vector<char> alive(M, 1);
+    std::vector<double> beta_m(M, 0.0);
+    std::vector<std::vector<double>> theta(M), Y(M);
+    int start = 0, ncyc = 0;
+    while (true) {
+        for (int i = 0; i < M; i++)
+            if (alive[i]) meff[i] = m;
+        for (int j = start; j < m; j++) {
+            TRC(bhv<VT>(c, Vk(j), w));
+            TRC((bdot<VT, VT>(c, j + 1, V, n, w, q)));
+            TRC((baxpy<VT, VT>(c, j + 1, V, n, q, nullptr, w)));
+            TRC((bdot<VT, VT>(c, j + 1, V, n, w, q2)));
+            TRC((baxpy<VT, VT>(c, j + 1, V, n, q2, nullptr, w)));
+            TRC(bnorm2(c, w, nn));
+            for (int i = 0; i < M; i++) {
+                sc[i] = 0.0;
+                if (!alive[i]) continue;
+                const double a = q[(size_t)j * M + i].real() + q2[(size_t)j * M + i].real();
+                T[i][(size_t)j * m + j] = a;
+                const double beta = std::sqrt(std::max(nn[i], 0.0));
+                if (beta < 1e-13 * std::max(1.0, std::fabs(a))) {   // invariant subspace: the space is exact
+                    meff[i] = j + 1;
+                    beta_m[i] = 0.0;
+                    alive[i] = 0;
+                    continue;
+                }
+                sc[i] = 1.0 / beta;
+                if (j + 1 < m) T[i][(size_t)j * m + j + 1] = T[i][(size_t)(j + 1) * m + j] = beta;
+                else beta_m[i] = beta;   // V[m] holds the normalised residual
+            }
+            TRC((bscale<VT, VT>(c, sc, w, Vk(j + 1))));
+        }
+        // Rayleigh-Ritz per instance and the stopping rule
+        bool conv = true;
+        for (int i = 0; i < M; i++) {
+            const int me = meff[i];
+            std::vector<double> Tm((size_t)me * me);
+            for (int a = 0; a < me; a++)
+                for (int b = 0; b < me; b++) Tm[(size_t)a * me + b] = T[i][(size_t)a * m + b];
+            if (!symeig_jacobi(Tm, me, theta[i], Y[i])) TRC(ed_fail(ED_ENOTCONV, "Rayleigh-Ritz eigensolver did not converge"));
+            for (int k = 0; k < std::min(nev, me); k++)
+                if (beta_m[i] * std::fabs(Y[i][(me - 1) + (size_t)k * me]) > tol * std::max(1.0, std::fabs(theta[i][k])))
+                    conv = false;
+        }
+        ncyc++;
+        if (conv || ncyc >= p.ncycles) break;
+        // restart: W_k = sum_l Y[l, k] V_l per instance (k < kk; zero past an
+        // exhausted instance's size), then V_k <- W_k and V_kk <- V_m
+        const int kmax = *std::max_element(meff.begin(), meff.end());
+        std::vector<double> t((size_t)kmax * M);
+        for (int k = 0; k < kk; k++) {
+            std::fill(t.begin(), t.end(), 0.0);
+            for (int i = 0; i < M; i++)
+                if (k < meff[i])
+                    for (int l = 0; l < meff[i]; l++) t[(size_t)l * M + i] = Y[i][l + (size_t)k * meff[i]];
+            cd2* Wk = W + (int64_t)k * n;
+            THIP(hipMemsetAsync(Wk, 0, n * sizeof(cd2), c.st));
+            TRC(baccumulate<VT>(c, kmax, V, n, t, Wk));
+        }
+        for (int k = 0; k < kk; k++)
+            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(c.g), dim3(kRedThreads), 0, c.st, n, 1.0, W + (int64_t)k * n, Vk(k));
+        THIP(hipGetLastError());
+        THIP(hipMemcpyAsync(Vk(kk), Vk(m), n * sizeof(VT), hipMemcpyDeviceToDevice, c.st));
+        for (int i = 0; i < M; i++) {
+            std::vector<double>& Ti = T[i];
+            std::fill(Ti.begin(), Ti.end(), 0.0);
+            const int me = meff[i], kept = std::min(kk, me);
+            for (int k = 0; k < kept; k++) {
+                Ti[(size_t)k * m + k] = theta[i][k];
+                if (alive[i]) Ti[(size_t)k * m + kk] = Ti[(size_t)kk * m + k] = beta_m[i] * Y[i][(me - 1) + (size_t)k * me];
+            }
+            if (!alive[i]) meff[i] = kept;   // an exhausted instance keeps its exact Ritz pairs only
+        }
+        start = kk;
+    }
+    // Ritz vectors of the nev lowest pairs, unit per instance
+    const int kmax = *std::max_element(meff.begin(), meff.end());
+    std::vector<double> t((size_t)kmax * M);
+    for (int k = 0; k < nev; k++) {
+        std::fill(t.begin(), t.end(), 0.0);
+        for (int i = 0; i < M; i++)
+            if (k < meff[i])
+                for (int l = 0; l < meff[i]; l++) t[(size_t)l * M + i] = Y[i][l + (size_t)k * meff[i]];
+        cd2* x = nullptr;
+        THIP(hipMalloc(&x, n * sizeof(cd2)));
+        prev.push_back(x);
+        THIP(hipMemsetAsync(x, 0, n * sizeof(cd2), c.st));
+        TRC(baccumulate<VT>(c, kmax, V, n, t, x));
+        TRC(bnorm2(c, x, nn));
+        for (int i = 0; i < M; i++) sc[i] = nn[i] > 0.0 ? 1.0 / std::sqrt(nn[i]) : 0.0;
+        TRC((bscale<cd2, cd2>(c, sc, x, x)));
+        for (int i = 0; i < M; i++) evals[(size_t)i * nev + k] = k < meff[i] ? theta[i][k] : 0.0;
+    }
+    THIP(hipStreamSynchronize(c.st));
+    cleanup();
+    return ED_OK;
+#undef THIP
+#undef TRC
+}
+
 // FP64 Rayleigh quotients and residual norms of unit interleaved vectors x.
 int bfp64_check(BCtx& c, const cd2* x, std::vector<double>& rq, std::vector<double>& res) {
     cd2* y = nullptr;
@@ -2701,18 +2861,29 @@ extern "C" int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double
     HIPCHK(hipEventCreate(&c.e1));
     std::vector<cd2*> prev;
     int rc;
-    if (p->precision == ED_C128) {
-        rc = run_block_full<cd2>(c, *p, p->tol, nullptr, prev, evals);
+    // ED_LZ_FULL: one full-reorthogonalisation run per eigenvalue, deflated
+    // against the earlier ones; ED_LZ_THICK: the nev lowest pairs of every
+    // instance from one thick-restarted Krylov space of `iterations` vectors
+    const bool thick = p->mode == ED_LZ_THICK;
+    auto run = [&](bool c128, double tl, const std::vector<cd2*>* st, std::vector<cd2*>& out) -> int {
+        if (thick)
+            return c128 ? run_block_trl<cd2>(c, *p, tl, st, out, evals) : run_block_trl<cf2>(c, *p, tl, st, out, evals);
+        return c128 ? run_block_full<cd2>(c, *p, tl, st, out, evals) : run_block_full<cf2>(c, *p, tl, st, out, evals);
+    };
+    if (p->mode != ED_LZ_FULL && !thick) {
+        rc = ed_fail(ED_EINVAL, "block Lanczos runs ED_LZ_FULL or ED_LZ_THICK, got mode %d", p->mode);
+    } else if (p->precision == ED_C128) {
+        rc = run(true, p->tol, nullptr, prev);
     } else {
         // complex64 Krylov vectors resolve residuals down to ~1e-6 relative;
         // when tol asks for less, the complex64 Ritz vectors start a short
         // complex128 run that polishes them to tol.
         const double floor64 = 1e-6;
-        rc = run_block_full<cf2>(c, *p, std::max(p->tol, floor64), nullptr, prev, evals);
+        rc = run(false, std::max(p->tol, floor64), nullptr, prev);
         if (rc == ED_OK && p->tol < floor64) {
             std::vector<cd2*> coarse;
             coarse.swap(prev);
-            rc = run_block_full<cd2>(c, *p, p->tol, &coarse, prev, evals);
+            rc = run(true, p->tol, &coarse, prev);
             for (auto* v : coarse) dfree(v);
         }
     }

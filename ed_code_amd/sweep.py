@@ -8,6 +8,11 @@ the M instances share every H*V pass (ed_block_hv) and one Lanczos loop
 (ed_block_lanczos).
 
     python -m ed_code_amd.sweep --lattice tri6x5 --instances 4 --nev 2 --tol 1e-8
+
+By default each sector runs the block thick-restart Lanczos (24 Krylov
+vectors per instance, the nev pairs from one space); --thick 0 runs one
+full-reorthogonalisation Lanczos per eigenvalue with deflation instead
+(30-site triangular torus, 30 sectors x 4 couplings: 44 s against 65 s).
 """
 from __future__ import annotations
 
@@ -18,7 +23,8 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from . import (ED_C64, ED_C128, ED_LZ_FULL, Basis, BlockOperator, Lattice, MultiBlockOperator, Operator, init)
+from . import (ED_C64, ED_C128, ED_LZ_FULL, ED_LZ_THICK, Basis, BlockOperator, Lattice, MultiBlockOperator, Operator,
+               init)
 from . import lattices as L
 from .lattices import Model
 
@@ -45,9 +51,12 @@ def coupling_instances(lattice: str, M: int, seed: int = 7) -> List[Model]:
 
 def sweep_sectors(models: Sequence[Model], n_up: int, nev: int = 2, tol: float = 1e-8, iterations: int = 300,
                   precision: int = ED_C64, sectors: Optional[Sequence[tuple]] = None, seed: int = 1234,
-                  log: Optional[Callable[[str], None]] = None) -> dict:
+                  log: Optional[Callable[[str], None]] = None, thick: int = 0) -> dict:
     """Lowest nev eigenvalues of every instance in every momentum sector
-    (or the listed (kx, ky) sectors).  Returns per-sector results and totals."""
+    (or the listed (kx, ky) sectors).  thick > 0: the block thick-restart
+    Lanczos with that many Krylov vectors per instance (all nev pairs from
+    one space); 0: one full-reorthogonalisation run of up to `iterations`
+    steps per eigenvalue.  Returns per-sector results and totals."""
     lat = Lattice.of(models[0])
     if sectors is None:
         sectors = [(kx, ky) for kx in range(lat.L1) for ky in range(lat.L2)]
@@ -59,7 +68,11 @@ def sweep_sectors(models: Sequence[Model], n_up: int, nev: int = 2, tol: float =
         if basis.D == 0:
             continue
         blk = BlockOperator(basis, models)
-        ev, st = blk.lanczos(nev=nev, iterations=iterations, tol=tol, seed=seed, precision=precision)
+        if thick > 0:
+            ev, st = blk.lanczos(nev=nev, iterations=thick, ncycles=max(1, iterations), tol=tol, seed=seed,
+                                 precision=precision, mode=ED_LZ_THICK)
+        else:
+            ev, st = blk.lanczos(nev=nev, iterations=iterations, tol=tol, seed=seed, precision=precision)
         dt = time.perf_counter() - t0
         hv_calls += st["hv_calls"]
         res.append(dict(kx=kx, ky=ky, D=basis.D, evals=ev.tolist(), residual=st["residual"][:len(models)],
@@ -168,6 +181,10 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--sectors", type=int, default=0, help="only the first n sectors (0 = all)")
     ap.add_argument("--compare", type=int, default=0, help="also run the one-at-a-time loop on n sectors")
+    ap.add_argument("--thick", type=int, default=24,
+                    help="block thick-restart Lanczos with this many Krylov vectors per instance, at most "
+                         "--iterations restarts (0: one full-reorthogonalisation run of up to --iterations "
+                         "steps per eigenvalue)")
     ap.add_argument("--multi", action="store_true",
                     help="multi-sector block operators: every Lanczos step one H*V launch over all sectors of a group")
     a = ap.parse_args(argv)
@@ -180,9 +197,11 @@ def main(argv=None):
         secs = secs[:a.sectors]
     prec = ED_C64 if a.precision == "c64" else ED_C128
     import sys
-    sweep = sweep_sectors_multi if a.multi else sweep_sectors
-    out = sweep(models, n_up, a.nev, a.tol, a.iterations, prec, secs,
-                log=lambda s: print(s, file=sys.stderr, flush=True))
+    logf = lambda s: print(s, file=sys.stderr, flush=True)
+    if a.multi:
+        out = sweep_sectors_multi(models, n_up, a.nev, a.tol, a.iterations, prec, secs, log=logf)
+    else:
+        out = sweep_sectors(models, n_up, a.nev, a.tol, a.iterations, prec, secs, log=logf, thick=a.thick)
     out["lattice"], out["instances"], out["n_up"] = a.lattice, a.instances, n_up
     if a.compare:
         cmp_secs = [(r["kx"], r["ky"]) for r in out["sectors"][:a.compare]]

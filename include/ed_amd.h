@@ -328,18 +328,23 @@ int ed_block_fast(ed_block blk);
  * V and W must not alias.  Asynchronous on `stream`. */
 int ed_block_hv(ed_block blk, int32_t dtype, const void* V, void* W, void* stream);
 /* Lowest p->nev eigenvalues of each instance (evals[m*nev + k], ascending),
- * all instances advanced together: full-reorthogonalisation Lanczos, one
- * eigenvalue at a time with deflation against the earlier Ritz vectors, at
- * most p->iterations steps each; an instance stops when the residual
- * estimate of its lowest Ritz pair is below p->tol * max(1, |E|).  Krylov
- * vectors are stored in p->precision, alpha/beta and reorthogonalisation
- * coefficients are FP64.  With complex64 storage the run stops at residual
- * 1e-6 and, if p->tol is smaller, its Ritz vectors start a complex128 run
- * that reaches p->tol; each eigenvalue is then the FP64 Rayleigh quotient of
- * its Ritz vector.  M must be 1, 2, 4 or 8.  stats (optional): H*V calls and
- * times, and the FP64 residual ||H x - E x|| and Rayleigh quotient of the
- * lowest Ritz vector of each instance (residual[m], rayleigh[m]).
- * p->mode and p->ncycles are ignored. */
+ * all instances advanced together.  p->mode ED_LZ_FULL: full-
+ * reorthogonalisation Lanczos, one eigenvalue at a time with deflation
+ * against the earlier Ritz vectors, at most p->iterations steps each; an
+ * instance stops when the residual estimate of its lowest Ritz pair is below
+ * p->tol * max(1, |E|).  ED_LZ_THICK: thick-restart Lanczos with
+ * p->iterations Krylov vectors per instance and at most p->ncycles
+ * restarts, the nev lowest pairs of each instance from one space, until
+ * every instance's nev residual estimates are below p->tol * max(1, |E|).
+ * Krylov vectors are stored in p->precision, alpha/beta and
+ * reorthogonalisation coefficients are FP64.  With complex64 storage the run
+ * stops at residual 1e-6 and, if p->tol is smaller, its Ritz vectors start a
+ * complex128 run that reaches p->tol (ED_LZ_THICK: from the sum of each
+ * instance's Ritz vectors); each eigenvalue is then the FP64 Rayleigh
+ * quotient of its Ritz vector.  M must be 1, 2, 4 or 8.  stats (optional):
+ * H*V calls and times, and the FP64 residual ||H x - E x|| and Rayleigh
+ * quotient of the lowest Ritz vector of each instance (residual[m],
+ * rayleigh[m]).  p->ncycles is ignored by ED_LZ_FULL. */
 int ed_block_lanczos(ed_block blk, const ed_lanczos_params* p, double* evals, ed_lanczos_stats* stats);
 
 /* ---- Multi-sector block operators: M coupling instances on each of nsec
