@@ -144,18 +144,20 @@ THICK_RESTARTS = 40
 
 
 def footprint(rows: int, M: int, iterations: int, nev: int) -> int:
-    """Device bytes of one job in complex64: Krylov vectors (the block
-    solver's with its complex128 polish; the thick restart's complex64 basis
-    and complex128 restart vectors for one request with nev > 1; three
-    vectors for one request with nev = 1), work vectors, the gather images of
-    both precisions (at most 6 elements per row), FP64 Ritz vectors, basis,
-    row positions and directory."""
+    """Device bytes of one job in complex64: Krylov vectors (for a batch the
+    block thick restart's complex128 polish phase, its largest; the thick
+    restart's complex64 basis and complex128 restart vectors for one request
+    with nev > 1; three vectors for one request with nev = 1), work vectors,
+    the gather images of both precisions (at most 6 elements per row), FP64
+    Ritz vectors, basis, row positions and directory."""
     if M == 1 and nev == 1:
         krylov = 4 * rows * 8 + rows * 16
     elif M == 1:
         krylov = (THICK_VECTORS + 1) * rows * 8 + THICK_VECTORS * rows * 16
     else:
-        krylov = (iterations + 1) * rows * M * 8 + (min(iterations, 60) + 1) * rows * M * 16
+        # the block thick restart's complex128 polish: its basis, restart
+        # vectors and the complex64 Ritz vectors it starts from
+        krylov = ((THICK_VECTORS + 1) * 16 + THICK_VECTORS * 16 + nev * 16) * rows * M
     work = rows * M * (6 * (8 + 16) + 2 * 16) + nev * rows * M * 16
     basis = rows * (8 * 2 + 4)
     return krylov + work + basis
@@ -195,7 +197,9 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
     torus 2.8-3.1 s for nev = 4 against 6.2-6.8 s for the block solver with
     its complex128 polish), both in complex64 with the FP64 Rayleigh-quotient
     eigenvalues and residual bounds; batches of one sector run the block
-    full-reorthogonalisation Lanczos."""
+    thick-restart Lanczos (complex64, polished in complex128 to tol), or on
+    sectors smaller than its Krylov space the block full-reorthogonalisation
+    Lanczos."""
     models = [model_for(r.lattice, r.J2, r.lam, r.Ch) for r in reqs]
     if len(reqs) == 1 and reqs[0].nev == 1:
         op = Operator(basis, models[0])
@@ -210,6 +214,11 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
         n = len(e)
         ev = np.array([e])
         err = np.array([st["eval_error"][:n]])
+    elif basis.D > THICK_VECTORS:
+        blk = BlockOperator(basis, models)
+        ev, st = blk.lanczos(nev=reqs[0].nev, iterations=THICK_VECTORS, ncycles=THICK_RESTARTS, tol=reqs[0].tol,
+                             precision=ED_C64, mode=ED_LZ_THICK)
+        err = st["eval_error"]
     else:
         blk = BlockOperator(basis, models)
         ev, st = blk.lanczos(nev=reqs[0].nev, iterations=iterations, tol=reqs[0].tol, precision=ED_C64)
