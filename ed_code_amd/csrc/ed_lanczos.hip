@@ -538,8 +538,10 @@ int run_full(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double
 // 602) for the nev lowest eigenpairs with at most m = iterations Krylov
 // vectors: the ARPACK-like solver of the "SR" drivers (ARPACK_sector.cpp:
 // 4-187 runs znaupd with ncv Arnoldi vectors).  Each expansion step
-// reorthogonalises against all kept vectors (two classical Gram-Schmidt
-// passes); the projected matrix is the Lanczos tridiagonal bordered by the
+// removes the known part of H v_j (alpha v_j and beta v_{j-1}, or after a
+// restart the couplings to the kept Ritz vectors), then reorthogonalises
+// against all kept and new vectors (classical Gram-Schmidt, a second pass
+// when the first removed more than 1 - 1/sqrt(2) of the norm); the projected matrix is the Lanczos tridiagonal bordered by the
 // arrowhead row of the kept Ritz pairs.  After m vectors the Rayleigh-Ritz
 // pairs are formed (symeig_jacobi); if the nev lowest have residual
 // beta_m |y_m,i| <= tol max(1, |theta_i|) the run stops, else the
@@ -571,13 +573,31 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
         meff = m;
         for (int j = start; j < m; j++) {
             RC(hv<VT>(c, Vk(j), w));
-            RC((multidot<VT, VT>(c, j + 1, V, D, w, q)));
-            RC((multiaxpy<VT, VT>(c, j + 1, V, D, q, 1.0, w)));
-            RC((multidot<VT, VT>(c, j + 1, V, D, w, q2)));
-            RC((multiaxpy<VT, VT>(c, j + 1, V, D, q2, 1.0, w)));
-            T[(size_t)j * m + j] = q[j].real() + q2[j].real();
-            double bb;
-            RC(dnorm2(c, w, bb));
+            // the known part of H v_j first: alpha_j v_j and beta_{j-1} v_{j-1},
+            // or after a restart the couplings to the kept Ritz vectors
+            RC((multidot<VT, VT>(c, 1, Vk(j), 0, w, q)));
+            double alpha = q[0].real();
+            const int lo = j > start ? j - 1 : 0;
+            std::vector<std::complex<double>> cf((size_t)(j + 1 - lo));
+            for (int l = lo; l < j; l++) cf[l - lo] = T[(size_t)l * m + j];
+            cf[j - lo] = alpha;
+            RC((multiaxpy<VT, VT>(c, j + 1 - lo, Vk(lo), D, cf, 1.0, w)));
+            // Gram-Schmidt against every kept and new vector; the second pass
+            // only when the first removed more than 1 - 1/sqrt(2) of the norm
+            double nb2, bb = 0.0;
+            bool have_bb = false;
+            RC(dnorm2(c, w, nb2));
+            for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1) {
+                    RC(dnorm2(c, w, bb));
+                    if (bb > 0.5 * nb2) { have_bb = true; break; }
+                }
+                RC((multidot<VT, VT>(c, j + 1, V, D, w, q2)));
+                alpha += q2[j].real();
+                RC((multiaxpy<VT, VT>(c, j + 1, V, D, q2, 1.0, w)));
+            }
+            T[(size_t)j * m + j] = alpha;
+            if (!have_bb) RC(dnorm2(c, w, bb));
             const double beta = std::sqrt(bb);
             if (beta < 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * m + j]))) {   // invariant subspace
                 meff = j + 1;
@@ -1435,9 +1455,11 @@ int run_block_full(BCtx& c, const ed_lanczos_params& p, double tol, const std::v
 }
 
 // Thick-restart Lanczos (run_trl) for the M instances of a block operator in
-// lock step: one block H*V per expansion step for all instances, two
-// classical Gram-Schmidt passes against the instance's kept and new vectors
-// (per-instance coefficients), the Rayleigh-Ritz of each instance's
+// lock step: one block H*V per expansion step for all instances, the known
+// part of H v_j removed first, then classical Gram-Schmidt against the
+// instance's kept and new vectors (per-instance coefficients; a second pass
+// when the first removed more than 1 - 1/sqrt(2) of any live instance's
+// norm), the Rayleigh-Ritz of each instance's
 // arrowhead-bordered tridiagonal, restart of every instance from its
 // nev + (m - nev)/2 lowest Ritz vectors and its normalised residual, until
 // every instance's nev lowest pairs have residual estimate
@@ -1503,15 +1525,41 @@ int run_block_trl(BCtx& c, const ed_lanczos_params& p, double tol, const std::ve
             if (alive[i]) meff[i] = m;
         for (int j = start; j < m; j++) {
             TRC(bhv<VT>(c, Vk(j), w));
-            TRC((bdot<VT, VT>(c, j + 1, V, n, w, q)));
-            TRC((baxpy<VT, VT>(c, j + 1, V, n, q, nullptr, w)));
-            TRC((bdot<VT, VT>(c, j + 1, V, n, w, q2)));
-            TRC((baxpy<VT, VT>(c, j + 1, V, n, q2, nullptr, w)));
-            TRC(bnorm2(c, w, nn));
+            // the known part of H v_j first (per instance): alpha_j v_j and
+            // beta_{j-1} v_{j-1}, or after a restart the couplings to the kept
+            // Ritz vectors
+            TRC((bdot<VT, VT>(c, 1, Vk(j), 0, w, q)));
+            std::vector<double> alpha(M);
+            for (int i = 0; i < M; i++) alpha[i] = q[i].real();
+            const int lo = j > start ? j - 1 : 0;
+            std::vector<std::complex<double>> cf((size_t)(j + 1 - lo) * M);
+            for (int i = 0; i < M; i++) {
+                for (int l = lo; l < j; l++) cf[(size_t)(l - lo) * M + i] = T[i][(size_t)l * m + j];
+                cf[(size_t)(j - lo) * M + i] = alpha[i];
+            }
+            TRC((baxpy<VT, VT>(c, j + 1 - lo, Vk(lo), n, cf, nullptr, w)));
+            // Gram-Schmidt against every kept and new vector; the second pass
+            // when the first removed more than 1 - 1/sqrt(2) of the norm of
+            // any live instance
+            std::vector<double> nb0;
+            TRC(bnorm2(c, w, nb0));
+            bool have_nn = false;
+            for (int pass = 0; pass < 2; pass++) {
+                if (pass == 1) {
+                    TRC(bnorm2(c, w, nn));
+                    bool need = false;
+                    for (int i = 0; i < M; i++) need |= alive[i] && nn[i] <= 0.5 * nb0[i];
+                    if (!need) { have_nn = true; break; }
+                }
+                TRC((bdot<VT, VT>(c, j + 1, V, n, w, q2)));
+                for (int i = 0; i < M; i++) alpha[i] += q2[(size_t)j * M + i].real();
+                TRC((baxpy<VT, VT>(c, j + 1, V, n, q2, nullptr, w)));
+            }
+            if (!have_nn) TRC(bnorm2(c, w, nn));
             for (int i = 0; i < M; i++) {
                 sc[i] = 0.0;
                 if (!alive[i]) continue;
-                const double a = q[(size_t)j * M + i].real() + q2[(size_t)j * M + i].real();
+                const double a = alpha[i];
                 T[i][(size_t)j * m + j] = a;
                 const double beta = std::sqrt(std::max(nn[i], 0.0));
                 if (beta < 1e-13 * std::max(1.0, std::fabs(a))) {   // invariant subspace: the space is exact
