@@ -42,9 +42,9 @@ static thread_local std::string g_err;
 hipStream_t setup_stream() {
     thread_local hipStream_t st[16] = {};
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
     // created once per thread and device and kept for the thread's life; a
-    // failed creation leaves the legacy stream (0)
+    // device past the table or a failed creation uses the legacy stream (0)
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
     if (!st[dev] && hipStreamCreateWithFlags(&st[dev], hipStreamNonBlocking) != hipSuccess) st[dev] = 0;
     return st[dev];
 }

@@ -36,8 +36,8 @@ struct BEnt {            // per term: flip partner y, chiral z-site k0 (-1 for a
 struct BCoef {           // per (term, instance): bond part (or the chiral entry)
     double cre, cim, dJ2;
 };
-struct BChir {           // per (term, instance), merged terms: the two chiral entries
-    double cre[2], cim[2];
+struct alignas(16) BChir {   // per (term, instance), merged terms: the pair's coefficient at s_x = 0
+    cd2 c[4];                // for the z-site bits s_k0 | s_k1 << 1 (merged_coef4; conjugated for s_x = 1)
 };
 
 inline unsigned nblk(int64_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -101,8 +101,9 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
     extern __shared__ __align__(16) unsigned char smem[];
     BChir* chir = (BChir*)smem;                                // nterm*M (merged terms)
     cd2* phase = (cd2*)(chir + (MG ? nterm * M : 0));          // nphase
-    BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M
-    TFam* fam = (TFam*)(coef + nterm * M);                     // nfam
+    BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M (terms that are not merged)
+    double* dj2 = (double*)coef;                               // nterm*M (merged terms: only dJ2)
+    TFam* fam = MG ? (TFam*)(dj2 + nterm * M) : (TFam*)(coef + nterm * M);   // nfam
     int* foff = (int*)(fam + nfam);                            // nfam (+1 pad)
     BEnt* ent = (BEnt*)(foff + nfam + (nfam & 1));             // nterm
     double* ezz0 = (double*)(ent + nterm);                     // M
@@ -111,7 +112,10 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
     if (MG)
         for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) chir[q] = gchir[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = gphase[q];
-    for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
+    if (MG)
+        for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) dj2[q] = gcoef[q].dJ2;
+    else
+        for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) { fam[q] = gfam[q]; foff[q] = gfoff[q]; }
     for (int q = threadIdx.x; q < nterm; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < M; q += blockDim.x) ezz0[q] = gezz0[q];
@@ -169,31 +173,25 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
             const BEnt E = ent[ti[q]];
             const bool bond = MG || E.k0 < 0;
             const bool sx = (s >> xs[q]) & 1u, sy = (s >> E.y) & 1u, sk = (s >> (E.k0 & 31)) & 1u;
-            // merged terms: signs and j-site bits of the two chiral entries
-            const double sg0 = ((s >> (E.k0 & 31)) & 1u) ? 1.0 : -1.0, sg1 = ((s >> (E.k1 & 31)) & 1u) ? 1.0 : -1.0;
-            const bool j0 = (E.flags & 2) ? sx : sy, j1 = (E.flags & 8) ? sx : sy;
+            // merged terms: the pair's coefficient by its two z-site bits
+            const int cidx = (int)(((s >> (E.k0 & 31)) & 1u) | (((s >> (E.k1 & 31)) & 1u) << 1));
             cd2 ph{1.0, 0.0};
             if (!TRIV) ph = phase[code[q]];
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const BCoef C = coef[ti[q] * M + m];
                 double cr, ci;
-                if (bond) {
+                if (MG) {
+                    if (ok[q]) ez[m] -= dj2[ti[q] * M + m];
+                    const cd2 co = chir[ti[q] * M + m].c[cidx];
+                    cr = co.re;
+                    ci = sx ? -co.im : co.im;
+                } else if (bond) {
+                    const BCoef C = coef[ti[q] * M + m];
                     if (ok[q]) ez[m] -= C.dJ2;
                     cr = C.cre;
                     ci = sx ? -C.cim : C.cim;
-                    if (MG) {
-                        const BChir X = chir[ti[q] * M + m];
-                        if (E.flags & 1) {
-                            cr += -X.cim[0] * sg0;
-                            ci += (j0 ? X.cre[0] : -X.cre[0]) * sg0;
-                        }
-                        if (E.flags & 4) {
-                            cr += -X.cim[1] * sg1;
-                            ci += (j1 ? X.cre[1] : -X.cre[1]) * sg1;
-                        }
-                    }
                 } else {
+                    const BCoef C = coef[ti[q] * M + m];
                     const double sg = sk ? 1.0 : -1.0;
                     cr = -C.cim * sg;
                     ci = (sy ? C.cre : -C.cre) * sg;
@@ -265,8 +263,8 @@ void union_lists(const ed_terms* T, int M, int m, std::vector<Bond>& bonds, std:
 size_t block_smem(const ed_block_s* K) {
     const ed_basis_s* B = K->b;
     const int nf = K->nfam;
-    return (K->mg ? (size_t)K->nterm * K->M * sizeof(BChir) : 0) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
-           (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
+    return (size_t)K->nterm * K->M * (K->mg ? sizeof(BChir) + sizeof(double) : sizeof(BCoef)) +
+           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)nf * sizeof(TFam) +
            (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)K->nterm * sizeof(BEnt) + (size_t)K->M * sizeof(double) +
            (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)B->img_nmap * sizeof(uint16_t);
 }
@@ -462,8 +460,9 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
                         const TEntry& E = ents[m][f * 32 + x];
                         bc.push_back(BCoef{E.cre, E.cim, E.dJ2});
                         if (K->mg) {
-                            const TChir& X = chxs[m][f * 32 + x];
-                            bx.push_back(BChir{{X.cre[0], X.cre[1]}, {X.cim[0], X.cim[1]}});
+                                BChir X;
+                            merged_coef4(E, chxs[m][f * 32 + x], X.c);
+                            bx.push_back(X);
                         }
                     }
                 }
@@ -576,8 +575,9 @@ __global__ void __launch_bounds__(256) k_hv_mblock_c(int nsec, const MSec* __res
     extern __shared__ __align__(16) unsigned char smem[];
     BChir* chir = (BChir*)smem;                                // nterm*M (merged terms)
     cd2* phase = (cd2*)(chir + (MG ? nterm * M : 0));          // nphase
-    BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M
-    TFam* fam = (TFam*)(coef + nterm * M);                     // nfam
+    BCoef* coef = (BCoef*)(phase + nphase);                    // nterm*M (terms that are not merged)
+    double* dj2 = (double*)coef;                               // nterm*M (merged terms: only dJ2)
+    TFam* fam = MG ? (TFam*)(dj2 + nterm * M) : (TFam*)(coef + nterm * M);   // nfam
     int* foff = (int*)(fam + nfam);                            // nfam (+1 pad)
     BEnt* ent = (BEnt*)(foff + nfam + (nfam & 1));             // nterm
     double* ezz0 = (double*)(ent + nterm);                     // M
@@ -588,7 +588,10 @@ __global__ void __launch_bounds__(256) k_hv_mblock_c(int nsec, const MSec* __res
     if (MG)
         for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) chir[q] = gchir[q];
     for (int q = threadIdx.x; q < nphase; q += blockDim.x) phase[q] = S.phase[q];
-    for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
+    if (MG)
+        for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) dj2[q] = gcoef[q].dJ2;
+    else
+        for (int q = threadIdx.x; q < nterm * M; q += blockDim.x) coef[q] = gcoef[q];
     for (int q = threadIdx.x; q < nfam; q += blockDim.x) { fam[q] = gfam[q]; foff[q] = gfoff[q]; }
     for (int q = threadIdx.x; q < nterm; q += blockDim.x) ent[q] = gent[q];
     for (int q = threadIdx.x; q < M; q += blockDim.x) ezz0[q] = gezz0[q];
@@ -643,29 +646,24 @@ __global__ void __launch_bounds__(256) k_hv_mblock_c(int nsec, const MSec* __res
             const BEnt E = ent[ti[q]];
             const bool bond = MG || E.k0 < 0;
             const bool sx = (s >> xs[q]) & 1u, sy = (s >> E.y) & 1u, sk = (s >> (E.k0 & 31)) & 1u;
-            const double sg0 = ((s >> (E.k0 & 31)) & 1u) ? 1.0 : -1.0, sg1 = ((s >> (E.k1 & 31)) & 1u) ? 1.0 : -1.0;
-            const bool j0 = (E.flags & 2) ? sx : sy, j1 = (E.flags & 8) ? sx : sy;
+            // merged terms: the pair's coefficient by its two z-site bits
+            const int cidx = (int)(((s >> (E.k0 & 31)) & 1u) | (((s >> (E.k1 & 31)) & 1u) << 1));
             const cd2 ph = phase[code[q]];
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const BCoef Cf = coef[ti[q] * M + m];
                 double cr, ci;
-                if (bond) {
+                if (MG) {
+                    if (ok[q]) ez[m] -= dj2[ti[q] * M + m];
+                    const cd2 co = chir[ti[q] * M + m].c[cidx];
+                    cr = co.re;
+                    ci = sx ? -co.im : co.im;
+                } else if (bond) {
+                    const BCoef Cf = coef[ti[q] * M + m];
                     if (ok[q]) ez[m] -= Cf.dJ2;
                     cr = Cf.cre;
                     ci = sx ? -Cf.cim : Cf.cim;
-                    if (MG) {
-                        const BChir X = chir[ti[q] * M + m];
-                        if (E.flags & 1) {
-                            cr += -X.cim[0] * sg0;
-                            ci += (j0 ? X.cre[0] : -X.cre[0]) * sg0;
-                        }
-                        if (E.flags & 4) {
-                            cr += -X.cim[1] * sg1;
-                            ci += (j1 ? X.cre[1] : -X.cre[1]) * sg1;
-                        }
-                    }
                 } else {
+                    const BCoef Cf = coef[ti[q] * M + m];
                     const double sg = sk ? 1.0 : -1.0;
                     cr = -Cf.cim * sg;
                     ci = (sy ? Cf.cre : -Cf.cre) * sg;
@@ -713,8 +711,8 @@ size_t mblock_smem(const ed_mblock_s* MB) {
     const ed_block_s* K = MB->blocks[0];
     const ed_basis_s* B = K->b;
     const int nf = K->nfam;
-    return (K->mg ? (size_t)K->nterm * K->M * sizeof(BChir) : 0) + (size_t)B->L1 * B->L2 * sizeof(cd2) +
-           (size_t)K->nterm * K->M * sizeof(BCoef) + (size_t)nf * sizeof(TFam) +
+    return (size_t)K->nterm * K->M * (K->mg ? sizeof(BChir) + sizeof(double) : sizeof(BCoef)) +
+           (size_t)B->L1 * B->L2 * sizeof(cd2) + (size_t)nf * sizeof(TFam) +
            (size_t)(nf + (nf & 1)) * sizeof(int) + (size_t)K->nterm * sizeof(BEnt) + (size_t)K->M * sizeof(double) +
            (size_t)B->rtab32_n * sizeof(uint32_t) + (size_t)MB->nmap_max * sizeof(uint16_t);
 }

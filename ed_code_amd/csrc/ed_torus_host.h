@@ -132,6 +132,22 @@ inline bool torus_fam_extras(const std::vector<TFam>& fam, const std::vector<TEn
 // operations and order of the per-element formula (the bond part, then each
 // chiral entry's -cim sg and +-cre sg); mki[f * 32 + x] = k0 | k1 << 8, an
 // absent entry's z-site set to x.
+// The four coefficients of one merged member (see torus_merged_tables).
+inline void merged_coef4(const TEntry& E, const TChir& X, cd2 out[4]) {
+    for (int idx = 0; idx < 4; idx++) {
+        const int sx = 0, sk[2] = {idx & 1, (idx >> 1) & 1};
+        double cr = E.cre, ci = sx ? -E.cim : E.cim;
+        for (int h = 0; h < 2; h++) {
+            if (!((X.flags >> (2 * h)) & 1)) continue;
+            const double sg = sk[h] ? 1.0 : -1.0;
+            const int jbit = ((X.flags >> (2 * h + 1)) & 1) ? sx : 1 - sx;
+            cr += -X.cim[h] * sg;
+            ci += (jbit ? X.cre[h] : -X.cre[h]) * sg;
+        }
+        out[idx] = cd2{cr, ci};
+    }
+}
+
 inline void torus_merged_tables(const std::vector<TFam>& fam, const std::vector<TEntry>& ent,
                                 const std::vector<TChir>& chx, std::vector<uint32_t>& mki, std::vector<cd2>& mco) {
     mki.assign(kMaxFam * 32, 0u);
@@ -140,22 +156,10 @@ inline void torus_merged_tables(const std::vector<TFam>& fam, const std::vector<
         for (int x = 0; x < 32; x++) {
             if (!((fam[f].S >> x) & 1u)) continue;
             const size_t m = f * 32 + x;
-            const TEntry& E = ent[m];
             const TChir& X = chx[m];
             const int k0 = (X.flags & 1) ? X.k[0] : x, k1 = (X.flags & 4) ? X.k[1] : x;
             mki[m] = (uint32_t)k0 | ((uint32_t)k1 << 8);
-            for (int idx = 0; idx < 4; idx++) {
-                const int sx = 0, sk[2] = {idx & 1, (idx >> 1) & 1};
-                double cr = E.cre, ci = sx ? -E.cim : E.cim;
-                for (int h = 0; h < 2; h++) {
-                    if (!((X.flags >> (2 * h)) & 1)) continue;
-                    const double sg = sk[h] ? 1.0 : -1.0;
-                    const int jbit = ((X.flags >> (2 * h + 1)) & 1) ? sx : 1 - sx;
-                    cr += -X.cim[h] * sg;
-                    ci += (jbit ? X.cre[h] : -X.cre[h]) * sg;
-                }
-                mco[m * 4 + idx] = cd2{cr, ci};
-            }
+            merged_coef4(ent[m], X, &mco[m * 4]);
         }
 }
 
