@@ -194,7 +194,7 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
     """One job: a single request for the lowest eigenvalue runs the
     graph-captured three-vector Lanczos, a single request for several the
     thick-restart Lanczos (THICK_VECTORS Krylov vectors; on the 32-site
-    torus 2.8-3.1 s for nev = 4 against 6.2-6.8 s for the block solver with
+    torus 2.3-2.5 s for nev = 4 against 6.2-6.8 s for the block solver with
     its complex128 polish), both in complex64 with the FP64 Rayleigh-quotient
     eigenvalues and residual bounds; batches of one sector run the block
     thick-restart Lanczos (complex64, polished in complex128 to tol), or on
