@@ -320,8 +320,8 @@ class Operator:
         rotation kernel, 4 compiled-shape K=2 at 8 waves, 5 compiled-shape K=8,
         6 two-row 64-bit kernel, 7 compiled-shape K=4 software-pipelined,
         8 two-row kernel gathering u by row through the directory, 9 / 10
-        compiled-shape with every row split over two / four lanes; block 0 =
-        default."""
+        compiled-shape with every row split over two / four lanes, 11 as 9
+        with K=4 per lane; block 0 = default."""
         check(lib().ed_op_tune(self.h, path, block))
 
     def tuning(self, dtype: str = "c128"):

@@ -845,7 +845,7 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 10) return ed_fail(ED_EINVAL, "path %d not in 0..10", path);
+    if (path < 0 || path > 11) return ed_fail(ED_EINVAL, "path %d not in 0..11", path);
     if ((path == 6 || path == 8) && !op->use_rows2)
         return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
     if (((path >= 2 && path <= 5) || path == 7 || path >= 9) && !op->use_torus)

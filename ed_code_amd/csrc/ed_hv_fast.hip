@@ -265,8 +265,16 @@ __global__ void __launch_bounds__(512) k_hv_torus_s4(ED_TORUS_C_ARGS) {
     hv_torus_split_body<TC, TRIV, MODE, 4, 2, VT>(ED_TORUS_C_PASS);
 }
 
+// Variant G: rows split over two lanes, four elements in flight per lane
+// (a lane's share of ~33 elements on the 32-site torus wastes fewer batch
+// slots than with eight).
+template <class TC, bool TRIV, int MODE, typename VT>
+__global__ void __launch_bounds__(512) k_hv_torus_s2k4(ED_TORUS_C_ARGS) {
+    hv_torus_split_body<TC, TRIV, MODE, 4, 1, VT>(ED_TORUS_C_PASS);
+}
+
 // Lanes per row of a variant (1 for the one-lane variants 0-3).
-static int variant_lanes(int var) { return var == 4 ? 2 : var == 5 ? 4 : 1; }
+static int variant_lanes(int var) { return (var == 4 || var == 6) ? 2 : var == 5 ? 4 : 1; }
 
 template <class TC, typename VT>
 static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, size_t sm, int var, int bs) {
@@ -307,6 +315,8 @@ static void launch_c(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t s
         ED_GO_MODES(k_hv_torus_p4)
     } else if (var == 4) {
         ED_GO_MODES(k_hv_torus_s2)
+    } else if (var == 6) {
+        ED_GO_MODES(k_hv_torus_s2k4)
     } else {
         ED_GO_MODES(k_hv_torus_s4)
     }
@@ -386,9 +396,9 @@ static constexpr int dtype_slot() {
 // fastest depends on the register pressure of the shape, the vector precision
 // and the momentum (measured on the 32-site torus: K=4 at 256 threads for
 // complex64 at k=0, K=8 at 256 for complex128, K=2 at 8 waves for k != 0).
-// The split variants (4, 5) serve families of at most kSplitMaxFam.
-static const int kTuneCfg[12][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256},
-                                    {3, 512}, {3, 256}, {4, 512}, {4, 256}, {5, 512}, {5, 256}};
+// The split variants (4, 5, 6) serve families of at most kSplitMaxFam.
+static const int kTuneCfg[14][2] = {{0, 512}, {0, 256}, {1, 512}, {1, 256}, {2, 512}, {2, 256}, {3, 512},
+                                    {3, 256}, {4, 512}, {4, 256}, {5, 512}, {5, 256}, {6, 512}, {6, 256}};
 
 static bool variant_fits(const ed_op_s* op, int var) {
     const int nf = op->tmode == kModeMerged ? op->nmfam : op->nfam;
@@ -447,7 +457,8 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
 // Kernel configuration the next launch uses: path 2 (compiled shape),
 // 3 (run-time shape); variant 0 (K=4), 1 (K=2 at 8 waves), 2 (K=8),
 // 3 (K=4 pipelined), 4 (two lanes per row, K=8), 5 (four lanes per row,
-// K=4), -1 before tuning or for the run-time kernel.
+// K=4), 6 (two lanes per row, K=4), -1 before tuning or for the run-time
+// kernel.
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     const bool compiled = op->tune_path != 3 && compiled_fits(op);
     path = compiled ? 2 : 3;
@@ -460,6 +471,7 @@ void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
     else if (op->tune_path == 7) var = 3;
     else if (op->tune_path == 9) var = variant_fits(op, 4) ? 4 : 0;
     else if (op->tune_path == 10) var = variant_fits(op, 5) ? 5 : 0;
+    else if (op->tune_path == 11) var = variant_fits(op, 6) ? 6 : 0;
     else if (op->sel_var[dslot] >= 0) {
         var = op->sel_var[dslot];
         bs = op->sel_block[dslot];

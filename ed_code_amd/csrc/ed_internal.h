@@ -141,7 +141,7 @@ struct ed_op_s {
     int tune_path = 0;        // 0 auto, 1 generic walk, 2/4/5 compiled shape K=4 / K=2 / K=8, 3 run-time shape,
                               // 6 two-row 64-bit kernel, 7 compiled shape K=4 pipelined, 8 as 6 with u by
                               // row through the directory, 9/10 compiled shape with rows split over 2 / 4
-                              // lanes
+                              // lanes, 11 as 9 with K=4
     int tune_block = 256;
     bool block_fixed = false; // block set by ed_op_tune
     int sel_var[2] = {-1, -1};   // tuned variant per precision (0 complex128, 1 complex64)

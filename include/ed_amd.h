@@ -116,11 +116,12 @@ void ed_op_destroy(ed_op op);
  * by row through the directory (the layout the kernel falls back to when its
  * wide gather image does not fit in device memory), 9 = as 2 with every row
  * split over two lanes (K=8 each), 10 = as 2 with every row split over four
- * lanes (K=4 each).  Variants 2, 4, 5 and 7 sum a row's elements in one order
- * and give identical results; 9 and 10 sum each lane's share separately and
- * agree with them to rounding (identical among themselves for one lane
- * count).  Paths 9 and 10 fall back to 2 for more than 10 term families.
- * Paths 2-5, 7, 9 and 10 need a lattice whose translations are bit rotations,
+ * lanes (K=4 each), 11 = as 9 with K=4 per lane.  Variants 2, 4, 5 and 7 sum
+ * a row's elements in one order and give identical results; 9, 10 and 11 sum
+ * each lane's share separately and agree with them to rounding (identical
+ * among themselves for one lane count: 9 and 11).  Paths 9-11 fall back to 2
+ * for more than 10 term families.
+ * Paths 2-5, 7 and 9-11 need a lattice whose translations are bit rotations,
  * N <= 32 and no bowtie terms; all but 3 also a built-in shape; 6 and 8 a
  * compiled two-row shape and no bowtie terms.
  * block = threads per block (64..512, multiple of 64; 0 = default, tuned
@@ -128,8 +129,9 @@ void ed_op_destroy(ed_op op);
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
 /* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3, 6
  * or 8 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
- * pipelined, 4 = two lanes per row, 5 = four lanes per row, -1 = not yet
- * tuned or not applicable) and threads per block. */
+ * pipelined, 4 = two lanes per row, 5 = four lanes per row, 6 = two lanes
+ * per row with K=4, -1 = not yet tuned or not applicable) and threads per
+ * block. */
 int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
  * total number of off-diagonal elements (one counting pass on the GPU; no

@@ -145,15 +145,16 @@ def test_full_sector_vs_reference_shv(gpu_ready, name, model, n, k, dim):
     # bit-identical to each other, the split ones (rows over two and four
     # lanes) bit-identical per lane count
     out = {}
-    for path in (2, 4, 5, 7, 9, 10):
+    for path in (2, 4, 5, 7, 9, 10, 11):
         op.tune(path)
         out[path] = _gpu_hv(op, v, torch.complex128)
         assert _relerr(out[path], wr) <= 1e-12, path
         assert _relerr(_gpu_hv(op, v, torch.complex64), wr) <= 1e-6, path
     for path in (4, 5, 7):
         assert np.array_equal(out[path], out[2]), path
+    assert np.array_equal(out[11], out[9])
     for b in (256, 512):
-        for path in (9, 10):
+        for path in (9, 10, 11):
             op.tune(path, b)
             assert np.array_equal(_gpu_hv(op, v, torch.complex128), out[path]), (path, b)
 

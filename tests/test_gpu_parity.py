@@ -208,7 +208,7 @@ def test_gpu_rotation_fast_path_vs_generic(gpu_ready, name, model, n, kx, ky):
     v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
     wo = o.hv(v)
     out = {}
-    for path in (2, 4, 5, 7, 9, 10, 3, 1):
+    for path in (2, 4, 5, 7, 9, 10, 11, 3, 1):
         op.tune(path)
         out[path] = _hv(op, v, torch.complex128)
         assert np.linalg.norm(out[path] - wo) <= 1e-12 * np.linalg.norm(wo), path
@@ -217,8 +217,10 @@ def test_gpu_rotation_fast_path_vs_generic(gpu_ready, name, model, n, kx, ky):
     assert np.array_equal(out[2], out[4]) and np.array_equal(out[2], out[5]) and np.array_equal(out[2], out[7])
     # the split variants (two and four lanes per row) sum each lane's share
     # separately: the same sums to rounding
-    for path in (9, 10):
+    for path in (9, 10, 11):
         assert np.linalg.norm(out[path] - out[2]) <= 1e-14 * np.linalg.norm(out[2]), path
+    # two lanes per row with four or eight elements in flight: the same order
+    assert np.array_equal(out[9], out[11])
 
 
 def test_gpu_autotune_picks_identical_result(gpu_ready):
@@ -237,12 +239,12 @@ def test_gpu_autotune_picks_identical_result(gpu_ready):
     v = torch.randn(basis.D, dtype=torch.complex64, device="cuda", generator=g)
     w_auto = op.hv(v)
     path, var, block = op.tuning("c64")
-    assert path == 2 and var in (0, 1, 2, 3, 4, 5) and block in (256, 512)
+    assert path == 2 and var in (0, 1, 2, 3, 4, 5, 6) and block in (256, 512)
     assert op.tuning("c128")[1] == -1          # tuned per precision
     # variants that sum a row in the same order as the pick: bit-identical;
     # the others (one lane per row against rows split over lanes): to rounding
-    same = {0: (2, 4, 5, 7), 1: (2, 4, 5, 7), 2: (2, 4, 5, 7), 3: (2, 4, 5, 7), 4: (9,), 5: (10,)}[var]
-    for p in (2, 4, 5, 7, 9, 10):
+    same = {0: (2, 4, 5, 7), 1: (2, 4, 5, 7), 2: (2, 4, 5, 7), 3: (2, 4, 5, 7), 4: (9, 11), 5: (10,), 6: (9, 11)}[var]
+    for p in (2, 4, 5, 7, 9, 10, 11):
         for b in (256, 512):
             op.tune(p, b)
             w = op.hv(v)
