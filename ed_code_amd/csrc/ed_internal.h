@@ -21,6 +21,16 @@ int ed_fail(int code, const char* fmt, ...);
 constexpr unsigned kXcdTileRows = 32768;
 __device__ inline unsigned xcd_tile_block_rows(unsigned rows_per_block) {
     const unsigned nb = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    if ((rows_per_block & (rows_per_block - 1u)) == 0u) {
+        // a power of two (every block size of the tuned kernels): the same
+        // mapping with shifts in place of the divisions
+        const unsigned lt = rows_per_block >= kXcdTileRows
+                                ? 0u
+                                : (unsigned)(__builtin_ctz(kXcdTileRows) - __builtin_ctz(rows_per_block));
+        const unsigned full = nb >> (3u + lt);
+        return blockIdx.x < (full << (3u + lt)) ? ((((slot >> lt) << 3) + xcd) << lt) + (slot & ((1u << lt) - 1u))
+                                                 : blockIdx.x;
+    }
     const unsigned T = rows_per_block >= kXcdTileRows ? 1u : kXcdTileRows / rows_per_block;
     const unsigned full = nb / (8 * T);
     return blockIdx.x < full * 8 * T ? ((slot / T) * 8 + xcd) * T + (slot % T) : blockIdx.x;

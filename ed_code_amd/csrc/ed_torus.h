@@ -636,18 +636,39 @@ ED_HD uint64_t split_counts(uint32_t s, const RowTabs& T, int first, int step) {
     return c;
 }
 
+// Lane q's part of the row's diagonal: ezz0 on lane 0 and, in the uniform
+// class, the antiparallel counts of families q, q + step, ... times their
+// Delta J/2 (the lanes' parts are added with the off-diagonal sums).
+template <int MODE>
+ED_HD double split_diag(uint64_t counts, const RowTabs& T, int q, int step, double ezz0) {
+    double e = q == 0 ? ezz0 : 0.0;
+    if (MODE == kModeUniform)
+        for (int f = q; f < T.nfam; f += step) e = fma(-(double)split_count(counts, f), T.fco[f].im, e);
+    return e;
+}
+
+// x without its k lowest set bits (k < popcount(x)): the position of the
+// (k+1)-th set bit by halving the window five times.
+ED_HD uint32_t drop_low_set_bits(uint32_t x, int k) {
+    int p = 0;
+#pragma unroll
+    for (int w = 16; w >= 1; w >>= 1) {
+        const int c = popc32((x >> p) & ((1u << w) - 1u));
+        if (c <= k) {
+            k -= c;
+            p += w;
+        }
+    }
+    return x & ~((1u << p) - 1u);
+}
+
 template <class TC, bool TRIV, int MODE, int K, class JOf, class LoadU, class Gf>
 ED_HD double torus_row_share(uint32_t s, int q, int lpr_log2, uint64_t counts, const RowTabs& T, double ezz0,
                              const RankInfo& R, JOf&& jof, LoadU&& loadu, Gf&& g) {
     int n = 0;
     for (int f = 0; f < T.nfam; f++) n += split_count(counts, f);
     const int st = (q * n) >> lpr_log2, en = ((q + 1) * n) >> lpr_log2;
-    double ezz = 0.0;
-    if (q == 0) {
-        ezz = ezz0;
-        if (MODE == kModeUniform)
-            for (int f = 0; f < T.nfam; f++) ezz = fma(-(double)split_count(counts, f), T.fco[f].im, ezz);
-    }
+    double ezz = split_diag<MODE>(counts, T, q, 1 << lpr_log2, ezz0);
     int left = en - st;
     if (left <= 0) return ezz;
     // the family holding element st, then its (st - cum) earlier members skipped
@@ -655,7 +676,7 @@ ED_HD double torus_row_share(uint32_t s, int q, int lpr_log2, uint64_t counts, c
     int cum = 0;
     while (cum + split_count(counts, c.f) <= st) cum += split_count(counts, c.f++);
     fam_enter<TC, MODE, false>(s, T, c, ezz);
-    for (int k = st - cum; k > 0; k--) c.cur &= c.cur - 1u;
+    if (st > cum) c.cur = drop_low_set_bits(c.cur, st - cum);
     bool more = true;
     while (more) {
         RowBatch<K> B;
