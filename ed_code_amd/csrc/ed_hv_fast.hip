@@ -151,8 +151,9 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int o) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Row body of the split kernels: 2^LPRL consecutive lanes share a row, each
-// taking one consecutive share of its elements (torus_row_share), K in
+// Row body of the split kernels: 2^LPRL lanes share a row (lanes 4 apart, so
+// that each quad of lanes holds four rows), each lane q taking one
+// consecutive share of its elements (torus_row_share), K in
 // flight; the lanes' partial sums meet in a fixed tree (lane q with q ^ 1,
 // then q ^ 2, ...).  A workgroup then covers blockDim / 2^LPRL rows, so the
 // rows in flight on an XCD -- whose targets compete for its L2 -- are fewer
@@ -176,8 +177,12 @@ __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* _
                                            nphase, map);
     const unsigned rpb = blockDim.x >> LPRL;
     const unsigned lb = xcd_tile_block_rows(rpb);
-    const int q = (int)(threadIdx.x & (LPR - 1));
-    const int64_t i = (int64_t)lb * rpb + (threadIdx.x >> LPRL);
+    // lanes 4 apart share a row (lane q of the row in bits 2.., the row in
+    // the others): a quad of lanes, whose gathers the vector L1 coalesces,
+    // holds four consecutive rows at the same share
+    const unsigned t = threadIdx.x;
+    const int q = (int)((t >> 2) & (LPR - 1));
+    const int64_t i = (int64_t)lb * rpb + (((t >> (2 + LPRL)) << 2) | (t & 3u));
     const bool live = i < D;
     uint32_t s = 0;
     double nrm = 0.0;
@@ -189,7 +194,7 @@ __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* _
     // antiparallel count of every family: lane q counts families q, q + LPR, ...
     uint64_t counts = live ? split_counts<TC>(s, T, q, LPR) : 0u;
 #pragma unroll
-    for (int o = 1; o < LPR; o <<= 1) counts |= shfl_xor_u64(counts, o);
+    for (int o = 1; o < LPR; o <<= 1) counts |= shfl_xor_u64(counts, 4 * o);
     double ar = 0.0, ai = 0.0, ez = 0.0;
     auto jof = [&](uint32_t r, bool ok) { return image_index(map, cb, zidx, r, ok); };
     auto loadu = [&](uint32_t j) { return load_c32(u, j); };
@@ -202,11 +207,12 @@ __device__ __forceinline__ void hv_torus_split_body(int64_t D, const uint64_t* _
         }
     };
     if (live) ez = torus_row_share<TC, TRIV, MODE, K>(s, q, LPRL, counts, T, ezz0, R, jof, loadu, acc);
+    // lane q adds lane q ^ 1, then q ^ 2 (lanes 4 and 8 apart)
 #pragma unroll
     for (int o = 1; o < LPR; o <<= 1) {
-        ar += __shfl_xor(ar, o);
-        ai += __shfl_xor(ai, o);
-        ez += __shfl_xor(ez, o);
+        ar += __shfl_xor(ar, 4 * o);
+        ai += __shfl_xor(ai, 4 * o);
+        ez += __shfl_xor(ez, 4 * o);
     }
     if (live && q == 0) {
         const cd2 vi = load_c(v, i);
