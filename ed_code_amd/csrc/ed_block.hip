@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* _
                 ti[q] = foff[f] + popc32(fam[f].S & ((1u << x) - 1u));
                 const uint32_t t = s ^ (1u << x) ^ (1u << ent[ti[q]].y);
                 int cd;
-                const uint32_t rep = torus_rep_c<TC, !TRIV>(t, cd);
+                const uint32_t rep = torus_rep_c<TC, TRIV ? kRepOnly : kRepAny>(t, cd);
                 code[q] = cd;
                 rk[q] = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
             }
@@ -628,7 +628,7 @@ __global__ void __launch_bounds__(256) k_hv_mblock_c(int nsec, const MSec* __res
                 ti[q] = foff[f] + popc32(fam[f].S & ((1u << x) - 1u));
                 const uint32_t tt = s ^ (1u << x) ^ (1u << ent[ti[q]].y);
                 int cd;
-                const uint32_t rep = torus_rep_c<TC, true>(tt, cd);
+                const uint32_t rep = torus_rep_c<TC, kRepAny>(tt, cd);
                 code[q] = cd;
                 rk[q] = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
             }

@@ -299,9 +299,20 @@ ED_HD uint32_t rot_rt(uint32_t s, uint32_t q) {
     return alignbit32(hi, lo, (uint32_t)TC::N - q) & TC::nmask;
 }
 
-// torus_rep for a compile-time shape; same result and phase code.  The
-// candidates are compared top-aligned (cat_top), one instruction each.
-template <class TC, bool TRACK>
+// What torus_rep_c reports besides the representative: nothing, the
+// reference's phase code (the first minimum of its translation walk), or the
+// code of some translation that maps t to its representative.  Two such
+// translations differ by a stabiliser element of the representative; when
+// the representative belongs to the sector every stabiliser element has
+// phase 1, so both codes give the target the same phase (to rounding: the
+// table evaluates each code's own angle), and a target outside the sector
+// reads a zero of the gather image whatever its phase.
+constexpr int kRepOnly = 0, kRepFirst = 1, kRepAny = 2;
+
+// torus_rep for a compile-time shape; same result and, with kRepFirst, the
+// same phase code.  The candidates are compared top-aligned (cat_top), one
+// instruction each.
+template <class TC, int TRACK>
 ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
     constexpr int S = 32 - TC::N;
     uint32_t best = 0xffffffffu;
@@ -343,7 +354,23 @@ ED_HD uint32_t torus_rep_c(uint32_t t, int& code) {
         best = m < best ? m : best;
     }
     const uint32_t rep = best >> S;
-    if (TRACK) {
+    if (TRACK == kRepAny) {
+        // the lowest field rotation bx whose row holds the minimum and the
+        // lowest word rotation b of that row reaching it: the translation
+        // lx = -bx mod L1, ly = -b mod L2
+        int bx = 0;
+        uint32_t sa = fr[0];
+#pragma unroll
+        for (int a = TC::L1 - 1; a >= 0; a--)
+            if (top_eq<TC>(mins[a], best)) { bx = a; sa = fr[a]; }
+        uint32_t hi, lo;
+        cat_top<TC>(sa, hi, lo);
+        int b = 0;
+#pragma unroll
+        for (int y = TC::L2 - 1; y > 0; y--)
+            if (top_eq<TC>(rot_top<TC>(hi, lo, y * TC::W), best)) b = y;
+        code = (bx == 0 ? 0 : TC::L1 - bx) * TC::L2 + (b == 0 ? 0 : TC::L2 - b);
+    } else if (TRACK == kRepFirst) {
         // the reference's first minimum: the lowest field rotation bx whose
         // row holds the minimum, then the lowest step y of the walk's
         // cumulative T2 count, y = (b + bx) mod L2 for the word rotation b
@@ -402,7 +429,7 @@ ED_HD double torus_row_c(uint32_t s, int nfam, const TFam* fam, const TEntry* en
             ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
         }
         int code;
-        uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
+        uint32_t rep = torus_rep_c<TC, TRIV ? kRepOnly : kRepAny>(t, code);
         uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, rtab, (TC::N + 7) / 8);
         int64_t j = dir_lookup(r, dir);
         if (j < 0) continue;
@@ -535,7 +562,7 @@ ED_HD void fill_batch(uint32_t s, const RowTabs& T, const RankInfo& R, RowCur& c
                 ci = (((s >> E.y) & 1u) ? E.cre : -E.cre) * sg;
             }
             int code;
-            uint32_t rep = torus_rep_c<TC, !TRIV>(t, code);
+            uint32_t rep = torus_rep_c<TC, TRIV ? kRepOnly : kRepAny>(t, code);
             uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, T.rtab, (TC::N + 7) / 8);
             if (!TRIV) {
                 // explicit FMAs: every variant rounds the same way

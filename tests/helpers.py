@@ -306,6 +306,9 @@ def hostcore():
         H.hc_hv_torus_c.argtypes = [vp] * 3
         H.hc_torus_rep_c.restype = C.c_int
         H.hc_torus_rep_c.argtypes = [vp, C.c_uint32, vp, vp]
+        H.hc_torus_rep_c_any.restype = C.c_int
+        H.hc_torus_rep_c_any.argtypes = [vp, C.c_uint32, vp, vp]
+        H.hc_phase.argtypes = [vp, C.c_int, vp, vp]
         H.hc_rank_split.restype = C.c_uint64
         H.hc_rank_split.argtypes = [vp, C.c_uint64]
         H.hc_hv_rows2.restype = C.c_int
@@ -401,6 +404,19 @@ class HostCore:
         if hostcore().hc_torus_rep_c(self.h, t, C.addressof(rep), C.addressof(code)):
             return None
         return rep.value, code.value
+
+    def torus_rep_c_any(self, t):
+        """The kernels' search: representative and the code of some
+        minimising translation."""
+        rep, code = C.c_uint32(), C.c_int()
+        if hostcore().hc_torus_rep_c_any(self.h, t, C.addressof(rep), C.addressof(code)):
+            return None
+        return rep.value, code.value
+
+    def phase(self, code):
+        re, im = C.c_double(), C.c_double()
+        hostcore().hc_phase(self.h, code, C.addressof(re), C.addressof(im))
+        return complex(re.value, im.value)
 
     def generic_rep(self, t):
         rep, code = C.c_uint64(), C.c_int()

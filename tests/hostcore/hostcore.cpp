@@ -581,7 +581,7 @@ int hc_hv_torus_s(void* p, const double* v, double* w, int lanes) {
 
 // Compile-time-shape search (TorusC) for the shapes the tests use; returns 1
 // if the lattice has none of them.
-int hc_torus_rep_c(void* p, uint32_t t, uint32_t* rep, int* code) {
+int hc_torus_rep_c_mode(void* p, uint32_t t, uint32_t* rep, int* code, int mode) {
     HC* h = (HC*)p;
     Torus T;
     std::vector<int> a(h->N), b(h->N);
@@ -591,12 +591,29 @@ int hc_torus_rep_c(void* p, uint32_t t, uint32_t* rep, int* code) {
     }
     if (!torus_detect(a, b, T)) return 1;
     int L2 = T.kind == 2 ? 1 : T.L2;
-#define HC_SHAPE(n, wd, l2) \
-    if (T.N == n && T.w == wd && L2 == l2) { *rep = torus_rep_c<TorusC<n, wd, l2>, true>(t, *code); return 0; }
+#define HC_SHAPE(n, wd, l2)                                                                                \
+    if (T.N == n && T.w == wd && L2 == l2) {                                                               \
+        *rep = mode == kRepAny ? torus_rep_c<TorusC<n, wd, l2>, kRepAny>(t, *code)                         \
+                               : torus_rep_c<TorusC<n, wd, l2>, kRepFirst>(t, *code);                      \
+        return 0;                                                                                          \
+    }
     HC_SHAPE(32, 4, 8) HC_SHAPE(30, 6, 5) HC_SHAPE(27, 3, 9) HC_SHAPE(24, 24, 1)
     HC_SHAPE(16, 4, 4) HC_SHAPE(20, 4, 5) HC_SHAPE(18, 6, 3) HC_SHAPE(16, 16, 1)
 #undef HC_SHAPE
     return 1;
+}
+
+// The representative and the reference's phase code (first minimum), or
+// the code of some minimising translation (the kernels' search).
+int hc_torus_rep_c(void* p, uint32_t t, uint32_t* rep, int* code) { return hc_torus_rep_c_mode(p, t, rep, code, kRepFirst); }
+int hc_torus_rep_c_any(void* p, uint32_t t, uint32_t* rep, int* code) {
+    return hc_torus_rep_c_mode(p, t, rep, code, kRepAny);
+}
+// The sector's phase for a translation code.
+void hc_phase(void* p, int code, double* re, double* im) {
+    HC* h = (HC*)p;
+    *re = h->ph[code].re;
+    *im = h->ph[code].im;
 }
 
 // Same through the generic walk of ed_core.h.

@@ -1,6 +1,7 @@
 """The kernel arithmetic (ed_code_amd/csrc/ed_core.h) compiled for the CPU,
 checked against the oracle and the golden fixtures.  These run the same
 functions the gfx950 kernels run, so logic errors surface without a GPU."""
+import itertools
 import math
 import os
 
@@ -192,6 +193,36 @@ def test_compiled_shape_search_matches_generic_walk(model, n, k):
             orbit |= cur
         t = sum(1 << x for x in orbit)
         assert h.torus_rep_c(t) == h.generic_rep(t), (p, q, sorted(orbit))
+
+
+ANYMIN = [(L.square(4, 4), 8, (1, 2)), (L.square(4, 4), 8, (2, 2)), (L.triangular(6, 3), 9, (2, 1)),
+          (L.triangular(6, 3), 9, (3, 0)), (L.square(4, 5), 10, (2, 1)), (L.triangular(4, 4), 8, (2, 0))]
+
+
+@pytest.mark.parametrize("model,n,k", ANYMIN, ids=[f"{m.name}_k{k[0]}{k[1]}" for m, n, k in ANYMIN])
+def test_any_minimising_translation_gives_the_reference_phase(model, n, k):
+    """The kernels take the code of some minimising translation instead of the
+    reference's first minimum.  On every state of the sector's weight: the
+    same representative, and wherever the representative belongs to the
+    sector (the only targets whose phase enters H*v) the same phase as the
+    reference's state_ph from the oracle; the two codes differ on some states
+    (representatives with nontrivial stabilisers)."""
+    o = P.OracleBasis(model, n, *k)
+    tab = o.tables(full=True)
+    rep_loc, sph = tab["rep_loc"], tab["state_ph"]
+    h = Hp.HostCore(model, 1, *k)
+    differ = 0
+    for bits in itertools.combinations(range(model.n_sites), n):
+        t = sum(1 << b for b in bits)
+        r0, c0 = h.torus_rep_c(t)
+        r1, c1 = h.torus_rep_c_any(t)
+        assert r1 == r0
+        differ += c1 != c0
+        q = Hp.cns_rank(t)
+        if rep_loc[q] >= 0:
+            assert abs(h.phase(c1) - sph[q]) <= 1e-14, (t, c0, c1)
+            assert abs(h.phase(c0) - sph[q]) == 0.0, (t, c0)
+    assert differ > 0
 
 
 BATCHED = [(L.square(4, 4, J2=0.5, lam=0.8), 8, (1, 3)), (L.square(4, 4, J2=0.5), 8, (0, 0)),
