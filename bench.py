@@ -118,17 +118,17 @@ def _pmc_run(counter, extra, tag, tmp):
     return [(row["Kernel_Name"], float(row["Counter_Value"])) for row in csv.DictReader(open(files[0]))]
 
 
-def measure_traffic(workload, dtype, kernel_key, tune=None):
-    """HBM-side bytes per launch of the dominant kernel: FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) from two rocprofv3 passes, each scaled by
-    what the same counter reads for a device copy of 2 GiB."""
+def measure_traffic(workload, dtype, kernel_key, tune=None, k="0,0"):
+    """HBM-side bytes per launch of the dominant kernel (sector momentum k):
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) from two rocprofv3 passes,
+    each scaled by what the same counter reads for a device copy of 2 GiB."""
     import shutil
     import statistics
     import tempfile
     if not shutil.which("rocprofv3"):
         return None
     tmp = tempfile.mkdtemp(prefix="edprof_", dir="/tmp")
-    inner = ["--inner", "--workload", workload, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
+    inner = ["--inner", "--workload", workload, "--k", k, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
     if tune is not None:
         inner += ["--tune", f"{tune[0]},{tune[1]}"]
     copy = ["--copy-calibration", "--steps", "2"]
@@ -146,7 +146,7 @@ def measure_traffic(workload, dtype, kernel_key, tune=None):
             "write_scale": fw, "dispatches": len(f_k)}
 
 
-def measure_valu(workload, dtype, kernel_key, tune=None):
+def measure_valu(workload, dtype, kernel_key, tune=None, k="0,0"):
     """VALU wave-instructions per launch of the dominant kernel (one
     rocprofv3 pass of SQ_INSTS_VALU), for the integer-issue bound."""
     import shutil
@@ -155,7 +155,7 @@ def measure_valu(workload, dtype, kernel_key, tune=None):
     if not shutil.which("rocprofv3"):
         return None
     tmp = tempfile.mkdtemp(prefix="edprof_", dir="/tmp")
-    inner = ["--inner", "--workload", workload, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
+    inner = ["--inner", "--workload", workload, "--k", k, "--dtype", dtype, "--steps", "3", "--warmup", "0"]
     if tune is not None:
         inner += ["--tune", f"{tune[0]},{tune[1]}"]
     vals = [v for n, v in _pmc_run("SQ_INSTS_VALU", inner, "valu", tmp) if kernel_key in n]
@@ -357,7 +357,7 @@ def main():
     traffic = None
     if not args.no_traffic:
         try:
-            traffic = measure_traffic(args.workload, main_dt, r["kernel"], r["tune"])
+            traffic = measure_traffic(args.workload, main_dt, r["kernel"], r["tune"], args.k)
             log(f"[bench] traffic {traffic}")
         except Exception as e:  # counters are reported, never required
             log(f"[bench] traffic measurement failed: {e!r}")
@@ -380,7 +380,7 @@ def main():
         log(f"[bench] gather ceiling failed: {e!r}")
     if not args.no_traffic:
         try:
-            insts = measure_valu(args.workload, main_dt, r["kernel"], r["tune"])
+            insts = measure_valu(args.workload, main_dt, r["kernel"], r["tune"], args.k)
             if insts:
                 cus = torch.cuda.get_device_properties(0).multi_processor_count
                 # one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs per CU, 2.4 GHz
