@@ -12,6 +12,9 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "ed_amd.h"
 #include "ed_core.h"
@@ -412,6 +415,22 @@ static bool variant_fits(const ed_op_s* op, int var) {
 }
 static const int64_t kTuneMinRows = int64_t(1) << 20;
 
+// Picks kept for the process: operators of the same compiled shape, term
+// class, momentum class, family count, precision and size class (the bit
+// length of D) take the pick the first of them measured, without timing the
+// candidates again (a server builds an operator per request).
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::pair<int, int>> g_tune_picks;
+
+static TuneKey tune_key(const ed_op_s* op, int dslot) {
+    const ed_basis_s* B = op->b;
+    const int nf = op->tmode == kModeMerged ? op->nmfam : op->nfam;
+    int dlog = 0;
+    for (int64_t d = B->D; d > 1; d >>= 1) dlog++;
+    return TuneKey(B->torus.kind, B->torus.N, B->torus.w, B->torus.L2, op->tmode, B->trivial_k ? 1 : 0, nf, dslot, dlog);
+}
+
 // Chooses the configuration of the compiled-shape kernel for this operator
 // and precision on the first call: below kTuneMinRows rows, or while the
 // stream is being captured, the default (K=4, 512 threads, or the block fixed
@@ -425,9 +444,21 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
     if (op->sel_var[k] >= 0) return ED_OK;
     const int64_t D = op->b->D;
     int best_var = 0, best_bs = op->block_fixed ? op->tune_block : 512;
+    const TuneKey key = tune_key(op, k);
+    if (D >= kTuneMinRows && !op->block_fixed) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_picks.find(key);
+        if (it != g_tune_picks.end()) {
+            op->sel_var[k] = it->second.first;
+            op->sel_block[k] = it->second.second;
+            return ED_OK;
+        }
+    }
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIPCHK(hipStreamIsCapturing(st, &cs));
+    bool timed = false;
     if (D >= kTuneMinRows && cs == hipStreamCaptureStatusNone) {
+        timed = true;
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -457,6 +488,10 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
     }
     op->sel_var[k] = best_var;
     op->sel_block[k] = best_bs;
+    if (timed && !op->block_fixed) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_picks.emplace(key, std::make_pair(best_var, best_bs));
+    }
     return ED_OK;
 }
 

@@ -12,7 +12,7 @@ requests:
   instances: one H*V pass serves all of them);
 * concurrency -- several worker threads, each solve on its own HIP stream,
   so small sectors do not leave the GPU idle between Lanczos steps; a lone
-  lowest-eigenvalue request runs the graph-captured solver (no host round
+  lowest-eigenvalue request runs a graph-captured solver (no host round
   trip per step);
 * HBM admission -- a job starts only when its footprint (basis, Krylov
   vectors, work vectors) fits next to the running jobs under the budget, by
@@ -42,7 +42,7 @@ from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
-from . import ED_C64, ED_LZ_LEAN, ED_LZ_THICK, Basis, BlockOperator, Lattice, Operator, init
+from . import ED_C64, ED_LZ_FULL, ED_LZ_LEAN, ED_LZ_THICK, Basis, BlockOperator, Lattice, Operator, init
 from . import lattices as L
 from .lattices import Model
 
@@ -137,21 +137,32 @@ def estimate_rows(lattice: str, n_up: int) -> int:
     return math.comb(m.n_sites, n_up) // (L1 * L2) + 1
 
 
-# A single request for several eigenvalues runs the thick-restart Lanczos
-# with this many Krylov vectors and at most this many restarts.
-THICK_VECTORS = 60
-THICK_RESTARTS = 40
+# Requests for several eigenvalues run the thick-restart Lanczos with this
+# many Krylov vectors and at most this many restarts (nev = 4 on the 32-site
+# torus: 1.7-1.9 s with 32 vectors, 2.1-2.3 s with 60, 1.9-2.1 s with 24).
+THICK_VECTORS = 32
+THICK_RESTARTS = 60
+# A single request for the lowest eigenvalue keeps its whole complex64
+# Krylov basis (full reorthogonalisation, one H*v per step) when the basis
+# fits in this many bytes, else it runs the three-vector solver, which
+# regenerates the Ritz vector with a second pass of H*v.
+FULL_BASIS_MAX = 64 * GiB
+
+
+def full_basis_fits(rows: int, iterations: int) -> bool:
+    return (iterations + 1) * (rows + 64) * 8 <= FULL_BASIS_MAX
 
 
 def footprint(rows: int, M: int, iterations: int, nev: int) -> int:
     """Device bytes of one job in complex64: Krylov vectors (for a batch the
     block thick restart's complex128 polish phase, its largest; the thick
     restart's complex64 basis and complex128 restart vectors for one request
-    with nev > 1; three vectors for one request with nev = 1), work vectors,
+    with nev > 1; for one request with nev = 1 the whole basis when it fits
+    FULL_BASIS_MAX, else three vectors), work vectors,
     the gather images of both precisions (at most 6 elements per row), FP64
     Ritz vectors, basis, row positions and directory."""
     if M == 1 and nev == 1:
-        krylov = 4 * rows * 8 + rows * 16
+        krylov = ((iterations + 1) * (rows + 64) * 8 if full_basis_fits(rows, iterations) else 4 * rows * 8) + rows * 16
     elif M == 1:
         krylov = (THICK_VECTORS + 1) * rows * 8 + THICK_VECTORS * rows * 16
     else:
@@ -192,7 +203,11 @@ class BasisCache:
 
 def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
     """One job: a single request for the lowest eigenvalue runs the
-    graph-captured three-vector Lanczos, a single request for several the
+    graph-captured full-reorthogonalisation Lanczos with the selective second
+    pass (one H*v per step; on the 32-site torus 0.4-0.9 s against 0.75-1.4 s
+    for the three-vector solver, which needs a second pass of H*v to rebuild
+    the Ritz vector), or that three-vector solver when the basis would exceed
+    FULL_BASIS_MAX; a single request for several the
     thick-restart Lanczos (THICK_VECTORS Krylov vectors; on the 32-site
     torus 2.3-2.5 s for nev = 4 against 6.2-6.8 s for the block solver with
     its complex128 polish), both in complex64 with the FP64 Rayleigh-quotient
@@ -203,8 +218,12 @@ def solve(basis: Basis, reqs: List[Request], iterations: int) -> None:
     models = [model_for(r.lattice, r.J2, r.lam, r.Ch) for r in reqs]
     if len(reqs) == 1 and reqs[0].nev == 1:
         op = Operator(basis, models[0])
-        e, st = op.lanczos(nev=1, iterations=max(iterations, 400), ncycles=1, tol=reqs[0].tol, precision=ED_C64,
-                           mode=ED_LZ_LEAN, graph=True)
+        if full_basis_fits(basis.D, iterations):
+            e, st = op.lanczos(nev=1, iterations=iterations, ncycles=1, tol=reqs[0].tol, precision=ED_C64,
+                               mode=ED_LZ_FULL, selective=True, graph=True)
+        else:
+            e, st = op.lanczos(nev=1, iterations=max(iterations, 400), ncycles=1, tol=reqs[0].tol, precision=ED_C64,
+                               mode=ED_LZ_LEAN, graph=True)
         ev = np.array([[e[0]]])
         err = np.array([[st["eval_error"][0]]])
     elif len(reqs) == 1 and basis.D > THICK_VECTORS:

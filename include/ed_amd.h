@@ -107,7 +107,10 @@ void ed_op_destroy(ed_op op);
  * compare the paths and by benchmarks): path 0 = automatic (rotation fast path
  * when available; for the built-in lattice shapes the first H*v of each
  * precision on a sector of at least 2^20 rows times the kernel variants and
- * block sizes and keeps the fastest), 1 = generic translation walk,
+ * block sizes and keeps the fastest; the pick is kept for the process, and
+ * later operators of the same shape, term class, momentum class, family
+ * count, precision and size class take it without timing), 1 = generic
+ * translation walk,
  * 2 = rotation fast path with the lattice shape compiled in (variant K=4
  * elements in flight), 3 = rotation fast path with the shape as run-time
  * parameters, 4 = as 2 with K=2 at 8 waves per SIMD, 5 = as 2 with K=8,
