@@ -9,10 +9,11 @@ the M instances share every H*V pass (ed_block_hv) and one Lanczos loop
 
     python -m ed_code_amd.sweep --lattice tri6x5 --instances 4 --nev 2 --tol 1e-8
 
-By default each sector runs the block thick-restart Lanczos (24 Krylov
-vectors per instance, the nev pairs from one space); --thick 0 runs one
-full-reorthogonalisation Lanczos per eigenvalue with deflation instead
-(30-site triangular torus, 30 sectors x 4 couplings: 44 s against 65 s).
+By default each sector runs the block thick-restart Lanczos (16 Krylov
+vectors per instance, at least 4 nev + 8; the nev pairs from one space);
+--thick 0 runs one full-reorthogonalisation Lanczos per eigenvalue with
+deflation instead (30-site triangular torus, 30 sectors x 4 couplings,
+nev = 2: 31 s with 16 vectors, 33 s with 24, 35 s with 32).
 """
 from __future__ import annotations
 
@@ -54,8 +55,8 @@ def sweep_sectors(models: Sequence[Model], n_up: int, nev: int = 2, tol: float =
                   log: Optional[Callable[[str], None]] = None, thick: int = 0) -> dict:
     """Lowest nev eigenvalues of every instance in every momentum sector
     (or the listed (kx, ky) sectors).  thick > 0: the block thick-restart
-    Lanczos with that many Krylov vectors per instance (all nev pairs from
-    one space); 0: one full-reorthogonalisation run of up to `iterations`
+    Lanczos with that many Krylov vectors per instance, at least 4 nev + 8
+    (all nev pairs from one space); 0: one full-reorthogonalisation run of up to `iterations`
     steps per eigenvalue.  Returns per-sector results and totals."""
     lat = Lattice.of(models[0])
     if sectors is None:
@@ -69,8 +70,8 @@ def sweep_sectors(models: Sequence[Model], n_up: int, nev: int = 2, tol: float =
             continue
         blk = BlockOperator(basis, models)
         if thick > 0:
-            ev, st = blk.lanczos(nev=nev, iterations=thick, ncycles=max(1, iterations), tol=tol, seed=seed,
-                                 precision=precision, mode=ED_LZ_THICK)
+            ev, st = blk.lanczos(nev=nev, iterations=max(thick, 4 * nev + 8), ncycles=max(1, iterations), tol=tol,
+                                 seed=seed, precision=precision, mode=ED_LZ_THICK)
         else:
             ev, st = blk.lanczos(nev=nev, iterations=iterations, tol=tol, seed=seed, precision=precision)
         dt = time.perf_counter() - t0
@@ -181,8 +182,9 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["c64", "c128"], default="c64")
     ap.add_argument("--sectors", type=int, default=0, help="only the first n sectors (0 = all)")
     ap.add_argument("--compare", type=int, default=0, help="also run the one-at-a-time loop on n sectors")
-    ap.add_argument("--thick", type=int, default=24,
-                    help="block thick-restart Lanczos with this many Krylov vectors per instance, at most "
+    ap.add_argument("--thick", type=int, default=16,
+                    help="block thick-restart Lanczos with this many Krylov vectors per instance (at least "
+                         "4 nev + 8), at most "
                          "--iterations restarts (0: one full-reorthogonalisation run of up to --iterations "
                          "steps per eigenvalue)")
     ap.add_argument("--multi", action="store_true",
