@@ -423,6 +423,12 @@ using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
 static std::mutex g_tune_mu;
 static std::map<TuneKey, std::pair<int, int>> g_tune_picks;
 
+extern "C" int ed_tune_cache_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_picks.clear();
+    return ED_OK;
+}
+
 static TuneKey tune_key(const ed_op_s* op, int dslot) {
     const ed_basis_s* B = op->b;
     const int nf = op->tmode == kModeMerged ? op->nmfam : op->nfam;

@@ -136,6 +136,10 @@ int ed_op_tune(ed_op op, int32_t path, int32_t block);
  * per row with K=4, -1 = not yet tuned or not applicable) and threads per
  * block. */
 int ed_op_tuning(ed_op op, int32_t dtype, int32_t* path, int32_t* variant, int32_t* block);
+/* Forgets the autotuner picks kept for the process (no reference
+ * counterpart): later operators time their candidates again.  Operators
+ * already tuned keep their pick. */
+int ed_tune_cache_clear(void);
 /* Number of distinct nonzero off-diagonal (row, target) pairs, G, and the
  * total number of off-diagonal elements (one counting pass on the GPU; no
  * reference counterpart: the G of the algorithmic-bytes model). */
