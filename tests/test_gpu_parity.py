@@ -258,6 +258,38 @@ def test_gpu_autotune_picks_identical_result(gpu_ready):
     assert op.tuning("c64")[2] == 256
 
 
+def test_gpu_autotune_picks_kept_for_the_process(gpu_ready):
+    """A second operator of the same shape, class and sector size takes the
+    pick the first one measured (no timing: its first H*v is bit-identical
+    to the first operator's); after ed_tune_cache_clear a new operator times
+    the candidates again and keeps one of them."""
+    import torch
+    ed = _ed()
+    from ed_code_amd._lib import lib
+    m = L.triangular(6, 5, Ch=0.07)
+    print("[case] tri6x5 autotune cache", flush=True)
+    basis = ed.Basis(ed.Lattice.of(m), 15, 1, 0)
+    assert basis.D > 1 << 20
+    lib().ed_tune_cache_clear()
+    g = torch.Generator(device="cuda").manual_seed(5)
+    v = torch.randn(basis.D, dtype=torch.complex64, device="cuda", generator=g)
+    a = ed.Operator(basis, m)
+    wa = a.hv(v)
+    pick = a.tuning("c64")
+    assert pick[0] == 2 and pick[1] >= 0
+    b = ed.Operator(basis, m)
+    assert b.tuning("c64")[1] == -1
+    wb = b.hv(v)
+    assert b.tuning("c64") == pick
+    assert torch.equal(wa, wb)
+    lib().ed_tune_cache_clear()
+    c = ed.Operator(basis, m)
+    wc = c.hv(v)
+    p2 = c.tuning("c64")
+    assert p2[0] == 2 and p2[1] in (0, 1, 2, 3, 4, 5, 6) and p2[2] in (256, 512)
+    assert torch.linalg.norm(wc - wa) <= 1e-6 * torch.linalg.norm(wa)
+
+
 TWO_ROW = [("tilted20_k00", (4, 2), (-2, 4), 10, (0, 0)), ("tilted20_k71", (4, 2), (-2, 4), 10, (7, 1)),
            ("tilted20_n9_k31", (4, 2), (-2, 4), 9, (3, 1))]
 
