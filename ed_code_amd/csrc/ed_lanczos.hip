@@ -192,6 +192,42 @@ __global__ void __launch_bounds__(kRedThreads) k_accumulate(int64_t D, const XT*
     }
 }
 
+// Thick-restart change of basis in one pass over the Krylov vectors: for the
+// outputs k0 <= k < k0 + ko (ko <= KO), W_k[e] = sum_{l < nl} t[(l kk + k) M
+// + m] X_l[e], m = threadIdx.x % M the instance of element e (the grid stride
+// is a multiple of M), summed in FP64 in the order l = 0, 1, ...; W_k = W +
+// (k - k0) ldw.  W may be X itself when one launch writes every output: a
+// thread reads all inputs of an element before it writes that element.
+template <int KO, typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_restart(int64_t n, int M, const VT* X, int64_t ldx, int nl,
+                                                          const double* __restrict__ t, int kk, int k0, int ko, VT* W,
+                                                          int64_t ldw) {
+    extern __shared__ double tsm[];   // [nl][KO][M]
+    for (int q = threadIdx.x; q < nl * KO * M; q += blockDim.x) {
+        const int l = q / (KO * M), r = q % (KO * M), k = r / M, m = r % M;
+        tsm[q] = k < ko ? t[((int64_t)l * kk + k0 + k) * M + m] : 0.0;
+    }
+    __syncthreads();
+    const int m = (int)(threadIdx.x % M);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        double ar[KO], ai[KO];
+#pragma unroll
+        for (int k = 0; k < KO; k++) { ar[k] = 0.0; ai[k] = 0.0; }
+        for (int l = 0; l < nl; l++) {
+            const cd2 x = ldn(X + (int64_t)l * ldx, e);
+            const double* tl = tsm + l * KO * M + m;
+#pragma unroll
+            for (int k = 0; k < KO; k++) {
+                ar[k] = fma(tl[k * M], x.re, ar[k]);
+                ai[k] = fma(tl[k * M], x.im, ai[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KO; k++)
+            if (k < ko) st(W + (int64_t)k * ldw, e, ar[k], ai[k]);
+    }
+}
+
 struct Ctx {
     ed_op_s* op;
     int64_t D;
@@ -215,6 +251,41 @@ unsigned grid_for(int64_t D) { return (unsigned)std::min<int64_t>(kRedBlocks, (D
 // for its own store (stores and loads share one counter) before the loads of
 // its next element; the results are the same either way.
 unsigned grid_stream(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kRedThreads - 1) / kRedThreads); }
+
+// V_k <- sum_{l < nl} t[(l kk + k) M + m] V_l for k < kk (m the instance of
+// each element, M of them interleaved; M = 1 for one instance): the kept
+// Ritz vectors of a thick restart.  t has nl * kk * M entries (zeros past an
+// instance's basis); d_t holds at least that many doubles, scratch at least
+// kk * ldv elements.  One pass over the basis writes up to 16 outputs, in
+// place when they are all of them.
+template <typename VT>
+int restart_basis(hipStream_t st, int64_t n, int M, VT* V, int64_t ldv, int nl, const std::vector<double>& t, int kk,
+                  double* d_t, VT* scratch) {
+    if (kk <= 0) return ED_OK;
+    HIPCHK(hipMemcpyAsync(d_t, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    const bool wide = (size_t)nl * 16 * M * sizeof(double) <= 48 * 1024;
+    const int ko_max = wide ? 16 : 8;
+    const size_t sm = (size_t)nl * ko_max * M * sizeof(double);
+    if (sm > 64 * 1024) return ed_fail(ED_EINVAL, "restart coefficients exceed the LDS (%d vectors, %d instances)", nl, M);
+    const unsigned g = grid_for(n);
+    const bool inplace = kk <= ko_max;
+    for (int k0 = 0; k0 < kk; k0 += ko_max) {
+        const int ko = std::min(ko_max, kk - k0);
+        VT* out = inplace ? V : scratch + (int64_t)k0 * ldv;
+        if (wide)
+            hipLaunchKernelGGL((k_restart<16, VT>), dim3(g), dim3(kRedThreads), sm, st, n, M, V, ldv, nl, d_t, kk, k0, ko,
+                               out, ldv);
+        else
+            hipLaunchKernelGGL((k_restart<8, VT>), dim3(g), dim3(kRedThreads), sm, st, n, M, V, ldv, nl, d_t, kk, k0, ko,
+                               out, ldv);
+        HIPCHK(hipGetLastError());
+    }
+    if (!inplace) HIPCHK(hipMemcpyAsync(V, scratch, (size_t)kk * ldv * sizeof(VT), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return ED_OK;
+}
+// Whether restart_basis can stage nl vectors' coefficients for M instances.
+inline bool restart_fits(int nl, int M) { return (size_t)nl * 8 * M * sizeof(double) <= 64 * 1024; }
 
 
 template <typename XT, typename YT, bool PAIR>
@@ -575,14 +646,26 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
     unsigned g = grid_for(D);
     VT *V = nullptr, *w = nullptr;
     cd2* W = nullptr;
-    HIPCHK(hipMalloc(&V, (size_t)(m + 1) * D * sizeof(VT)));
-    HIPCHK(hipMalloc(&w, D * sizeof(VT)));
-    HIPCHK(hipMalloc(&W, (size_t)m * D * sizeof(cd2)));
+    double* Tr = nullptr;   // restart coefficients
+    auto cleanup = [&]() { dfree(V); dfree(w); dfree(W); dfree(Tr); };
+#define TRC1(expr)                 \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != ED_OK) {        \
+            cleanup();             \
+            return rc_;            \
+        }                          \
+    } while (0)
+#define THIP1(expr) TRC1((expr) == hipSuccess ? ED_OK : ed_fail(ED_EDEVICE, "%s failed", #expr))
+    THIP1(hipMalloc(&V, (size_t)(m + 1) * D * sizeof(VT)));
+    THIP1(hipMalloc(&w, D * sizeof(VT)));
+    THIP1(hipMalloc(&W, (size_t)m * D * sizeof(cd2)));
+    THIP1(hipMalloc(&Tr, (size_t)m * m * sizeof(double)));
     auto Vk = [&](int k) { return V + (int64_t)k * D; };
     std::vector<std::complex<double>> q, q2;
     hipLaunchKernelGGL(k_start<VT>, dim3((unsigned)((D + 255) / 256)), dim3(256), 0, c.st, D, p.seed, Vk(0));
     double nn;
-    RC(dnorm2(c, Vk(0), nn));
+    TRC1(dnorm2(c, Vk(0), nn));
     hipLaunchKernelGGL((k_scale_copy<VT, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), Vk(0), Vk(0));
     std::vector<double> T((size_t)m * m, 0.0), theta, Y;
     int start = 0, meff = m, ncyc = 0;
@@ -591,32 +674,32 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
     while (!done) {
         meff = m;
         for (int j = start; j < m; j++) {
-            RC(hv<VT>(c, Vk(j), w));
+            TRC1(hv<VT>(c, Vk(j), w));
             // the known part of H v_j first: alpha_j v_j and beta_{j-1} v_{j-1},
             // or after a restart the couplings to the kept Ritz vectors
-            RC((multidot<VT, VT>(c, 1, Vk(j), 0, w, q)));
+            TRC1((multidot<VT, VT>(c, 1, Vk(j), 0, w, q)));
             double alpha = q[0].real();
             const int lo = j > start ? j - 1 : 0;
             std::vector<std::complex<double>> cf((size_t)(j + 1 - lo));
             for (int l = lo; l < j; l++) cf[l - lo] = T[(size_t)l * m + j];
             cf[j - lo] = alpha;
-            RC((multiaxpy<VT, VT>(c, j + 1 - lo, Vk(lo), D, cf, 1.0, w)));
+            TRC1((multiaxpy<VT, VT>(c, j + 1 - lo, Vk(lo), D, cf, 1.0, w)));
             // Gram-Schmidt against every kept and new vector; the second pass
             // only when the first removed more than 1 - 1/sqrt(2) of the norm
             double nb2, bb = 0.0;
             bool have_bb = false;
-            RC(dnorm2(c, w, nb2));
+            TRC1(dnorm2(c, w, nb2));
             for (int pass = 0; pass < 2; pass++) {
                 if (pass == 1) {
-                    RC(dnorm2(c, w, bb));
+                    TRC1(dnorm2(c, w, bb));
                     if (bb > 0.5 * nb2) { have_bb = true; break; }
                 }
-                RC((multidot<VT, VT>(c, j + 1, V, D, w, q2)));
+                TRC1((multidot<VT, VT>(c, j + 1, V, D, w, q2)));
                 alpha += q2[j].real();
-                RC((multiaxpy<VT, VT>(c, j + 1, V, D, q2, 1.0, w)));
+                TRC1((multiaxpy<VT, VT>(c, j + 1, V, D, q2, 1.0, w)));
             }
             T[(size_t)j * m + j] = alpha;
-            if (!have_bb) RC(dnorm2(c, w, bb));
+            if (!have_bb) TRC1(dnorm2(c, w, bb));
             const double beta = std::sqrt(bb);
             if (beta < 1e-13 * std::max(1.0, std::fabs(T[(size_t)j * m + j]))) {   // invariant subspace
                 meff = j + 1;
@@ -633,7 +716,7 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
         std::vector<double> Tm((size_t)meff * meff);
         for (int i = 0; i < meff; i++)
             for (int k = 0; k < meff; k++) Tm[(size_t)i * meff + k] = T[(size_t)i * m + k];
-        if (!symeig_jacobi(Tm, meff, theta, Y)) return ed_fail(ED_ENOTCONV, "Rayleigh-Ritz eigensolver did not converge");
+        if (!symeig_jacobi(Tm, meff, theta, Y)) TRC1(ed_fail(ED_ENOTCONV, "Rayleigh-Ritz eigensolver did not converge"));
         bool conv = true;
         for (int i = 0; i < std::min(nev, meff); i++)
             if (beta_m * std::fabs(Y[(meff - 1) + (size_t)i * meff]) > p.tol * std::max(1.0, std::fabs(theta[i])))
@@ -645,16 +728,25 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
         }
         // restart with the kk lowest Ritz vectors and the residual
         const int kk = std::min(nev + (meff - nev) / 2, meff - 1);
-        std::vector<double> y(meff);
-        for (int i = 0; i < kk; i++) {
-            cd2* Wi = W + (int64_t)i * D;
-            HIPCHK(hipMemsetAsync(Wi, 0, D * sizeof(cd2), c.st));
-            for (int l = 0; l < meff; l++) y[l] = Y[l + (size_t)i * meff];
-            RC((accumulate<VT>(c, meff, V, D, y, Wi)));
+        if (restart_fits(meff, 1)) {
+            // the kept Ritz vectors in one pass over the basis
+            std::vector<double> tr((size_t)meff * kk);
+            for (int l = 0; l < meff; l++)
+                for (int i = 0; i < kk; i++) tr[(size_t)l * kk + i] = Y[l + (size_t)i * meff];
+            TRC1((restart_basis<VT>(c.st, D, 1, V, D, meff, tr, kk, Tr, (VT*)W)));
+        } else {
+            std::vector<double> y(meff);
+            for (int i = 0; i < kk; i++) {
+                cd2* Wi = W + (int64_t)i * D;
+                THIP1(hipMemsetAsync(Wi, 0, D * sizeof(cd2), c.st));
+                for (int l = 0; l < meff; l++) y[l] = Y[l + (size_t)i * meff];
+                TRC1((accumulate<VT>(c, meff, V, D, y, Wi)));
+            }
+            for (int i = 0; i < kk; i++)
+                hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, W + (int64_t)i * D,
+                                   Vk(i));
         }
-        for (int i = 0; i < kk; i++)
-            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0, W + (int64_t)i * D, Vk(i));
-        HIPCHK(hipMemcpyAsync(Vk(kk), Vk(m), D * sizeof(VT), hipMemcpyDeviceToDevice, c.st));
+        THIP1(hipMemcpyAsync(Vk(kk), Vk(m), D * sizeof(VT), hipMemcpyDeviceToDevice, c.st));
         std::fill(T.begin(), T.end(), 0.0);
         for (int i = 0; i < kk; i++) {
             T[(size_t)i * m + i] = theta[i];
@@ -666,18 +758,20 @@ int run_trl(Ctx& c, const ed_lanczos_params& p, std::vector<cd2*>& prev, double*
     std::vector<double> y(meff);
     for (int i = 0; i < std::min(nev, meff); i++) {
         cd2* x = nullptr;
-        HIPCHK(hipMalloc(&x, D * sizeof(cd2)));
-        HIPCHK(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
+        THIP1(hipMalloc(&x, D * sizeof(cd2)));
+        THIP1(hipMemsetAsync(x, 0, D * sizeof(cd2), c.st));
         for (int l = 0; l < meff; l++) y[l] = Y[l + (size_t)i * meff];
-        RC((accumulate<VT>(c, meff, V, D, y, x)));
-        RC(dnorm2(c, x, nn));
+        TRC1((accumulate<VT>(c, meff, V, D, y, x)));
+        TRC1(dnorm2(c, x, nn));
         hipLaunchKernelGGL((k_scale_copy<cd2, cd2>), dim3(g), dim3(kRedThreads), 0, c.st, D, 1.0 / std::sqrt(nn), x, x);
         prev.push_back(x);
         evals[i] = theta[i];
     }
-    HIPCHK(hipStreamSynchronize(c.st));
-    dfree(V); dfree(w); dfree(W);
+    THIP1(hipStreamSynchronize(c.st));
+    cleanup();
     return ED_OK;
+#undef TRC1
+#undef THIP1
 }
 
 // Memory-lean Lanczos: three Krylov vectors, no reorthogonalisation, lowest
@@ -1502,7 +1596,8 @@ int run_block_trl(BCtx& c, const ed_lanczos_params& p, double tol, const std::ve
     const int kk = std::max(nev, std::min(nev + (m - nev) / 2, m - 1));
     VT *V = nullptr, *w = nullptr;
     cd2* W = nullptr;
-    auto cleanup = [&]() { dfree(V); dfree(w); dfree(W); };
+    double* Tr = nullptr;   // restart coefficients
+    auto cleanup = [&]() { dfree(V); dfree(w); dfree(W); dfree(Tr); };
 #define TRC(expr)                  \
     do {                           \
         int rc_ = (expr);          \
@@ -1515,6 +1610,7 @@ int run_block_trl(BCtx& c, const ed_lanczos_params& p, double tol, const std::ve
     THIP(hipMalloc(&V, (size_t)(m + 1) * n * sizeof(VT)));
     THIP(hipMalloc(&w, n * sizeof(VT)));
     THIP(hipMalloc(&W, (size_t)std::max(kk, nev) * n * sizeof(cd2)));
+    THIP(hipMalloc(&Tr, (size_t)m * kk * M * sizeof(double)));
     auto Vk = [&](int k) { return V + (int64_t)k * n; };
     std::vector<std::complex<double>> q, q2;
     std::vector<double> nn, sc(M);
@@ -1610,19 +1706,31 @@ int run_block_trl(BCtx& c, const ed_lanczos_params& p, double tol, const std::ve
         // restart: W_k = sum_l Y[l, k] V_l per instance (k < kk; zero past an
         // exhausted instance's size), then V_k <- W_k and V_kk <- V_m
         const int kmax = *std::max_element(meff.begin(), meff.end());
-        std::vector<double> t((size_t)kmax * M);
-        for (int k = 0; k < kk; k++) {
-            std::fill(t.begin(), t.end(), 0.0);
-            for (int i = 0; i < M; i++)
-                if (k < meff[i])
-                    for (int l = 0; l < meff[i]; l++) t[(size_t)l * M + i] = Y[i][l + (size_t)k * meff[i]];
-            cd2* Wk = W + (int64_t)k * n;
-            THIP(hipMemsetAsync(Wk, 0, n * sizeof(cd2), c.st));
-            TRC(baccumulate<VT>(c, kmax, V, n, t, Wk));
+        if (restart_fits(kmax, M)) {
+            // the kept Ritz vectors of every instance in one pass over the basis
+            std::vector<double> tr((size_t)kmax * kk * M, 0.0);
+            for (int k = 0; k < kk; k++)
+                for (int i = 0; i < M; i++)
+                    if (k < meff[i])
+                        for (int l = 0; l < meff[i]; l++)
+                            tr[((size_t)l * kk + k) * M + i] = Y[i][l + (size_t)k * meff[i]];
+            TRC((restart_basis<VT>(c.st, n, M, V, n, kmax, tr, kk, Tr, (VT*)W)));
+        } else {
+            std::vector<double> t((size_t)kmax * M);
+            for (int k = 0; k < kk; k++) {
+                std::fill(t.begin(), t.end(), 0.0);
+                for (int i = 0; i < M; i++)
+                    if (k < meff[i])
+                        for (int l = 0; l < meff[i]; l++) t[(size_t)l * M + i] = Y[i][l + (size_t)k * meff[i]];
+                cd2* Wk = W + (int64_t)k * n;
+                THIP(hipMemsetAsync(Wk, 0, n * sizeof(cd2), c.st));
+                TRC(baccumulate<VT>(c, kmax, V, n, t, Wk));
+            }
+            for (int k = 0; k < kk; k++)
+                hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(c.g), dim3(kRedThreads), 0, c.st, n, 1.0,
+                                   W + (int64_t)k * n, Vk(k));
+            THIP(hipGetLastError());
         }
-        for (int k = 0; k < kk; k++)
-            hipLaunchKernelGGL((k_scale_copy<cd2, VT>), dim3(c.g), dim3(kRedThreads), 0, c.st, n, 1.0, W + (int64_t)k * n, Vk(k));
-        THIP(hipGetLastError());
         THIP(hipMemcpyAsync(Vk(kk), Vk(m), n * sizeof(VT), hipMemcpyDeviceToDevice, c.st));
         for (int i = 0; i < M; i++) {
             std::vector<double>& Ti = T[i];
