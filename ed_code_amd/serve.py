@@ -140,7 +140,7 @@ def estimate_rows(lattice: str, n_up: int) -> int:
 
 # Requests for several eigenvalues run the thick-restart Lanczos with this
 # many Krylov vectors and at most this many restarts (nev = 4 on the 32-site
-# torus: 1.7-1.9 s with 32 vectors, 2.1-2.3 s with 60, 1.9-2.1 s with 24).
+# torus: 1.45-1.65 s with 32 vectors, 1.75-1.85 s with 60, 1.4-1.9 s with 24).
 THICK_VECTORS = 32
 THICK_RESTARTS = 60
 # A single request for the lowest eigenvalue keeps its whole complex64
