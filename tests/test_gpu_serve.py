@@ -13,7 +13,10 @@ def test_serving_loop_matches_plain_loop_and_dense(gpu_ready):
     from ed_code_amd import serve as S
     ed.init(0)
     print("[case] serve chain16/tri4x4 k=0 burst", flush=True)
-    reqs = S.make_requests(12, rate=200.0, seed=3, lattices=["chain16", "tri4x4"], k_zero=True)
+    # a burst: all 12 requests arrive within microseconds, so the ones the
+    # first dispatch does not take are pending together, and 12 requests over
+    # at most 8 batch keys (lattice, Sz, nev at k=0) must share one
+    reqs = S.make_requests(12, rate=1e6, seed=3, lattices=["chain16", "tri4x4"], k_zero=True)
     srv = S.Server(workers=3, iterations=200)
     t0, wall = srv.run(reqs)
     assert not srv.errors
@@ -21,7 +24,7 @@ def test_serving_loop_matches_plain_loop_and_dense(gpu_ready):
     assert out["answered"] == len(reqs)
     assert max(out["batch_sizes"]) >= 2          # a burst at k=0 batches
     assert srv.cache.hits >= 1
-    base = S.make_requests(12, rate=200.0, seed=3, lattices=["chain16", "tri4x4"], k_zero=True)
+    base = S.make_requests(12, rate=1e6, seed=3, lattices=["chain16", "tri4x4"], k_zero=True)
     S.run_baseline(base, iterations=200)
     for r, b in zip(reqs, base):
         assert r.rid == b.rid
