@@ -87,7 +87,7 @@ __device__ __forceinline__ void st_c(cf2* p, double re, double im) { *p = cf2{(f
 // single-instance rotation kernel (torus_row_batched).  Terms are numbered
 // densely: term = foff[f] + (number of sites of family f below x).
 template <class TC, bool TRIV, bool MG, int M, int K, typename VT>
-__global__ void __launch_bounds__(256) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+__global__ void __launch_bounds__(M <= 4 ? 512 : 256) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                     const TFam* __restrict__ gfam, const int* __restrict__ gfoff,
                                                     int nterm, const BEnt* __restrict__ gent,
                                                     const BCoef* __restrict__ gcoef, const BChir* __restrict__ gchir,
@@ -273,7 +273,9 @@ template <class TC, int M, typename VT>
 void launch_block_c(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t st) {
     ed_basis_s* B = K->b;
     constexpr int KB = M >= 4 ? 2 : 4;
-    const unsigned bs = 256;
+    // the tables in LDS (up to 64 KiB) are one copy per workgroup: 512
+    // threads put twice the waves on a CU (M <= 4 stays within 128 VGPRs)
+    const unsigned bs = M <= 4 ? 512 : 256;
     dim3 grid(nblk(B->D, bs)), blk(bs);
     size_t sm = block_smem(K);
     int nph = B->L1 * B->L2;
