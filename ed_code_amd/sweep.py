@@ -13,7 +13,7 @@ By default each sector runs the block thick-restart Lanczos (16 Krylov
 vectors per instance, at least 4 nev + 8; the nev pairs from one space);
 --thick 0 runs one full-reorthogonalisation Lanczos per eigenvalue with
 deflation instead (30-site triangular torus, 30 sectors x 4 couplings,
-nev = 2: 26 s with 16 vectors).
+nev = 2: 23 s with 16 vectors).
 """
 from __future__ import annotations
 
