@@ -87,7 +87,7 @@ __device__ __forceinline__ void st_c(cf2* p, double re, double im) { *p = cf2{(f
 // single-instance rotation kernel (torus_row_batched).  Terms are numbered
 // densely: term = foff[f] + (number of sites of family f below x).
 template <class TC, bool TRIV, bool MG, int M, int K, typename VT>
-__global__ void __launch_bounds__(M <= 4 ? 512 : 256) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+__global__ void __launch_bounds__(512, M >= 8 ? 4 : 2) k_hv_block_c(int64_t D, const uint64_t* __restrict__ rows, int nfam,
                                                     const TFam* __restrict__ gfam, const int* __restrict__ gfoff,
                                                     int nterm, const BEnt* __restrict__ gent,
                                                     const BCoef* __restrict__ gcoef, const BChir* __restrict__ gchir,
@@ -272,10 +272,11 @@ size_t block_smem(const ed_block_s* K) {
 template <class TC, int M, typename VT>
 void launch_block_c(ed_block_s* K, const VT* u, const VT* v, VT* w, hipStream_t st) {
     ed_basis_s* B = K->b;
-    constexpr int KB = M >= 4 ? 2 : 4;
-    // the tables in LDS (up to 64 KiB) are one copy per workgroup: 512
-    // threads put twice the waves on a CU (M <= 4 stays within 128 VGPRs)
-    const unsigned bs = M <= 4 ? 512 : 256;
+    constexpr int KB = M >= 8 ? 1 : M >= 4 ? 2 : 4;
+    // the tables in LDS are one copy per workgroup: for M <= 4 (59 KB with
+    // chiral terms, within 128 VGPRs) 512 threads put twice the waves on a
+    // CU; M = 8 needs 156-198 VGPRs, so a CU holds 2 waves per SIMD either way
+    const unsigned bs = 512;
     dim3 grid(nblk(B->D, bs)), blk(bs);
     size_t sm = block_smem(K);
     int nph = B->L1 * B->L2;
@@ -486,7 +487,7 @@ extern "C" int ed_block_create(ed_basis b, int32_t M, const ed_terms* terms, ed_
                 ed_block_destroy(K);
                 return rc;
             }
-            K->fast = block_smem(K) <= 64 * 1024;
+            K->fast = block_smem(K) <= lds_per_block();
         }
     }
     *out = K;
@@ -891,7 +892,7 @@ extern "C" int ed_mblock_create(const ed_basis* bases, int32_t nsec, int32_t M, 
         tiles += (B->D + kMTile - 1) / kMTile;
         img += (int64_t)(B->img_nslot + 1) << B->img_cb;
     }
-    if (fast && tiles > 0 && tiles < (int64_t(1) << 31) && mblock_smem(MB) <= 64 * 1024) {
+    if (fast && tiles > 0 && tiles < (int64_t(1) << 31) && mblock_smem(MB) <= lds_per_block()) {
         // sectors with no rows take no tile; the search over first tiles
         // needs them strictly ordered, so they stay out of the queue
         for (size_t q = 0; q < MB->secs.size(); q++)

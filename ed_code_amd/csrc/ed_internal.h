@@ -60,6 +60,18 @@ inline hipError_t scopy(void* dst, const void* src, size_t n, hipMemcpyKind kind
 inline hipError_t szero(void* p, size_t n) { return hipMemsetAsync(p, 0, n, setup_stream()); }
 inline hipError_t ssync() { return hipStreamSynchronize(setup_stream()); }
 
+// LDS one workgroup may allocate on the current device: a whole CU's
+// 160 KiB on gfx950; 64 KiB, the size of earlier CDNA parts, if the query fails
+inline size_t lds_per_block() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+        (void)hipGetLastError();
+        return 64 * 1024;
+    }
+    return (size_t)v;
+}
+
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
         hipError_t e_ = (x);                                                                       \

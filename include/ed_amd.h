@@ -325,8 +325,9 @@ int ed_lanczos(ed_op op, const ed_lanczos_params* p, double* evals, double* evec
  * gather addressing of every gathered element.  The term sets must have the same
  * bond, chiral and bowtie site lists (same order); J, Jz, Ch, ChCh and phases
  * may differ per instance.  M in 1..8; M in {2, 4, 8} on a built-in torus
- * shape without bowtie terms runs the shared kernel, other cases apply the
- * instances one by one. */
+ * shape without bowtie terms, whose coefficient and rank tables fit the LDS
+ * of one workgroup (160 KiB on gfx950), runs the shared kernel; other cases
+ * apply the instances one by one. */
 typedef struct ed_block_s* ed_block;
 int ed_block_create(ed_basis basis, int32_t M, const ed_terms* terms, ed_block* out);
 void ed_block_destroy(ed_block blk);

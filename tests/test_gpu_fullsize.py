@@ -183,6 +183,29 @@ def test_block_kernel_tri6x5_vs_reference_shv(gpu_ready):
         assert _relerr(W64[:, m], wr) <= 1e-6, m
 
 
+def test_block_kernel_tri6x5_m8_vs_single(gpu_ready):
+    """M=8 on the 30-site triangular torus with chirality: the pair tables
+    of eight instances and the rank tables take 78 KB of LDS, which one
+    gfx950 workgroup may hold, so the shared kernel runs; each instance's
+    column against the single-instance operator of its couplings."""
+    import torch
+    ed = _ed()
+    print("[case] block tri6x5 M=8 k=(1,2)", flush=True)
+    chs = [0.02, 0.05, 0.09, 0.12, 0.16, 0.21, 0.25, 0.3]
+    models = [L.triangular(6, 5, Ch=c) for c in chs]
+    basis = ed.Basis(ed.Lattice.of(models[0]), 15, 1, 2)
+    blk = ed.BlockOperator(basis, models)
+    assert blk.fast
+    V = np.stack([P.start_vector(40 + m, basis.D) for m in range(8)], axis=1)
+    Vd = torch.from_numpy(np.ascontiguousarray(V)).to("cuda")
+    W = blk.hv(Vd).cpu().numpy()
+    W64 = blk.hv(Vd.to(torch.complex64)).to(torch.complex128).cpu().numpy()
+    for m, mod in enumerate(models):
+        ws = _gpu_hv(ed.Operator(basis, mod), np.ascontiguousarray(V[:, m]), torch.complex128)
+        assert _relerr(W[:, m], ws) <= 1e-12, m
+        assert _relerr(W64[:, m], ws) <= 1e-6, m
+
+
 @pytest.mark.parametrize("k", [(0, 0), (1, 4)], ids=["k00", "k14"])
 def test_tri3x9_vs_oracle_full(gpu_ready, k):
     """27-site 3x9 triangular torus with chirality, "Sz=0" (n_ones = 13 as
