@@ -308,8 +308,13 @@ def main():
             op.tune(0)
             op.hv(v, w)          # re-tune this precision outside the later timed regions
         kname = {(2, 0): "k_hv_torus_c4", (2, 1): "k_hv_torus_c2w8", (2, 2): "k_hv_torus_c8",
-                 (2, 3): "k_hv_torus_p4", (2, 4): "k_hv_torus_s2", (2, 5): "k_hv_torus_s4", (2, 6): "k_hv_torus_s2k4", (3, -1): "k_hv_torus", (6, -1): "k_hv_rows2"}.get((path, var), "k_hv_rows")
-        results[dt_name] = dict(kernel=kname, block=blk, tune=({0: 2, 1: 4, 2: 5, 3: 7, 4: 9, 5: 10, 6: 11}.get(var, 2), blk) if path == 2 else None,
+                 (2, 3): "k_hv_torus_p4", (2, 4): "k_hv_torus_s2", (2, 5): "k_hv_torus_s4", (2, 6): "k_hv_torus_s2k4", (3, -1): "k_hv_torus", (6, -1): "k_hv_rows2",
+                 (12, -1): "k_hv_elist"}.get((path, var), "k_hv_rows")
+        tn = ({0: 2, 1: 4, 2: 5, 3: 7, 4: 9, 5: 10, 6: 11}.get(var, 2), blk) if path == 2 else (12, blk) if path == 12 else None
+        if path == 12:
+            ne, slots = op.elist_stats()
+            log(f"[bench] stored element list: {ne} elements in {slots} slots ({4 * slots / 1e9:.2f} GB)")
+        results[dt_name] = dict(kernel=kname, block=blk, tune=tn,
                                 generic_walk_kernel_ms=gen_ms, hv_per_s=args.steps / el, ms_per_step=1e3 * el / args.steps, kernel_ms=kavg,
                                 alg_bytes=B, achieved_gbs=B / (kavg * 1e-3) / 1e9)
         log(f"[bench] {dt_name}: {results[dt_name]}")

@@ -321,8 +321,17 @@ class Operator:
         6 two-row 64-bit kernel, 7 compiled-shape K=4 software-pipelined,
         8 two-row kernel gathering u by row through the directory, 9 / 10
         compiled-shape with every row split over two / four lanes, 11 as 9
-        with K=4 per lane; block 0 = default."""
+        with K=4 per lane, 12 stored element list (built by the first H*v:
+        each element's image element and family kept in HBM, then streamed;
+        uniform real bond families at k = 0 only); block 0 = default."""
         check(lib().ed_op_tune(self.h, path, block))
+
+    def elist_stats(self):
+        """(elements, 4-byte slots) of the stored element list; (0, 0) before
+        the first H*v on path 12 builds it."""
+        e, s = C.c_int64(), C.c_int64()
+        check(lib().ed_op_elist_stats(self.h, C.byref(e), C.byref(s)))
+        return e.value, s.value
 
     def tuning(self, dtype: str = "c128"):
         """(path, variant, block) the next H*v of this precision runs."""

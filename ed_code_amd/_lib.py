@@ -62,6 +62,7 @@ SIGNATURES = {
     "ed_op_create": (C.c_int, [_VP, C.POINTER(Terms), C.POINTER(_VP)]),
     "ed_op_destroy": (None, [_VP]),
     "ed_op_tune": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "ed_op_elist_stats": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "ed_op_tuning": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ed_tune_cache_clear": (C.c_int, []),
     "ed_op_count_targets": (C.c_int, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -845,9 +845,13 @@ static int op_setup(ed_op_s* op, const std::vector<Bond>& bonds, const std::vect
 
 extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (!op) return ed_fail(ED_EINVAL, "null op");
-    if (path < 0 || path > 11) return ed_fail(ED_EINVAL, "path %d not in 0..11", path);
+    if (path < 0 || path > 12) return ed_fail(ED_EINVAL, "path %d not in 0..12", path);
     if ((path == 6 || path == 8) && !op->use_rows2)
         return ed_fail(ED_EINVAL, "two-row 64-bit path not available for this operator");
+    if (path == 12) {
+        int rc = torus_elist_check(op);
+        if (rc != ED_OK) return rc;
+    }
     if (((path >= 2 && path <= 5) || path == 7 || path >= 9) && !op->use_torus)
         return ed_fail(ED_EINVAL, "rotation fast path not available for this operator");
     if ((path == 2 || path == 4 || path == 5 || path == 7 || path >= 9) && !torus_shape_compiled(op->torus))
@@ -858,6 +862,13 @@ extern "C" int ed_op_tune(ed_op op, int32_t path, int32_t block) {
     if (block) op->tune_block = block;
     else if (op->use_torus) op->tune_block = 512;
     op->sel_var[0] = op->sel_var[1] = -1;
+    return ED_OK;
+}
+
+extern "C" int ed_op_elist_stats(ed_op op, int64_t* elements, int64_t* slots) {
+    if (!op || !elements || !slots) return ed_fail(ED_EINVAL, "null argument");
+    *elements = op->d_el ? op->nelem : 0;
+    *slots = op->d_el ? op->eslots : 0;
     return ED_OK;
 }
 
@@ -890,7 +901,8 @@ extern "C" void ed_op_destroy(ed_op op) {
                     (void*)op->d_tent, op->d_fam64, (void*)op->d_foff64, (void*)op->d_ent64, (void*)op->d_mfam,
                     (void*)op->d_ment, (void*)op->d_tfmk, (void*)op->d_tfco,
                     (void*)op->d_mfmk, (void*)op->d_mki, (void*)op->d_mco, op->d_img64, op->d_img128, op->d_wimg64,
-                    op->d_wimg128})
+                    op->d_wimg128, (void*)op->d_ebase, (void*)op->d_erl, (void*)op->d_el, (void*)op->d_ediag,
+                    (void*)op->d_ectab})
         if (p) dfree(p);
     delete op;
 }
@@ -1008,6 +1020,12 @@ static int launch_hv(ed_op_s* op, const VT* v, VT* w, hipStream_t st, hipEvent_t
     dim3 grid(nblocks(D, 256)), blk(256);
     if (!ext && op->use_torus && op->tune_path != 1 && op->tune_path != 6 && op->tune_path != 8) {
         int rc = torus_prepare<VT>(op, u, v, w, st);
+        if (rc != ED_OK) return rc;
+    }
+    if (op->tune_path == 12 && !op->d_el) {
+        // the stored element list is built by the first H*v on path 12
+        int rc = ext ? ed_fail(ED_EINVAL, "the stored element list is built by a first ed_hv on path 12") : may_alloc();
+        if (rc == ED_OK) rc = torus_elist_prepare(op, st);
         if (rc != ED_OK) return rc;
     }
     if (ev0) HIPCHK(hipEventRecord(ev0, st));

@@ -7,6 +7,7 @@
 // 12-byte directory entry, then the gather of u = v/norm.  The rank tables,
 // family entries and phase table are staged in LDS once per block.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstring>
@@ -501,12 +502,259 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
     return ED_OK;
 }
 
+// ---- stored element list (path 12) ----
+// The off-diagonal elements of a sector with uniform real bond families at
+// k = 0 (kModeUniform), generated once by the compiled-shape row machinery and
+// kept in HBM: per element the gather-image element of its target and its
+// family, packed in 32 bits.  The H*v kernel then streams these words and
+// gathers, with no orbit search or rank: the instruction-free form of the
+// same gathers, against which the matrix-free kernels are measured.  Rows
+// are stored in groups of 64 (one wave): element q of row 64 g + l at
+// ebase[g] + 64 q + l, so that a wave's loads of the list are coalesced;
+// each group takes as many slots per row as its longest row.
+
+constexpr int kEGroup = 64;
+
+// Pass 1 (el null): the element count of each row into rl.  Pass 2: the
+// packed elements at their slots and the row's diagonal, in the order of
+// torus_row_batched (families in order, members by increasing site).
+template <class TC>
+__global__ void __launch_bounds__(256) k_elist(int64_t D, const uint64_t* __restrict__ rows, int nfam,
+                                               const TFam* __restrict__ gfam, const uint32_t* __restrict__ gfmk,
+                                               const cd2* __restrict__ gfco, double ezz0, RankInfo R,
+                                               const uint32_t* __restrict__ grtab, int rtab_n,
+                                               const uint16_t* __restrict__ gmap, int nmap, int cb, uint32_t zidx,
+                                               const cd2* __restrict__ gphase, int nphase, int jbits,
+                                               uint16_t* __restrict__ rl, const int64_t* __restrict__ ebase,
+                                               uint32_t* __restrict__ el, double* __restrict__ diag) {
+    uint16_t* map;
+    const RowTabs T = stage_row_tabs<kModeUniform>(nfam, gfam, nullptr, nullptr, nullptr, gfmk, gfco, grtab, rtab_n,
+                                                   gmap, nmap, gphase, nphase, map);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint32_t s = (uint32_t)(rows[i] & kStateMask);
+    uint32_t* out = el ? el + ebase[i / kEGroup] + (i % kEGroup) : nullptr;
+    double ezz = ezz0;
+    int n = 0;
+    for (int f = 0; f < T.nfam; f++) {
+        uint32_t cur = fam_mask_c<TC>(s, T.fam[f]);
+        ezz = fma(-(double)popc32(cur), T.fco[f].im, ezz);
+        while (cur != 0u) {
+            const int x = ctz32(cur);
+            cur &= cur - 1u;
+            if (out) {
+                int code;
+                const uint32_t rep = torus_rep_c<TC, kRepOnly>(s ^ T.fmk[f * 32 + x], code);
+                const uint32_t r = rank_tab<uint32_t, uint32_t>(rep, R, T.rtab, (TC::N + 7) / 8);
+                out[(int64_t)n * kEGroup] = image_index(map, cb, zidx, r, true) | ((uint32_t)f << jbits);
+            }
+            n++;
+        }
+    }
+    if (out) diag[i] = ezz;
+    else rl[i] = (uint16_t)n;
+}
+
+// Slots of each group: 64 times its longest row (the last group's missing
+// rows count as empty).
+__global__ void k_elist_groups(int64_t D, const uint16_t* __restrict__ rl, int64_t ng, int64_t* __restrict__ gs) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    int m = 0;
+    for (int l = 0; l < kEGroup; l++) {
+        const int64_t i = g * kEGroup + l;
+        if (i < D) m = max(m, (int)rl[i]);
+    }
+    gs[g] = (int64_t)m * kEGroup;
+}
+
+__global__ void k_elist_ctab(int nfam, const cd2* __restrict__ gfco, double* __restrict__ ct) {
+    const int f = threadIdx.x;
+    if (f < nfam) ct[f] = gfco[f].re;
+}
+
+// H*v from the stored list: K elements in flight per lane; the same FMAs in
+// the same order as the compiled-shape kernels, with an empty slot adding
+// 0 times the zero slot as theirs do.  The list's index is clamped to the
+// row's last element so that every load is issued unconditionally.
+template <int K, typename VT>
+__global__ void __launch_bounds__(512) k_hv_elist(int64_t D, const uint64_t* __restrict__ rows,
+                                                  const double* __restrict__ ntab, const int64_t* __restrict__ ebase,
+                                                  const uint16_t* __restrict__ rl, const uint32_t* __restrict__ el,
+                                                  const double* __restrict__ diag, const double* __restrict__ gct,
+                                                  int nfam, int jbits, uint32_t zidx, const VT* __restrict__ u,
+                                                  const VT* __restrict__ v, VT* __restrict__ w) {
+    __shared__ double ct[kMaxFam];
+    for (int q = threadIdx.x; q < nfam; q += blockDim.x) ct[q] = gct[q];
+    __syncthreads();
+    const unsigned lb = xcd_tile_block();
+    const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const double nrm = ntab[(int)(rows[i] >> kLidShift)];
+    const uint32_t* p = el + ebase[i / kEGroup] + (i % kEGroup);
+    const int n = rl[i];
+    const uint32_t jm = (1u << jbits) - 1u;
+    double ar = 0.0, ai = 0.0;
+    for (int q = 0; q < n; q += K) {
+        uint32_t j[K];
+        double c[K];
+#pragma unroll
+        for (int h = 0; h < K; h++) {
+            const uint32_t e = p[(int64_t)min(q + h, n - 1) * kEGroup];
+            const bool ok = q + h < n;
+            j[h] = ok ? (e & jm) : zidx;
+            c[h] = ok ? ct[e >> jbits] : 0.0;
+        }
+        cd2 x[K];
+#pragma unroll
+        for (int h = 0; h < K; h++) x[h] = load_c32(u, j[h]);
+#pragma unroll
+        for (int h = 0; h < K; h++) {
+            ar = fma(c[h], x[h].re, ar);
+            ai = fma(c[h], x[h].im, ai);
+        }
+    }
+    const cd2 vi = load_c(v, i);
+    const double ez = diag[i];
+    VT o;
+    o.re = fma(nrm, ar, ez * vi.re);
+    o.im = fma(nrm, ai, ez * vi.im);
+    w[i] = o;
+}
+
+static int bit_length(uint64_t x) {
+    int b = 0;
+    while (x) {
+        b++;
+        x >>= 1;
+    }
+    return b;
+}
+
+// Whether the operator can take the stored element list.
+int torus_elist_check(const ed_op_s* op) {
+    const ed_basis_s* B = op->b;
+    if (!op->use_torus || !torus_shape_compiled(op->torus) || !compiled_fits(op))
+        return ed_fail(ED_EINVAL, "stored element list: rotation fast path with a compiled shape required");
+    if (op->tmode != kModeUniform || !B->trivial_k)
+        return ed_fail(ED_EINVAL, "stored element list: uniform real bond families at k = 0 only");
+    const uint64_t ne = (uint64_t)(B->img_nslot + 1) << B->img_cb;
+    if (bit_length(ne - 1) > 31 || bit_length(ne - 1) + bit_length((uint64_t)(op->nfam > 1 ? op->nfam - 1 : 0)) > 32)
+        return ed_fail(ED_EINVAL, "stored element list: image element and family do not pack into 32 bits");
+    return ED_OK;
+}
+
+template <class TC>
+static void launch_elist_build(ed_op_s* op, hipStream_t st, size_t sm, bool emit) {
+    ed_basis_s* B = op->b;
+    const uint32_t zidx = B->img_nslot << B->img_cb;
+    hipLaunchKernelGGL((k_elist<TC>), dim3((unsigned)((B->D + 255) / 256)), dim3(256), sm, st, B->D, B->d_rows,
+                       op->nfam, op->d_tfam, op->d_tfmk, op->d_tfco, op->ezz0, B->rinfo, B->d_rtab32, B->rtab32_n,
+                       B->d_img_map, (int)B->img_nmap, B->img_cb, zidx, B->d_phase, B->L1 * B->L2, op->ejbits,
+                       op->d_erl, emit ? op->d_ebase : nullptr, emit ? op->d_el : nullptr, op->d_ediag);
+}
+
+static bool elist_build(ed_op_s* op, hipStream_t st, bool emit) {
+    const Torus& T = op->torus;
+    const int L2 = T.kind == 2 ? 1 : T.L2;
+    const size_t sm = torus_c_smem_bytes(op);
+#define ED_ELIST(n, wd, l2)                                        \
+    if (T.N == n && T.w == wd && L2 == l2) {                       \
+        launch_elist_build<TorusC<n, wd, l2>>(op, st, sm, emit);   \
+        return true;                                               \
+    }
+    ED_TORUS_SHAPES(ED_ELIST)
+#undef ED_ELIST
+    return false;
+}
+
+// Builds the operator's stored element list (on the caller's stream, which
+// is then synchronised once to size the list).
+int torus_elist_prepare(ed_op_s* op, hipStream_t st) {
+    int rc = torus_elist_check(op);
+    if (rc != ED_OK) return rc;
+    ed_basis_s* B = op->b;
+    const int64_t D = B->D, ng = (D + kEGroup - 1) / kEGroup;
+    const uint64_t ne = (uint64_t)(B->img_nslot + 1) << B->img_cb;
+    op->ejbits = bit_length(ne - 1);
+    auto fail = [&](int code) {
+        for (void* q : {(void*)op->d_ebase, (void*)op->d_erl, (void*)op->d_el, (void*)op->d_ediag, (void*)op->d_ectab})
+            dfree(q);
+        op->d_ebase = nullptr;
+        op->d_erl = nullptr;
+        op->d_el = nullptr;
+        op->d_ediag = nullptr;
+        op->d_ectab = nullptr;
+        return code;
+    };
+#define ED_EL(x)                                                                                            \
+    do {                                                                                                    \
+        hipError_t e_ = (x);                                                                                \
+        if (e_ != hipSuccess)                                                                               \
+            return fail(ed_fail(e_ == hipErrorOutOfMemory ? ED_ENOMEM : ED_EDEVICE, "%s: %s", #x,           \
+                                hipGetErrorString(e_)));                                                    \
+    } while (0)
+    ED_EL(hipMalloc((void**)&op->d_erl, D * sizeof(uint16_t)));
+    ED_EL(hipMalloc((void**)&op->d_ebase, (ng + 1) * sizeof(int64_t)));
+    ED_EL(hipMalloc((void**)&op->d_ediag, D * sizeof(double)));
+    ED_EL(hipMalloc((void**)&op->d_ectab, kMaxFam * sizeof(double)));
+    if (!elist_build(op, st, false)) return fail(ed_fail(ED_EINVAL, "stored element list: shape not compiled"));
+    ED_EL(hipGetLastError());
+    int64_t* gs = nullptr;
+    ED_EL(hipMalloc((void**)&gs, (ng + 1) * sizeof(int64_t)));
+    hipLaunchKernelGGL(k_elist_groups, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, D, op->d_erl, ng, gs);
+    hipLaunchKernelGGL(k_elist_ctab, dim3(1), dim3(kMaxFam), 0, st, op->nfam, op->d_tfco, op->d_ectab);
+    size_t tmp_bytes = 0;
+    void* tmp = nullptr;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, gs, op->d_ebase, (size_t)(ng + 1), st);
+    if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
+    if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, gs, op->d_ebase, (size_t)(ng + 1), st);
+    int64_t slots = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&slots, op->d_ebase + ng, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dfree(tmp);
+    dfree(gs);
+    ED_EL(e);
+    op->eslots = slots;
+    ED_EL(hipMalloc((void**)&op->d_el, (size_t)std::max<int64_t>(slots, 1) * sizeof(uint32_t)));
+    elist_build(op, st, true);
+    ED_EL(hipGetLastError());
+    int64_t nel = 0;
+    {
+        std::vector<uint16_t> h(D);
+        ED_EL(hipMemcpyAsync(h.data(), op->d_erl, D * sizeof(uint16_t), hipMemcpyDeviceToHost, st));
+        ED_EL(hipStreamSynchronize(st));
+        for (uint16_t c : h) nel += c;
+    }
+    op->nelem = nel;
+#undef ED_EL
+    return ED_OK;
+}
+
+template <typename VT>
+static int launch_elist(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st, int bs) {
+    ed_basis_s* B = op->b;
+    if (!op->d_el) return ed_fail(ED_EINVAL, "stored element list not built");
+    const uint32_t zidx = B->img_nslot << B->img_cb;
+    hipLaunchKernelGGL((k_hv_elist<4, VT>), dim3((unsigned)((B->D + bs - 1) / bs)), dim3(bs), 0, st, B->D, B->d_rows,
+                       B->d_ntab, op->d_ebase, op->d_erl, op->d_el, op->d_ediag, op->d_ectab, op->nfam, op->ejbits,
+                       zidx, u, v, w);
+    HIPCHK(hipGetLastError());
+    return ED_OK;
+}
+
 // Kernel configuration the next launch uses: path 2 (compiled shape),
 // 3 (run-time shape); variant 0 (K=4), 1 (K=2 at 8 waves), 2 (K=8),
 // 3 (K=4 pipelined), 4 (two lanes per row, K=8), 5 (four lanes per row,
 // K=4), 6 (two lanes per row, K=4), -1 before tuning or for the run-time
 // kernel.
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs) {
+    if (op->tune_path == 12) {
+        path = 12;
+        var = -1;
+        bs = op->tune_block;
+        return;
+    }
     const bool compiled = op->tune_path != 3 && compiled_fits(op);
     path = compiled ? 2 : 3;
     bs = op->tune_block;
@@ -531,6 +779,7 @@ int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st
     const int64_t D = B->D;
     int path, var, bs;
     torus_config(op, dtype_slot<VT>(), path, var, bs);
+    if (path == 12) return launch_elist<VT>(op, u, v, w, st, bs);
     if (path == 2) {
         if (!launch_compiled<VT>(op, u, v, w, st, var < 0 ? 0 : var, bs))
             return ed_fail(ED_EINVAL, "compiled-shape kernel does not fit this sector");

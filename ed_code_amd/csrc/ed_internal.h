@@ -163,7 +163,7 @@ struct ed_op_s {
     int tune_path = 0;        // 0 auto, 1 generic walk, 2/4/5 compiled shape K=4 / K=2 / K=8, 3 run-time shape,
                               // 6 two-row 64-bit kernel, 7 compiled shape K=4 pipelined, 8 as 6 with u by
                               // row through the directory, 9/10 compiled shape with rows split over 2 / 4
-                              // lanes, 11 as 9 with K=4
+                              // lanes, 11 as 9 with K=4, 12 stored element list
     int tune_block = 256;
     bool block_fixed = false; // block set by ed_op_tune
     int sel_var[2] = {-1, -1};   // tuned variant per precision (0 complex128, 1 complex64)
@@ -176,6 +176,17 @@ struct ed_op_s {
     void* d_wimg128 = nullptr;
     bool wimg_off[2] = {false, false};   // wide image did not fit (per precision: 0 complex128, 1 complex64)
     std::vector<edc::Bond> bonds;
+    // stored element list (path 12, ed_hv_fast.hip): rows in groups of 64;
+    // element q of row 64 g + l at el[ebase[g] + 64 q + l], packed as the
+    // image element | family << ejbits; erl[i] elements in row i
+    int64_t* d_ebase = nullptr;   // [ngroups + 1]
+    uint16_t* d_erl = nullptr;    // [D]
+    uint32_t* d_el = nullptr;     // [ebase[ngroups]]
+    double* d_ediag = nullptr;    // [D] diagonal of each row
+    double* d_ectab = nullptr;    // [nfam] coefficient of each family
+    int ejbits = 0;
+    int64_t eslots = 0;           // ebase[ngroups]: stored slots, padding included
+    int64_t nelem = 0;            // elements
 };
 
 int basis_geo(const ed_basis_s* B, edc::Geo& G);
@@ -189,6 +200,8 @@ int launch_hv_torus(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st
 template <typename VT>
 int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st);
 void torus_config(const ed_op_s* op, int dslot, int& path, int& var, int& bs);
+int torus_elist_check(const ed_op_s* op);
+int torus_elist_prepare(ed_op_s* op, hipStream_t st);
 bool torus_shape_compiled(const edc::Torus& T);
 bool torus_uses_image(const ed_op_s* op);
 int basis_image_build(ed_basis_s* B);

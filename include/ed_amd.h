@@ -123,13 +123,25 @@ void ed_op_destroy(ed_op op);
  * a row's elements in one order and give identical results; 9, 10 and 11 sum
  * each lane's share separately and agree with them to rounding (identical
  * among themselves for one lane count: 9 and 11).  Paths 9-11 fall back to 2
- * for more than 10 term families.
- * Paths 2-5, 7 and 9-11 need a lattice whose translations are bit rotations,
+ * for more than 10 term families.  12 = stored element list: the first H*v
+ * on this path generates every off-diagonal element of the sector once
+ * (image element of the target and bond family, 4 bytes each, rows in groups
+ * of 64 padded to the group's longest row; about 5 GB for the 32-site
+ * square torus at Sz = 0, k = 0) and keeps it with the operator; later H*v
+ * stream the list and gather, with no orbit search or rank, and give the
+ * results of path 2.  Only for uniform real bond families (every member of a
+ * family with the same J) at k = 0, whose image element and family pack into
+ * 32 bits; never chosen by path 0.
+ * Paths 2-5, 7 and 9-12 need a lattice whose translations are bit rotations,
  * N <= 32 and no bowtie terms; all but 3 also a built-in shape; 6 and 8 a
  * compiled two-row shape and no bowtie terms.
  * block = threads per block (64..512, multiple of 64; 0 = default, tuned
  * under path 0).  Resets any earlier tuning. */
 int ed_op_tune(ed_op op, int32_t path, int32_t block);
+/* Size of the operator's stored element list (path 12): the elements and the
+ * 4-byte slots kept, padding included; 0 and 0 before it is built.  No
+ * counterpart in the reference, whose sHv is matrix-free. */
+int ed_op_elist_stats(ed_op op, int64_t* elements, int64_t* slots);
 /* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3, 6
  * or 8 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
  * pipelined, 4 = two lanes per row, 5 = four lanes per row, 6 = two lanes

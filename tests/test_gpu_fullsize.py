@@ -157,6 +157,12 @@ def test_full_sector_vs_reference_shv(gpu_ready, name, model, n, k, dim):
         for path in (9, 10, 11):
             op.tune(path, b)
             assert np.array_equal(_gpu_hv(op, v, torch.complex128), out[path]), (path, b)
+    if name == "sq4x8_k00":
+        # the stored element list: the same elements, order and FMAs as path 2
+        op.tune(12)
+        assert np.array_equal(_gpu_hv(op, v, torch.complex128), out[2])
+        ne, slots = op.elist_stats()
+        assert ne >= op.count_targets()[0] and slots >= ne
 
 
 @needs_ref

@@ -223,6 +223,67 @@ def test_gpu_rotation_fast_path_vs_generic(gpu_ready, name, model, n, kx, ky):
     assert np.array_equal(out[9], out[11])
 
 
+STORED = [
+    ("sq4x4_k00", L.square(4, 4, J2=0.5, lam=0.7), 8),
+    ("sq4x5_k00", L.square(4, 5, J2=0.5), 10),
+    ("sq4x5_k00_n7", L.square(4, 5, J2=0.3, lam=1.3), 7),
+    ("chain16_k00", L.chain(16, J=0.8), 8),
+]
+
+
+@pytest.mark.parametrize("name,model,n", STORED, ids=[c[0] for c in STORED])
+def test_gpu_stored_element_list(gpu_ready, name, model, n):
+    """Path 12 (the stored element list) against the oracle and bit-identical
+    to path 2 (same elements, same order, same FMAs); the list holds one
+    element per antiparallel bond of every representative, counted here from
+    the oracle's representatives and the model's bonds."""
+    import torch
+    ed = _ed()
+    print(f"[case] stored list {name}", flush=True)
+    o = P.OracleBasis(model, n, 0, 0)
+    basis = ed.Basis(ed.Lattice.of(model), n, 0, 0)
+    op = ed.Operator(basis, model)
+    rng = np.random.default_rng(21)
+    v = rng.standard_normal(o.D) + 1j * rng.standard_normal(o.D)
+    wo = o.hv(v)
+    op.tune(2)
+    w2 = _hv(op, v, torch.complex128)
+    w2f = _hv(op, v, torch.complex64)
+    op.tune(12)
+    assert op.elist_stats() == (0, 0)
+    w = _hv(op, v, torch.complex128)
+    assert op.tuning("c128")[0] == 12 and op.tuning("c64")[0] == 12
+    assert np.linalg.norm(w - wo) <= 1e-12 * np.linalg.norm(wo)
+    assert np.array_equal(w, w2)
+    assert np.array_equal(_hv(op, v, torch.complex64), w2f)
+    reps = o.tables(full=False)["reps"]
+    anti = np.zeros(o.D, np.int64)
+    for x, y, J, _ in model.bonds:
+        if abs(J) > 1e-16:
+            anti += ((reps >> x) & 1) != ((reps >> y) & 1)
+    ne, slots = op.elist_stats()
+    assert ne == int(anti.sum())
+    groups = [anti[g:g + 64].max() * 64 for g in range(0, o.D, 64)]
+    assert slots == int(np.sum(groups))
+    # a second H*v reuses the list
+    assert np.array_equal(_hv(op, v, torch.complex128), w2)
+
+
+def test_gpu_stored_element_list_refused(gpu_ready):
+    """The stored list covers uniform real bond families at k = 0; complex
+    coefficients (k != 0, chiral terms) are refused by the tune call."""
+    ed = _ed()
+    m = L.triangular(4, 4, Ch=0.3)
+    for k in ((0, 0), (1, 2)):
+        op = ed.Operator(ed.Basis(ed.Lattice.of(m), 8, *k), m)
+        with pytest.raises(RuntimeError, match="stored element list"):
+            op.tune(12)
+    m = L.square(4, 4, J2=0.5)
+    op = ed.Operator(ed.Basis(ed.Lattice.of(m), 8, 1, 3), m)
+    with pytest.raises(RuntimeError, match="stored element list"):
+        op.tune(12)
+
+
 def test_gpu_autotune_picks_identical_result(gpu_ready):
     """On a sector above the tuning threshold (30-site triangular torus,
     D ~ 5.2e6) the automatic path times the variants on the first H*v; its
