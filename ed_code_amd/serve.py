@@ -18,9 +18,12 @@ requests:
   vectors, work vectors) fits next to the running jobs under the budget, by
   default 85 % of the device memory free when the server starts
   (hipMemGetInfo, through torch).
-* Exclusive large jobs -- a job on a sector of at least --exclusive-rows rows
-  (the 32-site torus) runs with the GPU to itself: its H*v already fills the
-  device, and sharing only stretches its latency.
+* heavy slots -- at most --heavy-slots (1) jobs on sectors of at least
+  --heavy-rows rows (2^22: the 30- and 32-site tori, whose H*v fills the
+  device) run at once: two such solves share the caches their gathers rely
+  on, and each runs slower than alone.  Lighter jobs are admitted beside
+  them up to the worker count, and pass a heavy job that waits for its
+  slot, so that a short request is not queued behind a long one.
 
 Latency is completion time minus arrival time.  The baseline is the plain
 loop: first come first served, one request at a time, basis built per
@@ -263,14 +266,15 @@ def device_budget(fraction: float = 0.85) -> int:
 
 class Server:
     def __init__(self, workers: int = 4, hbm_budget: Optional[int] = None, cache_budget: int = 24 * GiB,
-                 max_batch: int = 8, iterations: int = 300, exclusive_rows: int = 1 << 24):
+                 max_batch: int = 8, iterations: int = 300, heavy_rows: int = 1 << 22, heavy_slots: int = 1):
         if hbm_budget is None:
             hbm_budget = device_budget()
         self.workers = workers
-        # a job of at least this many rows runs with the GPU to itself (its
-        # H*v saturates the device; sharing only stretches its latency)
-        self.exclusive_rows = exclusive_rows
-        self.exclusive = False
+        # jobs of at least heavy_rows rows (their H*v fills the device) take
+        # one of heavy_slots slots; lighter jobs need none
+        self.heavy_rows = heavy_rows
+        self.heavy_slots = heavy_slots
+        self.heavy = 0
         self.budget = hbm_budget
         self.max_batch = max_batch
         self.iterations = iterations
@@ -280,7 +284,7 @@ class Server:
         self.running = 0
         self.errors: List[str] = []
 
-    def _job(self, reqs: List[Request], fp: int):
+    def _job(self, reqs: List[Request], fp: int, heavy: bool):
         try:
             basis = self.cache.get(sector_key(reqs[0]))
             solve(basis, reqs, self.iterations)
@@ -290,8 +294,7 @@ class Server:
             with self.cv:
                 self.in_use -= fp
                 self.running -= 1
-                if self.running == 0:
-                    self.exclusive = False
+                self.heavy -= int(heavy)
                 self.cv.notify_all()
 
     def run(self, requests: List[Request]) -> Tuple[float, float]:
@@ -308,23 +311,34 @@ class Server:
                 while future and future[0].arrival_abs <= now:
                     pending.append(future.pop(0))
                 started = False
-                while pending and self.running < self.workers and not self.exclusive:
-                    head = pending[0]
-                    group = [p for p in pending if batch_key(p) == batch_key(head)]
-                    M = 8 if len(group) >= 8 else 4 if len(group) >= 4 else 2 if len(group) >= 2 else 1
-                    M = min(M, self.max_batch)
-                    group = group[:M]
-                    rows = estimate_rows(head.lattice, sector_key(head)[1])
-                    fp = footprint(rows, M, self.iterations, head.nev)
-                    big = rows >= self.exclusive_rows
-                    if (self.in_use + fp > self.budget or big) and self.running:
+                while pending and self.running < self.workers:
+                    # the first pending job in arrival order that may start: a
+                    # heavy job waits for a heavy slot (lighter ones pass it),
+                    # a job that does not fit the HBM budget stops the scan
+                    job = None
+                    for head in pending:
+                        rows = estimate_rows(head.lattice, sector_key(head)[1])
+                        heavy = rows >= self.heavy_rows
+                        if heavy and self.heavy >= self.heavy_slots:
+                            continue
+                        group = [p for p in pending if batch_key(p) == batch_key(head)]
+                        M = 8 if len(group) >= 8 else 4 if len(group) >= 4 else 2 if len(group) >= 2 else 1
+                        M = min(M, self.max_batch)
+                        group = group[:M]
+                        fp = footprint(rows, M, self.iterations, head.nev)
+                        if self.in_use + fp > self.budget and self.running:
+                            break
+                        job = (group, fp, heavy)
                         break
+                    if job is None:
+                        break
+                    group, fp, heavy = job
                     for g in group:
                         pending.remove(g)
                     self.in_use += fp
                     self.running += 1
-                    self.exclusive = big
-                    th = threading.Thread(target=self._job, args=(group, fp), daemon=True)
+                    self.heavy += int(heavy)
+                    th = threading.Thread(target=self._job, args=(group, fp, heavy), daemon=True)
                     th.start()
                     threads.append(th)
                     started = True
@@ -366,13 +380,13 @@ def summarize(requests: List[Request], wall: float, t_base: float) -> dict:
 
 
 def run_stream(requests: int, rate: float, seed: int, lattices: List[str], workers: int = 4,
-               iterations: int = 300, k_zero: bool = False, exclusive_rows: int = 1 << 24,
+               iterations: int = 300, k_zero: bool = False, heavy_rows: int = 1 << 22, heavy_slots: int = 1,
                baseline: bool = True) -> dict:
     """One seeded request stream through the server and, on the same stream,
     the plain loop: latency percentiles, throughput, per-eigenvalue FP64
     error bounds, and the largest difference between the two answers."""
     reqs = make_requests(requests, rate, seed, lattices, k_zero)
-    srv = Server(workers=workers, iterations=iterations, exclusive_rows=exclusive_rows or (1 << 62))
+    srv = Server(workers=workers, iterations=iterations, heavy_rows=heavy_rows, heavy_slots=heavy_slots)
     # warm-up outside the timed runs (GPU context, library load, first kernels)
     warm = Request(-1, "chain24", 0.0, 0, 0, 0.1, 1.0, 0.0, 1, 1e-10, 0.0)
     solve(srv.cache.get(sector_key(warm)), [warm], iterations)
@@ -380,7 +394,8 @@ def run_stream(requests: int, rate: float, seed: int, lattices: List[str], worke
     lib().ed_tune_cache_clear()
     t0, wall = srv.run(reqs)
     out = dict(config=dict(requests=requests, rate=rate, seed=seed, lattices=lattices, workers=workers,
-                           k_zero=k_zero, exclusive_rows=exclusive_rows, hbm_budget_bytes=srv.budget),
+                           k_zero=k_zero, heavy_rows=heavy_rows, heavy_slots=heavy_slots,
+                           hbm_budget_bytes=srv.budget),
                server=summarize(reqs, wall, t0), cache=dict(hits=srv.cache.hits, misses=srv.cache.misses),
                errors=srv.errors)
     served = {r.rid: r.evals for r in reqs}
@@ -407,12 +422,13 @@ def main(argv=None):
     ap.add_argument("--iterations", type=int, default=300)
     ap.add_argument("--k-zero", action="store_true", help="every request at k=0 (coupling scans)")
     ap.add_argument("--no-baseline", action="store_true")
-    ap.add_argument("--exclusive-rows", type=int, default=1 << 24,
-                    help="jobs on sectors of at least this many rows run alone (0: never)")
+    ap.add_argument("--heavy-rows", type=int, default=1 << 22,
+                    help="jobs on sectors of at least this many rows take a heavy slot")
+    ap.add_argument("--heavy-slots", type=int, default=1, help="heavy jobs running at once")
     a = ap.parse_args(argv)
     init(0)
     out = run_stream(a.requests, a.rate, a.seed, a.lattices.split(","), a.workers, a.iterations, a.k_zero,
-                     a.exclusive_rows, baseline=not a.no_baseline)
+                     a.heavy_rows, a.heavy_slots, baseline=not a.no_baseline)
     print(json.dumps(out), flush=True)
 
 
