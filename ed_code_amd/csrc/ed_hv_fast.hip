@@ -441,9 +441,9 @@ static TuneKey tune_key(const ed_op_s* op, int dslot) {
 // Chooses the configuration of the compiled-shape kernel for this operator
 // and precision on the first call: below kTuneMinRows rows, or while the
 // stream is being captured, the default (K=4, 512 threads, or the block fixed
-// by ed_op_tune) is taken; above, each candidate runs on the caller's vectors
-// (one untimed launch, then the best of two timed ones) and the fastest is
-// kept.  Every candidate writes the same w.
+// by ed_op_tune) is taken; above, the candidates run on the caller's vectors
+// (one untimed launch, each candidate timed once, the three fastest twice
+// more) and the fastest is kept.  Every candidate writes the same w.
 template <typename VT>
 int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) {
     if (!op->use_torus || op->tune_path != 0 || !compiled_fits(op)) return ED_OK;
@@ -469,20 +469,48 @@ int torus_prepare(ed_op_s* op, const VT* u, const VT* v, VT* w, hipStream_t st) 
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        float best = 1e30f;
-        for (const auto& c : kTuneCfg) {
+        // one untimed launch, every candidate timed once, then the three
+        // fastest twice more: the best of their three timings wins
+        auto timed_launch = [&](int var, int bs, float& ms) -> int {
+            HIPCHK(hipEventRecord(e0, st));
+            launch_compiled<VT>(op, u, v, w, st, var, bs);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            return ED_OK;
+        };
+        std::vector<std::pair<float, int>> first;
+        float ms = 0.f;
+        bool warm = false;
+        for (int ci = 0; ci < (int)(sizeof(kTuneCfg) / sizeof(kTuneCfg[0])); ci++) {
+            const int* c = kTuneCfg[ci];
             if (op->block_fixed && c[1] != op->tune_block) continue;
             if (!variant_fits(op, c[0])) continue;
-            float t = 1e30f;
-            for (int rep = 0; rep < 3; rep++) {
-                HIPCHK(hipEventRecord(e0, st));
-                launch_compiled<VT>(op, u, v, w, st, c[0], c[1]);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipEventRecord(e1, st));
-                HIPCHK(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0) t = std::min(t, ms);
+            int rc = ED_OK;
+            if (!warm) rc = timed_launch(c[0], c[1], ms);
+            warm = true;
+            if (rc == ED_OK) rc = timed_launch(c[0], c[1], ms);
+            if (rc != ED_OK) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
+            }
+            first.emplace_back(ms, ci);
+        }
+        std::sort(first.begin(), first.end());
+        float best = 1e30f;
+        for (size_t r = 0; r < first.size() && r < 3; r++) {
+            const int* c = kTuneCfg[first[r].second];
+            float t = first[r].first;
+            for (int rep = 0; rep < 2; rep++) {
+                int rc = timed_launch(c[0], c[1], ms);
+                if (rc != ED_OK) {
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    return rc;
+                }
+                t = std::min(t, ms);
             }
             if (t < best) {
                 best = t;
