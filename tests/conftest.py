@@ -20,7 +20,11 @@ def _oracle_built():
     """The oracle library is test infrastructure; build it if it is missing."""
     so = os.path.join(ROOT, "oracle", "libedoracle.so")
     if not os.path.exists(so):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+        import fcntl
+        with open(so + ".lock", "w") as lk:       # parallel test workers build it once
+            fcntl.flock(lk, fcntl.LOCK_EX)
+            if not os.path.exists(so):
+                subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
     yield
 
 

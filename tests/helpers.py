@@ -225,6 +225,24 @@ def momentum_projector(model, n, kx, ky, st):
 _hc = None
 
 
+def _build_locked(out, deps, cmd):
+    """Builds `out` with `cmd + ["-o", tmp]` when it is missing or older than
+    a dependency: under a file lock (parallel test workers build it once),
+    into a temporary file renamed into place (no worker loads a partial
+    file)."""
+    import fcntl
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    stale = lambda: not os.path.exists(out) or any(os.path.getmtime(f) > os.path.getmtime(out) for f in deps)
+    if not stale():
+        return
+    with open(out + ".lock", "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        if stale():
+            tmp = f"{out}.{os.getpid()}.tmp"
+            subprocess.run(cmd + ["-o", tmp], check=True)
+            os.replace(tmp, out)
+
+
 def refapi_driver():
     """tests/refapi/driver.cpp (a caller of the entry points of
     include/ed_refapi.h that the reference's main does not reach) linked
@@ -233,10 +251,8 @@ def refapi_driver():
     src = os.path.join(HERE, "refapi", "driver.cpp")
     lib = os.path.join(ROOT, "ed_code_amd", "libed_amd.so")
     hdr = os.path.join(ROOT, "include", "ed_refapi.h")
-    if not os.path.exists(out) or max(os.path.getmtime(src), os.path.getmtime(hdr)) > os.path.getmtime(out):
-        os.makedirs(os.path.dirname(out), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", out,
-                        "-L", os.path.dirname(lib), "-led_amd", "-Wl,-rpath,$ORIGIN/../../../ed_code_amd"], check=True)
+    _build_locked(out, [src, hdr], ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src,
+                                    "-L", os.path.dirname(lib), "-led_amd", "-Wl,-rpath,$ORIGIN/../../../ed_code_amd"])
     return out
 
 
@@ -261,10 +277,8 @@ def hostcore():
         out = os.path.join(HERE, "hostcore", "_build", "libhostcore.so")
         src = os.path.join(HERE, "hostcore", "hostcore.cpp")
         hdrs = glob.glob(os.path.join(ROOT, "ed_code_amd", "csrc", "*.h"))
-        if not os.path.exists(out) or any(os.path.getmtime(f) > os.path.getmtime(out) for f in hdrs + [src]):
-            os.makedirs(os.path.dirname(out), exist_ok=True)
-            subprocess.run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
-                            "-I", os.path.join(ROOT, "ed_code_amd", "csrc"), src, "-o", out], check=True)
+        _build_locked(out, hdrs + [src], ["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17",
+                                          "-I", os.path.join(ROOT, "ed_code_amd", "csrc"), src])
         H = C.CDLL(out)
         vp = C.c_void_p
         H.hc_build.restype = vp
