@@ -142,8 +142,8 @@ int ed_op_tune(ed_op op, int32_t path, int32_t block);
  * 4-byte slots kept, padding included; 0 and 0 before it is built.  No
  * counterpart in the reference, whose sHv is matrix-free. */
 int ed_op_elist_stats(ed_op op, int64_t* elements, int64_t* slots);
-/* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3, 6
- * or 8 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
+/* The configuration the next ed_hv of precision dtype runs: path (1, 2, 3, 6,
+ * 8 or 12 as above), variant (0 = K=4, 1 = K=2 at 8 waves, 2 = K=8, 3 = K=4
  * pipelined, 4 = two lanes per row, 5 = four lanes per row, 6 = two lanes
  * per row with K=4, -1 = not yet tuned or not applicable) and threads per
  * block. */
